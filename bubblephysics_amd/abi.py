@@ -1,0 +1,239 @@
+"""ctypes mirror of include/bubblephysics.h and the loader of the HIP library.
+
+The library is built in-tree (``bubblephysics_amd/lib/libbubblephysics_hip.so``) by
+``__graft_entry__.build()``. There is no CPU fallback: a missing library raises
+:class:`LibraryMissing`, and a machine without a GPU gets ``BP_ERR_RESOURCE_UNAVAILABLE``
+from the create calls.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+import numpy as np
+
+LIB_DIR = Path(__file__).resolve().parent / "lib"
+LIB_PATH = LIB_DIR / "libbubblephysics_hip.so"
+
+# Status codes (bp_status).
+BP_OK = 0
+BP_ERR_INVALID_ARGUMENT = 1
+BP_ERR_RESOURCE_UNAVAILABLE = 2
+BP_ERR_ALLOCATION_FAILED = 3
+BP_ERR_COMMAND_SETUP_FAILED = 4
+BP_ERR_COMMAND_FAILED = 5
+BP_ERR_INVALID_SNAPSHOT = 6
+BP_ERR_CAPACITY_OVERFLOW = 7
+BP_ERR_FRAME_IN_FLIGHT = 8
+BP_ERR_NON_FINITE = 9
+BP_ERR_GUARD_FAILURE = 10
+BP_ERR_CONTACT_VIOLATION = 11
+BP_ERR_SUBSTEP_LIMIT_EXCEEDED = 12
+BP_ERR_UNAVAILABLE = 13
+
+STATUS_NAMES = {
+    0: "ok", 1: "invalidArgument", 2: "resourceUnavailable", 3: "allocationFailed", 4: "commandSetupFailed",
+    5: "commandFailed", 6: "invalidSnapshot", 7: "capacityOverflow", 8: "frameInFlight", 9: "nonFinite",
+    10: "guardFailure", 11: "contactViolation", 12: "substepLimitExceeded", 13: "unavailable",
+}
+
+BP_TERM_REMAINING, BP_TERM_STIFFNESS, BP_TERM_DISTANCE, BP_TERM_MARGIN = 0, 1, 2, 3
+BP_TERM_NONE = 0xFFFFFFFF
+BP_VIOLATION_NONE, BP_VIOLATION_PAIR, BP_VIOLATION_INSIDE_BODY, BP_VIOLATION_EDGE_SIDE = 0, 1, 2, 3
+
+BP_BUBBLE_ENABLED, BP_BUBBLE_STATIC, BP_BUBBLE_WRAP, BP_BUBBLE_FORCE_DRIVEN = 1, 2, 4, 8
+BP_REGION_UNCHANGED, BP_REGION_REMOVED, BP_REGION_WRAPPED = 0, 1, 2
+BP_SEGMENT_HAS_OWNER, BP_SEGMENT_POLYGON_EDGE, BP_SEGMENT_CONVEX_START, BP_SEGMENT_CONVEX_END = 1, 2, 4, 8
+BP_SEGMENT_ALLOWED_SIDE = 16
+BP_SHAPE_POLYGON, BP_SHAPE_CIRCLE, BP_SHAPE_CAPSULE = 0, 1, 2
+BP_BODY_STATIC, BP_BODY_KINEMATIC = 1, 2
+
+
+class Bubble(ctypes.Structure):
+    _fields_ = [
+        ("id", ctypes.c_int64), ("center", ctypes.c_float * 2), ("previous_center", ctypes.c_float * 2),
+        ("velocity", ctypes.c_float * 2), ("rotation", ctypes.c_float), ("angular_velocity", ctypes.c_float),
+        ("mass", ctypes.c_float), ("inverse_mass", ctypes.c_float), ("radius", ctypes.c_float),
+        ("contact_stiffness", ctypes.c_float), ("damping", ctypes.c_float), ("flags", ctypes.c_uint32),
+        ("category", ctypes.c_uint32), ("mask", ctypes.c_uint32),
+    ]
+
+
+class Segment(ctypes.Structure):
+    _fields_ = [
+        ("id", ctypes.c_int64), ("owner_id", ctypes.c_int64), ("a", ctypes.c_float * 2), ("b", ctypes.c_float * 2),
+        ("previous_start", ctypes.c_float * 2), ("outward_sign", ctypes.c_float), ("allowed_side", ctypes.c_float),
+        ("flags", ctypes.c_uint32), ("category", ctypes.c_uint32), ("mask", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
+    ]
+
+
+class Body(ctypes.Structure):
+    _fields_ = [
+        ("id", ctypes.c_uint64), ("pose", ctypes.c_float * 3), ("previous", ctypes.c_float * 3),
+        ("velocity", ctypes.c_float * 3), ("bounding_radius", ctypes.c_float), ("core_radius", ctypes.c_float),
+        ("half_length", ctypes.c_float), ("max_linear_speed", ctypes.c_float), ("max_angular_speed", ctypes.c_float),
+        ("shape", ctypes.c_uint32), ("flags", ctypes.c_uint32), ("category", ctypes.c_uint32), ("mask", ctypes.c_uint32),
+        ("vertex_offset", ctypes.c_uint32), ("vertex_count", ctypes.c_uint32), ("triangle_offset", ctypes.c_uint32),
+        ("triangle_count", ctypes.c_uint32),
+    ]
+
+
+class Motor(ctypes.Structure):
+    _fields_ = [("bubble_id", ctypes.c_int64), ("force", ctypes.c_float * 2), ("torque", ctypes.c_float),
+                ("inertia", ctypes.c_float)]
+
+
+class Band(ctypes.Structure):
+    _fields_ = [("bubble_id", ctypes.c_int64), ("target", ctypes.c_float * 2), ("stiffness", ctypes.c_float),
+                ("maximum_force", ctypes.c_float)]
+
+
+class WorldConfig(ctypes.Structure):
+    _fields_ = [
+        ("time_step", ctypes.c_float), ("contact_tolerance", ctypes.c_float), ("separation_tolerance", ctypes.c_float),
+        ("region", ctypes.c_float * 4), ("global_force", ctypes.c_float * 2), ("apply_region", ctypes.c_uint32),
+        ("candidate_limit", ctypes.c_uint32), ("substep_split", ctypes.c_uint32), ("minimum_margin_only", ctypes.c_uint32),
+        ("grid_bucket_count", ctypes.c_uint32), ("assert_every_substep", ctypes.c_uint32),
+        ("injection_enabled", ctypes.c_uint32), ("injection_after_substep", ctypes.c_uint32),
+        ("injection_object", ctypes.c_int64), ("injection_position", ctypes.c_float * 2),
+        ("force_cooperative", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
+    ]
+
+
+class FailureDetail(ctypes.Structure):
+    _fields_ = [("object_ids", ctypes.c_uint64 * 2), ("violation_kind", ctypes.c_int32),
+                ("edge_segment_id_low", ctypes.c_int32), ("distance", ctypes.c_float), ("limiting_term", ctypes.c_uint32)]
+
+
+class FrameInfo(ctypes.Structure):
+    _fields_ = [
+        ("status", ctypes.c_int32), ("substep_count", ctypes.c_uint32), ("grid_rebuild_count", ctypes.c_uint32),
+        ("candidate_pair_count", ctypes.c_uint32), ("candidate_overflow", ctypes.c_uint32),
+        ("max_candidates_per_object", ctypes.c_uint32), ("execution_path", ctypes.c_uint32),
+        ("group_width", ctypes.c_uint32), ("region_removals", ctypes.c_uint32), ("region_wraps", ctypes.c_uint32),
+        ("gpu_milliseconds", ctypes.c_float), ("simulated_time", ctypes.c_float), ("limiter_object", ctypes.c_uint64),
+        ("limiter_term", ctypes.c_uint32), ("reserved", ctypes.c_uint32), ("detail", FailureDetail),
+    ]
+
+
+class BatchWorldDesc(ctypes.Structure):
+    _fields_ = [
+        ("config", WorldConfig),
+        ("bubbles", ctypes.c_void_p), ("bubble_count", ctypes.c_uint32),
+        ("segments", ctypes.c_void_p), ("segment_count", ctypes.c_uint32),
+        ("bodies", ctypes.c_void_p), ("body_count", ctypes.c_uint32),
+        ("vertices", ctypes.c_void_p), ("vertex_count", ctypes.c_uint32),
+        ("triangles", ctypes.c_void_p), ("triangle_count", ctypes.c_uint32),
+        ("motors", ctypes.c_void_p), ("motor_count", ctypes.c_uint32),
+        ("bands", ctypes.c_void_p), ("band_count", ctypes.c_uint32),
+    ]
+
+
+# numpy views of the record arrays (same layout as the C structures).
+BUBBLE_DTYPE = np.dtype(Bubble)
+SEGMENT_DTYPE = np.dtype(Segment)
+BODY_DTYPE = np.dtype(Body)
+MOTOR_DTYPE = np.dtype(Motor)
+BAND_DTYPE = np.dtype(Band)
+FRAME_DTYPE = np.dtype(FrameInfo)
+
+# Every exported function of the C ABI with its argument and result types.
+_P = ctypes.c_void_p
+_U32, _I32, _I64, _U64, _F = ctypes.c_uint32, ctypes.c_int32, ctypes.c_int64, ctypes.c_uint64, ctypes.c_float
+SIGNATURES = {
+    "bp_abi_version": ([], _U32),
+    "bp_world_config_default": ([_P], None),
+    "bp_status_string": ([_I32], ctypes.c_char_p),
+    "bp_last_error": ([], ctypes.c_char_p),
+    "bp_world_create": ([_P, _P], _I32),
+    "bp_world_destroy": ([_P], None),
+    "bp_world_set_config": ([_P, _P], _I32),
+    "bp_world_set_bubbles": ([_P, _P, _U32], _I32),
+    "bp_world_update_bubbles": ([_P, _P, _P, _U32], _I32),
+    "bp_world_set_segments": ([_P, _P, _U32], _I32),
+    "bp_world_set_bodies": ([_P, _P, _U32, _P, _U32, _P, _U32], _I32),
+    "bp_world_set_forces": ([_P, _P, _U32, _P, _U32], _I32),
+    "bp_world_step": ([_P, _I32, _P], _I32),
+    "bp_world_read_bubbles": ([_P, _P, _U32], _I32),
+    "bp_world_read_bodies": ([_P, _P, _U32], _I32),
+    "bp_world_bubble_count": ([_P], _U32),
+    "bp_world_read_adjacency": ([_P, _P, _U32, _P, _U32, _P], _I32),
+    "bp_world_read_step_history": ([_P, _P, _U32, _P], _I32),
+    "bp_world_debug_fail_next_command": ([_P], _I32),
+    "bp_static_polygon_edges": ([_P, _U32, _I64, _I64, _U32, _P, _U32, _P], _I32),
+    "bp_polygon_body_geometry": ([_P, _U32, _P, _P, _P, _P, _P, _P], _I32),
+    "bp_place_discs": ([_P, _U32, _P, _F, _F, _P, _U32, _U64, _U32, _P, _P], _I32),
+    "bp_batch_create": ([_P], _I32),
+    "bp_batch_destroy": ([_P], None),
+    "bp_batch_add_world": ([_P, _P, _P], _I32),
+    "bp_batch_world_count": ([_P], _U32),
+    "bp_batch_step": ([_P, _U32, _P], _I32),
+    "bp_batch_read_bubbles": ([_P, _U32, _P, _U32], _I32),
+    "bp_batch_last_gpu_milliseconds": ([_P], _F),
+}
+
+
+class LibraryMissing(RuntimeError):
+    """The HIP library is not built; run __graft_entry__.build()."""
+
+
+class BubblePhysicsError(RuntimeError):
+    def __init__(self, status: int, where: str, text: str = ""):
+        self.status = status
+        super().__init__(f"{where}: {STATUS_NAMES.get(status, status)} ({status}) {text}".strip())
+
+
+_lib = None
+
+
+def load_library(path: os.PathLike | None = None) -> ctypes.CDLL:
+    """Loads the HIP library once; raises LibraryMissing when it has not been built."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path is not None else LIB_PATH
+    if not p.exists():
+        raise LibraryMissing(f"{p} is missing: build it with __graft_entry__.build()")
+    lib = ctypes.CDLL(str(p))
+    for name, (args, res) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    if lib.bp_abi_version() != 1:
+        raise RuntimeError("bubblephysics ABI version mismatch")
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def ptr(array: np.ndarray | None) -> int | None:
+    """Address of a C-contiguous numpy array (None for empty or missing arrays)."""
+    if array is None or array.size == 0:
+        return None
+    assert array.flags["C_CONTIGUOUS"], "arrays passed to the C ABI must be contiguous"
+    return array.ctypes.data
+
+
+def check(status: int, where: str) -> None:
+    if status != BP_OK:
+        lib = load_library()
+        raise BubblePhysicsError(status, where, lib.bp_last_error().decode(errors="replace"))
+
+
+def default_config() -> WorldConfig:
+    cfg = WorldConfig()
+    load_library().bp_world_config_default(ctypes.byref(cfg))
+    return cfg
+
+
+def default_config_values() -> WorldConfig:
+    """Reference defaults without loading the library (NativeSolverInput.swift:128-143)."""
+    cfg = WorldConfig()
+    cfg.time_step = 1.0 / 60.0
+    cfg.contact_tolerance = 0.001
+    cfg.separation_tolerance = 0.002
+    cfg.candidate_limit = 128
+    cfg.substep_split = 1
+    cfg.grid_bucket_count = 0
+    return cfg

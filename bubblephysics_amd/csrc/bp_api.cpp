@@ -1,0 +1,780 @@
+// bp_api.cpp -- host side of the C ABI (include/bubblephysics.h).
+//
+// A bp_batch owns one device arena holding every world in structure-of-arrays form. Worlds
+// are grouped by size class; each class (workgroup width 64, 256, 512 or 1024) runs on its
+// own HIP stream, because worlds never exchange data: a class advances all its steps without
+// waiting for the others. A bp_world is a batch of one world.
+//
+// Publication follows NativeMetalSolver.executeSnapshot (NativeMetalSolver.swift:1179-1201):
+// the kernel writes the step into the unpublished side of the state and flips the published
+// side only when the step succeeded, so a failed step leaves the last published state intact.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/bubblephysics.h"
+#include "bp_device.h"
+
+extern "C" hipError_t bp_launch_world_step(int block, const bpd::Arena *arena, bpd::WorldDesc *descs, bpd::WorldStatus *stats,
+                                           const uint32_t *order, uint32_t count, hipStream_t stream);
+
+namespace {
+
+thread_local std::string g_lastError;
+
+void set_error(const std::string &text) { g_lastError = text; }
+
+#define BP_HIP(expr)                                                                                   \
+    do {                                                                                               \
+        hipError_t bp_e = (expr);                                                                      \
+        if (bp_e != hipSuccess) {                                                                      \
+            set_error(std::string(#expr) + ": " + hipGetErrorString(bp_e));                           \
+            return BP_ERR_COMMAND_FAILED;                                                              \
+        }                                                                                              \
+    } while (0)
+
+constexpr uint32_t kClassBlocks[4] = {64, 256, 512, 1024};
+constexpr uint32_t kAdjPerBubble = 64;       // CSR capacity per bubble (grows on overflow)
+
+uint32_t block_for(uint32_t n) {
+    if (n <= 64) return 0;
+    if (n <= 256) return 1;
+    if (n <= 512) return 2;
+    return 3;
+}
+
+uint32_t auto_buckets(uint32_t n) {
+    uint32_t b = 16;
+    while (b < n && b < bpd::kMaxBuckets) b <<= 1;
+    return b;
+}
+
+struct HostWorld {
+    bp_world_config config{};
+    std::vector<bp_bubble> bubbles;
+    std::vector<bp_segment> segments;
+    std::vector<bp_body> bodies;
+    std::vector<float> vertices;        // float2 pairs
+    std::vector<uint32_t> triangles;    // 4 per triangle
+    std::vector<bp_motor> motors;
+    std::vector<bp_band> bands;
+    uint32_t adjPerBubble = kAdjPerBubble;
+};
+
+template <class T> struct DevBuf {
+    T *ptr = nullptr;
+    size_t count = 0;
+};
+
+}  // namespace
+
+struct bp_batch {
+    std::vector<HostWorld> worlds;
+    bool packed = false;
+    bool deviceValid = false;          // device state holds the published state of every world
+    std::vector<void *> allocations;
+    bpd::Arena arena{};
+    bpd::WorldDesc *descs = nullptr;
+    std::vector<bpd::WorldDesc> hostDescs;
+    uint32_t *orders[4] = {nullptr, nullptr, nullptr, nullptr};
+    std::vector<uint32_t> classWorlds[4];
+    hipStream_t streams[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t startEvent = nullptr, endEvents[4] = {nullptr, nullptr, nullptr, nullptr};
+    bpd::WorldStatus *stats = nullptr;
+    size_t statsCapacity = 0;          // in WorldStatus records
+    std::vector<bpd::WorldStatus> hostStats;
+    float lastGpuMs = 0.0f;
+    bool inFlight = false;
+    uint32_t totalBubbles = 0;
+    std::mutex mutex;
+
+    ~bp_batch() { release(); }
+
+    void release() {
+        for (void *p : allocations) (void)hipFree(p);
+        allocations.clear();
+        for (int k = 0; k < 4; ++k) {
+            if (streams[k]) (void)hipStreamDestroy(streams[k]);
+            if (endEvents[k]) (void)hipEventDestroy(endEvents[k]);
+            streams[k] = nullptr; endEvents[k] = nullptr;
+        }
+        if (startEvent) (void)hipEventDestroy(startEvent);
+        startEvent = nullptr;
+        stats = nullptr; statsCapacity = 0; descs = nullptr;
+        for (auto &o : orders) o = nullptr;
+        packed = false;
+    }
+
+    template <class T> bp_status alloc(T *&out, size_t count) {
+        void *p = nullptr;
+        size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e != hipSuccess) { set_error(std::string("hipMalloc: ") + hipGetErrorString(e)); return BP_ERR_ALLOCATION_FAILED; }
+        allocations.push_back(p);
+        out = static_cast<T *>(p);
+        return BP_OK;
+    }
+    template <class T> bp_status upload(T *dst, const std::vector<T> &src) {
+        if (src.empty()) return BP_OK;
+        BP_HIP(hipMemcpy(dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+        return BP_OK;
+    }
+
+    bp_status pack();
+    bp_status download_state();
+    bp_status step(uint32_t steps, bp_frame_info *infos);
+};
+
+namespace {
+
+bp_status validate_world(const HostWorld &w) {
+    const bp_world_config &c = w.config;
+    if (!(c.time_step > 0.0f) || !std::isfinite(c.time_step)) { set_error("time_step must be finite and positive"); return BP_ERR_INVALID_SNAPSHOT; }
+    for (size_t i = 1; i < w.bubbles.size(); ++i)
+        if (!(w.bubbles[i - 1].id < w.bubbles[i].id)) { set_error("bubbles must be sorted by strictly increasing id"); return BP_ERR_INVALID_SNAPSHOT; }
+    for (size_t i = 1; i < w.segments.size(); ++i)
+        if (!(w.segments[i - 1].id < w.segments[i].id)) { set_error("segments must be sorted by strictly increasing id"); return BP_ERR_INVALID_SNAPSHOT; }
+    for (const bp_body &b : w.bodies) {
+        if ((b.flags & (BP_BODY_STATIC | BP_BODY_KINEMATIC)) == 0 && !w.bubbles.empty()) {
+            set_error("dynamic rigid bodies cannot share a world with bubbles");
+            return BP_ERR_INVALID_SNAPSHOT;
+        }
+        if ((uint64_t)b.vertex_offset + b.vertex_count > w.vertices.size() / 2 ||
+            (uint64_t)b.triangle_offset + b.triangle_count > w.triangles.size() / 4) {
+            set_error("body geometry range outside the vertex or triangle arrays");
+            return BP_ERR_INVALID_SNAPSHOT;
+        }
+        if (b.vertex_count == 0) { set_error("body without vertices"); return BP_ERR_INVALID_SNAPSHOT; }
+    }
+    for (size_t t = 0; t + 3 < w.triangles.size(); t += 4) (void)t;
+    if (w.bubbles.size() > (1u << 30) || w.segments.size() > (1u << 30) || w.bodies.size() > (1u << 30)) {
+        set_error("world too large"); return BP_ERR_CAPACITY_OVERFLOW;
+    }
+    return BP_OK;
+}
+
+}  // namespace
+
+bp_status bp_batch::pack() {
+    release();
+    uint32_t nb = 0, ns = 0, nbod = 0, nv = 0, nt = 0, nm = 0, nband = 0;
+    uint64_t adj = 0;
+    for (auto &w : worlds) {
+        bp_status s = validate_world(w);
+        if (s != BP_OK) return s;
+        nb += (uint32_t)w.bubbles.size(); ns += (uint32_t)w.segments.size(); nbod += (uint32_t)w.bodies.size();
+        nv += (uint32_t)(w.vertices.size() / 2); nt += (uint32_t)(w.triangles.size() / 4);
+        nm += (uint32_t)w.motors.size(); nband += (uint32_t)w.bands.size();
+        adj += (uint64_t)std::max<size_t>(w.bubbles.size(), 16) * w.adjPerBubble;
+    }
+    if (adj > 0xffffffffull) { set_error("adjacency capacity exceeds 2^32 entries"); return BP_ERR_CAPACITY_OVERFLOW; }
+    totalBubbles = nb;
+    bpd::Arena &A = arena;
+    bp_status s = BP_OK;
+#define BP_ALLOC(field, count) do { if ((s = alloc(field, count)) != BP_OK) return s; } while (0)
+    for (int side = 0; side < 2; ++side) {
+        BP_ALLOC(A.pos[side], nb); BP_ALLOC(A.start[side], nb); BP_ALLOC(A.vel[side], nb);
+        BP_ALLOC(A.rot[side], nb); BP_ALLOC(A.omega[side], nb); BP_ALLOC(A.region[side], nb);
+        BP_ALLOC(A.bodyPose[side], nbod); BP_ALLOC(A.bodyPrev[side], nbod);
+    }
+    BP_ALLOC(A.id, nb); BP_ALLOC(A.mass, nb); BP_ALLOC(A.invm, nb); BP_ALLOC(A.radius, nb); BP_ALLOC(A.Kc, nb);
+    BP_ALLOC(A.gamma, nb); BP_ALLOC(A.flags, nb); BP_ALLOC(A.cat, nb); BP_ALLOC(A.mask, nb);
+    BP_ALLOC(A.motor, nb); BP_ALLOC(A.band, nb);
+    BP_ALLOC(A.acc, nb); BP_ALLOC(A.a0, nb); BP_ALLOC(A.ref, nb); BP_ALLOC(A.margin, nb);
+    BP_ALLOC(A.rowStart, nb); BP_ALLOC(A.rowLen, nb); BP_ALLOC(A.grid, nb);
+    BP_ALLOC(A.omega2, 1); BP_ALLOC(A.bound, 1);
+    BP_ALLOC(A.adj, adj);
+    BP_ALLOC(A.segAB, ns); BP_ALLOC(A.segRegion, ns); BP_ALLOC(A.segFlags, ns); BP_ALLOC(A.segCat, ns);
+    BP_ALLOC(A.segMask, ns); BP_ALLOC(A.segId, ns); BP_ALLOC(A.segOwner, ns);
+    BP_ALLOC(A.bodyMotion, nbod); BP_ALLOC(A.bodyShape, nbod); BP_ALLOC(A.bodyLimits, nbod);
+    BP_ALLOC(A.bodyRanges, nbod); BP_ALLOC(A.bodyKind, nbod); BP_ALLOC(A.bodyId, nbod);
+    BP_ALLOC(A.verts, nv); BP_ALLOC(A.tris, nt);
+    BP_ALLOC(A.motors, nm); BP_ALLOC(A.bands, nband);
+    BP_ALLOC(A.history, (size_t)worlds.size() * bpd::kFrameLimit);
+    BP_ALLOC(descs, worlds.size());
+    for (int k = 0; k < 4; ++k) BP_ALLOC(orders[k], worlds.size());
+#undef BP_ALLOC
+
+    // Host staging of every array.
+    std::vector<float2> pos(nb), start(nb), vel(nb), bpose2(nbod), verts(nv), zero2(nb, make_float2(0, 0));
+    std::vector<float> rot(nb), om(nb), mass(nb), invm(nb), radius(nb), Kc(nb), gamma(nb);
+    std::vector<uint32_t> flags(nb), cat(nb), mask(nb), region(nb, 0);
+    std::vector<int32_t> motorIdx(nb, -1), bandIdx(nb, -1);
+    std::vector<int64_t> ids(nb), segId(ns), segOwner(ns);
+    std::vector<float4> segAB(ns), segRegion(ns), bodyPose(nbod), bodyMotion(nbod), bodyShape(nbod), motors(nm), bands(nband);
+    std::vector<uint32_t> segFlags(ns), segCat(ns), segMask(ns);
+    std::vector<float2> bodyLimits(nbod);
+    std::vector<uint4> bodyRanges(nbod), bodyKind(nbod), tris(nt);
+    std::vector<uint64_t> bodyId(nbod);
+    hostDescs.assign(worlds.size(), bpd::WorldDesc{});
+    uint32_t ob = 0, os = 0, obd = 0, ov = 0, ot = 0, om_ = 0, oband = 0;
+    uint64_t oadj = 0;
+    for (auto &k : classWorlds) k.clear();
+    for (size_t wi = 0; wi < worlds.size(); ++wi) {
+        const HostWorld &w = worlds[wi];
+        bpd::WorldDesc &d = hostDescs[wi];
+        uint32_t n = (uint32_t)w.bubbles.size();
+        d.bubbleOffset = ob; d.bubbleCount = n;
+        d.segmentOffset = os; d.segmentCount = (uint32_t)w.segments.size();
+        d.bodyOffset = obd; d.bodyCount = (uint32_t)w.bodies.size();
+        d.vertexOffset = ov; d.triangleOffset = ot;
+        d.motorOffset = om_; d.motorCount = (uint32_t)w.motors.size();
+        d.bandOffset = oband; d.bandCount = (uint32_t)w.bands.size();
+        d.adjOffset = (uint32_t)oadj; d.adjCapacity = std::max<uint32_t>(n, 16) * w.adjPerBubble;
+        d.historyOffset = (uint32_t)(wi * bpd::kFrameLimit);
+        d.pub = 0;
+        const bp_world_config &c = w.config;
+        d.dt = c.time_step; d.contactTol = c.contact_tolerance; d.sepTol = c.separation_tolerance;
+        for (int k = 0; k < 4; ++k) d.region[k] = c.region[k];
+        d.globalForce[0] = c.global_force[0]; d.globalForce[1] = c.global_force[1];
+        d.applyRegion = c.apply_region;
+        d.limit = std::min(std::max(c.candidate_limit, 1u), 128u);
+        d.split = std::max(c.substep_split, 1u);
+        d.minimumMarginOnly = c.minimum_margin_only;
+        d.buckets = c.grid_bucket_count == 0 ? auto_buckets(n) : std::min(c.grid_bucket_count, bpd::kMaxBuckets);
+        d.assertEverySubstep = c.assert_every_substep;
+        d.injectionEnabled = c.injection_enabled; d.injectionAfter = c.injection_after_substep;
+        d.injectionObject = c.injection_object;
+        d.injectionPosition[0] = c.injection_position[0]; d.injectionPosition[1] = c.injection_position[1];
+        d.failed = 0;
+        for (uint32_t i = 0; i < n; ++i) {
+            const bp_bubble &b = w.bubbles[i];
+            uint32_t g = ob + i;
+            pos[g] = make_float2(b.center[0], b.center[1]);
+            start[g] = make_float2(b.previous_center[0], b.previous_center[1]);
+            vel[g] = make_float2(b.velocity[0], b.velocity[1]);
+            rot[g] = b.rotation; om[g] = b.angular_velocity;
+            mass[g] = b.mass; invm[g] = b.inverse_mass; radius[g] = b.radius; Kc[g] = b.contact_stiffness;
+            gamma[g] = b.damping; flags[g] = b.flags; cat[g] = b.category; mask[g] = b.mask; ids[g] = b.id;
+        }
+        // Motor and band records are resolved to bubble indices once (ids are unique and sorted).
+        std::vector<bp_motor> ms = w.motors;
+        std::sort(ms.begin(), ms.end(), [](const bp_motor &a, const bp_motor &b) { return a.bubble_id < b.bubble_id; });
+        for (uint32_t k = 0; k < ms.size(); ++k) {
+            motors[om_ + k] = make_float4(ms[k].force[0], ms[k].force[1], ms[k].torque, ms[k].inertia);
+            auto it = std::lower_bound(w.bubbles.begin(), w.bubbles.end(), ms[k].bubble_id,
+                                       [](const bp_bubble &b, int64_t id) { return b.id < id; });
+            if (it != w.bubbles.end() && it->id == ms[k].bubble_id) motorIdx[ob + (it - w.bubbles.begin())] = (int32_t)k;
+        }
+        std::vector<bp_band> bs = w.bands;
+        std::sort(bs.begin(), bs.end(), [](const bp_band &a, const bp_band &b) { return a.bubble_id < b.bubble_id; });
+        for (uint32_t k = 0; k < bs.size(); ++k) {
+            bands[oband + k] = make_float4(bs[k].target[0], bs[k].target[1], bs[k].stiffness, bs[k].maximum_force);
+            auto it = std::lower_bound(w.bubbles.begin(), w.bubbles.end(), bs[k].bubble_id,
+                                       [](const bp_bubble &b, int64_t id) { return b.id < id; });
+            if (it != w.bubbles.end() && it->id == bs[k].bubble_id) bandIdx[ob + (it - w.bubbles.begin())] = (int32_t)k;
+        }
+        for (uint32_t j = 0; j < w.segments.size(); ++j) {
+            const bp_segment &sg = w.segments[j];
+            uint32_t g = os + j;
+            segAB[g] = make_float4(sg.a[0], sg.a[1], sg.b[0], sg.b[1]);
+            segRegion[g] = make_float4(sg.previous_start[0], sg.previous_start[1], sg.outward_sign, sg.allowed_side);
+            segFlags[g] = sg.flags; segCat[g] = sg.category; segMask[g] = sg.mask; segId[g] = sg.id; segOwner[g] = sg.owner_id;
+        }
+        for (uint32_t k = 0; k < w.bodies.size(); ++k) {
+            const bp_body &b = w.bodies[k];
+            uint32_t g = obd + k;
+            bodyPose[g] = make_float4(b.pose[0], b.pose[1], b.pose[2], b.previous[2]);
+            bpose2[g] = make_float2(b.previous[0], b.previous[1]);
+            bodyMotion[g] = make_float4(b.velocity[0], b.velocity[1], b.velocity[2], 0.0f);
+            bodyShape[g] = make_float4(b.bounding_radius, b.core_radius, b.half_length, 0.0f);
+            bodyLimits[g] = make_float2(b.max_linear_speed, b.max_angular_speed);
+            bodyRanges[g] = make_uint4(b.vertex_offset, b.vertex_count, b.triangle_offset, b.triangle_count);
+            bodyKind[g] = make_uint4(b.shape, b.flags, b.category, b.mask);
+            bodyId[g] = b.id;
+        }
+        for (size_t v = 0; v < w.vertices.size() / 2; ++v) verts[ov + v] = make_float2(w.vertices[2 * v], w.vertices[2 * v + 1]);
+        for (size_t t = 0; t < w.triangles.size() / 4; ++t)
+            tris[ot + t] = make_uint4(w.triangles[4 * t], w.triangles[4 * t + 1], w.triangles[4 * t + 2], w.triangles[4 * t + 3]);
+        classWorlds[block_for(n)].push_back((uint32_t)wi);
+        ob += n; os += (uint32_t)w.segments.size(); obd += (uint32_t)w.bodies.size();
+        ov += (uint32_t)(w.vertices.size() / 2); ot += (uint32_t)(w.triangles.size() / 4);
+        om_ += (uint32_t)w.motors.size(); oband += (uint32_t)w.bands.size();
+        oadj += (uint64_t)d.adjCapacity;
+    }
+    // Longest-processing-time order inside each class: the largest worlds start first.
+    for (auto &k : classWorlds)
+        std::stable_sort(k.begin(), k.end(), [&](uint32_t a, uint32_t b) { return hostDescs[a].bubbleCount > hostDescs[b].bubbleCount; });
+    if ((s = upload(A.pos[0], pos)) || (s = upload(A.start[0], start)) || (s = upload(A.vel[0], vel)) ||
+        (s = upload(A.rot[0], rot)) || (s = upload(A.omega[0], om)) || (s = upload(A.region[0], region)) ||
+        (s = upload(A.id, ids)) || (s = upload(A.mass, mass)) || (s = upload(A.invm, invm)) || (s = upload(A.radius, radius)) ||
+        (s = upload(A.Kc, Kc)) || (s = upload(A.gamma, gamma)) || (s = upload(A.flags, flags)) || (s = upload(A.cat, cat)) ||
+        (s = upload(A.mask, mask)) || (s = upload(A.motor, motorIdx)) || (s = upload(A.band, bandIdx)) ||
+        (s = upload(A.acc, zero2)) || (s = upload(A.segAB, segAB)) || (s = upload(A.segRegion, segRegion)) ||
+        (s = upload(A.segFlags, segFlags)) || (s = upload(A.segCat, segCat)) || (s = upload(A.segMask, segMask)) ||
+        (s = upload(A.segId, segId)) || (s = upload(A.segOwner, segOwner)) || (s = upload(A.bodyPose[0], bodyPose)) ||
+        (s = upload(A.bodyPrev[0], bpose2)) || (s = upload(A.bodyMotion, bodyMotion)) || (s = upload(A.bodyShape, bodyShape)) ||
+        (s = upload(A.bodyLimits, bodyLimits)) || (s = upload(A.bodyRanges, bodyRanges)) || (s = upload(A.bodyKind, bodyKind)) ||
+        (s = upload(A.bodyId, bodyId)) || (s = upload(A.verts, verts)) || (s = upload(A.tris, tris)) ||
+        (s = upload(A.motors, motors)) || (s = upload(A.bands, bands)) || (s = upload(descs, hostDescs)))
+        return s;
+    for (int k = 0; k < 4; ++k)
+        if ((s = upload(orders[k], classWorlds[k])) != BP_OK) return s;
+    for (int k = 0; k < 4; ++k) {
+        BP_HIP(hipStreamCreateWithFlags(&streams[k], hipStreamNonBlocking));
+        BP_HIP(hipEventCreate(&endEvents[k]));
+    }
+    BP_HIP(hipEventCreate(&startEvent));
+    packed = true;
+    deviceValid = true;
+    return BP_OK;
+}
+
+// Copies the published state of every world back into the host mirror.
+bp_status bp_batch::download_state() {
+    if (!packed || !deviceValid) return BP_OK;
+    BP_HIP(hipMemcpy(hostDescs.data(), descs, hostDescs.size() * sizeof(bpd::WorldDesc), hipMemcpyDeviceToHost));
+    uint32_t nb = totalBubbles;
+    std::vector<float2> pos[2], start[2], vel[2];
+    std::vector<float> rot[2], om[2];
+    size_t nbod = 0;
+    for (auto &w : worlds) nbod += w.bodies.size();
+    std::vector<float4> bodyPose[2];
+    std::vector<float2> bodyPrev[2];
+    std::vector<float4> bodyMotion(nbod);
+    for (int side = 0; side < 2; ++side) {
+        pos[side].resize(nb); start[side].resize(nb); vel[side].resize(nb); rot[side].resize(nb); om[side].resize(nb);
+        bodyPose[side].resize(nbod); bodyPrev[side].resize(nbod);
+        if (nb) {
+            BP_HIP(hipMemcpy(pos[side].data(), arena.pos[side], nb * sizeof(float2), hipMemcpyDeviceToHost));
+            BP_HIP(hipMemcpy(start[side].data(), arena.start[side], nb * sizeof(float2), hipMemcpyDeviceToHost));
+            BP_HIP(hipMemcpy(vel[side].data(), arena.vel[side], nb * sizeof(float2), hipMemcpyDeviceToHost));
+            BP_HIP(hipMemcpy(rot[side].data(), arena.rot[side], nb * sizeof(float), hipMemcpyDeviceToHost));
+            BP_HIP(hipMemcpy(om[side].data(), arena.omega[side], nb * sizeof(float), hipMemcpyDeviceToHost));
+        }
+        if (nbod) {
+            BP_HIP(hipMemcpy(bodyPose[side].data(), arena.bodyPose[side], nbod * sizeof(float4), hipMemcpyDeviceToHost));
+            BP_HIP(hipMemcpy(bodyPrev[side].data(), arena.bodyPrev[side], nbod * sizeof(float2), hipMemcpyDeviceToHost));
+        }
+    }
+    if (nbod) BP_HIP(hipMemcpy(bodyMotion.data(), arena.bodyMotion, nbod * sizeof(float4), hipMemcpyDeviceToHost));
+    for (size_t wi = 0; wi < worlds.size(); ++wi) {
+        HostWorld &w = worlds[wi];
+        const bpd::WorldDesc &d = hostDescs[wi];
+        uint32_t p = d.pub;
+        for (uint32_t i = 0; i < w.bubbles.size(); ++i) {
+            uint32_t g = d.bubbleOffset + i;
+            bp_bubble &b = w.bubbles[i];
+            b.center[0] = pos[p][g].x; b.center[1] = pos[p][g].y;
+            b.previous_center[0] = start[p][g].x; b.previous_center[1] = start[p][g].y;
+            b.velocity[0] = vel[p][g].x; b.velocity[1] = vel[p][g].y;
+            b.rotation = rot[p][g]; b.angular_velocity = om[p][g];
+        }
+        for (uint32_t k = 0; k < w.bodies.size(); ++k) {
+            uint32_t g = d.bodyOffset + k;
+            bp_body &b = w.bodies[k];
+            b.pose[0] = bodyPose[p][g].x; b.pose[1] = bodyPose[p][g].y; b.pose[2] = bodyPose[p][g].z;
+            b.previous[0] = bodyPrev[p][g].x; b.previous[1] = bodyPrev[p][g].y; b.previous[2] = bodyPose[p][g].w;
+            b.velocity[0] = bodyMotion[g].x; b.velocity[1] = bodyMotion[g].y; b.velocity[2] = bodyMotion[g].z;
+        }
+    }
+    return BP_OK;
+}
+
+static void fill_info(const bpd::WorldStatus &s, bp_frame_info *out) {
+    std::memset(out, 0, sizeof *out);
+    out->status = s.status;
+    out->substep_count = s.substeps;
+    out->grid_rebuild_count = s.rebuilds;
+    out->candidate_pair_count = s.candidatePairs;
+    out->candidate_overflow = s.candidateOverflow;
+    out->max_candidates_per_object = s.maxCandidates;
+    out->execution_path = BP_PATH_WORKGROUP;
+    out->group_width = s.groupWidth;
+    out->region_removals = s.removals;
+    out->region_wraps = s.wraps;
+    out->simulated_time = s.time[1];
+    out->limiter_object = s.limiterObject;
+    out->limiter_term = s.limiterTerm;
+    out->detail.object_ids[0] = s.violationObjects[0];
+    out->detail.object_ids[1] = s.violationObjects[1];
+    out->detail.violation_kind = (int32_t)s.violationKind;
+    out->detail.edge_segment_id_low = s.violationKind == BP_VIOLATION_EDGE_SIDE ? (int32_t)(uint32_t)s.violationSegment : -1;
+    out->detail.distance = s.violationKind == BP_VIOLATION_NONE || s.violationKind == BP_VIOLATION_INSIDE_BODY
+                               ? std::nanf("") : s.violationDetail;
+    out->detail.limiting_term = s.status == BP_ERR_SUBSTEP_LIMIT_EXCEEDED ? s.limiterTerm : BP_TERM_NONE;
+    if (s.status == BP_ERR_SUBSTEP_LIMIT_EXCEEDED) {
+        out->detail.object_ids[0] = s.limiterObject;
+        out->detail.object_ids[1] = 0;
+    }
+}
+
+bp_status bp_batch::step(uint32_t steps, bp_frame_info *infos) {
+    if (!packed) {
+        bp_status s = pack();
+        if (s != BP_OK) return s;
+    }
+    size_t W = worlds.size();
+    if (W == 0 || steps == 0) return BP_OK;
+    size_t need = (size_t)steps * W;
+    if (statsCapacity < need) {
+        bp_status s = alloc(stats, need);
+        if (s != BP_OK) return s;
+        statsCapacity = need;
+    }
+    BP_HIP(hipMemset(stats, 0, need * sizeof(bpd::WorldStatus)));
+    BP_HIP(hipDeviceSynchronize());
+    BP_HIP(hipEventRecord(startEvent, streams[0]));
+    for (int k = 1; k < 4; ++k) BP_HIP(hipStreamWaitEvent(streams[k], startEvent, 0));
+    for (int k = 0; k < 4; ++k) {
+        uint32_t count = (uint32_t)classWorlds[k].size();
+        for (uint32_t st = 0; st < steps && count; ++st) {
+            hipError_t e = bp_launch_world_step((int)kClassBlocks[k], &arena, descs, stats + (size_t)st * W, orders[k], count, streams[k]);
+            if (e != hipSuccess) { set_error(std::string("kernel launch: ") + hipGetErrorString(e)); return BP_ERR_COMMAND_FAILED; }
+        }
+        BP_HIP(hipEventRecord(endEvents[k], streams[k]));
+    }
+    for (int k = 0; k < 4; ++k) BP_HIP(hipEventSynchronize(endEvents[k]));
+    float ms = 0.0f;
+    for (int k = 0; k < 4; ++k) {
+        float e = 0.0f;
+        BP_HIP(hipEventElapsedTime(&e, startEvent, endEvents[k]));
+        ms = std::max(ms, e);
+    }
+    lastGpuMs = ms;
+    hostStats.resize(need);
+    BP_HIP(hipMemcpy(hostStats.data(), stats, need * sizeof(bpd::WorldStatus), hipMemcpyDeviceToHost));
+    if (infos)
+        for (size_t i = 0; i < need; ++i) {
+            fill_info(hostStats[i], &infos[i]);
+            infos[i].gpu_milliseconds = ms / steps;
+            if (!hostStats[i].executed) infos[i].status = BP_ERR_UNAVAILABLE;
+        }
+    return BP_OK;
+}
+
+// ---------------------------------------------------------------------------------------
+// C ABI.
+// ---------------------------------------------------------------------------------------
+extern "C" {
+
+uint32_t bp_abi_version(void) { return BP_ABI_VERSION; }
+
+void bp_world_config_default(bp_world_config *c) {
+    if (!c) return;
+    std::memset(c, 0, sizeof *c);
+    c->time_step = 1.0f / 60.0f;
+    c->contact_tolerance = 0.001f;
+    c->separation_tolerance = 0.002f;
+    c->candidate_limit = 128;
+    c->substep_split = 1;
+    c->grid_bucket_count = 0;    // automatic: next power of two >= bubble count, up to 8192
+}
+
+const char *bp_status_string(bp_status s) {
+    switch (s) {
+        case BP_OK: return "ok";
+        case BP_ERR_INVALID_ARGUMENT: return "invalid argument";
+        case BP_ERR_RESOURCE_UNAVAILABLE: return "resource unavailable";
+        case BP_ERR_ALLOCATION_FAILED: return "allocation failed";
+        case BP_ERR_COMMAND_SETUP_FAILED: return "command setup failed";
+        case BP_ERR_COMMAND_FAILED: return "command failed";
+        case BP_ERR_INVALID_SNAPSHOT: return "invalid snapshot";
+        case BP_ERR_CAPACITY_OVERFLOW: return "capacity overflow";
+        case BP_ERR_FRAME_IN_FLIGHT: return "frame in flight";
+        case BP_ERR_NON_FINITE: return "non-finite state";
+        case BP_ERR_GUARD_FAILURE: return "guard failure";
+        case BP_ERR_CONTACT_VIOLATION: return "contact violation";
+        case BP_ERR_SUBSTEP_LIMIT_EXCEEDED: return "substep limit exceeded";
+        case BP_ERR_UNAVAILABLE: return "unavailable";
+        default: return "unknown status";
+    }
+}
+
+const char *bp_last_error(void) { return g_lastError.c_str(); }
+
+static bp_status device_available() {
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count == 0) {
+        set_error("no HIP device is available");
+        return BP_ERR_RESOURCE_UNAVAILABLE;
+    }
+    return BP_OK;
+}
+
+// ---- batch ----
+bp_status bp_batch_create(bp_batch **out) {
+    if (!out) return BP_ERR_INVALID_ARGUMENT;
+    bp_status s = device_available();
+    if (s != BP_OK) return s;
+    *out = new bp_batch();
+    return BP_OK;
+}
+
+void bp_batch_destroy(bp_batch *b) { delete b; }
+
+static bp_status world_from_desc(HostWorld &w, const bp_batch_world_desc *d) {
+    if (!d) return BP_ERR_INVALID_ARGUMENT;
+    if ((d->bubble_count && !d->bubbles) || (d->segment_count && !d->segments) || (d->body_count && !d->bodies) ||
+        (d->vertex_count && !d->vertices) || (d->triangle_count && !d->triangles) || (d->motor_count && !d->motors) ||
+        (d->band_count && !d->bands)) {
+        set_error("null array with a non-zero count");
+        return BP_ERR_INVALID_ARGUMENT;
+    }
+    w.config = d->config;
+    w.bubbles.assign(d->bubbles, d->bubbles + d->bubble_count);
+    w.segments.assign(d->segments, d->segments + d->segment_count);
+    w.bodies.assign(d->bodies, d->bodies + d->body_count);
+    w.vertices.assign(d->vertices, d->vertices + 2 * (size_t)d->vertex_count);
+    w.triangles.assign(d->triangles, d->triangles + 4 * (size_t)d->triangle_count);
+    w.motors.assign(d->motors, d->motors + d->motor_count);
+    w.bands.assign(d->bands, d->bands + d->band_count);
+    return validate_world(w);
+}
+
+bp_status bp_batch_add_world(bp_batch *b, const bp_batch_world_desc *desc, uint32_t *out_index) {
+    if (!b) return BP_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(b->mutex);
+    if (b->inFlight) return BP_ERR_FRAME_IN_FLIGHT;
+    bp_status s = b->download_state();
+    if (s != BP_OK) return s;
+    HostWorld w;
+    s = world_from_desc(w, desc);
+    if (s != BP_OK) return s;
+    b->worlds.push_back(std::move(w));
+    b->packed = false;
+    if (out_index) *out_index = (uint32_t)(b->worlds.size() - 1);
+    return BP_OK;
+}
+
+uint32_t bp_batch_world_count(const bp_batch *b) { return b ? (uint32_t)b->worlds.size() : 0u; }
+
+bp_status bp_batch_step(bp_batch *b, uint32_t steps, bp_frame_info *infos) {
+    if (!b) return BP_ERR_INVALID_ARGUMENT;
+    std::unique_lock<std::mutex> lock(b->mutex, std::try_to_lock);
+    if (!lock.owns_lock() || b->inFlight) return BP_ERR_FRAME_IN_FLIGHT;
+    b->inFlight = true;
+    bp_status s = b->step(steps, infos);
+    b->inFlight = false;
+    return s;
+}
+
+bp_status bp_batch_read_bubbles(bp_batch *b, uint32_t world, bp_bubble *out, uint32_t count) {
+    if (!b || world >= b->worlds.size()) return BP_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(b->mutex);
+    bp_status s = b->download_state();
+    if (s != BP_OK) return s;
+    const HostWorld &w = b->worlds[world];
+    if (count != w.bubbles.size() || (count && !out)) return BP_ERR_INVALID_ARGUMENT;
+    std::copy(w.bubbles.begin(), w.bubbles.end(), out);
+    return BP_OK;
+}
+
+float bp_batch_last_gpu_milliseconds(const bp_batch *b) { return b ? b->lastGpuMs : 0.0f; }
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------------------
+// Plain world: a batch of one world (NativeMetalSolver).
+// ---------------------------------------------------------------------------------------
+struct bp_world {
+    bp_batch batch;
+    bool failNext = false;
+    bool inFlight = false;
+    std::vector<float> history;
+    std::vector<uint32_t> rowOffsets, entries;
+};
+
+extern "C" {
+
+bp_status bp_world_create(const bp_world_config *config, bp_world **out) {
+    if (!config || !out) return BP_ERR_INVALID_ARGUMENT;
+    bp_status s = device_available();
+    if (s != BP_OK) return s;
+    bp_world *w = new bp_world();
+    HostWorld hw;
+    hw.config = *config;
+    w->batch.worlds.push_back(hw);
+    *out = w;
+    return BP_OK;
+}
+
+void bp_world_destroy(bp_world *w) { delete w; }
+
+static HostWorld &host_of(bp_world *w) { return w->batch.worlds[0]; }
+
+static bp_status sync_host(bp_world *w) {
+    bp_status s = w->batch.download_state();
+    w->batch.packed = false;
+    return s;
+}
+
+bp_status bp_world_set_config(bp_world *w, const bp_world_config *c) {
+    if (!w || !c) return BP_ERR_INVALID_ARGUMENT;
+    if (w->inFlight) return BP_ERR_FRAME_IN_FLIGHT;
+    bp_status s = sync_host(w);
+    if (s != BP_OK) return s;
+    host_of(w).config = *c;
+    return BP_OK;
+}
+
+bp_status bp_world_set_bubbles(bp_world *w, const bp_bubble *bubbles, uint32_t count) {
+    if (!w || (count && !bubbles)) return BP_ERR_INVALID_ARGUMENT;
+    if (w->inFlight) return BP_ERR_FRAME_IN_FLIGHT;
+    bp_status s = sync_host(w);
+    if (s != BP_OK) return s;
+    for (uint32_t i = 1; i < count; ++i)
+        if (!(bubbles[i - 1].id < bubbles[i].id)) { set_error("bubbles must be sorted by strictly increasing id"); return BP_ERR_INVALID_SNAPSHOT; }
+    host_of(w).bubbles.assign(bubbles, bubbles + count);
+    return BP_OK;
+}
+
+bp_status bp_world_update_bubbles(bp_world *w, const uint32_t *indices, const bp_bubble *bubbles, uint32_t count) {
+    if (!w || (count && (!indices || !bubbles))) return BP_ERR_INVALID_ARGUMENT;
+    if (w->inFlight) return BP_ERR_FRAME_IN_FLIGHT;
+    bp_status s = sync_host(w);
+    if (s != BP_OK) return s;
+    HostWorld &hw = host_of(w);
+    for (uint32_t k = 0; k < count; ++k) {
+        if (indices[k] >= hw.bubbles.size()) return BP_ERR_INVALID_SNAPSHOT;
+        if (bubbles[k].id != hw.bubbles[indices[k]].id) { set_error("delta changes the id of a bubble"); return BP_ERR_INVALID_SNAPSHOT; }
+    }
+    for (uint32_t k = 0; k < count; ++k) hw.bubbles[indices[k]] = bubbles[k];
+    return BP_OK;
+}
+
+bp_status bp_world_set_segments(bp_world *w, const bp_segment *segments, uint32_t count) {
+    if (!w || (count && !segments)) return BP_ERR_INVALID_ARGUMENT;
+    if (w->inFlight) return BP_ERR_FRAME_IN_FLIGHT;
+    bp_status s = sync_host(w);
+    if (s != BP_OK) return s;
+    host_of(w).segments.assign(segments, segments + count);
+    return BP_OK;
+}
+
+bp_status bp_world_set_bodies(bp_world *w, const bp_body *bodies, uint32_t count, const float *vertices, uint32_t vertex_count,
+                              const uint32_t *triangles, uint32_t triangle_count) {
+    if (!w || (count && !bodies) || (vertex_count && !vertices) || (triangle_count && !triangles)) return BP_ERR_INVALID_ARGUMENT;
+    if (w->inFlight) return BP_ERR_FRAME_IN_FLIGHT;
+    bp_status s = sync_host(w);
+    if (s != BP_OK) return s;
+    HostWorld &hw = host_of(w);
+    hw.bodies.assign(bodies, bodies + count);
+    hw.vertices.assign(vertices, vertices + 2 * (size_t)vertex_count);
+    hw.triangles.assign(triangles, triangles + 4 * (size_t)triangle_count);
+    return BP_OK;
+}
+
+bp_status bp_world_set_forces(bp_world *w, const bp_motor *motors, uint32_t motor_count, const bp_band *bands, uint32_t band_count) {
+    if (!w || (motor_count && !motors) || (band_count && !bands)) return BP_ERR_INVALID_ARGUMENT;
+    if (w->inFlight) return BP_ERR_FRAME_IN_FLIGHT;
+    bp_status s = sync_host(w);
+    if (s != BP_OK) return s;
+    HostWorld &hw = host_of(w);
+    hw.motors.assign(motors, motors + motor_count);
+    hw.bands.assign(bands, bands + band_count);
+    return BP_OK;
+}
+
+bp_status bp_world_step(bp_world *w, int32_t scenario_step, bp_frame_info *out) {
+    (void)scenario_step;
+    if (!w) return BP_ERR_INVALID_ARGUMENT;
+    if (w->inFlight) return BP_ERR_FRAME_IN_FLIGHT;
+    if (w->failNext) {
+        w->failNext = false;
+        set_error("command executor failure requested by the test switch");
+        if (out) { std::memset(out, 0, sizeof *out); out->status = BP_ERR_COMMAND_FAILED; }
+        return BP_ERR_COMMAND_FAILED;
+    }
+    w->inFlight = true;
+    bp_frame_info info{};
+    bp_status s = BP_OK;
+    for (int attempt = 0; attempt < 4; ++attempt) {
+        s = w->batch.step(1, &info);
+        if (s != BP_OK) break;
+        if (info.status != BP_ERR_CAPACITY_OVERFLOW) break;
+        // The adjacency did not fit: nothing was published; grow and run the step again.
+        HostWorld &hw = host_of(w);
+        s = w->batch.download_state();
+        if (s != BP_OK) break;
+        hw.adjPerBubble *= 4;
+        w->batch.packed = false;
+    }
+    w->inFlight = false;
+    if (s != BP_OK) {
+        if (out) { std::memset(out, 0, sizeof *out); out->status = s; }
+        return s;
+    }
+    // A failed world keeps its published state and may be stepped again after new input.
+    if (info.status != BP_OK) {
+        bpd::WorldDesc d;
+        if (hipMemcpy(&d, w->batch.descs, sizeof d, hipMemcpyDeviceToHost) == hipSuccess && d.failed) {
+            d.failed = 0;
+            (void)hipMemcpy(w->batch.descs, &d, sizeof d, hipMemcpyHostToDevice);
+        }
+    }
+    if (out) *out = info;
+    return info.status;
+}
+
+bp_status bp_world_read_bubbles(bp_world *w, bp_bubble *out, uint32_t count) {
+    if (!w) return BP_ERR_INVALID_ARGUMENT;
+    bp_status s = w->batch.download_state();
+    if (s != BP_OK) return s;
+    const HostWorld &hw = host_of(w);
+    if (count != hw.bubbles.size() || (count && !out)) return BP_ERR_INVALID_ARGUMENT;
+    std::copy(hw.bubbles.begin(), hw.bubbles.end(), out);
+    return BP_OK;
+}
+
+bp_status bp_world_read_bodies(bp_world *w, bp_body *out, uint32_t count) {
+    if (!w) return BP_ERR_INVALID_ARGUMENT;
+    bp_status s = w->batch.download_state();
+    if (s != BP_OK) return s;
+    const HostWorld &hw = host_of(w);
+    if (count != hw.bodies.size() || (count && !out)) return BP_ERR_INVALID_ARGUMENT;
+    std::copy(hw.bodies.begin(), hw.bodies.end(), out);
+    return BP_OK;
+}
+
+uint32_t bp_world_bubble_count(const bp_world *w) { return w ? (uint32_t)w->batch.worlds[0].bubbles.size() : 0u; }
+
+bp_status bp_world_read_adjacency(bp_world *w, uint32_t *row_offsets, uint32_t offsets_count, uint32_t *entries,
+                                  uint32_t entries_capacity, uint32_t *out_entry_count) {
+    if (!w || !w->batch.packed) return BP_ERR_INVALID_ARGUMENT;
+    const bpd::WorldDesc &d = w->batch.hostDescs[0];
+    uint32_t n = d.bubbleCount;
+    if (!row_offsets || offsets_count != n + 1) return BP_ERR_INVALID_ARGUMENT;
+    std::vector<uint32_t> start(n), len(n);
+    if (n) {
+        BP_HIP(hipMemcpy(start.data(), w->batch.arena.rowStart + d.bubbleOffset, n * 4, hipMemcpyDeviceToHost));
+        BP_HIP(hipMemcpy(len.data(), w->batch.arena.rowLen + d.bubbleOffset, n * 4, hipMemcpyDeviceToHost));
+    }
+    uint32_t total = 0;
+    for (uint32_t i = 0; i < n; ++i) { row_offsets[i] = total; total += len[i]; }
+    row_offsets[n] = total;
+    if (out_entry_count) *out_entry_count = total;
+    if (entries && total) {
+        if (entries_capacity < total) return BP_ERR_CAPACITY_OVERFLOW;
+        uint32_t cap = d.adjCapacity;
+        std::vector<uint32_t> all(cap);
+        BP_HIP(hipMemcpy(all.data(), w->batch.arena.adj + d.adjOffset, (size_t)cap * 4, hipMemcpyDeviceToHost));
+        for (uint32_t i = 0; i < n; ++i)
+            for (uint32_t k = 0; k < len[i]; ++k) entries[row_offsets[i] + k] = all[start[i] + k];
+    }
+    return BP_OK;
+}
+
+bp_status bp_world_read_step_history(bp_world *w, float *out, uint32_t capacity, uint32_t *out_count) {
+    if (!w || !w->batch.packed || !out) return BP_ERR_INVALID_ARGUMENT;
+    uint32_t count = std::min<uint32_t>(capacity, bpd::kFrameLimit);
+    const bpd::WorldDesc &d = w->batch.hostDescs[0];
+    BP_HIP(hipMemcpy(out, w->batch.arena.history + d.historyOffset, count * sizeof(float), hipMemcpyDeviceToHost));
+    if (out_count) *out_count = w->batch.hostStats.empty() ? 0u : std::min(w->batch.hostStats[0].substeps, count);
+    return BP_OK;
+}
+
+bp_status bp_world_debug_fail_next_command(bp_world *w) {
+    if (!w) return BP_ERR_INVALID_ARGUMENT;
+    w->failNext = true;
+    return BP_OK;
+}
+
+}  // extern "C"

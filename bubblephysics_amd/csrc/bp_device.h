@@ -1,0 +1,340 @@
+// bp_device.h -- device-side records and per-bubble physics of the world step (gfx950).
+//
+// The functions here are the per-object arithmetic of the BubblePhysics 4.0.0 step, in the
+// order the reference evaluates it, so that the FP32 results round identically to the CPU
+// oracle. Citations are to the reference repository:
+//   Force    = Sources/BubblePhysicsNativeMetal/Shaders/NativeForceKernels.metal-source
+//   Geometry = Sources/BubblePhysicsNativeMetal/Shaders/NativeGeometryKernels.metal-source
+// This translation unit is compiled with -ffp-contract=off: every + - * / sqrt rounds once,
+// and the only fused multiply-add is the explicit one of past_segment_end.
+#ifndef BP_DEVICE_H
+#define BP_DEVICE_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "bp_detmath.h"
+
+namespace bpd {
+
+#define BPD __device__ __forceinline__
+
+// ---------------------------------------------------------------------------------------
+// Arena: structure-of-arrays device storage of every world of a batch.
+// Per-bubble dynamic state exists twice: side `pub` of a world is its published state,
+// the other side is written by the step and becomes published only when the step succeeds.
+// ---------------------------------------------------------------------------------------
+struct Arena {
+    // Bubble state, two sides.
+    float2 *pos[2];
+    float2 *start[2];
+    float2 *vel[2];
+    float *rot[2];
+    float *omega[2];
+    uint32_t *region[2];
+    // Bubble properties.
+    int64_t *id;
+    float *mass, *invm, *radius, *Kc, *gamma;
+    uint32_t *flags;          // BP_BUBBLE_* bits
+    uint32_t *cat, *mask;
+    int32_t *motor, *band;    // per-bubble record index in the world's motor/band arrays, -1 none
+    // Step scratch, per bubble.
+    float2 *acc, *a0, *ref;
+    float *margin;
+    uint32_t *rowStart, *rowLen, *grid;
+    float *omega2, *bound;    // per-bubble reduction inputs (debug / cooperative path)
+    // Adjacency entries: (kind << 30) | neighbour, CSR per world.
+    uint32_t *adj;
+    // Segments.
+    float4 *segAB;            // a.xy, b.xy
+    float4 *segRegion;        // previous_start.xy, outward sign, allowed side
+    uint32_t *segFlags, *segCat, *segMask;
+    int64_t *segId, *segOwner;
+    // Bodies: pose (x, y, angle, previous angle) and previous centre, two sides.
+    float4 *bodyPose[2];
+    float2 *bodyPrev[2];
+    float4 *bodyMotion;       // vx, vy, omega, damping (0)
+    float4 *bodyShape;        // bounding radius, core radius, half length, unused
+    float2 *bodyLimits;       // max linear, max angular speed
+    uint4 *bodyRanges;        // vertex offset/count, triangle offset/count (world-relative)
+    uint4 *bodyKind;          // shape, flags, category, mask
+    uint64_t *bodyId;
+    float2 *verts;
+    uint4 *tris;
+    float4 *motors;           // force.xy, torque, inertia
+    float4 *bands;            // target.xy, stiffness, maximum force
+    float *history;           // 1024 h values per world
+};
+
+// Per-world constants and the published side. Written by the host, read by the kernel;
+// `pub` and `failed` are updated by the kernel at the end of each step.
+struct WorldDesc {
+    uint32_t bubbleOffset, bubbleCount;
+    uint32_t segmentOffset, segmentCount;
+    uint32_t bodyOffset, bodyCount;
+    uint32_t vertexOffset, triangleOffset;
+    uint32_t motorOffset, motorCount;
+    uint32_t bandOffset, bandCount;
+    uint32_t adjOffset, adjCapacity;
+    uint32_t historyOffset;
+    uint32_t pub;
+    float dt, contactTol, sepTol;
+    float region[4];
+    float globalForce[2];
+    uint32_t applyRegion, limit, split, minimumMarginOnly, buckets;
+    uint32_t assertEverySubstep, injectionEnabled, injectionAfter;
+    int64_t injectionObject;
+    float injectionPosition[2];
+    uint32_t failed;          // sticky: a world that failed a step is not advanced again
+    uint32_t reserved;
+};
+
+// Per-world result of the last step (bp_frame_info fields).
+struct WorldStatus {
+    int32_t status;
+    uint32_t substeps, rebuilds, candidatePairs, candidateOverflow, maxCandidates;
+    uint32_t removals, wraps;
+    float time[4];
+    uint64_t limiterObject;
+    uint32_t limiterTerm;
+    uint32_t violationKind;
+    int64_t violationSegment;
+    uint64_t violationObjects[2];
+    float violationDetail;
+    uint32_t adjacencyEntries;
+    uint32_t groupWidth;
+    uint32_t executed;        // 1 when the kernel ran this world's step
+};
+
+// ---------------------------------------------------------------------------------------
+// float2 helpers (componentwise, one rounding per operation).
+// ---------------------------------------------------------------------------------------
+BPD float2 f2(float x, float y) { return make_float2(x, y); }
+BPD float2 add(float2 a, float2 b) { return f2(a.x + b.x, a.y + b.y); }
+BPD float2 sub(float2 a, float2 b) { return f2(a.x - b.x, a.y - b.y); }
+BPD float2 neg(float2 a) { return f2(-a.x, -a.y); }
+BPD float2 mul(float2 a, float s) { return f2(a.x * s, a.y * s); }
+BPD float2 mul(float s, float2 a) { return f2(s * a.x, s * a.y); }
+BPD float2 dvs(float2 a, float s) { return f2(a.x / s, a.y / s); }
+BPD float2 subs(float2 a, float s) { return f2(a.x - s, a.y - s); }
+BPD float2 adds(float2 a, float s) { return f2(a.x + s, a.y + s); }
+BPD float dot2(float2 a, float2 b) { return a.x * b.x + a.y * b.y; }
+BPD float len2(float2 a) { return __builtin_sqrtf(a.x * a.x + a.y * a.y); }
+BPD float dist2(float2 a, float2 b) { return len2(sub(a, b)); }
+BPD float cross2(float2 a, float2 b) { return a.x * b.y - a.y * b.x; }
+BPD float2 min2(float2 a, float2 b) { return f2(fminf(a.x, b.x), fminf(a.y, b.y)); }
+BPD float2 max2(float2 a, float2 b) { return f2(fmaxf(a.x, b.x), fmaxf(a.y, b.y)); }
+BPD bool finite2(float2 a) { return isfinite(a.x) && isfinite(a.y); }
+BPD float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
+BPD float inff() { return __builtin_inff(); }
+
+constexpr float kFloatMin = 1.17549435e-38f;   // FLT_MIN
+constexpr uint32_t kFrameLimit = 1024;         // NativeMetalSubstep.frameLimit
+constexpr uint32_t kMaxBuckets = 8192;         // hashed-grid buckets held in LDS
+constexpr uint32_t kKindShift = 30;
+constexpr uint32_t kIndexMask = (1u << 30) - 1;
+
+// Bubble flag bits (bp_bubble.flags) and the region result.
+constexpr uint32_t kEnabled = 1u, kStatic = 2u, kWrap = 4u, kForceDriven = 8u;
+
+BPD bool filter_accepts(uint32_t ac, uint32_t am, uint32_t bc, uint32_t bm) {   // Geometry:130-132
+    return (ac & bm) != 0 && (bc & am) != 0;
+}
+
+// ---------------------------------------------------------------------------------------
+// Contact law, Force:50-102.
+// ---------------------------------------------------------------------------------------
+struct Law { float force, epsA, epsB, stiffness; };
+
+BPD float epsilon_from_force_ratio(float q) {
+    if (!(q > 0.0f)) return 0.0f;
+    return 2.0f * q / ((2.0f * q + 1.0f) + __builtin_sqrtf(4.0f * q + 1.0f));
+}
+BPD float rest_from_force_ratio(float q) {
+    return 2.0f / (1.0f + __builtin_sqrtf(fmaxf(4.0f * q, 0.0f) + 1.0f));
+}
+BPD Law wall_law(float K, float r, float distanceValue) {
+    Law law = {0.0f, 0.0f, 0.0f, 0.0f};
+    float d = fmaxf(distanceValue, r * 1e-6f);
+    if (!(K > 0.0f) || !(r > 0.0f) || !(d < r)) return law;
+    float rest = d / r;
+    law.epsA = (r - d) / r;
+    law.force = K * (r - d) * r / (d * d);
+    law.stiffness = K * (1.0f + law.epsA) / (r * rest * rest * rest);
+    return law;
+}
+BPD Law pair_law(float Ka, float ra, float Kb, float rb, float distanceValue) {
+    Law law = {0.0f, 0.0f, 0.0f, 0.0f};
+    float R = ra + rb;
+    float d = fmaxf(distanceValue, R * 1e-6f);
+    float overlap = R - d;
+    if (!(Ka > 0.0f) || !(Kb > 0.0f) || !(overlap > 0.0f)) return law;
+    float lowest = fminf(Ka, Kb) * overlap * R / (d * d);
+    float force = lowest;
+    if (Ka != Kb) {
+        for (int step = 0; step < 12; ++step) {
+            float qa = force / Ka, qb = force / Kb;
+            float ea = epsilon_from_force_ratio(qa), eb = epsilon_from_force_ratio(qb);
+            float sa = rest_from_force_ratio(qa), sb = rest_from_force_ratio(qb);
+            float residual = ra * ea + rb * eb - overlap;
+            float slope = ra * sa * sa * sa / ((1.0f + ea) * Ka) + rb * sb * sb * sb / ((1.0f + eb) * Kb);
+            if (!(slope > kFloatMin)) break;
+            force = fmaxf(force - residual / slope, lowest);
+        }
+    }
+    float qa = force / Ka, qb = force / Kb;
+    float ea = epsilon_from_force_ratio(qa), eb = epsilon_from_force_ratio(qb);
+    float sa = rest_from_force_ratio(qa), sb = rest_from_force_ratio(qb);
+    float compliance = ra * sa * sa * sa / ((1.0f + ea) * Ka) + rb * sb * sb * sb / ((1.0f + eb) * Kb);
+    law.force = force; law.epsA = ea; law.epsB = eb;
+    law.stiffness = compliance > kFloatMin ? 1.0f / compliance : 0.0f;
+    return law;
+}
+
+// ---------------------------------------------------------------------------------------
+// Segments, Geometry:436-440, 665-668; Force:112-140.
+// ---------------------------------------------------------------------------------------
+BPD float2 closest_point(float2 p, float2 a, float2 b) {
+    float2 edge = sub(b, a);
+    float square = dot2(edge, edge);
+    float t = square > kFloatMin ? clampf(dot2(sub(p, a), edge) / square, 0.0f, 1.0f) : 0.0f;
+    return add(a, mul(edge, t));
+}
+BPD bool past_segment_end(float2 center, float2 from, float2 to) {
+    float2 edge = sub(to, from), rel = sub(center, from);
+    return __builtin_fmaf(rel.x, edge.x, rel.y * edge.y) >= __builtin_fmaf(edge.x, edge.x, edge.y * edge.y);
+}
+
+struct Seg {
+    float2 a, b, prevStart;
+    float outward;
+    uint32_t flags;
+};
+constexpr uint32_t kSegPolygon = 2u, kSegConvexStart = 4u;
+
+BPD bool wall_region(const Seg &s, float2 center, float2 &normal, float &distanceValue) {
+    float2 start = s.a, end = s.b, edge = sub(end, start);
+    float edgeLength = len2(edge);
+    float2 base = edgeLength > 0.0f ? dvs(f2(-edge.y, edge.x), edgeLength) : f2(0.0f, 1.0f);
+    if (s.flags & kSegPolygon) {
+        float2 outward = mul(base, s.outward);
+        float along = dot2(sub(center, start), edge);
+        bool convexStart = (s.flags & kSegConvexStart) != 0;
+        if (convexStart && along <= 0.0f && past_segment_end(center, s.prevStart, start)) {
+            float2 delta = sub(center, start);
+            distanceValue = len2(delta);
+            normal = distanceValue > 0.0f ? dvs(delta, distanceValue) : outward;
+        } else if (along > 0.0f && !past_segment_end(center, start, end) && dot2(sub(center, start), outward) >= 0.0f) {
+            normal = outward;
+            distanceValue = dot2(sub(center, start), outward);
+        } else {
+            return false;
+        }
+    } else {
+        float2 delta = sub(center, closest_point(center, s.a, s.b));
+        distanceValue = len2(delta);
+        normal = distanceValue > 0.0f ? dvs(delta, distanceValue) : base;
+    }
+    return true;
+}
+
+// ---------------------------------------------------------------------------------------
+// Rigid bodies as bubble obstacles, Geometry:134-143, 467-490; Force:151-210.
+// ---------------------------------------------------------------------------------------
+struct BodyView {
+    float4 pose;      // x, y, angle, previous angle
+    float2 prev;
+    float4 motion;    // vx, vy, omega, damping
+    float4 shape;     // bounding radius, core radius, half length
+    uint4 ranges;     // vertex offset/count, triangle offset/count
+    uint4 kind;       // shape, flags, category, mask
+};
+constexpr uint32_t kBodyStatic = 1u, kBodyKinematic = 2u;
+
+BPD bool body_is_round(const BodyView &b) { return b.kind.x != 0u; }
+BPD bool body_immovable(const BodyView &b) { return (b.kind.y & (kBodyStatic | kBodyKinematic)) != 0u; }
+BPD float2 rotate(float2 p, float angle) {
+    float c = bpm::cosf_det(angle), s = bpm::sinf_det(angle);
+    return f2(p.x * c - p.y * s, p.x * s + p.y * c);
+}
+BPD float2 body_vertex(const BodyView &b, const float2 *verts, uint32_t i) {
+    return add(f2(b.pose.x, b.pose.y), rotate(verts[b.ranges.x + i], b.pose.z));
+}
+BPD bool body_inside(float2 p, const BodyView &b, const float2 *verts, const uint4 *tris) {
+    if (body_is_round(b)) {
+        float2 a = body_vertex(b, verts, 0);
+        float2 c = b.ranges.y > 1 ? body_vertex(b, verts, 1) : a;
+        return dist2(p, closest_point(p, a, c)) <= b.shape.y;
+    }
+    float2 local = rotate(sub(p, f2(b.pose.x, b.pose.y)), -b.pose.z);
+    for (uint32_t i = 0; i < b.ranges.w; ++i) {
+        uint4 t = tris[b.ranges.z + i];
+        float2 a = verts[b.ranges.x + t.x], bb = verts[b.ranges.x + t.y], c = verts[b.ranges.x + t.z];
+        if (cross2(sub(bb, a), sub(local, a)) >= 0.0f && cross2(sub(c, bb), sub(local, bb)) >= 0.0f &&
+            cross2(sub(a, c), sub(local, c)) >= 0.0f)
+            return true;
+    }
+    return false;
+}
+BPD BodyView body_at_fraction(BodyView b, float fraction) {
+    b.pose.x = b.prev.x + (b.pose.x - b.prev.x) * fraction;
+    b.pose.y = b.prev.y + (b.pose.y - b.prev.y) * fraction;
+    b.pose.z = b.pose.w + (b.pose.z - b.pose.w) * fraction;
+    return b;
+}
+struct Obstacle { float2 normal; float distanceValue; float speedBound; };
+
+BPD Obstacle body_obstacle(const BodyView &source, float fraction, const float2 *verts, const uint4 *tris, float2 center) {
+    BodyView body = body_at_fraction(source, fraction);
+    Obstacle result;
+    float2 point, normal;
+    float distanceValue;
+    if (body_is_round(body)) {
+        float2 a = body_vertex(body, verts, 0);
+        float2 c = body.ranges.y > 1 ? body_vertex(body, verts, 1) : a;
+        point = closest_point(center, a, c);
+        float2 delta = sub(center, point);
+        float away = len2(delta);
+        float2 axis = sub(c, a);
+        float2 perp = f2(-axis.y, axis.x);
+        float2 fallback = len2(axis) > 1e-6f ? dvs(perp, len2(perp)) : f2(0.0f, 1.0f);
+        normal = away > 1e-7f ? dvs(delta, away) : fallback;
+        distanceValue = away - body.shape.y;
+        point = add(point, mul(normal, body.shape.y));
+    } else {
+        float best = inff();
+        float2 nearest = f2(0.0f, 0.0f), outward = f2(0.0f, 1.0f);
+        for (uint32_t e = 0; e < body.ranges.y; ++e) {
+            float2 p = body_vertex(body, verts, e), q = body_vertex(body, verts, (e + 1) % body.ranges.y);
+            float2 candidate = closest_point(center, p, q);
+            float d = dist2(center, candidate);
+            if (d < best) {
+                best = d; nearest = candidate;
+                float2 edge = sub(q, p);
+                float size = len2(edge);
+                outward = size > kFloatMin ? dvs(f2(edge.y, -edge.x), size) : f2(0.0f, 1.0f);
+            }
+        }
+        bool inside = body_inside(center, body, verts, tris);
+        normal = inside ? outward : (best > 1e-7f ? dvs(sub(center, nearest), best) : outward);
+        distanceValue = inside ? -best : best;
+        point = nearest;
+    }
+    float2 arm = sub(point, f2(body.pose.x, body.pose.y));
+    result.normal = normal;
+    result.distanceValue = distanceValue;
+    result.speedBound = len2(f2(body.motion.x, body.motion.y)) + fabsf(body.motion.z) * len2(arm);
+    return result;
+}
+
+// Exact half impulse with drag, Force:754-758.
+BPD float half_impulse_factor(float gamma, float halfStep) {
+    float x = gamma * halfStep;
+    if (x < 0.05f)
+        return halfStep * (1.0f - x * 0.5f + x * x * (1.0f / 6.0f) - x * x * x * (1.0f / 24.0f) + x * x * x * x * (1.0f / 120.0f));
+    return (1.0f - bpm::expf_det(-x)) / gamma;
+}
+
+}  // namespace bpd
+
+#endif  // BP_DEVICE_H

@@ -1,0 +1,1000 @@
+// bp_world_kernel.hip -- one workgroup advances one world by one step (gfx950).
+//
+// MI355X design of the reference's single-dispatch world kernel
+// (NativeForceKernels.metal-source:1236-1307, nativeSubstepLoopSingleGroup) and, for every world
+// size, of its multi-dispatch equivalent (NativeMetalSolver.swift:801-833, NativeMetalSubstepLoop
+// .swift:95-120). A launch holds one workgroup per world, so one launch advances a whole batch
+// of independent worlds; a plain world is a batch of one.
+//
+// Per step and world:
+//   1. begin step and kinematic-body prediction (Force:285-293, PostSolve:66-82);
+//   2. grid build: a0, margin and x0, cell width, a counting-sorted hashed grid in LDS, then a
+//      CSR adjacency (rows sorted by (kind, neighbour)) in two passes without atomics on rows;
+//   3. substep loop with three passes per substep instead of the reference's four:
+//        D: kick and drift (phase 0),
+//        F: contact forces + kick (phase 1) + omega^2 + margin bound, fused because the
+//           reduction of the next substep reads exactly the positions the forces used,
+//        H: d/4 bound with h_max, then h (Force:625-736) evaluated redundantly by every lane;
+//   4. gated end of step: region, finite check, contact assertion, status and publication.
+// Reductions keep the reference's tie rule (equal values -> lower bubble index), and every
+// per-bubble sum runs in adjacency-row order, so the result is independent of the block size.
+#include "bp_device.h"
+#include "../../include/bubblephysics.h"
+
+namespace bpd {
+
+struct Ctl {           // NativeSubstepControl in registers, uniform across the workgroup
+    float t0, t1, h, split;
+    uint32_t c0, c1, rebuild, limit;
+    uint64_t limObj;
+    uint32_t limTerm;
+};
+
+template <int BLOCK> struct Shared {
+    uint32_t buckets[kMaxBuckets];
+    float redV[BLOCK / 64];
+    uint32_t redO[BLOCK / 64];
+    float redV2[BLOCK / 64];
+    uint32_t redO2[BLOCK / 64];
+    uint32_t scan[BLOCK / 64];
+    uint32_t counters[8];
+};
+
+// ---------------------------------------------------------------------------------------
+// Workgroup primitives.
+// ---------------------------------------------------------------------------------------
+BPD void wave_max_owner(float &v, uint32_t &o) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        float ov = __shfl_xor(v, off, 64);
+        uint32_t oo = __shfl_xor(o, off, 64);
+        if (ov > v || (ov == v && oo < o)) { v = ov; o = oo; }
+    }
+}
+BPD void wave_min_owner(float &v, uint32_t &o) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        float ov = __shfl_xor(v, off, 64);
+        uint32_t oo = __shfl_xor(o, off, 64);
+        if (ov < v || (ov == v && oo < o)) { v = ov; o = oo; }
+    }
+}
+
+// Max of (a, owner) and min of (b, owner) over the workgroup; results broadcast to every lane.
+template <int BLOCK>
+BPD void block_maxmin(Shared<BLOCK> &sh, float &a, uint32_t &ao, float &b, uint32_t &bo) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    wave_max_owner(a, ao);
+    wave_min_owner(b, bo);
+    __syncthreads();
+    if (lane == 0) { sh.redV[wave] = a; sh.redO[wave] = ao; sh.redV2[wave] = b; sh.redO2[wave] = bo; }
+    __syncthreads();
+    a = sh.redV[0]; ao = sh.redO[0]; b = sh.redV2[0]; bo = sh.redO2[0];
+#pragma unroll
+    for (int w = 1; w < BLOCK / 64; ++w) {
+        float v = sh.redV[w]; uint32_t o = sh.redO[w];
+        if (v > a || (v == a && o < ao)) { a = v; ao = o; }
+        v = sh.redV2[w]; o = sh.redO2[w];
+        if (v < b || (v == b && o < bo)) { b = v; bo = o; }
+    }
+}
+template <int BLOCK>
+BPD void block_min(Shared<BLOCK> &sh, float &b, uint32_t &bo) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    wave_min_owner(b, bo);
+    __syncthreads();
+    if (lane == 0) { sh.redV2[wave] = b; sh.redO2[wave] = bo; }
+    __syncthreads();
+    b = sh.redV2[0]; bo = sh.redO2[0];
+#pragma unroll
+    for (int w = 1; w < BLOCK / 64; ++w) {
+        float v = sh.redV2[w]; uint32_t o = sh.redO2[w];
+        if (v < b || (v == b && o < bo)) { b = v; bo = o; }
+    }
+}
+template <int BLOCK>
+BPD float block_maxf(Shared<BLOCK> &sh, float v) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+    __syncthreads();
+    if (lane == 0) sh.redV[wave] = v;
+    __syncthreads();
+    float r = sh.redV[0];
+#pragma unroll
+    for (int w = 1; w < BLOCK / 64; ++w) r = fmaxf(r, sh.redV[w]);
+    return r;
+}
+// Exclusive prefix sum of one value per lane; returns the lane's offset and the tile total.
+template <int BLOCK>
+BPD uint32_t block_exclusive_scan(Shared<BLOCK> &sh, uint32_t value, uint32_t &total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t inclusive = value;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t other = __shfl_up(inclusive, off, 64);
+        if (lane >= off) inclusive += other;
+    }
+    __syncthreads();
+    if (lane == 63) sh.scan[wave] = inclusive;
+    __syncthreads();
+    uint32_t before = 0;
+    total = 0;
+#pragma unroll
+    for (int w = 0; w < BLOCK / 64; ++w) {
+        uint32_t s = sh.scan[w];
+        if (w < wave) before += s;
+        total += s;
+    }
+    return before + inclusive - value;
+}
+
+// ---------------------------------------------------------------------------------------
+// The world view of one workgroup.
+// ---------------------------------------------------------------------------------------
+struct World {
+    Arena A;
+    const WorldDesc *d;
+    uint32_t n, ns, nb, w;       // w: working side
+    uint32_t o;                  // bubble offset
+    float2 *pos, *start, *vel;
+    float *rot, *omega;
+    uint32_t *region;
+    const int64_t *id;
+    const float *invmRaw, *radius, *Kc, *gamma, *mass;
+    const uint32_t *flags, *cat, *mask;
+    const int32_t *motor, *band;
+    float2 *acc, *a0, *ref;
+    float *margin;
+    uint32_t *rowStart, *rowLen, *grid, *adj;
+    const float4 *motors, *bands;
+    const float2 *verts;
+    const uint4 *tris;
+    float cellWidth;
+    float2 globalForce;
+    float sepTol;
+
+    BPD bool is_static(uint32_t i) const {
+        return !(invmRaw[i] > 0.0f) || ((flags[i] & kEnabled) && (flags[i] & kStatic));
+    }
+    BPD float inv_mass(uint32_t i) const { return is_static(i) ? 0.0f : invmRaw[i]; }
+    BPD float2 eff_vel(uint32_t i) const { return is_static(i) ? f2(0.0f, 0.0f) : vel[i]; }
+
+    BPD Seg segment(uint32_t j) const {
+        Seg s;
+        float4 ab = A.segAB[d->segmentOffset + j];
+        float4 rg = A.segRegion[d->segmentOffset + j];
+        s.a = f2(ab.x, ab.y); s.b = f2(ab.z, ab.w);
+        s.prevStart = f2(rg.x, rg.y);
+        s.outward = rg.z;
+        s.flags = A.segFlags[d->segmentOffset + j];
+        return s;
+    }
+    BPD BodyView body(uint32_t k) const {
+        BodyView b;
+        uint32_t g = d->bodyOffset + k;
+        b.pose = A.bodyPose[w][g];
+        b.prev = A.bodyPrev[w][g];
+        b.motion = A.bodyMotion[g];
+        b.shape = A.bodyShape[g];
+        b.ranges = A.bodyRanges[g];
+        b.kind = A.bodyKind[g];
+        return b;
+    }
+
+    // Non-contact acceleration a0 at `p`, Force:450-470 (global force, band, motor).
+    BPD float2 non_contact(uint32_t i, float2 p) const {
+        float invm = inv_mass(i);
+        if (!(flags[i] & kForceDriven) || !(invm > 0.0f)) return f2(0.0f, 0.0f);
+        float2 force = globalForce;
+        int32_t bi = band[i];
+        if (bi >= 0) {
+            float4 bd = bands[bi];
+            float2 bf = mul(sub(f2(bd.x, bd.y), p), bd.z);
+            float lim = bd.w, magnitude = len2(bf);
+            if (lim > 0.0f && magnitude > lim) bf = mul(bf, lim / magnitude);
+            force = add(force, bf);
+        }
+        int32_t mi = motor[i];
+        if (mi >= 0) {
+            float4 md = motors[mi];
+            force = add(force, f2(md.x, md.y));
+        }
+        return mul(force, invm);
+    }
+
+    BPD int64_t cell_x(float2 p) const { return (int64_t)floorf(p.x / cellWidth); }
+    BPD int64_t cell_y(float2 p) const { return (int64_t)floorf(p.y / cellWidth); }
+    static BPD uint32_t bucket_of(int64_t cx, int64_t cy, uint32_t buckets) {    // Force:1000-1002
+        return ((uint32_t)cx * 73856093u ^ (uint32_t)cy * 19349663u) % buckets;
+    }
+
+    // Pair predicate of Geometry:262-273, evaluated with the lower index as the row owner.
+    BPD bool pair_candidate(uint32_t lo, uint32_t hi) const {
+        if (!filter_accepts(cat[lo], mask[lo], cat[hi], mask[hi])) return false;
+        float reach = radius[lo] + radius[hi] + margin[lo] + margin[hi] + sepTol;
+        float2 a = pos[lo], b = pos[hi];
+        return fabsf(a.x - b.x) <= reach && fabsf(a.y - b.y) <= reach;
+    }
+    // Segment predicate, Geometry:279-290.
+    BPD bool segment_candidate(uint32_t i, uint32_t j) const {
+        uint32_t g = d->segmentOffset + j;
+        if (!filter_accepts(cat[i], mask[i], A.segCat[g], A.segMask[g])) return false;
+        float4 ab = A.segAB[g];
+        float reach = radius[i] + margin[i] + sepTol;
+        float2 lower = subs(min2(f2(ab.x, ab.y), f2(ab.z, ab.w)), reach);
+        float2 upper = adds(max2(f2(ab.x, ab.y), f2(ab.z, ab.w)), reach);
+        float2 p = pos[i];
+        return p.x >= lower.x && p.y >= lower.y && p.x <= upper.x && p.y <= upper.y;
+    }
+    // Body predicate, Geometry:361-379.
+    BPD bool body_candidate(uint32_t i, uint32_t k) const {
+        uint32_t g = d->bodyOffset + k;
+        uint4 kind = A.bodyKind[g];
+        bool isStatic = (kind.y & kBodyStatic) != 0;
+        bool obstacle = isStatic ? (kind.x != 0u) : ((kind.y & kBodyKinematic) != 0u);
+        if (!obstacle) return false;
+        if (!filter_accepts(cat[i], mask[i], kind.z, kind.w)) return false;
+        float4 pose = A.bodyPose[w][g];
+        float2 prev = A.bodyPrev[w][g];
+        float br = A.bodyShape[g].x;
+        float2 bodyLow = subs(min2(f2(pose.x, pose.y), prev), br);
+        float2 bodyHigh = adds(max2(f2(pose.x, pose.y), prev), br);
+        float reach = radius[i] + fmaxf(margin[i], 0.0f);
+        float2 p = pos[i];
+        float2 lo = subs(p, reach), hi = adds(p, reach);
+        return !(hi.x < bodyLow.x || hi.y < bodyLow.y || bodyHigh.x < lo.x || bodyHigh.y < lo.y);
+    }
+};
+
+// Enumerates the adjacency row of bubble i: pairs (both directions) from the 3x3 cells of
+// the hashed grid, then segments, then bodies. `emit(entry)` receives packed entries;
+// `forward` counts the reference's candidate-row entries (pairs j > i and segments).
+template <class Emit>
+BPD void enumerate_row(const World &W, const uint32_t *ends, uint32_t buckets, uint32_t i, Emit emit, uint32_t &forward) {
+    forward = 0;
+    float2 p = W.pos[i];
+    if (!finite2(p)) return;                                  // not in the grid: no pair, no segment row
+    bool ownRow = finite2(W.start[i]);                        // Geometry:305-307
+    int64_t cx = W.cell_x(p), cy = W.cell_y(p);
+    for (int dx = -1; dx <= 1; ++dx)
+        for (int dy = -1; dy <= 1; ++dy) {
+            int64_t tx = cx + dx, ty = cy + dy;
+            uint32_t b = World::bucket_of(tx, ty, buckets);
+            uint32_t lo = b == 0 ? 0u : ends[b - 1], hi = ends[b];
+            for (uint32_t k = lo; k < hi; ++k) {
+                uint32_t j = W.grid[k];
+                if (j == i) continue;
+                float2 q = W.pos[j];
+                if (W.cell_x(q) != tx || W.cell_y(q) != ty) continue;
+                if (j > i) {
+                    if (!ownRow) continue;
+                    if (!W.pair_candidate(i, j)) continue;
+                    ++forward;
+                } else {
+                    if (!finite2(W.start[j])) continue;
+                    if (!W.pair_candidate(j, i)) continue;
+                }
+                emit(j);
+            }
+        }
+    if (ownRow)
+        for (uint32_t j = 0; j < W.ns; ++j)
+            if (W.segment_candidate(i, j)) { ++forward; emit((1u << kKindShift) | j); }
+    for (uint32_t k = 0; k < W.nb; ++k)
+        if (W.body_candidate(i, k)) emit((2u << kKindShift) | k);
+}
+
+// Bodies are enumerated even for a bubble outside the grid (NaN compares like the reference).
+template <class Emit>
+BPD void enumerate_bodies_only(const World &W, uint32_t i, Emit emit) {
+    for (uint32_t k = 0; k < W.nb; ++k)
+        if (W.body_candidate(i, k)) emit((2u << kKindShift) | k);
+}
+
+struct RowStats { uint32_t dropped, maxDemand, pairs, overflowRows; };
+
+// Grid build of one workgroup: Force:1040-1134 with a CSR adjacency. Returns false when the
+// adjacency does not fit the world's capacity.
+template <int BLOCK>
+BPD bool build_grid(World &W, Shared<BLOCK> &sh, float remaining, uint32_t limit, RowStats &stats, uint32_t &entries) {
+    const uint32_t n = W.n;
+    const WorldDesc &d = *W.d;
+    // a0, margin and x0 (Force:1046-1049, 492-499).
+    for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
+        float2 p = W.pos[i];
+        float2 a = W.non_contact(i, p);
+        W.a0[i] = a;
+        float t = fmaxf(remaining, 0.0f);
+        float travel = len2(W.eff_vel(i)) * t + 0.5f * len2(a) * t * t;
+        W.margin[i] = d.minimumMarginOnly ? 0.25f * W.radius[i] : fmaxf(1.05f * travel, 0.25f * W.radius[i]);
+        W.ref[i] = p;
+    }
+    __syncthreads();
+    // Cell width (Force:1052-1064).
+    float width = W.sepTol;
+    for (uint32_t i = threadIdx.x; i < n; i += BLOCK) width = fmaxf(width, 2.0f * (W.radius[i] + W.margin[i]) + W.sepTol);
+    width = block_maxf<BLOCK>(sh, width);
+    W.cellWidth = fmaxf(width, kFloatMin);
+    // Hashed grid: count, exclusive prefix, scatter (Force:1066-1105).
+    uint32_t buckets = d.buckets;
+    for (uint32_t b = threadIdx.x; b < buckets; b += BLOCK) sh.buckets[b] = 0;
+    if (threadIdx.x < 8) sh.counters[threadIdx.x] = 0;
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
+        float2 p = W.pos[i];
+        if (!finite2(p)) continue;
+        atomicAdd(&sh.buckets[World::bucket_of(W.cell_x(p), W.cell_y(p), buckets)], 1u);
+    }
+    __syncthreads();
+    {
+        uint32_t perLane = (buckets + BLOCK - 1) / BLOCK;
+        uint32_t first = min(threadIdx.x * perLane, buckets), last = min(first + perLane, buckets);
+        uint32_t sum = 0;
+        for (uint32_t b = first; b < last; ++b) sum += sh.buckets[b];
+        uint32_t total;
+        uint32_t running = block_exclusive_scan<BLOCK>(sh, sum, total);
+        __syncthreads();
+        for (uint32_t b = first; b < last; ++b) { uint32_t own = sh.buckets[b]; sh.buckets[b] = running; running += own; }
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
+        float2 p = W.pos[i];
+        if (!finite2(p)) continue;
+        uint32_t slot = atomicAdd(&sh.buckets[World::bucket_of(W.cell_x(p), W.cell_y(p), buckets)], 1u);
+        W.grid[slot] = i;
+    }
+    __syncthreads();
+    // Pass 1: row lengths and the reference's overflow counters.
+    const uint32_t *ends = sh.buckets;
+    uint32_t droppedLocal = 0, maxLocal = 0, pairsLocal = 0, overflowLocal = 0;
+    for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
+        uint32_t count = 0, forward = 0;
+        if (finite2(W.pos[i])) enumerate_row(W, ends, buckets, i, [&](uint32_t) { ++count; }, forward);
+        else enumerate_bodies_only(W, i, [&](uint32_t) { ++count; });
+        W.rowLen[i] = count;
+        if (forward > limit) { droppedLocal += forward - limit; overflowLocal = 1; }
+        maxLocal = max(maxLocal, max(forward, count));
+        if (count > limit) { droppedLocal += count - limit; overflowLocal = 1; }
+        pairsLocal += min(forward, limit);
+    }
+    atomicAdd(&sh.counters[0], droppedLocal);
+    atomicMax(&sh.counters[1], maxLocal);
+    atomicAdd(&sh.counters[2], pairsLocal);
+    atomicOr(&sh.counters[3], overflowLocal);
+    __syncthreads();
+    // Row starts: tiled exclusive scan of the row lengths in bubble order.
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < n; base += BLOCK) {
+        uint32_t i = base + threadIdx.x;
+        uint32_t len = i < n ? W.rowLen[i] : 0u;
+        uint32_t total;
+        uint32_t offset = block_exclusive_scan<BLOCK>(sh, len, total);
+        if (i < n) W.rowStart[i] = carry + offset;
+        carry += total;
+    }
+    entries = carry;
+    stats.dropped = sh.counters[0];
+    stats.maxDemand = sh.counters[1];
+    stats.pairs = sh.counters[2];
+    stats.overflowRows = sh.counters[3];
+    if (carry > d.adjCapacity) return false;
+    __syncthreads();
+    // Pass 2: fill rows, then order each row by (kind, neighbour) (Contact:222-238).
+    for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
+        uint32_t *row = W.adj + W.rowStart[i];
+        uint32_t count = 0, forward = 0;
+        auto put = [&](uint32_t e) { row[count++] = e; };
+        if (finite2(W.pos[i])) enumerate_row(W, ends, buckets, i, put, forward);
+        else enumerate_bodies_only(W, i, put);
+        // Only the pair block can be out of order (hash-bucket order); segments and bodies follow in order.
+        uint32_t pairs = 0;
+        while (pairs < count && (row[pairs] >> kKindShift) == 0u) ++pairs;
+        for (uint32_t a = 1; a < pairs; ++a) {
+            uint32_t v = row[a];
+            uint32_t loc = a;
+            while (loc > 0 && v < row[loc - 1]) { row[loc] = row[loc - 1]; --loc; }
+            row[loc] = v;
+        }
+    }
+    __syncthreads();
+    return true;
+}
+
+// ---------------------------------------------------------------------------------------
+// Substep passes.
+// ---------------------------------------------------------------------------------------
+
+// F: contact forces, phase 1 (or phase 2 after a grid build), omega^2 and the margin bound.
+template <int BLOCK>
+BPD void pass_forces(World &W, Shared<BLOCK> &sh, const Ctl &c, uint32_t phase, float &w2Max, uint32_t &w2Owner,
+                     float &mbMin, uint32_t &mbOwner) {
+    const uint32_t n = W.n;
+    float fraction = c.t0 > 0.0f ? clampf(c.t1 / c.t0, 0.0f, 1.0f) : 1.0f;
+    float h = c.h, halfStep = 0.5f * h;
+    float bestW = -1.0f, bestM = inff();
+    uint32_t ownW = 0, ownM = 0;
+    for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
+        float2 p = W.pos[i];
+        float invm = W.inv_mass(i);
+        float r = W.radius[i], K = W.Kc[i];
+        float2 total = f2(0.0f, 0.0f);
+        float w2 = 0.0f;
+        if (invm > 0.0f && W.band[i] >= 0) w2 = w2 + W.bands[W.band[i]].z * invm;
+        const uint32_t *row = W.adj + W.rowStart[i];
+        uint32_t len = W.rowLen[i];
+        for (uint32_t e = 0; e < len; ++e) {
+            uint32_t entry = row[e], kind = entry >> kKindShift, j = entry & kIndexMask;
+            if (kind == 0u) {
+                uint32_t lo = min(i, j), hi = max(i, j);
+                float2 delta = sub(W.pos[hi], W.pos[lo]);
+                float dd = len2(delta);
+                float2 normal = dd > 0.0f ? dvs(delta, dd) : f2(1.0f, 0.0f);
+                Law law = pair_law(W.Kc[lo], W.radius[lo], W.Kc[hi], W.radius[hi], dd);
+                total = add(total, mul(i == lo ? -law.force : law.force, normal));
+                if (invm > 0.0f) w2 = w2 + law.stiffness * (invm + __builtin_sqrtf(invm * W.inv_mass(j)));
+            } else if (kind == 1u) {
+                float2 normal; float dd;
+                if (!wall_region(W.segment(j), p, normal, dd) || !(dd < r)) continue;
+                Law law = wall_law(K, r, dd);
+                total = add(total, mul(law.force, normal));
+                if (invm > 0.0f) w2 = w2 + law.stiffness * invm;
+            } else {
+                Obstacle ob = body_obstacle(W.body(j), fraction, W.verts, W.tris, p);
+                Law law = wall_law(K, r, ob.distanceValue);
+                total = add(total, mul(law.force, ob.normal));
+                if (invm > 0.0f) w2 = w2 + law.stiffness * invm;
+            }
+        }
+        float2 v = W.vel[i];
+        float2 a;
+        if (phase == 2) {
+            a = add(mul(total, invm), W.a0[i]);
+            W.acc[i] = a;
+        } else {
+            a = W.acc[i];
+            if (invm > 0.0f) {
+                float gamma = (W.flags[i] & kForceDriven) ? fmaxf(W.gamma[i], 0.0f) : 0.0f;
+                float decay = bpm::expf_det(-gamma * halfStep), factor = half_impulse_factor(gamma, halfStep);
+                a = add(mul(total, invm), W.non_contact(i, p));
+                v = add(mul(v, decay), mul(a, factor));
+                W.acc[i] = a;
+                W.vel[i] = v;
+            }
+        }
+        // omega^2 (Force:510-541) and margin bound (Force:604-614) for the next reduction.
+        if (w2 > bestW) { bestW = w2; ownW = i; }
+        float mb;
+        if (!(invm > 0.0f)) mb = inff();
+        else {
+            float left = W.margin[i] - len2(sub(p, W.ref[i]));
+            if (isnan(left)) mb = inff();
+            else if (left <= 0.0f) mb = 0.0f;
+            else {
+                float speed = len2(v), acl = len2(a);
+                float den = speed + __builtin_sqrtf(speed * speed + 2.0f * acl * left);
+                mb = den > 0.0f ? 2.0f * left / den : inff();
+            }
+        }
+        if (mb < bestM) { bestM = mb; ownM = i; }
+    }
+    block_maxmin<BLOCK>(sh, bestW, ownW, bestM, ownM);
+    w2Max = bestW; w2Owner = ownW; mbMin = bestM; mbOwner = ownM;
+}
+
+// omega^2 and margin bound alone (after a diagnostic state injection moved a centre).
+template <int BLOCK>
+BPD void pass_bounds(World &W, Shared<BLOCK> &sh, const Ctl &c, float &w2Max, uint32_t &w2Owner, float &mbMin, uint32_t &mbOwner) {
+    const uint32_t n = W.n;
+    float fraction = c.t0 > 0.0f ? clampf(c.t1 / c.t0, 0.0f, 1.0f) : 1.0f;
+    float bestW = -1.0f, bestM = inff();
+    uint32_t ownW = 0, ownM = 0;
+    for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
+        float2 p = W.pos[i];
+        float invm = W.inv_mass(i);
+        float r = W.radius[i], K = W.Kc[i];
+        float w2 = 0.0f;
+        if (invm > 0.0f) {
+            if (W.band[i] >= 0) w2 = w2 + W.bands[W.band[i]].z * invm;
+            const uint32_t *row = W.adj + W.rowStart[i];
+            uint32_t len = W.rowLen[i];
+            for (uint32_t e = 0; e < len; ++e) {
+                uint32_t entry = row[e], kind = entry >> kKindShift, j = entry & kIndexMask;
+                if (kind == 0u) {
+                    uint32_t lo = min(i, j), hi = max(i, j);
+                    float dd = len2(sub(W.pos[hi], W.pos[lo]));
+                    Law law = pair_law(W.Kc[lo], W.radius[lo], W.Kc[hi], W.radius[hi], dd);
+                    w2 = w2 + law.stiffness * (invm + __builtin_sqrtf(invm * W.inv_mass(j)));
+                } else if (kind == 1u) {
+                    float2 normal; float dd;
+                    if (!wall_region(W.segment(j), p, normal, dd) || !(dd < r)) continue;
+                    w2 = w2 + wall_law(K, r, dd).stiffness * invm;
+                } else {
+                    Obstacle ob = body_obstacle(W.body(j), fraction, W.verts, W.tris, p);
+                    w2 = w2 + wall_law(K, r, ob.distanceValue).stiffness * invm;
+                }
+            }
+        }
+        if (w2 > bestW) { bestW = w2; ownW = i; }
+        float mb;
+        if (!(invm > 0.0f)) mb = inff();
+        else {
+            float left = W.margin[i] - len2(sub(p, W.ref[i]));
+            if (isnan(left)) mb = inff();
+            else if (left <= 0.0f) mb = 0.0f;
+            else {
+                float speed = len2(W.vel[i]), acl = len2(W.acc[i]);
+                float den = speed + __builtin_sqrtf(speed * speed + 2.0f * acl * left);
+                mb = den > 0.0f ? 2.0f * left / den : inff();
+            }
+        }
+        if (mb < bestM) { bestM = mb; ownM = i; }
+    }
+    block_maxmin<BLOCK>(sh, bestW, ownW, bestM, ownM);
+    w2Max = bestW; w2Owner = ownW; mbMin = bestM; mbOwner = ownM;
+}
+
+// H: the d/4 bound (Force:550-598).
+template <int BLOCK>
+BPD void pass_distance(World &W, Shared<BLOCK> &sh, const Ctl &c, float hMax, float &dbMin, uint32_t &dbOwner) {
+    const uint32_t n = W.n;
+    float fraction = c.t0 > 0.0f ? clampf(c.t1 / c.t0, 0.0f, 1.0f) : 1.0f;
+    float bestD = inff();
+    uint32_t ownD = 0;
+    for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
+        if (!(W.inv_mass(i) > 0.0f)) continue;                // inf never replaces the initial best
+        float2 p = W.pos[i], v = W.vel[i];
+        float r = W.radius[i];
+        float accI = len2(W.acc[i]);
+        float best = inff();
+        const uint32_t *row = W.adj + W.rowStart[i];
+        uint32_t len = W.rowLen[i];
+        for (uint32_t e = 0; e < len; ++e) {
+            uint32_t entry = row[e], kind = entry >> kKindShift, j = entry & kIndexMask;
+            float dd, approach, accSum, reach, bodySpeed = 0.0f;
+            if (kind == 0u) {
+                float2 delta = sub(W.pos[j], p);
+                dd = len2(delta);
+                float2 normal = dd > 0.0f ? dvs(delta, dd) : f2(1.0f, 0.0f);
+                approach = dot2(sub(v, W.eff_vel(j)), normal);
+                accSum = accI + len2(W.acc[j]);
+                reach = r + W.radius[j];
+            } else if (kind == 1u) {
+                float2 normal;
+                if (!wall_region(W.segment(j), p, normal, dd)) continue;
+                approach = -dot2(v, normal);
+                accSum = accI;
+                reach = r;
+            } else {
+                Obstacle ob = body_obstacle(W.body(j), fraction, W.verts, W.tris, p);
+                dd = ob.distanceValue;
+                approach = -dot2(v, ob.normal);
+                bodySpeed = ob.speedBound;
+                accSum = accI;
+                reach = r;
+            }
+            bool active = dd < reach;
+            float dc = fmaxf(dd, reach * 1e-6f);
+            float speed;
+            if (active) speed = fabsf(approach) + bodySpeed;
+            else {
+                if (approach + bodySpeed + accSum * hMax * 0.5f <= 0.0f) continue;
+                speed = approach + bodySpeed;
+            }
+            float s = __builtin_sqrtf(speed * speed + accSum * dc * 0.5f);
+            float den = s + speed;
+            if (!(den > 0.0f)) continue;
+            best = fminf(best, dc / (2.0f * den));
+        }
+        if (best < bestD) { bestD = best; ownD = i; }
+    }
+    block_min<BLOCK>(sh, bestD, ownD);
+    dbMin = bestD; dbOwner = ownD;
+}
+
+// D: kick and drift, phase 0 (Force:775-791).
+template <int BLOCK>
+BPD void pass_drift(World &W, const Ctl &c) {
+    const uint32_t n = W.n;
+    float h = c.h, halfStep = 0.5f * h;
+    for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
+        float invm = W.inv_mass(i);
+        if (!(invm > 0.0f)) continue;
+        bool driven = (W.flags[i] & kForceDriven) != 0;
+        float gamma = driven ? fmaxf(W.gamma[i], 0.0f) : 0.0f;
+        float decay = bpm::expf_det(-gamma * halfStep), factor = half_impulse_factor(gamma, halfStep);
+        float2 v = add(mul(W.vel[i], decay), mul(W.acc[i], factor));
+        float om = W.omega[i];
+        float angular = om * bpm::expf_det(-gamma * h);
+        int32_t mi = W.motor[i];
+        if (driven && mi >= 0) {
+            float4 md = W.motors[mi];
+            if (md.w > 0.0f) angular = angular + md.z / md.w * half_impulse_factor(gamma, h);
+        }
+        W.pos[i] = add(W.pos[i], mul(v, h));
+        W.vel[i] = v;
+        W.rot[i] = W.rot[i] + om * h;
+        W.omega[i] = angular;
+    }
+}
+
+// Contact assertion (Force:865-918): the lowest-index bubble with a violation is recorded.
+// Pairs are searched in the 3x3 cells of a grid of the current centres (coincident centres
+// share a cell), bodies and one-sided edges by direct loops.
+struct Violation { uint32_t kind; uint64_t other; int64_t segment; float detail; };
+
+// Returns true (uniformly) when some bubble violates; the lowest violating bubble writes the
+// record (NativeContactAssertion) into the world status.
+template <int BLOCK>
+BPD bool assert_contacts(World &W, Shared<BLOCK> &sh, const Ctl &c, WorldStatus *st) {
+    const uint32_t n = W.n;
+    const uint32_t buckets = W.d->buckets;
+    if (!(W.cellWidth > 0.0f)) W.cellWidth = 1.0f;
+    __syncthreads();
+    for (uint32_t b = threadIdx.x; b < buckets; b += BLOCK) sh.buckets[b] = 0;
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
+        float2 p = W.pos[i];
+        if (!finite2(p)) continue;
+        atomicAdd(&sh.buckets[World::bucket_of(W.cell_x(p), W.cell_y(p), buckets)], 1u);
+    }
+    __syncthreads();
+    {
+        uint32_t perLane = (buckets + BLOCK - 1) / BLOCK;
+        uint32_t first = min(threadIdx.x * perLane, buckets), last = min(first + perLane, buckets);
+        uint32_t sum = 0;
+        for (uint32_t b = first; b < last; ++b) sum += sh.buckets[b];
+        uint32_t total;
+        uint32_t running = block_exclusive_scan<BLOCK>(sh, sum, total);
+        __syncthreads();
+        for (uint32_t b = first; b < last; ++b) { uint32_t own = sh.buckets[b]; sh.buckets[b] = running; running += own; }
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
+        float2 p = W.pos[i];
+        if (!finite2(p)) continue;
+        uint32_t slot = atomicAdd(&sh.buckets[World::bucket_of(W.cell_x(p), W.cell_y(p), buckets)], 1u);
+        W.grid[slot] = i;
+    }
+    __syncthreads();
+    float fraction = c.t0 > 0.0f ? clampf(c.t1 / c.t0, 0.0f, 1.0f) : 1.0f;
+    uint32_t firstIndex = 0xffffffffu;
+    Violation mine = {0u, 0u, 0, 0.0f};
+    for (uint32_t i = threadIdx.x; i < n && firstIndex == 0xffffffffu; i += BLOCK) {
+        float2 center = W.pos[i], start = W.start[i];
+        if (!finite2(center) || W.region[i] == BP_REGION_REMOVED) continue;
+        bool movable = W.inv_mass(i) > 0.0f;
+        uint32_t bestJ = 0xffffffffu;
+        float bestD = 0.0f;
+        int64_t cx = W.cell_x(center), cy = W.cell_y(center);
+        for (int dx = -1; dx <= 1; ++dx)
+            for (int dy = -1; dy <= 1; ++dy) {
+                int64_t tx = cx + dx, ty = cy + dy;
+                uint32_t b = World::bucket_of(tx, ty, buckets);
+                uint32_t lo = b == 0 ? 0u : sh.buckets[b - 1], hi = sh.buckets[b];
+                for (uint32_t k = lo; k < hi; ++k) {
+                    uint32_t j = W.grid[k];
+                    if (j <= i || j >= bestJ) continue;
+                    float2 q = W.pos[j];
+                    if (W.cell_x(q) != tx || W.cell_y(q) != ty) continue;
+                    if (W.region[j] == BP_REGION_REMOVED || !filter_accepts(W.cat[i], W.mask[i], W.cat[j], W.mask[j])) continue;
+                    if (!movable && !(W.inv_mass(j) > 0.0f)) continue;
+                    float dd = len2(sub(q, center));
+                    if (dd <= 0.0f) { bestJ = j; bestD = dd; }
+                }
+            }
+        if (bestJ != 0xffffffffu) {
+            firstIndex = i; mine.kind = BP_VIOLATION_PAIR; mine.other = (uint64_t)W.id[bestJ]; mine.detail = bestD;
+            break;
+        }
+        if (!movable) continue;
+        bool found = false;
+        for (uint32_t k = 0; k < W.nb && !found; ++k) {
+            BodyView body = W.body(k);
+            if (!body_immovable(body) || !filter_accepts(W.cat[i], W.mask[i], body.kind.z, body.kind.w)) continue;
+            if (body_inside(center, body_at_fraction(body, fraction), W.verts, W.tris)) {
+                found = true;
+                mine.kind = BP_VIOLATION_INSIDE_BODY; mine.other = W.A.bodyId[W.d->bodyOffset + k]; mine.detail = 0.0f;
+            }
+        }
+        for (uint32_t j = 0; j < W.ns && !found; ++j) {
+            Seg s = W.segment(j);
+            uint32_t g = W.d->segmentOffset + j;
+            if (!(s.flags & kSegPolygon) || !filter_accepts(W.cat[i], W.mask[i], W.A.segCat[g], W.A.segMask[g])) continue;
+            float2 edge = sub(s.b, s.a);
+            float lengthSquared = dot2(edge, edge);
+            if (!(lengthSquared > 0.0f)) continue;
+            float2 outward = mul(dvs(f2(-edge.y, edge.x), __builtin_sqrtf(lengthSquared)), s.outward);
+            float before = dot2(sub(start, s.a), outward), after = dot2(sub(center, s.a), outward);
+            if (!(before > 0.0f) || !(after <= 0.0f)) continue;
+            float2 crossing = add(start, mul(sub(center, start), before / (before - after)));
+            float along = dot2(sub(crossing, s.a), edge) / lengthSquared;
+            if (along < 0.0f || along > 1.0f) continue;
+            found = true;
+            mine.kind = BP_VIOLATION_EDGE_SIDE; mine.segment = W.A.segId[g];
+            mine.other = (uint64_t)W.A.segOwner[g]; mine.detail = after;
+        }
+        if (found) { firstIndex = i; break; }
+    }
+    uint32_t winner = firstIndex;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) winner = min(winner, (uint32_t)__shfl_xor((int)winner, off, 64));
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) sh.scan[threadIdx.x >> 6] = winner;
+    __syncthreads();
+    winner = 0xffffffffu;
+    for (int w = 0; w < BLOCK / 64; ++w) winner = min(winner, sh.scan[w]);
+    if (winner == 0xffffffffu) return false;
+    if (firstIndex == winner) {
+        st->violationKind = mine.kind;
+        st->violationSegment = mine.segment;
+        st->violationObjects[0] = (uint64_t)W.id[winner];
+        st->violationObjects[1] = mine.other;
+        st->violationDetail = mine.detail;
+    }
+    return true;
+}
+
+// ---------------------------------------------------------------------------------------
+// The step kernel.
+// ---------------------------------------------------------------------------------------
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void world_step_kernel(Arena A, WorldDesc *descs, WorldStatus *stats,
+                                                           const uint32_t *order, uint32_t count) {
+    __shared__ Shared<BLOCK> sh;
+    if (blockIdx.x >= count) return;
+    const uint32_t wi = order[blockIdx.x];
+    WorldDesc *d = descs + wi;
+    WorldStatus *st = stats + wi;
+    if (d->failed) {
+        if (threadIdx.x == 0) st->executed = 0;
+        return;
+    }
+    const uint32_t pub = d->pub, wside = 1u - pub;
+    World W;
+    W.A = A; W.d = d; W.n = d->bubbleCount; W.ns = d->segmentCount; W.nb = d->bodyCount; W.w = wside;
+    const uint32_t o = d->bubbleOffset;
+    W.o = o;
+    W.pos = A.pos[wside] + o; W.start = A.start[wside] + o; W.vel = A.vel[wside] + o;
+    W.rot = A.rot[wside] + o; W.omega = A.omega[wside] + o; W.region = A.region[wside] + o;
+    W.id = A.id + o; W.invmRaw = A.invm + o; W.radius = A.radius + o; W.Kc = A.Kc + o; W.gamma = A.gamma + o;
+    W.mass = A.mass + o; W.flags = A.flags + o; W.cat = A.cat + o; W.mask = A.mask + o;
+    W.motor = A.motor + o; W.band = A.band + o;
+    W.acc = A.acc + o; W.a0 = A.a0 + o; W.ref = A.ref + o; W.margin = A.margin + o;
+    W.rowStart = A.rowStart + o; W.rowLen = A.rowLen + o; W.grid = A.grid + o; W.adj = A.adj + d->adjOffset;
+    W.motors = A.motors + d->motorOffset; W.bands = A.bands + d->bandOffset;
+    W.verts = A.verts + d->vertexOffset; W.tris = A.tris + d->triangleOffset;
+    W.cellWidth = 0.0f;
+    W.globalForce = f2(d->globalForce[0], d->globalForce[1]);
+    W.sepTol = d->sepTol;
+    const uint32_t n = W.n;
+    float *history = A.history + d->historyOffset;
+
+    // 1. Begin step (Force:285-293) on a working copy of the published state.
+    for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
+        float2 p = A.pos[pub][o + i];
+        float2 v = A.vel[pub][o + i];
+        float om = A.omega[pub][o + i];
+        uint32_t f = A.flags[o + i];
+        if ((f & kEnabled) && (f & kStatic)) { v = f2(0.0f, 0.0f); om = 0.0f; }
+        W.pos[i] = p; W.start[i] = p; W.vel[i] = v; W.rot[i] = A.rot[pub][o + i]; W.omega[i] = om;
+        W.region[i] = BP_REGION_UNCHANGED;
+    }
+    // Kinematic and static bodies (PostSolve:66-82 for the bodies a bubble world may hold).
+    uint32_t safety = 0;
+    for (uint32_t k = threadIdx.x; k < W.nb; k += BLOCK) {
+        uint32_t g = d->bodyOffset + k;
+        float4 pose = A.bodyPose[pub][g];
+        float4 motion = A.bodyMotion[g];
+        uint4 kind = A.bodyKind[g];
+        float2 prev = f2(pose.x, pose.y);
+        pose.w = pose.z;
+        if (kind.y & kBodyStatic) { motion.x = motion.y = motion.z = 0.0f; }
+        else {
+            float dt = d->dt;
+            motion.z = motion.z * bpm::expf_det(-motion.w * dt);
+            float2 lin = f2(motion.x, motion.y);
+            float2 lim = A.bodyLimits[g];
+            float speed = len2(lin);
+            if (lim.x > 0.0f && speed > lim.x) lin = mul(lin, lim.x / speed);
+            motion.x = lin.x; motion.y = lin.y;
+            if (lim.y > 0.0f) motion.z = clampf(motion.z, -lim.y, lim.y);
+            pose.x = pose.x + motion.x * dt;
+            pose.y = pose.y + motion.y * dt;
+            pose.z = pose.z + motion.z * dt;
+            if (!isfinite(pose.x) || !isfinite(pose.y) || !isfinite(pose.z) || !isfinite(pose.w) || !isfinite(motion.x) ||
+                !isfinite(motion.y) || !isfinite(motion.z))
+                safety |= 1u;
+        }
+        A.bodyPose[wside][g] = pose;
+        A.bodyPrev[wside][g] = prev;
+        A.bodyMotion[g] = motion;
+    }
+    __syncthreads();
+
+    const uint32_t limit = d->limit;
+    Ctl c;
+    c.t0 = d->dt; c.t1 = n == 0 ? d->dt : 0.0f; c.h = 0.0f; c.split = (float)(d->split < 1 ? 1u : d->split);
+    c.c0 = c.c1 = c.rebuild = c.limit = 0; c.limObj = 0; c.limTerm = BP_TERM_REMAINING;
+    uint32_t ov0 = 0, ov1 = 0, ov2 = 0, ov3 = 0, pairs = 0, entriesTotal = 0;
+    int32_t status = BP_OK;
+    bool claimed = false;                 // sticky contact-assertion record of the frame
+
+    if (n > 0) {
+        float w2Max, mbMin, dbMin;
+        uint32_t w2Own, mbOwn, dbOwn;
+        RowStats rs;
+        bool ok = build_grid<BLOCK>(W, sh, c.t0 - c.t1, limit, rs, entriesTotal);
+        ov0 += rs.dropped; ov1 = max(ov1, rs.maxDemand); pairs = rs.pairs;
+        if (!ok) status = BP_ERR_CAPACITY_OVERFLOW;
+        if (status == BP_OK) {
+            c.h = 0.0f;
+            pass_forces<BLOCK>(W, sh, c, 2u, w2Max, w2Own, mbMin, mbOwn);
+            for (uint32_t iteration = 0; iteration < 2 * kFrameLimit + 2; ++iteration) {
+                // Reduction of h (Force:625-736).
+                float remaining = c.t0 - c.t1;
+                bool over = !(remaining > 0.0f), stopped = c.rebuild != 0 || c.limit != 0;
+                bool limited = c.c1 >= kFrameLimit;
+                if (over || stopped || limited) {
+                    if (limited && !over && !stopped) c.limit = 1;
+                    c.h = 0.0f;
+                } else {
+                    float stiffnessBound = w2Max > 0.0f ? 1.0f / __builtin_sqrtf(w2Max) : inff();
+                    float hMax = fminf(remaining, stiffnessBound);
+                    pass_distance<BLOCK>(W, sh, c, hMax, dbMin, dbOwn);
+                    if (mbMin <= 0.0f) {
+                        c.rebuild = 1; c.h = 0.0f;
+                        c.limObj = (uint64_t)W.id[mbOwn]; c.limTerm = BP_TERM_MARGIN;
+                    } else {
+                        float split = fmaxf(c.split, 1.0f);
+                        float h = remaining;
+                        uint32_t term = BP_TERM_REMAINING;
+                        uint64_t object = 0;
+                        if (split > 1.0f && c.t0 / split < h) h = c.t0 / split;
+                        if (stiffnessBound / split < h) { h = stiffnessBound / split; term = BP_TERM_STIFFNESS; object = (uint64_t)W.id[w2Own]; }
+                        if (dbMin / split < h) { h = dbMin / split; term = BP_TERM_DISTANCE; object = (uint64_t)W.id[dbOwn]; }
+                        if (mbMin < h) { h = mbMin; term = BP_TERM_MARGIN; object = (uint64_t)W.id[mbOwn]; c.rebuild = 1; }
+                        h = fmaxf(h, c.t0 * 1e-7f);
+                        uint32_t frameSubstep = c.c1;
+                        if (h >= remaining || remaining - h <= c.t0 * 1e-5f) { h = remaining; c.t1 = c.t0; }
+                        else c.t1 = c.t1 + h;
+                        c.h = h;
+                        c.c0 += 1;
+                        c.c1 = frameSubstep + 1;
+                        c.limObj = object; c.limTerm = term;
+                        if (threadIdx.x == 0) history[frameSubstep] = h;
+                    }
+                }
+                bool finished = !(c.t0 - c.t1 > 0.0f);
+                if (!(c.h > 0.0f)) {
+                    if (c.limit || finished || !c.rebuild) break;
+                    c.c0 = 0; c.rebuild = 0;
+                    ov2 = max(ov2, ov0); ov0 = 0; ov3 += 1;
+                    __syncthreads();
+                    ok = build_grid<BLOCK>(W, sh, c.t0 - c.t1, limit, rs, entriesTotal);
+                    ov0 += rs.dropped; ov1 = max(ov1, rs.maxDemand); pairs = rs.pairs;
+                    if (!ok) { status = BP_ERR_CAPACITY_OVERFLOW; break; }
+                    pass_forces<BLOCK>(W, sh, c, 2u, w2Max, w2Own, mbMin, mbOwn);
+                    continue;
+                }
+                pass_drift<BLOCK>(W, c);
+                __syncthreads();
+                pass_forces<BLOCK>(W, sh, c, 1u, w2Max, w2Own, mbMin, mbOwn);
+                if (d->injectionEnabled && c.c1 == d->injectionAfter + 1) {
+                    for (uint32_t i = threadIdx.x; i < n; i += BLOCK)
+                        if (W.id[i] == d->injectionObject)
+                            W.pos[i] = f2(d->injectionPosition[0], d->injectionPosition[1]);
+                    __syncthreads();
+                    pass_bounds<BLOCK>(W, sh, c, w2Max, w2Own, mbMin, mbOwn);
+                }
+                if (d->assertEverySubstep && !claimed) {
+                    if (assert_contacts<BLOCK>(W, sh, c, st)) claimed = true;
+                    __syncthreads();
+                }
+            }
+            ov2 = max(ov2, ov0);
+        }
+    }
+    // 4. Gated end of step (NativeMetalSolver.swift:837-945; Force:822-828).
+    bool finished = c.limit == 0 && !(c.t0 - c.t1 > 0.0f);
+    uint32_t removals = 0, wraps = 0;
+    if (status == BP_OK && finished) {
+        if (d->applyRegion) {                                   // PostSolve:536-566
+            float2 lo = f2(d->region[0], d->region[1]), hi = f2(d->region[2], d->region[3]);
+            for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
+                uint32_t f = W.flags[i];
+                if (!(f & kEnabled) || (f & kStatic)) continue;
+                float2 p = W.pos[i];
+                float r = W.radius[i];
+                float2 nearest = f2(clampf(p.x, lo.x, hi.x), clampf(p.y, lo.y, hi.y));
+                if (len2(sub(p, nearest)) <= r) continue;
+                if (!(f & kWrap)) { W.region[i] = BP_REGION_REMOVED; continue; }
+                float2 span = adds(sub(hi, lo), 2.0f * r);
+                float delta[2] = {0.0f, 0.0f};
+                float pp[2] = {p.x, p.y}, rl[2] = {lo.x, lo.y}, rh[2] = {hi.x, hi.y}, sp[2] = {span.x, span.y};
+                for (int axis = 0; axis < 2; ++axis) {
+                    if (pp[axis] > rh[axis]) delta[axis] = -sp[axis] * fmaxf(1.0f, ceilf((pp[axis] - r - rh[axis]) / sp[axis]));
+                    else if (pp[axis] < rl[axis]) delta[axis] = sp[axis] * fmaxf(1.0f, ceilf((rl[axis] - r - pp[axis]) / sp[axis]));
+                }
+                float2 dl = f2(delta[0], delta[1]);
+                float2 wrapped = add(p, dl);
+                float2 nearestWrapped = f2(clampf(wrapped.x, lo.x, hi.x), clampf(wrapped.y, lo.y, hi.y));
+                if (len2(sub(wrapped, nearestWrapped)) > r) dl = add(dl, sub(nearestWrapped, wrapped));
+                float2 np = add(p, dl), ns = add(W.start[i], dl);
+                if (!finite2(np) || !finite2(ns)) { safety |= 1u; continue; }
+                W.pos[i] = np; W.start[i] = ns;
+                W.region[i] = BP_REGION_WRAPPED;
+            }
+        }
+        for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {     // PostSolve:5-27
+            if (!finite2(W.pos[i]) || !finite2(W.start[i]) || !finite2(W.vel[i]) || !isfinite(W.rot[i]) ||
+                !isfinite(W.omega[i]) || !isfinite(W.mass[i]) || !isfinite(W.invmRaw[i]) || !isfinite(W.radius[i]))
+                safety |= 1u;
+        }
+        for (uint32_t j = threadIdx.x; j < W.ns; j += BLOCK) {
+            Seg s = W.segment(j);
+            float2 e = sub(s.b, s.a);
+            if (!finite2(s.a) || !finite2(s.b) || !finite2(e) || !isfinite(dot2(e, e))) safety |= 1u;
+        }
+        __syncthreads();
+        if (!claimed && n > 0 && assert_contacts<BLOCK>(W, sh, c, st)) claimed = true;
+    }
+    // Safety bits of every lane.
+    __syncthreads();
+    if (threadIdx.x == 0) sh.counters[5] = 0;
+    __syncthreads();
+    if (safety) atomicOr(&sh.counters[5], safety);
+    uint32_t rmLocal = 0, wrLocal = 0;
+    for (uint32_t i = threadIdx.x; i < n; i += BLOCK) { rmLocal += W.region[i] == BP_REGION_REMOVED; wrLocal += W.region[i] == BP_REGION_WRAPPED; }
+    if (threadIdx.x == 0) { sh.counters[6] = 0; sh.counters[7] = 0; }
+    __syncthreads();
+    atomicAdd(&sh.counters[6], rmLocal);
+    atomicAdd(&sh.counters[7], wrLocal);
+    __syncthreads();
+    removals = sh.counters[6]; wraps = sh.counters[7];
+    uint32_t safetyAll = sh.counters[5];
+    // Error precedence (NativeMetalSolver.swift:1026-1040) and the host finite check (:1067).
+    if (status == BP_OK) {
+        if (c.limit) status = BP_ERR_SUBSTEP_LIMIT_EXCEEDED;
+        else if (safetyAll & 1u) status = BP_ERR_NON_FINITE;
+        else if (claimed) status = BP_ERR_CONTACT_VIOLATION;
+        else if (n > 0 && !finished) status = BP_ERR_COMMAND_FAILED;   // loop ended without finishing
+    }
+    if (threadIdx.x == 0) {
+        st->status = status;
+        st->substeps = c.c1;
+        st->rebuilds = ov3;
+        st->candidatePairs = pairs;
+        st->candidateOverflow = ov2;
+        st->maxCandidates = ov1;
+        st->removals = removals;
+        st->wraps = wraps;
+        st->time[0] = c.t0; st->time[1] = c.t1; st->time[2] = c.h; st->time[3] = c.split;
+        st->limiterObject = c.limObj;
+        st->limiterTerm = c.limTerm;
+        if (!claimed) {
+            st->violationKind = 0; st->violationSegment = 0; st->violationObjects[0] = 0; st->violationObjects[1] = 0;
+            st->violationDetail = 0.0f;
+        }
+        st->adjacencyEntries = entriesTotal;
+        st->groupWidth = BLOCK;
+        st->executed = 1;
+        if (status == BP_OK) d->pub = wside;
+        else if (status != BP_ERR_CAPACITY_OVERFLOW) d->failed = 1;
+    }
+}
+
+}  // namespace bpd
+
+// Host launchers (used by bp_api.cpp).
+extern "C" hipError_t bp_launch_world_step(int block, const bpd::Arena *arena, bpd::WorldDesc *descs, bpd::WorldStatus *stats,
+                                           const uint32_t *order, uint32_t count, hipStream_t stream) {
+    if (count == 0) return hipSuccess;
+    switch (block) {
+        case 256: hipLaunchKernelGGL(bpd::world_step_kernel<256>, dim3(count), dim3(256), 0, stream, *arena, descs, stats, order, count); break;
+        case 512: hipLaunchKernelGGL(bpd::world_step_kernel<512>, dim3(count), dim3(512), 0, stream, *arena, descs, stats, order, count); break;
+        case 1024: hipLaunchKernelGGL(bpd::world_step_kernel<1024>, dim3(count), dim3(1024), 0, stream, *arena, descs, stats, order, count); break;
+        default: hipLaunchKernelGGL(bpd::world_step_kernel<64>, dim3(count), dim3(64), 0, stream, *arena, descs, stats, order, count); break;
+    }
+    return hipGetLastError();
+}

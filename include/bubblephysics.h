@@ -1,0 +1,324 @@
+/*
+ * bubblephysics.h -- C ABI of the MI355X-native world step of BubblePhysics 4.0.0.
+ *
+ * This header is the drop-in boundary for the reference's solver seam
+ *   NativeMetalFrameExecuting.executeFrame(input: NativeSolverInput, scenarioStep:) -> NativeSolverFrame
+ *   (Sources/BubblePhysicsNativeMetal/NativeSolverInput.swift:255-264),
+ * implemented in the reference by NativeMetalSolver.executeSnapshot
+ *   (Sources/BubblePhysicsNativeMetal/NativeMetalSolver.swift:380-1218).
+ *
+ * Plain C types only: fixed-width integers, float arrays and opaque handles.
+ * The library owns every device allocation; input arrays are borrowed for the
+ * duration of a call; output arrays are allocated by the caller.
+ *
+ * Record layouts are shared by the HIP library (libbubblephysics_hip.so) and
+ * by the CPU oracle used in the tests.
+ */
+#ifndef BUBBLEPHYSICS_H
+#define BUBBLEPHYSICS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BP_ABI_VERSION 1u
+
+/* ------------------------------------------------------------------------ */
+/* Status codes.                                                              */
+/* One code per case of NativeMetalSolverError (NativeMetalSolver.swift:5-7)  */
+/* and NativeMetalWorldError (NativeSolverInput.swift:266-272).               */
+/* ------------------------------------------------------------------------ */
+typedef int32_t bp_status;
+enum {
+    BP_OK = 0,
+    BP_ERR_INVALID_ARGUMENT = 1,       /* malformed call (null handle, size mismatch)        */
+    BP_ERR_RESOURCE_UNAVAILABLE = 2,   /* NativeMetalSolverError.resourceUnavailable (no GPU) */
+    BP_ERR_ALLOCATION_FAILED = 3,      /* NativeMetalSolverError.allocationFailed             */
+    BP_ERR_COMMAND_SETUP_FAILED = 4,   /* NativeMetalSolverError.commandSetupFailed           */
+    BP_ERR_COMMAND_FAILED = 5,         /* NativeMetalSolverError.commandFailed                */
+    BP_ERR_INVALID_SNAPSHOT = 6,       /* NativeMetalSolverError.invalidSnapshot              */
+    BP_ERR_CAPACITY_OVERFLOW = 7,      /* NativeMetalSolverError.capacityOverflow             */
+    BP_ERR_FRAME_IN_FLIGHT = 8,        /* NativeMetalWorldError.frameInFlight                 */
+    BP_ERR_NON_FINITE = 9,             /* NativeMetalWorldError.nonFinite                     */
+    BP_ERR_GUARD_FAILURE = 10,         /* NativeMetalWorldError.guardFailure                  */
+    BP_ERR_CONTACT_VIOLATION = 11,     /* NativeMetalWorldError.contactViolation              */
+    BP_ERR_SUBSTEP_LIMIT_EXCEEDED = 12,/* NativeMetalWorldError.substepLimitExceeded          */
+    BP_ERR_UNAVAILABLE = 13            /* NativeMetalWorldError.unavailable                   */
+};
+
+/* Term that fixed h in the last substep: NativeMetalSubstep.Term
+ * (NativeForceRecords.swift:104-106). */
+enum { BP_TERM_REMAINING = 0, BP_TERM_STIFFNESS = 1, BP_TERM_DISTANCE = 2, BP_TERM_MARGIN = 3, BP_TERM_NONE = 0xffffffffu };
+
+/* Kind of contact violation: NativeMetalContactAssertion.Kind (NativeForceRecords.swift:44-46). */
+enum { BP_VIOLATION_NONE = 0, BP_VIOLATION_PAIR = 1, BP_VIOLATION_INSIDE_BODY = 2, BP_VIOLATION_EDGE_SIDE = 3 };
+
+/* Execution path (NativeMetalExecutionPath, NativeMetalBackend.swift:60-65), extended with the
+ * MI355X forms of this library. */
+enum {
+    BP_PATH_WORKGROUP = 0,        /* whole world in one workgroup (reference: singleDispatch)     */
+    BP_PATH_COOPERATIVE = 1,      /* world spread over many workgroups (reference: multipleDispatches) */
+    BP_PATH_NONE = 2              /* world without bubbles: no substep loop                       */
+};
+
+/* ------------------------------------------------------------------------ */
+/* Records.                                                                   */
+/* ------------------------------------------------------------------------ */
+
+/* bp_bubble.flags: publicFlags.xyz and the force-driven marker contact.w of NativeBubble
+ * (NativeGeometryKernels.metal-source:4-17; NativePublicWorld.swift:203-212). */
+enum {
+    BP_BUBBLE_ENABLED = 1u,       /* publicFlags.x */
+    BP_BUBBLE_STATIC = 2u,        /* publicFlags.y: motion role .static            */
+    BP_BUBBLE_WRAP = 4u,          /* publicFlags.z: region exit policy .wrap        */
+    BP_BUBBLE_FORCE_DRIVEN = 8u   /* contact.w: roles dynamic, motorDriven, playerDriven */
+};
+
+/* Region result written by the step (publicFlags.w, NativePostSolveKernels.metal-source:536-566). */
+enum { BP_REGION_UNCHANGED = 0, BP_REGION_REMOVED = 1, BP_REGION_WRAPPED = 2 };
+
+/* One bubble: NativeSolverBubble (NativeSolverInput.swift:18-57) with the public properties the
+ * step reads (NativePublicProperties, NativePublicWorld.swift:68-94). 72 bytes. */
+typedef struct bp_bubble {
+    int64_t id;                 /* ObjectID raw value; a world's bubbles are sorted by id        */
+    float center[2];            /* position.xy                                                    */
+    float previous_center[2];   /* position.zw: output only, centre at the start of the step      */
+    float velocity[2];          /* motion.xy                                                      */
+    float rotation;             /* motion.z                                                       */
+    float angular_velocity;     /* motion.w                                                       */
+    float mass;                 /* physical.x                                                     */
+    float inverse_mass;         /* physical.y; 0 = static                                         */
+    float radius;               /* physical.z (targetRadius)                                      */
+    float contact_stiffness;    /* contact.y (K_c); <= 0 means no contact force                   */
+    float damping;              /* material.x (gamma, 1/s)                                        */
+    uint32_t flags;             /* BP_BUBBLE_*                                                    */
+    uint32_t category;          /* filter.x                                                       */
+    uint32_t mask;              /* filter.y                                                       */
+} bp_bubble;
+
+/* bp_segment.flags (NativeMetalSegment.init, NativeMetalBufferLayout.swift:64-78). */
+enum {
+    BP_SEGMENT_HAS_OWNER = 1u,    /* flags.y */
+    BP_SEGMENT_POLYGON_EDGE = 2u, /* flags.w bit 0: one-sided static polygon edge with regions */
+    BP_SEGMENT_CONVEX_START = 4u, /* flags.w bit 1 */
+    BP_SEGMENT_CONVEX_END = 8u,   /* flags.w bit 2 (informative) */
+    BP_SEGMENT_ALLOWED_SIDE = 16u /* flags.z: allowed_side is set */
+};
+
+/* One static segment: NativeSolverSegment (NativeSolverInput.swift:80-114) packed as
+ * NativeMetalSegment. A world's segments are sorted by id. 64 bytes. */
+typedef struct bp_segment {
+    int64_t id;
+    int64_t owner_id;
+    float a[2];                 /* current.xy */
+    float b[2];                 /* current.zw */
+    float previous_start[2];    /* region.xy: start of the previous polygon edge */
+    float outward_sign;         /* region.z */
+    float allowed_side;         /* collision.x */
+    uint32_t flags;             /* BP_SEGMENT_* */
+    uint32_t category;
+    uint32_t mask;
+    uint32_t reserved;
+} bp_segment;
+
+/* bp_body.shape (NativeRigidBody.shape.x, NativeRigidPolygonState.swift:34-36). */
+enum { BP_SHAPE_POLYGON = 0, BP_SHAPE_CIRCLE = 1, BP_SHAPE_CAPSULE = 2 };
+/* bp_body.flags. */
+enum { BP_BODY_STATIC = 1u, BP_BODY_KINEMATIC = 2u };
+
+/* A rigid body as a bubble obstacle: the fields of NativeRigidBody
+ * (NativeRigidPolygonState.swift:24-40) that the bubble step reads. Poses are centre-of-mass
+ * poses; vertices are local (centroid-relative) and shared in one array per world.
+ * Dynamic bodies never share a world with bubbles (NativeCompleteStepper.swift:71-88) and are
+ * rejected. 96 bytes. */
+typedef struct bp_body {
+    uint64_t id;
+    float pose[3];              /* x, y, angle at the end of the last step             */
+    float previous[3];          /* output: x, y, angle at the start of the last step   */
+    float velocity[3];          /* vx, vy, angular velocity (kinematic drive)          */
+    float bounding_radius;      /* physical.z                                          */
+    float core_radius;          /* shape.y                                             */
+    float half_length;          /* shape.z                                             */
+    float max_linear_speed;     /* limits.x, 0 = none                                   */
+    float max_angular_speed;    /* limits.y, 0 = none                                   */
+    uint32_t shape;             /* BP_SHAPE_*                                           */
+    uint32_t flags;             /* BP_BODY_*                                            */
+    uint32_t category;
+    uint32_t mask;
+    uint32_t vertex_offset;     /* ranges.x */
+    uint32_t vertex_count;      /* ranges.y */
+    uint32_t triangle_offset;   /* ranges.z */
+    uint32_t triangle_count;    /* ranges.w */
+} bp_body;
+
+/* Motor drive of a .motorDriven bubble: NativeMetalMotor (NativeForceRecords.swift:29-32). */
+typedef struct bp_motor {
+    int64_t bubble_id;
+    float force[2];
+    float torque;
+    float inertia;
+} bp_motor;
+
+/* Rubber band of a .playerDriven bubble: NativeMetalBandDrive (NativeForceRecords.swift:9-12). */
+typedef struct bp_band {
+    int64_t bubble_id;
+    float target[2];
+    float stiffness;
+    float maximum_force;        /* 0 = no limit */
+} bp_band;
+
+/* Step configuration: NativeSolverConfiguration (NativeSolverInput.swift:116-162) plus the
+ * package switches of NativeMetalConfiguration (NativeMetalBackend.swift:2-34). */
+typedef struct bp_world_config {
+    float time_step;              /* dt                                                    */
+    float contact_tolerance;      /* thresholds.z (0.001)                                  */
+    float separation_tolerance;   /* thresholds.w (0.002)                                  */
+    float region[4];              /* min.x, min.y, max.x, max.y                            */
+    float global_force[2];        /* NativeMetalForceControl.global                        */
+    uint32_t apply_region;        /* 1 = public path: region removal/wrap at step end     */
+    uint32_t candidate_limit;     /* k per object, 1..128 (default 128)                    */
+    uint32_t substep_split;       /* substepSplitMultiplier, >= 1                          */
+    uint32_t minimum_margin_only; /* 1 = margin r/4 only                                   */
+    uint32_t grid_bucket_count;   /* hashed-grid buckets, 1..2048 (default 2048)           */
+    uint32_t assert_every_substep;/* 1 = contact assertion after each substep              */
+    uint32_t injection_enabled;   /* injectedSubstepState present                          */
+    uint32_t injection_after_substep;
+    int64_t injection_object;
+    float injection_position[2];
+    uint32_t force_cooperative;   /* 1 = multi-workgroup path even for small worlds        */
+    uint32_t reserved;
+} bp_world_config;
+
+/* Details of a failure: NativeMetalFailureDetail (NativeSolverInput.swift:280-290). */
+typedef struct bp_failure_detail {
+    uint64_t object_ids[2];
+    int32_t violation_kind;       /* BP_VIOLATION_*                                  */
+    int32_t edge_segment_id_low;  /* low 32 bits of the segment id for edge side, else -1 */
+    float distance;               /* NaN when absent                                  */
+    uint32_t limiting_term;       /* BP_TERM_*                                         */
+} bp_failure_detail;
+
+/* Per-step result: telemetry of NativeMetalFrameTelemetry (NativeMetalBackend.swift:72-122)
+ * and the overflow counters of NativeSolverFrame (NativeSolverInput.swift:247-250). */
+typedef struct bp_frame_info {
+    bp_status status;
+    uint32_t substep_count;          /* substepControl.counts.y                         */
+    uint32_t grid_rebuild_count;     /* rebuilds forced by the margin term              */
+    uint32_t candidate_pair_count;   /* contactControl.counts.x of the last grid build  */
+    uint32_t candidate_overflow;     /* dropped candidates, max over grid builds        */
+    uint32_t max_candidates_per_object;
+    uint32_t execution_path;         /* BP_PATH_*                                       */
+    uint32_t group_width;            /* threads of the workgroup that ran the loop      */
+    uint32_t region_removals;
+    uint32_t region_wraps;
+    float gpu_milliseconds;
+    float simulated_time;            /* substepControl.time.y at the end                */
+    uint64_t limiter_object;         /* substepControl.limiter.x                        */
+    uint32_t limiter_term;           /* substepControl.limiter.y                        */
+    uint32_t reserved;
+    bp_failure_detail detail;
+} bp_frame_info;
+
+/* ------------------------------------------------------------------------ */
+/* Library.                                                                   */
+/* ------------------------------------------------------------------------ */
+uint32_t bp_abi_version(void);
+/* Fills a configuration with the reference defaults (NativeSolverConfiguration.init,
+ * NativeSolverInput.swift:128-143; NativeMetalConfiguration, NativeMetalBackend.swift:2-34). */
+void bp_world_config_default(bp_world_config *config);
+const char *bp_status_string(bp_status status);
+/* Last error text of the calling thread (device error strings, argument checks). */
+const char *bp_last_error(void);
+
+/* ------------------------------------------------------------------------ */
+/* Plain world: one world per handle (NativeMetalSolver, NativeMetalSolver.swift:255-378).     */
+/* ------------------------------------------------------------------------ */
+typedef struct bp_world bp_world;
+
+bp_status bp_world_create(const bp_world_config *config, bp_world **out_world);
+void bp_world_destroy(bp_world *world);
+bp_status bp_world_set_config(bp_world *world, const bp_world_config *config);
+/* Full snapshot of the bubbles (replaces the published state; ids sorted ascending). */
+bp_status bp_world_set_bubbles(bp_world *world, const bp_bubble *bubbles, uint32_t count);
+/* Delta upload: bubbleChanges of NativeMetalSnapshot (NativeMetalBufferLayout.swift:209-289). */
+bp_status bp_world_update_bubbles(bp_world *world, const uint32_t *indices, const bp_bubble *bubbles, uint32_t count);
+bp_status bp_world_set_segments(bp_world *world, const bp_segment *segments, uint32_t count);
+bp_status bp_world_set_bodies(bp_world *world, const bp_body *bodies, uint32_t count,
+                              const float *vertices, uint32_t vertex_count,
+                              const uint32_t *triangles, uint32_t triangle_count);
+bp_status bp_world_set_forces(bp_world *world, const bp_motor *motors, uint32_t motor_count,
+                              const bp_band *bands, uint32_t band_count);
+/* One step of dt (executeFrame). On any error the published state is unchanged. */
+bp_status bp_world_step(bp_world *world, int32_t scenario_step, bp_frame_info *out_info);
+bp_status bp_world_read_bubbles(bp_world *world, bp_bubble *out, uint32_t count);
+bp_status bp_world_read_bodies(bp_world *world, bp_body *out, uint32_t count);
+uint32_t bp_world_bubble_count(const bp_world *world);
+/* Adjacency of the last grid build (test seam, like NativeMetalSolver.candidateKeys):
+ * row_offsets[count+1], entries packed as (kind << 30) | neighbour index. */
+bp_status bp_world_read_adjacency(bp_world *world, uint32_t *row_offsets, uint32_t offsets_count,
+                                  uint32_t *entries, uint32_t entries_capacity, uint32_t *out_entry_count);
+/* h of each substep of the last step (stepHistory), up to 1024 values. */
+bp_status bp_world_read_step_history(bp_world *world, float *out, uint32_t capacity, uint32_t *out_count);
+/* Test switch mirroring the injected NativeMetalCommandExecuting of the reference
+ * (NativeMetalSolverLifecycleTests.swift:292-308): the next step fails with BP_ERR_COMMAND_FAILED
+ * before touching the device. */
+bp_status bp_world_debug_fail_next_command(bp_world *world);
+
+/* ------------------------------------------------------------------------ */
+/* Host geometry preparation.                                                */
+/* ------------------------------------------------------------------------ */
+/* Static polygon -> one-sided solver segments with edge regions
+ * (NativeStaticPolygonEdges.segments, NativeStaticPolygonEdges.swift:16-37). Polygons of zero
+ * area give plain two-sided segments. Writes *out_count segments (ids first_id, first_id+1, ...). */
+bp_status bp_static_polygon_edges(const float *points, uint32_t count, int64_t first_id, int64_t owner_id, uint32_t has_owner,
+                                  bp_segment *out, uint32_t capacity, uint32_t *out_count);
+/* Polygon rigid shape -> centroid, centroid-relative counter-clockwise vertices, ear-clipped
+ * triangles (a, b, c, 0) and bounding radius (NativeRigidGeometry.init,
+ * NativeRigidPolygonState.swift:89-133). out_vertices holds 2*count floats, out_triangles 4*count. */
+bp_status bp_polygon_body_geometry(const float *local_points, uint32_t count, float *out_centroid, float *out_vertices,
+                                   uint32_t *out_vertex_count, uint32_t *out_triangles, uint32_t *out_triangle_count,
+                                   float *out_radius);
+
+/* Random sequential placement of non-overlapping discs in a rectangle (host spawning helper,
+ * in the role of PlacementGeometry, Sources/BubblePhysicsRuntime/PlacementGeometry.swift:42-50).
+ * region = (min.x, min.y, max.x, max.y); obstacles are (x, y, r) triples to keep clear of; discs
+ * that find no place after `attempts` tries get NaN centres. */
+bp_status bp_place_discs(const float *radii, uint32_t count, const float *region, float inset, float gap,
+                         const float *obstacles, uint32_t obstacle_count, uint64_t seed, uint32_t attempts,
+                         float *out_xy, uint32_t *out_placed);
+
+/* ------------------------------------------------------------------------ */
+/* Batch: many independent worlds advanced by one launch per step.             */
+/* No reference counterpart; built on the same kernels as bp_world.            */
+/* ------------------------------------------------------------------------ */
+typedef struct bp_batch bp_batch;
+
+typedef struct bp_batch_world_desc {
+    bp_world_config config;
+    const bp_bubble *bubbles;   uint32_t bubble_count;
+    const bp_segment *segments; uint32_t segment_count;
+    const bp_body *bodies;      uint32_t body_count;
+    const float *vertices;      uint32_t vertex_count;     /* float2 pairs */
+    const uint32_t *triangles;  uint32_t triangle_count;   /* uint4 records (a, b, c, 0) */
+    const bp_motor *motors;     uint32_t motor_count;
+    const bp_band *bands;       uint32_t band_count;
+} bp_batch_world_desc;
+
+bp_status bp_batch_create(bp_batch **out_batch);
+void bp_batch_destroy(bp_batch *batch);
+/* Adds a world; returns its index. Worlds are packed on the next step. */
+bp_status bp_batch_add_world(bp_batch *batch, const bp_batch_world_desc *desc, uint32_t *out_index);
+uint32_t bp_batch_world_count(const bp_batch *batch);
+/* Advances every world by `steps` steps; infos[world_count * steps] (step-major) may be null.
+ * A failed world stops at its failing step and keeps its last published state. */
+bp_status bp_batch_step(bp_batch *batch, uint32_t steps, bp_frame_info *infos);
+bp_status bp_batch_read_bubbles(bp_batch *batch, uint32_t world, bp_bubble *out, uint32_t count);
+/* Device time of the last bp_batch_step (HIP events around the launches). */
+float bp_batch_last_gpu_milliseconds(const bp_batch *batch);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BUBBLEPHYSICS_H */

@@ -1,0 +1,132 @@
+"""ctypes wrapper of the CPU oracle (oracle/bp_oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and by the CPU-baseline
+leg of bench.py, as the checker of the HIP library. Nothing in bubblephysics_amd imports it.
+"""
+from __future__ import annotations
+
+import ctypes
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+ORACLE_DIR = Path(__file__).resolve().parent
+LIB_PATH = ORACLE_DIR / "lib" / "libbp_oracle.so"
+
+
+class OracleFrame(ctypes.Structure):
+    _fields_ = [
+        ("status", ctypes.c_int32), ("substeps", ctypes.c_uint32), ("rebuilds", ctypes.c_uint32),
+        ("candidate_pairs", ctypes.c_uint32), ("candidate_overflow", ctypes.c_uint32), ("max_candidates", ctypes.c_uint32),
+        ("removals", ctypes.c_uint32), ("wraps", ctypes.c_uint32), ("elapsed", ctypes.c_double),
+        ("limiter_object", ctypes.c_uint64), ("limiter_term", ctypes.c_uint32), ("violation_kind", ctypes.c_uint32),
+        ("violation_segment", ctypes.c_int64), ("violation_objects", ctypes.c_uint64 * 2), ("violation_detail", ctypes.c_double),
+    ]
+
+
+FRAME_DTYPE = np.dtype(OracleFrame)
+_lib = None
+
+
+def build() -> Path:
+    """Builds the oracle library with its Makefile (g++, strict IEEE flags)."""
+    subprocess.run(["make", "-s", "-C", str(ORACLE_DIR)], check=True)
+    return LIB_PATH
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        lib = ctypes.CDLL(str(LIB_PATH))
+        P, U32, I32 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int
+        lib.bpo_world_run.argtypes = [P, P, U32, P, U32, P, U32, P, U32, P, U32, P, U32, P, U32, U32, I32, U32, U32, P, P, P,
+                                      U32, P]
+        lib.bpo_world_run.restype = ctypes.c_int
+        lib.bpo_epsilon_f32.argtypes = [P, P, U32]
+        lib.bpo_wall_law_f32.argtypes = [P, P, U32]
+        lib.bpo_pair_law_f32.argtypes = [P, P, P, U32]
+        lib.bpo_half_impulse_factor_f32.argtypes = [P, P, P, U32]
+        lib.bpo_detmath_f32.argtypes = [ctypes.c_int, P, P, U32]
+        _lib = lib
+    return _lib
+
+
+def _p(a):
+    return None if a is None or a.size == 0 else a.ctypes.data
+
+
+def run(scene, steps: int | None = None, *, precision: int = 32, assert_mode: int = 1, grid: str = "auto",
+        adjacency: bool = False, history: bool = False):
+    """Runs `steps` steps of a scene in the oracle; returns (bubbles, bodies, frames, extras).
+
+    The scene is not modified. grid: "auto" (hashed for <= 1024 bubbles like the reference's
+    single-dispatch path, sorted otherwise), "hashed" or "sorted".
+    """
+    lib = load()
+    steps = scene.steps if steps is None else steps
+    bubbles = scene.bubbles.copy()
+    bodies = scene.bodies.copy()
+    frames = np.zeros(max(steps, 1), FRAME_DTYPE)
+    n = len(bubbles)
+    row_offsets = entries = hist = None
+    if adjacency:
+        row_offsets = np.zeros(n + 1, np.uint32)
+        entries = np.zeros(max(n * 256, 16), np.uint32)
+    if history:
+        hist = np.zeros(1024, np.float32)
+    verts = np.ascontiguousarray(scene.vertices, np.float32)
+    tris = np.ascontiguousarray(scene.triangles, np.uint32)
+    override = {"auto": 0, "hashed": 1, "sorted": 2}[grid]
+    executed = lib.bpo_world_run(ctypes.byref(scene.config), _p(bubbles), n, _p(scene.segments), len(scene.segments),
+                                 _p(bodies), len(bodies), _p(verts), len(verts), _p(tris), len(tris), _p(scene.motors),
+                                 len(scene.motors), _p(scene.bands), len(scene.bands), steps, precision, assert_mode,
+                                 override, _p(frames), _p(row_offsets), _p(entries), 0 if entries is None else len(entries),
+                                 _p(hist))
+    extras = {"executed": executed}
+    if adjacency:
+        extras["row_offsets"] = row_offsets
+        extras["entries"] = entries[: row_offsets[-1]].copy()
+    if history:
+        extras["history"] = hist
+    return bubbles, bodies, frames[:executed], extras
+
+
+def epsilon(q):
+    q = np.ascontiguousarray(q, np.float32)
+    out = np.empty_like(q)
+    load().bpo_epsilon_f32(_p(q), _p(out), len(q))
+    return out
+
+
+def wall_law(inputs4):
+    x = np.ascontiguousarray(inputs4, np.float32).reshape(-1, 4)
+    out = np.empty_like(x)
+    load().bpo_wall_law_f32(_p(x), _p(out), len(x))
+    return out
+
+
+def pair_law(inputs4, distances):
+    x = np.ascontiguousarray(inputs4, np.float32).reshape(-1, 4)
+    d = np.ascontiguousarray(distances, np.float32)
+    out = np.empty_like(x)
+    load().bpo_pair_law_f32(_p(x), _p(d), _p(out), len(x))
+    return out
+
+
+def half_impulse_factor(gamma, half):
+    g = np.ascontiguousarray(gamma, np.float32)
+    h = np.ascontiguousarray(half, np.float32)
+    out = np.empty_like(g)
+    load().bpo_half_impulse_factor_f32(_p(g), _p(h), _p(out), len(g))
+    return out
+
+
+def detmath(which: str, x):
+    code = {"sin": 0, "cos": 1, "exp": 2, "log": 3}[which]
+    x = np.ascontiguousarray(x, np.float32)
+    out = np.empty_like(x)
+    load().bpo_detmath_f32(code, _p(x), _p(out), len(x))
+    return out
