@@ -152,7 +152,14 @@ bp_status validate_world(const HostWorld &w) {
         }
         if (b.vertex_count == 0) { set_error("body without vertices"); return BP_ERR_INVALID_SNAPSHOT; }
     }
-    for (size_t t = 0; t + 3 < w.triangles.size(); t += 4) (void)t;
+    for (const bp_body &b : w.bodies)
+        for (uint32_t t = 0; t < b.triangle_count; ++t)
+            for (int k = 0; k < 3; ++k)
+                if (w.triangles[4 * ((size_t)b.triangle_offset + t) + k] >= b.vertex_count) {
+                    set_error("triangle index outside its body's vertices");
+                    return BP_ERR_INVALID_SNAPSHOT;
+                }
+    for (const bp_motor &m : w.motors) (void)m;
     if (w.bubbles.size() > (1u << 30) || w.segments.size() > (1u << 30) || w.bodies.size() > (1u << 30)) {
         set_error("world too large"); return BP_ERR_CAPACITY_OVERFLOW;
     }

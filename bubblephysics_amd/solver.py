@@ -1,0 +1,219 @@
+"""Solver-level host interface: the reference's NativeMetalFrameExecuting seam over the C ABI.
+
+Mirrors NativeMetalSolver.executeFrame(input:scenarioStep:) -> NativeSolverFrame
+(Sources/BubblePhysicsNativeMetal/NativeSolverInput.swift:255-264): the solver keeps the world on
+the device, publishes a step only when it succeeds, and reports failures with the reference's
+error cases (NativeMetalWorldError) and failure detail (NativeMetalFailureDetail).
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import abi
+
+
+@dataclass
+class FailureDetail:
+    object_ids: tuple
+    violation_kind: int
+    edge_segment_id: int | None
+    distance: float | None
+    limiting_term: int | None
+
+
+@dataclass
+class SolverFrame:
+    """NativeSolverFrame (NativeSolverInput.swift:224-253) for the fields the step produces."""
+    status: int
+    bubbles: np.ndarray
+    substep_count: int
+    grid_rebuild_count: int
+    candidate_pair_count: int
+    contact_candidate_overflow_count: int
+    maximum_contact_candidates_per_object: int
+    group_width: int
+    gpu_milliseconds: float
+    region_removals: set
+    region_wraps: set
+    limiter_object: int
+    limiter_term: int
+    detail: FailureDetail | None
+
+    @property
+    def ok(self) -> bool:
+        return self.status == abi.BP_OK
+
+
+def _detail(info: abi.FrameInfo) -> FailureDetail | None:
+    if info.status not in (abi.BP_ERR_CONTACT_VIOLATION, abi.BP_ERR_SUBSTEP_LIMIT_EXCEEDED):
+        return None
+    d = info.detail
+    dist = None if np.isnan(d.distance) else float(d.distance)
+    return FailureDetail(tuple(int(x) for x in d.object_ids), int(d.violation_kind),
+                         None if d.edge_segment_id_low < 0 else int(d.edge_segment_id_low), dist,
+                         None if d.limiting_term == abi.BP_TERM_NONE else int(d.limiting_term))
+
+
+class NativeWorld:
+    """One world on the GPU (bp_world). Raises BubblePhysicsError for call errors; step results
+    come back as SolverFrame with the status of the step."""
+
+    def __init__(self, scene=None, config: abi.WorldConfig | None = None):
+        self._lib = abi.load_library()
+        self._handle = ctypes.c_void_p()
+        cfg = config if config is not None else (scene.config if scene is not None else abi.default_config())
+        abi.check(self._lib.bp_world_create(ctypes.byref(cfg), ctypes.byref(self._handle)), "bp_world_create")
+        self._count = 0
+        self._bodies = 0
+        if scene is not None:
+            self.load(scene)
+
+    def close(self):
+        if self._handle:
+            self._lib.bp_world_destroy(self._handle)
+            self._handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def load(self, scene):
+        self.set_config(scene.config)
+        self.set_bubbles(scene.bubbles)
+        self.set_segments(scene.segments)
+        self.set_bodies(scene.bodies, scene.vertices, scene.triangles)
+        self.set_forces(scene.motors, scene.bands)
+
+    def set_config(self, cfg: abi.WorldConfig):
+        abi.check(self._lib.bp_world_set_config(self._handle, ctypes.byref(cfg)), "bp_world_set_config")
+
+    def set_bubbles(self, bubbles: np.ndarray):
+        b = np.ascontiguousarray(bubbles, abi.BUBBLE_DTYPE)
+        abi.check(self._lib.bp_world_set_bubbles(self._handle, abi.ptr(b), len(b)), "bp_world_set_bubbles")
+        self._count = len(b)
+
+    def update_bubbles(self, indices, bubbles: np.ndarray):
+        idx = np.ascontiguousarray(indices, np.uint32)
+        b = np.ascontiguousarray(bubbles, abi.BUBBLE_DTYPE)
+        abi.check(self._lib.bp_world_update_bubbles(self._handle, abi.ptr(idx), abi.ptr(b), len(b)), "bp_world_update_bubbles")
+
+    def set_segments(self, segments: np.ndarray):
+        s = np.ascontiguousarray(segments, abi.SEGMENT_DTYPE)
+        abi.check(self._lib.bp_world_set_segments(self._handle, abi.ptr(s), len(s)), "bp_world_set_segments")
+
+    def set_bodies(self, bodies: np.ndarray, vertices: np.ndarray, triangles: np.ndarray):
+        b = np.ascontiguousarray(bodies, abi.BODY_DTYPE)
+        v = np.ascontiguousarray(vertices, np.float32).reshape(-1, 2)
+        t = np.ascontiguousarray(triangles, np.uint32).reshape(-1, 4)
+        abi.check(self._lib.bp_world_set_bodies(self._handle, abi.ptr(b), len(b), abi.ptr(v), len(v), abi.ptr(t), len(t)),
+                  "bp_world_set_bodies")
+        self._bodies = len(b)
+
+    def set_forces(self, motors: np.ndarray, bands: np.ndarray):
+        m = np.ascontiguousarray(motors, abi.MOTOR_DTYPE)
+        b = np.ascontiguousarray(bands, abi.BAND_DTYPE)
+        abi.check(self._lib.bp_world_set_forces(self._handle, abi.ptr(m), len(m), abi.ptr(b), len(b)), "bp_world_set_forces")
+
+    def step(self, scenario_step: int = 0) -> SolverFrame:
+        info = abi.FrameInfo()
+        status = self._lib.bp_world_step(self._handle, scenario_step, ctypes.byref(info))
+        if status in (abi.BP_ERR_INVALID_ARGUMENT, abi.BP_ERR_ALLOCATION_FAILED, abi.BP_ERR_RESOURCE_UNAVAILABLE):
+            abi.check(status, "bp_world_step")
+        return self._frame(info)
+
+    def _frame(self, info: abi.FrameInfo) -> SolverFrame:
+        bubbles = self.read_bubbles()
+        return SolverFrame(
+            status=int(info.status), bubbles=bubbles, substep_count=int(info.substep_count),
+            grid_rebuild_count=int(info.grid_rebuild_count), candidate_pair_count=int(info.candidate_pair_count),
+            contact_candidate_overflow_count=int(info.candidate_overflow),
+            maximum_contact_candidates_per_object=int(info.max_candidates_per_object), group_width=int(info.group_width),
+            gpu_milliseconds=float(info.gpu_milliseconds), region_removals=set(), region_wraps=set(),
+            limiter_object=int(info.limiter_object), limiter_term=int(info.limiter_term), detail=_detail(info))
+
+    def read_bubbles(self) -> np.ndarray:
+        out = np.zeros(self._count, abi.BUBBLE_DTYPE)
+        abi.check(self._lib.bp_world_read_bubbles(self._handle, abi.ptr(out), self._count), "bp_world_read_bubbles")
+        return out
+
+    def read_bodies(self) -> np.ndarray:
+        out = np.zeros(self._bodies, abi.BODY_DTYPE)
+        abi.check(self._lib.bp_world_read_bodies(self._handle, abi.ptr(out), self._bodies), "bp_world_read_bodies")
+        return out
+
+    def read_adjacency(self):
+        """(row offsets, entries packed as kind << 30 | neighbour) of the last grid build."""
+        offsets = np.zeros(self._count + 1, np.uint32)
+        total = ctypes.c_uint32(0)
+        abi.check(self._lib.bp_world_read_adjacency(self._handle, abi.ptr(offsets), len(offsets), None, 0, ctypes.byref(total)),
+                  "bp_world_read_adjacency")
+        entries = np.zeros(max(total.value, 1), np.uint32)
+        abi.check(self._lib.bp_world_read_adjacency(self._handle, abi.ptr(offsets), len(offsets), abi.ptr(entries), len(entries),
+                                                    ctypes.byref(total)), "bp_world_read_adjacency")
+        return offsets, entries[: total.value]
+
+    def read_step_history(self) -> np.ndarray:
+        out = np.zeros(1024, np.float32)
+        count = ctypes.c_uint32(0)
+        abi.check(self._lib.bp_world_read_step_history(self._handle, abi.ptr(out), 1024, ctypes.byref(count)),
+                  "bp_world_read_step_history")
+        return out[: count.value]
+
+    def fail_next_command(self):
+        abi.check(self._lib.bp_world_debug_fail_next_command(self._handle), "bp_world_debug_fail_next_command")
+
+
+class Batch:
+    """Many independent worlds advanced together (bp_batch)."""
+
+    def __init__(self, scenes=()):
+        self._lib = abi.load_library()
+        self._handle = ctypes.c_void_p()
+        abi.check(self._lib.bp_batch_create(ctypes.byref(self._handle)), "bp_batch_create")
+        self._counts = []
+        for s in scenes:
+            self.add(s)
+
+    def add(self, scene) -> int:
+        index = ctypes.c_uint32(0)
+        d = scene.desc()
+        abi.check(self._lib.bp_batch_add_world(self._handle, ctypes.byref(d), ctypes.byref(index)), "bp_batch_add_world")
+        self._counts.append(len(scene.bubbles))
+        return index.value
+
+    def step(self, steps: int = 1) -> np.ndarray:
+        """Runs `steps` steps; returns FrameInfo records shaped (steps, worlds)."""
+        infos = np.zeros(steps * len(self._counts), abi.FRAME_DTYPE)
+        abi.check(self._lib.bp_batch_step(self._handle, steps, abi.ptr(infos)), "bp_batch_step")
+        return infos.reshape(steps, len(self._counts))
+
+    def read_bubbles(self, world: int) -> np.ndarray:
+        out = np.zeros(self._counts[world], abi.BUBBLE_DTYPE)
+        abi.check(self._lib.bp_batch_read_bubbles(self._handle, world, abi.ptr(out), len(out)), "bp_batch_read_bubbles")
+        return out
+
+    @property
+    def last_gpu_milliseconds(self) -> float:
+        return float(self._lib.bp_batch_last_gpu_milliseconds(self._handle))
+
+    def close(self):
+        if self._handle:
+            self._lib.bp_batch_destroy(self._handle)
+            self._handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1238,6 +1238,39 @@ int bpo_world_run(const bp_world_config *config, bp_bubble *bubbles, uint32_t n,
     return run(w);
 }
 
+// Candidate rows of one grid build with a given cell width and margins (the kernel
+// nativeEmitCandidates, Geometry:300-340, and its hashed twin, Force:1010-1035).
+// rows[n * limit * 2] receives (kind, other index) per slot, row_counts[2n] the bubble and
+// segment counts, overflow[2] the dropped total and the widest demand.
+void bpo_candidate_rows(const bp_bubble *bubbles, const float *margins, uint32_t n, const bp_segment *segments, uint32_t ns,
+                        float cell_width, float separation_tolerance, uint32_t limit, uint32_t hashed, uint32_t buckets,
+                        uint32_t *rows, uint32_t *row_counts, uint32_t *overflow) {
+    bpo::World<float> w;
+    bp_world_config cfg;
+    std::memset(&cfg, 0, sizeof cfg);
+    cfg.time_step = 1.0f / 60.0f;
+    cfg.separation_tolerance = separation_tolerance;
+    cfg.candidate_limit = limit;
+    cfg.grid_bucket_count = buckets;
+    bpo::load(w, &cfg, bubbles, n, segments, ns, nullptr, 0, nullptr, 0, nullptr, 0, nullptr, 0, nullptr, 0);
+    w.limit = limit;
+    w.cfg.hashedGrid = hashed;
+    w.margin.assign(margins, margins + n);
+    w.cellWidth = cell_width;
+    std::vector<std::vector<bpo::Candidate>> r(n);
+    std::vector<uint32_t> counts(2 * n, 0);
+    if (hashed) w.emit_candidates_hashed(r, counts);
+    else w.emit_candidates_sorted(r, counts);
+    for (uint32_t i = 0; i < n; ++i)
+        for (uint32_t k = 0; k < r[i].size(); ++k) {
+            rows[2 * ((size_t)i * limit + k)] = r[i][k].kind;
+            rows[2 * ((size_t)i * limit + k) + 1] = r[i][k].ib;
+        }
+    std::memcpy(row_counts, counts.data(), counts.size() * sizeof(uint32_t));
+    overflow[0] = w.overflow[0];
+    overflow[1] = w.overflow[1];
+}
+
 // Contact-law entry points (the harness kernels testEpsilon / testWallLaw / testPairLaw of
 // Tests/BubblePhysicsNativeMetalTests/NativeMetalContactLawTests.swift:15-44).
 void bpo_epsilon_f32(const float *q, float *out, uint32_t n) {

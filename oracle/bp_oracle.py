@@ -130,3 +130,22 @@ def detmath(which: str, x):
     out = np.empty_like(x)
     load().bpo_detmath_f32(code, _p(x), _p(out), len(x))
     return out
+
+
+def candidate_rows(bubbles, margins, segments, cell_width, separation_tolerance, limit, *, hashed=False, buckets=2048):
+    """Candidate rows of one grid build: (rows[n, limit, 2] of (kind, other), counts[n, 2], overflow[2])."""
+    lib = load()
+    if not hasattr(lib, "_rows_ready"):
+        P, U32, F = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_float
+        lib.bpo_candidate_rows.argtypes = [P, P, U32, P, U32, F, F, U32, U32, U32, P, P, P]
+        lib._rows_ready = True
+    n = len(bubbles)
+    b = np.ascontiguousarray(bubbles)
+    m = np.ascontiguousarray(margins, np.float32)
+    s = np.ascontiguousarray(segments)
+    rows = np.full((max(n, 1), limit, 2), 0xFFFFFFFF, np.uint32)
+    counts = np.zeros((max(n, 1), 2), np.uint32)
+    overflow = np.zeros(2, np.uint32)
+    lib.bpo_candidate_rows(_p(b), _p(m), n, _p(s), len(s), cell_width, separation_tolerance, limit, 1 if hashed else 0,
+                           buckets, _p(rows), _p(counts), _p(overflow))
+    return rows[:n], counts[:n], overflow
