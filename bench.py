@@ -1,0 +1,182 @@
+"""Benchmark: bubble-substeps/sec of the world step on one MI355X.
+
+Workload (default "batched"): 4,096 independent worlds with bubble counts log-uniform in
+[16, 4096] (about 3.0M bubbles), radii U[6, 20], 0-16 kinematic bodies each, walls, advanced
+together: one step advances every world by one frame dt = 1/60. The plain single-world step
+(1,024 bubbles + 64 kinematic bodies) is timed beside it, and the CPU oracle on the host cores
+gives the reported baseline.
+
+One JSON line on stdout. value = bubble-substeps/s = sum over worlds of (bubbles x substeps) / wall
+time of the timed steps, with every world resident on the GPU before timing starts.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBPS = 8000.0
+CLASS_WIDTH = (64, 256, 512, 1024)
+
+
+def bytes_per_bubble_substep(mean_degree: float) -> float:
+    """Compulsory HBM bytes per bubble-substep: 168 + 8 * mean adjacency-row length (positions,
+    velocities, accelerations, radius, stiffness, inverse mass, flags, margin, x0 read and written
+    across the kick-drift, force and reduction passes, plus the adjacency row read twice)."""
+    return 168.0 + 8.0 * mean_degree
+
+
+def percentile(values, q):
+    return float(np.percentile(np.asarray(values, np.float64), q)) if len(values) else float("nan")
+
+
+def build_batched(worlds: int):
+    from bubblephysics_amd import scenes
+    return scenes.config4(worlds=worlds)
+
+
+def run_batched(args, scenes_list):
+    from bubblephysics_amd import solver
+    batch = solver.Batch(scenes_list)
+    sizes = np.array([len(s.bubbles) for s in scenes_list], np.int64)
+    for _ in range(args.warmup):
+        batch.step(1)
+    latencies, substep_bubbles = [], 0
+    class_ms = np.zeros(4)
+    class_bytes = np.zeros(4)
+    statuses = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s0 = time.perf_counter()
+        infos = batch.step(1)[0]
+        latencies.append((time.perf_counter() - s0) * 1e3)
+        ok = infos["status"] == 0
+        statuses.append(int((~ok).sum()))
+        substeps = infos["substep_count"].astype(np.int64)
+        substep_bubbles += int(np.sum(sizes * substeps))
+        ms, _ = batch.last_class_timing()
+        class_ms += ms
+        deg = infos["adjacency_entries"].astype(np.float64) / np.maximum(sizes, 1)
+        per = sizes * substeps * (168.0 + 8.0 * deg)
+        for k, width in enumerate(CLASS_WIDTH):
+            class_bytes[k] += float(per[infos["group_width"] == width].sum())
+    elapsed = time.perf_counter() - t0
+    return {
+        "elapsed": elapsed, "substep_bubbles": substep_bubbles, "latencies": latencies, "class_ms": class_ms / args.steps,
+        "class_bytes": class_bytes / args.steps, "failed_worlds": max(statuses) if statuses else 0,
+        "bubbles": int(sizes.sum()), "worlds": len(sizes),
+    }
+
+
+def run_plain(steps: int):
+    from bubblephysics_amd import scenes, solver
+    scene = scenes.config2()
+    world = solver.NativeWorld(scene)
+    n = len(scene.bubbles)
+    for k in range(3):
+        world.step(k)
+    lat, total = [], 0
+    t0 = time.perf_counter()
+    for k in range(steps):
+        s0 = time.perf_counter()
+        f = world.step(k)
+        lat.append((time.perf_counter() - s0) * 1e3)
+        total += n * f.substep_count
+    elapsed = time.perf_counter() - t0
+    return {"value": total / elapsed, "p50_ms": percentile(lat, 50), "p99_ms": percentile(lat, 99), "bubbles": n}
+
+
+def cpu_baseline(scenes_list, budget_s: float):
+    """The oracle (FP32, one thread) on the first worlds of the same batch, one step each,
+    until the time budget is used."""
+    from oracle import bp_oracle
+    total, used, worlds = 0, 0.0, 0
+    for s in scenes_list:
+        t0 = time.perf_counter()
+        _, _, frames, _ = bp_oracle.run(s, 1, assert_mode=1)
+        used += time.perf_counter() - t0
+        total += len(s.bubbles) * int(frames[0]["substeps"])
+        worlds += 1
+        if used >= budget_s:
+            break
+    return {"value": total / used if used > 0 else float("nan"), "worlds": worlds, "seconds": used}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--worlds", type=int, default=4096)
+    ap.add_argument("--plain-steps", type=int, default=60)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: the worlds are independent and this build measures one MI355X "
+              f"(--gpus {args.gpus} is not supported)", file=sys.stderr)
+        sys.exit(2)
+
+    t_build = time.perf_counter()
+    scenes_list = build_batched(args.worlds)
+    print(f"[bench] built {len(scenes_list)} worlds in {time.perf_counter() - t_build:.1f}s", file=sys.stderr, flush=True)
+    res = run_batched(args, scenes_list)
+    print(f"[bench] batched done: {res['elapsed']:.2f}s for {args.steps} steps", file=sys.stderr, flush=True)
+    plain = run_plain(args.plain_steps)
+    print(f"[bench] plain done: {plain['value']:.3g} bubble-substeps/s", file=sys.stderr, flush=True)
+    cpu = None if args.no_cpu else cpu_baseline(scenes_list, args.cpu_seconds)
+
+    value = res["substep_bubbles"] / res["elapsed"]
+    k = int(np.argmax(res["class_ms"]))
+    dur_s = res["class_ms"][k] / 1e3
+    achieved = res["class_bytes"][k] / dur_s / 1e9 if dur_s > 0 else 0.0
+    line = {
+        "metric": "bubble-substeps/sec on 1 MI355X (batched/served vs plain) + p50/p99 step latency",
+        "value": value,
+        "unit": "bubble-substeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": res["elapsed"] / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"batched: {res['worlds']} worlds, bubbles log-uniform in [16, 4096] ({res['bubbles']} total), "
+                        f"radii U[6,20], 0-16 kinematic bodies per world, walls, dt 1/60",
+            "worlds": res["worlds"], "bubbles": res["bubbles"], "parallelism": "one workgroup per world",
+        },
+        "p50_step_ms": percentile(res["latencies"], 50),
+        "p99_step_ms": percentile(res["latencies"], 99),
+        "failed_worlds": res["failed_worlds"],
+        "plain": {"workload": "plain: 1 world, 1024 bubbles + 64 kinematic bodies", "value": plain["value"],
+                  "p50_step_ms": plain["p50_ms"], "p99_step_ms": plain["p99_ms"]},
+        "batched_over_plain": value / plain["value"] if plain["value"] > 0 else None,
+        "roofline": {
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
+            "kernel": f"world_step_kernel<{CLASS_WIDTH[k]}>",
+            "kernel_ms_per_launch": res["class_ms"][k],
+        },
+        "cpu_baseline": None if cpu is None else {
+            "value": cpu["value"], "unit": "bubble-substeps/s", "cores": 1, "kind": "port",
+            "sample": f"oracle FP32, one thread, one step of the first {cpu['worlds']} worlds of the same batch "
+                      f"({cpu['seconds']:.1f}s)",
+        },
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -113,7 +113,7 @@ class FrameInfo(ctypes.Structure):
         ("max_candidates_per_object", ctypes.c_uint32), ("execution_path", ctypes.c_uint32),
         ("group_width", ctypes.c_uint32), ("region_removals", ctypes.c_uint32), ("region_wraps", ctypes.c_uint32),
         ("gpu_milliseconds", ctypes.c_float), ("simulated_time", ctypes.c_float), ("limiter_object", ctypes.c_uint64),
-        ("limiter_term", ctypes.c_uint32), ("reserved", ctypes.c_uint32), ("detail", FailureDetail),
+        ("limiter_term", ctypes.c_uint32), ("adjacency_entries", ctypes.c_uint32), ("detail", FailureDetail),
     ]
 
 
@@ -143,6 +143,7 @@ _P = ctypes.c_void_p
 _U32, _I32, _I64, _U64, _F = ctypes.c_uint32, ctypes.c_int32, ctypes.c_int64, ctypes.c_uint64, ctypes.c_float
 SIGNATURES = {
     "bp_abi_version": ([], _U32),
+    "bp_abi_record_sizes": ([_P], None),
     "bp_world_config_default": ([_P], None),
     "bp_status_string": ([_I32], ctypes.c_char_p),
     "bp_last_error": ([], ctypes.c_char_p),
@@ -171,6 +172,7 @@ SIGNATURES = {
     "bp_batch_step": ([_P, _U32, _P], _I32),
     "bp_batch_read_bubbles": ([_P, _U32, _P, _U32], _I32),
     "bp_batch_last_gpu_milliseconds": ([_P], _F),
+    "bp_batch_last_class_timing": ([_P, _P, _P], _I32),
 }
 
 

@@ -21,7 +21,7 @@
 #include "../../include/bubblephysics.h"
 #include "bp_device.h"
 
-extern "C" hipError_t bp_launch_world_step(int block, const bpd::Arena *arena, bpd::WorldDesc *descs, bpd::WorldStatus *stats,
+extern "C" __attribute__((visibility("hidden"))) hipError_t bp_launch_world_step(int block, const bpd::Arena *arena, bpd::WorldDesc *descs, bpd::WorldStatus *stats,
                                            const uint32_t *order, uint32_t count, hipStream_t stream);
 
 namespace {
@@ -90,6 +90,7 @@ struct bp_batch {
     size_t statsCapacity = 0;          // in WorldStatus records
     std::vector<bpd::WorldStatus> hostStats;
     float lastGpuMs = 0.0f;
+    float lastClassMs[4] = {0, 0, 0, 0};
     bool inFlight = false;
     uint32_t totalBubbles = 0;
     std::mutex mutex;
@@ -195,7 +196,7 @@ bp_status bp_batch::pack() {
     BP_ALLOC(A.motor, nb); BP_ALLOC(A.band, nb);
     BP_ALLOC(A.acc, nb); BP_ALLOC(A.a0, nb); BP_ALLOC(A.ref, nb); BP_ALLOC(A.margin, nb);
     BP_ALLOC(A.rowStart, nb); BP_ALLOC(A.rowLen, nb); BP_ALLOC(A.grid, nb);
-    BP_ALLOC(A.omega2, 1); BP_ALLOC(A.bound, 1);
+    BP_ALLOC(A.omega2, 1); BP_ALLOC(A.bound, 1); BP_ALLOC(A.bodyHits, nbod);
     BP_ALLOC(A.adj, adj);
     BP_ALLOC(A.segAB, ns); BP_ALLOC(A.segRegion, ns); BP_ALLOC(A.segFlags, ns); BP_ALLOC(A.segCat, ns);
     BP_ALLOC(A.segMask, ns); BP_ALLOC(A.segId, ns); BP_ALLOC(A.segOwner, ns);
@@ -399,6 +400,7 @@ static void fill_info(const bpd::WorldStatus &s, bp_frame_info *out) {
     out->simulated_time = s.time[1];
     out->limiter_object = s.limiterObject;
     out->limiter_term = s.limiterTerm;
+    out->adjacency_entries = s.adjacencyEntries;
     out->detail.object_ids[0] = s.violationObjects[0];
     out->detail.object_ids[1] = s.violationObjects[1];
     out->detail.violation_kind = (int32_t)s.violationKind;
@@ -442,6 +444,7 @@ bp_status bp_batch::step(uint32_t steps, bp_frame_info *infos) {
     for (int k = 0; k < 4; ++k) {
         float e = 0.0f;
         BP_HIP(hipEventElapsedTime(&e, startEvent, endEvents[k]));
+        lastClassMs[k] = classWorlds[k].empty() ? 0.0f : e;
         ms = std::max(ms, e);
     }
     lastGpuMs = ms;
@@ -462,6 +465,13 @@ bp_status bp_batch::step(uint32_t steps, bp_frame_info *infos) {
 extern "C" {
 
 uint32_t bp_abi_version(void) { return BP_ABI_VERSION; }
+
+void bp_abi_record_sizes(uint32_t *out) {
+    if (!out) return;
+    out[0] = sizeof(bp_bubble); out[1] = sizeof(bp_segment); out[2] = sizeof(bp_body); out[3] = sizeof(bp_motor);
+    out[4] = sizeof(bp_band); out[5] = sizeof(bp_world_config); out[6] = sizeof(bp_frame_info);
+    out[7] = sizeof(bp_batch_world_desc);
+}
 
 void bp_world_config_default(bp_world_config *c) {
     if (!c) return;
@@ -575,6 +585,15 @@ bp_status bp_batch_read_bubbles(bp_batch *b, uint32_t world, bp_bubble *out, uin
 }
 
 float bp_batch_last_gpu_milliseconds(const bp_batch *b) { return b ? b->lastGpuMs : 0.0f; }
+
+bp_status bp_batch_last_class_timing(const bp_batch *b, float *ms4, uint32_t *worlds4) {
+    if (!b) return BP_ERR_INVALID_ARGUMENT;
+    for (int k = 0; k < 4; ++k) {
+        if (ms4) ms4[k] = b->lastClassMs[k];
+        if (worlds4) worlds4[k] = (uint32_t)b->classWorlds[k].size();
+    }
+    return BP_OK;
+}
 
 }  // extern "C"
 

@@ -43,6 +43,7 @@ struct Arena {
     float *margin;
     uint32_t *rowStart, *rowLen, *grid;
     float *omega2, *bound;    // per-bubble reduction inputs (debug / cooperative path)
+    uint32_t *bodyHits;       // per body: bubbles in its candidate row at the last grid build
     // Adjacency entries: (kind << 30) | neighbour, CSR per world.
     uint32_t *adj;
     // Segments.

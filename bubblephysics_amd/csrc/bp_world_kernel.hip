@@ -316,6 +316,7 @@ BPD bool build_grid(World &W, Shared<BLOCK> &sh, float remaining, uint32_t limit
     for (uint32_t i = threadIdx.x; i < n; i += BLOCK) width = fmaxf(width, 2.0f * (W.radius[i] + W.margin[i]) + W.sepTol);
     width = block_maxf<BLOCK>(sh, width);
     W.cellWidth = fmaxf(width, kFloatMin);
+    for (uint32_t k = threadIdx.x; k < W.nb; k += BLOCK) W.A.bodyHits[d.bodyOffset + k] = 0;
     // Hashed grid: count, exclusive prefix, scatter (Force:1066-1105).
     uint32_t buckets = d.buckets;
     for (uint32_t b = threadIdx.x; b < buckets; b += BLOCK) sh.buckets[b] = 0;
@@ -348,15 +349,27 @@ BPD bool build_grid(World &W, Shared<BLOCK> &sh, float remaining, uint32_t limit
     // Pass 1: row lengths and the reference's overflow counters.
     const uint32_t *ends = sh.buckets;
     uint32_t droppedLocal = 0, maxLocal = 0, pairsLocal = 0, overflowLocal = 0;
+    uint32_t *bodyHits = W.A.bodyHits + d.bodyOffset;
     for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
         uint32_t count = 0, forward = 0;
-        if (finite2(W.pos[i])) enumerate_row(W, ends, buckets, i, [&](uint32_t) { ++count; }, forward);
-        else enumerate_bodies_only(W, i, [&](uint32_t) { ++count; });
+        auto tally = [&](uint32_t e) {
+            ++count;
+            if ((e >> kKindShift) == 2u) atomicAdd(&bodyHits[e & kIndexMask], 1u);
+        };
+        if (finite2(W.pos[i])) enumerate_row(W, ends, buckets, i, tally, forward);
+        else enumerate_bodies_only(W, i, tally);
         W.rowLen[i] = count;
         if (forward > limit) { droppedLocal += forward - limit; overflowLocal = 1; }
         maxLocal = max(maxLocal, max(forward, count));
         if (count > limit) { droppedLocal += count - limit; overflowLocal = 1; }
         pairsLocal += min(forward, limit);
+    }
+    __syncthreads();
+    // Body rows of the reference (Geometry:410-414): their length enters the same counters.
+    for (uint32_t k = threadIdx.x; k < W.nb; k += BLOCK) {
+        uint32_t hits = bodyHits[k];
+        maxLocal = max(maxLocal, hits);
+        if (hits > limit) { droppedLocal += hits - limit; overflowLocal = 1; }
     }
     atomicAdd(&sh.counters[0], droppedLocal);
     atomicMax(&sh.counters[1], maxLocal);
@@ -987,7 +1000,7 @@ __global__ __launch_bounds__(BLOCK) void world_step_kernel(Arena A, WorldDesc *d
 }  // namespace bpd
 
 // Host launchers (used by bp_api.cpp).
-extern "C" hipError_t bp_launch_world_step(int block, const bpd::Arena *arena, bpd::WorldDesc *descs, bpd::WorldStatus *stats,
+extern "C" __attribute__((visibility("hidden"))) hipError_t bp_launch_world_step(int block, const bpd::Arena *arena, bpd::WorldDesc *descs, bpd::WorldStatus *stats,
                                            const uint32_t *order, uint32_t count, hipStream_t stream) {
     if (count == 0) return hipSuccess;
     switch (block) {

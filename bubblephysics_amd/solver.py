@@ -203,6 +203,13 @@ class Batch:
         abi.check(self._lib.bp_batch_read_bubbles(self._handle, world, abi.ptr(out), len(out)), "bp_batch_read_bubbles")
         return out
 
+    def last_class_timing(self):
+        """(device milliseconds, world count) per workgroup-width class (64, 256, 512, 1024)."""
+        ms = np.zeros(4, np.float32)
+        worlds = np.zeros(4, np.uint32)
+        abi.check(self._lib.bp_batch_last_class_timing(self._handle, abi.ptr(ms), abi.ptr(worlds)), "bp_batch_last_class_timing")
+        return ms, worlds
+
     @property
     def last_gpu_milliseconds(self) -> float:
         return float(self._lib.bp_batch_last_gpu_milliseconds(self._handle))

@@ -217,7 +217,7 @@ typedef struct bp_frame_info {
     float simulated_time;            /* substepControl.time.y at the end                */
     uint64_t limiter_object;         /* substepControl.limiter.x                        */
     uint32_t limiter_term;           /* substepControl.limiter.y                        */
-    uint32_t reserved;
+    uint32_t adjacency_entries;      /* entries of the adjacency of the last grid build */
     bp_failure_detail detail;
 } bp_frame_info;
 
@@ -225,6 +225,9 @@ typedef struct bp_frame_info {
 /* Library.                                                                   */
 /* ------------------------------------------------------------------------ */
 uint32_t bp_abi_version(void);
+/* sizeof of bp_bubble, bp_segment, bp_body, bp_motor, bp_band, bp_world_config, bp_frame_info,
+ * bp_batch_world_desc, in that order (the role of NativeMetalShaderABITests.swift:61-381). */
+void bp_abi_record_sizes(uint32_t *out8);
 /* Fills a configuration with the reference defaults (NativeSolverConfiguration.init,
  * NativeSolverInput.swift:128-143; NativeMetalConfiguration, NativeMetalBackend.swift:2-34). */
 void bp_world_config_default(bp_world_config *config);
@@ -317,6 +320,9 @@ bp_status bp_batch_step(bp_batch *batch, uint32_t steps, bp_frame_info *infos);
 bp_status bp_batch_read_bubbles(bp_batch *batch, uint32_t world, bp_bubble *out, uint32_t count);
 /* Device time of the last bp_batch_step (HIP events around the launches). */
 float bp_batch_last_gpu_milliseconds(const bp_batch *batch);
+/* Per workgroup-width class (64, 256, 512, 1024 threads) of the last bp_batch_step: device time
+ * of that class's launches (HIP events on its stream) and the number of worlds in it. */
+bp_status bp_batch_last_class_timing(const bp_batch *batch, float *milliseconds4, uint32_t *worlds4);
 
 #ifdef __cplusplus
 }

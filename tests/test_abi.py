@@ -1,0 +1,52 @@
+"""The C ABI library loads on the CPU, exports every function include/*.h declares, and its record
+layouts agree with the Python mirror (the role of NativeMetalShaderABITests.swift:61-381).
+No compute call is made here: that needs a GPU."""
+import ctypes
+import re
+from pathlib import Path
+
+import numpy as np
+
+from bubblephysics_amd import abi
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def declared_functions():
+    names = set()
+    for header in (ROOT / "include").glob("*.h"):
+        text = re.sub(r"/\*.*?\*/", "", header.read_text(), flags=re.S)
+        for m in re.finditer(r"^[A-Za-z_][\w\s\*]*?\b(bp_\w+)\s*\(", text, flags=re.M):
+            names.add(m.group(1))
+    return names
+
+
+def test_library_exports_every_declared_function(hip_library):
+    names = declared_functions()
+    assert len(names) >= 30
+    missing = [n for n in sorted(names) if not hasattr(hip_library, n)]
+    assert not missing, missing
+    assert set(abi.SIGNATURES) == names, set(abi.SIGNATURES) ^ names
+
+
+def test_record_layouts_match_the_header(hip_library):
+    sizes = (ctypes.c_uint32 * 8)()
+    hip_library.bp_abi_record_sizes(sizes)
+    expected = [ctypes.sizeof(t) for t in (abi.Bubble, abi.Segment, abi.Body, abi.Motor, abi.Band, abi.WorldConfig,
+                                          abi.FrameInfo, abi.BatchWorldDesc)]
+    assert list(sizes) == expected
+    assert abi.BUBBLE_DTYPE.itemsize == 72 and abi.SEGMENT_DTYPE.itemsize == 64 and abi.BODY_DTYPE.itemsize == 96
+
+
+def test_defaults_are_the_reference_defaults(hip_library):
+    cfg = abi.default_config()
+    # NativeSolverConfiguration.init (NativeSolverInput.swift:128-143), NativeMetalConfiguration defaults.
+    assert cfg.time_step == np.float32(1 / 60)
+    assert cfg.contact_tolerance == np.float32(0.001) and cfg.separation_tolerance == np.float32(0.002)
+    assert cfg.candidate_limit == 128 and cfg.substep_split == 1
+
+
+def test_status_strings_cover_every_code(hip_library):
+    for code, name in abi.STATUS_NAMES.items():
+        assert hip_library.bp_status_string(code)
+    assert hip_library.bp_status_string(999) == b"unknown status"
