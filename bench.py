@@ -25,7 +25,6 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBPS = 8000.0
-CLASS_WIDTH = (64, 256, 512, 1024)
 
 
 def bytes_per_bubble_substep(mean_degree: float) -> float:
@@ -51,8 +50,9 @@ def run_batched(args, scenes_list):
     for _ in range(args.warmup):
         batch.step(1)
     latencies, substep_bubbles = [], 0
-    class_ms = np.zeros(4)
-    class_bytes = np.zeros(4)
+    classes = int(batch._lib.bp_batch_class_count())
+    class_ms = np.zeros(classes)
+    class_bytes = np.zeros(classes)
     statuses = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -63,14 +63,15 @@ def run_batched(args, scenes_list):
         statuses.append(int((~ok).sum()))
         substeps = infos["substep_count"].astype(np.int64)
         substep_bubbles += int(np.sum(sizes * substeps))
-        ms, _ = batch.last_class_timing()
+        ms, _, threads, capacity = batch.last_class_timing()
         class_ms += ms
         deg = infos["adjacency_entries"].astype(np.float64) / np.maximum(sizes, 1)
-        per = sizes * substeps * (168.0 + 8.0 * deg)
-        for k, width in enumerate(CLASS_WIDTH):
-            class_bytes[k] += float(per[infos["group_width"] == width].sum())
+        per = sizes * substeps * bytes_per_bubble_substep(deg)
+        for k in range(classes):
+            class_bytes[k] += float(per[infos["kernel_class"] == k].sum())
     elapsed = time.perf_counter() - t0
     return {
+        "class_threads": threads, "class_capacity": capacity,
         "elapsed": elapsed, "substep_bubbles": substep_bubbles, "latencies": latencies, "class_ms": class_ms / args.steps,
         "class_bytes": class_bytes / args.steps, "failed_worlds": max(statuses) if statuses else 0,
         "bubbles": int(sizes.sum()), "worlds": len(sizes),
@@ -166,7 +167,8 @@ def main():
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
-            "kernel": f"world_step_kernel<{CLASS_WIDTH[k]}>",
+            "kernel": (f"world_lds_kernel<{int(res['class_threads'][k])} threads, {int(res['class_capacity'][k])} bubbles>"
+                       if int(res['class_capacity'][k]) < 2**32 - 1 else "world_step_kernel<1024> (global state)"),
             "kernel_ms_per_launch": res["class_ms"][k],
         },
         "cpu_baseline": None if cpu is None else {

@@ -113,7 +113,8 @@ class FrameInfo(ctypes.Structure):
         ("max_candidates_per_object", ctypes.c_uint32), ("execution_path", ctypes.c_uint32),
         ("group_width", ctypes.c_uint32), ("region_removals", ctypes.c_uint32), ("region_wraps", ctypes.c_uint32),
         ("gpu_milliseconds", ctypes.c_float), ("simulated_time", ctypes.c_float), ("limiter_object", ctypes.c_uint64),
-        ("limiter_term", ctypes.c_uint32), ("adjacency_entries", ctypes.c_uint32), ("detail", FailureDetail),
+        ("limiter_term", ctypes.c_uint32), ("adjacency_entries", ctypes.c_uint32), ("kernel_class", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32), ("detail", FailureDetail),
     ]
 
 
@@ -172,7 +173,9 @@ SIGNATURES = {
     "bp_batch_step": ([_P, _U32, _P], _I32),
     "bp_batch_read_bubbles": ([_P, _U32, _P, _U32], _I32),
     "bp_batch_last_gpu_milliseconds": ([_P], _F),
-    "bp_batch_last_class_timing": ([_P, _P, _P], _I32),
+    "bp_batch_last_class_timing": ([_P, _P, _P, _P, _P], _I32),
+    "bp_batch_class_count": ([], _U32),
+    "bp_batch_set_world_class": ([_P, _U32, _I32], _I32),
 }
 
 

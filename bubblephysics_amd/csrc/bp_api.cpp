@@ -21,8 +21,12 @@
 #include "../../include/bubblephysics.h"
 #include "bp_device.h"
 
-extern "C" __attribute__((visibility("hidden"))) hipError_t bp_launch_world_step(int block, const bpd::Arena *arena, bpd::WorldDesc *descs, bpd::WorldStatus *stats,
+extern "C" hipError_t bp_launch_world_step(int block, const bpd::Arena *arena, bpd::WorldDesc *descs, bpd::WorldStatus *stats,
                                            const uint32_t *order, uint32_t count, hipStream_t stream);
+extern "C" hipError_t bp_launch_world_lds(int id, const bpd::Arena *arena, bpd::WorldDesc *descs, bpd::WorldStatus *stats,
+                                          const uint32_t *order, uint32_t count, hipStream_t stream);
+extern "C" int bp_lds_config_count();
+extern "C" void bp_lds_config(int id, uint32_t *threads, uint32_t *per, uint32_t *rowcap);
 
 namespace {
 
@@ -39,14 +43,36 @@ void set_error(const std::string &text) { g_lastError = text; }
         }                                                                                              \
     } while (0)
 
-constexpr uint32_t kClassBlocks[4] = {64, 256, 512, 1024};
 constexpr uint32_t kAdjPerBubble = 64;       // CSR capacity per bubble (grows on overflow)
+constexpr int kClassCount = 11;              // ten LDS configurations and the global-state kernel
+constexpr int kGlobalClass = 10;
+constexpr int kStreams = 4;
 
-uint32_t block_for(uint32_t n) {
+// Kernel configuration of a world: the LDS-resident kernel for up to 4,096 bubbles, the
+// global-state kernel above. Classes 0-9 are the LDS configurations of bp_world_lds.hip.
+int default_class_for(uint32_t n) {
     if (n <= 64) return 0;
-    if (n <= 256) return 1;
-    if (n <= 512) return 2;
-    return 3;
+    if (n <= 128) return 1;
+    if (n <= 256) return 2;
+    if (n <= 512) return 3;
+    if (n <= 1024) return 4;
+    if (n <= 2048) return 7;
+    if (n <= 4096) return 9;
+    return kGlobalClass;
+}
+
+uint32_t class_threads(int k) {
+    if (k == kGlobalClass) return 1024;
+    uint32_t t, p, r;
+    bp_lds_config(k, &t, &p, &r);
+    return t;
+}
+
+uint32_t class_capacity(int k) {
+    if (k == kGlobalClass) return 0xffffffffu;
+    uint32_t t, p, r;
+    bp_lds_config(k, &t, &p, &r);
+    return t * p;
 }
 
 uint32_t auto_buckets(uint32_t n) {
@@ -65,6 +91,8 @@ struct HostWorld {
     std::vector<bp_motor> motors;
     std::vector<bp_band> bands;
     uint32_t adjPerBubble = kAdjPerBubble;
+    int forcedClass = -1;               // kernel configuration chosen by the tuner, -1 = default
+    uint32_t exportAdjacency = 0;
 };
 
 template <class T> struct DevBuf {
@@ -80,17 +108,19 @@ struct bp_batch {
     bool deviceValid = false;          // device state holds the published state of every world
     std::vector<void *> allocations;
     bpd::Arena arena{};
+    bpd::Arena *arenaDev = nullptr;
     bpd::WorldDesc *descs = nullptr;
     std::vector<bpd::WorldDesc> hostDescs;
-    uint32_t *orders[4] = {nullptr, nullptr, nullptr, nullptr};
-    std::vector<uint32_t> classWorlds[4];
-    hipStream_t streams[4] = {nullptr, nullptr, nullptr, nullptr};
-    hipEvent_t startEvent = nullptr, endEvents[4] = {nullptr, nullptr, nullptr, nullptr};
+    uint32_t *orders[kClassCount] = {};
+    std::vector<uint32_t> classWorlds[kClassCount];
+    hipStream_t streams[kStreams] = {};
+    hipEvent_t startEvent = nullptr, endEvents[kStreams] = {};
+    hipEvent_t classStart[kClassCount] = {}, classEnd[kClassCount] = {};
     bpd::WorldStatus *stats = nullptr;
     size_t statsCapacity = 0;          // in WorldStatus records
     std::vector<bpd::WorldStatus> hostStats;
     float lastGpuMs = 0.0f;
-    float lastClassMs[4] = {0, 0, 0, 0};
+    float lastClassMs[kClassCount] = {};
     bool inFlight = false;
     uint32_t totalBubbles = 0;
     std::mutex mutex;
@@ -100,14 +130,19 @@ struct bp_batch {
     void release() {
         for (void *p : allocations) (void)hipFree(p);
         allocations.clear();
-        for (int k = 0; k < 4; ++k) {
+        for (int k = 0; k < kStreams; ++k) {
             if (streams[k]) (void)hipStreamDestroy(streams[k]);
             if (endEvents[k]) (void)hipEventDestroy(endEvents[k]);
             streams[k] = nullptr; endEvents[k] = nullptr;
         }
+        for (int k = 0; k < kClassCount; ++k) {
+            if (classStart[k]) (void)hipEventDestroy(classStart[k]);
+            if (classEnd[k]) (void)hipEventDestroy(classEnd[k]);
+            classStart[k] = classEnd[k] = nullptr;
+        }
         if (startEvent) (void)hipEventDestroy(startEvent);
         startEvent = nullptr;
-        stats = nullptr; statsCapacity = 0; descs = nullptr;
+        stats = nullptr; statsCapacity = 0; descs = nullptr; arenaDev = nullptr;
         for (auto &o : orders) o = nullptr;
         packed = false;
     }
@@ -206,7 +241,8 @@ bp_status bp_batch::pack() {
     BP_ALLOC(A.motors, nm); BP_ALLOC(A.bands, nband);
     BP_ALLOC(A.history, (size_t)worlds.size() * bpd::kFrameLimit);
     BP_ALLOC(descs, worlds.size());
-    for (int k = 0; k < 4; ++k) BP_ALLOC(orders[k], worlds.size());
+    BP_ALLOC(arenaDev, 1);
+    for (int k = 0; k < kClassCount; ++k) BP_ALLOC(orders[k], worlds.size());
 #undef BP_ALLOC
 
     // Host staging of every array.
@@ -251,6 +287,7 @@ bp_status bp_batch::pack() {
         d.injectionObject = c.injection_object;
         d.injectionPosition[0] = c.injection_position[0]; d.injectionPosition[1] = c.injection_position[1];
         d.failed = 0;
+        d.exportAdjacency = w.exportAdjacency;
         for (uint32_t i = 0; i < n; ++i) {
             const bp_bubble &b = w.bubbles[i];
             uint32_t g = ob + i;
@@ -300,7 +337,9 @@ bp_status bp_batch::pack() {
         for (size_t v = 0; v < w.vertices.size() / 2; ++v) verts[ov + v] = make_float2(w.vertices[2 * v], w.vertices[2 * v + 1]);
         for (size_t t = 0; t < w.triangles.size() / 4; ++t)
             tris[ot + t] = make_uint4(w.triangles[4 * t], w.triangles[4 * t + 1], w.triangles[4 * t + 2], w.triangles[4 * t + 3]);
-        classWorlds[block_for(n)].push_back((uint32_t)wi);
+        int cls = default_class_for(n);
+        if (w.forcedClass >= 0 && (uint32_t)n <= class_capacity(w.forcedClass)) cls = w.forcedClass;
+        classWorlds[cls].push_back((uint32_t)wi);
         ob += n; os += (uint32_t)w.segments.size(); obd += (uint32_t)w.bodies.size();
         ov += (uint32_t)(w.vertices.size() / 2); ot += (uint32_t)(w.triangles.size() / 4);
         om_ += (uint32_t)w.motors.size(); oband += (uint32_t)w.bands.size();
@@ -322,11 +361,16 @@ bp_status bp_batch::pack() {
         (s = upload(A.bodyId, bodyId)) || (s = upload(A.verts, verts)) || (s = upload(A.tris, tris)) ||
         (s = upload(A.motors, motors)) || (s = upload(A.bands, bands)) || (s = upload(descs, hostDescs)))
         return s;
-    for (int k = 0; k < 4; ++k)
+    for (int k = 0; k < kClassCount; ++k)
         if ((s = upload(orders[k], classWorlds[k])) != BP_OK) return s;
-    for (int k = 0; k < 4; ++k) {
+    BP_HIP(hipMemcpy(arenaDev, &arena, sizeof arena, hipMemcpyHostToDevice));
+    for (int k = 0; k < kStreams; ++k) {
         BP_HIP(hipStreamCreateWithFlags(&streams[k], hipStreamNonBlocking));
         BP_HIP(hipEventCreate(&endEvents[k]));
+    }
+    for (int k = 0; k < kClassCount; ++k) {
+        BP_HIP(hipEventCreate(&classStart[k]));
+        BP_HIP(hipEventCreate(&classEnd[k]));
     }
     BP_HIP(hipEventCreate(&startEvent));
     packed = true;
@@ -430,29 +474,45 @@ bp_status bp_batch::step(uint32_t steps, bp_frame_info *infos) {
     BP_HIP(hipMemset(stats, 0, need * sizeof(bpd::WorldStatus)));
     BP_HIP(hipDeviceSynchronize());
     BP_HIP(hipEventRecord(startEvent, streams[0]));
-    for (int k = 1; k < 4; ++k) BP_HIP(hipStreamWaitEvent(streams[k], startEvent, 0));
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 1; k < kStreams; ++k) BP_HIP(hipStreamWaitEvent(streams[k], startEvent, 0));
+    // Largest classes first so that their long-running workgroups start early on their stream.
+    for (int k = kClassCount - 1; k >= 0; --k) {
         uint32_t count = (uint32_t)classWorlds[k].size();
-        for (uint32_t st = 0; st < steps && count; ++st) {
-            hipError_t e = bp_launch_world_step((int)kClassBlocks[k], &arena, descs, stats + (size_t)st * W, orders[k], count, streams[k]);
+        if (!count) continue;
+        hipStream_t stream = streams[k % kStreams];
+        BP_HIP(hipEventRecord(classStart[k], stream));
+        for (uint32_t st = 0; st < steps; ++st) {
+            hipError_t e = k == kGlobalClass
+                ? bp_launch_world_step(1024, arenaDev, descs, stats + (size_t)st * W, orders[k], count, stream)
+                : bp_launch_world_lds(k, arenaDev, descs, stats + (size_t)st * W, orders[k], count, stream);
             if (e != hipSuccess) { set_error(std::string("kernel launch: ") + hipGetErrorString(e)); return BP_ERR_COMMAND_FAILED; }
         }
-        BP_HIP(hipEventRecord(endEvents[k], streams[k]));
+        BP_HIP(hipEventRecord(classEnd[k], stream));
     }
-    for (int k = 0; k < 4; ++k) BP_HIP(hipEventSynchronize(endEvents[k]));
+    for (int k = 0; k < kStreams; ++k) BP_HIP(hipEventRecord(endEvents[k], streams[k]));
+    for (int k = 0; k < kStreams; ++k) BP_HIP(hipEventSynchronize(endEvents[k]));
     float ms = 0.0f;
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < kStreams; ++k) {
         float e = 0.0f;
         BP_HIP(hipEventElapsedTime(&e, startEvent, endEvents[k]));
-        lastClassMs[k] = classWorlds[k].empty() ? 0.0f : e;
         ms = std::max(ms, e);
+    }
+    for (int k = 0; k < kClassCount; ++k) {
+        lastClassMs[k] = 0.0f;
+        if (classWorlds[k].empty()) continue;
+        BP_HIP(hipEventElapsedTime(&lastClassMs[k], classStart[k], classEnd[k]));
     }
     lastGpuMs = ms;
     hostStats.resize(need);
     BP_HIP(hipMemcpy(hostStats.data(), stats, need * sizeof(bpd::WorldStatus), hipMemcpyDeviceToHost));
+    std::vector<uint32_t> worldClass(W, 0);
+    for (int k = 0; k < kClassCount; ++k)
+        for (uint32_t wi : classWorlds[k]) worldClass[wi] = (uint32_t)k;
     if (infos)
         for (size_t i = 0; i < need; ++i) {
             fill_info(hostStats[i], &infos[i]);
+            infos[i].kernel_class = worldClass[i % W];
+            if (worldClass[i % W] == (uint32_t)kGlobalClass) infos[i].execution_path = BP_PATH_WORKGROUP;
             infos[i].gpu_milliseconds = ms / steps;
             if (!hostStats[i].executed) infos[i].status = BP_ERR_UNAVAILABLE;
         }
@@ -586,12 +646,27 @@ bp_status bp_batch_read_bubbles(bp_batch *b, uint32_t world, bp_bubble *out, uin
 
 float bp_batch_last_gpu_milliseconds(const bp_batch *b) { return b ? b->lastGpuMs : 0.0f; }
 
-bp_status bp_batch_last_class_timing(const bp_batch *b, float *ms4, uint32_t *worlds4) {
+bp_status bp_batch_last_class_timing(const bp_batch *b, float *ms, uint32_t *worlds, uint32_t *threads, uint32_t *capacity) {
     if (!b) return BP_ERR_INVALID_ARGUMENT;
-    for (int k = 0; k < 4; ++k) {
-        if (ms4) ms4[k] = b->lastClassMs[k];
-        if (worlds4) worlds4[k] = (uint32_t)b->classWorlds[k].size();
+    for (int k = 0; k < kClassCount; ++k) {
+        if (ms) ms[k] = b->lastClassMs[k];
+        if (worlds) worlds[k] = (uint32_t)b->classWorlds[k].size();
+        if (threads) threads[k] = class_threads(k);
+        if (capacity) capacity[k] = class_capacity(k);
     }
+    return BP_OK;
+}
+
+uint32_t bp_batch_class_count(void) { return kClassCount; }
+
+bp_status bp_batch_set_world_class(bp_batch *b, uint32_t world, int32_t cls) {
+    if (!b || world >= b->worlds.size() || cls < -1 || cls >= kClassCount) return BP_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(b->mutex);
+    bp_status s = b->download_state();
+    if (s != BP_OK) return s;
+    if (cls >= 0 && b->worlds[world].bubbles.size() > class_capacity(cls)) return BP_ERR_INVALID_ARGUMENT;
+    b->worlds[world].forcedClass = cls;
+    b->packed = false;
     return BP_OK;
 }
 
@@ -617,6 +692,7 @@ bp_status bp_world_create(const bp_world_config *config, bp_world **out) {
     bp_world *w = new bp_world();
     HostWorld hw;
     hw.config = *config;
+    hw.exportAdjacency = 1;
     w->batch.worlds.push_back(hw);
     *out = w;
     return BP_OK;

@@ -1,0 +1,385 @@
+// bp_common.h -- workgroup primitives and per-world geometry shared by the world-step kernels.
+//
+// Reductions keep the tie rule of nativeReduceSubstepInGroup (NativeForceKernels.metal-source:
+// 646-698): equal values go to the lower bubble index, so a reduction's result does not depend on
+// the workgroup width or on which lane held which bubble.
+#ifndef BP_COMMON_H
+#define BP_COMMON_H
+
+#include "bp_device.h"
+#include "../../include/bubblephysics.h"
+
+namespace bpd {
+
+struct Ctl {           // NativeSubstepControl, held in registers and uniform across the workgroup
+    float t0, t1, h, split;
+    uint32_t c0, c1, rebuild, limit;
+    uint64_t limObj;
+    uint32_t limTerm;
+};
+
+template <int BLOCK> struct Red {
+    float v[BLOCK / 64];
+    uint32_t o[BLOCK / 64];
+    float v2[BLOCK / 64];
+    uint32_t o2[BLOCK / 64];
+    uint32_t scan[BLOCK / 64];
+    uint32_t counters[8];
+};
+
+BPD void wave_max_owner(float &v, uint32_t &o) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        float ov = __shfl_xor(v, off, 64);
+        uint32_t oo = __shfl_xor(o, off, 64);
+        if (ov > v || (ov == v && oo < o)) { v = ov; o = oo; }
+    }
+}
+BPD void wave_min_owner(float &v, uint32_t &o) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        float ov = __shfl_xor(v, off, 64);
+        uint32_t oo = __shfl_xor(o, off, 64);
+        if (ov < v || (ov == v && oo < o)) { v = ov; o = oo; }
+    }
+}
+
+// Max of (a, owner) and min of (b, owner) over the workgroup, broadcast to every lane.
+template <int BLOCK>
+BPD void block_maxmin(Red<BLOCK> &r, float &a, uint32_t &ao, float &b, uint32_t &bo) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    wave_max_owner(a, ao);
+    wave_min_owner(b, bo);
+    if (BLOCK == 64) return;
+    __syncthreads();
+    if (lane == 0) { r.v[wave] = a; r.o[wave] = ao; r.v2[wave] = b; r.o2[wave] = bo; }
+    __syncthreads();
+    a = r.v[0]; ao = r.o[0]; b = r.v2[0]; bo = r.o2[0];
+#pragma unroll
+    for (int w = 1; w < BLOCK / 64; ++w) {
+        float v = r.v[w]; uint32_t o = r.o[w];
+        if (v > a || (v == a && o < ao)) { a = v; ao = o; }
+        v = r.v2[w]; o = r.o2[w];
+        if (v < b || (v == b && o < bo)) { b = v; bo = o; }
+    }
+}
+template <int BLOCK>
+BPD void block_min(Red<BLOCK> &r, float &b, uint32_t &bo) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    wave_min_owner(b, bo);
+    if (BLOCK == 64) return;
+    __syncthreads();
+    if (lane == 0) { r.v2[wave] = b; r.o2[wave] = bo; }
+    __syncthreads();
+    b = r.v2[0]; bo = r.o2[0];
+#pragma unroll
+    for (int w = 1; w < BLOCK / 64; ++w) {
+        float v = r.v2[w]; uint32_t o = r.o2[w];
+        if (v < b || (v == b && o < bo)) { b = v; bo = o; }
+    }
+}
+template <int BLOCK>
+BPD float block_maxf(Red<BLOCK> &r, float v) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+    if (BLOCK == 64) return v;
+    __syncthreads();
+    if (lane == 0) r.v[wave] = v;
+    __syncthreads();
+    float m = r.v[0];
+#pragma unroll
+    for (int w = 1; w < BLOCK / 64; ++w) m = fmaxf(m, r.v[w]);
+    return m;
+}
+template <int BLOCK>
+BPD uint32_t block_min_u32(Red<BLOCK> &r, uint32_t v) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, off, 64));
+    if (BLOCK == 64) return v;
+    __syncthreads();
+    if (lane == 0) r.scan[wave] = v;
+    __syncthreads();
+    uint32_t m = r.scan[0];
+#pragma unroll
+    for (int w = 1; w < BLOCK / 64; ++w) m = min(m, r.scan[w]);
+    return m;
+}
+// Exclusive prefix sum of one value per lane; returns the lane's offset and the total.
+template <int BLOCK>
+BPD uint32_t block_exclusive_scan(Red<BLOCK> &r, uint32_t value, uint32_t &total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t inclusive = value;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t other = __shfl_up(inclusive, off, 64);
+        if (lane >= off) inclusive += other;
+    }
+    if (BLOCK == 64) {
+        total = __shfl(inclusive, 63, 64);
+        return inclusive - value;
+    }
+    __syncthreads();
+    if (lane == 63) r.scan[wave] = inclusive;
+    __syncthreads();
+    uint32_t before = 0;
+    total = 0;
+#pragma unroll
+    for (int w = 0; w < BLOCK / 64; ++w) {
+        uint32_t s = r.scan[w];
+        if (w < wave) before += s;
+        total += s;
+    }
+    return before + inclusive - value;
+}
+
+// Exclusive prefix over a bucket array in LDS, each lane owning a contiguous chunk (Force:1080-1094).
+template <int BLOCK>
+BPD void bucket_prefix(Red<BLOCK> &r, uint32_t *buckets, uint32_t count) {
+    uint32_t perLane = (count + BLOCK - 1) / BLOCK;
+    uint32_t first = min((uint32_t)threadIdx.x * perLane, count), last = min(first + perLane, count);
+    uint32_t sum = 0;
+    for (uint32_t b = first; b < last; ++b) sum += buckets[b];
+    uint32_t total;
+    uint32_t running = block_exclusive_scan<BLOCK>(r, sum, total);
+    __syncthreads();
+    for (uint32_t b = first; b < last; ++b) { uint32_t own = buckets[b]; buckets[b] = running; running += own; }
+}
+
+BPD uint32_t bucket_of(int64_t cx, int64_t cy, uint32_t buckets) {      // Force:1000-1002
+    return ((uint32_t)cx * 73856093u ^ (uint32_t)cy * 19349663u) % buckets;
+}
+BPD int64_t cell_coord(float x, float width) { return (int64_t)floorf(x / width); }
+
+// ---------------------------------------------------------------------------------------
+// Geometry, force records and configuration of one world (working side of the body poses).
+// ---------------------------------------------------------------------------------------
+struct Geo {
+    const Arena *A;
+    uint32_t ns, nb;
+    uint32_t so, bo, vo, to, mo, bao;   // segment, body, vertex, triangle, motor, band offsets
+    uint32_t side;                      // working side of the body poses
+    float2 globalForce;
+    float sepTol;
+
+    BPD const float2 *verts() const { return A->verts + vo; }
+    BPD const uint4 *tris() const { return A->tris + to; }
+    BPD uint32_t segCat(uint32_t j) const { return A->segCat[so + j]; }
+    BPD uint32_t segMask(uint32_t j) const { return A->segMask[so + j]; }
+    BPD int64_t segId(uint32_t j) const { return A->segId[so + j]; }
+    BPD int64_t segOwner(uint32_t j) const { return A->segOwner[so + j]; }
+    BPD uint64_t bodyId(uint32_t k) const { return A->bodyId[bo + k]; }
+    BPD uint32_t *bodyHits() const { return A->bodyHits + bo; }
+    BPD float4 band(int32_t k) const { return A->bands[bao + k]; }
+    BPD float4 motor(int32_t k) const { return A->motors[mo + k]; }
+
+    BPD Seg segment(uint32_t j) const {
+        Seg s;
+        float4 ab = A->segAB[so + j], rg = A->segRegion[so + j];
+        s.a = f2(ab.x, ab.y); s.b = f2(ab.z, ab.w);
+        s.prevStart = f2(rg.x, rg.y);
+        s.outward = rg.z;
+        s.flags = A->segFlags[so + j];
+        return s;
+    }
+    BPD BodyView body(uint32_t k) const {
+        BodyView b;
+        uint32_t g = bo + k;
+        b.pose = A->bodyPose[side][g]; b.prev = A->bodyPrev[side][g]; b.motion = A->bodyMotion[g];
+        b.shape = A->bodyShape[g]; b.ranges = A->bodyRanges[g]; b.kind = A->bodyKind[g];
+        return b;
+    }
+    // Segment candidate of a bubble, Geometry:279-290.
+    BPD bool segment_candidate(float2 p, float radius, float margin, uint32_t cat, uint32_t mask, uint32_t j) const {
+        if (!filter_accepts(cat, mask, segCat(j), segMask(j))) return false;
+        float4 ab = A->segAB[so + j];
+        float reach = radius + margin + sepTol;
+        float2 lower = subs(min2(f2(ab.x, ab.y), f2(ab.z, ab.w)), reach);
+        float2 upper = adds(max2(f2(ab.x, ab.y), f2(ab.z, ab.w)), reach);
+        return p.x >= lower.x && p.y >= lower.y && p.x <= upper.x && p.y <= upper.y;
+    }
+    // Bubble-body candidate, Geometry:361-379 (NaN centres compare like the reference).
+    BPD bool body_candidate(float2 p, float radius, float margin, uint32_t cat, uint32_t mask, uint32_t k) const {
+        uint32_t g = bo + k;
+        uint4 kind = A->bodyKind[g];
+        bool isStatic = (kind.y & kBodyStatic) != 0;
+        bool obstacle = isStatic ? (kind.x != 0u) : ((kind.y & kBodyKinematic) != 0u);
+        if (!obstacle) return false;
+        if (!filter_accepts(cat, mask, kind.z, kind.w)) return false;
+        float4 pose = A->bodyPose[side][g];
+        float2 prev = A->bodyPrev[side][g];
+        float br = A->bodyShape[g].x;
+        float2 bodyLow = subs(min2(f2(pose.x, pose.y), prev), br);
+        float2 bodyHigh = adds(max2(f2(pose.x, pose.y), prev), br);
+        float reach = radius + fmaxf(margin, 0.0f);
+        float2 lo = subs(p, reach), hi = adds(p, reach);
+        return !(hi.x < bodyLow.x || hi.y < bodyLow.y || bodyHigh.x < lo.x || bodyHigh.y < lo.y);
+    }
+    // Non-contact acceleration a0 (Force:450-470): global force, band, motor.
+    BPD float2 non_contact(uint32_t flags, float invm, int32_t bandIndex, int32_t motorIndex, float2 p) const {
+        if (!(flags & kForceDriven) || !(invm > 0.0f)) return f2(0.0f, 0.0f);
+        float2 force = globalForce;
+        if (bandIndex >= 0) {
+            float4 bd = band(bandIndex);
+            float2 bf = mul(sub(f2(bd.x, bd.y), p), bd.z);
+            float lim = bd.w, magnitude = len2(bf);
+            if (lim > 0.0f && magnitude > lim) bf = mul(bf, lim / magnitude);
+            force = add(force, bf);
+        }
+        if (motorIndex >= 0) {
+            float4 md = motor(motorIndex);
+            force = add(force, f2(md.x, md.y));
+        }
+        return mul(force, invm);
+    }
+};
+
+BPD Geo make_geo(const Arena *A, const WorldDesc *d, uint32_t side) {
+    Geo g;
+    g.A = A;
+    g.ns = d->segmentCount; g.nb = d->bodyCount;
+    g.so = d->segmentOffset; g.bo = d->bodyOffset; g.vo = d->vertexOffset; g.to = d->triangleOffset;
+    g.mo = d->motorOffset; g.bao = d->bandOffset;
+    g.side = side;
+    g.globalForce = f2(d->globalForce[0], d->globalForce[1]);
+    g.sepTol = d->sepTol;
+    return g;
+}
+
+// Kinematic and static bodies at the start of a step (PostSolve:66-82 for the bodies a bubble
+// world may hold): previous <- pose, kinematic pose advanced by its clamped velocity.
+template <int BLOCK>
+BPD uint32_t predict_bodies(const Arena *A, const WorldDesc *d, uint32_t pub, uint32_t wside) {
+    uint32_t safety = 0;
+    for (uint32_t k = threadIdx.x; k < d->bodyCount; k += BLOCK) {
+        uint32_t g = d->bodyOffset + k;
+        float4 pose = A->bodyPose[pub][g];
+        float4 motion = A->bodyMotion[g];
+        uint4 kind = A->bodyKind[g];
+        float2 prev = f2(pose.x, pose.y);
+        pose.w = pose.z;
+        if (kind.y & kBodyStatic) {
+            motion.x = motion.y = motion.z = 0.0f;
+        } else {
+            float dt = d->dt;
+            motion.z = motion.z * bpm::expf_det(-motion.w * dt);
+            float2 lin = f2(motion.x, motion.y);
+            float2 lim = A->bodyLimits[g];
+            float speed = len2(lin);
+            if (lim.x > 0.0f && speed > lim.x) lin = mul(lin, lim.x / speed);
+            motion.x = lin.x; motion.y = lin.y;
+            if (lim.y > 0.0f) motion.z = clampf(motion.z, -lim.y, lim.y);
+            pose.x = pose.x + motion.x * dt;
+            pose.y = pose.y + motion.y * dt;
+            pose.z = pose.z + motion.z * dt;
+            if (!isfinite(pose.x) || !isfinite(pose.y) || !isfinite(pose.z) || !isfinite(pose.w) || !isfinite(motion.x) ||
+                !isfinite(motion.y) || !isfinite(motion.z))
+                safety |= 1u;
+        }
+        A->bodyPose[wside][g] = pose;
+        A->bodyPrev[wside][g] = prev;
+        A->bodyMotion[g] = motion;
+    }
+    return safety;
+}
+
+// Reduction of h for the whole world once the three terms are known (Force:633-736).
+// Every lane evaluates it with the same inputs, so the control state stays uniform.
+BPD void reduce_h(Ctl &c, float stiffnessBound, uint32_t w2Owner, float dbMin, uint32_t dbOwner, float mbMin,
+                  uint32_t mbOwner, const int64_t *ids, float *history, uint32_t lane) {
+    float remaining = c.t0 - c.t1;
+    if (mbMin <= 0.0f) {
+        c.rebuild = 1; c.h = 0.0f;
+        c.limObj = (uint64_t)ids[mbOwner]; c.limTerm = BP_TERM_MARGIN;
+        return;
+    }
+    float split = fmaxf(c.split, 1.0f);
+    float h = remaining;
+    uint32_t term = BP_TERM_REMAINING;
+    uint64_t object = 0;
+    if (split > 1.0f && c.t0 / split < h) h = c.t0 / split;
+    if (stiffnessBound / split < h) { h = stiffnessBound / split; term = BP_TERM_STIFFNESS; object = (uint64_t)ids[w2Owner]; }
+    if (dbMin / split < h) { h = dbMin / split; term = BP_TERM_DISTANCE; object = (uint64_t)ids[dbOwner]; }
+    if (mbMin < h) { h = mbMin; term = BP_TERM_MARGIN; object = (uint64_t)ids[mbOwner]; c.rebuild = 1; }
+    h = fmaxf(h, c.t0 * 1e-7f);
+    uint32_t frameSubstep = c.c1;
+    if (h >= remaining || remaining - h <= c.t0 * 1e-5f) { h = remaining; c.t1 = c.t0; }
+    else c.t1 = c.t1 + h;
+    c.h = h;
+    c.c0 += 1;
+    c.c1 = frameSubstep + 1;
+    c.limObj = object; c.limTerm = term;
+    if (lane == 0) history[frameSubstep] = h;
+}
+
+// Region removal or wrap of one bubble (PostSolve:536-566). Returns the region result and
+// updates p and start; sets safety bit 1 (and leaves the record) when the wrap is not finite.
+BPD uint32_t apply_region(float2 &p, float2 &start, float r, uint32_t flags, float2 lo, float2 hi, uint32_t &safety) {
+    if (!(flags & kEnabled) || (flags & kStatic)) return BP_REGION_UNCHANGED;
+    float2 nearest = f2(clampf(p.x, lo.x, hi.x), clampf(p.y, lo.y, hi.y));
+    if (len2(sub(p, nearest)) <= r) return BP_REGION_UNCHANGED;
+    if (!(flags & kWrap)) return BP_REGION_REMOVED;
+    float2 span = adds(sub(hi, lo), 2.0f * r);
+    float dx = 0.0f, dy = 0.0f;
+    if (p.x > hi.x) dx = -span.x * fmaxf(1.0f, ceilf((p.x - r - hi.x) / span.x));
+    else if (p.x < lo.x) dx = span.x * fmaxf(1.0f, ceilf((lo.x - r - p.x) / span.x));
+    if (p.y > hi.y) dy = -span.y * fmaxf(1.0f, ceilf((p.y - r - hi.y) / span.y));
+    else if (p.y < lo.y) dy = span.y * fmaxf(1.0f, ceilf((lo.y - r - p.y) / span.y));
+    float2 dl = f2(dx, dy);
+    float2 wrapped = add(p, dl);
+    float2 nearestWrapped = f2(clampf(wrapped.x, lo.x, hi.x), clampf(wrapped.y, lo.y, hi.y));
+    if (len2(sub(wrapped, nearestWrapped)) > r) dl = add(dl, sub(nearestWrapped, wrapped));
+    float2 np = add(p, dl), ns = add(start, dl);
+    if (!finite2(np) || !finite2(ns)) { safety |= 1u; return BP_REGION_UNCHANGED; }
+    p = np;
+    start = ns;
+    return BP_REGION_WRAPPED;
+}
+
+struct Violation { uint32_t kind; uint64_t other; int64_t segment; float detail; };
+
+// Body and one-sided edge assertions of a movable bubble (Force:889-917), in the reference's order.
+BPD bool assert_body_and_edges(const Geo &g, float2 center, float2 start, uint32_t cat, uint32_t mask, float fraction,
+                               Violation &out) {
+    for (uint32_t k = 0; k < g.nb; ++k) {
+        BodyView body = g.body(k);
+        if (!body_immovable(body) || !filter_accepts(cat, mask, body.kind.z, body.kind.w)) continue;
+        if (body_inside(center, body_at_fraction(body, fraction), g.verts(), g.tris())) {
+            out.kind = BP_VIOLATION_INSIDE_BODY; out.other = g.bodyId(k); out.segment = 0; out.detail = 0.0f;
+            return true;
+        }
+    }
+    for (uint32_t j = 0; j < g.ns; ++j) {
+        Seg s = g.segment(j);
+        if (!(s.flags & kSegPolygon) || !filter_accepts(cat, mask, g.segCat(j), g.segMask(j))) continue;
+        float2 edge = sub(s.b, s.a);
+        float lengthSquared = dot2(edge, edge);
+        if (!(lengthSquared > 0.0f)) continue;
+        float2 outward = mul(dvs(f2(-edge.y, edge.x), __builtin_sqrtf(lengthSquared)), s.outward);
+        float before = dot2(sub(start, s.a), outward), after = dot2(sub(center, s.a), outward);
+        if (!(before > 0.0f) || !(after <= 0.0f)) continue;
+        float2 crossing = add(start, mul(sub(center, start), before / (before - after)));
+        float along = dot2(sub(crossing, s.a), edge) / lengthSquared;
+        if (along < 0.0f || along > 1.0f) continue;
+        out.kind = BP_VIOLATION_EDGE_SIDE; out.segment = g.segId(j); out.other = (uint64_t)g.segOwner(j); out.detail = after;
+        return true;
+    }
+    return false;
+}
+
+BPD float step_fraction(const Ctl &c) { return c.t0 > 0.0f ? clampf(c.t1 / c.t0, 0.0f, 1.0f) : 1.0f; }
+
+// Status word of a finished step (error precedence, NativeMetalSolver.swift:1026-1040).
+BPD int32_t final_status(int32_t status, const Ctl &c, uint32_t safetyAll, bool claimed, bool finished, uint32_t n) {
+    if (status != BP_OK) return status;
+    if (c.limit) return BP_ERR_SUBSTEP_LIMIT_EXCEEDED;
+    if (safetyAll & 1u) return BP_ERR_NON_FINITE;
+    if (claimed) return BP_ERR_CONTACT_VIOLATION;
+    if (n > 0 && !finished) return BP_ERR_COMMAND_FAILED;
+    return BP_OK;
+}
+
+}  // namespace bpd
+
+#endif  // BP_COMMON_H

@@ -87,7 +87,7 @@ struct WorldDesc {
     int64_t injectionObject;
     float injectionPosition[2];
     uint32_t failed;          // sticky: a world that failed a step is not advanced again
-    uint32_t reserved;
+    uint32_t exportAdjacency; // 1 = leave the rows of the last grid build in rowStart/rowLen/adj
 };
 
 // Per-world result of the last step (bp_frame_info fields).

@@ -18,122 +18,20 @@
 //   4. gated end of step: region, finite check, contact assertion, status and publication.
 // Reductions keep the reference's tie rule (equal values -> lower bubble index), and every
 // per-bubble sum runs in adjacency-row order, so the result is independent of the block size.
-#include "bp_device.h"
-#include "../../include/bubblephysics.h"
+#include "bp_common.h"
 
 namespace bpd {
 
-struct Ctl {           // NativeSubstepControl in registers, uniform across the workgroup
-    float t0, t1, h, split;
-    uint32_t c0, c1, rebuild, limit;
-    uint64_t limObj;
-    uint32_t limTerm;
-};
-
 template <int BLOCK> struct Shared {
     uint32_t buckets[kMaxBuckets];
-    float redV[BLOCK / 64];
-    uint32_t redO[BLOCK / 64];
-    float redV2[BLOCK / 64];
-    uint32_t redO2[BLOCK / 64];
-    uint32_t scan[BLOCK / 64];
-    uint32_t counters[8];
+    Red<BLOCK> red;
 };
-
-// ---------------------------------------------------------------------------------------
-// Workgroup primitives.
-// ---------------------------------------------------------------------------------------
-BPD void wave_max_owner(float &v, uint32_t &o) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        float ov = __shfl_xor(v, off, 64);
-        uint32_t oo = __shfl_xor(o, off, 64);
-        if (ov > v || (ov == v && oo < o)) { v = ov; o = oo; }
-    }
-}
-BPD void wave_min_owner(float &v, uint32_t &o) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        float ov = __shfl_xor(v, off, 64);
-        uint32_t oo = __shfl_xor(o, off, 64);
-        if (ov < v || (ov == v && oo < o)) { v = ov; o = oo; }
-    }
-}
-
-// Max of (a, owner) and min of (b, owner) over the workgroup; results broadcast to every lane.
-template <int BLOCK>
-BPD void block_maxmin(Shared<BLOCK> &sh, float &a, uint32_t &ao, float &b, uint32_t &bo) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    wave_max_owner(a, ao);
-    wave_min_owner(b, bo);
-    __syncthreads();
-    if (lane == 0) { sh.redV[wave] = a; sh.redO[wave] = ao; sh.redV2[wave] = b; sh.redO2[wave] = bo; }
-    __syncthreads();
-    a = sh.redV[0]; ao = sh.redO[0]; b = sh.redV2[0]; bo = sh.redO2[0];
-#pragma unroll
-    for (int w = 1; w < BLOCK / 64; ++w) {
-        float v = sh.redV[w]; uint32_t o = sh.redO[w];
-        if (v > a || (v == a && o < ao)) { a = v; ao = o; }
-        v = sh.redV2[w]; o = sh.redO2[w];
-        if (v < b || (v == b && o < bo)) { b = v; bo = o; }
-    }
-}
-template <int BLOCK>
-BPD void block_min(Shared<BLOCK> &sh, float &b, uint32_t &bo) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    wave_min_owner(b, bo);
-    __syncthreads();
-    if (lane == 0) { sh.redV2[wave] = b; sh.redO2[wave] = bo; }
-    __syncthreads();
-    b = sh.redV2[0]; bo = sh.redO2[0];
-#pragma unroll
-    for (int w = 1; w < BLOCK / 64; ++w) {
-        float v = sh.redV2[w]; uint32_t o = sh.redO2[w];
-        if (v < b || (v == b && o < bo)) { b = v; bo = o; }
-    }
-}
-template <int BLOCK>
-BPD float block_maxf(Shared<BLOCK> &sh, float v) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
-    __syncthreads();
-    if (lane == 0) sh.redV[wave] = v;
-    __syncthreads();
-    float r = sh.redV[0];
-#pragma unroll
-    for (int w = 1; w < BLOCK / 64; ++w) r = fmaxf(r, sh.redV[w]);
-    return r;
-}
-// Exclusive prefix sum of one value per lane; returns the lane's offset and the tile total.
-template <int BLOCK>
-BPD uint32_t block_exclusive_scan(Shared<BLOCK> &sh, uint32_t value, uint32_t &total) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    uint32_t inclusive = value;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        uint32_t other = __shfl_up(inclusive, off, 64);
-        if (lane >= off) inclusive += other;
-    }
-    __syncthreads();
-    if (lane == 63) sh.scan[wave] = inclusive;
-    __syncthreads();
-    uint32_t before = 0;
-    total = 0;
-#pragma unroll
-    for (int w = 0; w < BLOCK / 64; ++w) {
-        uint32_t s = sh.scan[w];
-        if (w < wave) before += s;
-        total += s;
-    }
-    return before + inclusive - value;
-}
 
 // ---------------------------------------------------------------------------------------
 // The world view of one workgroup.
 // ---------------------------------------------------------------------------------------
 struct World {
-    Arena A;
+    const Arena *A;
     const WorldDesc *d;
     uint32_t n, ns, nb, w;       // w: working side
     uint32_t o;                  // bubble offset
@@ -162,23 +60,23 @@ struct World {
 
     BPD Seg segment(uint32_t j) const {
         Seg s;
-        float4 ab = A.segAB[d->segmentOffset + j];
-        float4 rg = A.segRegion[d->segmentOffset + j];
+        float4 ab = A->segAB[d->segmentOffset + j];
+        float4 rg = A->segRegion[d->segmentOffset + j];
         s.a = f2(ab.x, ab.y); s.b = f2(ab.z, ab.w);
         s.prevStart = f2(rg.x, rg.y);
         s.outward = rg.z;
-        s.flags = A.segFlags[d->segmentOffset + j];
+        s.flags = A->segFlags[d->segmentOffset + j];
         return s;
     }
     BPD BodyView body(uint32_t k) const {
         BodyView b;
         uint32_t g = d->bodyOffset + k;
-        b.pose = A.bodyPose[w][g];
-        b.prev = A.bodyPrev[w][g];
-        b.motion = A.bodyMotion[g];
-        b.shape = A.bodyShape[g];
-        b.ranges = A.bodyRanges[g];
-        b.kind = A.bodyKind[g];
+        b.pose = A->bodyPose[w][g];
+        b.prev = A->bodyPrev[w][g];
+        b.motion = A->bodyMotion[g];
+        b.shape = A->bodyShape[g];
+        b.ranges = A->bodyRanges[g];
+        b.kind = A->bodyKind[g];
         return b;
     }
 
@@ -205,9 +103,6 @@ struct World {
 
     BPD int64_t cell_x(float2 p) const { return (int64_t)floorf(p.x / cellWidth); }
     BPD int64_t cell_y(float2 p) const { return (int64_t)floorf(p.y / cellWidth); }
-    static BPD uint32_t bucket_of(int64_t cx, int64_t cy, uint32_t buckets) {    // Force:1000-1002
-        return ((uint32_t)cx * 73856093u ^ (uint32_t)cy * 19349663u) % buckets;
-    }
 
     // Pair predicate of Geometry:262-273, evaluated with the lower index as the row owner.
     BPD bool pair_candidate(uint32_t lo, uint32_t hi) const {
@@ -219,8 +114,8 @@ struct World {
     // Segment predicate, Geometry:279-290.
     BPD bool segment_candidate(uint32_t i, uint32_t j) const {
         uint32_t g = d->segmentOffset + j;
-        if (!filter_accepts(cat[i], mask[i], A.segCat[g], A.segMask[g])) return false;
-        float4 ab = A.segAB[g];
+        if (!filter_accepts(cat[i], mask[i], A->segCat[g], A->segMask[g])) return false;
+        float4 ab = A->segAB[g];
         float reach = radius[i] + margin[i] + sepTol;
         float2 lower = subs(min2(f2(ab.x, ab.y), f2(ab.z, ab.w)), reach);
         float2 upper = adds(max2(f2(ab.x, ab.y), f2(ab.z, ab.w)), reach);
@@ -230,14 +125,14 @@ struct World {
     // Body predicate, Geometry:361-379.
     BPD bool body_candidate(uint32_t i, uint32_t k) const {
         uint32_t g = d->bodyOffset + k;
-        uint4 kind = A.bodyKind[g];
+        uint4 kind = A->bodyKind[g];
         bool isStatic = (kind.y & kBodyStatic) != 0;
         bool obstacle = isStatic ? (kind.x != 0u) : ((kind.y & kBodyKinematic) != 0u);
         if (!obstacle) return false;
         if (!filter_accepts(cat[i], mask[i], kind.z, kind.w)) return false;
-        float4 pose = A.bodyPose[w][g];
-        float2 prev = A.bodyPrev[w][g];
-        float br = A.bodyShape[g].x;
+        float4 pose = A->bodyPose[w][g];
+        float2 prev = A->bodyPrev[w][g];
+        float br = A->bodyShape[g].x;
         float2 bodyLow = subs(min2(f2(pose.x, pose.y), prev), br);
         float2 bodyHigh = adds(max2(f2(pose.x, pose.y), prev), br);
         float reach = radius[i] + fmaxf(margin[i], 0.0f);
@@ -260,7 +155,7 @@ BPD void enumerate_row(const World &W, const uint32_t *ends, uint32_t buckets, u
     for (int dx = -1; dx <= 1; ++dx)
         for (int dy = -1; dy <= 1; ++dy) {
             int64_t tx = cx + dx, ty = cy + dy;
-            uint32_t b = World::bucket_of(tx, ty, buckets);
+            uint32_t b = bucket_of(tx, ty, buckets);
             uint32_t lo = b == 0 ? 0u : ends[b - 1], hi = ends[b];
             for (uint32_t k = lo; k < hi; ++k) {
                 uint32_t j = W.grid[k];
@@ -314,18 +209,18 @@ BPD bool build_grid(World &W, Shared<BLOCK> &sh, float remaining, uint32_t limit
     // Cell width (Force:1052-1064).
     float width = W.sepTol;
     for (uint32_t i = threadIdx.x; i < n; i += BLOCK) width = fmaxf(width, 2.0f * (W.radius[i] + W.margin[i]) + W.sepTol);
-    width = block_maxf<BLOCK>(sh, width);
+    width = block_maxf<BLOCK>(sh.red, width);
     W.cellWidth = fmaxf(width, kFloatMin);
-    for (uint32_t k = threadIdx.x; k < W.nb; k += BLOCK) W.A.bodyHits[d.bodyOffset + k] = 0;
+    for (uint32_t k = threadIdx.x; k < W.nb; k += BLOCK) W.A->bodyHits[d.bodyOffset + k] = 0;
     // Hashed grid: count, exclusive prefix, scatter (Force:1066-1105).
     uint32_t buckets = d.buckets;
     for (uint32_t b = threadIdx.x; b < buckets; b += BLOCK) sh.buckets[b] = 0;
-    if (threadIdx.x < 8) sh.counters[threadIdx.x] = 0;
+    if (threadIdx.x < 8) sh.red.counters[threadIdx.x] = 0;
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
         float2 p = W.pos[i];
         if (!finite2(p)) continue;
-        atomicAdd(&sh.buckets[World::bucket_of(W.cell_x(p), W.cell_y(p), buckets)], 1u);
+        atomicAdd(&sh.buckets[bucket_of(W.cell_x(p), W.cell_y(p), buckets)], 1u);
     }
     __syncthreads();
     {
@@ -334,7 +229,7 @@ BPD bool build_grid(World &W, Shared<BLOCK> &sh, float remaining, uint32_t limit
         uint32_t sum = 0;
         for (uint32_t b = first; b < last; ++b) sum += sh.buckets[b];
         uint32_t total;
-        uint32_t running = block_exclusive_scan<BLOCK>(sh, sum, total);
+        uint32_t running = block_exclusive_scan<BLOCK>(sh.red, sum, total);
         __syncthreads();
         for (uint32_t b = first; b < last; ++b) { uint32_t own = sh.buckets[b]; sh.buckets[b] = running; running += own; }
     }
@@ -342,14 +237,14 @@ BPD bool build_grid(World &W, Shared<BLOCK> &sh, float remaining, uint32_t limit
     for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
         float2 p = W.pos[i];
         if (!finite2(p)) continue;
-        uint32_t slot = atomicAdd(&sh.buckets[World::bucket_of(W.cell_x(p), W.cell_y(p), buckets)], 1u);
+        uint32_t slot = atomicAdd(&sh.buckets[bucket_of(W.cell_x(p), W.cell_y(p), buckets)], 1u);
         W.grid[slot] = i;
     }
     __syncthreads();
     // Pass 1: row lengths and the reference's overflow counters.
     const uint32_t *ends = sh.buckets;
     uint32_t droppedLocal = 0, maxLocal = 0, pairsLocal = 0, overflowLocal = 0;
-    uint32_t *bodyHits = W.A.bodyHits + d.bodyOffset;
+    uint32_t *bodyHits = W.A->bodyHits + d.bodyOffset;
     for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
         uint32_t count = 0, forward = 0;
         auto tally = [&](uint32_t e) {
@@ -371,10 +266,10 @@ BPD bool build_grid(World &W, Shared<BLOCK> &sh, float remaining, uint32_t limit
         maxLocal = max(maxLocal, hits);
         if (hits > limit) { droppedLocal += hits - limit; overflowLocal = 1; }
     }
-    atomicAdd(&sh.counters[0], droppedLocal);
-    atomicMax(&sh.counters[1], maxLocal);
-    atomicAdd(&sh.counters[2], pairsLocal);
-    atomicOr(&sh.counters[3], overflowLocal);
+    atomicAdd(&sh.red.counters[0], droppedLocal);
+    atomicMax(&sh.red.counters[1], maxLocal);
+    atomicAdd(&sh.red.counters[2], pairsLocal);
+    atomicOr(&sh.red.counters[3], overflowLocal);
     __syncthreads();
     // Row starts: tiled exclusive scan of the row lengths in bubble order.
     uint32_t carry = 0;
@@ -382,15 +277,15 @@ BPD bool build_grid(World &W, Shared<BLOCK> &sh, float remaining, uint32_t limit
         uint32_t i = base + threadIdx.x;
         uint32_t len = i < n ? W.rowLen[i] : 0u;
         uint32_t total;
-        uint32_t offset = block_exclusive_scan<BLOCK>(sh, len, total);
+        uint32_t offset = block_exclusive_scan<BLOCK>(sh.red, len, total);
         if (i < n) W.rowStart[i] = carry + offset;
         carry += total;
     }
     entries = carry;
-    stats.dropped = sh.counters[0];
-    stats.maxDemand = sh.counters[1];
-    stats.pairs = sh.counters[2];
-    stats.overflowRows = sh.counters[3];
+    stats.dropped = sh.red.counters[0];
+    stats.maxDemand = sh.red.counters[1];
+    stats.pairs = sh.red.counters[2];
+    stats.overflowRows = sh.red.counters[3];
     if (carry > d.adjCapacity) return false;
     __syncthreads();
     // Pass 2: fill rows, then order each row by (kind, neighbour) (Contact:222-238).
@@ -491,7 +386,7 @@ BPD void pass_forces(World &W, Shared<BLOCK> &sh, const Ctl &c, uint32_t phase, 
         }
         if (mb < bestM) { bestM = mb; ownM = i; }
     }
-    block_maxmin<BLOCK>(sh, bestW, ownW, bestM, ownM);
+    block_maxmin<BLOCK>(sh.red, bestW, ownW, bestM, ownM);
     w2Max = bestW; w2Owner = ownW; mbMin = bestM; mbOwner = ownM;
 }
 
@@ -543,7 +438,7 @@ BPD void pass_bounds(World &W, Shared<BLOCK> &sh, const Ctl &c, float &w2Max, ui
         }
         if (mb < bestM) { bestM = mb; ownM = i; }
     }
-    block_maxmin<BLOCK>(sh, bestW, ownW, bestM, ownM);
+    block_maxmin<BLOCK>(sh.red, bestW, ownW, bestM, ownM);
     w2Max = bestW; w2Owner = ownW; mbMin = bestM; mbOwner = ownM;
 }
 
@@ -601,7 +496,7 @@ BPD void pass_distance(World &W, Shared<BLOCK> &sh, const Ctl &c, float hMax, fl
         }
         if (best < bestD) { bestD = best; ownD = i; }
     }
-    block_min<BLOCK>(sh, bestD, ownD);
+    block_min<BLOCK>(sh.red, bestD, ownD);
     dbMin = bestD; dbOwner = ownD;
 }
 
@@ -634,7 +529,6 @@ BPD void pass_drift(World &W, const Ctl &c) {
 // Contact assertion (Force:865-918): the lowest-index bubble with a violation is recorded.
 // Pairs are searched in the 3x3 cells of a grid of the current centres (coincident centres
 // share a cell), bodies and one-sided edges by direct loops.
-struct Violation { uint32_t kind; uint64_t other; int64_t segment; float detail; };
 
 // Returns true (uniformly) when some bubble violates; the lowest violating bubble writes the
 // record (NativeContactAssertion) into the world status.
@@ -649,7 +543,7 @@ BPD bool assert_contacts(World &W, Shared<BLOCK> &sh, const Ctl &c, WorldStatus 
     for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
         float2 p = W.pos[i];
         if (!finite2(p)) continue;
-        atomicAdd(&sh.buckets[World::bucket_of(W.cell_x(p), W.cell_y(p), buckets)], 1u);
+        atomicAdd(&sh.buckets[bucket_of(W.cell_x(p), W.cell_y(p), buckets)], 1u);
     }
     __syncthreads();
     {
@@ -658,7 +552,7 @@ BPD bool assert_contacts(World &W, Shared<BLOCK> &sh, const Ctl &c, WorldStatus 
         uint32_t sum = 0;
         for (uint32_t b = first; b < last; ++b) sum += sh.buckets[b];
         uint32_t total;
-        uint32_t running = block_exclusive_scan<BLOCK>(sh, sum, total);
+        uint32_t running = block_exclusive_scan<BLOCK>(sh.red, sum, total);
         __syncthreads();
         for (uint32_t b = first; b < last; ++b) { uint32_t own = sh.buckets[b]; sh.buckets[b] = running; running += own; }
     }
@@ -666,7 +560,7 @@ BPD bool assert_contacts(World &W, Shared<BLOCK> &sh, const Ctl &c, WorldStatus 
     for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
         float2 p = W.pos[i];
         if (!finite2(p)) continue;
-        uint32_t slot = atomicAdd(&sh.buckets[World::bucket_of(W.cell_x(p), W.cell_y(p), buckets)], 1u);
+        uint32_t slot = atomicAdd(&sh.buckets[bucket_of(W.cell_x(p), W.cell_y(p), buckets)], 1u);
         W.grid[slot] = i;
     }
     __syncthreads();
@@ -683,7 +577,7 @@ BPD bool assert_contacts(World &W, Shared<BLOCK> &sh, const Ctl &c, WorldStatus 
         for (int dx = -1; dx <= 1; ++dx)
             for (int dy = -1; dy <= 1; ++dy) {
                 int64_t tx = cx + dx, ty = cy + dy;
-                uint32_t b = World::bucket_of(tx, ty, buckets);
+                uint32_t b = bucket_of(tx, ty, buckets);
                 uint32_t lo = b == 0 ? 0u : sh.buckets[b - 1], hi = sh.buckets[b];
                 for (uint32_t k = lo; k < hi; ++k) {
                     uint32_t j = W.grid[k];
@@ -707,13 +601,13 @@ BPD bool assert_contacts(World &W, Shared<BLOCK> &sh, const Ctl &c, WorldStatus 
             if (!body_immovable(body) || !filter_accepts(W.cat[i], W.mask[i], body.kind.z, body.kind.w)) continue;
             if (body_inside(center, body_at_fraction(body, fraction), W.verts, W.tris)) {
                 found = true;
-                mine.kind = BP_VIOLATION_INSIDE_BODY; mine.other = W.A.bodyId[W.d->bodyOffset + k]; mine.detail = 0.0f;
+                mine.kind = BP_VIOLATION_INSIDE_BODY; mine.other = W.A->bodyId[W.d->bodyOffset + k]; mine.detail = 0.0f;
             }
         }
         for (uint32_t j = 0; j < W.ns && !found; ++j) {
             Seg s = W.segment(j);
             uint32_t g = W.d->segmentOffset + j;
-            if (!(s.flags & kSegPolygon) || !filter_accepts(W.cat[i], W.mask[i], W.A.segCat[g], W.A.segMask[g])) continue;
+            if (!(s.flags & kSegPolygon) || !filter_accepts(W.cat[i], W.mask[i], W.A->segCat[g], W.A->segMask[g])) continue;
             float2 edge = sub(s.b, s.a);
             float lengthSquared = dot2(edge, edge);
             if (!(lengthSquared > 0.0f)) continue;
@@ -724,8 +618,8 @@ BPD bool assert_contacts(World &W, Shared<BLOCK> &sh, const Ctl &c, WorldStatus 
             float along = dot2(sub(crossing, s.a), edge) / lengthSquared;
             if (along < 0.0f || along > 1.0f) continue;
             found = true;
-            mine.kind = BP_VIOLATION_EDGE_SIDE; mine.segment = W.A.segId[g];
-            mine.other = (uint64_t)W.A.segOwner[g]; mine.detail = after;
+            mine.kind = BP_VIOLATION_EDGE_SIDE; mine.segment = W.A->segId[g];
+            mine.other = (uint64_t)W.A->segOwner[g]; mine.detail = after;
         }
         if (found) { firstIndex = i; break; }
     }
@@ -733,10 +627,10 @@ BPD bool assert_contacts(World &W, Shared<BLOCK> &sh, const Ctl &c, WorldStatus 
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) winner = min(winner, (uint32_t)__shfl_xor((int)winner, off, 64));
     __syncthreads();
-    if ((threadIdx.x & 63) == 0) sh.scan[threadIdx.x >> 6] = winner;
+    if ((threadIdx.x & 63) == 0) sh.red.scan[threadIdx.x >> 6] = winner;
     __syncthreads();
     winner = 0xffffffffu;
-    for (int w = 0; w < BLOCK / 64; ++w) winner = min(winner, sh.scan[w]);
+    for (int w = 0; w < BLOCK / 64; ++w) winner = min(winner, sh.red.scan[w]);
     if (winner == 0xffffffffu) return false;
     if (firstIndex == winner) {
         st->violationKind = mine.kind;
@@ -752,7 +646,7 @@ BPD bool assert_contacts(World &W, Shared<BLOCK> &sh, const Ctl &c, WorldStatus 
 // The step kernel.
 // ---------------------------------------------------------------------------------------
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void world_step_kernel(Arena A, WorldDesc *descs, WorldStatus *stats,
+__global__ __launch_bounds__(BLOCK) void world_step_kernel(const Arena *__restrict__ A, WorldDesc *descs, WorldStatus *stats,
                                                            const uint32_t *order, uint32_t count) {
     __shared__ Shared<BLOCK> sh;
     if (blockIdx.x >= count) return;
@@ -768,38 +662,38 @@ __global__ __launch_bounds__(BLOCK) void world_step_kernel(Arena A, WorldDesc *d
     W.A = A; W.d = d; W.n = d->bubbleCount; W.ns = d->segmentCount; W.nb = d->bodyCount; W.w = wside;
     const uint32_t o = d->bubbleOffset;
     W.o = o;
-    W.pos = A.pos[wside] + o; W.start = A.start[wside] + o; W.vel = A.vel[wside] + o;
-    W.rot = A.rot[wside] + o; W.omega = A.omega[wside] + o; W.region = A.region[wside] + o;
-    W.id = A.id + o; W.invmRaw = A.invm + o; W.radius = A.radius + o; W.Kc = A.Kc + o; W.gamma = A.gamma + o;
-    W.mass = A.mass + o; W.flags = A.flags + o; W.cat = A.cat + o; W.mask = A.mask + o;
-    W.motor = A.motor + o; W.band = A.band + o;
-    W.acc = A.acc + o; W.a0 = A.a0 + o; W.ref = A.ref + o; W.margin = A.margin + o;
-    W.rowStart = A.rowStart + o; W.rowLen = A.rowLen + o; W.grid = A.grid + o; W.adj = A.adj + d->adjOffset;
-    W.motors = A.motors + d->motorOffset; W.bands = A.bands + d->bandOffset;
-    W.verts = A.verts + d->vertexOffset; W.tris = A.tris + d->triangleOffset;
+    W.pos = A->pos[wside] + o; W.start = A->start[wside] + o; W.vel = A->vel[wside] + o;
+    W.rot = A->rot[wside] + o; W.omega = A->omega[wside] + o; W.region = A->region[wside] + o;
+    W.id = A->id + o; W.invmRaw = A->invm + o; W.radius = A->radius + o; W.Kc = A->Kc + o; W.gamma = A->gamma + o;
+    W.mass = A->mass + o; W.flags = A->flags + o; W.cat = A->cat + o; W.mask = A->mask + o;
+    W.motor = A->motor + o; W.band = A->band + o;
+    W.acc = A->acc + o; W.a0 = A->a0 + o; W.ref = A->ref + o; W.margin = A->margin + o;
+    W.rowStart = A->rowStart + o; W.rowLen = A->rowLen + o; W.grid = A->grid + o; W.adj = A->adj + d->adjOffset;
+    W.motors = A->motors + d->motorOffset; W.bands = A->bands + d->bandOffset;
+    W.verts = A->verts + d->vertexOffset; W.tris = A->tris + d->triangleOffset;
     W.cellWidth = 0.0f;
     W.globalForce = f2(d->globalForce[0], d->globalForce[1]);
     W.sepTol = d->sepTol;
     const uint32_t n = W.n;
-    float *history = A.history + d->historyOffset;
+    float *history = A->history + d->historyOffset;
 
     // 1. Begin step (Force:285-293) on a working copy of the published state.
     for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
-        float2 p = A.pos[pub][o + i];
-        float2 v = A.vel[pub][o + i];
-        float om = A.omega[pub][o + i];
-        uint32_t f = A.flags[o + i];
+        float2 p = A->pos[pub][o + i];
+        float2 v = A->vel[pub][o + i];
+        float om = A->omega[pub][o + i];
+        uint32_t f = A->flags[o + i];
         if ((f & kEnabled) && (f & kStatic)) { v = f2(0.0f, 0.0f); om = 0.0f; }
-        W.pos[i] = p; W.start[i] = p; W.vel[i] = v; W.rot[i] = A.rot[pub][o + i]; W.omega[i] = om;
+        W.pos[i] = p; W.start[i] = p; W.vel[i] = v; W.rot[i] = A->rot[pub][o + i]; W.omega[i] = om;
         W.region[i] = BP_REGION_UNCHANGED;
     }
     // Kinematic and static bodies (PostSolve:66-82 for the bodies a bubble world may hold).
     uint32_t safety = 0;
     for (uint32_t k = threadIdx.x; k < W.nb; k += BLOCK) {
         uint32_t g = d->bodyOffset + k;
-        float4 pose = A.bodyPose[pub][g];
-        float4 motion = A.bodyMotion[g];
-        uint4 kind = A.bodyKind[g];
+        float4 pose = A->bodyPose[pub][g];
+        float4 motion = A->bodyMotion[g];
+        uint4 kind = A->bodyKind[g];
         float2 prev = f2(pose.x, pose.y);
         pose.w = pose.z;
         if (kind.y & kBodyStatic) { motion.x = motion.y = motion.z = 0.0f; }
@@ -807,7 +701,7 @@ __global__ __launch_bounds__(BLOCK) void world_step_kernel(Arena A, WorldDesc *d
             float dt = d->dt;
             motion.z = motion.z * bpm::expf_det(-motion.w * dt);
             float2 lin = f2(motion.x, motion.y);
-            float2 lim = A.bodyLimits[g];
+            float2 lim = A->bodyLimits[g];
             float speed = len2(lin);
             if (lim.x > 0.0f && speed > lim.x) lin = mul(lin, lim.x / speed);
             motion.x = lin.x; motion.y = lin.y;
@@ -819,9 +713,9 @@ __global__ __launch_bounds__(BLOCK) void world_step_kernel(Arena A, WorldDesc *d
                 !isfinite(motion.y) || !isfinite(motion.z))
                 safety |= 1u;
         }
-        A.bodyPose[wside][g] = pose;
-        A.bodyPrev[wside][g] = prev;
-        A.bodyMotion[g] = motion;
+        A->bodyPose[wside][g] = pose;
+        A->bodyPrev[wside][g] = prev;
+        A->bodyMotion[g] = motion;
     }
     __syncthreads();
 
@@ -954,18 +848,18 @@ __global__ __launch_bounds__(BLOCK) void world_step_kernel(Arena A, WorldDesc *d
     }
     // Safety bits of every lane.
     __syncthreads();
-    if (threadIdx.x == 0) sh.counters[5] = 0;
+    if (threadIdx.x == 0) sh.red.counters[5] = 0;
     __syncthreads();
-    if (safety) atomicOr(&sh.counters[5], safety);
+    if (safety) atomicOr(&sh.red.counters[5], safety);
     uint32_t rmLocal = 0, wrLocal = 0;
     for (uint32_t i = threadIdx.x; i < n; i += BLOCK) { rmLocal += W.region[i] == BP_REGION_REMOVED; wrLocal += W.region[i] == BP_REGION_WRAPPED; }
-    if (threadIdx.x == 0) { sh.counters[6] = 0; sh.counters[7] = 0; }
+    if (threadIdx.x == 0) { sh.red.counters[6] = 0; sh.red.counters[7] = 0; }
     __syncthreads();
-    atomicAdd(&sh.counters[6], rmLocal);
-    atomicAdd(&sh.counters[7], wrLocal);
+    atomicAdd(&sh.red.counters[6], rmLocal);
+    atomicAdd(&sh.red.counters[7], wrLocal);
     __syncthreads();
-    removals = sh.counters[6]; wraps = sh.counters[7];
-    uint32_t safetyAll = sh.counters[5];
+    removals = sh.red.counters[6]; wraps = sh.red.counters[7];
+    uint32_t safetyAll = sh.red.counters[5];
     // Error precedence (NativeMetalSolver.swift:1026-1040) and the host finite check (:1067).
     if (status == BP_OK) {
         if (c.limit) status = BP_ERR_SUBSTEP_LIMIT_EXCEEDED;
@@ -1004,10 +898,10 @@ extern "C" __attribute__((visibility("hidden"))) hipError_t bp_launch_world_step
                                            const uint32_t *order, uint32_t count, hipStream_t stream) {
     if (count == 0) return hipSuccess;
     switch (block) {
-        case 256: hipLaunchKernelGGL(bpd::world_step_kernel<256>, dim3(count), dim3(256), 0, stream, *arena, descs, stats, order, count); break;
-        case 512: hipLaunchKernelGGL(bpd::world_step_kernel<512>, dim3(count), dim3(512), 0, stream, *arena, descs, stats, order, count); break;
-        case 1024: hipLaunchKernelGGL(bpd::world_step_kernel<1024>, dim3(count), dim3(1024), 0, stream, *arena, descs, stats, order, count); break;
-        default: hipLaunchKernelGGL(bpd::world_step_kernel<64>, dim3(count), dim3(64), 0, stream, *arena, descs, stats, order, count); break;
+        case 256: hipLaunchKernelGGL(bpd::world_step_kernel<256>, dim3(count), dim3(256), 0, stream, arena, descs, stats, order, count); break;
+        case 512: hipLaunchKernelGGL(bpd::world_step_kernel<512>, dim3(count), dim3(512), 0, stream, arena, descs, stats, order, count); break;
+        case 1024: hipLaunchKernelGGL(bpd::world_step_kernel<1024>, dim3(count), dim3(1024), 0, stream, arena, descs, stats, order, count); break;
+        default: hipLaunchKernelGGL(bpd::world_step_kernel<64>, dim3(count), dim3(64), 0, stream, arena, descs, stats, order, count); break;
     }
     return hipGetLastError();
 }

@@ -204,11 +204,16 @@ class Batch:
         return out
 
     def last_class_timing(self):
-        """(device milliseconds, world count) per workgroup-width class (64, 256, 512, 1024)."""
-        ms = np.zeros(4, np.float32)
-        worlds = np.zeros(4, np.uint32)
-        abi.check(self._lib.bp_batch_last_class_timing(self._handle, abi.ptr(ms), abi.ptr(worlds)), "bp_batch_last_class_timing")
-        return ms, worlds
+        """Per kernel class: (device milliseconds, worlds, threads per workgroup, bubble capacity)."""
+        k = self._lib.bp_batch_class_count()
+        ms = np.zeros(k, np.float32)
+        worlds, threads, cap = (np.zeros(k, np.uint32) for _ in range(3))
+        abi.check(self._lib.bp_batch_last_class_timing(self._handle, abi.ptr(ms), abi.ptr(worlds), abi.ptr(threads),
+                                                       abi.ptr(cap)), "bp_batch_last_class_timing")
+        return ms, worlds, threads, cap
+
+    def set_world_class(self, world: int, cls: int):
+        abi.check(self._lib.bp_batch_set_world_class(self._handle, world, cls), "bp_batch_set_world_class")
 
     @property
     def last_gpu_milliseconds(self) -> float:

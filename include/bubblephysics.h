@@ -218,6 +218,8 @@ typedef struct bp_frame_info {
     uint64_t limiter_object;         /* substepControl.limiter.x                        */
     uint32_t limiter_term;           /* substepControl.limiter.y                        */
     uint32_t adjacency_entries;      /* entries of the adjacency of the last grid build */
+    uint32_t kernel_class;           /* kernel configuration that ran the step (bp_batch classes) */
+    uint32_t reserved;
     bp_failure_detail detail;
 } bp_frame_info;
 
@@ -320,9 +322,15 @@ bp_status bp_batch_step(bp_batch *batch, uint32_t steps, bp_frame_info *infos);
 bp_status bp_batch_read_bubbles(bp_batch *batch, uint32_t world, bp_bubble *out, uint32_t count);
 /* Device time of the last bp_batch_step (HIP events around the launches). */
 float bp_batch_last_gpu_milliseconds(const bp_batch *batch);
-/* Per workgroup-width class (64, 256, 512, 1024 threads) of the last bp_batch_step: device time
- * of that class's launches (HIP events on its stream) and the number of worlds in it. */
-bp_status bp_batch_last_class_timing(const bp_batch *batch, float *milliseconds4, uint32_t *worlds4);
+/* Kernel configurations ("classes") a world can run in: LDS-resident configurations for small
+ * worlds and the global-state kernel. For each class of the last bp_batch_step: device time of its
+ * launches (HIP events on its stream), worlds in it, threads per workgroup and bubble capacity.
+ * Arrays hold bp_batch_class_count() entries; any may be null. */
+uint32_t bp_batch_class_count(void);
+bp_status bp_batch_last_class_timing(const bp_batch *batch, float *milliseconds, uint32_t *worlds, uint32_t *threads,
+                                     uint32_t *capacity);
+/* Pins a world to a class (-1 = automatic); used by the autotuner. */
+bp_status bp_batch_set_world_class(bp_batch *batch, uint32_t world, int32_t cls);
 
 #ifdef __cplusplus
 }

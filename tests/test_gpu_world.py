@@ -179,3 +179,21 @@ def test_pair_only_world_conserves_momentum_bitwise_in_forces(oracle):
     p = (b["mass"][:, None].astype(np.float64) * b["velocity"].astype(np.float64)).sum(axis=0)
     p0 = (scene.bubbles["mass"][:, None].astype(np.float64) * scene.bubbles["velocity"].astype(np.float64)).sum(axis=0)
     assert np.allclose(p, p0, rtol=1e-5, atol=1e-4)
+
+
+def test_every_kernel_configuration_gives_the_same_bits(oracle, hip_library):
+    # Autotuner safety: the LDS configurations and the global-state kernel are interchangeable.
+    rng = np.random.Generator(np.random.PCG64(5))
+    scene = scenes.batch_world(rng, 60, bodies_max=6)
+    classes = int(hip_library.bp_batch_class_count())
+    batch = solver.Batch([scene] * classes)
+    for k in range(classes):
+        batch.set_world_class(k, k)
+    steps = 5
+    infos = batch.step(steps)
+    cb, _, cf, _ = oracle.run(scene, steps)
+    assert sorted(set(int(x) for x in infos[0]["kernel_class"])) == list(range(classes))
+    for k in range(classes):
+        assert np.array_equal(infos[:, k]["substep_count"], cf["substeps"]), k
+        assert np.array_equal(infos[:, k]["candidate_pair_count"], cf["candidate_pairs"]), k
+        assert_same_state(batch.read_bubbles(k), cb, f"class {k}")
