@@ -26,7 +26,7 @@ extern "C" hipError_t bp_launch_world_step(int block, const bpd::Arena *arena, b
 extern "C" hipError_t bp_launch_world_lds(int id, const bpd::Arena *arena, bpd::WorldDesc *descs, bpd::WorldStatus *stats,
                                           const uint32_t *order, uint32_t count, hipStream_t stream);
 extern "C" int bp_lds_config_count();
-extern "C" void bp_lds_config(int id, uint32_t *threads, uint32_t *per, uint32_t *rowcap);
+extern "C" void bp_lds_config(int id, uint32_t *threads, uint32_t *per, uint32_t *rowcap, uint32_t *caps);
 
 namespace {
 
@@ -48,31 +48,27 @@ constexpr int kClassCount = 11;              // ten LDS configurations and the g
 constexpr int kGlobalClass = 10;
 constexpr int kStreams = 4;
 
-// Kernel configuration of a world: the LDS-resident kernel for up to 4,096 bubbles, the
-// global-state kernel above. Classes 0-9 are the LDS configurations of bp_world_lds.hip.
-int default_class_for(uint32_t n) {
-    if (n <= 64) return 0;
-    if (n <= 128) return 1;
-    if (n <= 256) return 2;
-    if (n <= 512) return 3;
-    if (n <= 1024) return 4;
-    if (n <= 2048) return 7;
-    if (n <= 4096) return 9;
-    return kGlobalClass;
-}
-
 uint32_t class_threads(int k) {
     if (k == kGlobalClass) return 1024;
     uint32_t t, p, r;
-    bp_lds_config(k, &t, &p, &r);
+    bp_lds_config(k, &t, &p, &r, nullptr);
     return t;
 }
 
 uint32_t class_capacity(int k) {
     if (k == kGlobalClass) return 0xffffffffu;
     uint32_t t, p, r;
-    bp_lds_config(k, &t, &p, &r);
+    bp_lds_config(k, &t, &p, &r, nullptr);
     return t * p;
+}
+
+// Whether a world of n bubbles with the given geometry fits configuration k (the global-state
+// kernel takes every world).
+bool class_fits(int k, size_t n, size_t segments, size_t bodies, size_t vertices, size_t triangles) {
+    if (k == kGlobalClass) return true;
+    uint32_t t, p, r, caps[4];
+    bp_lds_config(k, &t, &p, &r, caps);
+    return n <= (size_t)t * p && segments <= caps[0] && bodies <= caps[1] && vertices <= caps[2] && triangles <= caps[3];
 }
 
 uint32_t auto_buckets(uint32_t n) {
@@ -94,6 +90,20 @@ struct HostWorld {
     int forcedClass = -1;               // kernel configuration chosen by the tuner, -1 = default
     uint32_t exportAdjacency = 0;
 };
+
+bool class_fits(int k, const HostWorld &w) {
+    return class_fits(k, w.bubbles.size(), w.segments.size(), w.bodies.size(), w.vertices.size() / 2, w.triangles.size() / 4);
+}
+
+// Default kernel configuration of a world: the narrowest LDS configuration whose bubble and
+// geometry capacities hold it (Classes 0-9 are the LDS configurations of bp_world_lds.hip),
+// else the global-state kernel.
+int default_class_for(const HostWorld &w) {
+    static const int kBySize[] = {0, 1, 2, 3, 4, 7, 9};
+    for (int k : kBySize)
+        if (class_fits(k, w)) return k;
+    return kGlobalClass;
+}
 
 template <class T> struct DevBuf {
     T *ptr = nullptr;
@@ -268,6 +278,7 @@ bp_status bp_batch::pack() {
         d.segmentOffset = os; d.segmentCount = (uint32_t)w.segments.size();
         d.bodyOffset = obd; d.bodyCount = (uint32_t)w.bodies.size();
         d.vertexOffset = ov; d.triangleOffset = ot;
+        d.vertexCount = (uint32_t)(w.vertices.size() / 2); d.triangleCount = (uint32_t)(w.triangles.size() / 4);
         d.motorOffset = om_; d.motorCount = (uint32_t)w.motors.size();
         d.bandOffset = oband; d.bandCount = (uint32_t)w.bands.size();
         d.adjOffset = (uint32_t)oadj; d.adjCapacity = std::max<uint32_t>(n, 16) * w.adjPerBubble;
@@ -337,8 +348,8 @@ bp_status bp_batch::pack() {
         for (size_t v = 0; v < w.vertices.size() / 2; ++v) verts[ov + v] = make_float2(w.vertices[2 * v], w.vertices[2 * v + 1]);
         for (size_t t = 0; t < w.triangles.size() / 4; ++t)
             tris[ot + t] = make_uint4(w.triangles[4 * t], w.triangles[4 * t + 1], w.triangles[4 * t + 2], w.triangles[4 * t + 3]);
-        int cls = default_class_for(n);
-        if (w.forcedClass >= 0 && (uint32_t)n <= class_capacity(w.forcedClass)) cls = w.forcedClass;
+        int cls = default_class_for(w);
+        if (w.forcedClass >= 0 && class_fits(w.forcedClass, w)) cls = w.forcedClass;
         classWorlds[cls].push_back((uint32_t)wi);
         ob += n; os += (uint32_t)w.segments.size(); obd += (uint32_t)w.bodies.size();
         ov += (uint32_t)(w.vertices.size() / 2); ot += (uint32_t)(w.triangles.size() / 4);
@@ -664,7 +675,7 @@ bp_status bp_batch_set_world_class(bp_batch *b, uint32_t world, int32_t cls) {
     std::lock_guard<std::mutex> lock(b->mutex);
     bp_status s = b->download_state();
     if (s != BP_OK) return s;
-    if (cls >= 0 && b->worlds[world].bubbles.size() > class_capacity(cls)) return BP_ERR_INVALID_ARGUMENT;
+    if (cls >= 0 && !class_fits(cls, b->worlds[world])) return BP_ERR_INVALID_ARGUMENT;
     b->worlds[world].forcedClass = cls;
     b->packed = false;
     return BP_OK;

@@ -181,6 +181,7 @@ struct Geo {
         s.prevStart = f2(rg.x, rg.y);
         s.outward = rg.z;
         s.flags = A->segFlags[so + j];
+        s.base = segment_base(s.a, s.b);
         return s;
     }
     BPD BodyView body(uint32_t k) const {
@@ -217,7 +218,7 @@ struct Geo {
         return !(hi.x < bodyLow.x || hi.y < bodyLow.y || bodyHigh.x < lo.x || bodyHigh.y < lo.y);
     }
     // Non-contact acceleration a0 (Force:450-470): global force, band, motor.
-    BPD float2 non_contact(uint32_t flags, float invm, int32_t bandIndex, int32_t motorIndex, float2 p) const {
+    BPD float2 non_contact_with(uint32_t flags, float invm, int32_t bandIndex, bool hasMotor, float2 motorForce, float2 p) const {
         if (!(flags & kForceDriven) || !(invm > 0.0f)) return f2(0.0f, 0.0f);
         float2 force = globalForce;
         if (bandIndex >= 0) {
@@ -227,11 +228,13 @@ struct Geo {
             if (lim > 0.0f && magnitude > lim) bf = mul(bf, lim / magnitude);
             force = add(force, bf);
         }
-        if (motorIndex >= 0) {
-            float4 md = motor(motorIndex);
-            force = add(force, f2(md.x, md.y));
-        }
+        if (hasMotor) force = add(force, motorForce);
         return mul(force, invm);
+    }
+    BPD float2 non_contact(uint32_t flags, float invm, int32_t bandIndex, int32_t motorIndex, float2 p) const {
+        float2 mf = f2(0.0f, 0.0f);
+        if (motorIndex >= 0) { float4 md = motor(motorIndex); mf = f2(md.x, md.y); }
+        return non_contact_with(flags, invm, bandIndex, motorIndex >= 0, mf, p);
     }
 };
 
@@ -345,7 +348,7 @@ BPD bool assert_body_and_edges(const Geo &g, float2 center, float2 start, uint32
     for (uint32_t k = 0; k < g.nb; ++k) {
         BodyView body = g.body(k);
         if (!body_immovable(body) || !filter_accepts(cat, mask, body.kind.z, body.kind.w)) continue;
-        if (body_inside(center, body_at_fraction(body, fraction), g.verts(), g.tris())) {
+        if (frame_inside(center, body_frame(body, fraction), g.verts(), g.tris())) {
             out.kind = BP_VIOLATION_INSIDE_BODY; out.other = g.bodyId(k); out.segment = 0; out.detail = 0.0f;
             return true;
         }

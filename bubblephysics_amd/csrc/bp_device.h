@@ -74,6 +74,7 @@ struct WorldDesc {
     uint32_t segmentOffset, segmentCount;
     uint32_t bodyOffset, bodyCount;
     uint32_t vertexOffset, triangleOffset;
+    uint32_t vertexCount, triangleCount;
     uint32_t motorOffset, motorCount;
     uint32_t bandOffset, bandCount;
     uint32_t adjOffset, adjCapacity;
@@ -208,17 +209,23 @@ BPD bool past_segment_end(float2 center, float2 from, float2 to) {
 
 struct Seg {
     float2 a, b, prevStart;
+    float2 base;          // unit left normal of b - a, (0, 1) for a degenerate segment
     float outward;
     uint32_t flags;
 };
 constexpr uint32_t kSegPolygon = 2u, kSegConvexStart = 4u;
 
+BPD float2 segment_base(float2 a, float2 b) {
+    float2 edge = sub(b, a);
+    float edgeLength = len2(edge);
+    return edgeLength > 0.0f ? dvs(f2(-edge.y, edge.x), edgeLength) : f2(0.0f, 1.0f);
+}
+
+// Contact region of a segment or one-sided polygon edge (Force:112-140).
 BPD bool wall_region(const Seg &s, float2 center, float2 &normal, float &distanceValue) {
     float2 start = s.a, end = s.b, edge = sub(end, start);
-    float edgeLength = len2(edge);
-    float2 base = edgeLength > 0.0f ? dvs(f2(-edge.y, edge.x), edgeLength) : f2(0.0f, 1.0f);
     if (s.flags & kSegPolygon) {
-        float2 outward = mul(base, s.outward);
+        float2 outward = mul(s.base, s.outward);
         float along = dot2(sub(center, start), edge);
         bool convexStart = (s.flags & kSegConvexStart) != 0;
         if (convexStart && along <= 0.0f && past_segment_end(center, s.prevStart, start)) {
@@ -234,7 +241,7 @@ BPD bool wall_region(const Seg &s, float2 center, float2 &normal, float &distanc
     } else {
         float2 delta = sub(center, closest_point(center, s.a, s.b));
         distanceValue = len2(delta);
-        normal = distanceValue > 0.0f ? dvs(delta, distanceValue) : base;
+        normal = distanceValue > 0.0f ? dvs(delta, distanceValue) : s.base;
     }
     return true;
 }
@@ -254,45 +261,81 @@ constexpr uint32_t kBodyStatic = 1u, kBodyKinematic = 2u;
 
 BPD bool body_is_round(const BodyView &b) { return b.kind.x != 0u; }
 BPD bool body_immovable(const BodyView &b) { return (b.kind.y & (kBodyStatic | kBodyKinematic)) != 0u; }
-BPD float2 rotate(float2 p, float angle) {
-    float c = bpm::cosf_det(angle), s = bpm::sinf_det(angle);
-    return f2(p.x * c - p.y * s, p.x * s + p.y * c);
+
+// A body at one step fraction (Force:151-160): interpolated pose with the rotation evaluated
+// once, so that every vertex and every inside test of the body uses the same cos and sin.
+struct BodyFrame {
+    float2 at;            // interpolated position
+    float c, s;           // cos, sin of the interpolated angle (body to world)
+    float cn, sn;         // cos, sin of its negation (world to body)
+    float core;           // core radius of a round body
+    float4 motion;        // vx, vy, omega (clamped), damping
+    uint4 ranges;         // vertex offset/count, triangle offset/count
+    bool round;
+};
+
+// Interpolated position and the rotation terms (cos, sin, cos of -angle, sin of -angle).
+BPD void frame_pose(float4 pose, float2 prev, float fraction, float2 &at, float4 &cs) {
+    at = f2(prev.x + (pose.x - prev.x) * fraction, prev.y + (pose.y - prev.y) * fraction);
+    float angle = pose.w + (pose.z - pose.w) * fraction;
+    cs = make_float4(bpm::cosf_det(angle), bpm::sinf_det(angle), bpm::cosf_det(-angle), bpm::sinf_det(-angle));
 }
-BPD float2 body_vertex(const BodyView &b, const float2 *verts, uint32_t i) {
-    return add(f2(b.pose.x, b.pose.y), rotate(verts[b.ranges.x + i], b.pose.z));
+
+BPD BodyFrame body_frame(const BodyView &b, float fraction) {
+    BodyFrame f;
+    float4 cs;
+    frame_pose(b.pose, b.prev, fraction, f.at, cs);
+    f.c = cs.x; f.s = cs.y; f.cn = cs.z; f.sn = cs.w;
+    f.core = b.shape.y;
+    f.motion = b.motion;
+    f.ranges = b.ranges;
+    f.round = body_is_round(b);
+    return f;
 }
-BPD bool body_inside(float2 p, const BodyView &b, const float2 *verts, const uint4 *tris) {
-    if (body_is_round(b)) {
-        float2 a = body_vertex(b, verts, 0);
-        float2 c = b.ranges.y > 1 ? body_vertex(b, verts, 1) : a;
-        return dist2(p, closest_point(p, a, c)) <= b.shape.y;
+
+// Triangle corner indices from the global uint4 records or from 10-bit packed LDS words.
+BPD uint3 triangle_of(const uint4 *t, uint32_t i) { uint4 v = t[i]; return make_uint3(v.x, v.y, v.z); }
+BPD uint3 triangle_of(const uint32_t *t, uint32_t i) { uint32_t v = t[i]; return make_uint3(v & 1023u, (v >> 10) & 1023u, v >> 20); }
+
+template <class V>
+BPD float2 frame_vertex(const BodyFrame &f, V verts, uint32_t i) {
+    float2 p = verts[f.ranges.x + i];
+    return add(f.at, f2(p.x * f.c - p.y * f.s, p.x * f.s + p.y * f.c));
+}
+
+// Centre inside the body (Geometry:134-143, 467-490): round core distance, else the union of
+// the triangles in body coordinates.
+template <class V, class T>
+BPD bool frame_inside(float2 p, const BodyFrame &f, V verts, T tris) {
+    if (f.round) {
+        float2 a = frame_vertex(f, verts, 0);
+        float2 c = f.ranges.y > 1 ? frame_vertex(f, verts, 1) : a;
+        return dist2(p, closest_point(p, a, c)) <= f.core;
     }
-    float2 local = rotate(sub(p, f2(b.pose.x, b.pose.y)), -b.pose.z);
-    for (uint32_t i = 0; i < b.ranges.w; ++i) {
-        uint4 t = tris[b.ranges.z + i];
-        float2 a = verts[b.ranges.x + t.x], bb = verts[b.ranges.x + t.y], c = verts[b.ranges.x + t.z];
+    float2 q = sub(p, f.at);
+    float2 local = f2(q.x * f.cn - q.y * f.sn, q.x * f.sn + q.y * f.cn);
+    for (uint32_t i = 0; i < f.ranges.w; ++i) {
+        uint3 t = triangle_of(tris, f.ranges.z + i);
+        float2 a = verts[f.ranges.x + t.x], bb = verts[f.ranges.x + t.y], c = verts[f.ranges.x + t.z];
         if (cross2(sub(bb, a), sub(local, a)) >= 0.0f && cross2(sub(c, bb), sub(local, bb)) >= 0.0f &&
             cross2(sub(a, c), sub(local, c)) >= 0.0f)
             return true;
     }
     return false;
 }
-BPD BodyView body_at_fraction(BodyView b, float fraction) {
-    b.pose.x = b.prev.x + (b.pose.x - b.prev.x) * fraction;
-    b.pose.y = b.prev.y + (b.pose.y - b.prev.y) * fraction;
-    b.pose.z = b.pose.w + (b.pose.z - b.pose.w) * fraction;
-    return b;
-}
+
 struct Obstacle { float2 normal; float distanceValue; float speedBound; };
 
-BPD Obstacle body_obstacle(const BodyView &source, float fraction, const float2 *verts, const uint4 *tris, float2 center) {
-    BodyView body = body_at_fraction(source, fraction);
+// Body as an obstacle of a bubble centre (Force:162-210): normal, signed distance to the
+// surface and the speed bound of the nearest surface point.
+template <class V, class T>
+BPD Obstacle frame_obstacle(const BodyFrame &body, V verts, T tris, float2 center) {
     Obstacle result;
     float2 point, normal;
     float distanceValue;
-    if (body_is_round(body)) {
-        float2 a = body_vertex(body, verts, 0);
-        float2 c = body.ranges.y > 1 ? body_vertex(body, verts, 1) : a;
+    if (body.round) {
+        float2 a = frame_vertex(body, verts, 0);
+        float2 c = body.ranges.y > 1 ? frame_vertex(body, verts, 1) : a;
         point = closest_point(center, a, c);
         float2 delta = sub(center, point);
         float away = len2(delta);
@@ -300,13 +343,15 @@ BPD Obstacle body_obstacle(const BodyView &source, float fraction, const float2 
         float2 perp = f2(-axis.y, axis.x);
         float2 fallback = len2(axis) > 1e-6f ? dvs(perp, len2(perp)) : f2(0.0f, 1.0f);
         normal = away > 1e-7f ? dvs(delta, away) : fallback;
-        distanceValue = away - body.shape.y;
-        point = add(point, mul(normal, body.shape.y));
+        distanceValue = away - body.core;
+        point = add(point, mul(normal, body.core));
     } else {
         float best = inff();
         float2 nearest = f2(0.0f, 0.0f), outward = f2(0.0f, 1.0f);
-        for (uint32_t e = 0; e < body.ranges.y; ++e) {
-            float2 p = body_vertex(body, verts, e), q = body_vertex(body, verts, (e + 1) % body.ranges.y);
+        const uint32_t count = body.ranges.y;
+        float2 p = count > 0 ? frame_vertex(body, verts, 0) : f2(0.0f, 0.0f);
+        for (uint32_t e = 0; e < count; ++e) {
+            float2 q = frame_vertex(body, verts, e + 1 == count ? 0u : e + 1);
             float2 candidate = closest_point(center, p, q);
             float d = dist2(center, candidate);
             if (d < best) {
@@ -315,17 +360,22 @@ BPD Obstacle body_obstacle(const BodyView &source, float fraction, const float2 
                 float size = len2(edge);
                 outward = size > kFloatMin ? dvs(f2(edge.y, -edge.x), size) : f2(0.0f, 1.0f);
             }
+            p = q;
         }
-        bool inside = body_inside(center, body, verts, tris);
+        bool inside = frame_inside(center, body, verts, tris);
         normal = inside ? outward : (best > 1e-7f ? dvs(sub(center, nearest), best) : outward);
         distanceValue = inside ? -best : best;
         point = nearest;
     }
-    float2 arm = sub(point, f2(body.pose.x, body.pose.y));
+    float2 arm = sub(point, body.at);
     result.normal = normal;
     result.distanceValue = distanceValue;
     result.speedBound = len2(f2(body.motion.x, body.motion.y)) + fabsf(body.motion.z) * len2(arm);
     return result;
+}
+
+BPD Obstacle body_obstacle(const BodyView &source, float fraction, const float2 *verts, const uint4 *tris, float2 center) {
+    return frame_obstacle(body_frame(source, fraction), verts, tris, center);
 }
 
 // Exact half impulse with drag, Force:754-758.

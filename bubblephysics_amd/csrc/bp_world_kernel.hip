@@ -66,6 +66,7 @@ struct World {
         s.prevStart = f2(rg.x, rg.y);
         s.outward = rg.z;
         s.flags = A->segFlags[d->segmentOffset + j];
+        s.base = segment_base(s.a, s.b);
         return s;
     }
     BPD BodyView body(uint32_t k) const {
@@ -599,7 +600,7 @@ BPD bool assert_contacts(World &W, Shared<BLOCK> &sh, const Ctl &c, WorldStatus 
         for (uint32_t k = 0; k < W.nb && !found; ++k) {
             BodyView body = W.body(k);
             if (!body_immovable(body) || !filter_accepts(W.cat[i], W.mask[i], body.kind.z, body.kind.w)) continue;
-            if (body_inside(center, body_at_fraction(body, fraction), W.verts, W.tris)) {
+            if (frame_inside(center, body_frame(body, fraction), W.verts, W.tris)) {
                 found = true;
                 mine.kind = BP_VIOLATION_INSIDE_BODY; mine.other = W.A->bodyId[W.d->bodyOffset + k]; mine.detail = 0.0f;
             }
