@@ -370,6 +370,8 @@ template <class T> struct World {
     std::vector<uint32_t> demanded;
     std::vector<Candidate> candidates;        // compacted candidate list
     std::vector<T> stepHistory;
+    std::vector<uint32_t> stepTerms;          // limiting term of each substep of the step
+    std::vector<uint64_t> stepObjects;        // and the id of the bubble that set it (0 for the step's rest)
     SubstepControl<T> sc{};
     uint32_t overflow[4] = {0, 0, 0, 0};
     uint32_t safety = 0;
@@ -826,7 +828,11 @@ template <class T> struct World {
         sc.counts[0] += 1;
         sc.counts[1] = frameSubstep + 1;
         sc.limiter[0] = object; sc.limiter[1] = term;
-        if (frameSubstep < stepHistory.size()) stepHistory[frameSubstep] = h;
+        if (frameSubstep < stepHistory.size()) {
+            stepHistory[frameSubstep] = h;
+            stepTerms[frameSubstep] = (uint32_t)term;
+            stepObjects[frameSubstep] = object;
+        }
     }
 
     // Exact half impulse with drag, Force:754-758.
@@ -1009,6 +1015,8 @@ template <class T> struct World {
         acc.assign(n, v2(T(0), T(0))); cforce.assign(n, v2(T(0), T(0)));
         adjacency.assign(n, {}); demanded.assign(n, 0);
         stepHistory.assign(kFrameLimit, T(0));
+        stepTerms.assign(kFrameLimit, 0u);
+        stepObjects.assign(kFrameLimit, 0u);
         std::memset(overflow, 0, sizeof overflow);
         safety = 0; assertion = Assertion{}; candidateCount = 0; candidates.clear();
         sc = SubstepControl<T>{};
@@ -1196,7 +1204,8 @@ int bpo_world_run(const bp_world_config *config, bp_bubble *bubbles, uint32_t n,
                   bp_body *bodies, uint32_t nb, const float *vertices, uint32_t nv, const uint32_t *triangles, uint32_t nt,
                   const bp_motor *motors, uint32_t nm, const bp_band *bands, uint32_t nbands, uint32_t steps,
                   int precision, uint32_t assert_mode, uint32_t hashed_override, bpo_frame *frames,
-                  uint32_t *row_offsets, uint32_t *entries, uint32_t entries_capacity, float *step_history) {
+                  uint32_t *row_offsets, uint32_t *entries, uint32_t entries_capacity, float *step_history,
+                  uint32_t *step_terms, uint64_t *step_objects) {
     auto run = [&](auto &w) -> int {
         bpo::load(w, config, bubbles, n, segments, ns, bodies, nb, vertices, nv, triangles, nt, motors, nm, bands, nbands);
         w.cfg.assertMode = assert_mode;
@@ -1231,6 +1240,10 @@ int bpo_world_run(const bp_world_config *config, bp_bubble *bubbles, uint32_t n,
         }
         if (step_history)
             for (uint32_t k = 0; k < bpo::kFrameLimit; ++k) step_history[k] = (float)w.stepHistory[k];
+        if (step_terms)
+            for (uint32_t k = 0; k < bpo::kFrameLimit; ++k) step_terms[k] = w.stepTerms[k];
+        if (step_objects)
+            for (uint32_t k = 0; k < bpo::kFrameLimit; ++k) step_objects[k] = w.stepObjects[k];
         return executed;
     };
     if (precision == 64) { bpo::World<double> w; return run(w); }

@@ -43,7 +43,7 @@ def load():
         lib = ctypes.CDLL(str(LIB_PATH))
         P, U32, I32 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int
         lib.bpo_world_run.argtypes = [P, P, U32, P, U32, P, U32, P, U32, P, U32, P, U32, P, U32, U32, I32, U32, U32, P, P, P,
-                                      U32, P]
+                                      U32, P, P, P]
         lib.bpo_world_run.restype = ctypes.c_int
         lib.bpo_epsilon_f32.argtypes = [P, P, U32]
         lib.bpo_wall_law_f32.argtypes = [P, P, U32]
@@ -75,8 +75,11 @@ def run(scene, steps: int | None = None, *, precision: int = 32, assert_mode: in
     if adjacency:
         row_offsets = np.zeros(n + 1, np.uint32)
         entries = np.zeros(max(n * 256, 16), np.uint32)
+    terms = objects = None
     if history:
         hist = np.zeros(1024, np.float32)
+        terms = np.zeros(1024, np.uint32)
+        objects = np.zeros(1024, np.uint64)
     verts = np.ascontiguousarray(scene.vertices, np.float32)
     tris = np.ascontiguousarray(scene.triangles, np.uint32)
     override = {"auto": 0, "hashed": 1, "sorted": 2}[grid]
@@ -84,13 +87,17 @@ def run(scene, steps: int | None = None, *, precision: int = 32, assert_mode: in
                                  _p(bodies), len(bodies), _p(verts), len(verts), _p(tris), len(tris), _p(scene.motors),
                                  len(scene.motors), _p(scene.bands), len(scene.bands), steps, precision, assert_mode,
                                  override, _p(frames), _p(row_offsets), _p(entries), 0 if entries is None else len(entries),
-                                 _p(hist))
+                                 _p(hist), _p(terms), _p(objects))
     extras = {"executed": executed}
     if adjacency:
         extras["row_offsets"] = row_offsets
         extras["entries"] = entries[: row_offsets[-1]].copy()
     if history:
+        # h, limiting term and limiting bubble id of every substep of the last executed step.
+        count = int(frames[executed - 1]["substeps"]) if executed else 0
         extras["history"] = hist
+        extras["terms"] = terms[:count].copy()
+        extras["objects"] = objects[:count].copy()
     return bubbles, bodies, frames[:executed], extras
 
 

@@ -6,6 +6,7 @@ import numpy as np
 import pytest
 
 from bubblephysics_amd import abi, scenes, solver
+import kernel_scenes
 
 pytestmark = pytest.mark.gpu
 
@@ -197,3 +198,24 @@ def test_every_kernel_configuration_gives_the_same_bits(oracle, hip_library):
         assert np.array_equal(infos[:, k]["substep_count"], cf["substeps"]), k
         assert np.array_equal(infos[:, k]["candidate_pair_count"], cf["candidate_pairs"]), k
         assert_same_state(batch.read_bubbles(k), cb, f"class {k}")
+
+
+@pytest.mark.parametrize("name", kernel_scenes.PARITY_SCENES)
+def test_reference_kernel_scenes_match_oracle_bitwise(oracle, name):
+    # The scenes of the reference's substep-kernel tests, whose closed forms pin the oracle
+    # (test_oracle_world.py), stepped on the GPU: same h history, limiter and state bits.
+    scene, steps = kernel_scenes.parity_scene(name)
+    world = solver.NativeWorld(scene)
+    cur = scene.copy()
+    for k in range(steps):
+        frame = world.step(k)
+        cb, _, cf, ex = oracle.run(cur, 1, history=True)
+        c = cf[0]
+        assert (frame.status, frame.substep_count, frame.grid_rebuild_count) == (c["status"], c["substeps"], c["rebuilds"]), k
+        if frame.status != abi.BP_OK:
+            break
+        assert frame.limiter_term == c["limiter_term"] and frame.limiter_object == c["limiter_object"], k
+        h = world.read_step_history()
+        assert np.array_equal(bits(h), bits(ex["history"][: c["substeps"]])), k
+        assert_same_state(frame.bubbles, cb, f"{name} step {k}")
+        cur.bubbles = cb
