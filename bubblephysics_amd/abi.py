@@ -48,6 +48,7 @@ BP_SEGMENT_HAS_OWNER, BP_SEGMENT_POLYGON_EDGE, BP_SEGMENT_CONVEX_START, BP_SEGME
 BP_SEGMENT_ALLOWED_SIDE = 16
 BP_SHAPE_POLYGON, BP_SHAPE_CIRCLE, BP_SHAPE_CAPSULE = 0, 1, 2
 BP_BODY_STATIC, BP_BODY_KINEMATIC = 1, 2
+BP_FIELD_DIRECTIONAL, BP_FIELD_RADIAL, BP_FIELD_VORTEX = 0, 1, 2
 
 
 class Bubble(ctypes.Structure):
@@ -89,6 +90,14 @@ class Band(ctypes.Structure):
                 ("maximum_force", ctypes.c_float)]
 
 
+class Field(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_int64), ("position", ctypes.c_float * 2), ("rotation", ctypes.c_float),
+                ("falloff", ctypes.c_float), ("direction", ctypes.c_float * 2), ("strength", ctypes.c_float),
+                ("radius", ctypes.c_float), ("kind", ctypes.c_uint32), ("vertex_offset", ctypes.c_uint32),
+                ("vertex_count", ctypes.c_uint32), ("filter_offset", ctypes.c_uint32), ("filter_count", ctypes.c_uint32),
+                ("has_filter", ctypes.c_uint32)]
+
+
 class WorldConfig(ctypes.Structure):
     _fields_ = [
         ("time_step", ctypes.c_float), ("contact_tolerance", ctypes.c_float), ("separation_tolerance", ctypes.c_float),
@@ -128,6 +137,9 @@ class BatchWorldDesc(ctypes.Structure):
         ("triangles", ctypes.c_void_p), ("triangle_count", ctypes.c_uint32),
         ("motors", ctypes.c_void_p), ("motor_count", ctypes.c_uint32),
         ("bands", ctypes.c_void_p), ("band_count", ctypes.c_uint32),
+        ("fields", ctypes.c_void_p), ("field_count", ctypes.c_uint32),
+        ("field_vertices", ctypes.c_void_p), ("field_vertex_count", ctypes.c_uint32),
+        ("field_filter_ids", ctypes.c_void_p), ("field_filter_count", ctypes.c_uint32),
     ]
 
 
@@ -137,6 +149,7 @@ SEGMENT_DTYPE = np.dtype(Segment)
 BODY_DTYPE = np.dtype(Body)
 MOTOR_DTYPE = np.dtype(Motor)
 BAND_DTYPE = np.dtype(Band)
+FIELD_DTYPE = np.dtype(Field)
 FRAME_DTYPE = np.dtype(FrameInfo)
 
 # Every exported function of the C ABI with its argument and result types.
@@ -156,6 +169,7 @@ SIGNATURES = {
     "bp_world_set_segments": ([_P, _P, _U32], _I32),
     "bp_world_set_bodies": ([_P, _P, _U32, _P, _U32, _P, _U32], _I32),
     "bp_world_set_forces": ([_P, _P, _U32, _P, _U32], _I32),
+    "bp_world_set_fields": ([_P, _P, _U32, _P, _U32, _P, _U32], _I32),
     "bp_world_step": ([_P, _I32, _P], _I32),
     "bp_world_read_bubbles": ([_P, _P, _U32], _I32),
     "bp_world_read_bodies": ([_P, _P, _U32], _I32),

@@ -86,6 +86,9 @@ struct HostWorld {
     std::vector<uint32_t> triangles;    // 4 per triangle
     std::vector<bp_motor> motors;
     std::vector<bp_band> bands;
+    std::vector<bp_field> fields;
+    std::vector<float> fieldVertices;   // float2 pairs
+    std::vector<int64_t> fieldIds;      // filter ids, ascending within each field's range
     uint32_t adjPerBubble = kAdjPerBubble;
     int forcedClass = -1;               // kernel configuration chosen by the tuner, -1 = default
     uint32_t exportAdjacency = 0;
@@ -205,7 +208,20 @@ bp_status validate_world(const HostWorld &w) {
                     set_error("triangle index outside its body's vertices");
                     return BP_ERR_INVALID_SNAPSHOT;
                 }
-    for (const bp_motor &m : w.motors) (void)m;
+    for (const bp_field &f : w.fields) {
+        if (f.kind > BP_FIELD_VORTEX) { set_error("unknown force field kind"); return BP_ERR_INVALID_SNAPSHOT; }
+        if ((uint64_t)f.vertex_offset + f.vertex_count > w.fieldVertices.size() / 2 ||
+            (f.has_filter && (uint64_t)f.filter_offset + f.filter_count > w.fieldIds.size())) {
+            set_error("force field range outside the field vertex or filter arrays");
+            return BP_ERR_INVALID_SNAPSHOT;
+        }
+        if (f.has_filter)
+            for (uint32_t k = 1; k < f.filter_count; ++k)
+                if (!((uint64_t)w.fieldIds[f.filter_offset + k - 1] < (uint64_t)w.fieldIds[f.filter_offset + k])) {
+                    set_error("force field filter ids must be strictly increasing");
+                    return BP_ERR_INVALID_SNAPSHOT;
+                }
+    }
     if (w.bubbles.size() > (1u << 30) || w.segments.size() > (1u << 30) || w.bodies.size() > (1u << 30)) {
         set_error("world too large"); return BP_ERR_CAPACITY_OVERFLOW;
     }
@@ -216,7 +232,7 @@ bp_status validate_world(const HostWorld &w) {
 
 bp_status bp_batch::pack() {
     release();
-    uint32_t nb = 0, ns = 0, nbod = 0, nv = 0, nt = 0, nm = 0, nband = 0;
+    uint32_t nb = 0, ns = 0, nbod = 0, nv = 0, nt = 0, nm = 0, nband = 0, nf = 0, nfv = 0, nfi = 0;
     uint64_t adj = 0;
     for (auto &w : worlds) {
         bp_status s = validate_world(w);
@@ -224,6 +240,7 @@ bp_status bp_batch::pack() {
         nb += (uint32_t)w.bubbles.size(); ns += (uint32_t)w.segments.size(); nbod += (uint32_t)w.bodies.size();
         nv += (uint32_t)(w.vertices.size() / 2); nt += (uint32_t)(w.triangles.size() / 4);
         nm += (uint32_t)w.motors.size(); nband += (uint32_t)w.bands.size();
+        nf += (uint32_t)w.fields.size(); nfv += (uint32_t)(w.fieldVertices.size() / 2); nfi += (uint32_t)w.fieldIds.size();
         adj += (uint64_t)std::max<size_t>(w.bubbles.size(), 16) * w.adjPerBubble;
     }
     if (adj > 0xffffffffull) { set_error("adjacency capacity exceeds 2^32 entries"); return BP_ERR_CAPACITY_OVERFLOW; }
@@ -249,6 +266,7 @@ bp_status bp_batch::pack() {
     BP_ALLOC(A.bodyRanges, nbod); BP_ALLOC(A.bodyKind, nbod); BP_ALLOC(A.bodyId, nbod);
     BP_ALLOC(A.verts, nv); BP_ALLOC(A.tris, nt);
     BP_ALLOC(A.motors, nm); BP_ALLOC(A.bands, nband);
+    BP_ALLOC(A.fields, nf); BP_ALLOC(A.fieldVerts, nfv); BP_ALLOC(A.fieldIds, nfi);
     BP_ALLOC(A.history, (size_t)worlds.size() * bpd::kFrameLimit);
     BP_ALLOC(descs, worlds.size());
     BP_ALLOC(arenaDev, 1);
@@ -265,9 +283,12 @@ bp_status bp_batch::pack() {
     std::vector<uint32_t> segFlags(ns), segCat(ns), segMask(ns);
     std::vector<float2> bodyLimits(nbod);
     std::vector<uint4> bodyRanges(nbod), bodyKind(nbod), tris(nt);
+    std::vector<bpd::DevField> fields(nf);
+    std::vector<float2> fieldVerts(nfv);
+    std::vector<uint64_t> fieldIds(nfi);
     std::vector<uint64_t> bodyId(nbod);
     hostDescs.assign(worlds.size(), bpd::WorldDesc{});
-    uint32_t ob = 0, os = 0, obd = 0, ov = 0, ot = 0, om_ = 0, oband = 0;
+    uint32_t ob = 0, os = 0, obd = 0, ov = 0, ot = 0, om_ = 0, oband = 0, of = 0, ofv = 0, ofi = 0;
     uint64_t oadj = 0;
     for (auto &k : classWorlds) k.clear();
     for (size_t wi = 0; wi < worlds.size(); ++wi) {
@@ -281,6 +302,7 @@ bp_status bp_batch::pack() {
         d.vertexCount = (uint32_t)(w.vertices.size() / 2); d.triangleCount = (uint32_t)(w.triangles.size() / 4);
         d.motorOffset = om_; d.motorCount = (uint32_t)w.motors.size();
         d.bandOffset = oband; d.bandCount = (uint32_t)w.bands.size();
+        d.fieldOffset = of; d.fieldCount = (uint32_t)w.fields.size(); d.fieldVertexOffset = ofv; d.fieldIdOffset = ofi;
         d.adjOffset = (uint32_t)oadj; d.adjCapacity = std::max<uint32_t>(n, 16) * w.adjPerBubble;
         d.historyOffset = (uint32_t)(wi * bpd::kFrameLimit);
         d.pub = 0;
@@ -326,6 +348,19 @@ bp_status bp_batch::pack() {
                                        [](const bp_bubble &b, int64_t id) { return b.id < id; });
             if (it != w.bubbles.end() && it->id == bs[k].bubble_id) bandIdx[ob + (it - w.bubbles.begin())] = (int32_t)k;
         }
+        // Fields act in ascending id order (NativeForceInput sorts its objects, NativeForceRecords.swift:187).
+        std::vector<bp_field> fs = w.fields;
+        std::stable_sort(fs.begin(), fs.end(), [](const bp_field &a, const bp_field &b) { return a.id < b.id; });
+        for (uint32_t k = 0; k < fs.size(); ++k) {
+            const bp_field &f = fs[k];
+            fields[of + k].pose = make_float4(f.position[0], f.position[1], f.rotation, f.falloff);
+            fields[of + k].strength = make_float4(f.direction[0], f.direction[1], f.strength, f.radius);
+            fields[of + k].shape = make_uint4(f.kind, f.vertex_offset, f.vertex_count, 0u);
+            fields[of + k].filter = make_uint4(f.filter_offset, f.filter_count, f.has_filter ? 1u : 0u, 0u);
+        }
+        for (size_t v = 0; v < w.fieldVertices.size() / 2; ++v)
+            fieldVerts[ofv + v] = make_float2(w.fieldVertices[2 * v], w.fieldVertices[2 * v + 1]);
+        for (size_t k = 0; k < w.fieldIds.size(); ++k) fieldIds[ofi + k] = (uint64_t)w.fieldIds[k];
         for (uint32_t j = 0; j < w.segments.size(); ++j) {
             const bp_segment &sg = w.segments[j];
             uint32_t g = os + j;
@@ -354,6 +389,7 @@ bp_status bp_batch::pack() {
         ob += n; os += (uint32_t)w.segments.size(); obd += (uint32_t)w.bodies.size();
         ov += (uint32_t)(w.vertices.size() / 2); ot += (uint32_t)(w.triangles.size() / 4);
         om_ += (uint32_t)w.motors.size(); oband += (uint32_t)w.bands.size();
+        of += (uint32_t)w.fields.size(); ofv += (uint32_t)(w.fieldVertices.size() / 2); ofi += (uint32_t)w.fieldIds.size();
         oadj += (uint64_t)d.adjCapacity;
     }
     // Longest-processing-time order inside each class: the largest worlds start first.
@@ -370,7 +406,8 @@ bp_status bp_batch::pack() {
         (s = upload(A.bodyPrev[0], bpose2)) || (s = upload(A.bodyMotion, bodyMotion)) || (s = upload(A.bodyShape, bodyShape)) ||
         (s = upload(A.bodyLimits, bodyLimits)) || (s = upload(A.bodyRanges, bodyRanges)) || (s = upload(A.bodyKind, bodyKind)) ||
         (s = upload(A.bodyId, bodyId)) || (s = upload(A.verts, verts)) || (s = upload(A.tris, tris)) ||
-        (s = upload(A.motors, motors)) || (s = upload(A.bands, bands)) || (s = upload(descs, hostDescs)))
+        (s = upload(A.motors, motors)) || (s = upload(A.bands, bands)) || (s = upload(A.fields, fields)) ||
+        (s = upload(A.fieldVerts, fieldVerts)) || (s = upload(A.fieldIds, fieldIds)) || (s = upload(descs, hostDescs)))
         return s;
     for (int k = 0; k < kClassCount; ++k)
         if ((s = upload(orders[k], classWorlds[k])) != BP_OK) return s;
@@ -541,7 +578,7 @@ void bp_abi_record_sizes(uint32_t *out) {
     if (!out) return;
     out[0] = sizeof(bp_bubble); out[1] = sizeof(bp_segment); out[2] = sizeof(bp_body); out[3] = sizeof(bp_motor);
     out[4] = sizeof(bp_band); out[5] = sizeof(bp_world_config); out[6] = sizeof(bp_frame_info);
-    out[7] = sizeof(bp_batch_world_desc);
+    out[7] = sizeof(bp_batch_world_desc); out[8] = sizeof(bp_field);
 }
 
 void bp_world_config_default(bp_world_config *c) {
@@ -602,7 +639,8 @@ static bp_status world_from_desc(HostWorld &w, const bp_batch_world_desc *d) {
     if (!d) return BP_ERR_INVALID_ARGUMENT;
     if ((d->bubble_count && !d->bubbles) || (d->segment_count && !d->segments) || (d->body_count && !d->bodies) ||
         (d->vertex_count && !d->vertices) || (d->triangle_count && !d->triangles) || (d->motor_count && !d->motors) ||
-        (d->band_count && !d->bands)) {
+        (d->band_count && !d->bands) || (d->field_count && !d->fields) || (d->field_vertex_count && !d->field_vertices) ||
+        (d->field_filter_count && !d->field_filter_ids)) {
         set_error("null array with a non-zero count");
         return BP_ERR_INVALID_ARGUMENT;
     }
@@ -614,6 +652,9 @@ static bp_status world_from_desc(HostWorld &w, const bp_batch_world_desc *d) {
     w.triangles.assign(d->triangles, d->triangles + 4 * (size_t)d->triangle_count);
     w.motors.assign(d->motors, d->motors + d->motor_count);
     w.bands.assign(d->bands, d->bands + d->band_count);
+    w.fields.assign(d->fields, d->fields + d->field_count);
+    w.fieldVertices.assign(d->field_vertices, d->field_vertices + 2 * (size_t)d->field_vertex_count);
+    w.fieldIds.assign(d->field_filter_ids, d->field_filter_ids + d->field_filter_count);
     return validate_world(w);
 }
 
@@ -783,6 +824,19 @@ bp_status bp_world_set_forces(bp_world *w, const bp_motor *motors, uint32_t moto
     HostWorld &hw = host_of(w);
     hw.motors.assign(motors, motors + motor_count);
     hw.bands.assign(bands, bands + band_count);
+    return BP_OK;
+}
+
+bp_status bp_world_set_fields(bp_world *w, const bp_field *fields, uint32_t field_count, const float *vertices,
+                              uint32_t vertex_count, const int64_t *filter_ids, uint32_t filter_count) {
+    if (!w || (field_count && !fields) || (vertex_count && !vertices) || (filter_count && !filter_ids)) return BP_ERR_INVALID_ARGUMENT;
+    if (w->inFlight) return BP_ERR_FRAME_IN_FLIGHT;
+    bp_status s = sync_host(w);
+    if (s != BP_OK) return s;
+    HostWorld &hw = host_of(w);
+    hw.fields.assign(fields, fields + field_count);
+    hw.fieldVertices.assign(vertices, vertices + 2 * (size_t)vertex_count);
+    hw.fieldIds.assign(filter_ids, filter_ids + filter_count);
     return BP_OK;
 }
 

@@ -159,6 +159,7 @@ struct Geo {
     const Arena *A;
     uint32_t ns, nb;
     uint32_t so, bo, vo, to, mo, bao;   // segment, body, vertex, triangle, motor, band offsets
+    uint32_t fo, nf, fvo, fio;          // field offset and count, field-vertex and field-id offsets
     uint32_t side;                      // working side of the body poses
     float2 globalForce;
     float sepTol;
@@ -218,9 +219,18 @@ struct Geo {
         return !(hi.x < bodyLow.x || hi.y < bodyLow.y || bodyHigh.x < lo.x || bodyHigh.y < lo.y);
     }
     // Non-contact acceleration a0 (Force:450-470): global force, band, motor.
-    BPD float2 non_contact_with(uint32_t flags, float invm, int32_t bandIndex, bool hasMotor, float2 motorForce, float2 p) const {
+    // idp points at the bubble's id; it is read only when the world has force fields.
+    BPD float2 non_contact_with(uint32_t flags, float invm, const int64_t *idp, int32_t bandIndex, bool hasMotor, float2 motorForce,
+                                float2 p) const {
         if (!(flags & kForceDriven) || !(invm > 0.0f)) return f2(0.0f, 0.0f);
         float2 force = globalForce;
+        if (nf > 0) {
+            const uint64_t id = (uint64_t)*idp;
+            for (uint32_t k = 0; k < nf; ++k) {
+                const DevField f = A->fields[fo + k];
+                if (field_affects(f, A->fieldIds + fio, id)) force = add(force, field_force(f, A->fieldVerts + fvo, p));
+            }
+        }
         if (bandIndex >= 0) {
             float4 bd = band(bandIndex);
             float2 bf = mul(sub(f2(bd.x, bd.y), p), bd.z);
@@ -231,10 +241,10 @@ struct Geo {
         if (hasMotor) force = add(force, motorForce);
         return mul(force, invm);
     }
-    BPD float2 non_contact(uint32_t flags, float invm, int32_t bandIndex, int32_t motorIndex, float2 p) const {
+    BPD float2 non_contact(uint32_t flags, float invm, const int64_t *idp, int32_t bandIndex, int32_t motorIndex, float2 p) const {
         float2 mf = f2(0.0f, 0.0f);
         if (motorIndex >= 0) { float4 md = motor(motorIndex); mf = f2(md.x, md.y); }
-        return non_contact_with(flags, invm, bandIndex, motorIndex >= 0, mf, p);
+        return non_contact_with(flags, invm, idp, bandIndex, motorIndex >= 0, mf, p);
     }
 };
 
@@ -244,6 +254,7 @@ BPD Geo make_geo(const Arena *A, const WorldDesc *d, uint32_t side) {
     g.ns = d->segmentCount; g.nb = d->bodyCount;
     g.so = d->segmentOffset; g.bo = d->bodyOffset; g.vo = d->vertexOffset; g.to = d->triangleOffset;
     g.mo = d->motorOffset; g.bao = d->bandOffset;
+    g.fo = d->fieldOffset; g.nf = d->fieldCount; g.fvo = d->fieldVertexOffset; g.fio = d->fieldIdOffset;
     g.side = side;
     g.globalForce = f2(d->globalForce[0], d->globalForce[1]);
     g.sepTol = d->sepTol;

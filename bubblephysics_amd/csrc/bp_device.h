@@ -19,6 +19,15 @@ namespace bpd {
 
 #define BPD __device__ __forceinline__
 
+// Force field (NativeForceField, NativeForceKernels.metal-source:11-17); vertex and filter
+// offsets are relative to the world's field-vertex and field-id arrays.
+struct DevField {
+    float4 pose;              // position.xy, rotation, falloff
+    float4 strength;          // directional force.xy, radial/vortex strength, radius
+    uint4 shape;              // kind, vertex offset, vertex count (0 = circle), 0
+    uint4 filter;             // id offset, id count, filter present, 0
+};
+
 // ---------------------------------------------------------------------------------------
 // Arena: structure-of-arrays device storage of every world of a batch.
 // Per-bubble dynamic state exists twice: side `pub` of a world is its published state,
@@ -64,8 +73,12 @@ struct Arena {
     uint4 *tris;
     float4 *motors;           // force.xy, torque, inertia
     float4 *bands;            // target.xy, stiffness, maximum force
+    DevField *fields;
+    float2 *fieldVerts;       // local vertices of polygon field ranges
+    uint64_t *fieldIds;       // filter ids, ascending within each field's range
     float *history;           // 1024 h values per world
 };
+
 
 // Per-world constants and the published side. Written by the host, read by the kernel;
 // `pub` and `failed` are updated by the kernel at the end of each step.
@@ -77,6 +90,7 @@ struct WorldDesc {
     uint32_t vertexCount, triangleCount;
     uint32_t motorOffset, motorCount;
     uint32_t bandOffset, bandCount;
+    uint32_t fieldOffset, fieldCount, fieldVertexOffset, fieldIdOffset;
     uint32_t adjOffset, adjCapacity;
     uint32_t historyOffset;
     uint32_t pub;
@@ -376,6 +390,47 @@ BPD Obstacle frame_obstacle(const BodyFrame &body, V verts, T tris, float2 cente
 
 BPD Obstacle body_obstacle(const BodyView &source, float fraction, const float2 *verts, const uint4 *tris, float2 center) {
     return frame_obstacle(body_frame(source, fraction), verts, tris, center);
+}
+
+// ---------------------------------------------------------------------------------------
+// Force fields, Force:389-421.
+// ---------------------------------------------------------------------------------------
+// Force of one field at a point: range test in the field's frame (circle or polygon by ray
+// crossings), then the directional force turned by the field's rotation, or the radial or vortex
+// force with the (1 - d/R)^falloff profile.
+BPD float2 field_force(const DevField &f, const float2 *verts, float2 point) {
+    float2 delta = sub(point, f2(f.pose.x, f.pose.y));
+    float c = bpm::cosf_det(f.pose.z), s = bpm::sinf_det(f.pose.z);
+    float2 local = f2(c * delta.x + s * delta.y, -s * delta.x + c * delta.y);
+    float distanceValue = len2(local), radius = f.strength.w;
+    if (f.shape.z == 0u) {
+        if (!(distanceValue <= radius)) return f2(0.0f, 0.0f);
+    } else {
+        bool inside = false;
+        const uint32_t start = f.shape.y, count = f.shape.z;
+        for (uint32_t i = 0; i < count; ++i) {
+            float2 a = verts[start + i], b = verts[start + (i + 1 == count ? 0u : i + 1)];
+            if ((a.y > local.y) != (b.y > local.y) && local.x < (b.x - a.x) * (local.y - a.y) / (b.y - a.y) + a.x) inside = !inside;
+        }
+        if (!inside) return f2(0.0f, 0.0f);
+    }
+    if (f.shape.x == 0u) return f2(c * f.strength.x - s * f.strength.y, s * f.strength.x + c * f.strength.y);
+    if (!(distanceValue > 0.0f)) return f2(0.0f, 0.0f);
+    float profile = f.pose.w == 0.0f ? 1.0f : bpm::powf_det(fmaxf(0.0f, 1.0f - distanceValue / radius), f.pose.w);
+    float scale = f.strength.z * profile / distanceValue;
+    return f.shape.x == 2u ? mul(f2(-delta.y, delta.x), scale) : mul(delta, scale);
+}
+
+// Whether a field acts on a bubble: no filter acts on every bubble, a filter only on its listed
+// ids (binary search over the ascending list); an empty filter acts on none.
+BPD bool field_affects(const DevField &f, const uint64_t *ids, uint64_t id) {
+    if (f.filter.z == 0u) return true;
+    uint32_t low = f.filter.x, end = f.filter.x + f.filter.y, high = end;
+    while (low < high) {
+        uint32_t mid = low + (high - low) / 2;
+        if (ids[mid] < id) low = mid + 1; else high = mid;
+    }
+    return low < end && ids[low] == id;
 }
 
 // Exact half impulse with drag, Force:754-758.

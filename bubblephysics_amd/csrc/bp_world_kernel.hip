@@ -82,25 +82,9 @@ struct World {
     }
 
     // Non-contact acceleration a0 at `p`, Force:450-470 (global force, band, motor).
-    BPD float2 non_contact(uint32_t i, float2 p) const {
-        float invm = inv_mass(i);
-        if (!(flags[i] & kForceDriven) || !(invm > 0.0f)) return f2(0.0f, 0.0f);
-        float2 force = globalForce;
-        int32_t bi = band[i];
-        if (bi >= 0) {
-            float4 bd = bands[bi];
-            float2 bf = mul(sub(f2(bd.x, bd.y), p), bd.z);
-            float lim = bd.w, magnitude = len2(bf);
-            if (lim > 0.0f && magnitude > lim) bf = mul(bf, lim / magnitude);
-            force = add(force, bf);
-        }
-        int32_t mi = motor[i];
-        if (mi >= 0) {
-            float4 md = motors[mi];
-            force = add(force, f2(md.x, md.y));
-        }
-        return mul(force, invm);
-    }
+    Geo g;                       // geometry and force records of the world
+
+    BPD float2 non_contact(uint32_t i, float2 p) const { return g.non_contact(flags[i], inv_mass(i), id + i, band[i], motor[i], p); }
 
     BPD int64_t cell_x(float2 p) const { return (int64_t)floorf(p.x / cellWidth); }
     BPD int64_t cell_y(float2 p) const { return (int64_t)floorf(p.y / cellWidth); }
@@ -674,6 +658,7 @@ __global__ __launch_bounds__(BLOCK) void world_step_kernel(const Arena *__restri
     W.verts = A->verts + d->vertexOffset; W.tris = A->tris + d->triangleOffset;
     W.cellWidth = 0.0f;
     W.globalForce = f2(d->globalForce[0], d->globalForce[1]);
+    W.g = make_geo(A, d, wside);
     W.sepTol = d->sepTol;
     const uint32_t n = W.n;
     float *history = A->history + d->historyOffset;

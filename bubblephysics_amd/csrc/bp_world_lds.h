@@ -225,7 +225,7 @@ __global__ __launch_bounds__(BLOCK) void world_lds_kernel(const Arena *__restric
         return f;
     };
     auto a0_of = [&](int s, float invm, float2 pi) __attribute__((always_inline)) -> float2 {
-        return g.non_contact_with(fl[s], invm, bi[s], (fl[s] & kHasMotor) != 0u, mf[s], pi);
+        return g.non_contact_with(fl[s], invm, ids + lane + s * BLOCK, bi[s], (fl[s] & kHasMotor) != 0u, mf[s], pi);
     };
     // Candidate tests against the staged geometry (Geometry:279-290, 361-379).
     auto segment_candidate = [&](float2 pc, float radius, float margin, uint32_t cat, uint32_t mask, uint32_t j)

@@ -42,6 +42,9 @@ class Scene:
     triangles: np.ndarray = field(default_factory=lambda: np.zeros((0, 4), np.uint32))
     motors: np.ndarray = field(default_factory=lambda: np.zeros(0, abi.MOTOR_DTYPE))
     bands: np.ndarray = field(default_factory=lambda: np.zeros(0, abi.BAND_DTYPE))
+    fields: np.ndarray = field(default_factory=lambda: np.zeros(0, abi.FIELD_DTYPE))
+    field_vertices: np.ndarray = field(default_factory=lambda: np.zeros((0, 2), np.float32))
+    field_filter_ids: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int64))
     steps: int = 1
     placed_without_overlap: int = 0
 
@@ -49,13 +52,15 @@ class Scene:
         c = abi.WorldConfig()
         ctypes.memmove(ctypes.byref(c), ctypes.byref(self.config), ctypes.sizeof(c))
         return Scene(self.name, c, self.bubbles.copy(), self.segments.copy(), self.bodies.copy(), self.vertices.copy(),
-                     self.triangles.copy(), self.motors.copy(), self.bands.copy(), self.steps, self.placed_without_overlap)
+                     self.triangles.copy(), self.motors.copy(), self.bands.copy(), self.fields.copy(), self.field_vertices.copy(),
+                     self.field_filter_ids.copy(), self.steps, self.placed_without_overlap)
 
     def desc(self) -> abi.BatchWorldDesc:
         """C descriptor pointing at this scene's arrays (the scene must outlive the call)."""
         d = abi.BatchWorldDesc()
         d.config = self.config
-        for name in ("bubbles", "segments", "bodies", "vertices", "triangles", "motors", "bands"):
+        for name in ("bubbles", "segments", "bodies", "vertices", "triangles", "motors", "bands", "fields", "field_vertices",
+                     "field_filter_ids"):
             arr = getattr(self, name)
             if not arr.flags["C_CONTIGUOUS"]:
                 arr = np.ascontiguousarray(arr)
@@ -68,6 +73,9 @@ class Scene:
         d.triangle_count = len(self.triangles)
         d.motor_count = len(self.motors)
         d.band_count = len(self.bands)
+        d.field_count = len(self.fields)
+        d.field_vertex_count = len(self.field_vertices)
+        d.field_filter_count = len(self.field_filter_ids)
         return d
 
 

@@ -94,6 +94,8 @@ class NativeWorld:
         self.set_segments(scene.segments)
         self.set_bodies(scene.bodies, scene.vertices, scene.triangles)
         self.set_forces(scene.motors, scene.bands)
+        if len(scene.fields):
+            self.set_fields(scene.fields, scene.field_vertices, scene.field_filter_ids)
 
     def set_config(self, cfg: abi.WorldConfig):
         abi.check(self._lib.bp_world_set_config(self._handle, ctypes.byref(cfg)), "bp_world_set_config")
@@ -124,6 +126,13 @@ class NativeWorld:
         m = np.ascontiguousarray(motors, abi.MOTOR_DTYPE)
         b = np.ascontiguousarray(bands, abi.BAND_DTYPE)
         abi.check(self._lib.bp_world_set_forces(self._handle, abi.ptr(m), len(m), abi.ptr(b), len(b)), "bp_world_set_forces")
+
+    def set_fields(self, fields: np.ndarray, vertices: np.ndarray, filter_ids: np.ndarray):
+        f = np.ascontiguousarray(fields, abi.FIELD_DTYPE)
+        v = np.ascontiguousarray(vertices, np.float32).reshape(-1, 2)
+        ids = np.ascontiguousarray(filter_ids, np.int64)
+        abi.check(self._lib.bp_world_set_fields(self._handle, abi.ptr(f), len(f), abi.ptr(v), len(v), abi.ptr(ids), len(ids)),
+                  "bp_world_set_fields")
 
     def step(self, scenario_step: int = 0) -> SolverFrame:
         info = abi.FrameInfo()

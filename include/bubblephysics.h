@@ -169,6 +169,27 @@ typedef struct bp_band {
     float maximum_force;        /* 0 = no limit */
 } bp_band;
 
+/* Force field: NativeForceField (NativeForceKernels.metal-source:11-17), built as in
+ * NativeForceInput.appendField (NativeForceRecords.swift:212-239). Fields act in ascending id
+ * order on force-driven bubbles inside their range (a circle of `radius`, or a polygon given by
+ * `vertex_count` local vertices in the world's field-vertex array). */
+enum { BP_FIELD_DIRECTIONAL = 0u, BP_FIELD_RADIAL = 1u, BP_FIELD_VORTEX = 2u };
+typedef struct bp_field {
+    int64_t id;
+    float position[2];          /* pose.xy                                              */
+    float rotation;             /* pose.z                                               */
+    float falloff;              /* pose.w: exponent of (1 - d/R), 0 = constant          */
+    float direction[2];         /* strength.xy: force of a directional field (local)     */
+    float strength;             /* strength.z: radial or vortex strength                 */
+    float radius;               /* strength.w: circle radius or polygon fade radius      */
+    uint32_t kind;              /* BP_FIELD_*                                           */
+    uint32_t vertex_offset;     /* shape.y, into the world's field vertices             */
+    uint32_t vertex_count;      /* shape.z, 0 = circle                                  */
+    uint32_t filter_offset;     /* filter.x, into the world's filter ids (ascending)     */
+    uint32_t filter_count;      /* filter.y                                             */
+    uint32_t has_filter;        /* filter.z: 1 = acts only on the listed ids            */
+} bp_field;
+
 /* Step configuration: NativeSolverConfiguration (NativeSolverInput.swift:116-162) plus the
  * package switches of NativeMetalConfiguration (NativeMetalBackend.swift:2-34). */
 typedef struct bp_world_config {
@@ -228,8 +249,8 @@ typedef struct bp_frame_info {
 /* ------------------------------------------------------------------------ */
 uint32_t bp_abi_version(void);
 /* sizeof of bp_bubble, bp_segment, bp_body, bp_motor, bp_band, bp_world_config, bp_frame_info,
- * bp_batch_world_desc, in that order (the role of NativeMetalShaderABITests.swift:61-381). */
-void bp_abi_record_sizes(uint32_t *out8);
+ * bp_batch_world_desc, bp_field, in that order (the role of NativeMetalShaderABITests.swift:61-381). */
+void bp_abi_record_sizes(uint32_t *out9);
 /* Fills a configuration with the reference defaults (NativeSolverConfiguration.init,
  * NativeSolverInput.swift:128-143; NativeMetalConfiguration, NativeMetalBackend.swift:2-34). */
 void bp_world_config_default(bp_world_config *config);
@@ -255,6 +276,10 @@ bp_status bp_world_set_bodies(bp_world *world, const bp_body *bodies, uint32_t c
                               const uint32_t *triangles, uint32_t triangle_count);
 bp_status bp_world_set_forces(bp_world *world, const bp_motor *motors, uint32_t motor_count,
                               const bp_band *bands, uint32_t band_count);
+/* Force fields with their polygon vertices (float2 pairs) and filter ids (NativeForceInput
+ * fields, fieldVertices and fieldObjectIDs, NativeForceRecords.swift:177-183). */
+bp_status bp_world_set_fields(bp_world *world, const bp_field *fields, uint32_t field_count, const float *vertices,
+                              uint32_t vertex_count, const int64_t *filter_ids, uint32_t filter_count);
 /* One step of dt (executeFrame). On any error the published state is unchanged. */
 bp_status bp_world_step(bp_world *world, int32_t scenario_step, bp_frame_info *out_info);
 bp_status bp_world_read_bubbles(bp_world *world, bp_bubble *out, uint32_t count);
@@ -309,6 +334,9 @@ typedef struct bp_batch_world_desc {
     const uint32_t *triangles;  uint32_t triangle_count;   /* uint4 records (a, b, c, 0) */
     const bp_motor *motors;     uint32_t motor_count;
     const bp_band *bands;       uint32_t band_count;
+    const bp_field *fields;     uint32_t field_count;
+    const float *field_vertices; uint32_t field_vertex_count;   /* float2 pairs */
+    const int64_t *field_filter_ids; uint32_t field_filter_count;
 } bp_batch_world_desc;
 
 bp_status bp_batch_create(bp_batch **out_batch);

@@ -119,6 +119,12 @@ template <class T> struct Body {              // NativeRigidBody, Geometry:35-41
 
 template <class T> struct Motor { int64_t id; V2<T> force; T torque, inertia; };
 template <class T> struct Band { int64_t id; V2<T> target; T K, maxForce; };
+template <class T> struct Field {              // NativeForceField, Force:11-17
+    int64_t id;
+    V2<T> position; T rotation, falloff;
+    V2<T> direction; T strength, radius;
+    uint32_t kind, vOff, vCnt, fOff, fCnt, hasFilter;
+};
 
 struct Candidate {                            // NativeCandidate, Geometry:30
     int64_t pa, pb;                           // participants
@@ -362,6 +368,9 @@ template <class T> struct World {
     std::vector<uint32_t> tris;               // 4 per triangle
     std::vector<Motor<T>> motors;             // sorted by id
     std::vector<Band<T>> bands;               // sorted by id
+    std::vector<Field<T>> fields;             // sorted by id
+    std::vector<V2<T>> fieldVerts;
+    std::vector<uint64_t> fieldIds;           // ascending within each field's filter range
 
     // Working state of the step (NativeMetalPersistentState buffers).
     std::vector<T> margin;
@@ -398,10 +407,40 @@ template <class T> struct World {
         if (lim > 0 && magnitude > lim) force = force * (lim / magnitude);
         return force;
     }
+    // Field force at a point, Force:389-409.
+    V2<T> field_force(const Field<T> &f, V2<T> point) const {
+        V2<T> delta = point - f.position;
+        T c = Lim<T>::cos(f.rotation), s = Lim<T>::sin(f.rotation);
+        V2<T> local = v2(c * delta.x + s * delta.y, -s * delta.x + c * delta.y);
+        T distanceValue = length(local), radius = f.radius;
+        if (f.vCnt == 0) {
+            if (!(distanceValue <= radius)) return v2(T(0), T(0));
+        } else {
+            bool inside = false;
+            for (uint32_t i = 0; i < f.vCnt; ++i) {
+                V2<T> a = fieldVerts[f.vOff + i], b = fieldVerts[f.vOff + (i + 1) % f.vCnt];
+                if ((a.y > local.y) != (b.y > local.y) && local.x < (b.x - a.x) * (local.y - a.y) / (b.y - a.y) + a.x) inside = !inside;
+            }
+            if (!inside) return v2(T(0), T(0));
+        }
+        if (f.kind == 0) return v2(c * f.direction.x - s * f.direction.y, s * f.direction.x + c * f.direction.y);
+        if (!(distanceValue > 0)) return v2(T(0), T(0));
+        T profile = f.falloff == 0 ? T(1) : Lim<T>::pow(vmax(T(0), 1 - distanceValue / radius), f.falloff);
+        T scale = f.strength * profile / distanceValue;
+        return f.kind == 2 ? v2(-delta.y, delta.x) * scale : delta * scale;
+    }
+    bool field_affects(const Field<T> &f, uint64_t id) const {              // Force:412-421
+        if (!f.hasFilter) return true;
+        auto begin = fieldIds.begin() + f.fOff, end = begin + f.fCnt;
+        auto it = std::lower_bound(begin, end, id);
+        return it != end && *it == id;
+    }
     V2<T> non_contact_acceleration(const Bubble<T> &x, V2<T> position) const {   // Force:450-470
         T invm = inverse_mass(x);
         if (!x.forceDriven || !(invm > 0)) return v2(T(0), T(0));
         V2<T> force = v2(T(cfg.globalForce[0]), T(cfg.globalForce[1]));
+        for (const Field<T> &f : fields)
+            if (field_affects(f, (uint64_t)x.id)) force = force + field_force(f, position);
         if (!bands.empty()) {
             if (const Band<T> *band = find_band(x.id)) force = force + band_force(*band, position);
         }
@@ -1094,7 +1133,9 @@ template <class T> struct World {
 template <class T> void load(World<T> &w, const bp_world_config *config, const bp_bubble *bubbles, uint32_t n,
                              const bp_segment *segments, uint32_t ns, const bp_body *bodies, uint32_t nb,
                              const float *vertices, uint32_t nv, const uint32_t *triangles, uint32_t nt,
-                             const bp_motor *motors, uint32_t nm, const bp_band *bands, uint32_t nbands) {
+                             const bp_motor *motors, uint32_t nm, const bp_band *bands, uint32_t nbands,
+                             const bp_field *fields, uint32_t nf, const float *fieldVertices, uint32_t nfv,
+                             const int64_t *fieldIds, uint32_t nfi) {
     Config &c = w.cfg;
     c.dt = config->time_step; c.contactTol = config->contact_tolerance; c.sepTol = config->separation_tolerance;
     for (int k = 0; k < 4; ++k) c.region[k] = config->region[k];
@@ -1154,6 +1195,18 @@ template <class T> void load(World<T> &w, const bp_world_config *config, const b
         w.bands[i] = Band<T>{bands[i].bubble_id, v2(T(bands[i].target[0]), T(bands[i].target[1])), T(bands[i].stiffness), T(bands[i].maximum_force)};
     std::sort(w.motors.begin(), w.motors.end(), [](const Motor<T> &a, const Motor<T> &b) { return a.id < b.id; });
     std::sort(w.bands.begin(), w.bands.end(), [](const Band<T> &a, const Band<T> &b) { return a.id < b.id; });
+    w.fields.resize(nf);
+    for (uint32_t i = 0; i < nf; ++i) {
+        const bp_field &x = fields[i];
+        w.fields[i] = Field<T>{x.id, v2(T(x.position[0]), T(x.position[1])), T(x.rotation), T(x.falloff),
+                               v2(T(x.direction[0]), T(x.direction[1])), T(x.strength), T(x.radius),
+                               x.kind, x.vertex_offset, x.vertex_count, x.filter_offset, x.filter_count, x.has_filter ? 1u : 0u};
+    }
+    std::stable_sort(w.fields.begin(), w.fields.end(), [](const Field<T> &a, const Field<T> &b) { return a.id < b.id; });
+    w.fieldVerts.resize(nfv);
+    for (uint32_t i = 0; i < nfv; ++i) w.fieldVerts[i] = v2(T(fieldVertices[2 * i]), T(fieldVertices[2 * i + 1]));
+    w.fieldIds.resize(nfi);
+    for (uint32_t i = 0; i < nfi; ++i) w.fieldIds[i] = (uint64_t)fieldIds[i];
 }
 
 template <class T> void store(const World<T> &w, bp_bubble *bubbles, bp_body *bodies) {
@@ -1202,12 +1255,15 @@ typedef struct bpo_frame {
 // assert_mode: 0 = all-pairs loop of the reference, 1 = pruned with the same verdict.
 int bpo_world_run(const bp_world_config *config, bp_bubble *bubbles, uint32_t n, const bp_segment *segments, uint32_t ns,
                   bp_body *bodies, uint32_t nb, const float *vertices, uint32_t nv, const uint32_t *triangles, uint32_t nt,
-                  const bp_motor *motors, uint32_t nm, const bp_band *bands, uint32_t nbands, uint32_t steps,
+                  const bp_motor *motors, uint32_t nm, const bp_band *bands, uint32_t nbands,
+                  const bp_field *fields, uint32_t nf, const float *field_vertices, uint32_t nfv,
+                  const int64_t *field_ids, uint32_t nfi, uint32_t steps,
                   int precision, uint32_t assert_mode, uint32_t hashed_override, bpo_frame *frames,
                   uint32_t *row_offsets, uint32_t *entries, uint32_t entries_capacity, float *step_history,
                   uint32_t *step_terms, uint64_t *step_objects) {
     auto run = [&](auto &w) -> int {
-        bpo::load(w, config, bubbles, n, segments, ns, bodies, nb, vertices, nv, triangles, nt, motors, nm, bands, nbands);
+        bpo::load(w, config, bubbles, n, segments, ns, bodies, nb, vertices, nv, triangles, nt, motors, nm, bands, nbands,
+                  fields, nf, field_vertices, nfv, field_ids, nfi);
         w.cfg.assertMode = assert_mode;
         if (hashed_override == 1) w.cfg.hashedGrid = 1;
         if (hashed_override == 2) w.cfg.hashedGrid = 0;
@@ -1265,7 +1321,7 @@ void bpo_candidate_rows(const bp_bubble *bubbles, const float *margins, uint32_t
     cfg.separation_tolerance = separation_tolerance;
     cfg.candidate_limit = limit;
     cfg.grid_bucket_count = buckets;
-    bpo::load(w, &cfg, bubbles, n, segments, ns, nullptr, 0, nullptr, 0, nullptr, 0, nullptr, 0, nullptr, 0);
+    bpo::load(w, &cfg, bubbles, n, segments, ns, nullptr, 0, nullptr, 0, nullptr, 0, nullptr, 0, nullptr, 0, nullptr, 0, nullptr, 0, nullptr, 0);
     w.limit = limit;
     w.cfg.hashedGrid = hashed;
     w.margin.assign(margins, margins + n);
@@ -1315,5 +1371,19 @@ void bpo_detmath_f32(int which, const float *x, float *out, uint32_t n) {
     }
 }
 uint32_t bpo_abi_version(void) { return BP_ABI_VERSION; }
+
+// Non-contact acceleration a0 of every bubble at its centre (nativeAccelerationWithoutContacts,
+// Force:450-494), FP32, written to out[2n].
+void bpo_noncontact_acceleration(const bp_world_config *config, const bp_bubble *bubbles, uint32_t n, const bp_motor *motors,
+                                 uint32_t nm, const bp_band *bands, uint32_t nbands, const bp_field *fields, uint32_t nf,
+                                 const float *field_vertices, uint32_t nfv, const int64_t *field_ids, uint32_t nfi, float *out) {
+    bpo::World<float> w;
+    bpo::load(w, config, bubbles, n, nullptr, 0, nullptr, 0, nullptr, 0, nullptr, 0, motors, nm, bands, nbands, fields, nf,
+              field_vertices, nfv, field_ids, nfi);
+    for (uint32_t i = 0; i < n; ++i) {
+        bpo::V2<float> a = w.non_contact_acceleration(w.b[i], w.b[i].pos);
+        out[2 * i] = a.x; out[2 * i + 1] = a.y;
+    }
+}
 
 }  // extern "C"

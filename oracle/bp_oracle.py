@@ -42,8 +42,9 @@ def load():
             build()
         lib = ctypes.CDLL(str(LIB_PATH))
         P, U32, I32 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int
-        lib.bpo_world_run.argtypes = [P, P, U32, P, U32, P, U32, P, U32, P, U32, P, U32, P, U32, U32, I32, U32, U32, P, P, P,
-                                      U32, P, P, P]
+        lib.bpo_world_run.argtypes = [P, P, U32, P, U32, P, U32, P, U32, P, U32, P, U32, P, U32, P, U32, P, U32, P, U32,
+                                      U32, I32, U32, U32, P, P, P, U32, P, P, P]
+        lib.bpo_noncontact_acceleration.argtypes = [P, P, U32, P, U32, P, U32, P, U32, P, U32, P, U32, P]
         lib.bpo_world_run.restype = ctypes.c_int
         lib.bpo_epsilon_f32.argtypes = [P, P, U32]
         lib.bpo_wall_law_f32.argtypes = [P, P, U32]
@@ -82,10 +83,13 @@ def run(scene, steps: int | None = None, *, precision: int = 32, assert_mode: in
         objects = np.zeros(1024, np.uint64)
     verts = np.ascontiguousarray(scene.vertices, np.float32)
     tris = np.ascontiguousarray(scene.triangles, np.uint32)
+    fverts = np.ascontiguousarray(scene.field_vertices, np.float32).reshape(-1, 2)
+    fids = np.ascontiguousarray(scene.field_filter_ids, np.int64)
     override = {"auto": 0, "hashed": 1, "sorted": 2}[grid]
     executed = lib.bpo_world_run(ctypes.byref(scene.config), _p(bubbles), n, _p(scene.segments), len(scene.segments),
                                  _p(bodies), len(bodies), _p(verts), len(verts), _p(tris), len(tris), _p(scene.motors),
-                                 len(scene.motors), _p(scene.bands), len(scene.bands), steps, precision, assert_mode,
+                                 len(scene.motors), _p(scene.bands), len(scene.bands), _p(scene.fields), len(scene.fields),
+                                 _p(fverts), len(fverts), _p(fids), len(fids), steps, precision, assert_mode,
                                  override, _p(frames), _p(row_offsets), _p(entries), 0 if entries is None else len(entries),
                                  _p(hist), _p(terms), _p(objects))
     extras = {"executed": executed}
@@ -99,6 +103,18 @@ def run(scene, steps: int | None = None, *, precision: int = 32, assert_mode: in
         extras["terms"] = terms[:count].copy()
         extras["objects"] = objects[:count].copy()
     return bubbles, bodies, frames[:executed], extras
+
+
+def noncontact_acceleration(scene):
+    """a0 of every bubble of a scene at its centre (FP32), shape (n, 2)."""
+    n = len(scene.bubbles)
+    out = np.zeros((n, 2), np.float32)
+    fverts = np.ascontiguousarray(scene.field_vertices, np.float32).reshape(-1, 2)
+    fids = np.ascontiguousarray(scene.field_filter_ids, np.int64)
+    load().bpo_noncontact_acceleration(ctypes.byref(scene.config), _p(scene.bubbles), n, _p(scene.motors), len(scene.motors),
+                                       _p(scene.bands), len(scene.bands), _p(scene.fields), len(scene.fields), _p(fverts),
+                                       len(fverts), _p(fids), len(fids), _p(out))
+    return out
 
 
 def epsilon(q):

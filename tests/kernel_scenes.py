@@ -30,9 +30,18 @@ def wall_segments():
     return scenes.static_polygon_edges(SQUARE, 1, 100)
 
 
-def world(bodies, *, walls=False, global_force=(0.0, 0.0), dt=1.0 / 60.0, split=1, motors=(), bands=(),
+def field(id, kind, *, position=(0.0, 0.0), rotation=0.0, falloff=0.0, direction=(0.0, 0.0), strength=0.0, radius=None,
+          polygon=None, filter_ids=None):
+    """A force field as NativeForceInput.appendField builds it (NativeForceRecords.swift:212-239): a
+    circle of `radius`, or a polygon whose fade radius is its farthest vertex from the centre."""
+    return dict(id=id, kind=kind, position=position, rotation=rotation, falloff=falloff, direction=direction,
+                strength=strength, radius=radius, polygon=polygon, filter_ids=filter_ids)
+
+
+def world(bodies, *, walls=False, global_force=(0.0, 0.0), dt=1.0 / 60.0, split=1, motors=(), bands=(), fields=(),
           minimum_margin_only=0, region=(-1e5, -1e5, 1e5, 1e5)) -> scenes.Scene:
-    """A world of Rig bodies. motors: (bubble id, fx, fy, torque, inertia); bands: (bubble id, tx, ty, K, cap)."""
+    """A world of Rig bodies. motors: (bubble id, fx, fy, torque, inertia); bands: (bubble id, tx, ty, K, cap);
+    fields: records of field()."""
     cfg = scenes.make_config(region, dt=dt, apply_region=0)
     cfg.global_force[:] = [float(v) for v in global_force]
     cfg.substep_split = int(split)
@@ -69,7 +78,29 @@ def world(bodies, *, walls=False, global_force=(0.0, 0.0), dt=1.0 / 60.0, split=
         bd[k]["target"] = (tx, ty)
         bd[k]["stiffness"] = stiff
         bd[k]["maximum_force"] = cap
-    scene = scenes.Scene("kernel-test", cfg, b, motors=m, bands=bd)
+    f = np.zeros(len(fields), abi.FIELD_DTYPE)
+    verts, ids = [], []
+    for k, spec in enumerate(fields):
+        f[k]["id"] = spec["id"]
+        f[k]["kind"] = spec["kind"]
+        f[k]["position"] = spec["position"]
+        f[k]["rotation"] = spec["rotation"]
+        f[k]["falloff"] = spec["falloff"]
+        f[k]["direction"] = spec["direction"]
+        f[k]["strength"] = spec["strength"]
+        if spec["polygon"] is not None:
+            pts = np.asarray(spec["polygon"], np.float32)
+            f[k]["vertex_offset"], f[k]["vertex_count"] = len(verts), len(pts)
+            verts.extend(pts.tolist())
+            f[k]["radius"] = float(np.max(np.sqrt((pts.astype(np.float32) ** 2).sum(axis=1, dtype=np.float32))))
+        else:
+            f[k]["radius"] = spec["radius"]
+        if spec["filter_ids"] is not None:
+            f[k]["has_filter"] = 1
+            f[k]["filter_offset"], f[k]["filter_count"] = len(ids), len(spec["filter_ids"])
+            ids.extend(sorted(spec["filter_ids"]))
+    scene = scenes.Scene("kernel-test", cfg, b, motors=m, bands=bd, fields=f,
+                         field_vertices=np.asarray(verts, np.float32).reshape(-1, 2), field_filter_ids=np.asarray(ids, np.int64))
     if walls:
         scene.segments = wall_segments()
     return scene
@@ -81,7 +112,7 @@ def relative(value, expected):
 
 PARITY_SCENES = (["wall-stiffness-2.0", "wall-stiffness-0.4", "wall-stiffness-3.0"] + [f"winner-{w}" for w in (0, 31, 64, 100, 129)]
                  + ["pair-gershgorin", "pair-series", "approach", "turning", "ties", "bounce-1000", "pair-bounce", "margin-only",
-                    "drag-motor", "band-oscillator"])
+                    "drag-motor", "band-oscillator", "fields"])
 
 
 def parity_scene(name):
@@ -116,4 +147,21 @@ def parity_scenes():
                                      body(2, (50.0, 0.0), mass=2.0, role="path")],
                                     motors=[(1, 0.0, 0.0, 4.5, 9.0), (2, 0.0, 0.0, 4.5, 9.0)], global_force=(3.0, 0.0)), 4))
     out.append(("band-oscillator", world([body(1, (0.0, 0.0), mass=1.0)], bands=[(1, 50.0, 0.0, 40000.0, 0.0)], split=4), 2))
+    out.append(("fields", world([body(1, (10.0, 20.0), mass=0.5), body(2, (40.0, 20.0), mass=2.0),
+                                 body(3, (10.0, 60.0), role="path"), body(4, (-30.0, -30.0), role="static"),
+                                 body(5, (400.0, 0.0), mass=1.0), body(6, (-20.0, 35.0), mass=1.0)],
+                                fields=reference_fields(), global_force=(1.0, -2.0)), 4))
     return out
+
+
+BOX = [(-50.0, -50.0), (50.0, -50.0), (50.0, 50.0), (-50.0, 50.0)]
+
+
+def reference_fields():
+    """The five fields of NativeMetalSubstepKernelTests.swift:263-270."""
+    from bubblephysics_amd.abi import BP_FIELD_DIRECTIONAL as D, BP_FIELD_RADIAL as R, BP_FIELD_VORTEX as V
+    return [field(10, D, direction=(30.0, -40.0), radius=100.0, position=(5.0, -5.0), rotation=0.5),
+            field(11, R, strength=-50.0, falloff=2.0, radius=80.0),
+            field(12, V, strength=10.0, falloff=1.0, polygon=BOX, position=(5.0, 5.0), rotation=0.3),
+            field(13, R, strength=7.0, falloff=0.0, radius=200.0, filter_ids=[2]),
+            field(14, D, direction=(1000.0, 1000.0), radius=300.0, filter_ids=[])]

@@ -30,12 +30,13 @@ def test_library_exports_every_declared_function(hip_library):
 
 
 def test_record_layouts_match_the_header(hip_library):
-    sizes = (ctypes.c_uint32 * 8)()
+    sizes = (ctypes.c_uint32 * 9)()
     hip_library.bp_abi_record_sizes(sizes)
     expected = [ctypes.sizeof(t) for t in (abi.Bubble, abi.Segment, abi.Body, abi.Motor, abi.Band, abi.WorldConfig,
-                                          abi.FrameInfo, abi.BatchWorldDesc)]
+                                          abi.FrameInfo, abi.BatchWorldDesc, abi.Field)]
     assert list(sizes) == expected
     assert abi.BUBBLE_DTYPE.itemsize == 72 and abi.SEGMENT_DTYPE.itemsize == 64 and abi.BODY_DTYPE.itemsize == 96
+    assert abi.FIELD_DTYPE.itemsize == 64
 
 
 def test_defaults_are_the_reference_defaults(hip_library):

@@ -331,3 +331,55 @@ def test_planted_sign_mistake_in_the_distance_bound_fails_the_closed_form():
     wrong = 14.0 / (2 * (math.sqrt(1000.0 ** 2) - 1000.0 + 1e-30))
     assert relative(wrong, 14.0 / 4000) > 1e-4
     assert relative(distance_bound(14.0, 1000.0, 0.0), 14.0 / 4000) <= 1e-12
+
+
+def _radial(strength, falloff, radius, point):
+    """ContactLawReference.RadialField.force (ContactLawReference.swift:186-197)."""
+    d = math.hypot(point[0], point[1])
+    if not (d > 0 and d <= radius):
+        return np.zeros(2)
+    profile = 1.0 if falloff == 0 else max(0.0, 1 - d / radius) ** falloff
+    return np.asarray(point, np.float64) * (strength * profile / d)
+
+
+def test_fields_global_force_filters_roles_and_mass_set_a0(oracle):
+    # NativeMetalSubstepKernelTests.swift:263-308: directional (rotated), radial with falloff,
+    # vortex in a rotated box, a filtered field acting on id 2 only, an empty filter acting on none.
+    from kernel_scenes import reference_fields
+    bodies = [body(1, (10.0, 20.0), mass=0.5), body(2, (10.0, 20.0), mass=2.0), body(3, (10.0, 20.0), role="path"),
+              body(4, (10.0, 20.0), role="static"), body(5, (400.0, 0.0), mass=1.0)]
+    scene = world(bodies, fields=reference_fields(), global_force=(1.0, -2.0))
+    got = oracle.noncontact_acceleration(scene).astype(np.float64)
+
+    def expected(id, mass, point):
+        force = np.array([1.0, -2.0])
+        c, s = math.cos(0.5), math.sin(0.5)
+        delta = np.asarray(point) - np.array([5.0, -5.0])
+        local = np.array([c * delta[0] + s * delta[1], -s * delta[0] + c * delta[1]])
+        if np.hypot(*local) <= 100:
+            force += np.array([c * 30 - s * -40, s * 30 + c * -40])
+        force += _radial(-50, 2, 80, point)
+        c3, s3 = math.cos(0.3), math.sin(0.3)
+        d3 = np.asarray(point) - np.array([5.0, 5.0])
+        l3 = np.array([c3 * d3[0] + s3 * d3[1], -s3 * d3[0] + c3 * d3[1]])
+        if abs(l3[0]) <= 50 and abs(l3[1]) <= 50 and np.hypot(*d3) > 0:
+            r = 50 * math.sqrt(2)
+            scale = 10 * max(0.0, 1 - np.hypot(*l3) / r) ** 1 / np.hypot(*l3)
+            force += np.array([-d3[1], d3[0]]) * scale
+        if id == 2:
+            force += _radial(7, 0, 200, point)
+        return force / mass
+
+    for index, mass in [(0, 0.5), (1, 2.0), (4, 1.0)]:
+        want = expected(int(scene.bubbles["id"][index]), mass, scene.bubbles["center"][index].astype(np.float64))
+        assert np.linalg.norm(got[index] - want) / max(np.linalg.norm(want), 1e-9) <= 1e-4, index
+    assert np.any(got[0] != 0)
+    assert np.all(got[2] == 0) and np.all(got[3] == 0)
+
+
+def test_planted_field_rotation_mistake_is_caught():
+    # Turning the directional force by -0.5 instead of +0.5 moves it by far more than 1e-4.
+    c, s = math.cos(0.5), math.sin(0.5)
+    right = np.array([c * 30 - s * -40, s * 30 + c * -40])
+    wrong = np.array([c * 30 + s * -40, -s * 30 + c * -40])
+    assert np.linalg.norm(right - wrong) / np.linalg.norm(right) > 1e-4
