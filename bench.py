@@ -54,6 +54,7 @@ def run_batched(args, scenes_list):
     class_ms = np.zeros(classes)
     class_bytes = np.zeros(classes)
     statuses = []
+    overflow = 0
     t0 = time.perf_counter()
     for _ in range(args.steps):
         s0 = time.perf_counter()
@@ -61,6 +62,7 @@ def run_batched(args, scenes_list):
         latencies.append((time.perf_counter() - s0) * 1e3)
         ok = infos["status"] == 0
         statuses.append(int((~ok).sum()))
+        overflow = max(overflow, int(infos["candidate_overflow"].max()))
         substeps = infos["substep_count"].astype(np.int64)
         substep_bubbles += int(np.sum(sizes * substeps))
         ms, _, threads, capacity = batch.last_class_timing()
@@ -74,7 +76,7 @@ def run_batched(args, scenes_list):
         "class_threads": threads, "class_capacity": capacity,
         "elapsed": elapsed, "substep_bubbles": substep_bubbles, "latencies": latencies, "class_ms": class_ms / args.steps,
         "class_bytes": class_bytes / args.steps, "failed_worlds": max(statuses) if statuses else 0,
-        "bubbles": int(sizes.sum()), "worlds": len(sizes),
+        "bubbles": int(sizes.sum()), "worlds": len(sizes), "overflow": overflow,
     }
 
 
@@ -161,6 +163,7 @@ def main():
         "p50_step_ms": percentile(res["latencies"], 50),
         "p99_step_ms": percentile(res["latencies"], 99),
         "failed_worlds": res["failed_worlds"],
+        "candidate_overflow_max": res["overflow"],
         "plain": {"workload": "plain: 1 world, 1024 bubbles + 64 kinematic bodies", "value": plain["value"],
                   "p50_step_ms": plain["p50_ms"], "p99_step_ms": plain["p99_ms"]},
         "batched_over_plain": value / plain["value"] if plain["value"] > 0 else None,
