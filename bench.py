@@ -34,6 +34,16 @@ def bytes_per_bubble_substep(mean_degree: float) -> float:
     return 168.0 + 8.0 * mean_degree
 
 
+def kernel_name(k, res):
+    """Kernel of class k: the LDS configurations, the single-workgroup global-state kernel (class 10)
+    and the cooperative kernel (class 11)."""
+    if k == 10:
+        return "world_step_kernel<1024> (global state, one workgroup)"
+    if k == 11:
+        return "world_coop_kernel<256> (cooperative grid)"
+    return f"world_lds_kernel<{int(res['class_threads'][k])} threads, {int(res['class_capacity'][k])} bubbles>"
+
+
 def percentile(values, q):
     return float(np.percentile(np.asarray(values, np.float64), q)) if len(values) else float("nan")
 
@@ -170,8 +180,7 @@ def main():
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
-            "kernel": (f"world_lds_kernel<{int(res['class_threads'][k])} threads, {int(res['class_capacity'][k])} bubbles>"
-                       if int(res['class_capacity'][k]) < 2**32 - 1 else "world_step_kernel<1024> (global state)"),
+            "kernel": kernel_name(k, res),
             "kernel_ms_per_launch": res["class_ms"][k],
         },
         "cpu_baseline": None if cpu is None else {

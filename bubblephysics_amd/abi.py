@@ -49,6 +49,7 @@ BP_SEGMENT_ALLOWED_SIDE = 16
 BP_SHAPE_POLYGON, BP_SHAPE_CIRCLE, BP_SHAPE_CAPSULE = 0, 1, 2
 BP_BODY_STATIC, BP_BODY_KINEMATIC = 1, 2
 BP_FIELD_DIRECTIONAL, BP_FIELD_RADIAL, BP_FIELD_VORTEX = 0, 1, 2
+BP_PATH_WORKGROUP, BP_PATH_COOPERATIVE, BP_PATH_NONE = 0, 1, 2
 
 
 class Bubble(ctypes.Structure):

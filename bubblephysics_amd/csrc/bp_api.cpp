@@ -27,6 +27,10 @@ extern "C" hipError_t bp_launch_world_lds(int id, const bpd::Arena *arena, bpd::
                                           const uint32_t *order, uint32_t count, hipStream_t stream);
 extern "C" int bp_lds_config_count();
 extern "C" void bp_lds_config(int id, uint32_t *threads, uint32_t *per, uint32_t *rowcap, uint32_t *caps);
+extern "C" uint32_t bp_coop_max_groups();
+extern "C" uint32_t bp_coop_block();
+extern "C" hipError_t bp_launch_world_coop(const bpd::Arena *arena, bpd::WorldDesc *descs, bpd::WorldStatus *stats,
+                                           const uint32_t *orderEntry, uint32_t world, uint32_t groups, hipStream_t stream);
 
 namespace {
 
@@ -44,19 +48,22 @@ void set_error(const std::string &text) { g_lastError = text; }
     } while (0)
 
 constexpr uint32_t kAdjPerBubble = 64;       // CSR capacity per bubble (grows on overflow)
-constexpr int kClassCount = 11;              // ten LDS configurations and the global-state kernel
+constexpr int kClassCount = 12;              // ten LDS configurations, the global-state kernel, the cooperative kernel
 constexpr int kGlobalClass = 10;
+constexpr int kCoopClass = 11;
+constexpr uint32_t kLdsLargest = 4096;       // bubbles of the widest LDS configuration
 constexpr int kStreams = 4;
 
 uint32_t class_threads(int k) {
     if (k == kGlobalClass) return 1024;
+    if (k == kCoopClass) return bp_coop_block();
     uint32_t t, p, r;
     bp_lds_config(k, &t, &p, &r, nullptr);
     return t;
 }
 
 uint32_t class_capacity(int k) {
-    if (k == kGlobalClass) return 0xffffffffu;
+    if (k == kGlobalClass || k == kCoopClass) return 0xffffffffu;
     uint32_t t, p, r;
     bp_lds_config(k, &t, &p, &r, nullptr);
     return t * p;
@@ -65,7 +72,7 @@ uint32_t class_capacity(int k) {
 // Whether a world of n bubbles with the given geometry fits configuration k (the global-state
 // kernel takes every world).
 bool class_fits(int k, size_t n, size_t segments, size_t bodies, size_t vertices, size_t triangles) {
-    if (k == kGlobalClass) return true;
+    if (k == kGlobalClass || k == kCoopClass) return true;
     uint32_t t, p, r, caps[4];
     bp_lds_config(k, &t, &p, &r, caps);
     return n <= (size_t)t * p && segments <= caps[0] && bodies <= caps[1] && vertices <= caps[2] && triangles <= caps[3];
@@ -98,14 +105,22 @@ bool class_fits(int k, const HostWorld &w) {
     return class_fits(k, w.bubbles.size(), w.segments.size(), w.bodies.size(), w.vertices.size() / 2, w.triangles.size() / 4);
 }
 
-// Default kernel configuration of a world: the narrowest LDS configuration whose bubble and
-// geometry capacities hold it (Classes 0-9 are the LDS configurations of bp_world_lds.hip),
-// else the global-state kernel.
-int default_class_for(const HostWorld &w) {
+// Default kernel configuration of a world: the cooperative kernel for worlds above the widest
+// LDS configuration (or on request, as the reference's multi-dispatch path), else the narrowest
+// LDS configuration whose bubble and geometry capacities hold it (classes 0-9 are the LDS
+// configurations of bp_world_lds.hip), else the single-workgroup global-state kernel.
+int default_class_for(const HostWorld &w, bool cooperative) {
+    if (cooperative && (w.config.force_cooperative || w.bubbles.size() > kLdsLargest)) return kCoopClass;
     static const int kBySize[] = {0, 1, 2, 3, 4, 7, 9};
     for (int k : kBySize)
         if (class_fits(k, w)) return k;
     return kGlobalClass;
+}
+
+uint32_t next_pow2(uint32_t v, uint32_t lowest) {
+    uint32_t p = lowest;
+    while (p < v && p < (1u << 24)) p <<= 1;
+    return p;
 }
 
 template <class T> struct DevBuf {
@@ -126,6 +141,7 @@ struct bp_batch {
     std::vector<bpd::WorldDesc> hostDescs;
     uint32_t *orders[kClassCount] = {};
     std::vector<uint32_t> classWorlds[kClassCount];
+    std::vector<uint32_t> coopGroups;   // per world: workgroups of its cooperative launch (0 = not cooperative)
     hipStream_t streams[kStreams] = {};
     hipEvent_t startEvent = nullptr, endEvents[kStreams] = {};
     hipEvent_t classStart[kClassCount] = {}, classEnd[kClassCount] = {};
@@ -233,10 +249,24 @@ bp_status validate_world(const HostWorld &w) {
 bp_status bp_batch::pack() {
     release();
     uint32_t nb = 0, ns = 0, nbod = 0, nv = 0, nt = 0, nm = 0, nband = 0, nf = 0, nfv = 0, nfi = 0;
-    uint64_t adj = 0;
-    for (auto &w : worlds) {
+    uint64_t adj = 0, coopCells = 0;
+    uint32_t coopWorlds = 0;
+    const uint32_t coopMax = bp_coop_max_groups();
+    std::vector<int> worldClass(worlds.size());
+    coopGroups.assign(worlds.size(), 0);
+    for (size_t wi = 0; wi < worlds.size(); ++wi) {
+        const HostWorld &w = worlds[wi];
         bp_status s = validate_world(w);
         if (s != BP_OK) return s;
+        int cls = default_class_for(w, coopMax > 0);
+        if (w.forcedClass >= 0 && class_fits(w.forcedClass, w) && (w.forcedClass != kCoopClass || coopMax > 0)) cls = w.forcedClass;
+        worldClass[wi] = cls;
+        if (cls == kCoopClass) {
+            coopCells += next_pow2((uint32_t)w.bubbles.size(), 1024);
+            ++coopWorlds;
+            const uint32_t block = bp_coop_block();
+            coopGroups[wi] = std::max<uint32_t>(1, std::min<uint32_t>(coopMax, ((uint32_t)w.bubbles.size() + block - 1) / block));
+        }
         nb += (uint32_t)w.bubbles.size(); ns += (uint32_t)w.segments.size(); nbod += (uint32_t)w.bodies.size();
         nv += (uint32_t)(w.vertices.size() / 2); nt += (uint32_t)(w.triangles.size() / 4);
         nm += (uint32_t)w.motors.size(); nband += (uint32_t)w.bands.size();
@@ -268,6 +298,8 @@ bp_status bp_batch::pack() {
     BP_ALLOC(A.motors, nm); BP_ALLOC(A.bands, nband);
     BP_ALLOC(A.fields, nf); BP_ALLOC(A.fieldVerts, nfv); BP_ALLOC(A.fieldIds, nfi);
     BP_ALLOC(A.history, (size_t)worlds.size() * bpd::kFrameLimit);
+    BP_ALLOC(A.coopCells, coopCells); BP_ALLOC(A.coopScratch, (size_t)coopWorlds * bpd::kCoopScratchWords);
+    BP_ALLOC(A.bvhChild, 2 * (size_t)nbod); BP_ALLOC(A.bvhBox, 2 * (size_t)nbod); BP_ALLOC(A.bvhLeaf, nbod);
     BP_ALLOC(descs, worlds.size());
     BP_ALLOC(arenaDev, 1);
     for (int k = 0; k < kClassCount; ++k) BP_ALLOC(orders[k], worlds.size());
@@ -288,7 +320,7 @@ bp_status bp_batch::pack() {
     std::vector<uint64_t> fieldIds(nfi);
     std::vector<uint64_t> bodyId(nbod);
     hostDescs.assign(worlds.size(), bpd::WorldDesc{});
-    uint32_t ob = 0, os = 0, obd = 0, ov = 0, ot = 0, om_ = 0, oband = 0, of = 0, ofv = 0, ofi = 0;
+    uint32_t ob = 0, os = 0, obd = 0, ov = 0, ot = 0, om_ = 0, oband = 0, of = 0, ofv = 0, ofi = 0, occ = 0, ocs = 0;
     uint64_t oadj = 0;
     for (auto &k : classWorlds) k.clear();
     for (size_t wi = 0; wi < worlds.size(); ++wi) {
@@ -383,9 +415,13 @@ bp_status bp_batch::pack() {
         for (size_t v = 0; v < w.vertices.size() / 2; ++v) verts[ov + v] = make_float2(w.vertices[2 * v], w.vertices[2 * v + 1]);
         for (size_t t = 0; t < w.triangles.size() / 4; ++t)
             tris[ot + t] = make_uint4(w.triangles[4 * t], w.triangles[4 * t + 1], w.triangles[4 * t + 2], w.triangles[4 * t + 3]);
-        int cls = default_class_for(w);
-        if (w.forcedClass >= 0 && class_fits(w.forcedClass, w)) cls = w.forcedClass;
+        const int cls = worldClass[wi];
         classWorlds[cls].push_back((uint32_t)wi);
+        if (cls == kCoopClass) {
+            d.coopBuckets = next_pow2(n, 1024);
+            d.coopCellOffset = occ; occ += d.coopBuckets;
+            d.coopScratchOffset = ocs; ocs += bpd::kCoopScratchWords;
+        }
         ob += n; os += (uint32_t)w.segments.size(); obd += (uint32_t)w.bodies.size();
         ov += (uint32_t)(w.vertices.size() / 2); ot += (uint32_t)(w.triangles.size() / 4);
         om_ += (uint32_t)w.motors.size(); oband += (uint32_t)w.bands.size();
@@ -526,7 +562,7 @@ bp_status bp_batch::step(uint32_t steps, bp_frame_info *infos) {
     // Largest classes first so that their long-running workgroups start early on their stream.
     for (int k = kClassCount - 1; k >= 0; --k) {
         uint32_t count = (uint32_t)classWorlds[k].size();
-        if (!count) continue;
+        if (!count || k == kCoopClass) continue;
         hipStream_t stream = streams[k % kStreams];
         BP_HIP(hipEventRecord(classStart[k], stream));
         for (uint32_t st = 0; st < steps; ++st) {
@@ -536,6 +572,24 @@ bp_status bp_batch::step(uint32_t steps, bp_frame_info *infos) {
             if (e != hipSuccess) { set_error(std::string("kernel launch: ") + hipGetErrorString(e)); return BP_ERR_COMMAND_FAILED; }
         }
         BP_HIP(hipEventRecord(classEnd[k], stream));
+    }
+    // Cooperative worlds last, alone on the device: a cooperative grid needs all its workgroups
+    // resident at once, so it never shares the device with the other classes' launches.
+    if (!classWorlds[kCoopClass].empty()) {
+        hipStream_t stream = streams[0];
+        for (int k = 1; k < kStreams; ++k) {
+            BP_HIP(hipEventRecord(endEvents[k], streams[k]));
+            BP_HIP(hipStreamWaitEvent(stream, endEvents[k], 0));
+        }
+        BP_HIP(hipEventRecord(classStart[kCoopClass], stream));
+        for (uint32_t st = 0; st < steps; ++st)
+            for (size_t j = 0; j < classWorlds[kCoopClass].size(); ++j) {
+                const uint32_t wi = classWorlds[kCoopClass][j];
+                hipError_t e = bp_launch_world_coop(arenaDev, descs, stats + (size_t)st * W, orders[kCoopClass] + j, wi, coopGroups[wi],
+                                                    stream);
+                if (e != hipSuccess) { set_error(std::string("cooperative launch: ") + hipGetErrorString(e)); return BP_ERR_COMMAND_FAILED; }
+            }
+        BP_HIP(hipEventRecord(classEnd[kCoopClass], stream));
     }
     for (int k = 0; k < kStreams; ++k) BP_HIP(hipEventRecord(endEvents[k], streams[k]));
     for (int k = 0; k < kStreams; ++k) BP_HIP(hipEventSynchronize(endEvents[k]));
@@ -561,6 +615,7 @@ bp_status bp_batch::step(uint32_t steps, bp_frame_info *infos) {
             fill_info(hostStats[i], &infos[i]);
             infos[i].kernel_class = worldClass[i % W];
             if (worldClass[i % W] == (uint32_t)kGlobalClass) infos[i].execution_path = BP_PATH_WORKGROUP;
+            if (worldClass[i % W] == (uint32_t)kCoopClass) infos[i].execution_path = BP_PATH_COOPERATIVE;
             infos[i].gpu_milliseconds = ms / steps;
             if (!hostStats[i].executed) infos[i].status = BP_ERR_UNAVAILABLE;
         }

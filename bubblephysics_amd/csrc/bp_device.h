@@ -77,6 +77,21 @@ struct Arena {
     float2 *fieldVerts;       // local vertices of polygon field ranges
     uint64_t *fieldIds;       // filter ids, ascending within each field's range
     float *history;           // 1024 h values per world
+    // Cooperative (multi-workgroup) worlds only.
+    uint32_t *coopCells;      // hashed-grid buckets of each cooperative world
+    uint32_t *coopScratch;    // workgroup partials and counters of each cooperative world
+    uint2 *bvhChild;          // body LBVH: children of internal node i (bit 31 = leaf), 2 slots per body
+    float4 *bvhBox;           // boxes: internal nodes at [0, nb-1), leaves at [nb, 2 nb)
+    uint32_t *bvhLeaf;        // body index of each sorted leaf
+};
+
+// Layout of a cooperative world's scratch (u32 words) for a grid of up to kCoopMaxGroups workgroups.
+constexpr uint32_t kCoopMaxGroups = 2048;
+constexpr uint32_t kCoopCounters = 32;
+constexpr uint32_t kCoopScratchWords = 8 * kCoopMaxGroups + kCoopCounters;
+enum CoopCounter : uint32_t {
+    kCoDropped = 0, kCoMaxDemand, kCoPairs, kCoOverflowRows, kCoSafety, kCoRemovals, kCoWraps, kCoViolator,
+    kCoBvhBuilt, kCoBodySafety, kCoEntries
 };
 
 
@@ -91,6 +106,7 @@ struct WorldDesc {
     uint32_t motorOffset, motorCount;
     uint32_t bandOffset, bandCount;
     uint32_t fieldOffset, fieldCount, fieldVertexOffset, fieldIdOffset;
+    uint32_t coopCellOffset, coopBuckets, coopScratchOffset;   // cooperative worlds
     uint32_t adjOffset, adjCapacity;
     uint32_t historyOffset;
     uint32_t pub;

@@ -41,6 +41,7 @@ class SolverFrame:
     limiter_object: int
     limiter_term: int
     detail: FailureDetail | None
+    execution_path: int = 0          # BP_PATH_WORKGROUP or BP_PATH_COOPERATIVE
 
     @property
     def ok(self) -> bool:
@@ -149,7 +150,8 @@ class NativeWorld:
             contact_candidate_overflow_count=int(info.candidate_overflow),
             maximum_contact_candidates_per_object=int(info.max_candidates_per_object), group_width=int(info.group_width),
             gpu_milliseconds=float(info.gpu_milliseconds), region_removals=set(), region_wraps=set(),
-            limiter_object=int(info.limiter_object), limiter_term=int(info.limiter_term), detail=_detail(info))
+            limiter_object=int(info.limiter_object), limiter_term=int(info.limiter_term), detail=_detail(info),
+            execution_path=int(info.execution_path))
 
     def read_bubbles(self) -> np.ndarray:
         out = np.zeros(self._count, abi.BUBBLE_DTYPE)

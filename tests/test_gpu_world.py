@@ -2,6 +2,8 @@
 C ABI. Integer and index results (substep counts, rebuilds, candidate pairs, adjacency rows) must be
 equal; floating-point state is compared bitwise, which holds because kernels and oracle round every
 operation identically (the tolerance the step guarantees is 1e-5 relative; bitwise is stricter)."""
+import math
+
 import numpy as np
 import pytest
 
@@ -60,10 +62,41 @@ def test_config2_bodies_match_oracle_bitwise(oracle):
     run_both(scenes.config2(), 25, oracle, check_adjacency=True)
 
 
-def test_large_world_one_workgroup_matches_oracle(oracle):
+def test_large_world_cooperative_grid_matches_oracle(oracle):
+    # Above the widest LDS configuration a world runs on a cooperative grid of workgroups.
     rng = np.random.Generator(np.random.PCG64(7))
     scene = scenes.batch_world(rng, 5000, bodies_max=8)
-    run_both(scene, 8, oracle, check_adjacency=True)
+    world, _, _ = run_both(scene, 8, oracle, check_adjacency=True)
+    assert world.step(8).execution_path == abi.BP_PATH_COOPERATIVE
+
+
+def test_large_world_in_one_workgroup_matches_oracle(oracle):
+    # The single-workgroup global-state kernel (class 10) on the same world.
+    rng = np.random.Generator(np.random.PCG64(7))
+    scene = scenes.batch_world(rng, 5000, bodies_max=8)
+    batch = solver.Batch([scene])
+    batch.set_world_class(0, 10)
+    infos = batch.step(4)
+    cb, _, cf, _ = oracle.run(scene, 4)
+    assert (infos[:, 0]["kernel_class"] == 10).all()
+    assert np.array_equal(infos[:, 0]["substep_count"], cf["substeps"])
+    assert_same_state(batch.read_bubbles(0), cb, "one workgroup")
+
+
+def test_small_world_forced_onto_the_cooperative_grid_matches_oracle(oracle):
+    # force_cooperative: the reference's multi-dispatch path for a small world (sorted grid in
+    # the oracle, cooperative grid here); results are the same bits.
+    scene = scenes.config1()
+    scene.config.force_cooperative = 1
+    world, _, _ = run_both(scene, 12, oracle, check_adjacency=True)
+    assert world.step(12).execution_path == abi.BP_PATH_COOPERATIVE
+
+
+def test_cooperative_world_with_body_tree_matches_oracle(oracle):
+    # 20,000 bubbles and 300 kinematic bodies: body rows come from the LBVH of the sweep boxes.
+    n = 20000
+    scene = scenes.config3(n=n, bodies=300, side=16384.0 * math.sqrt(n / 262144))
+    run_both(scene, 3, oracle, check_adjacency=True)
 
 
 def test_hash_bucket_count_does_not_change_the_step(oracle):
