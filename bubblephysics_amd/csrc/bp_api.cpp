@@ -24,7 +24,8 @@
 extern "C" hipError_t bp_launch_world_step(int block, const bpd::Arena *arena, bpd::WorldDesc *descs, bpd::WorldStatus *stats,
                                            const uint32_t *order, uint32_t count, hipStream_t stream);
 extern "C" hipError_t bp_launch_world_lds(int id, const bpd::Arena *arena, bpd::WorldDesc *descs, bpd::WorldStatus *stats,
-                                          const uint32_t *order, uint32_t count, hipStream_t stream);
+                                          const uint32_t *order, uint32_t count, uint32_t steps, uint32_t statsStride,
+                                          hipStream_t stream);
 extern "C" int bp_lds_config_count();
 extern "C" void bp_lds_config(int id, uint32_t *threads, uint32_t *per, uint32_t *rowcap, uint32_t *caps);
 extern "C" uint32_t bp_coop_max_groups();
@@ -565,10 +566,15 @@ bp_status bp_batch::step(uint32_t steps, bp_frame_info *infos) {
         if (!count || k == kCoopClass) continue;
         hipStream_t stream = streams[k % kStreams];
         BP_HIP(hipEventRecord(classStart[k], stream));
-        for (uint32_t st = 0; st < steps; ++st) {
-            hipError_t e = k == kGlobalClass
-                ? bp_launch_world_step(1024, arenaDev, descs, stats + (size_t)st * W, orders[k], count, stream)
-                : bp_launch_world_lds(k, arenaDev, descs, stats + (size_t)st * W, orders[k], count, stream);
+        // An LDS configuration runs all the steps in one launch: each workgroup carries its world
+        // through them without waiting for the slowest world of the class at every step.
+        if (k == kGlobalClass) {
+            for (uint32_t st = 0; st < steps; ++st) {
+                hipError_t e = bp_launch_world_step(1024, arenaDev, descs, stats + (size_t)st * W, orders[k], count, stream);
+                if (e != hipSuccess) { set_error(std::string("kernel launch: ") + hipGetErrorString(e)); return BP_ERR_COMMAND_FAILED; }
+            }
+        } else {
+            hipError_t e = bp_launch_world_lds(k, arenaDev, descs, stats, orders[k], count, steps, (uint32_t)W, stream);
             if (e != hipSuccess) { set_error(std::string("kernel launch: ") + hipGetErrorString(e)); return BP_ERR_COMMAND_FAILED; }
         }
         BP_HIP(hipEventRecord(classEnd[k], stream));

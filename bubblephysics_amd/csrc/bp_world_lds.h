@@ -75,7 +75,8 @@ constexpr uint32_t kHasMotor = 1u << 31, kHasTorque = 1u << 30;   // register-on
 
 template <int BLOCK, int PER, int ROWCAP, int SEGCAP, int BODYCAP, int VERTCAP, int TRICAP>
 __global__ __launch_bounds__(BLOCK) void world_lds_kernel(const Arena *__restrict__ A, WorldDesc *descs, WorldStatus *stats,
-                                                          const uint32_t *order, uint32_t count) {
+                                                          const uint32_t *order, uint32_t count, uint32_t steps,
+                                                          uint32_t statsStride) {
     using L = LdsWorld<BLOCK, PER, ROWCAP, SEGCAP, BODYCAP, VERTCAP, TRICAP>;
     __shared__ L S;
     if (blockIdx.x >= count) return;
@@ -93,664 +94,671 @@ __global__ __launch_bounds__(BLOCK) void world_lds_kernel(const Arena *__restric
         if (lane == 0) { st->status = BP_ERR_COMMAND_FAILED; st->substeps = 0; st->executed = 1; }
         return;
     }
-    const uint32_t pub = d->pub, ws = 1u - pub;
-    const uint32_t n = d->bubbleCount, o = d->bubbleOffset;
-    const Geo g = make_geo(A, d, ws);
+    uint32_t pubNow = d->pub;                                 // published side, tracked in registers
+    for (uint32_t stepIndex = 0; stepIndex < steps; ++stepIndex) {
+        st = stats + (size_t)stepIndex * statsStride + wi;
+        const uint32_t pub = pubNow, ws = 1u - pub;
+        const uint32_t n = d->bubbleCount, o = d->bubbleOffset;
+        const Geo g = make_geo(A, d, ws);
 #define ids (A->id + o)
 #define catG (A->cat + o)
 #define maskG (A->mask + o)
 #define startG (A->start[ws] + o)
-    float *history = A->history + d->historyOffset;
-    const float sepTol = d->sepTol;
-    const uint32_t limit = d->limit;
-    const uint32_t buckets = min(d->buckets, L::NB);
+        float *history = A->history + d->historyOffset;
+        const float sepTol = d->sepTol;
+        const uint32_t limit = d->limit;
+        const uint32_t buckets = min(d->buckets, L::NB);
 
-    // ---- 1. Begin step (Force:285-293): own state into registers, neighbour fields into LDS.
-    float2 p[PER], v[PER], a[PER], ref[PER], mf[PER];
-    float rot[PER], om[PER], mg[PER], gm[PER], mq[PER];
-    uint32_t fl[PER], rs[PER], rl[PER], rp[PER], rq[PER];
-    int32_t bi[PER];
-    bool valid[PER];
-#pragma unroll
-    for (int s = 0; s < PER; ++s) {
-        uint32_t i = lane + s * BLOCK;
-        valid[s] = i < n;
-        p[s] = v[s] = a[s] = ref[s] = mf[s] = f2(0.0f, 0.0f);
-        rot[s] = om[s] = mg[s] = gm[s] = mq[s] = 0.0f;
-        fl[s] = 0; rs[s] = rl[s] = rp[s] = rq[s] = 0;
-        bi[s] = -1;
-        if (!valid[s]) continue;
-        uint32_t gi = o + i;
-        p[s] = A->pos[pub][gi];
-        v[s] = A->vel[pub][gi];
-        rot[s] = A->rot[pub][gi];
-        om[s] = A->omega[pub][gi];
-        fl[s] = A->flags[gi];
-        float invmRaw = A->invm[gi];
-        bool isStatic = !(invmRaw > 0.0f) || ((fl[s] & kEnabled) && (fl[s] & kStatic));
-        if ((fl[s] & kEnabled) && (fl[s] & kStatic)) { v[s] = f2(0.0f, 0.0f); om[s] = 0.0f; }
-        startG[i] = p[s];
-        S.pos[i] = p[s];
-        S.vel[i] = isStatic ? f2(0.0f, 0.0f) : v[s];
-        S.r[i] = A->radius[gi];
-        S.K[i] = A->Kc[gi];
-        S.invm[i] = isStatic ? 0.0f : invmRaw;
-        // Force records (Force:423-438, resolved to indices on the host) and drag, per step.
-        bi[s] = A->band[gi];
-        const int32_t mi = A->motor[gi];
-        const bool driven = (fl[s] & kForceDriven) != 0;
-        gm[s] = driven ? fmaxf(A->gamma[gi], 0.0f) : 0.0f;
-        fl[s] &= ~(kHasMotor | kHasTorque);
-        if (mi >= 0) {
-            const float4 md = g.motor(mi);
-            fl[s] |= kHasMotor;
-            mf[s] = f2(md.x, md.y);
-            if (driven && md.w > 0.0f) { fl[s] |= kHasTorque; mq[s] = md.z / md.w; }
-        }
-    }
-    uint32_t safety = predict_bodies<BLOCK>(A, d, pub, ws);
-    // Geometry into LDS. Bodies are read back by the lane that predicted them.
-    for (uint32_t k = lane; k < g.nb; k += BLOCK) {
-        const BodyView b = g.body(k);
-        S.bodyPose[k] = b.pose;
-        S.bodyExtent[k] = make_float4(b.prev.x, b.prev.y, b.shape.x, b.shape.y);
-        S.bodyMotion[k] = b.motion;
-        S.bodyRanges[k] = b.ranges;
-        S.bodyKind[k] = b.kind;
-    }
-    for (uint32_t j = lane; j < g.ns; j += BLOCK) {
-        const Seg sg = g.segment(j);
-        S.segAB[j] = make_float4(sg.a.x, sg.a.y, sg.b.x, sg.b.y);
-        S.segRegion[j] = make_float4(sg.prevStart.x, sg.prevStart.y, sg.base.x, sg.base.y);
-        S.segOutward[j] = sg.outward;
-        S.segFlags[j] = sg.flags;
-        S.segCat[j] = g.segCat(j);
-        S.segMask[j] = g.segMask(j);
-    }
-    for (uint32_t k = lane; k < d->vertexCount; k += BLOCK) S.verts[k] = A->verts[d->vertexOffset + k];
-    for (uint32_t k = lane; k < d->triangleCount; k += BLOCK) {
-        const uint4 t = A->tris[d->triangleOffset + k];
-        S.tris[k] = t.x | (t.y << 10) | (t.z << 20);
-    }
-    __syncthreads();
-
-    Ctl c;
-    c.t0 = d->dt; c.t1 = n == 0 ? d->dt : 0.0f; c.h = 0.0f; c.split = (float)(d->split < 1 ? 1u : d->split);
-    c.c0 = c.c1 = c.rebuild = c.limit = 0; c.limObj = 0; c.limTerm = BP_TERM_REMAINING;
-    uint32_t ov0 = 0, ov1 = 0, ov2 = 0, ov3 = 0, pairs = 0, entriesTotal = 0;
-    int32_t status = BP_OK;
-    bool claimed = false;
-    float cellWidth = 0.0f;
-    RowRef rows;
-    rows.lds = S.rows;
-    rows.global = A->adj + d->adjOffset;
-    rows.inLds = true;
-    const bool ldsRowsAllowed = ROWCAP > 0 && g.ns < kLdsIndexLimit && g.nb < kLdsIndexLimit;
-
-    auto segment_at = [&](uint32_t j) __attribute__((always_inline)) -> Seg {
-        Seg sg;
-        const float4 ab = S.segAB[j], rg = S.segRegion[j];
-        sg.a = f2(ab.x, ab.y); sg.b = f2(ab.z, ab.w);
-        sg.prevStart = f2(rg.x, rg.y);
-        sg.base = f2(rg.z, rg.w);
-        sg.outward = S.segOutward[j];
-        sg.flags = S.segFlags[j];
-        return sg;
-    };
-    // Body frames at the step fraction of the current pass, evaluated once per fraction.
-    float framesAt = -1.0f;
-    auto refresh_frames = [&](float fraction) __attribute__((always_inline)) {
-        if (g.nb == 0 || fraction == framesAt) return;
-        __syncthreads();
-        for (uint32_t k = lane; k < g.nb; k += BLOCK) {
-            const float4 ex = S.bodyExtent[k];
-            float2 at;
-            float4 cs;
-            frame_pose(S.bodyPose[k], f2(ex.x, ex.y), fraction, at, cs);
-            S.frameAt[k] = at;
-            S.frameCs[k] = cs;
-        }
-        __syncthreads();
-        framesAt = fraction;
-    };
-    auto frame_at = [&](uint32_t k) __attribute__((always_inline)) -> BodyFrame {
-        BodyFrame f;
-        const float4 cs = S.frameCs[k];
-        f.at = S.frameAt[k];
-        f.c = cs.x; f.s = cs.y; f.cn = cs.z; f.sn = cs.w;
-        f.core = S.bodyExtent[k].w;
-        f.motion = S.bodyMotion[k];
-        f.ranges = S.bodyRanges[k];
-        f.round = S.bodyKind[k].x != 0u;
-        return f;
-    };
-    auto a0_of = [&](int s, float invm, float2 pi) __attribute__((always_inline)) -> float2 {
-        return g.non_contact_with(fl[s], invm, ids + lane + s * BLOCK, bi[s], (fl[s] & kHasMotor) != 0u, mf[s], pi);
-    };
-    // Candidate tests against the staged geometry (Geometry:279-290, 361-379).
-    auto segment_candidate = [&](float2 pc, float radius, float margin, uint32_t cat, uint32_t mask, uint32_t j)
-        __attribute__((always_inline)) -> bool {
-        if (!filter_accepts(cat, mask, S.segCat[j], S.segMask[j])) return false;
-        const float4 ab = S.segAB[j];
-        const float reach = radius + margin + sepTol;
-        const float2 lower = subs(min2(f2(ab.x, ab.y), f2(ab.z, ab.w)), reach);
-        const float2 upper = adds(max2(f2(ab.x, ab.y), f2(ab.z, ab.w)), reach);
-        return pc.x >= lower.x && pc.y >= lower.y && pc.x <= upper.x && pc.y <= upper.y;
-    };
-    auto body_candidate = [&](float2 pc, float radius, float margin, uint32_t cat, uint32_t mask, uint32_t k)
-        __attribute__((always_inline)) -> bool {
-        const uint4 kind = S.bodyKind[k];
-        const bool isStatic = (kind.y & kBodyStatic) != 0;
-        const bool obstacle = isStatic ? (kind.x != 0u) : ((kind.y & kBodyKinematic) != 0u);
-        if (!obstacle) return false;
-        if (!filter_accepts(cat, mask, kind.z, kind.w)) return false;
-        const float4 pose = S.bodyPose[k], ex = S.bodyExtent[k];
-        const float2 prev = f2(ex.x, ex.y);
-        const float br = ex.z;
-        const float2 bodyLow = subs(min2(f2(pose.x, pose.y), prev), br);
-        const float2 bodyHigh = adds(max2(f2(pose.x, pose.y), prev), br);
-        const float reach = radius + fmaxf(margin, 0.0f);
-        const float2 lo = subs(pc, reach), hi = adds(pc, reach);
-        return !(hi.x < bodyLow.x || hi.y < bodyLow.y || bodyHigh.x < lo.x || bodyHigh.y < lo.y);
-    };
-
-    // Hashed grid of the current centres in LDS (Force:1066-1105).
-    auto build_hash = [&]() __attribute__((always_inline)) {
-        for (uint32_t b = lane; b < buckets; b += BLOCK) S.buckets[b] = 0;
-        __syncthreads();
-#pragma unroll
+        // ---- 1. Begin step (Force:285-293): own state into registers, neighbour fields into LDS.
+        float2 p[PER], v[PER], a[PER], ref[PER], mf[PER];
+        float rot[PER], om[PER], mg[PER], gm[PER], mq[PER];
+        uint32_t fl[PER], rs[PER], rl[PER], rp[PER], rq[PER];
+        int32_t bi[PER];
+        bool valid[PER];
+    #pragma unroll
         for (int s = 0; s < PER; ++s) {
-            if (!valid[s] || !finite2(p[s])) continue;
-            atomicAdd(&S.buckets[bucket_of(cell_coord(p[s].x, cellWidth), cell_coord(p[s].y, cellWidth), buckets)], 1u);
-        }
-        __syncthreads();
-        bucket_prefix<BLOCK>(S.red, S.buckets, buckets);
-        __syncthreads();
-#pragma unroll
-        for (int s = 0; s < PER; ++s) {
-            if (!valid[s] || !finite2(p[s])) continue;
-            uint32_t slot = atomicAdd(&S.buckets[bucket_of(cell_coord(p[s].x, cellWidth), cell_coord(p[s].y, cellWidth), buckets)], 1u);
-            S.grid[slot] = (uint16_t)(lane + s * BLOCK);
-        }
-        __syncthreads();
-    };
-
-    // Adjacency row of bubble i (pairs both ways from the 3x3 cells, segments, bodies).
-    auto enumerate = [&](const uint32_t i, const float2 pi, const float mi, auto emit, uint32_t &forward) __attribute__((always_inline)) {
-        forward = 0;
-        const float ri = S.r[i];
-        const uint32_t ci = catG[i], ki = maskG[i];
-        if (finite2(pi)) {
-            const bool ownRow = finite2(startG[i]);
-            const int64_t cx = cell_coord(pi.x, cellWidth), cy = cell_coord(pi.y, cellWidth);
-            for (int dx = -1; dx <= 1; ++dx)
-                for (int dy = -1; dy <= 1; ++dy) {
-                    const int64_t tx = cx + dx, ty = cy + dy;
-                    const uint32_t b = bucket_of(tx, ty, buckets);
-                    const uint32_t lo = b == 0 ? 0u : S.buckets[b - 1], hi = S.buckets[b];
-                    for (uint32_t k = lo; k < hi; ++k) {
-                        const uint32_t j = S.grid[k];
-                        if (j == i) continue;
-                        const float2 q = S.pos[j];
-                        if (cell_coord(q.x, cellWidth) != tx || cell_coord(q.y, cellWidth) != ty) continue;
-                        const uint32_t l = min(i, j), h = max(i, j);
-                        const float2 pl = l == i ? pi : q, ph = l == i ? q : pi;
-                        const float ml = l == i ? mi : S.aux[j], mh = l == i ? S.aux[j] : mi;
-                        const float reach = S.r[l] + S.r[h] + ml + mh + sepTol;
-                        if (!(fabsf(pl.x - ph.x) <= reach && fabsf(pl.y - ph.y) <= reach)) continue;
-                        if (!filter_accepts(catG[l], maskG[l], catG[h], maskG[h])) continue;
-                        if (j > i) {
-                            if (!ownRow) continue;
-                            ++forward;
-                        } else if (!finite2(startG[j])) {
-                            continue;
-                        }
-                        emit(j);
-                    }
-                }
-            if (ownRow)
-                for (uint32_t j = 0; j < g.ns; ++j)
-                    if (segment_candidate(pi, ri, mi, ci, ki, j)) { ++forward; emit((1u << 30) | j); }
-        }
-        for (uint32_t k = 0; k < g.nb; ++k)
-            if (body_candidate(pi, ri, mi, ci, ki, k)) emit((2u << 30) | k);
-    };
-
-    // Grid build (Force:1040-1134) with CSR rows. Returns false when the rows do not fit.
-    auto build = [&]() __attribute__((always_inline)) -> bool {
-        float remaining = fmaxf(c.t0 - c.t1, 0.0f);
-        float width = sepTol;
-#pragma unroll
-        for (int s = 0; s < PER; ++s) {
-            if (!valid[s]) continue;
             uint32_t i = lane + s * BLOCK;
-            float2 a0 = a0_of(s, S.invm[i], p[s]);
-            float travel = len2(S.vel[i]) * remaining + 0.5f * len2(a0) * remaining * remaining;
-            mg[s] = d->minimumMarginOnly ? 0.25f * S.r[i] : fmaxf(1.05f * travel, 0.25f * S.r[i]);
-            ref[s] = p[s];
-            S.aux[i] = mg[s];
-            width = fmaxf(width, 2.0f * (S.r[i] + mg[s]) + sepTol);
-        }
-        for (uint32_t k = lane; k < g.nb; k += BLOCK) g.bodyHits()[k] = 0;
-        width = block_maxf<BLOCK>(S.red, width);
-        cellWidth = fmaxf(width, kFloatMin);
-        if (lane < 8) S.red.counters[lane] = 0;
-        build_hash();
-        // Pass 1: row lengths and the reference's overflow counters (Geometry:291-297, 410-414).
-        uint32_t droppedLocal = 0, maxLocal = 0, pairsLocal = 0, mine = 0;
-#pragma unroll
-        for (int s = 0; s < PER; ++s) {
+            valid[s] = i < n;
+            p[s] = v[s] = a[s] = ref[s] = mf[s] = f2(0.0f, 0.0f);
+            rot[s] = om[s] = mg[s] = gm[s] = mq[s] = 0.0f;
+            fl[s] = 0; rs[s] = rl[s] = rp[s] = rq[s] = 0;
+            bi[s] = -1;
             if (!valid[s]) continue;
-            uint32_t cnt = 0, forward = 0;
-            enumerate(lane + s * BLOCK, p[s], mg[s], [&](uint32_t e) __attribute__((always_inline)) {
-                ++cnt;
-                if ((e >> 30) == 2u) atomicAdd(&g.bodyHits()[e & kIndexMask], 1u);
-            }, forward);
-            rl[s] = cnt;
-            mine += cnt;
-            if (forward > limit) droppedLocal += forward - limit;
-            if (cnt > limit) droppedLocal += cnt - limit;
-            maxLocal = max(maxLocal, max(forward, cnt));
-            pairsLocal += min(forward, limit);
-        }
-        __syncthreads();
-        for (uint32_t k = lane; k < g.nb; k += BLOCK) {
-            uint32_t hits = g.bodyHits()[k];
-            maxLocal = max(maxLocal, hits);
-            if (hits > limit) droppedLocal += hits - limit;
-        }
-        atomicAdd(&S.red.counters[0], droppedLocal);
-        atomicMax(&S.red.counters[1], maxLocal);
-        atomicAdd(&S.red.counters[2], pairsLocal);
-        uint32_t total;
-        uint32_t base = block_exclusive_scan<BLOCK>(S.red, mine, total);
-#pragma unroll
-        for (int s = 0; s < PER; ++s) { rs[s] = base; base += rl[s]; }
-        entriesTotal = total;
-        ov0 += S.red.counters[0];
-        ov1 = max(ov1, S.red.counters[1]);
-        pairs = S.red.counters[2];
-        rows.inLds = ldsRowsAllowed && total <= (uint32_t)ROWCAP;
-        if (!rows.inLds && total > d->adjCapacity) return false;
-        // Pass 2: fill the rows and order the pair block (Contact:222-238).
-#pragma unroll
-        for (int s = 0; s < PER; ++s) {
-            if (!valid[s]) continue;
-            uint32_t forward, cnt = 0;
-            if (rows.inLds) {
-                uint16_t *row = S.rows + rs[s];
-                enumerate(lane + s * BLOCK, p[s], mg[s], [&](uint32_t e) __attribute__((always_inline)) {
-                    row[cnt++] = (uint16_t)(((e >> 30) << kLdsKindShift) | (e & kIndexMask));
-                }, forward);
-                uint32_t pc = 0;
-                while (pc < cnt && (row[pc] >> kLdsKindShift) == 0u) ++pc;
-                uint32_t qc = pc;
-                while (qc < cnt && (row[qc] >> kLdsKindShift) == 1u) ++qc;
-                rp[s] = pc; rq[s] = qc;
-                for (uint32_t x = 1; x < pc; ++x) {
-                    uint16_t val = row[x];
-                    uint32_t loc = x;
-                    while (loc > 0 && val < row[loc - 1]) { row[loc] = row[loc - 1]; --loc; }
-                    row[loc] = val;
-                }
-            } else {
-                uint32_t *row = A->adj + d->adjOffset + rs[s];
-                enumerate(lane + s * BLOCK, p[s], mg[s], [&](uint32_t e) __attribute__((always_inline)) { row[cnt++] = e; }, forward);
-                uint32_t pc = 0;
-                while (pc < cnt && (row[pc] >> kKindShift) == 0u) ++pc;
-                uint32_t qc = pc;
-                while (qc < cnt && (row[qc] >> kKindShift) == 1u) ++qc;
-                rp[s] = pc; rq[s] = qc;
-                for (uint32_t x = 1; x < pc; ++x) {
-                    uint32_t val = row[x];
-                    uint32_t loc = x;
-                    while (loc > 0 && val < row[loc - 1]) { row[loc] = row[loc - 1]; --loc; }
-                    row[loc] = val;
-                }
-            }
-        }
-        __syncthreads();
-        return true;
-    };
-
-    // F: forces, kick (phase 1) or initial acceleration (phase 2), omega^2, margin bound.
-    float w2Max = -1.0f, mbMin = inff(), dbMin = inff();
-    uint32_t w2Own = 0, mbOwn = 0, dbOwn = 0;
-    auto pass_forces = [&](uint32_t phase, bool kick) __attribute__((always_inline)) {
-        const float fraction = step_fraction(c);
-        refresh_frames(fraction);
-        const float halfStep = 0.5f * c.h;
-        float bestW = -1.0f, bestM = inff();
-        uint32_t ownW = 0, ownM = 0;
-#pragma unroll
-        for (int s = 0; s < PER; ++s) {
-            if (!valid[s]) continue;
-            const uint32_t i = lane + s * BLOCK;
-            const float2 pi = p[s];
-            const float invm = S.invm[i], ri = S.r[i], Ki = S.K[i];
-            float2 total = f2(0.0f, 0.0f);
-            float w2 = 0.0f;
-            if (invm > 0.0f && bi[s] >= 0) w2 = w2 + g.band(bi[s]).z * invm;
-            // The row is (pairs, segments, bodies): the reference's summation order (Force:216-255).
-            const uint32_t start = rs[s], pairEnd = start + rp[s], segEnd = start + rq[s], end = start + rl[s];
-            for (uint32_t e = start; e < pairEnd; ++e) {
-                const uint32_t j = rows.index(e);
-                const uint32_t lo = min(i, j), hi = max(i, j);
-                const float2 delta = sub(S.pos[hi], S.pos[lo]);
-                const float dd = len2(delta);
-                const float2 normal = dd > 0.0f ? dvs(delta, dd) : f2(1.0f, 0.0f);
-                const Law law = pair_law(S.K[lo], S.r[lo], S.K[hi], S.r[hi], dd);
-                total = add(total, mul(i == lo ? -law.force : law.force, normal));
-                if (invm > 0.0f) w2 = w2 + law.stiffness * (invm + __builtin_sqrtf(invm * S.invm[j]));
-            }
-            for (uint32_t e = pairEnd; e < segEnd; ++e) {
-                float2 normal; float dd;
-                if (!wall_region(segment_at(rows.index(e)), pi, normal, dd) || !(dd < ri)) continue;
-                const Law law = wall_law(Ki, ri, dd);
-                total = add(total, mul(law.force, normal));
-                if (invm > 0.0f) w2 = w2 + law.stiffness * invm;
-            }
-            for (uint32_t e = segEnd; e < end; ++e) {
-                const Obstacle ob = frame_obstacle(frame_at(rows.index(e)), S.verts, S.tris, pi);
-                const Law law = wall_law(Ki, ri, ob.distanceValue);
-                total = add(total, mul(law.force, ob.normal));
-                if (invm > 0.0f) w2 = w2 + law.stiffness * invm;
-            }
-            if (kick) {
-                if (phase == 2) {
-                    a[s] = add(mul(total, invm), a0_of(s, invm, pi));
-                } else if (invm > 0.0f) {
-                    const float gamma = gm[s];
-                    const float decay = bpm::expf_det(-gamma * halfStep), factor = half_impulse_factor(gamma, halfStep);
-                    a[s] = add(mul(total, invm), a0_of(s, invm, pi));
-                    v[s] = add(mul(v[s], decay), mul(a[s], factor));
-                }
-                S.vel[i] = invm > 0.0f ? v[s] : f2(0.0f, 0.0f);
-                S.aux[i] = len2(a[s]);
-            }
-            if (w2 > bestW) { bestW = w2; ownW = i; }
-            float mb;
-            if (!(invm > 0.0f)) mb = inff();
-            else {
-                const float left = mg[s] - len2(sub(pi, ref[s]));
-                if (isnan(left)) mb = inff();
-                else if (left <= 0.0f) mb = 0.0f;
-                else {
-                    const float speed = len2(v[s]), acl = len2(a[s]);
-                    const float den = speed + __builtin_sqrtf(speed * speed + 2.0f * acl * left);
-                    mb = den > 0.0f ? 2.0f * left / den : inff();
-                }
-            }
-            if (mb < bestM) { bestM = mb; ownM = i; }
-        }
-        block_maxmin<BLOCK>(S.red, bestW, ownW, bestM, ownM);
-        w2Max = bestW; w2Own = ownW; mbMin = bestM; mbOwn = ownM;
-    };
-
-    // H: d/4 bound (Force:550-598).
-    auto pass_distance = [&](float hMax) __attribute__((always_inline)) {
-        const float fraction = step_fraction(c);
-        refresh_frames(fraction);
-        float bestD = inff();
-        uint32_t ownD = 0;
-        // One candidate's bound (Force:563-597); returns +inf for an exempt or degenerate one.
-        auto bound = [&](float dd, float approach, float accSum, float reach, float bodySpeed) __attribute__((always_inline)) -> float {
-            const bool active = dd < reach;
-            const float dc = fmaxf(dd, reach * 1e-6f);
-            float speed;
-            if (active) speed = fabsf(approach) + bodySpeed;
-            else {
-                if (approach + bodySpeed + accSum * hMax * 0.5f <= 0.0f) return inff();
-                speed = approach + bodySpeed;
-            }
-            const float sq = __builtin_sqrtf(speed * speed + accSum * dc * 0.5f);
-            const float den = sq + speed;
-            if (!(den > 0.0f)) return inff();
-            return dc / (2.0f * den);
-        };
-#pragma unroll
-        for (int s = 0; s < PER; ++s) {
-            if (!valid[s]) continue;
-            const uint32_t i = lane + s * BLOCK;
-            if (!(S.invm[i] > 0.0f)) continue;
-            const float2 pi = p[s], vi = v[s];
-            const float ri = S.r[i], accI = S.aux[i];
-            float best = inff();
-            const uint32_t start = rs[s], pairEnd = start + rp[s], segEnd = start + rq[s], end = start + rl[s];
-            for (uint32_t e = start; e < pairEnd; ++e) {
-                const uint32_t j = rows.index(e);
-                const float2 delta = sub(S.pos[j], pi);
-                const float dd = len2(delta);
-                const float2 normal = dd > 0.0f ? dvs(delta, dd) : f2(1.0f, 0.0f);
-                best = fminf(best, bound(dd, dot2(sub(vi, S.vel[j]), normal), accI + S.aux[j], ri + S.r[j], 0.0f));
-            }
-            for (uint32_t e = pairEnd; e < segEnd; ++e) {
-                float2 normal; float dd;
-                if (!wall_region(segment_at(rows.index(e)), pi, normal, dd)) continue;
-                best = fminf(best, bound(dd, -dot2(vi, normal), accI, ri, 0.0f));
-            }
-            for (uint32_t e = segEnd; e < end; ++e) {
-                const Obstacle ob = frame_obstacle(frame_at(rows.index(e)), S.verts, S.tris, pi);
-                best = fminf(best, bound(ob.distanceValue, -dot2(vi, ob.normal), accI, ri, ob.speedBound));
-            }
-            if (best < bestD) { bestD = best; ownD = i; }
-        }
-        block_min<BLOCK>(S.red, bestD, ownD);
-        dbMin = bestD; dbOwn = ownD;
-    };
-
-    // D: kick and drift, phase 0 (Force:775-791).
-    auto pass_drift = [&]() __attribute__((always_inline)) {
-        const float h = c.h, halfStep = 0.5f * h;
-#pragma unroll
-        for (int s = 0; s < PER; ++s) {
-            if (!valid[s]) continue;
-            const uint32_t i = lane + s * BLOCK;
-            if (!(S.invm[i] > 0.0f)) continue;
-            const float gamma = gm[s];
-            const float decay = bpm::expf_det(-gamma * halfStep), factor = half_impulse_factor(gamma, halfStep);
-            const float2 vn = add(mul(v[s], decay), mul(a[s], factor));
-            float angular = om[s] * bpm::expf_det(-gamma * h);
-            if (fl[s] & kHasTorque) angular = angular + mq[s] * half_impulse_factor(gamma, h);
-            p[s] = add(p[s], mul(vn, h));
-            v[s] = vn;
-            rot[s] = rot[s] + om[s] * h;
-            om[s] = angular;
+            uint32_t gi = o + i;
+            p[s] = A->pos[pub][gi];
+            v[s] = A->vel[pub][gi];
+            rot[s] = A->rot[pub][gi];
+            om[s] = A->omega[pub][gi];
+            fl[s] = A->flags[gi];
+            float invmRaw = A->invm[gi];
+            bool isStatic = !(invmRaw > 0.0f) || ((fl[s] & kEnabled) && (fl[s] & kStatic));
+            if ((fl[s] & kEnabled) && (fl[s] & kStatic)) { v[s] = f2(0.0f, 0.0f); om[s] = 0.0f; }
+            startG[i] = p[s];
             S.pos[i] = p[s];
-        }
-    };
-
-    // Contact assertion (Force:865-918): lowest violating bubble writes the record.
-    auto assert_contacts = [&]() __attribute__((always_inline)) -> bool {
-        if (!(cellWidth > 0.0f)) cellWidth = 1.0f;
-        __syncthreads();
-        build_hash();
-        const float fraction = step_fraction(c);
-        uint32_t firstIndex = 0xffffffffu;
-        Violation mine = {0u, 0u, 0, 0.0f};
-        const uint32_t *regionG = A->region[ws] + o;
-#pragma unroll
-        for (int s = 0; s < PER; ++s) {
-            if (!valid[s] || firstIndex != 0xffffffffu) continue;
-            const uint32_t i = lane + s * BLOCK;
-            const float2 center = p[s];
-            if (!finite2(center) || regionG[i] == BP_REGION_REMOVED) continue;
-            const bool movable = S.invm[i] > 0.0f;
-            uint32_t bestJ = 0xffffffffu;
-            float bestD = 0.0f;
-            const int64_t cx = cell_coord(center.x, cellWidth), cy = cell_coord(center.y, cellWidth);
-            for (int dx = -1; dx <= 1; ++dx)
-                for (int dy = -1; dy <= 1; ++dy) {
-                    const int64_t tx = cx + dx, ty = cy + dy;
-                    const uint32_t b = bucket_of(tx, ty, buckets);
-                    const uint32_t lo = b == 0 ? 0u : S.buckets[b - 1], hi = S.buckets[b];
-                    for (uint32_t k = lo; k < hi; ++k) {
-                        const uint32_t j = S.grid[k];
-                        if (j <= i || j >= bestJ) continue;
-                        const float2 q = S.pos[j];
-                        if (cell_coord(q.x, cellWidth) != tx || cell_coord(q.y, cellWidth) != ty) continue;
-                        const float dd = len2(sub(q, center));
-                        if (!(dd <= 0.0f)) continue;
-                        if (regionG[j] == BP_REGION_REMOVED || !filter_accepts(catG[i], maskG[i], catG[j], maskG[j])) continue;
-                        if (!movable && !(S.invm[j] > 0.0f)) continue;
-                        bestJ = j; bestD = dd;
-                    }
-                }
-            if (bestJ != 0xffffffffu) {
-                firstIndex = i; mine.kind = BP_VIOLATION_PAIR; mine.other = (uint64_t)ids[bestJ]; mine.detail = bestD;
-                continue;
+            S.vel[i] = isStatic ? f2(0.0f, 0.0f) : v[s];
+            S.r[i] = A->radius[gi];
+            S.K[i] = A->Kc[gi];
+            S.invm[i] = isStatic ? 0.0f : invmRaw;
+            // Force records (Force:423-438, resolved to indices on the host) and drag, per step.
+            bi[s] = A->band[gi];
+            const int32_t mi = A->motor[gi];
+            const bool driven = (fl[s] & kForceDriven) != 0;
+            gm[s] = driven ? fmaxf(A->gamma[gi], 0.0f) : 0.0f;
+            fl[s] &= ~(kHasMotor | kHasTorque);
+            if (mi >= 0) {
+                const float4 md = g.motor(mi);
+                fl[s] |= kHasMotor;
+                mf[s] = f2(md.x, md.y);
+                if (driven && md.w > 0.0f) { fl[s] |= kHasTorque; mq[s] = md.z / md.w; }
             }
-            if (!movable) continue;
-            if (assert_body_and_edges(g, center, startG[i], catG[i], maskG[i], fraction, mine)) firstIndex = i;
         }
-        const uint32_t winner = block_min_u32<BLOCK>(S.red, firstIndex);
-        if (winner == 0xffffffffu) return false;
-        if (firstIndex == winner) {
-            st->violationKind = mine.kind;
-            st->violationSegment = mine.segment;
-            st->violationObjects[0] = (uint64_t)ids[winner];
-            st->violationObjects[1] = mine.other;
-            st->violationDetail = mine.detail;
-        }
-        return true;
-    };
-
-    // ---- 2-3. Grid build and the substep loop (Force:1257-1304).
-    if (n > 0) {
-        bool ok = build();
-        if (!ok) status = BP_ERR_CAPACITY_OVERFLOW;
-        if (status == BP_OK) {
-            pass_forces(2u, true);
-            for (uint32_t iteration = 0; iteration < 2 * kFrameLimit + 2; ++iteration) {
-                const float remaining = c.t0 - c.t1;
-                const bool over = !(remaining > 0.0f), stopped = c.rebuild != 0 || c.limit != 0;
-                const bool limited = c.c1 >= kFrameLimit;
-                if (over || stopped || limited) {
-                    if (limited && !over && !stopped) c.limit = 1;
-                    c.h = 0.0f;
-                } else {
-                    const float stiffnessBound = w2Max > 0.0f ? 1.0f / __builtin_sqrtf(w2Max) : inff();
-                    pass_distance(fminf(remaining, stiffnessBound));
-                    reduce_h(c, stiffnessBound, w2Own, dbMin, dbOwn, mbMin, mbOwn, ids, history, lane);
-                }
-                const bool finished = !(c.t0 - c.t1 > 0.0f);
-                if (!(c.h > 0.0f)) {
-                    if (c.limit || finished || !c.rebuild) break;
-                    c.c0 = 0; c.rebuild = 0;
-                    ov2 = max(ov2, ov0); ov0 = 0; ov3 += 1;
-                    __syncthreads();
-                    if (!build()) { status = BP_ERR_CAPACITY_OVERFLOW; break; }
-                    pass_forces(2u, true);
-                    continue;
-                }
-                pass_drift();
-                __syncthreads();
-                pass_forces(1u, true);
-                if (d->injectionEnabled && c.c1 == d->injectionAfter + 1) {
-#pragma unroll
-                    for (int s = 0; s < PER; ++s) {
-                        const uint32_t i = lane + s * BLOCK;
-                        if (valid[s] && ids[i] == d->injectionObject) {
-                            p[s] = f2(d->injectionPosition[0], d->injectionPosition[1]);
-                            S.pos[i] = p[s];
-                        }
-                    }
-                    __syncthreads();
-                    pass_forces(1u, false);
-                }
-                if (d->assertEverySubstep && !claimed) {
-                    if (assert_contacts()) claimed = true;
-                    __syncthreads();
-                }
-            }
-            ov2 = max(ov2, ov0);
-        }
-    }
-
-    // ---- 4. Gated end of step: region, finite check, assertion (NativeMetalSolver.swift:837-945).
-    const bool finished = c.limit == 0 && !(c.t0 - c.t1 > 0.0f);
-    uint32_t region[PER];
-#pragma unroll
-    for (int s = 0; s < PER; ++s) region[s] = BP_REGION_UNCHANGED;
-    if (status == BP_OK && finished) {
-        const float2 lo = f2(d->region[0], d->region[1]), hi = f2(d->region[2], d->region[3]);
-#pragma unroll
-        for (int s = 0; s < PER; ++s) {
-            if (!valid[s]) continue;
-            const uint32_t i = lane + s * BLOCK;
-            if (d->applyRegion) {
-                float2 startI = startG[i];
-                region[s] = apply_region(p[s], startI, S.r[i], fl[s], lo, hi, safety);
-                if (region[s] == BP_REGION_WRAPPED) { startG[i] = startI; S.pos[i] = p[s]; }
-            }
-            const uint32_t gi = o + i;
-            A->region[ws][gi] = region[s];
-            if (!finite2(p[s]) || !finite2(startG[i]) || !finite2(v[s]) || !isfinite(rot[s]) || !isfinite(om[s]) ||
-                !isfinite(A->mass[gi]) || !isfinite(A->invm[gi]) || !isfinite(S.r[i]))
-                safety |= 1u;
+        uint32_t safety = predict_bodies<BLOCK>(A, d, pub, ws);
+        // Geometry into LDS. Bodies are read back by the lane that predicted them.
+        for (uint32_t k = lane; k < g.nb; k += BLOCK) {
+            const BodyView b = g.body(k);
+            S.bodyPose[k] = b.pose;
+            S.bodyExtent[k] = make_float4(b.prev.x, b.prev.y, b.shape.x, b.shape.y);
+            S.bodyMotion[k] = b.motion;
+            S.bodyRanges[k] = b.ranges;
+            S.bodyKind[k] = b.kind;
         }
         for (uint32_t j = lane; j < g.ns; j += BLOCK) {
             const Seg sg = g.segment(j);
-            const float2 e = sub(sg.b, sg.a);
-            if (!finite2(sg.a) || !finite2(sg.b) || !finite2(e) || !isfinite(dot2(e, e))) safety |= 1u;
+            S.segAB[j] = make_float4(sg.a.x, sg.a.y, sg.b.x, sg.b.y);
+            S.segRegion[j] = make_float4(sg.prevStart.x, sg.prevStart.y, sg.base.x, sg.base.y);
+            S.segOutward[j] = sg.outward;
+            S.segFlags[j] = sg.flags;
+            S.segCat[j] = g.segCat(j);
+            S.segMask[j] = g.segMask(j);
+        }
+        for (uint32_t k = lane; k < d->vertexCount; k += BLOCK) S.verts[k] = A->verts[d->vertexOffset + k];
+        for (uint32_t k = lane; k < d->triangleCount; k += BLOCK) {
+            const uint4 t = A->tris[d->triangleOffset + k];
+            S.tris[k] = t.x | (t.y << 10) | (t.z << 20);
         }
         __syncthreads();
-        if (!claimed && n > 0 && assert_contacts()) claimed = true;
-    }
 
-    // ---- Write-back of the working side, counters and publication.
-    uint32_t rmLocal = 0, wrLocal = 0;
-#pragma unroll
-    for (int s = 0; s < PER; ++s) {
-        if (!valid[s]) continue;
-        const uint32_t gi = o + lane + s * BLOCK;
-        A->pos[ws][gi] = p[s];
-        A->vel[ws][gi] = v[s];
-        A->rot[ws][gi] = rot[s];
-        A->omega[ws][gi] = om[s];
-        if (!(status == BP_OK && finished)) A->region[ws][gi] = BP_REGION_UNCHANGED;
-        rmLocal += region[s] == BP_REGION_REMOVED;
-        wrLocal += region[s] == BP_REGION_WRAPPED;
-        if (d->exportAdjacency) {
-            A->rowStart[gi] = rs[s];
-            A->rowLen[gi] = rl[s];
-            if (rows.inLds)
-                for (uint32_t e = 0; e < rl[s]; ++e) {
-                    uint32_t kind, j;
-                    rows.get(rs[s] + e, kind, j);
-                    A->adj[d->adjOffset + rs[s] + e] = (kind << kKindShift) | j;
+        Ctl c;
+        c.t0 = d->dt; c.t1 = n == 0 ? d->dt : 0.0f; c.h = 0.0f; c.split = (float)(d->split < 1 ? 1u : d->split);
+        c.c0 = c.c1 = c.rebuild = c.limit = 0; c.limObj = 0; c.limTerm = BP_TERM_REMAINING;
+        uint32_t ov0 = 0, ov1 = 0, ov2 = 0, ov3 = 0, pairs = 0, entriesTotal = 0;
+        int32_t status = BP_OK;
+        bool claimed = false;
+        float cellWidth = 0.0f;
+        RowRef rows;
+        rows.lds = S.rows;
+        rows.global = A->adj + d->adjOffset;
+        rows.inLds = true;
+        const bool ldsRowsAllowed = ROWCAP > 0 && g.ns < kLdsIndexLimit && g.nb < kLdsIndexLimit;
+
+        auto segment_at = [&](uint32_t j) __attribute__((always_inline)) -> Seg {
+            Seg sg;
+            const float4 ab = S.segAB[j], rg = S.segRegion[j];
+            sg.a = f2(ab.x, ab.y); sg.b = f2(ab.z, ab.w);
+            sg.prevStart = f2(rg.x, rg.y);
+            sg.base = f2(rg.z, rg.w);
+            sg.outward = S.segOutward[j];
+            sg.flags = S.segFlags[j];
+            return sg;
+        };
+        // Body frames at the step fraction of the current pass, evaluated once per fraction.
+        float framesAt = -1.0f;
+        auto refresh_frames = [&](float fraction) __attribute__((always_inline)) {
+            if (g.nb == 0 || fraction == framesAt) return;
+            __syncthreads();
+            for (uint32_t k = lane; k < g.nb; k += BLOCK) {
+                const float4 ex = S.bodyExtent[k];
+                float2 at;
+                float4 cs;
+                frame_pose(S.bodyPose[k], f2(ex.x, ex.y), fraction, at, cs);
+                S.frameAt[k] = at;
+                S.frameCs[k] = cs;
+            }
+            __syncthreads();
+            framesAt = fraction;
+        };
+        auto frame_at = [&](uint32_t k) __attribute__((always_inline)) -> BodyFrame {
+            BodyFrame f;
+            const float4 cs = S.frameCs[k];
+            f.at = S.frameAt[k];
+            f.c = cs.x; f.s = cs.y; f.cn = cs.z; f.sn = cs.w;
+            f.core = S.bodyExtent[k].w;
+            f.motion = S.bodyMotion[k];
+            f.ranges = S.bodyRanges[k];
+            f.round = S.bodyKind[k].x != 0u;
+            return f;
+        };
+        auto a0_of = [&](int s, float invm, float2 pi) __attribute__((always_inline)) -> float2 {
+            return g.non_contact_with(fl[s], invm, ids + lane + s * BLOCK, bi[s], (fl[s] & kHasMotor) != 0u, mf[s], pi);
+        };
+        // Candidate tests against the staged geometry (Geometry:279-290, 361-379).
+        auto segment_candidate = [&](float2 pc, float radius, float margin, uint32_t cat, uint32_t mask, uint32_t j)
+            __attribute__((always_inline)) -> bool {
+            if (!filter_accepts(cat, mask, S.segCat[j], S.segMask[j])) return false;
+            const float4 ab = S.segAB[j];
+            const float reach = radius + margin + sepTol;
+            const float2 lower = subs(min2(f2(ab.x, ab.y), f2(ab.z, ab.w)), reach);
+            const float2 upper = adds(max2(f2(ab.x, ab.y), f2(ab.z, ab.w)), reach);
+            return pc.x >= lower.x && pc.y >= lower.y && pc.x <= upper.x && pc.y <= upper.y;
+        };
+        auto body_candidate = [&](float2 pc, float radius, float margin, uint32_t cat, uint32_t mask, uint32_t k)
+            __attribute__((always_inline)) -> bool {
+            const uint4 kind = S.bodyKind[k];
+            const bool isStatic = (kind.y & kBodyStatic) != 0;
+            const bool obstacle = isStatic ? (kind.x != 0u) : ((kind.y & kBodyKinematic) != 0u);
+            if (!obstacle) return false;
+            if (!filter_accepts(cat, mask, kind.z, kind.w)) return false;
+            const float4 pose = S.bodyPose[k], ex = S.bodyExtent[k];
+            const float2 prev = f2(ex.x, ex.y);
+            const float br = ex.z;
+            const float2 bodyLow = subs(min2(f2(pose.x, pose.y), prev), br);
+            const float2 bodyHigh = adds(max2(f2(pose.x, pose.y), prev), br);
+            const float reach = radius + fmaxf(margin, 0.0f);
+            const float2 lo = subs(pc, reach), hi = adds(pc, reach);
+            return !(hi.x < bodyLow.x || hi.y < bodyLow.y || bodyHigh.x < lo.x || bodyHigh.y < lo.y);
+        };
+
+        // Hashed grid of the current centres in LDS (Force:1066-1105).
+        auto build_hash = [&]() __attribute__((always_inline)) {
+            for (uint32_t b = lane; b < buckets; b += BLOCK) S.buckets[b] = 0;
+            __syncthreads();
+    #pragma unroll
+            for (int s = 0; s < PER; ++s) {
+                if (!valid[s] || !finite2(p[s])) continue;
+                atomicAdd(&S.buckets[bucket_of(cell_coord(p[s].x, cellWidth), cell_coord(p[s].y, cellWidth), buckets)], 1u);
+            }
+            __syncthreads();
+            bucket_prefix<BLOCK>(S.red, S.buckets, buckets);
+            __syncthreads();
+    #pragma unroll
+            for (int s = 0; s < PER; ++s) {
+                if (!valid[s] || !finite2(p[s])) continue;
+                uint32_t slot = atomicAdd(&S.buckets[bucket_of(cell_coord(p[s].x, cellWidth), cell_coord(p[s].y, cellWidth), buckets)], 1u);
+                S.grid[slot] = (uint16_t)(lane + s * BLOCK);
+            }
+            __syncthreads();
+        };
+
+        // Adjacency row of bubble i (pairs both ways from the 3x3 cells, segments, bodies).
+        auto enumerate = [&](const uint32_t i, const float2 pi, const float mi, auto emit, uint32_t &forward) __attribute__((always_inline)) {
+            forward = 0;
+            const float ri = S.r[i];
+            const uint32_t ci = catG[i], ki = maskG[i];
+            if (finite2(pi)) {
+                const bool ownRow = finite2(startG[i]);
+                const int64_t cx = cell_coord(pi.x, cellWidth), cy = cell_coord(pi.y, cellWidth);
+                for (int dx = -1; dx <= 1; ++dx)
+                    for (int dy = -1; dy <= 1; ++dy) {
+                        const int64_t tx = cx + dx, ty = cy + dy;
+                        const uint32_t b = bucket_of(tx, ty, buckets);
+                        const uint32_t lo = b == 0 ? 0u : S.buckets[b - 1], hi = S.buckets[b];
+                        for (uint32_t k = lo; k < hi; ++k) {
+                            const uint32_t j = S.grid[k];
+                            if (j == i) continue;
+                            const float2 q = S.pos[j];
+                            if (cell_coord(q.x, cellWidth) != tx || cell_coord(q.y, cellWidth) != ty) continue;
+                            const uint32_t l = min(i, j), h = max(i, j);
+                            const float2 pl = l == i ? pi : q, ph = l == i ? q : pi;
+                            const float ml = l == i ? mi : S.aux[j], mh = l == i ? S.aux[j] : mi;
+                            const float reach = S.r[l] + S.r[h] + ml + mh + sepTol;
+                            if (!(fabsf(pl.x - ph.x) <= reach && fabsf(pl.y - ph.y) <= reach)) continue;
+                            if (!filter_accepts(catG[l], maskG[l], catG[h], maskG[h])) continue;
+                            if (j > i) {
+                                if (!ownRow) continue;
+                                ++forward;
+                            } else if (!finite2(startG[j])) {
+                                continue;
+                            }
+                            emit(j);
+                        }
+                    }
+                if (ownRow)
+                    for (uint32_t j = 0; j < g.ns; ++j)
+                        if (segment_candidate(pi, ri, mi, ci, ki, j)) { ++forward; emit((1u << 30) | j); }
+            }
+            for (uint32_t k = 0; k < g.nb; ++k)
+                if (body_candidate(pi, ri, mi, ci, ki, k)) emit((2u << 30) | k);
+        };
+
+        // Grid build (Force:1040-1134) with CSR rows. Returns false when the rows do not fit.
+        auto build = [&]() __attribute__((always_inline)) -> bool {
+            float remaining = fmaxf(c.t0 - c.t1, 0.0f);
+            float width = sepTol;
+    #pragma unroll
+            for (int s = 0; s < PER; ++s) {
+                if (!valid[s]) continue;
+                uint32_t i = lane + s * BLOCK;
+                float2 a0 = a0_of(s, S.invm[i], p[s]);
+                float travel = len2(S.vel[i]) * remaining + 0.5f * len2(a0) * remaining * remaining;
+                mg[s] = d->minimumMarginOnly ? 0.25f * S.r[i] : fmaxf(1.05f * travel, 0.25f * S.r[i]);
+                ref[s] = p[s];
+                S.aux[i] = mg[s];
+                width = fmaxf(width, 2.0f * (S.r[i] + mg[s]) + sepTol);
+            }
+            for (uint32_t k = lane; k < g.nb; k += BLOCK) g.bodyHits()[k] = 0;
+            width = block_maxf<BLOCK>(S.red, width);
+            cellWidth = fmaxf(width, kFloatMin);
+            if (lane < 8) S.red.counters[lane] = 0;
+            build_hash();
+            // Pass 1: row lengths and the reference's overflow counters (Geometry:291-297, 410-414).
+            uint32_t droppedLocal = 0, maxLocal = 0, pairsLocal = 0, mine = 0;
+    #pragma unroll
+            for (int s = 0; s < PER; ++s) {
+                if (!valid[s]) continue;
+                uint32_t cnt = 0, forward = 0;
+                enumerate(lane + s * BLOCK, p[s], mg[s], [&](uint32_t e) __attribute__((always_inline)) {
+                    ++cnt;
+                    if ((e >> 30) == 2u) atomicAdd(&g.bodyHits()[e & kIndexMask], 1u);
+                }, forward);
+                rl[s] = cnt;
+                mine += cnt;
+                if (forward > limit) droppedLocal += forward - limit;
+                if (cnt > limit) droppedLocal += cnt - limit;
+                maxLocal = max(maxLocal, max(forward, cnt));
+                pairsLocal += min(forward, limit);
+            }
+            __syncthreads();
+            for (uint32_t k = lane; k < g.nb; k += BLOCK) {
+                uint32_t hits = g.bodyHits()[k];
+                maxLocal = max(maxLocal, hits);
+                if (hits > limit) droppedLocal += hits - limit;
+            }
+            atomicAdd(&S.red.counters[0], droppedLocal);
+            atomicMax(&S.red.counters[1], maxLocal);
+            atomicAdd(&S.red.counters[2], pairsLocal);
+            uint32_t total;
+            uint32_t base = block_exclusive_scan<BLOCK>(S.red, mine, total);
+    #pragma unroll
+            for (int s = 0; s < PER; ++s) { rs[s] = base; base += rl[s]; }
+            entriesTotal = total;
+            ov0 += S.red.counters[0];
+            ov1 = max(ov1, S.red.counters[1]);
+            pairs = S.red.counters[2];
+            rows.inLds = ldsRowsAllowed && total <= (uint32_t)ROWCAP;
+            if (!rows.inLds && total > d->adjCapacity) return false;
+            // Pass 2: fill the rows and order the pair block (Contact:222-238).
+    #pragma unroll
+            for (int s = 0; s < PER; ++s) {
+                if (!valid[s]) continue;
+                uint32_t forward, cnt = 0;
+                if (rows.inLds) {
+                    uint16_t *row = S.rows + rs[s];
+                    enumerate(lane + s * BLOCK, p[s], mg[s], [&](uint32_t e) __attribute__((always_inline)) {
+                        row[cnt++] = (uint16_t)(((e >> 30) << kLdsKindShift) | (e & kIndexMask));
+                    }, forward);
+                    uint32_t pc = 0;
+                    while (pc < cnt && (row[pc] >> kLdsKindShift) == 0u) ++pc;
+                    uint32_t qc = pc;
+                    while (qc < cnt && (row[qc] >> kLdsKindShift) == 1u) ++qc;
+                    rp[s] = pc; rq[s] = qc;
+                    for (uint32_t x = 1; x < pc; ++x) {
+                        uint16_t val = row[x];
+                        uint32_t loc = x;
+                        while (loc > 0 && val < row[loc - 1]) { row[loc] = row[loc - 1]; --loc; }
+                        row[loc] = val;
+                    }
+                } else {
+                    uint32_t *row = A->adj + d->adjOffset + rs[s];
+                    enumerate(lane + s * BLOCK, p[s], mg[s], [&](uint32_t e) __attribute__((always_inline)) { row[cnt++] = e; }, forward);
+                    uint32_t pc = 0;
+                    while (pc < cnt && (row[pc] >> kKindShift) == 0u) ++pc;
+                    uint32_t qc = pc;
+                    while (qc < cnt && (row[qc] >> kKindShift) == 1u) ++qc;
+                    rp[s] = pc; rq[s] = qc;
+                    for (uint32_t x = 1; x < pc; ++x) {
+                        uint32_t val = row[x];
+                        uint32_t loc = x;
+                        while (loc > 0 && val < row[loc - 1]) { row[loc] = row[loc - 1]; --loc; }
+                        row[loc] = val;
+                    }
                 }
+            }
+            __syncthreads();
+            return true;
+        };
+
+        // F: forces, kick (phase 1) or initial acceleration (phase 2), omega^2, margin bound.
+        float w2Max = -1.0f, mbMin = inff(), dbMin = inff();
+        uint32_t w2Own = 0, mbOwn = 0, dbOwn = 0;
+        auto pass_forces = [&](uint32_t phase, bool kick) __attribute__((always_inline)) {
+            const float fraction = step_fraction(c);
+            refresh_frames(fraction);
+            const float halfStep = 0.5f * c.h;
+            float bestW = -1.0f, bestM = inff();
+            uint32_t ownW = 0, ownM = 0;
+    #pragma unroll
+            for (int s = 0; s < PER; ++s) {
+                if (!valid[s]) continue;
+                const uint32_t i = lane + s * BLOCK;
+                const float2 pi = p[s];
+                const float invm = S.invm[i], ri = S.r[i], Ki = S.K[i];
+                float2 total = f2(0.0f, 0.0f);
+                float w2 = 0.0f;
+                if (invm > 0.0f && bi[s] >= 0) w2 = w2 + g.band(bi[s]).z * invm;
+                // The row is (pairs, segments, bodies): the reference's summation order (Force:216-255).
+                const uint32_t start = rs[s], pairEnd = start + rp[s], segEnd = start + rq[s], end = start + rl[s];
+                for (uint32_t e = start; e < pairEnd; ++e) {
+                    const uint32_t j = rows.index(e);
+                    const uint32_t lo = min(i, j), hi = max(i, j);
+                    const float2 delta = sub(S.pos[hi], S.pos[lo]);
+                    const float dd = len2(delta);
+                    const float2 normal = dd > 0.0f ? dvs(delta, dd) : f2(1.0f, 0.0f);
+                    const Law law = pair_law(S.K[lo], S.r[lo], S.K[hi], S.r[hi], dd);
+                    total = add(total, mul(i == lo ? -law.force : law.force, normal));
+                    if (invm > 0.0f) w2 = w2 + law.stiffness * (invm + __builtin_sqrtf(invm * S.invm[j]));
+                }
+                for (uint32_t e = pairEnd; e < segEnd; ++e) {
+                    float2 normal; float dd;
+                    if (!wall_region(segment_at(rows.index(e)), pi, normal, dd) || !(dd < ri)) continue;
+                    const Law law = wall_law(Ki, ri, dd);
+                    total = add(total, mul(law.force, normal));
+                    if (invm > 0.0f) w2 = w2 + law.stiffness * invm;
+                }
+                for (uint32_t e = segEnd; e < end; ++e) {
+                    const Obstacle ob = frame_obstacle(frame_at(rows.index(e)), S.verts, S.tris, pi);
+                    const Law law = wall_law(Ki, ri, ob.distanceValue);
+                    total = add(total, mul(law.force, ob.normal));
+                    if (invm > 0.0f) w2 = w2 + law.stiffness * invm;
+                }
+                if (kick) {
+                    if (phase == 2) {
+                        a[s] = add(mul(total, invm), a0_of(s, invm, pi));
+                    } else if (invm > 0.0f) {
+                        const float gamma = gm[s];
+                        const float decay = bpm::expf_det(-gamma * halfStep), factor = half_impulse_factor(gamma, halfStep);
+                        a[s] = add(mul(total, invm), a0_of(s, invm, pi));
+                        v[s] = add(mul(v[s], decay), mul(a[s], factor));
+                    }
+                    S.vel[i] = invm > 0.0f ? v[s] : f2(0.0f, 0.0f);
+                    S.aux[i] = len2(a[s]);
+                }
+                if (w2 > bestW) { bestW = w2; ownW = i; }
+                float mb;
+                if (!(invm > 0.0f)) mb = inff();
+                else {
+                    const float left = mg[s] - len2(sub(pi, ref[s]));
+                    if (isnan(left)) mb = inff();
+                    else if (left <= 0.0f) mb = 0.0f;
+                    else {
+                        const float speed = len2(v[s]), acl = len2(a[s]);
+                        const float den = speed + __builtin_sqrtf(speed * speed + 2.0f * acl * left);
+                        mb = den > 0.0f ? 2.0f * left / den : inff();
+                    }
+                }
+                if (mb < bestM) { bestM = mb; ownM = i; }
+            }
+            block_maxmin<BLOCK>(S.red, bestW, ownW, bestM, ownM);
+            w2Max = bestW; w2Own = ownW; mbMin = bestM; mbOwn = ownM;
+        };
+
+        // H: d/4 bound (Force:550-598).
+        auto pass_distance = [&](float hMax) __attribute__((always_inline)) {
+            const float fraction = step_fraction(c);
+            refresh_frames(fraction);
+            float bestD = inff();
+            uint32_t ownD = 0;
+            // One candidate's bound (Force:563-597); returns +inf for an exempt or degenerate one.
+            auto bound = [&](float dd, float approach, float accSum, float reach, float bodySpeed) __attribute__((always_inline)) -> float {
+                const bool active = dd < reach;
+                const float dc = fmaxf(dd, reach * 1e-6f);
+                float speed;
+                if (active) speed = fabsf(approach) + bodySpeed;
+                else {
+                    if (approach + bodySpeed + accSum * hMax * 0.5f <= 0.0f) return inff();
+                    speed = approach + bodySpeed;
+                }
+                const float sq = __builtin_sqrtf(speed * speed + accSum * dc * 0.5f);
+                const float den = sq + speed;
+                if (!(den > 0.0f)) return inff();
+                return dc / (2.0f * den);
+            };
+    #pragma unroll
+            for (int s = 0; s < PER; ++s) {
+                if (!valid[s]) continue;
+                const uint32_t i = lane + s * BLOCK;
+                if (!(S.invm[i] > 0.0f)) continue;
+                const float2 pi = p[s], vi = v[s];
+                const float ri = S.r[i], accI = S.aux[i];
+                float best = inff();
+                const uint32_t start = rs[s], pairEnd = start + rp[s], segEnd = start + rq[s], end = start + rl[s];
+                for (uint32_t e = start; e < pairEnd; ++e) {
+                    const uint32_t j = rows.index(e);
+                    const float2 delta = sub(S.pos[j], pi);
+                    const float dd = len2(delta);
+                    const float2 normal = dd > 0.0f ? dvs(delta, dd) : f2(1.0f, 0.0f);
+                    best = fminf(best, bound(dd, dot2(sub(vi, S.vel[j]), normal), accI + S.aux[j], ri + S.r[j], 0.0f));
+                }
+                for (uint32_t e = pairEnd; e < segEnd; ++e) {
+                    float2 normal; float dd;
+                    if (!wall_region(segment_at(rows.index(e)), pi, normal, dd)) continue;
+                    best = fminf(best, bound(dd, -dot2(vi, normal), accI, ri, 0.0f));
+                }
+                for (uint32_t e = segEnd; e < end; ++e) {
+                    const Obstacle ob = frame_obstacle(frame_at(rows.index(e)), S.verts, S.tris, pi);
+                    best = fminf(best, bound(ob.distanceValue, -dot2(vi, ob.normal), accI, ri, ob.speedBound));
+                }
+                if (best < bestD) { bestD = best; ownD = i; }
+            }
+            block_min<BLOCK>(S.red, bestD, ownD);
+            dbMin = bestD; dbOwn = ownD;
+        };
+
+        // D: kick and drift, phase 0 (Force:775-791).
+        auto pass_drift = [&]() __attribute__((always_inline)) {
+            const float h = c.h, halfStep = 0.5f * h;
+    #pragma unroll
+            for (int s = 0; s < PER; ++s) {
+                if (!valid[s]) continue;
+                const uint32_t i = lane + s * BLOCK;
+                if (!(S.invm[i] > 0.0f)) continue;
+                const float gamma = gm[s];
+                const float decay = bpm::expf_det(-gamma * halfStep), factor = half_impulse_factor(gamma, halfStep);
+                const float2 vn = add(mul(v[s], decay), mul(a[s], factor));
+                float angular = om[s] * bpm::expf_det(-gamma * h);
+                if (fl[s] & kHasTorque) angular = angular + mq[s] * half_impulse_factor(gamma, h);
+                p[s] = add(p[s], mul(vn, h));
+                v[s] = vn;
+                rot[s] = rot[s] + om[s] * h;
+                om[s] = angular;
+                S.pos[i] = p[s];
+            }
+        };
+
+        // Contact assertion (Force:865-918): lowest violating bubble writes the record.
+        auto assert_contacts = [&]() __attribute__((always_inline)) -> bool {
+            if (!(cellWidth > 0.0f)) cellWidth = 1.0f;
+            __syncthreads();
+            build_hash();
+            const float fraction = step_fraction(c);
+            uint32_t firstIndex = 0xffffffffu;
+            Violation mine = {0u, 0u, 0, 0.0f};
+            const uint32_t *regionG = A->region[ws] + o;
+    #pragma unroll
+            for (int s = 0; s < PER; ++s) {
+                if (!valid[s] || firstIndex != 0xffffffffu) continue;
+                const uint32_t i = lane + s * BLOCK;
+                const float2 center = p[s];
+                if (!finite2(center) || regionG[i] == BP_REGION_REMOVED) continue;
+                const bool movable = S.invm[i] > 0.0f;
+                uint32_t bestJ = 0xffffffffu;
+                float bestD = 0.0f;
+                const int64_t cx = cell_coord(center.x, cellWidth), cy = cell_coord(center.y, cellWidth);
+                for (int dx = -1; dx <= 1; ++dx)
+                    for (int dy = -1; dy <= 1; ++dy) {
+                        const int64_t tx = cx + dx, ty = cy + dy;
+                        const uint32_t b = bucket_of(tx, ty, buckets);
+                        const uint32_t lo = b == 0 ? 0u : S.buckets[b - 1], hi = S.buckets[b];
+                        for (uint32_t k = lo; k < hi; ++k) {
+                            const uint32_t j = S.grid[k];
+                            if (j <= i || j >= bestJ) continue;
+                            const float2 q = S.pos[j];
+                            if (cell_coord(q.x, cellWidth) != tx || cell_coord(q.y, cellWidth) != ty) continue;
+                            const float dd = len2(sub(q, center));
+                            if (!(dd <= 0.0f)) continue;
+                            if (regionG[j] == BP_REGION_REMOVED || !filter_accepts(catG[i], maskG[i], catG[j], maskG[j])) continue;
+                            if (!movable && !(S.invm[j] > 0.0f)) continue;
+                            bestJ = j; bestD = dd;
+                        }
+                    }
+                if (bestJ != 0xffffffffu) {
+                    firstIndex = i; mine.kind = BP_VIOLATION_PAIR; mine.other = (uint64_t)ids[bestJ]; mine.detail = bestD;
+                    continue;
+                }
+                if (!movable) continue;
+                if (assert_body_and_edges(g, center, startG[i], catG[i], maskG[i], fraction, mine)) firstIndex = i;
+            }
+            const uint32_t winner = block_min_u32<BLOCK>(S.red, firstIndex);
+            if (winner == 0xffffffffu) return false;
+            if (firstIndex == winner) {
+                st->violationKind = mine.kind;
+                st->violationSegment = mine.segment;
+                st->violationObjects[0] = (uint64_t)ids[winner];
+                st->violationObjects[1] = mine.other;
+                st->violationDetail = mine.detail;
+            }
+            return true;
+        };
+
+        // ---- 2-3. Grid build and the substep loop (Force:1257-1304).
+        if (n > 0) {
+            bool ok = build();
+            if (!ok) status = BP_ERR_CAPACITY_OVERFLOW;
+            if (status == BP_OK) {
+                pass_forces(2u, true);
+                for (uint32_t iteration = 0; iteration < 2 * kFrameLimit + 2; ++iteration) {
+                    const float remaining = c.t0 - c.t1;
+                    const bool over = !(remaining > 0.0f), stopped = c.rebuild != 0 || c.limit != 0;
+                    const bool limited = c.c1 >= kFrameLimit;
+                    if (over || stopped || limited) {
+                        if (limited && !over && !stopped) c.limit = 1;
+                        c.h = 0.0f;
+                    } else {
+                        const float stiffnessBound = w2Max > 0.0f ? 1.0f / __builtin_sqrtf(w2Max) : inff();
+                        pass_distance(fminf(remaining, stiffnessBound));
+                        reduce_h(c, stiffnessBound, w2Own, dbMin, dbOwn, mbMin, mbOwn, ids, history, lane);
+                    }
+                    const bool finished = !(c.t0 - c.t1 > 0.0f);
+                    if (!(c.h > 0.0f)) {
+                        if (c.limit || finished || !c.rebuild) break;
+                        c.c0 = 0; c.rebuild = 0;
+                        ov2 = max(ov2, ov0); ov0 = 0; ov3 += 1;
+                        __syncthreads();
+                        if (!build()) { status = BP_ERR_CAPACITY_OVERFLOW; break; }
+                        pass_forces(2u, true);
+                        continue;
+                    }
+                    pass_drift();
+                    __syncthreads();
+                    pass_forces(1u, true);
+                    if (d->injectionEnabled && c.c1 == d->injectionAfter + 1) {
+    #pragma unroll
+                        for (int s = 0; s < PER; ++s) {
+                            const uint32_t i = lane + s * BLOCK;
+                            if (valid[s] && ids[i] == d->injectionObject) {
+                                p[s] = f2(d->injectionPosition[0], d->injectionPosition[1]);
+                                S.pos[i] = p[s];
+                            }
+                        }
+                        __syncthreads();
+                        pass_forces(1u, false);
+                    }
+                    if (d->assertEverySubstep && !claimed) {
+                        if (assert_contacts()) claimed = true;
+                        __syncthreads();
+                    }
+                }
+                ov2 = max(ov2, ov0);
+            }
         }
-    }
-    __syncthreads();
-    if (lane < 8) S.red.counters[lane] = 0;
-    __syncthreads();
-    if (safety) atomicOr(&S.red.counters[5], safety);
-    atomicAdd(&S.red.counters[6], rmLocal);
-    atomicAdd(&S.red.counters[7], wrLocal);
-    __syncthreads();
-    status = final_status(status, c, S.red.counters[5], claimed, finished, n);
-    if (lane == 0) {
-        st->status = status;
-        st->substeps = c.c1;
-        st->rebuilds = ov3;
-        st->candidatePairs = pairs;
-        st->candidateOverflow = ov2;
-        st->maxCandidates = ov1;
-        st->removals = S.red.counters[6];
-        st->wraps = S.red.counters[7];
-        st->time[0] = c.t0; st->time[1] = c.t1; st->time[2] = c.h; st->time[3] = c.split;
-        st->limiterObject = c.limObj;
-        st->limiterTerm = c.limTerm;
-        if (!claimed) {
-            st->violationKind = 0; st->violationSegment = 0; st->violationObjects[0] = 0; st->violationObjects[1] = 0;
-            st->violationDetail = 0.0f;
+
+        // ---- 4. Gated end of step: region, finite check, assertion (NativeMetalSolver.swift:837-945).
+        const bool finished = c.limit == 0 && !(c.t0 - c.t1 > 0.0f);
+        uint32_t region[PER];
+    #pragma unroll
+        for (int s = 0; s < PER; ++s) region[s] = BP_REGION_UNCHANGED;
+        if (status == BP_OK && finished) {
+            const float2 lo = f2(d->region[0], d->region[1]), hi = f2(d->region[2], d->region[3]);
+    #pragma unroll
+            for (int s = 0; s < PER; ++s) {
+                if (!valid[s]) continue;
+                const uint32_t i = lane + s * BLOCK;
+                if (d->applyRegion) {
+                    float2 startI = startG[i];
+                    region[s] = apply_region(p[s], startI, S.r[i], fl[s], lo, hi, safety);
+                    if (region[s] == BP_REGION_WRAPPED) { startG[i] = startI; S.pos[i] = p[s]; }
+                }
+                const uint32_t gi = o + i;
+                A->region[ws][gi] = region[s];
+                if (!finite2(p[s]) || !finite2(startG[i]) || !finite2(v[s]) || !isfinite(rot[s]) || !isfinite(om[s]) ||
+                    !isfinite(A->mass[gi]) || !isfinite(A->invm[gi]) || !isfinite(S.r[i]))
+                    safety |= 1u;
+            }
+            for (uint32_t j = lane; j < g.ns; j += BLOCK) {
+                const Seg sg = g.segment(j);
+                const float2 e = sub(sg.b, sg.a);
+                if (!finite2(sg.a) || !finite2(sg.b) || !finite2(e) || !isfinite(dot2(e, e))) safety |= 1u;
+            }
+            __syncthreads();
+            if (!claimed && n > 0 && assert_contacts()) claimed = true;
         }
-        st->adjacencyEntries = entriesTotal;
-        st->groupWidth = BLOCK;
-        st->executed = 1;
-        if (status == BP_OK) d->pub = ws;
-        else if (status != BP_ERR_CAPACITY_OVERFLOW) d->failed = 1;
+
+        // ---- Write-back of the working side, counters and publication.
+        uint32_t rmLocal = 0, wrLocal = 0;
+    #pragma unroll
+        for (int s = 0; s < PER; ++s) {
+            if (!valid[s]) continue;
+            const uint32_t gi = o + lane + s * BLOCK;
+            A->pos[ws][gi] = p[s];
+            A->vel[ws][gi] = v[s];
+            A->rot[ws][gi] = rot[s];
+            A->omega[ws][gi] = om[s];
+            if (!(status == BP_OK && finished)) A->region[ws][gi] = BP_REGION_UNCHANGED;
+            rmLocal += region[s] == BP_REGION_REMOVED;
+            wrLocal += region[s] == BP_REGION_WRAPPED;
+            if (d->exportAdjacency) {
+                A->rowStart[gi] = rs[s];
+                A->rowLen[gi] = rl[s];
+                if (rows.inLds)
+                    for (uint32_t e = 0; e < rl[s]; ++e) {
+                        uint32_t kind, j;
+                        rows.get(rs[s] + e, kind, j);
+                        A->adj[d->adjOffset + rs[s] + e] = (kind << kKindShift) | j;
+                    }
+            }
+        }
+        __syncthreads();
+        if (lane < 8) S.red.counters[lane] = 0;
+        __syncthreads();
+        if (safety) atomicOr(&S.red.counters[5], safety);
+        atomicAdd(&S.red.counters[6], rmLocal);
+        atomicAdd(&S.red.counters[7], wrLocal);
+        __syncthreads();
+        status = final_status(status, c, S.red.counters[5], claimed, finished, n);
+        if (lane == 0) {
+            st->status = status;
+            st->substeps = c.c1;
+            st->rebuilds = ov3;
+            st->candidatePairs = pairs;
+            st->candidateOverflow = ov2;
+            st->maxCandidates = ov1;
+            st->removals = S.red.counters[6];
+            st->wraps = S.red.counters[7];
+            st->time[0] = c.t0; st->time[1] = c.t1; st->time[2] = c.h; st->time[3] = c.split;
+            st->limiterObject = c.limObj;
+            st->limiterTerm = c.limTerm;
+            if (!claimed) {
+                st->violationKind = 0; st->violationSegment = 0; st->violationObjects[0] = 0; st->violationObjects[1] = 0;
+                st->violationDetail = 0.0f;
+            }
+            st->adjacencyEntries = entriesTotal;
+            st->groupWidth = BLOCK;
+            st->executed = 1;
+            if (status == BP_OK) d->pub = ws;
+            else if (status != BP_ERR_CAPACITY_OVERFLOW) d->failed = 1;
+        }
+        if (status != BP_OK) break;                           // a failed world stops; later steps stay unexecuted
+        pubNow = ws;
+        __syncthreads();                                      // LDS is reused by the next step
     }
 #undef ids
 #undef catG

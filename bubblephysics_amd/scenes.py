@@ -293,7 +293,8 @@ def _random_bodies(rng, builder, count_poly, count_circle, count_capsule, region
         while np.diff(np.concatenate([angles, angles[:1] + 2 * math.pi])).max() >= math.pi:
             angles = np.sort(rng.uniform(0.0, 2.0 * math.pi, k))
         pts = np.column_stack([rad * np.cos(angles), rad * np.sin(angles)]).astype(np.float32)
-        shapes.append(("polygon", rad, pts))
+        # The body sits on its centroid, so its reach is the farthest vertex from the centroid.
+        shapes.append(("polygon", polygon_geometry(pts)[3], pts))
     for _ in range(count_circle):
         shapes.append(("circle", float(rng.uniform(8.0, 30.0)), None))
     for _ in range(count_capsule):
@@ -337,18 +338,20 @@ def config2(steps: int = 300, seed_offset: int = 2) -> Scene:
                      segments=segments, steps=steps, placed=int(keep.sum()))
 
 
-def config3(steps: int = 100, seed_offset: int = 3, n: int = 262144, bodies: int = 4096, side: float = 16384.0) -> Scene:
-    """Large world: 262,144 bubbles (log-uniform radii 2..24) and 4,096 kinematic bodies in 16384^2."""
+def config3(steps: int = 100, seed_offset: int = 3, n: int = 262144, bodies: int = 4096, side: float = 16384.0,
+            speed: float = 50.0) -> Scene:
+    """Large world: 262,144 bubbles (log-uniform radii 2..24) and 4,096 kinematic bodies in 16384^2,
+    bubble and body velocities N(0, speed)."""
     rng = np.random.Generator(np.random.PCG64(BASE_SEED + seed_offset))
     region = (0.0, 0.0, side, side)
     builder = BodyBuilder()
     segments = boundary_walls(side, side, builder, first_body_id=10_000_001)
     third = bodies // 3
-    obstacles = _random_bodies(rng, builder, bodies - 2 * third, third, third, region, first_id=1_000_001)
+    obstacles = _random_bodies(rng, builder, bodies - 2 * third, third, third, region, first_id=1_000_001, speed=speed)
     radii = np.exp(rng.uniform(math.log(2.0), math.log(24.0), n)).astype(np.float32)
     centres, placed = place(radii, region, rng, attempts=200, obstacles=obstacles)
     keep = ~np.isnan(centres[:, 0])
-    velocities = rng.normal(0.0, 50.0, (n, 2)).astype(np.float32)
+    velocities = rng.normal(0.0, speed, (n, 2)).astype(np.float32)
     cfg = make_config(region)
     return _assemble("config3-large", cfg, centres[keep], radii[keep], velocities[keep], builder=builder,
                      segments=segments, steps=steps, placed=int(keep.sum()))
