@@ -2,12 +2,13 @@
 
 Workload (default "batched"): 4,096 independent worlds with bubble counts log-uniform in
 [16, 4096] (about 3.0M bubbles), radii U[6, 20], 0-16 kinematic bodies each, walls, advanced
-together: one step advances every world by one frame dt = 1/60. The plain single-world step
-(1,024 bubbles + 64 kinematic bodies) is timed beside it, and the CPU oracle on the host cores
-gives the reported baseline.
+together: the timed call advances every world by `--steps` frames of dt = 1/60, one launch per
+kernel class, each workgroup carrying its world through all the frames. Beside it: the plain
+single-world step (1,024 bubbles + 64 kinematic bodies, one launch per frame), the 262,144-bubble
+world of config 3 on the cooperative grid, and the CPU oracle on the host cores.
 
-One JSON line on stdout. value = bubble-substeps/s = sum over worlds of (bubbles x substeps) / wall
-time of the timed steps, with every world resident on the GPU before timing starts.
+One JSON line on stdout. value = bubble-substeps/s = sum over worlds and frames of
+(bubbles x substeps) / wall time of the timed call, with every world resident on the GPU first.
 """
 from __future__ import annotations
 
@@ -54,40 +55,61 @@ def build_batched(worlds: int):
 
 
 def run_batched(args, scenes_list):
+    """Throughput: one call advances every world `steps` steps (one launch per kernel class, each
+    workgroup carrying its world through all the steps). Latency: `latency_steps` calls of one step."""
     from bubblephysics_amd import solver
     batch = solver.Batch(scenes_list)
     sizes = np.array([len(s.bubbles) for s in scenes_list], np.int64)
-    for _ in range(args.warmup):
-        batch.step(1)
-    latencies, substep_bubbles = [], 0
+    if args.warmup:
+        batch.step(args.warmup)
     classes = int(batch._lib.bp_batch_class_count())
-    class_ms = np.zeros(classes)
-    class_bytes = np.zeros(classes)
-    statuses = []
-    overflow = 0
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        s0 = time.perf_counter()
-        infos = batch.step(1)[0]
-        latencies.append((time.perf_counter() - s0) * 1e3)
-        ok = infos["status"] == 0
-        statuses.append(int((~ok).sum()))
-        overflow = max(overflow, int(infos["candidate_overflow"].max()))
-        substeps = infos["substep_count"].astype(np.int64)
-        substep_bubbles += int(np.sum(sizes * substeps))
-        ms, _, threads, capacity = batch.last_class_timing()
-        class_ms += ms
-        deg = infos["adjacency_entries"].astype(np.float64) / np.maximum(sizes, 1)
-        per = sizes * substeps * bytes_per_bubble_substep(deg)
-        for k in range(classes):
-            class_bytes[k] += float(per[infos["kernel_class"] == k].sum())
+    infos = batch.step(args.steps)
     elapsed = time.perf_counter() - t0
+    ms, _, threads, capacity = batch.last_class_timing()
+    executed = infos["status"] != abi_unavailable()
+    substeps = np.where(executed, infos["substep_count"], 0).astype(np.int64)
+    substep_bubbles = int((substeps * sizes[None, :]).sum())
+    deg = infos["adjacency_entries"].astype(np.float64) / np.maximum(sizes[None, :], 1)
+    per = substeps * sizes[None, :] * bytes_per_bubble_substep(deg)
+    class_bytes = np.zeros(classes)
+    for k in range(classes):
+        class_bytes[k] = float(per[:, infos[0]["kernel_class"] == k].sum())
+    failed = int((infos[-1]["status"] != 0).sum())
+    latencies = []
+    for _ in range(args.latency_steps):
+        s0 = time.perf_counter()
+        batch.step(1)
+        latencies.append((time.perf_counter() - s0) * 1e3)
     return {
-        "class_threads": threads, "class_capacity": capacity,
-        "elapsed": elapsed, "substep_bubbles": substep_bubbles, "latencies": latencies, "class_ms": class_ms / args.steps,
-        "class_bytes": class_bytes / args.steps, "failed_worlds": max(statuses) if statuses else 0,
-        "bubbles": int(sizes.sum()), "worlds": len(sizes), "overflow": overflow,
+        "class_threads": threads, "class_capacity": capacity, "elapsed": elapsed, "substep_bubbles": substep_bubbles,
+        "latencies": latencies, "class_ms": ms, "class_bytes": class_bytes, "failed_worlds": failed,
+        "bubbles": int(sizes.sum()), "worlds": len(sizes), "overflow": int(infos["candidate_overflow"].max()),
     }
+
+
+def abi_unavailable():
+    from bubblephysics_amd import abi
+    return abi.BP_ERR_UNAVAILABLE
+
+
+def run_large(steps: int):
+    """The 262,144-bubble world of config 3 on the cooperative grid."""
+    from bubblephysics_amd import scenes, solver
+    scene = scenes.config3()
+    world = solver.NativeWorld(scene)
+    n = len(scene.bubbles)
+    world.step(0)
+    total, statuses, gpu_ms = 0, [], 0.0
+    t0 = time.perf_counter()
+    for k in range(1, steps + 1):
+        f = world.step(k)
+        statuses.append(f.status)
+        total += n * f.substep_count
+        gpu_ms += f.gpu_milliseconds
+    elapsed = time.perf_counter() - t0
+    return {"value": total / elapsed, "ms_per_step": elapsed / steps * 1e3, "gpu_ms_per_step": gpu_ms / steps,
+            "bubbles": n, "bodies": len(scene.bodies), "statuses": sorted(set(statuses))}
 
 
 def run_plain(steps: int):
@@ -131,6 +153,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--worlds", type=int, default=4096)
     ap.add_argument("--plain-steps", type=int, default=60)
+    ap.add_argument("--latency-steps", type=int, default=10)
+    ap.add_argument("--large-steps", type=int, default=5)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -146,11 +170,14 @@ def main():
     print(f"[bench] batched done: {res['elapsed']:.2f}s for {args.steps} steps", file=sys.stderr, flush=True)
     plain = run_plain(args.plain_steps)
     print(f"[bench] plain done: {plain['value']:.3g} bubble-substeps/s", file=sys.stderr, flush=True)
+    large = run_large(args.large_steps) if args.large_steps > 0 else None
+    if large:
+        print(f"[bench] large done: {large['value']:.3g} bubble-substeps/s", file=sys.stderr, flush=True)
     cpu = None if args.no_cpu else cpu_baseline(scenes_list, args.cpu_seconds)
 
     value = res["substep_bubbles"] / res["elapsed"]
     k = int(np.argmax(res["class_ms"]))
-    dur_s = res["class_ms"][k] / 1e3
+    dur_s = res["class_ms"][k] / 1e3                 # one launch of the class covers all the timed steps
     achieved = res["class_bytes"][k] / dur_s / 1e9 if dur_s > 0 else 0.0
     line = {
         "metric": "bubble-substeps/sec on 1 MI355X (batched/served vs plain) + p50/p99 step latency",
@@ -170,13 +197,17 @@ def main():
                         f"radii U[6,20], 0-16 kinematic bodies per world, walls, dt 1/60",
             "worlds": res["worlds"], "bubbles": res["bubbles"], "parallelism": "one workgroup per world",
         },
-        "p50_step_ms": percentile(res["latencies"], 50),
+        "p50_step_ms": percentile(res["latencies"], 50),     # one step of every world, host clock
         "p99_step_ms": percentile(res["latencies"], 99),
         "failed_worlds": res["failed_worlds"],
         "candidate_overflow_max": res["overflow"],
         "plain": {"workload": "plain: 1 world, 1024 bubbles + 64 kinematic bodies", "value": plain["value"],
                   "p50_step_ms": plain["p50_ms"], "p99_step_ms": plain["p99_ms"]},
         "batched_over_plain": value / plain["value"] if plain["value"] > 0 else None,
+        "large": None if large is None else {
+            "workload": f"config 3: 1 world, {large['bubbles']} bubbles + {large['bodies']} bodies (kinematic and walls), "
+                        "cooperative grid", "value": large["value"], "ms_per_step": large["ms_per_step"],
+            "gpu_ms_per_step": large["gpu_ms_per_step"], "statuses": large["statuses"]},
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBPS, "traffic": None,

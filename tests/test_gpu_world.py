@@ -252,3 +252,24 @@ def test_reference_kernel_scenes_match_oracle_bitwise(oracle, name):
         assert np.array_equal(bits(h), bits(ex["history"][: c["substeps"]])), k
         assert_same_state(frame.bubbles, cb, f"{name} step {k}")
         cur.bubbles = cb
+
+
+def test_config3_large_world_matches_the_oracle_fixture():
+    # The 262,144-bubble world of config 3 (cooperative grid, body LBVH): counters and the SHA-256
+    # of the state bits after each step equal the oracle's (tests/golden/config3_oracle.json, written
+    # by tests/golden/make_config3_fixture.py; the oracle takes about a minute per step here).
+    import hashlib
+    import json
+    from pathlib import Path
+    golden = json.loads((Path(__file__).resolve().parent / "golden" / "config3_oracle.json").read_text())
+    scene = scenes.config3()
+    world = solver.NativeWorld(scene)
+    for k, want in enumerate(golden["steps"]):
+        f = world.step(k)
+        got = (f.status, f.substep_count, f.grid_rebuild_count, f.candidate_pair_count, f.maximum_contact_candidates_per_object)
+        assert got == (want["status"], want["substeps"], want["rebuilds"], want["candidate_pairs"], want["max_candidates"]), k
+        h = hashlib.sha256()
+        for field in STATE_FIELDS:
+            h.update(np.ascontiguousarray(f.bubbles[field]).tobytes())
+        assert h.hexdigest() == want["state_sha256"], k
+        assert f.execution_path == abi.BP_PATH_COOPERATIVE
