@@ -177,7 +177,7 @@ def main():
 
     value = res["substep_bubbles"] / res["elapsed"]
     k = int(np.argmax(res["class_ms"]))
-    dur_s = res["class_ms"][k] / 1e3                 # one launch of the class covers all the timed steps
+    dur_s = float(res["class_ms"][k]) / 1e3          # one launch of the class covers all the timed steps
     achieved = res["class_bytes"][k] / dur_s / 1e9 if dur_s > 0 else 0.0
     line = {
         "metric": "bubble-substeps/sec on 1 MI355X (batched/served vs plain) + p50/p99 step latency",
@@ -212,7 +212,7 @@ def main():
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
             "kernel": kernel_name(k, res),
-            "kernel_ms_per_launch": res["class_ms"][k],
+            "kernel_ms_per_launch": float(res["class_ms"][k]),
         },
         "cpu_baseline": None if cpu is None else {
             "value": cpu["value"], "unit": "bubble-substeps/s", "cores": 1, "kind": "port",
@@ -220,7 +220,7 @@ def main():
                       f"({cpu['seconds']:.1f}s)",
         },
     }
-    print(json.dumps(line))
+    print(json.dumps(line, default=float))
 
 
 if __name__ == "__main__":

@@ -190,6 +190,12 @@ SIGNATURES = {
     "bp_batch_last_gpu_milliseconds": ([_P], _F),
     "bp_batch_last_class_timing": ([_P, _P, _P, _P, _P], _I32),
     "bp_batch_class_count": ([], _U32),
+    "bp_batch_step_worlds": ([_P, _P, _U32, _U32, _P], _I32),
+    "bp_batch_submit_worlds": ([_P, _P, _U32, _U32], _I32),
+    "bp_batch_poll": ([_P, _P], _I32),
+    "bp_batch_write_bubbles": ([_P, _U32, _P, _U32], _I32),
+    "bp_batch_set_resident": ([_P, _U32, _U32], _I32),
+    "bp_batch_world_footprint": ([_P, _U32, _P], _I32),
     "bp_batch_set_world_class": ([_P, _U32, _I32], _I32),
 }
 

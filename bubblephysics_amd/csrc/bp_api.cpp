@@ -100,6 +100,9 @@ struct HostWorld {
     uint32_t adjPerBubble = kAdjPerBubble;
     int forcedClass = -1;               // kernel configuration chosen by the tuner, -1 = default
     uint32_t exportAdjacency = 0;
+    bool resident = true;               // false: held in host memory only (serving eviction)
+    uint32_t capacity = 0;              // bubble slots of its device range (>= bubbles), set by pack
+    int packedClass = -1;               // kernel configuration of the last pack (-1 = not on the device)
 };
 
 bool class_fits(int k, const HostWorld &w) {
@@ -152,6 +155,12 @@ struct bp_batch {
     float lastGpuMs = 0.0f;
     float lastClassMs[kClassCount] = {};
     bool inFlight = false;
+    bool pending = false;              // launches queued by launch() and not yet collected
+    uint32_t pendingSteps = 0;
+    std::vector<uint32_t> pendingList; // worlds of the pending launches, in the caller's order
+    std::vector<uint32_t> subsetHost;  // per class, the listed worlds (longest first)
+    uint32_t *subsetOrders = nullptr;  // device copy of subsetHost
+    uint32_t launchOffset[kClassCount] = {}, launchCount[kClassCount] = {};
     uint32_t totalBubbles = 0;
     std::mutex mutex;
 
@@ -172,7 +181,8 @@ struct bp_batch {
         }
         if (startEvent) (void)hipEventDestroy(startEvent);
         startEvent = nullptr;
-        stats = nullptr; statsCapacity = 0; descs = nullptr; arenaDev = nullptr;
+        stats = nullptr; statsCapacity = 0; descs = nullptr; arenaDev = nullptr; subsetOrders = nullptr;
+        pending = false;
         for (auto &o : orders) o = nullptr;
         packed = false;
     }
@@ -194,6 +204,11 @@ struct bp_batch {
 
     bp_status pack();
     bp_status download_state();
+    bp_status download_world(uint32_t wi);
+    bp_status write_bubbles(uint32_t wi, const bp_bubble *bubbles, uint32_t count);
+    uint64_t footprint(uint32_t wi) const;
+    bp_status launch(const uint32_t *subset, uint32_t subsetCount, uint32_t steps);
+    bp_status finish(bp_frame_info *infos, bool wait);
     bp_status step(uint32_t steps, bp_frame_info *infos);
 };
 
@@ -261,18 +276,30 @@ bp_status bp_batch::pack() {
         if (s != BP_OK) return s;
         int cls = default_class_for(w, coopMax > 0);
         if (w.forcedClass >= 0 && class_fits(w.forcedClass, w) && (w.forcedClass != kCoopClass || coopMax > 0)) cls = w.forcedClass;
+        if (!w.resident) cls = -1;
         worldClass[wi] = cls;
+        // Slots for the world's bubbles: some room to grow in place (serving transactions), within
+        // the bubble capacity of an LDS configuration.
+        const uint32_t n0 = (uint32_t)w.bubbles.size();
+        uint32_t slots = 0;
+        if (w.resident) {
+            slots = n0 + std::max<uint32_t>(8, n0 / 8);
+            if (cls >= 0 && cls < kGlobalClass) slots = std::min<uint32_t>(slots, class_capacity(cls));
+            slots = std::max(slots, n0);
+        }
+        worlds[wi].capacity = slots;
+        worlds[wi].packedClass = cls;
         if (cls == kCoopClass) {
             coopCells += next_pow2((uint32_t)w.bubbles.size(), 1024);
             ++coopWorlds;
             const uint32_t block = bp_coop_block();
             coopGroups[wi] = std::max<uint32_t>(1, std::min<uint32_t>(coopMax, ((uint32_t)w.bubbles.size() + block - 1) / block));
         }
-        nb += (uint32_t)w.bubbles.size(); ns += (uint32_t)w.segments.size(); nbod += (uint32_t)w.bodies.size();
+        nb += slots; ns += (uint32_t)w.segments.size(); nbod += (uint32_t)w.bodies.size();
         nv += (uint32_t)(w.vertices.size() / 2); nt += (uint32_t)(w.triangles.size() / 4);
         nm += (uint32_t)w.motors.size(); nband += (uint32_t)w.bands.size();
         nf += (uint32_t)w.fields.size(); nfv += (uint32_t)(w.fieldVertices.size() / 2); nfi += (uint32_t)w.fieldIds.size();
-        adj += (uint64_t)std::max<size_t>(w.bubbles.size(), 16) * w.adjPerBubble;
+        adj += w.resident ? (uint64_t)std::max<uint32_t>(slots, 16) * w.adjPerBubble : 0;
     }
     if (adj > 0xffffffffull) { set_error("adjacency capacity exceeds 2^32 entries"); return BP_ERR_CAPACITY_OVERFLOW; }
     totalBubbles = nb;
@@ -304,6 +331,7 @@ bp_status bp_batch::pack() {
     BP_ALLOC(descs, worlds.size());
     BP_ALLOC(arenaDev, 1);
     for (int k = 0; k < kClassCount; ++k) BP_ALLOC(orders[k], worlds.size());
+    BP_ALLOC(subsetOrders, worlds.size());
 #undef BP_ALLOC
 
     // Host staging of every array.
@@ -328,7 +356,7 @@ bp_status bp_batch::pack() {
         const HostWorld &w = worlds[wi];
         bpd::WorldDesc &d = hostDescs[wi];
         uint32_t n = (uint32_t)w.bubbles.size();
-        d.bubbleOffset = ob; d.bubbleCount = n;
+        d.bubbleOffset = ob; d.bubbleCount = w.resident ? n : 0;
         d.segmentOffset = os; d.segmentCount = (uint32_t)w.segments.size();
         d.bodyOffset = obd; d.bodyCount = (uint32_t)w.bodies.size();
         d.vertexOffset = ov; d.triangleOffset = ot;
@@ -336,7 +364,7 @@ bp_status bp_batch::pack() {
         d.motorOffset = om_; d.motorCount = (uint32_t)w.motors.size();
         d.bandOffset = oband; d.bandCount = (uint32_t)w.bands.size();
         d.fieldOffset = of; d.fieldCount = (uint32_t)w.fields.size(); d.fieldVertexOffset = ofv; d.fieldIdOffset = ofi;
-        d.adjOffset = (uint32_t)oadj; d.adjCapacity = std::max<uint32_t>(n, 16) * w.adjPerBubble;
+        d.adjOffset = (uint32_t)oadj; d.adjCapacity = w.resident ? std::max<uint32_t>(w.capacity, 16) * w.adjPerBubble : 0;
         d.historyOffset = (uint32_t)(wi * bpd::kFrameLimit);
         d.pub = 0;
         const bp_world_config &c = w.config;
@@ -354,7 +382,8 @@ bp_status bp_batch::pack() {
         d.injectionPosition[0] = c.injection_position[0]; d.injectionPosition[1] = c.injection_position[1];
         d.failed = 0;
         d.exportAdjacency = w.exportAdjacency;
-        for (uint32_t i = 0; i < n; ++i) {
+        const uint32_t resident = w.resident ? n : 0u;       // bubbles staged in the device range
+        for (uint32_t i = 0; i < resident; ++i) {
             const bp_bubble &b = w.bubbles[i];
             uint32_t g = ob + i;
             pos[g] = make_float2(b.center[0], b.center[1]);
@@ -371,7 +400,7 @@ bp_status bp_batch::pack() {
             motors[om_ + k] = make_float4(ms[k].force[0], ms[k].force[1], ms[k].torque, ms[k].inertia);
             auto it = std::lower_bound(w.bubbles.begin(), w.bubbles.end(), ms[k].bubble_id,
                                        [](const bp_bubble &b, int64_t id) { return b.id < id; });
-            if (it != w.bubbles.end() && it->id == ms[k].bubble_id) motorIdx[ob + (it - w.bubbles.begin())] = (int32_t)k;
+            if (w.resident && it != w.bubbles.end() && it->id == ms[k].bubble_id) motorIdx[ob + (it - w.bubbles.begin())] = (int32_t)k;
         }
         std::vector<bp_band> bs = w.bands;
         std::sort(bs.begin(), bs.end(), [](const bp_band &a, const bp_band &b) { return a.bubble_id < b.bubble_id; });
@@ -379,7 +408,7 @@ bp_status bp_batch::pack() {
             bands[oband + k] = make_float4(bs[k].target[0], bs[k].target[1], bs[k].stiffness, bs[k].maximum_force);
             auto it = std::lower_bound(w.bubbles.begin(), w.bubbles.end(), bs[k].bubble_id,
                                        [](const bp_bubble &b, int64_t id) { return b.id < id; });
-            if (it != w.bubbles.end() && it->id == bs[k].bubble_id) bandIdx[ob + (it - w.bubbles.begin())] = (int32_t)k;
+            if (w.resident && it != w.bubbles.end() && it->id == bs[k].bubble_id) bandIdx[ob + (it - w.bubbles.begin())] = (int32_t)k;
         }
         // Fields act in ascending id order (NativeForceInput sorts its objects, NativeForceRecords.swift:187).
         std::vector<bp_field> fs = w.fields;
@@ -417,13 +446,13 @@ bp_status bp_batch::pack() {
         for (size_t t = 0; t < w.triangles.size() / 4; ++t)
             tris[ot + t] = make_uint4(w.triangles[4 * t], w.triangles[4 * t + 1], w.triangles[4 * t + 2], w.triangles[4 * t + 3]);
         const int cls = worldClass[wi];
-        classWorlds[cls].push_back((uint32_t)wi);
+        if (cls >= 0) classWorlds[cls].push_back((uint32_t)wi);
         if (cls == kCoopClass) {
             d.coopBuckets = next_pow2(n, 1024);
             d.coopCellOffset = occ; occ += d.coopBuckets;
             d.coopScratchOffset = ocs; ocs += bpd::kCoopScratchWords;
         }
-        ob += n; os += (uint32_t)w.segments.size(); obd += (uint32_t)w.bodies.size();
+        ob += w.capacity; os += (uint32_t)w.segments.size(); obd += (uint32_t)w.bodies.size();
         ov += (uint32_t)(w.vertices.size() / 2); ot += (uint32_t)(w.triangles.size() / 4);
         om_ += (uint32_t)w.motors.size(); oband += (uint32_t)w.bands.size();
         of += (uint32_t)w.fields.size(); ofv += (uint32_t)(w.fieldVertices.size() / 2); ofi += (uint32_t)w.fieldIds.size();
@@ -493,6 +522,7 @@ bp_status bp_batch::download_state() {
     if (nbod) BP_HIP(hipMemcpy(bodyMotion.data(), arena.bodyMotion, nbod * sizeof(float4), hipMemcpyDeviceToHost));
     for (size_t wi = 0; wi < worlds.size(); ++wi) {
         HostWorld &w = worlds[wi];
+        if (!w.resident) continue;                         // its host copy is the state
         const bpd::WorldDesc &d = hostDescs[wi];
         uint32_t p = d.pub;
         for (uint32_t i = 0; i < w.bubbles.size(); ++i) {
@@ -543,12 +573,163 @@ static void fill_info(const bpd::WorldStatus &s, bp_frame_info *out) {
     }
 }
 
-bp_status bp_batch::step(uint32_t steps, bp_frame_info *infos) {
+// Copies the published state of one resident world back into its host mirror.
+bp_status bp_batch::download_world(uint32_t wi) {
+    HostWorld &w = worlds[wi];
+    if (!packed || !deviceValid || !w.resident) return BP_OK;
+    bpd::WorldDesc &d = hostDescs[wi];
+    BP_HIP(hipMemcpy(&d, descs + wi, sizeof d, hipMemcpyDeviceToHost));
+    const uint32_t n = (uint32_t)w.bubbles.size(), p = d.pub, o = d.bubbleOffset;
+    std::vector<float2> pos(n), start(n), vel(n);
+    std::vector<float> rot(n), om(n);
+    if (n) {
+        BP_HIP(hipMemcpy(pos.data(), arena.pos[p] + o, n * sizeof(float2), hipMemcpyDeviceToHost));
+        BP_HIP(hipMemcpy(start.data(), arena.start[p] + o, n * sizeof(float2), hipMemcpyDeviceToHost));
+        BP_HIP(hipMemcpy(vel.data(), arena.vel[p] + o, n * sizeof(float2), hipMemcpyDeviceToHost));
+        BP_HIP(hipMemcpy(rot.data(), arena.rot[p] + o, n * sizeof(float), hipMemcpyDeviceToHost));
+        BP_HIP(hipMemcpy(om.data(), arena.omega[p] + o, n * sizeof(float), hipMemcpyDeviceToHost));
+    }
+    for (uint32_t i = 0; i < n; ++i) {
+        bp_bubble &b = w.bubbles[i];
+        b.center[0] = pos[i].x; b.center[1] = pos[i].y;
+        b.previous_center[0] = start[i].x; b.previous_center[1] = start[i].y;
+        b.velocity[0] = vel[i].x; b.velocity[1] = vel[i].y;
+        b.rotation = rot[i]; b.angular_velocity = om[i];
+    }
+    const uint32_t nb = (uint32_t)w.bodies.size();
+    if (nb) {
+        std::vector<float4> pose(nb), motion(nb);
+        std::vector<float2> prev(nb);
+        BP_HIP(hipMemcpy(pose.data(), arena.bodyPose[p] + d.bodyOffset, nb * sizeof(float4), hipMemcpyDeviceToHost));
+        BP_HIP(hipMemcpy(prev.data(), arena.bodyPrev[p] + d.bodyOffset, nb * sizeof(float2), hipMemcpyDeviceToHost));
+        BP_HIP(hipMemcpy(motion.data(), arena.bodyMotion + d.bodyOffset, nb * sizeof(float4), hipMemcpyDeviceToHost));
+        for (uint32_t k = 0; k < nb; ++k) {
+            bp_body &b = w.bodies[k];
+            b.pose[0] = pose[k].x; b.pose[1] = pose[k].y; b.pose[2] = pose[k].z;
+            b.previous[0] = prev[k].x; b.previous[1] = prev[k].y; b.previous[2] = pose[k].w;
+            b.velocity[0] = motion[k].x; b.velocity[1] = motion[k].y; b.velocity[2] = motion[k].z;
+        }
+    }
+    return BP_OK;
+}
+
+// Replaces the bubbles of a world (a serving transaction). Within the world's slots and kernel
+// configuration the device range is rewritten in place; otherwise the batch is repacked at the
+// next step.
+bp_status bp_batch::write_bubbles(uint32_t wi, const bp_bubble *bubbles, uint32_t count) {
+    HostWorld &w = worlds[wi];
+    for (uint32_t i = 1; i < count; ++i)
+        if (!(bubbles[i - 1].id < bubbles[i].id)) { set_error("bubbles must be sorted by strictly increasing id"); return BP_ERR_INVALID_SNAPSHOT; }
+    const bool inPlace = packed && w.resident && w.packedClass >= 0 && count <= w.capacity &&
+                         class_fits(w.packedClass, count, w.segments.size(), w.bodies.size(), w.vertices.size() / 2,
+                                    w.triangles.size() / 4);
+    if (!inPlace) {
+        bp_status s = download_state();
+        if (s != BP_OK) return s;
+        w.bubbles.assign(bubbles, bubbles + count);
+        packed = false;
+        return BP_OK;
+    }
+    bpd::WorldDesc &d = hostDescs[wi];
+    BP_HIP(hipMemcpy(&d, descs + wi, sizeof d, hipMemcpyDeviceToHost));
+    std::vector<float2> pos(count), start(count), vel(count);
+    std::vector<float> rot(count), om(count), mass(count), invm(count), radius(count), Kc(count), gamma(count);
+    std::vector<uint32_t> flags(count), cat(count), mask(count), region(count, 0);
+    std::vector<int32_t> motorIdx(count, -1), bandIdx(count, -1);
+    std::vector<int64_t> ids(count);
+    for (uint32_t i = 0; i < count; ++i) {
+        const bp_bubble &b = bubbles[i];
+        pos[i] = make_float2(b.center[0], b.center[1]);
+        start[i] = make_float2(b.previous_center[0], b.previous_center[1]);
+        vel[i] = make_float2(b.velocity[0], b.velocity[1]);
+        rot[i] = b.rotation; om[i] = b.angular_velocity;
+        mass[i] = b.mass; invm[i] = b.inverse_mass; radius[i] = b.radius; Kc[i] = b.contact_stiffness;
+        gamma[i] = b.damping; flags[i] = b.flags; cat[i] = b.category; mask[i] = b.mask; ids[i] = b.id;
+    }
+    // Motor and band records keep their ids; their bubble indices are resolved again (the
+    // device holds the records sorted by bubble id, as pack() wrote them).
+    std::vector<bp_motor> ms = w.motors;
+    std::sort(ms.begin(), ms.end(), [](const bp_motor &a, const bp_motor &b) { return a.bubble_id < b.bubble_id; });
+    std::vector<bp_band> bs = w.bands;
+    std::sort(bs.begin(), bs.end(), [](const bp_band &a, const bp_band &b) { return a.bubble_id < b.bubble_id; });
+    for (uint32_t i = 0; i < count; ++i) {
+        auto m = std::lower_bound(ms.begin(), ms.end(), ids[i], [](const bp_motor &r, int64_t id) { return r.bubble_id < id; });
+        if (m != ms.end() && m->bubble_id == ids[i]) motorIdx[i] = (int32_t)(m - ms.begin());
+        auto b = std::lower_bound(bs.begin(), bs.end(), ids[i], [](const bp_band &r, int64_t id) { return r.bubble_id < id; });
+        if (b != bs.end() && b->bubble_id == ids[i]) bandIdx[i] = (int32_t)(b - bs.begin());
+    }
+    const uint32_t o = d.bubbleOffset;
+    auto put = [&](auto *dst, const auto &src) -> bp_status {
+        if (!src.empty()) BP_HIP(hipMemcpy(dst + o, src.data(), src.size() * sizeof(src[0]), hipMemcpyHostToDevice));
+        return BP_OK;
+    };
+    bp_status s = BP_OK;
+    for (int side = 0; side < 2; ++side)
+        if ((s = put(arena.pos[side], pos)) || (s = put(arena.start[side], start)) || (s = put(arena.vel[side], vel)) ||
+            (s = put(arena.rot[side], rot)) || (s = put(arena.omega[side], om)) || (s = put(arena.region[side], region)))
+            return s;
+    if ((s = put(arena.mass, mass)) || (s = put(arena.invm, invm)) || (s = put(arena.radius, radius)) || (s = put(arena.Kc, Kc)) ||
+        (s = put(arena.gamma, gamma)) || (s = put(arena.flags, flags)) || (s = put(arena.cat, cat)) || (s = put(arena.mask, mask)) ||
+        (s = put(arena.id, ids)) || (s = put(arena.motor, motorIdx)) || (s = put(arena.band, bandIdx)))
+        return s;
+    d.bubbleCount = count;
+    d.buckets = w.config.grid_bucket_count == 0 ? auto_buckets(count) : std::min(w.config.grid_bucket_count, bpd::kMaxBuckets);
+    d.failed = 0;
+    BP_HIP(hipMemcpy(descs + wi, &d, sizeof d, hipMemcpyHostToDevice));
+    w.bubbles.assign(bubbles, bubbles + count);
+    return BP_OK;
+}
+
+// Device bytes held by a world: its bubble slots in every per-bubble array, its adjacency
+// capacity and its geometry.
+uint64_t bp_batch::footprint(uint32_t wi) const {
+    const HostWorld &w = worlds[wi];
+    const uint32_t n = (uint32_t)w.bubbles.size();
+    const uint64_t slots = (packed && w.resident) ? w.capacity : n + std::max<uint32_t>(8, n / 8);   // as pack() reserves
+    const uint64_t perSlot = 2 * (3 * sizeof(float2) + 2 * sizeof(float) + sizeof(uint32_t))   // two-sided state
+                             + sizeof(int64_t) + 6 * sizeof(float) + 3 * sizeof(uint32_t) + 2 * sizeof(int32_t)
+                             + 3 * sizeof(float2) + sizeof(float) + 3 * sizeof(uint32_t);     // properties, scratch
+    const uint64_t adjacency = std::max<uint64_t>(slots, 16) * w.adjPerBubble * sizeof(uint32_t);
+    const uint64_t geometry = w.segments.size() * 64 + w.bodies.size() * 160 + w.vertices.size() * sizeof(float) +
+                              w.triangles.size() * sizeof(uint32_t) + (w.motors.size() + w.bands.size()) * sizeof(float4);
+    return slots * perSlot + adjacency + geometry + bpd::kFrameLimit * sizeof(float) + sizeof(bpd::WorldDesc);
+}
+
+// Queues `steps` steps of the listed worlds (every resident world when `subset` is null) on the
+// batch's streams and returns without waiting; finish() collects them.
+bp_status bp_batch::launch(const uint32_t *subset, uint32_t subsetCount, uint32_t steps) {
     if (!packed) {
         bp_status s = pack();
         if (s != BP_OK) return s;
     }
-    size_t W = worlds.size();
+    const size_t W = worlds.size();
+    pendingSteps = steps;
+    pendingList.clear();
+    if (subset) {
+        std::vector<uint8_t> member(W, 0);
+        for (uint32_t j = 0; j < subsetCount; ++j) {
+            const uint32_t wi = subset[j];
+            if (wi >= W || member[wi]) { set_error("world list holds an unknown or repeated world"); return BP_ERR_INVALID_ARGUMENT; }
+            if (!worlds[wi].resident) { set_error("world is not resident"); return BP_ERR_UNAVAILABLE; }
+            member[wi] = 1;
+            pendingList.push_back(wi);
+        }
+        // Per class, the listed worlds in the class's longest-first order.
+        subsetHost.clear();
+        for (int k = 0; k < kClassCount; ++k) {
+            launchOffset[k] = (uint32_t)subsetHost.size();
+            for (uint32_t wi : classWorlds[k])
+                if (member[wi]) subsetHost.push_back(wi);
+            launchCount[k] = (uint32_t)subsetHost.size() - launchOffset[k];
+        }
+        if (!subsetHost.empty())
+            BP_HIP(hipMemcpy(subsetOrders, subsetHost.data(), subsetHost.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    } else {
+        for (size_t wi = 0; wi < W; ++wi) pendingList.push_back((uint32_t)wi);
+        for (int k = 0; k < kClassCount; ++k) { launchOffset[k] = 0; launchCount[k] = (uint32_t)classWorlds[k].size(); }
+    }
+    auto orderOf = [&](int k) -> const uint32_t * { return subset ? subsetOrders + launchOffset[k] : orders[k]; };
+    auto worldOf = [&](int k, uint32_t j) -> uint32_t { return subset ? subsetHost[launchOffset[k] + j] : classWorlds[k][j]; };
     if (W == 0 || steps == 0) return BP_OK;
     size_t need = (size_t)steps * W;
     if (statsCapacity < need) {
@@ -556,13 +737,12 @@ bp_status bp_batch::step(uint32_t steps, bp_frame_info *infos) {
         if (s != BP_OK) return s;
         statsCapacity = need;
     }
-    BP_HIP(hipMemset(stats, 0, need * sizeof(bpd::WorldStatus)));
-    BP_HIP(hipDeviceSynchronize());
+    BP_HIP(hipMemsetAsync(stats, 0, need * sizeof(bpd::WorldStatus), streams[0]));
     BP_HIP(hipEventRecord(startEvent, streams[0]));
     for (int k = 1; k < kStreams; ++k) BP_HIP(hipStreamWaitEvent(streams[k], startEvent, 0));
     // Largest classes first so that their long-running workgroups start early on their stream.
     for (int k = kClassCount - 1; k >= 0; --k) {
-        uint32_t count = (uint32_t)classWorlds[k].size();
+        const uint32_t count = launchCount[k];
         if (!count || k == kCoopClass) continue;
         hipStream_t stream = streams[k % kStreams];
         BP_HIP(hipEventRecord(classStart[k], stream));
@@ -570,18 +750,18 @@ bp_status bp_batch::step(uint32_t steps, bp_frame_info *infos) {
         // through them without waiting for the slowest world of the class at every step.
         if (k == kGlobalClass) {
             for (uint32_t st = 0; st < steps; ++st) {
-                hipError_t e = bp_launch_world_step(1024, arenaDev, descs, stats + (size_t)st * W, orders[k], count, stream);
+                hipError_t e = bp_launch_world_step(1024, arenaDev, descs, stats + (size_t)st * W, orderOf(k), count, stream);
                 if (e != hipSuccess) { set_error(std::string("kernel launch: ") + hipGetErrorString(e)); return BP_ERR_COMMAND_FAILED; }
             }
         } else {
-            hipError_t e = bp_launch_world_lds(k, arenaDev, descs, stats, orders[k], count, steps, (uint32_t)W, stream);
+            hipError_t e = bp_launch_world_lds(k, arenaDev, descs, stats, orderOf(k), count, steps, (uint32_t)W, stream);
             if (e != hipSuccess) { set_error(std::string("kernel launch: ") + hipGetErrorString(e)); return BP_ERR_COMMAND_FAILED; }
         }
         BP_HIP(hipEventRecord(classEnd[k], stream));
     }
     // Cooperative worlds last, alone on the device: a cooperative grid needs all its workgroups
     // resident at once, so it never shares the device with the other classes' launches.
-    if (!classWorlds[kCoopClass].empty()) {
+    if (launchCount[kCoopClass]) {
         hipStream_t stream = streams[0];
         for (int k = 1; k < kStreams; ++k) {
             BP_HIP(hipEventRecord(endEvents[k], streams[k]));
@@ -589,16 +769,36 @@ bp_status bp_batch::step(uint32_t steps, bp_frame_info *infos) {
         }
         BP_HIP(hipEventRecord(classStart[kCoopClass], stream));
         for (uint32_t st = 0; st < steps; ++st)
-            for (size_t j = 0; j < classWorlds[kCoopClass].size(); ++j) {
-                const uint32_t wi = classWorlds[kCoopClass][j];
-                hipError_t e = bp_launch_world_coop(arenaDev, descs, stats + (size_t)st * W, orders[kCoopClass] + j, wi, coopGroups[wi],
-                                                    stream);
+            for (uint32_t j = 0; j < launchCount[kCoopClass]; ++j) {
+                const uint32_t wi = worldOf(kCoopClass, j);
+                hipError_t e = bp_launch_world_coop(arenaDev, descs, stats + (size_t)st * W, orderOf(kCoopClass) + j, wi,
+                                                    coopGroups[wi], stream);
                 if (e != hipSuccess) { set_error(std::string("cooperative launch: ") + hipGetErrorString(e)); return BP_ERR_COMMAND_FAILED; }
             }
         BP_HIP(hipEventRecord(classEnd[kCoopClass], stream));
     }
     for (int k = 0; k < kStreams; ++k) BP_HIP(hipEventRecord(endEvents[k], streams[k]));
-    for (int k = 0; k < kStreams; ++k) BP_HIP(hipEventSynchronize(endEvents[k]));
+    pending = true;
+    return BP_OK;
+}
+
+// Collects the launches of launch(): infos[step * listed + j] for the j-th listed world. Without
+// `wait`, returns BP_ERR_FRAME_IN_FLIGHT while the device is still working.
+bp_status bp_batch::finish(bp_frame_info *infos, bool wait) {
+    if (!pending) return BP_OK;
+    for (int k = 0; k < kStreams; ++k) {
+        if (wait) {
+            BP_HIP(hipEventSynchronize(endEvents[k]));
+        } else {
+            hipError_t e = hipEventQuery(endEvents[k]);
+            if (e == hipErrorNotReady) return BP_ERR_FRAME_IN_FLIGHT;
+            if (e != hipSuccess) { set_error(std::string("hipEventQuery: ") + hipGetErrorString(e)); pending = false; return BP_ERR_COMMAND_FAILED; }
+        }
+    }
+    pending = false;
+    const size_t W = worlds.size();
+    const uint32_t steps = pendingSteps;
+    if (W == 0 || steps == 0) return BP_OK;
     float ms = 0.0f;
     for (int k = 0; k < kStreams; ++k) {
         float e = 0.0f;
@@ -607,25 +807,34 @@ bp_status bp_batch::step(uint32_t steps, bp_frame_info *infos) {
     }
     for (int k = 0; k < kClassCount; ++k) {
         lastClassMs[k] = 0.0f;
-        if (classWorlds[k].empty()) continue;
+        if (!launchCount[k]) continue;
         BP_HIP(hipEventElapsedTime(&lastClassMs[k], classStart[k], classEnd[k]));
     }
     lastGpuMs = ms;
+    const size_t need = (size_t)steps * W;
     hostStats.resize(need);
     BP_HIP(hipMemcpy(hostStats.data(), stats, need * sizeof(bpd::WorldStatus), hipMemcpyDeviceToHost));
-    std::vector<uint32_t> worldClass(W, 0);
-    for (int k = 0; k < kClassCount; ++k)
-        for (uint32_t wi : classWorlds[k]) worldClass[wi] = (uint32_t)k;
-    if (infos)
-        for (size_t i = 0; i < need; ++i) {
-            fill_info(hostStats[i], &infos[i]);
-            infos[i].kernel_class = worldClass[i % W];
-            if (worldClass[i % W] == (uint32_t)kGlobalClass) infos[i].execution_path = BP_PATH_WORKGROUP;
-            if (worldClass[i % W] == (uint32_t)kCoopClass) infos[i].execution_path = BP_PATH_COOPERATIVE;
-            infos[i].gpu_milliseconds = ms / steps;
-            if (!hostStats[i].executed) infos[i].status = BP_ERR_UNAVAILABLE;
+    if (!infos) return BP_OK;
+    const size_t listed = pendingList.size();
+    for (uint32_t st = 0; st < steps; ++st)
+        for (size_t j = 0; j < listed; ++j) {
+            const uint32_t wi = pendingList[j];
+            bp_frame_info &out = infos[(size_t)st * listed + j];
+            fill_info(hostStats[(size_t)st * W + wi], &out);
+            const int cls = worlds[wi].packedClass;
+            out.kernel_class = cls < 0 ? 0xffffffffu : (uint32_t)cls;
+            if (cls == kGlobalClass) out.execution_path = BP_PATH_WORKGROUP;
+            if (cls == kCoopClass) out.execution_path = BP_PATH_COOPERATIVE;
+            out.gpu_milliseconds = ms / steps;
+            if (!hostStats[(size_t)st * W + wi].executed) out.status = BP_ERR_UNAVAILABLE;
         }
     return BP_OK;
+}
+
+bp_status bp_batch::step(uint32_t steps, bp_frame_info *infos) {
+    bp_status s = launch(nullptr, 0, steps);
+    if (s != BP_OK) return s;
+    return finish(infos, true);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -722,7 +931,7 @@ static bp_status world_from_desc(HostWorld &w, const bp_batch_world_desc *d) {
 bp_status bp_batch_add_world(bp_batch *b, const bp_batch_world_desc *desc, uint32_t *out_index) {
     if (!b) return BP_ERR_INVALID_ARGUMENT;
     std::lock_guard<std::mutex> lock(b->mutex);
-    if (b->inFlight) return BP_ERR_FRAME_IN_FLIGHT;
+    if (b->inFlight || b->pending) return BP_ERR_FRAME_IN_FLIGHT;
     bp_status s = b->download_state();
     if (s != BP_OK) return s;
     HostWorld w;
@@ -739,17 +948,70 @@ uint32_t bp_batch_world_count(const bp_batch *b) { return b ? (uint32_t)b->world
 bp_status bp_batch_step(bp_batch *b, uint32_t steps, bp_frame_info *infos) {
     if (!b) return BP_ERR_INVALID_ARGUMENT;
     std::unique_lock<std::mutex> lock(b->mutex, std::try_to_lock);
-    if (!lock.owns_lock() || b->inFlight) return BP_ERR_FRAME_IN_FLIGHT;
+    if (!lock.owns_lock() || b->inFlight || b->pending) return BP_ERR_FRAME_IN_FLIGHT;
     b->inFlight = true;
     bp_status s = b->step(steps, infos);
     b->inFlight = false;
     return s;
 }
 
+bp_status bp_batch_step_worlds(bp_batch *b, const uint32_t *worlds, uint32_t count, uint32_t steps, bp_frame_info *infos) {
+    if (!b || (count && !worlds)) return BP_ERR_INVALID_ARGUMENT;
+    std::unique_lock<std::mutex> lock(b->mutex, std::try_to_lock);
+    if (!lock.owns_lock() || b->inFlight || b->pending) return BP_ERR_FRAME_IN_FLIGHT;
+    b->inFlight = true;
+    bp_status s = b->launch(worlds, count, steps);
+    if (s == BP_OK) s = b->finish(infos, true);
+    b->inFlight = false;
+    return s;
+}
+
+bp_status bp_batch_submit_worlds(bp_batch *b, const uint32_t *worlds, uint32_t count, uint32_t steps) {
+    if (!b || (count && !worlds)) return BP_ERR_INVALID_ARGUMENT;
+    std::unique_lock<std::mutex> lock(b->mutex, std::try_to_lock);
+    if (!lock.owns_lock() || b->inFlight || b->pending) return BP_ERR_FRAME_IN_FLIGHT;
+    return b->launch(worlds, count, steps);
+}
+
+bp_status bp_batch_poll(bp_batch *b, bp_frame_info *infos) {
+    if (!b) return BP_ERR_INVALID_ARGUMENT;
+    std::unique_lock<std::mutex> lock(b->mutex, std::try_to_lock);
+    if (!lock.owns_lock() || b->inFlight) return BP_ERR_FRAME_IN_FLIGHT;
+    if (!b->pending) { set_error("nothing was submitted"); return BP_ERR_INVALID_ARGUMENT; }
+    return b->finish(infos, false);
+}
+
+bp_status bp_batch_write_bubbles(bp_batch *b, uint32_t world, const bp_bubble *bubbles, uint32_t count) {
+    if (!b || world >= b->worlds.size() || (count && !bubbles)) return BP_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(b->mutex);
+    if (b->inFlight || b->pending) return BP_ERR_FRAME_IN_FLIGHT;
+    return b->write_bubbles(world, bubbles, count);
+}
+
+bp_status bp_batch_set_resident(bp_batch *b, uint32_t world, uint32_t resident) {
+    if (!b || world >= b->worlds.size()) return BP_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(b->mutex);
+    if (b->inFlight || b->pending) return BP_ERR_FRAME_IN_FLIGHT;
+    HostWorld &w = b->worlds[world];
+    if (w.resident == (resident != 0)) return BP_OK;
+    bp_status s = b->download_state();            // every world's newest state, before the layout changes
+    if (s != BP_OK) return s;
+    w.resident = resident != 0;
+    b->packed = false;
+    return BP_OK;
+}
+
+bp_status bp_batch_world_footprint(const bp_batch *b, uint32_t world, uint64_t *bytes) {
+    if (!b || world >= b->worlds.size() || !bytes) return BP_ERR_INVALID_ARGUMENT;
+    *bytes = b->footprint(world);
+    return BP_OK;
+}
+
 bp_status bp_batch_read_bubbles(bp_batch *b, uint32_t world, bp_bubble *out, uint32_t count) {
     if (!b || world >= b->worlds.size()) return BP_ERR_INVALID_ARGUMENT;
     std::lock_guard<std::mutex> lock(b->mutex);
-    bp_status s = b->download_state();
+    if (b->pending) return BP_ERR_FRAME_IN_FLIGHT;
+    bp_status s = b->download_world(world);
     if (s != BP_OK) return s;
     const HostWorld &w = b->worlds[world];
     if (count != w.bubbles.size() || (count && !out)) return BP_ERR_INVALID_ARGUMENT;
@@ -775,6 +1037,7 @@ uint32_t bp_batch_class_count(void) { return kClassCount; }
 bp_status bp_batch_set_world_class(bp_batch *b, uint32_t world, int32_t cls) {
     if (!b || world >= b->worlds.size() || cls < -1 || cls >= kClassCount) return BP_ERR_INVALID_ARGUMENT;
     std::lock_guard<std::mutex> lock(b->mutex);
+    if (b->inFlight || b->pending) return BP_ERR_FRAME_IN_FLIGHT;
     bp_status s = b->download_state();
     if (s != BP_OK) return s;
     if (cls >= 0 && !class_fits(cls, b->worlds[world])) return BP_ERR_INVALID_ARGUMENT;

@@ -226,6 +226,45 @@ class Batch:
     def set_world_class(self, world: int, cls: int):
         abi.check(self._lib.bp_batch_set_world_class(self._handle, world, cls), "bp_batch_set_world_class")
 
+    # ---- serving -------------------------------------------------------------------------
+    def step_worlds(self, worlds, steps: int = 1) -> np.ndarray:
+        """Advances only the listed worlds; returns FrameInfo records shaped (steps, len(worlds))."""
+        w = np.ascontiguousarray(worlds, np.uint32)
+        infos = np.zeros(steps * len(w), abi.FRAME_DTYPE)
+        abi.check(self._lib.bp_batch_step_worlds(self._handle, abi.ptr(w), len(w), steps, abi.ptr(infos)),
+                  "bp_batch_step_worlds")
+        return infos.reshape(steps, len(w))
+
+    def submit_worlds(self, worlds, steps: int = 1):
+        """Queues a step of the listed worlds and returns at once (collect it with poll())."""
+        self._pending = np.ascontiguousarray(worlds, np.uint32)
+        self._pending_steps = steps
+        abi.check(self._lib.bp_batch_submit_worlds(self._handle, abi.ptr(self._pending), len(self._pending), steps),
+                  "bp_batch_submit_worlds")
+
+    def poll(self):
+        """None while the submitted step runs, then its FrameInfo records (steps, len(worlds))."""
+        infos = np.zeros(self._pending_steps * len(self._pending), abi.FRAME_DTYPE)
+        status = self._lib.bp_batch_poll(self._handle, abi.ptr(infos))
+        if status == abi.BP_ERR_FRAME_IN_FLIGHT:
+            return None
+        abi.check(status, "bp_batch_poll")
+        return infos.reshape(self._pending_steps, len(self._pending))
+
+    def write_bubbles(self, world: int, bubbles: np.ndarray):
+        """Replaces a world's bubbles between steps (a transaction)."""
+        b = np.ascontiguousarray(bubbles, abi.BUBBLE_DTYPE)
+        abi.check(self._lib.bp_batch_write_bubbles(self._handle, world, abi.ptr(b), len(b)), "bp_batch_write_bubbles")
+        self._counts[world] = len(b)
+
+    def set_resident(self, world: int, resident: bool):
+        abi.check(self._lib.bp_batch_set_resident(self._handle, world, 1 if resident else 0), "bp_batch_set_resident")
+
+    def footprint(self, world: int) -> int:
+        out = ctypes.c_uint64(0)
+        abi.check(self._lib.bp_batch_world_footprint(self._handle, world, ctypes.byref(out)), "bp_batch_world_footprint")
+        return int(out.value)
+
     @property
     def last_gpu_milliseconds(self) -> float:
         return float(self._lib.bp_batch_last_gpu_milliseconds(self._handle))

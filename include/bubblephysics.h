@@ -360,6 +360,27 @@ bp_status bp_batch_last_class_timing(const bp_batch *batch, float *milliseconds,
 /* Pins a world to a class (-1 = automatic); used by the autotuner. */
 bp_status bp_batch_set_world_class(bp_batch *batch, uint32_t world, int32_t cls);
 
+/* ------------------------------------------------------------------------ */
+/* Serving: continuous batching over resident worlds. No reference counterpart. */
+/* ------------------------------------------------------------------------ */
+/* Advances only the listed worlds (distinct, resident) by `steps` steps; infos[steps * count]
+ * (step-major, list order) may be null. */
+bp_status bp_batch_step_worlds(bp_batch *batch, const uint32_t *worlds, uint32_t count, uint32_t steps,
+                               bp_frame_info *infos);
+/* Non-blocking form: queues the launches and returns. bp_batch_poll returns BP_ERR_FRAME_IN_FLIGHT
+ * while they run, then BP_OK with infos filled as by bp_batch_step_worlds. One submission at a
+ * time; other batch calls return BP_ERR_FRAME_IN_FLIGHT until it is collected. */
+bp_status bp_batch_submit_worlds(bp_batch *batch, const uint32_t *worlds, uint32_t count, uint32_t steps);
+bp_status bp_batch_poll(bp_batch *batch, bp_frame_info *infos);
+/* Replaces a world's bubbles (ids strictly increasing; the count may change): a transaction
+ * applied between steps. Within the slots the world was packed with it is written in place. */
+bp_status bp_batch_write_bubbles(bp_batch *batch, uint32_t world, const bp_bubble *bubbles, uint32_t count);
+/* Residency: a non-resident world lives in host memory only and takes no device memory after the
+ * next step repacks the batch; it cannot be stepped until it is resident again. */
+bp_status bp_batch_set_resident(bp_batch *batch, uint32_t world, uint32_t resident);
+/* Device bytes a world holds (or would hold) while resident. */
+bp_status bp_batch_world_footprint(const bp_batch *batch, uint32_t world, uint64_t *bytes);
+
 #ifdef __cplusplus
 }
 #endif
