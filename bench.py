@@ -67,8 +67,8 @@ def run_batched(args, scenes_list):
     infos = batch.step(args.steps)
     elapsed = time.perf_counter() - t0
     ms, _, threads, capacity = batch.last_class_timing()
-    executed = infos["status"] != abi_unavailable()
-    substeps = np.where(executed, infos["substep_count"], 0).astype(np.int64)
+    ok = infos["status"] == 0                      # failed and unexecuted steps do not count
+    substeps = np.where(ok, infos["substep_count"], 0).astype(np.int64)
     substep_bubbles = int((substeps * sizes[None, :]).sum())
     deg = infos["adjacency_entries"].astype(np.float64) / np.maximum(sizes[None, :], 1)
     per = substeps * sizes[None, :] * bytes_per_bubble_substep(deg)
@@ -88,11 +88,6 @@ def run_batched(args, scenes_list):
     }
 
 
-def abi_unavailable():
-    from bubblephysics_amd import abi
-    return abi.BP_ERR_UNAVAILABLE
-
-
 def run_large(steps: int):
     """The 262,144-bubble world of config 3 on the cooperative grid."""
     from bubblephysics_amd import scenes, solver
@@ -110,6 +105,21 @@ def run_large(steps: int):
     elapsed = time.perf_counter() - t0
     return {"value": total / elapsed, "ms_per_step": elapsed / steps * 1e3, "gpu_ms_per_step": gpu_ms / steps,
             "bubbles": n, "bodies": len(scene.bodies), "statuses": sorted(set(statuses))}
+
+
+def run_served(args):
+    """Serving: Poisson step requests (10% with transactions) over resident worlds, continuous batching."""
+    from bubblephysics_amd import serving
+    t = time.perf_counter()
+    worlds = serving.serving_worlds(args.serve_worlds, seed=20261206)
+    print(f"[bench] built {len(worlds)} serving worlds in {time.perf_counter() - t:.1f}s", file=sys.stderr, flush=True)
+    server = serving.Server(worlds, memory_budget=int(200e9))
+    server.batch.step(1)                                   # packs and warms every kernel class
+    requests = serving.poisson_requests(len(worlds), args.serve_rate, args.serve_seconds, seed=20261207)
+    stats = server.run(requests).summary()
+    stats["bubbles"] = int(sum(len(w.bubbles) for w in worlds))
+    stats["worlds"] = len(worlds)
+    return stats
 
 
 def run_plain(steps: int):
@@ -155,6 +165,9 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=60)
     ap.add_argument("--latency-steps", type=int, default=10)
     ap.add_argument("--large-steps", type=int, default=5)
+    ap.add_argument("--serve-seconds", type=float, default=30.0)
+    ap.add_argument("--serve-rate", type=float, default=20000.0)
+    ap.add_argument("--serve-worlds", type=int, default=8192)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -168,11 +181,15 @@ def main():
     print(f"[bench] built {len(scenes_list)} worlds in {time.perf_counter() - t_build:.1f}s", file=sys.stderr, flush=True)
     res = run_batched(args, scenes_list)
     print(f"[bench] batched done: {res['elapsed']:.2f}s for {args.steps} steps", file=sys.stderr, flush=True)
-    plain = run_plain(args.plain_steps)
+    plain = run_plain(args.plain_steps) if args.plain_steps > 0 else {"value": float("nan"), "p50_ms": float("nan"),
+                                                                        "p99_ms": float("nan"), "bubbles": 0}
     print(f"[bench] plain done: {plain['value']:.3g} bubble-substeps/s", file=sys.stderr, flush=True)
     large = run_large(args.large_steps) if args.large_steps > 0 else None
     if large:
         print(f"[bench] large done: {large['value']:.3g} bubble-substeps/s", file=sys.stderr, flush=True)
+    served = run_served(args) if args.serve_seconds > 0 else None
+    if served:
+        print(f"[bench] served: {served}", file=sys.stderr, flush=True)
     cpu = None if args.no_cpu else cpu_baseline(scenes_list, args.cpu_seconds)
 
     value = res["substep_bubbles"] / res["elapsed"]
@@ -203,11 +220,16 @@ def main():
         "candidate_overflow_max": res["overflow"],
         "plain": {"workload": "plain: 1 world, 1024 bubbles + 64 kinematic bodies", "value": plain["value"],
                   "p50_step_ms": plain["p50_ms"], "p99_step_ms": plain["p99_ms"]},
-        "batched_over_plain": value / plain["value"] if plain["value"] > 0 else None,
+        "batched_over_plain": value / plain["value"] if plain["value"] > 0 else None,   # NaN compares false
         "large": None if large is None else {
             "workload": f"config 3: 1 world, {large['bubbles']} bubbles + {large['bodies']} bodies (kinematic and walls), "
                         "cooperative grid", "value": large["value"], "ms_per_step": large["ms_per_step"],
             "gpu_ms_per_step": large["gpu_ms_per_step"], "statuses": large["statuses"]},
+        "served": None if served is None else {
+            "workload": f"serving: {served['worlds']} resident worlds, bubbles log-uniform in [32, 16384] "
+                        f"({served['bubbles']} total), Poisson {args.serve_rate:.0f} step requests/s for "
+                        f"{args.serve_seconds:.0f}s, 10% with transactions, budget 200 GB",
+            **{k: v for k, v in served.items() if k not in ("worlds", "bubbles")}},
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBPS, "traffic": None,

@@ -29,6 +29,10 @@ extern "C" hipError_t bp_launch_world_lds(int id, const bpd::Arena *arena, bpd::
 extern "C" int bp_lds_config_count();
 extern "C" void bp_lds_config(int id, uint32_t *threads, uint32_t *per, uint32_t *rowcap, uint32_t *caps);
 extern "C" uint32_t bp_coop_max_groups();
+extern "C" hipError_t bp_launch_gather_world(const bpd::Arena *arena, const bpd::WorldDesc *desc, bp_bubble *out, uint32_t count,
+                                             hipStream_t stream);
+extern "C" hipError_t bp_launch_scatter_world(const bpd::Arena *arena, uint32_t offset, uint32_t count, const bp_bubble *in,
+                                              const int32_t *motorIdx, const int32_t *bandIdx, hipStream_t stream);
 extern "C" uint32_t bp_coop_block();
 extern "C" hipError_t bp_launch_world_coop(const bpd::Arena *arena, bpd::WorldDesc *descs, bpd::WorldStatus *stats,
                                            const uint32_t *orderEntry, uint32_t world, uint32_t groups, hipStream_t stream);
@@ -160,6 +164,9 @@ struct bp_batch {
     std::vector<uint32_t> pendingList; // worlds of the pending launches, in the caller's order
     std::vector<uint32_t> subsetHost;  // per class, the listed worlds (longest first)
     uint32_t *subsetOrders = nullptr;  // device copy of subsetHost
+    bp_bubble *xferBubbles = nullptr;  // one world's records in transit (transactions)
+    int32_t *xferIndices = nullptr;    // their motor and band indices
+    uint32_t xferCapacity = 0;
     uint32_t launchOffset[kClassCount] = {}, launchCount[kClassCount] = {};
     uint32_t totalBubbles = 0;
     std::mutex mutex;
@@ -182,6 +189,7 @@ struct bp_batch {
         if (startEvent) (void)hipEventDestroy(startEvent);
         startEvent = nullptr;
         stats = nullptr; statsCapacity = 0; descs = nullptr; arenaDev = nullptr; subsetOrders = nullptr;
+        xferBubbles = nullptr; xferIndices = nullptr; xferCapacity = 0;
         pending = false;
         for (auto &o : orders) o = nullptr;
         packed = false;
@@ -278,15 +286,10 @@ bp_status bp_batch::pack() {
         if (w.forcedClass >= 0 && class_fits(w.forcedClass, w) && (w.forcedClass != kCoopClass || coopMax > 0)) cls = w.forcedClass;
         if (!w.resident) cls = -1;
         worldClass[wi] = cls;
-        // Slots for the world's bubbles: some room to grow in place (serving transactions), within
-        // the bubble capacity of an LDS configuration.
+        // Slots for the world's bubbles, with room to grow in place (serving transactions); a world
+        // that outgrows its kernel configuration moves to a wider one without moving its data.
         const uint32_t n0 = (uint32_t)w.bubbles.size();
-        uint32_t slots = 0;
-        if (w.resident) {
-            slots = n0 + std::max<uint32_t>(8, n0 / 8);
-            if (cls >= 0 && cls < kGlobalClass) slots = std::min<uint32_t>(slots, class_capacity(cls));
-            slots = std::max(slots, n0);
-        }
+        const uint32_t slots = w.resident ? n0 + std::max<uint32_t>(8, n0 / 8) : 0u;
         worlds[wi].capacity = slots;
         worlds[wi].packedClass = cls;
         if (cls == kCoopClass) {
@@ -332,6 +335,10 @@ bp_status bp_batch::pack() {
     BP_ALLOC(arenaDev, 1);
     for (int k = 0; k < kClassCount; ++k) BP_ALLOC(orders[k], worlds.size());
     BP_ALLOC(subsetOrders, worlds.size());
+    xferCapacity = 1;
+    for (const auto &w : worlds) xferCapacity = std::max(xferCapacity, w.capacity);
+    BP_ALLOC(xferBubbles, xferCapacity);
+    BP_ALLOC(xferIndices, 2 * (size_t)xferCapacity);
 #undef BP_ALLOC
 
     // Host staging of every array.
@@ -573,42 +580,25 @@ static void fill_info(const bpd::WorldStatus &s, bp_frame_info *out) {
     }
 }
 
-// Copies the published state of one resident world back into its host mirror.
+// Copies the published bubble state of one resident world back into its host mirror (one
+// gather kernel and one copy; body poses are refreshed by download_state before a repack).
 bp_status bp_batch::download_world(uint32_t wi) {
     HostWorld &w = worlds[wi];
     if (!packed || !deviceValid || !w.resident) return BP_OK;
-    bpd::WorldDesc &d = hostDescs[wi];
-    BP_HIP(hipMemcpy(&d, descs + wi, sizeof d, hipMemcpyDeviceToHost));
-    const uint32_t n = (uint32_t)w.bubbles.size(), p = d.pub, o = d.bubbleOffset;
-    std::vector<float2> pos(n), start(n), vel(n);
-    std::vector<float> rot(n), om(n);
-    if (n) {
-        BP_HIP(hipMemcpy(pos.data(), arena.pos[p] + o, n * sizeof(float2), hipMemcpyDeviceToHost));
-        BP_HIP(hipMemcpy(start.data(), arena.start[p] + o, n * sizeof(float2), hipMemcpyDeviceToHost));
-        BP_HIP(hipMemcpy(vel.data(), arena.vel[p] + o, n * sizeof(float2), hipMemcpyDeviceToHost));
-        BP_HIP(hipMemcpy(rot.data(), arena.rot[p] + o, n * sizeof(float), hipMemcpyDeviceToHost));
-        BP_HIP(hipMemcpy(om.data(), arena.omega[p] + o, n * sizeof(float), hipMemcpyDeviceToHost));
-    }
+    const uint32_t n = (uint32_t)w.bubbles.size();
+    if (n == 0) return BP_OK;
+    std::vector<bp_bubble> rec(n);
+    hipError_t e = bp_launch_gather_world(arenaDev, descs + wi, xferBubbles, n, streams[0]);
+    if (e != hipSuccess) { set_error(std::string("gather: ") + hipGetErrorString(e)); return BP_ERR_COMMAND_FAILED; }
+    BP_HIP(hipMemcpyAsync(rec.data(), xferBubbles, n * sizeof(bp_bubble), hipMemcpyDeviceToHost, streams[0]));
+    BP_HIP(hipStreamSynchronize(streams[0]));
     for (uint32_t i = 0; i < n; ++i) {
         bp_bubble &b = w.bubbles[i];
-        b.center[0] = pos[i].x; b.center[1] = pos[i].y;
-        b.previous_center[0] = start[i].x; b.previous_center[1] = start[i].y;
-        b.velocity[0] = vel[i].x; b.velocity[1] = vel[i].y;
-        b.rotation = rot[i]; b.angular_velocity = om[i];
-    }
-    const uint32_t nb = (uint32_t)w.bodies.size();
-    if (nb) {
-        std::vector<float4> pose(nb), motion(nb);
-        std::vector<float2> prev(nb);
-        BP_HIP(hipMemcpy(pose.data(), arena.bodyPose[p] + d.bodyOffset, nb * sizeof(float4), hipMemcpyDeviceToHost));
-        BP_HIP(hipMemcpy(prev.data(), arena.bodyPrev[p] + d.bodyOffset, nb * sizeof(float2), hipMemcpyDeviceToHost));
-        BP_HIP(hipMemcpy(motion.data(), arena.bodyMotion + d.bodyOffset, nb * sizeof(float4), hipMemcpyDeviceToHost));
-        for (uint32_t k = 0; k < nb; ++k) {
-            bp_body &b = w.bodies[k];
-            b.pose[0] = pose[k].x; b.pose[1] = pose[k].y; b.pose[2] = pose[k].z;
-            b.previous[0] = prev[k].x; b.previous[1] = prev[k].y; b.previous[2] = pose[k].w;
-            b.velocity[0] = motion[k].x; b.velocity[1] = motion[k].y; b.velocity[2] = motion[k].z;
-        }
+        const bp_bubble &r = rec[i];
+        b.center[0] = r.center[0]; b.center[1] = r.center[1];
+        b.previous_center[0] = r.previous_center[0]; b.previous_center[1] = r.previous_center[1];
+        b.velocity[0] = r.velocity[0]; b.velocity[1] = r.velocity[1];
+        b.rotation = r.rotation; b.angular_velocity = r.angular_velocity;
     }
     return BP_OK;
 }
@@ -620,9 +610,7 @@ bp_status bp_batch::write_bubbles(uint32_t wi, const bp_bubble *bubbles, uint32_
     HostWorld &w = worlds[wi];
     for (uint32_t i = 1; i < count; ++i)
         if (!(bubbles[i - 1].id < bubbles[i].id)) { set_error("bubbles must be sorted by strictly increasing id"); return BP_ERR_INVALID_SNAPSHOT; }
-    const bool inPlace = packed && w.resident && w.packedClass >= 0 && count <= w.capacity &&
-                         class_fits(w.packedClass, count, w.segments.size(), w.bodies.size(), w.vertices.size() / 2,
-                                    w.triangles.size() / 4);
+    const bool inPlace = packed && w.resident && w.packedClass >= 0 && count <= w.capacity;
     if (!inPlace) {
         bp_status s = download_state();
         if (s != BP_OK) return s;
@@ -631,52 +619,52 @@ bp_status bp_batch::write_bubbles(uint32_t wi, const bp_bubble *bubbles, uint32_
         return BP_OK;
     }
     bpd::WorldDesc &d = hostDescs[wi];
-    BP_HIP(hipMemcpy(&d, descs + wi, sizeof d, hipMemcpyDeviceToHost));
-    std::vector<float2> pos(count), start(count), vel(count);
-    std::vector<float> rot(count), om(count), mass(count), invm(count), radius(count), Kc(count), gamma(count);
-    std::vector<uint32_t> flags(count), cat(count), mask(count), region(count, 0);
-    std::vector<int32_t> motorIdx(count, -1), bandIdx(count, -1);
-    std::vector<int64_t> ids(count);
-    for (uint32_t i = 0; i < count; ++i) {
-        const bp_bubble &b = bubbles[i];
-        pos[i] = make_float2(b.center[0], b.center[1]);
-        start[i] = make_float2(b.previous_center[0], b.previous_center[1]);
-        vel[i] = make_float2(b.velocity[0], b.velocity[1]);
-        rot[i] = b.rotation; om[i] = b.angular_velocity;
-        mass[i] = b.mass; invm[i] = b.inverse_mass; radius[i] = b.radius; Kc[i] = b.contact_stiffness;
-        gamma[i] = b.damping; flags[i] = b.flags; cat[i] = b.category; mask[i] = b.mask; ids[i] = b.id;
-    }
+    BP_HIP(hipMemcpy(&d, descs + wi, sizeof d, hipMemcpyDeviceToHost));   // the device side holds `pub`
     // Motor and band records keep their ids; their bubble indices are resolved again (the
     // device holds the records sorted by bubble id, as pack() wrote them).
+    std::vector<int32_t> indices(2 * (size_t)count, -1);
     std::vector<bp_motor> ms = w.motors;
     std::sort(ms.begin(), ms.end(), [](const bp_motor &a, const bp_motor &b) { return a.bubble_id < b.bubble_id; });
     std::vector<bp_band> bs = w.bands;
     std::sort(bs.begin(), bs.end(), [](const bp_band &a, const bp_band &b) { return a.bubble_id < b.bubble_id; });
     for (uint32_t i = 0; i < count; ++i) {
-        auto m = std::lower_bound(ms.begin(), ms.end(), ids[i], [](const bp_motor &r, int64_t id) { return r.bubble_id < id; });
-        if (m != ms.end() && m->bubble_id == ids[i]) motorIdx[i] = (int32_t)(m - ms.begin());
-        auto b = std::lower_bound(bs.begin(), bs.end(), ids[i], [](const bp_band &r, int64_t id) { return r.bubble_id < id; });
-        if (b != bs.end() && b->bubble_id == ids[i]) bandIdx[i] = (int32_t)(b - bs.begin());
+        const int64_t id = bubbles[i].id;
+        auto m = std::lower_bound(ms.begin(), ms.end(), id, [](const bp_motor &r, int64_t v) { return r.bubble_id < v; });
+        if (m != ms.end() && m->bubble_id == id) indices[i] = (int32_t)(m - ms.begin());
+        auto b = std::lower_bound(bs.begin(), bs.end(), id, [](const bp_band &r, int64_t v) { return r.bubble_id < v; });
+        if (b != bs.end() && b->bubble_id == id) indices[count + i] = (int32_t)(b - bs.begin());
     }
-    const uint32_t o = d.bubbleOffset;
-    auto put = [&](auto *dst, const auto &src) -> bp_status {
-        if (!src.empty()) BP_HIP(hipMemcpy(dst + o, src.data(), src.size() * sizeof(src[0]), hipMemcpyHostToDevice));
-        return BP_OK;
-    };
-    bp_status s = BP_OK;
-    for (int side = 0; side < 2; ++side)
-        if ((s = put(arena.pos[side], pos)) || (s = put(arena.start[side], start)) || (s = put(arena.vel[side], vel)) ||
-            (s = put(arena.rot[side], rot)) || (s = put(arena.omega[side], om)) || (s = put(arena.region[side], region)))
-            return s;
-    if ((s = put(arena.mass, mass)) || (s = put(arena.invm, invm)) || (s = put(arena.radius, radius)) || (s = put(arena.Kc, Kc)) ||
-        (s = put(arena.gamma, gamma)) || (s = put(arena.flags, flags)) || (s = put(arena.cat, cat)) || (s = put(arena.mask, mask)) ||
-        (s = put(arena.id, ids)) || (s = put(arena.motor, motorIdx)) || (s = put(arena.band, bandIdx)))
-        return s;
+    if (count) {
+        BP_HIP(hipMemcpyAsync(xferBubbles, bubbles, count * sizeof(bp_bubble), hipMemcpyHostToDevice, streams[0]));
+        BP_HIP(hipMemcpyAsync(xferIndices, indices.data(), indices.size() * sizeof(int32_t), hipMemcpyHostToDevice, streams[0]));
+        hipError_t e = bp_launch_scatter_world(arenaDev, d.bubbleOffset, count, xferBubbles, xferIndices, xferIndices + count,
+                                               streams[0]);
+        if (e != hipSuccess) { set_error(std::string("scatter: ") + hipGetErrorString(e)); return BP_ERR_COMMAND_FAILED; }
+        BP_HIP(hipStreamSynchronize(streams[0]));
+    }
     d.bubbleCount = count;
     d.buckets = w.config.grid_bucket_count == 0 ? auto_buckets(count) : std::min(w.config.grid_bucket_count, bpd::kMaxBuckets);
     d.failed = 0;
     BP_HIP(hipMemcpy(descs + wi, &d, sizeof d, hipMemcpyHostToDevice));
     w.bubbles.assign(bubbles, bubbles + count);
+    // A world that no longer fits its kernel configuration moves to another one: only the order
+    // lists of the two configurations change. A world without cooperative storage takes the
+    // one-workgroup global kernel instead of the cooperative one.
+    if (!class_fits(w.packedClass, w)) {
+        int cls = default_class_for(w, d.coopBuckets != 0);
+        if (w.forcedClass >= 0 && class_fits(w.forcedClass, w) && (w.forcedClass != kCoopClass || d.coopBuckets != 0)) cls = w.forcedClass;
+        if (cls == kCoopClass && d.coopBuckets == 0) cls = kGlobalClass;
+        const int old = w.packedClass;
+        auto &from = classWorlds[old];
+        from.erase(std::remove(from.begin(), from.end(), wi), from.end());
+        auto &to = classWorlds[cls];
+        to.insert(std::upper_bound(to.begin(), to.end(), wi, [&](uint32_t a, uint32_t b) {
+                      return hostDescs[a].bubbleCount > hostDescs[b].bubbleCount;
+                  }), wi);
+        w.packedClass = cls;
+        bp_status s2 = BP_OK;
+        if ((s2 = upload(orders[old], from)) != BP_OK || (s2 = upload(orders[cls], to)) != BP_OK) return s2;
+    }
     return BP_OK;
 }
 

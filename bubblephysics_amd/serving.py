@@ -122,6 +122,8 @@ class ServeStats:
     evictions: int = 0
     admissions: int = 0
     failed: int = 0
+    transaction_seconds: float = 0.0   # host time spent applying transactions
+    launch_sizes: list = field(default_factory=list)
     latencies_ms: list = field(default_factory=list)
 
     def summary(self) -> dict:
@@ -133,6 +135,8 @@ class ServeStats:
             "p50_ms": float(np.percentile(lat, 50)) if len(lat) else math.nan,
             "p99_ms": float(np.percentile(lat, 99)) if len(lat) else math.nan,
             "evictions": self.evictions, "admissions": self.admissions, "failed_steps": self.failed,
+            "mean_worlds_per_launch": float(np.mean(self.launch_sizes)) if self.launch_sizes else 0.0,
+            "transaction_seconds": self.transaction_seconds,
         }
 
 
@@ -197,12 +201,15 @@ class Server:
                 busy = {w for w, _ in batch_list}
                 for w, r in batch_list:
                     self._admit(w, busy)
+                t_tx = time.perf_counter()
                 for w, r in batch_list:
                     if r.transaction:
                         nb = apply_transaction(self.batch.read_bubbles(w), r.transaction, r.seed)
                         self.batch.write_bubbles(w, nb)
                         self.counts[w] = len(nb)
+                self.stats.transaction_seconds += time.perf_counter() - t_tx
                 self.batch.submit_worlds([w for w, _ in batch_list], 1)
+                self.stats.launch_sizes.append(len(batch_list))
                 in_flight = batch_list
                 self.stats.launches += 1
                 continue
@@ -215,7 +222,8 @@ class Server:
                 info = infos[0, j]
                 r.done, r.status, r.substeps = done, int(info["status"]), int(info["substep_count"])
                 self.stats.latencies_ms.append((done - r.arrival) * 1e3)
-                self.stats.bubble_substeps += self.counts[w] * r.substeps
+                if r.status == abi.BP_OK:                      # work of failed steps is not counted
+                    self.stats.bubble_substeps += self.counts[w] * r.substeps
                 self.stats.served += 1
                 self.stats.failed += r.status != abi.BP_OK
                 if on_done:
