@@ -140,20 +140,76 @@ def run_plain(steps: int):
     return {"value": total / elapsed, "p50_ms": percentile(lat, 50), "p99_ms": percentile(lat, 99), "bubbles": n}
 
 
-def cpu_baseline(scenes_list, budget_s: float):
-    """The oracle (FP32, one thread) on the first worlds of the same batch, one step each,
-    until the time budget is used."""
+def cpu_baseline(scenes_list, budget_s: float, threads: int):
+    """The oracle (FP32) on the worlds of the same batch, one step each, spread over `threads` host
+    threads (the oracle library runs outside the interpreter lock), until the wall budget is used."""
+    from concurrent.futures import ThreadPoolExecutor
     from oracle import bp_oracle
-    total, used, worlds = 0, 0.0, 0
-    for s in scenes_list:
-        t0 = time.perf_counter()
+    bp_oracle.load()
+    deadline = time.perf_counter() + budget_s
+
+    def one(s):
+        if time.perf_counter() >= deadline:
+            return None
         _, _, frames, _ = bp_oracle.run(s, 1, assert_mode=1)
-        used += time.perf_counter() - t0
-        total += len(s.bubbles) * int(frames[0]["substeps"])
-        worlds += 1
-        if used >= budget_s:
-            break
-    return {"value": total / used if used > 0 else float("nan"), "worlds": worlds, "seconds": used}
+        return len(s.bubbles) * int(frames[0]["substeps"])
+
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=threads) as pool:
+        done = [r for r in pool.map(one, scenes_list) if r is not None]
+    used = time.perf_counter() - t0
+    return {"value": sum(done) / used if used > 0 else float("nan"), "worlds": len(done), "seconds": used,
+            "threads": threads}
+
+
+def fp64_drift(checkpoints=(1, 10, 30, 60)):
+    """Largest difference of bubble positions between the FP32 and FP64 oracle runs of the
+    256-bubble world (config 1) after each checkpoint's number of steps, absolute and relative to
+    the region size. The pile is chaotic, so the difference grows with the step count."""
+    from bubblephysics_amd import scenes
+    from oracle import bp_oracle
+    scene = scenes.config1()
+    r = scene.config.region
+    size = float(max(r[2] - r[0], r[3] - r[1]))
+    out = {"world": "config 1: 256 bubbles, walls, gravity"}
+    for steps in checkpoints:
+        b32, _, f32, _ = bp_oracle.run(scene, steps, precision=32, assert_mode=1)
+        b64, _, f64, _ = bp_oracle.run(scene, steps, precision=64, assert_mode=1)
+        d = np.abs(b32["center"].astype(np.float64) - b64["center"].astype(np.float64)).max(axis=1)
+        out[f"after_{steps}_steps"] = {"max_abs": float(d.max()), "median_abs": float(np.median(d)),
+                                       "max_rel_to_region": float(d.max()) / size, "completed": [len(f32), len(f64)]}
+    return out
+
+
+def copy_calibration(nbytes: int, repeats: int = 5):
+    """A plain device copy of `nbytes`, for scaling the FETCH_SIZE and WRITE_SIZE counters of a
+    profiler run: the counters of this kernel should read nbytes / 1024 KiB each."""
+    import torch
+    src = torch.empty(nbytes // 4, dtype=torch.int32, device="cuda").fill_(1)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(repeats):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / repeats * 1e3
+    print(json.dumps({"copy_bytes": nbytes, "ms_per_copy": ms, "GB/s": 2 * nbytes / ms / 1e6}))
+
+
+def recorded_traffic(kernel: str, steps: int, worlds: int):
+    """HBM bytes per launch of `kernel` (FETCH_SIZE + WRITE_SIZE, scaled by the copy calibration) as
+    recorded in profiles/traffic.txt for this workload, or None when no record matches."""
+    path = ROOT / "profiles" / "traffic.txt"
+    if not path.exists():
+        return None
+    for line in path.read_text().splitlines():
+        parts = line.split("|")
+        if len(parts) != 5 or line.lstrip().startswith("#"):
+            continue
+        name, st, wd, fetch, write = (p.strip() for p in parts)
+        if name == kernel and st == str(steps) and wd == str(worlds):
+            return float(fetch) + float(write)
+    return None
 
 
 def main():
@@ -169,12 +225,17 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=20000.0)
     ap.add_argument("--serve-worlds", type=int, default=8192)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-threads", type=int, default=min(os.cpu_count() or 1, 16))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-bytes", type=int, default=0, help="only run the copy calibration kernel of this size")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: the worlds are independent and this build measures one MI355X "
               f"(--gpus {args.gpus} is not supported)", file=sys.stderr)
         sys.exit(2)
+    if args.copy_bytes:
+        copy_calibration(args.copy_bytes)
+        return
 
     t_build = time.perf_counter()
     scenes_list = build_batched(args.worlds)
@@ -190,12 +251,18 @@ def main():
     served = run_served(args) if args.serve_seconds > 0 else None
     if served:
         print(f"[bench] served: {served}", file=sys.stderr, flush=True)
-    cpu = None if args.no_cpu else cpu_baseline(scenes_list, args.cpu_seconds)
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(scenes_list, args.cpu_seconds, args.cpu_threads)
+        cpu["drift"] = fp64_drift()
+        print(f"[bench] cpu done: {cpu['value']:.3g} bubble-substeps/s on {cpu['threads']} threads", file=sys.stderr,
+              flush=True)
 
     value = res["substep_bubbles"] / res["elapsed"]
     k = int(np.argmax(res["class_ms"]))
     dur_s = float(res["class_ms"][k]) / 1e3          # one launch of the class covers all the timed steps
     achieved = res["class_bytes"][k] / dur_s / 1e9 if dur_s > 0 else 0.0
+    traffic = recorded_traffic(kernel_name(k, res), args.steps, res["worlds"])
     line = {
         "metric": "bubble-substeps/sec on 1 MI355X (batched/served vs plain) + p50/p99 step latency",
         "value": value,
@@ -232,14 +299,15 @@ def main():
             **{k: v for k, v in served.items() if k not in ("worlds", "bubbles")}},
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
+            "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
             "kernel": kernel_name(k, res),
             "kernel_ms_per_launch": float(res["class_ms"][k]),
         },
         "cpu_baseline": None if cpu is None else {
-            "value": cpu["value"], "unit": "bubble-substeps/s", "cores": 1, "kind": "port",
-            "sample": f"oracle FP32, one thread, one step of the first {cpu['worlds']} worlds of the same batch "
-                      f"({cpu['seconds']:.1f}s)",
+            "value": cpu["value"], "unit": "bubble-substeps/s", "cores": cpu["threads"], "kind": "port",
+            "sample": f"oracle FP32, one step of each of the first {cpu['worlds']} worlds of the same batch, worlds "
+                      f"spread over {cpu['threads']} host threads ({cpu['seconds']:.1f}s wall)",
+            "fp64_drift": cpu["drift"],
         },
     }
     print(json.dumps(line, default=float))
