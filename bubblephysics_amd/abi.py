@@ -124,7 +124,7 @@ class FrameInfo(ctypes.Structure):
         ("group_width", ctypes.c_uint32), ("region_removals", ctypes.c_uint32), ("region_wraps", ctypes.c_uint32),
         ("gpu_milliseconds", ctypes.c_float), ("simulated_time", ctypes.c_float), ("limiter_object", ctypes.c_uint64),
         ("limiter_term", ctypes.c_uint32), ("adjacency_entries", ctypes.c_uint32), ("kernel_class", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32), ("detail", FailureDetail),
+        ("device_ticks", ctypes.c_uint32), ("detail", FailureDetail),
     ]
 
 

@@ -567,6 +567,7 @@ static void fill_info(const bpd::WorldStatus &s, bp_frame_info *out) {
     out->limiter_object = s.limiterObject;
     out->limiter_term = s.limiterTerm;
     out->adjacency_entries = s.adjacencyEntries;
+    out->device_ticks = s.ticks;
     out->detail.object_ids[0] = s.violationObjects[0];
     out->detail.object_ids[1] = s.violationObjects[1];
     out->detail.violation_kind = (int32_t)s.violationKind;

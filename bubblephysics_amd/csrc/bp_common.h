@@ -11,10 +11,14 @@
 
 namespace bpd {
 
-struct Ctl {           // NativeSubstepControl, held in registers and uniform across the workgroup
+constexpr uint32_t kNoOwner = 0xffffffffu;
+
+// NativeSubstepControl, held in registers and uniform across the workgroup. The limiter is kept
+// as the limiting bubble's index and turned into its id once, at the end of the step.
+struct Ctl {
     float t0, t1, h, split;
     uint32_t c0, c1, rebuild, limit;
-    uint64_t limObj;
+    uint32_t limOwner;
     uint32_t limTerm;
 };
 
@@ -147,10 +151,17 @@ BPD void bucket_prefix(Red<BLOCK> &r, uint32_t *buckets, uint32_t count) {
     for (uint32_t b = first; b < last; ++b) { uint32_t own = buckets[b]; buckets[b] = running; running += own; }
 }
 
+// The modulus is a mask when the bucket count is a power of two (the automatic counts are).
 BPD uint32_t bucket_of(int64_t cx, int64_t cy, uint32_t buckets) {      // Force:1000-1002
-    return ((uint32_t)cx * 73856093u ^ (uint32_t)cy * 19349663u) % buckets;
+    const uint32_t h = (uint32_t)cx * 73856093u ^ (uint32_t)cy * 19349663u;
+    return (buckets & (buckets - 1u)) == 0u ? (h & (buckets - 1u)) : h % buckets;
 }
-BPD int64_t cell_coord(float x, float width) { return (int64_t)floorf(x / width); }
+// floor(x / w) as int64; cells within the int32 range take the one-instruction conversion,
+// which gives the same integer.
+BPD int64_t cell_coord(float x, float width) {
+    const float f = floorf(x / width);
+    return fabsf(f) < 2147483648.0f ? (int64_t)(int32_t)f : (int64_t)f;
+}
 
 // ---------------------------------------------------------------------------------------
 // Geometry, force records and configuration of one world (working side of the body poses).
@@ -301,21 +312,21 @@ BPD uint32_t predict_bodies(const Arena *A, const WorldDesc *d, uint32_t pub, ui
 // Reduction of h for the whole world once the three terms are known (Force:633-736).
 // Every lane evaluates it with the same inputs, so the control state stays uniform.
 BPD void reduce_h(Ctl &c, float stiffnessBound, uint32_t w2Owner, float dbMin, uint32_t dbOwner, float mbMin,
-                  uint32_t mbOwner, const int64_t *ids, float *history, uint32_t lane) {
+                  uint32_t mbOwner, float *history, uint32_t lane) {
     float remaining = c.t0 - c.t1;
     if (mbMin <= 0.0f) {
         c.rebuild = 1; c.h = 0.0f;
-        c.limObj = (uint64_t)ids[mbOwner]; c.limTerm = BP_TERM_MARGIN;
+        c.limOwner = mbOwner; c.limTerm = BP_TERM_MARGIN;
         return;
     }
     float split = fmaxf(c.split, 1.0f);
     float h = remaining;
     uint32_t term = BP_TERM_REMAINING;
-    uint64_t object = 0;
+    uint32_t object = kNoOwner;
     if (split > 1.0f && c.t0 / split < h) h = c.t0 / split;
-    if (stiffnessBound / split < h) { h = stiffnessBound / split; term = BP_TERM_STIFFNESS; object = (uint64_t)ids[w2Owner]; }
-    if (dbMin / split < h) { h = dbMin / split; term = BP_TERM_DISTANCE; object = (uint64_t)ids[dbOwner]; }
-    if (mbMin < h) { h = mbMin; term = BP_TERM_MARGIN; object = (uint64_t)ids[mbOwner]; c.rebuild = 1; }
+    if (stiffnessBound / split < h) { h = stiffnessBound / split; term = BP_TERM_STIFFNESS; object = w2Owner; }
+    if (dbMin / split < h) { h = dbMin / split; term = BP_TERM_DISTANCE; object = dbOwner; }
+    if (mbMin < h) { h = mbMin; term = BP_TERM_MARGIN; object = mbOwner; c.rebuild = 1; }
     h = fmaxf(h, c.t0 * 1e-7f);
     uint32_t frameSubstep = c.c1;
     if (h >= remaining || remaining - h <= c.t0 * 1e-5f) { h = remaining; c.t1 = c.t0; }
@@ -323,8 +334,13 @@ BPD void reduce_h(Ctl &c, float stiffnessBound, uint32_t w2Owner, float dbMin, u
     c.h = h;
     c.c0 += 1;
     c.c1 = frameSubstep + 1;
-    c.limObj = object; c.limTerm = term;
+    c.limOwner = object; c.limTerm = term;
     if (lane == 0) history[frameSubstep] = h;
+}
+
+// Id of the limiting bubble (substepControl.limiter.x), 0 when no bubble limits the step.
+BPD uint64_t limiter_object(const Ctl &c, const int64_t *ids) {
+    return c.limOwner == kNoOwner ? 0u : (uint64_t)ids[c.limOwner];
 }
 
 // Region removal or wrap of one bubble (PostSolve:536-566). Returns the region result and

@@ -136,6 +136,7 @@ struct WorldStatus {
     uint32_t adjacencyEntries;
     uint32_t groupWidth;
     uint32_t executed;        // 1 when the kernel ran this world's step
+    uint32_t ticks;           // device wall-clock ticks (100 MHz) the workgroup spent on the step
 };
 
 // ---------------------------------------------------------------------------------------

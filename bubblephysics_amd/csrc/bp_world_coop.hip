@@ -497,6 +497,7 @@ __global__ __launch_bounds__(BLOCK) void world_coop_kernel(const Arena *__restri
     __shared__ Red<BLOCK> red;
     WorldDesc *d = descs + wi;
     WorldStatus *st = stats + wi;
+    const uint64_t ticks0 = wall_clock64();
     const uint32_t gtid = blockIdx.x * BLOCK + threadIdx.x;
     const Range R{gtid, gridDim.x * BLOCK};
     if (d->failed) {                                          // uniform: every workgroup returns
@@ -532,7 +533,7 @@ __global__ __launch_bounds__(BLOCK) void world_coop_kernel(const Arena *__restri
     const uint32_t limit = d->limit;
     Ctl c;
     c.t0 = d->dt; c.t1 = n == 0 ? d->dt : 0.0f; c.h = 0.0f; c.split = (float)(d->split < 1 ? 1u : d->split);
-    c.c0 = c.c1 = c.rebuild = c.limit = 0; c.limObj = 0; c.limTerm = BP_TERM_REMAINING;
+    c.c0 = c.c1 = c.rebuild = c.limit = 0; c.limOwner = kNoOwner; c.limTerm = BP_TERM_REMAINING;
     uint32_t ov1 = 0, ov2 = 0, ov3 = 0, pairs = 0, entriesTotal = 0, dropped = 0, ov0 = 0;
     int32_t status = BP_OK;
     bool claimed = false;
@@ -557,7 +558,7 @@ __global__ __launch_bounds__(BLOCK) void world_coop_kernel(const Arena *__restri
                     const float stiffnessBound = w2Max > 0.0f ? 1.0f / __builtin_sqrtf(w2Max) : inff();
                     pass_distance<BLOCK>(W, red, R, c, fminf(remaining, stiffnessBound), dbMin, dbOwn);
                     grid_min<BLOCK>(grid, red, s, dbMin, dbOwn);
-                    reduce_h(c, stiffnessBound, w2Own, dbMin, dbOwn, mbMin, mbOwn, W.id, history, gtid);
+                    reduce_h(c, stiffnessBound, w2Own, dbMin, dbOwn, mbMin, mbOwn, history, gtid);
                 }
                 const bool finished = !(c.t0 - c.t1 > 0.0f);
                 if (!(c.h > 0.0f)) {
@@ -638,7 +639,7 @@ __global__ __launch_bounds__(BLOCK) void world_coop_kernel(const Arena *__restri
         st->removals = s.counters[kCoRemovals];
         st->wraps = s.counters[kCoWraps];
         st->time[0] = c.t0; st->time[1] = c.t1; st->time[2] = c.h; st->time[3] = c.split;
-        st->limiterObject = c.limObj;
+        st->limiterObject = limiter_object(c, W.id);
         st->limiterTerm = c.limTerm;
         if (!claimed) {
             st->violationKind = 0; st->violationSegment = 0; st->violationObjects[0] = 0; st->violationObjects[1] = 0;
@@ -647,6 +648,7 @@ __global__ __launch_bounds__(BLOCK) void world_coop_kernel(const Arena *__restri
         st->adjacencyEntries = entriesTotal;
         st->groupWidth = BLOCK;
         st->executed = 1;
+        st->ticks = (uint32_t)(wall_clock64() - ticks0);
         if (status == BP_OK) d->pub = wside;
         else if (status != BP_ERR_CAPACITY_OVERFLOW) d->failed = 1;
     }

@@ -274,6 +274,7 @@ __global__ __launch_bounds__(BLOCK) void world_step_kernel(const Arena *__restri
     const uint32_t wi = order[blockIdx.x];
     WorldDesc *d = descs + wi;
     WorldStatus *st = stats + wi;
+    const uint64_t ticks0 = wall_clock64();
     if (d->failed) {
         if (threadIdx.x == 0) st->executed = 0;
         return;
@@ -329,7 +330,7 @@ __global__ __launch_bounds__(BLOCK) void world_step_kernel(const Arena *__restri
     const uint32_t limit = d->limit;
     Ctl c;
     c.t0 = d->dt; c.t1 = n == 0 ? d->dt : 0.0f; c.h = 0.0f; c.split = (float)(d->split < 1 ? 1u : d->split);
-    c.c0 = c.c1 = c.rebuild = c.limit = 0; c.limObj = 0; c.limTerm = BP_TERM_REMAINING;
+    c.c0 = c.c1 = c.rebuild = c.limit = 0; c.limOwner = kNoOwner; c.limTerm = BP_TERM_REMAINING;
     uint32_t ov0 = 0, ov1 = 0, ov2 = 0, ov3 = 0, pairs = 0, entriesTotal = 0;
     int32_t status = BP_OK;
     bool claimed = false;                 // sticky contact-assertion record of the frame
@@ -356,28 +357,7 @@ __global__ __launch_bounds__(BLOCK) void world_step_kernel(const Arena *__restri
                     float stiffnessBound = w2Max > 0.0f ? 1.0f / __builtin_sqrtf(w2Max) : inff();
                     float hMax = fminf(remaining, stiffnessBound);
                     pass_distance<BLOCK>(W, sh.red, Range{threadIdx.x, BLOCK}, c, hMax, dbMin, dbOwn);
-                    if (mbMin <= 0.0f) {
-                        c.rebuild = 1; c.h = 0.0f;
-                        c.limObj = (uint64_t)W.id[mbOwn]; c.limTerm = BP_TERM_MARGIN;
-                    } else {
-                        float split = fmaxf(c.split, 1.0f);
-                        float h = remaining;
-                        uint32_t term = BP_TERM_REMAINING;
-                        uint64_t object = 0;
-                        if (split > 1.0f && c.t0 / split < h) h = c.t0 / split;
-                        if (stiffnessBound / split < h) { h = stiffnessBound / split; term = BP_TERM_STIFFNESS; object = (uint64_t)W.id[w2Own]; }
-                        if (dbMin / split < h) { h = dbMin / split; term = BP_TERM_DISTANCE; object = (uint64_t)W.id[dbOwn]; }
-                        if (mbMin < h) { h = mbMin; term = BP_TERM_MARGIN; object = (uint64_t)W.id[mbOwn]; c.rebuild = 1; }
-                        h = fmaxf(h, c.t0 * 1e-7f);
-                        uint32_t frameSubstep = c.c1;
-                        if (h >= remaining || remaining - h <= c.t0 * 1e-5f) { h = remaining; c.t1 = c.t0; }
-                        else c.t1 = c.t1 + h;
-                        c.h = h;
-                        c.c0 += 1;
-                        c.c1 = frameSubstep + 1;
-                        c.limObj = object; c.limTerm = term;
-                        if (threadIdx.x == 0) history[frameSubstep] = h;
-                    }
+                    reduce_h(c, stiffnessBound, w2Own, dbMin, dbOwn, mbMin, mbOwn, history, threadIdx.x);
                 }
                 bool finished = !(c.t0 - c.t1 > 0.0f);
                 if (!(c.h > 0.0f)) {
@@ -484,7 +464,7 @@ __global__ __launch_bounds__(BLOCK) void world_step_kernel(const Arena *__restri
         st->removals = removals;
         st->wraps = wraps;
         st->time[0] = c.t0; st->time[1] = c.t1; st->time[2] = c.h; st->time[3] = c.split;
-        st->limiterObject = c.limObj;
+        st->limiterObject = limiter_object(c, W.id);
         st->limiterTerm = c.limTerm;
         if (!claimed) {
             st->violationKind = 0; st->violationSegment = 0; st->violationObjects[0] = 0; st->violationObjects[1] = 0;
@@ -493,6 +473,7 @@ __global__ __launch_bounds__(BLOCK) void world_step_kernel(const Arena *__restri
         st->adjacencyEntries = entriesTotal;
         st->groupWidth = BLOCK;
         st->executed = 1;
+        st->ticks = (uint32_t)(wall_clock64() - ticks0);
         if (status == BP_OK) d->pub = wside;
         else if (status != BP_ERR_CAPACITY_OVERFLOW) d->failed = 1;
     }

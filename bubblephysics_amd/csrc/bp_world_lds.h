@@ -97,6 +97,7 @@ __global__ __launch_bounds__(BLOCK) void world_lds_kernel(const Arena *__restric
     uint32_t pubNow = d->pub;                                 // published side, tracked in registers
     for (uint32_t stepIndex = 0; stepIndex < steps; ++stepIndex) {
         st = stats + (size_t)stepIndex * statsStride + wi;
+        const uint64_t ticks0 = wall_clock64();
         const uint32_t pub = pubNow, ws = 1u - pub;
         const uint32_t n = d->bubbleCount, o = d->bubbleOffset;
         const Geo g = make_geo(A, d, ws);
@@ -180,7 +181,7 @@ __global__ __launch_bounds__(BLOCK) void world_lds_kernel(const Arena *__restric
 
         Ctl c;
         c.t0 = d->dt; c.t1 = n == 0 ? d->dt : 0.0f; c.h = 0.0f; c.split = (float)(d->split < 1 ? 1u : d->split);
-        c.c0 = c.c1 = c.rebuild = c.limit = 0; c.limObj = 0; c.limTerm = BP_TERM_REMAINING;
+        c.c0 = c.c1 = c.rebuild = c.limit = 0; c.limOwner = kNoOwner; c.limTerm = BP_TERM_REMAINING;
         uint32_t ov0 = 0, ov1 = 0, ov2 = 0, ov3 = 0, pairs = 0, entriesTotal = 0;
         int32_t status = BP_OK;
         bool claimed = false;
@@ -296,12 +297,15 @@ __global__ __launch_bounds__(BLOCK) void world_lds_kernel(const Arena *__restric
                             const uint32_t j = S.grid[k];
                             if (j == i) continue;
                             const float2 q = S.pos[j];
-                            if (cell_coord(q.x, cellWidth) != tx || cell_coord(q.y, cellWidth) != ty) continue;
+                            // The reach test first: it is cheaper than the cell test and most
+                            // entries fail it (the conditions are independent, so the result is
+                            // the same in either order).
                             const uint32_t l = min(i, j), h = max(i, j);
                             const float2 pl = l == i ? pi : q, ph = l == i ? q : pi;
                             const float ml = l == i ? mi : S.aux[j], mh = l == i ? S.aux[j] : mi;
                             const float reach = S.r[l] + S.r[h] + ml + mh + sepTol;
                             if (!(fabsf(pl.x - ph.x) <= reach && fabsf(pl.y - ph.y) <= reach)) continue;
+                            if (cell_coord(q.x, cellWidth) != tx || cell_coord(q.y, cellWidth) != ty) continue;
                             if (!filter_accepts(catG[l], maskG[l], catG[h], maskG[h])) continue;
                             if (j > i) {
                                 if (!ownRow) continue;
@@ -564,12 +568,43 @@ __global__ __launch_bounds__(BLOCK) void world_lds_kernel(const Arena *__restric
             }
         };
 
+        // Body and one-sided edge assertions of a movable bubble (Force:889-917) against the
+        // staged geometry, in the reference's order; the frames are those of the current fraction.
+        auto assert_staged = [&](float2 center, float2 start, uint32_t cat, uint32_t mask, Violation &out)
+            __attribute__((always_inline)) -> bool {
+            for (uint32_t k = 0; k < g.nb; ++k) {
+                const uint4 kind = S.bodyKind[k];
+                if (!(kind.y & (kBodyStatic | kBodyKinematic)) || !filter_accepts(cat, mask, kind.z, kind.w)) continue;
+                if (frame_inside(center, frame_at(k), S.verts, S.tris)) {
+                    out.kind = BP_VIOLATION_INSIDE_BODY; out.other = g.bodyId(k); out.segment = 0; out.detail = 0.0f;
+                    return true;
+                }
+            }
+            for (uint32_t j = 0; j < g.ns; ++j) {
+                if (!(S.segFlags[j] & kSegPolygon) || !filter_accepts(cat, mask, S.segCat[j], S.segMask[j])) continue;
+                const float4 ab = S.segAB[j];
+                const float2 a = f2(ab.x, ab.y), edge = sub(f2(ab.z, ab.w), a);
+                const float lengthSquared = dot2(edge, edge);
+                if (!(lengthSquared > 0.0f)) continue;
+                const float2 outward = mul(dvs(f2(-edge.y, edge.x), __builtin_sqrtf(lengthSquared)), S.segOutward[j]);
+                const float before = dot2(sub(start, a), outward), after = dot2(sub(center, a), outward);
+                if (!(before > 0.0f) || !(after <= 0.0f)) continue;
+                const float2 crossing = add(start, mul(sub(center, start), before / (before - after)));
+                const float along = dot2(sub(crossing, a), edge) / lengthSquared;
+                if (along < 0.0f || along > 1.0f) continue;
+                out.kind = BP_VIOLATION_EDGE_SIDE; out.segment = g.segId(j); out.other = (uint64_t)g.segOwner(j); out.detail = after;
+                return true;
+            }
+            return false;
+        };
+
         // Contact assertion (Force:865-918): lowest violating bubble writes the record.
         auto assert_contacts = [&]() __attribute__((always_inline)) -> bool {
             if (!(cellWidth > 0.0f)) cellWidth = 1.0f;
             __syncthreads();
             build_hash();
             const float fraction = step_fraction(c);
+            refresh_frames(fraction);
             uint32_t firstIndex = 0xffffffffu;
             Violation mine = {0u, 0u, 0, 0.0f};
             const uint32_t *regionG = A->region[ws] + o;
@@ -592,9 +627,9 @@ __global__ __launch_bounds__(BLOCK) void world_lds_kernel(const Arena *__restric
                             const uint32_t j = S.grid[k];
                             if (j <= i || j >= bestJ) continue;
                             const float2 q = S.pos[j];
-                            if (cell_coord(q.x, cellWidth) != tx || cell_coord(q.y, cellWidth) != ty) continue;
                             const float dd = len2(sub(q, center));
                             if (!(dd <= 0.0f)) continue;
+                            if (cell_coord(q.x, cellWidth) != tx || cell_coord(q.y, cellWidth) != ty) continue;
                             if (regionG[j] == BP_REGION_REMOVED || !filter_accepts(catG[i], maskG[i], catG[j], maskG[j])) continue;
                             if (!movable && !(S.invm[j] > 0.0f)) continue;
                             bestJ = j; bestD = dd;
@@ -605,7 +640,7 @@ __global__ __launch_bounds__(BLOCK) void world_lds_kernel(const Arena *__restric
                     continue;
                 }
                 if (!movable) continue;
-                if (assert_body_and_edges(g, center, startG[i], catG[i], maskG[i], fraction, mine)) firstIndex = i;
+                if (assert_staged(center, startG[i], catG[i], maskG[i], mine)) firstIndex = i;
             }
             const uint32_t winner = block_min_u32<BLOCK>(S.red, firstIndex);
             if (winner == 0xffffffffu) return false;
@@ -620,6 +655,7 @@ __global__ __launch_bounds__(BLOCK) void world_lds_kernel(const Arena *__restric
         };
 
         // ---- 2-3. Grid build and the substep loop (Force:1257-1304).
+        const bool injection = d->injectionEnabled != 0, assertEvery = d->assertEverySubstep != 0;
         if (n > 0) {
             bool ok = build();
             if (!ok) status = BP_ERR_CAPACITY_OVERFLOW;
@@ -635,7 +671,7 @@ __global__ __launch_bounds__(BLOCK) void world_lds_kernel(const Arena *__restric
                     } else {
                         const float stiffnessBound = w2Max > 0.0f ? 1.0f / __builtin_sqrtf(w2Max) : inff();
                         pass_distance(fminf(remaining, stiffnessBound));
-                        reduce_h(c, stiffnessBound, w2Own, dbMin, dbOwn, mbMin, mbOwn, ids, history, lane);
+                        reduce_h(c, stiffnessBound, w2Own, dbMin, dbOwn, mbMin, mbOwn, history, lane);
                     }
                     const bool finished = !(c.t0 - c.t1 > 0.0f);
                     if (!(c.h > 0.0f)) {
@@ -650,7 +686,7 @@ __global__ __launch_bounds__(BLOCK) void world_lds_kernel(const Arena *__restric
                     pass_drift();
                     __syncthreads();
                     pass_forces(1u, true);
-                    if (d->injectionEnabled && c.c1 == d->injectionAfter + 1) {
+                    if (injection && c.c1 == d->injectionAfter + 1) {
     #pragma unroll
                         for (int s = 0; s < PER; ++s) {
                             const uint32_t i = lane + s * BLOCK;
@@ -662,7 +698,7 @@ __global__ __launch_bounds__(BLOCK) void world_lds_kernel(const Arena *__restric
                         __syncthreads();
                         pass_forces(1u, false);
                     }
-                    if (d->assertEverySubstep && !claimed) {
+                    if (assertEvery && !claimed) {
                         if (assert_contacts()) claimed = true;
                         __syncthreads();
                     }
@@ -744,7 +780,7 @@ __global__ __launch_bounds__(BLOCK) void world_lds_kernel(const Arena *__restric
             st->removals = S.red.counters[6];
             st->wraps = S.red.counters[7];
             st->time[0] = c.t0; st->time[1] = c.t1; st->time[2] = c.h; st->time[3] = c.split;
-            st->limiterObject = c.limObj;
+            st->limiterObject = limiter_object(c, ids);
             st->limiterTerm = c.limTerm;
             if (!claimed) {
                 st->violationKind = 0; st->violationSegment = 0; st->violationObjects[0] = 0; st->violationObjects[1] = 0;
@@ -753,6 +789,7 @@ __global__ __launch_bounds__(BLOCK) void world_lds_kernel(const Arena *__restric
             st->adjacencyEntries = entriesTotal;
             st->groupWidth = BLOCK;
             st->executed = 1;
+            st->ticks = (uint32_t)(wall_clock64() - ticks0);
             if (status == BP_OK) d->pub = ws;
             else if (status != BP_ERR_CAPACITY_OVERFLOW) d->failed = 1;
         }

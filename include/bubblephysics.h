@@ -240,7 +240,7 @@ typedef struct bp_frame_info {
     uint32_t limiter_term;           /* substepControl.limiter.y                        */
     uint32_t adjacency_entries;      /* entries of the adjacency of the last grid build */
     uint32_t kernel_class;           /* kernel configuration that ran the step (bp_batch classes) */
-    uint32_t reserved;
+    uint32_t device_ticks;           /* device wall-clock ticks (100 MHz) of the world's step, batched kernels */
     bp_failure_detail detail;
 } bp_frame_info;
 
