@@ -109,14 +109,18 @@ def run_large(steps: int):
 
 def run_served(args):
     """Serving: Poisson step requests (10% with transactions) over resident worlds, continuous batching."""
-    from bubblephysics_amd import serving
+    from bubblephysics_amd import serving, solver
     t = time.perf_counter()
     worlds = serving.serving_worlds(args.serve_worlds, seed=20261206)
     print(f"[bench] built {len(worlds)} serving worlds in {time.perf_counter() - t:.1f}s", file=sys.stderr, flush=True)
-    server = serving.Server(worlds, memory_budget=int(200e9))
-    server.batch.step(1)                                   # packs and warms every kernel class
+    batch = solver.Batch(worlds)
+    batch.step(1)                                          # packs and warms every kernel class
+    server = serving.Server(batch, memory_budget=int(200e9), streams=args.serve_streams,
+                            max_inflight_per_class=args.serve_inflight)
     requests = serving.poisson_requests(len(worlds), args.serve_rate, args.serve_seconds, seed=20261207)
+    print(f"[bench] serving {len(requests)} requests", file=sys.stderr, flush=True)
     stats = server.run(requests).summary()
+    server.close()
     stats["bubbles"] = int(sum(len(w.bubbles) for w in worlds))
     stats["worlds"] = len(worlds)
     return stats
@@ -224,6 +228,8 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=30.0)
     ap.add_argument("--serve-rate", type=float, default=20000.0)
     ap.add_argument("--serve-worlds", type=int, default=8192)
+    ap.add_argument("--serve-streams", type=int, default=8)
+    ap.add_argument("--serve-inflight", type=int, default=2)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=min(os.cpu_count() or 1, 16))
     ap.add_argument("--no-cpu", action="store_true")

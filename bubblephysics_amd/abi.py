@@ -107,7 +107,7 @@ class WorldConfig(ctypes.Structure):
         ("grid_bucket_count", ctypes.c_uint32), ("assert_every_substep", ctypes.c_uint32),
         ("injection_enabled", ctypes.c_uint32), ("injection_after_substep", ctypes.c_uint32),
         ("injection_object", ctypes.c_int64), ("injection_position", ctypes.c_float * 2),
-        ("force_cooperative", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
+        ("force_cooperative", ctypes.c_uint32), ("export_adjacency", ctypes.c_uint32),
     ]
 
 
@@ -144,6 +144,55 @@ class BatchWorldDesc(ctypes.Structure):
     ]
 
 
+class TuneBin(ctypes.Structure):
+    _fields_ = [
+        ("min_bubbles", ctypes.c_uint32), ("max_bubbles", ctypes.c_uint32), ("worlds", ctypes.c_uint32),
+        ("best_class", ctypes.c_int32), ("best_bucket_shift", ctypes.c_int32), ("best_rate", ctypes.c_float),
+        ("class_rate", ctypes.c_float * 12), ("bucket_rate", ctypes.c_float * 2),
+    ]
+
+
+class Audit(ctypes.Structure):
+    _fields_ = [
+        ("momentum", ctypes.c_double * 2), ("angular_momentum", ctypes.c_double), ("kinetic_energy", ctypes.c_double),
+        ("contact_energy", ctypes.c_double), ("contacts", ctypes.c_uint32), ("bubbles", ctypes.c_uint32),
+    ]
+
+
+# Serving (bp_server).
+BP_TX_NONE, BP_TX_IMPULSE, BP_TX_ADD, BP_TX_REMOVE, BP_TX_MERGE, BP_TX_SPLIT = 0, 1, 2, 3, 4, 5
+
+
+class Request(ctypes.Structure):
+    _fields_ = [
+        ("arrival", ctypes.c_double), ("tag", ctypes.c_uint64), ("world", ctypes.c_uint32),
+        ("transaction", ctypes.c_uint32), ("index", ctypes.c_uint32), ("impulse", ctypes.c_float * 2),
+        ("direction", ctypes.c_float * 2), ("reserved", ctypes.c_uint32),
+    ]
+
+
+class ServerResult(ctypes.Structure):
+    _fields_ = [
+        ("tag", ctypes.c_uint64), ("world", ctypes.c_uint32), ("status", ctypes.c_int32), ("substeps", ctypes.c_uint32),
+        ("bubbles", ctypes.c_uint32), ("arrival", ctypes.c_double), ("completed", ctypes.c_double),
+        ("launch_worlds", ctypes.c_uint32), ("kernel_class", ctypes.c_uint32), ("device_milliseconds", ctypes.c_float),
+        ("reserved", ctypes.c_uint32),
+    ]
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [
+        ("memory_budget", ctypes.c_uint64), ("streams", ctypes.c_uint32), ("max_inflight_per_class", ctypes.c_uint32),
+        ("max_worlds_per_launch", ctypes.c_uint32), ("big_world", ctypes.c_uint32),
+    ]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_uint64) for name in (
+        "submitted", "completed", "launches", "launched_worlds", "transactions", "rejected", "evictions", "admissions",
+        "repacks", "resident_bytes")]
+
+
 # numpy views of the record arrays (same layout as the C structures).
 BUBBLE_DTYPE = np.dtype(Bubble)
 SEGMENT_DTYPE = np.dtype(Segment)
@@ -152,6 +201,10 @@ MOTOR_DTYPE = np.dtype(Motor)
 BAND_DTYPE = np.dtype(Band)
 FIELD_DTYPE = np.dtype(Field)
 FRAME_DTYPE = np.dtype(FrameInfo)
+AUDIT_DTYPE = np.dtype(Audit)
+TUNE_DTYPE = np.dtype(TuneBin)
+REQUEST_DTYPE = np.dtype(Request)
+RESULT_DTYPE = np.dtype(ServerResult)
 
 # Every exported function of the C ABI with its argument and result types.
 _P = ctypes.c_void_p
@@ -178,6 +231,8 @@ SIGNATURES = {
     "bp_world_read_adjacency": ([_P, _P, _U32, _P, _U32, _P], _I32),
     "bp_world_read_step_history": ([_P, _P, _U32, _P], _I32),
     "bp_world_debug_fail_next_command": ([_P], _I32),
+    "bp_world_audit": ([_P, _P], _I32),
+    "bp_batch_audit": ([_P, _P, _U32], _I32),
     "bp_static_polygon_edges": ([_P, _U32, _I64, _I64, _U32, _P, _U32, _P], _I32),
     "bp_polygon_body_geometry": ([_P, _U32, _P, _P, _P, _P, _P, _P], _I32),
     "bp_place_discs": ([_P, _U32, _P, _F, _F, _P, _U32, _U64, _U32, _P, _P], _I32),
@@ -185,6 +240,7 @@ SIGNATURES = {
     "bp_batch_destroy": ([_P], None),
     "bp_batch_add_world": ([_P, _P, _P], _I32),
     "bp_batch_world_count": ([_P], _U32),
+    "bp_batch_bubble_count": ([_P, _U32], _U32),
     "bp_batch_step": ([_P, _U32, _P], _I32),
     "bp_batch_read_bubbles": ([_P, _U32, _P, _U32], _I32),
     "bp_batch_last_gpu_milliseconds": ([_P], _F),
@@ -197,6 +253,21 @@ SIGNATURES = {
     "bp_batch_set_resident": ([_P, _U32, _U32], _I32),
     "bp_batch_world_footprint": ([_P, _U32, _P], _I32),
     "bp_batch_set_world_class": ([_P, _U32, _I32], _I32),
+    "bp_batch_capture": ([_P, _U32], _I32),
+    "bp_batch_replay": ([_P, _P], _I32),
+    "bp_batch_autotune": ([_P, _U32, _P, _U32, _P], _I32),
+    "bp_batch_apply_tuning": ([_P, _P, _U32], _I32),
+    "bp_server_record_sizes": ([_P], None),
+    "bp_server_config_default": ([_P], None),
+    "bp_server_create": ([_P, _P, _P], _I32),
+    "bp_server_destroy": ([_P], None),
+    "bp_server_submit": ([_P, _P], _I32),
+    "bp_server_poll": ([_P, _P, _U32, _P], _I32),
+    "bp_server_wait_idle": ([_P], _I32),
+    "bp_server_run_trace": ([_P, _P, _U32, _P], _I32),
+    "bp_server_read_bubbles": ([_P, _U32, _P, _U32, _P], _I32),
+    "bp_server_get_stats": ([_P, _P], _I32),
+    "bp_server_clock": ([_P], ctypes.c_double),
 }
 
 

@@ -8,217 +8,10 @@
 // Publication follows NativeMetalSolver.executeSnapshot (NativeMetalSolver.swift:1179-1201):
 // the kernel writes the step into the unpublished side of the state and flips the published
 // side only when the step succeeded, so a failed step leaves the last published state intact.
-#include <hip/hip_runtime.h>
+#include "bp_batch.h"
 
-#include <algorithm>
-#include <cmath>
-#include <cstdio>
-#include <cstring>
-#include <mutex>
-#include <string>
-#include <vector>
+using namespace bph;
 
-#include "../../include/bubblephysics.h"
-#include "bp_device.h"
-
-extern "C" hipError_t bp_launch_world_step(int block, const bpd::Arena *arena, bpd::WorldDesc *descs, bpd::WorldStatus *stats,
-                                           const uint32_t *order, uint32_t count, hipStream_t stream);
-extern "C" hipError_t bp_launch_world_lds(int id, const bpd::Arena *arena, bpd::WorldDesc *descs, bpd::WorldStatus *stats,
-                                          const uint32_t *order, uint32_t count, uint32_t steps, uint32_t statsStride,
-                                          hipStream_t stream);
-extern "C" int bp_lds_config_count();
-extern "C" void bp_lds_config(int id, uint32_t *threads, uint32_t *per, uint32_t *rowcap, uint32_t *caps);
-extern "C" uint32_t bp_coop_max_groups();
-extern "C" hipError_t bp_launch_gather_world(const bpd::Arena *arena, const bpd::WorldDesc *desc, bp_bubble *out, uint32_t count,
-                                             hipStream_t stream);
-extern "C" hipError_t bp_launch_scatter_world(const bpd::Arena *arena, uint32_t offset, uint32_t count, const bp_bubble *in,
-                                              const int32_t *motorIdx, const int32_t *bandIdx, hipStream_t stream);
-extern "C" uint32_t bp_coop_block();
-extern "C" hipError_t bp_launch_world_coop(const bpd::Arena *arena, bpd::WorldDesc *descs, bpd::WorldStatus *stats,
-                                           const uint32_t *orderEntry, uint32_t world, uint32_t groups, hipStream_t stream);
-
-namespace {
-
-thread_local std::string g_lastError;
-
-void set_error(const std::string &text) { g_lastError = text; }
-
-#define BP_HIP(expr)                                                                                   \
-    do {                                                                                               \
-        hipError_t bp_e = (expr);                                                                      \
-        if (bp_e != hipSuccess) {                                                                      \
-            set_error(std::string(#expr) + ": " + hipGetErrorString(bp_e));                           \
-            return BP_ERR_COMMAND_FAILED;                                                              \
-        }                                                                                              \
-    } while (0)
-
-constexpr uint32_t kAdjPerBubble = 64;       // CSR capacity per bubble (grows on overflow)
-constexpr int kClassCount = 12;              // ten LDS configurations, the global-state kernel, the cooperative kernel
-constexpr int kGlobalClass = 10;
-constexpr int kCoopClass = 11;
-constexpr uint32_t kLdsLargest = 4096;       // bubbles of the widest LDS configuration
-constexpr int kStreams = 4;
-
-uint32_t class_threads(int k) {
-    if (k == kGlobalClass) return 1024;
-    if (k == kCoopClass) return bp_coop_block();
-    uint32_t t, p, r;
-    bp_lds_config(k, &t, &p, &r, nullptr);
-    return t;
-}
-
-uint32_t class_capacity(int k) {
-    if (k == kGlobalClass || k == kCoopClass) return 0xffffffffu;
-    uint32_t t, p, r;
-    bp_lds_config(k, &t, &p, &r, nullptr);
-    return t * p;
-}
-
-// Whether a world of n bubbles with the given geometry fits configuration k (the global-state
-// kernel takes every world).
-bool class_fits(int k, size_t n, size_t segments, size_t bodies, size_t vertices, size_t triangles) {
-    if (k == kGlobalClass || k == kCoopClass) return true;
-    uint32_t t, p, r, caps[4];
-    bp_lds_config(k, &t, &p, &r, caps);
-    return n <= (size_t)t * p && segments <= caps[0] && bodies <= caps[1] && vertices <= caps[2] && triangles <= caps[3];
-}
-
-uint32_t auto_buckets(uint32_t n) {
-    uint32_t b = 16;
-    while (b < n && b < bpd::kMaxBuckets) b <<= 1;
-    return b;
-}
-
-struct HostWorld {
-    bp_world_config config{};
-    std::vector<bp_bubble> bubbles;
-    std::vector<bp_segment> segments;
-    std::vector<bp_body> bodies;
-    std::vector<float> vertices;        // float2 pairs
-    std::vector<uint32_t> triangles;    // 4 per triangle
-    std::vector<bp_motor> motors;
-    std::vector<bp_band> bands;
-    std::vector<bp_field> fields;
-    std::vector<float> fieldVertices;   // float2 pairs
-    std::vector<int64_t> fieldIds;      // filter ids, ascending within each field's range
-    uint32_t adjPerBubble = kAdjPerBubble;
-    int forcedClass = -1;               // kernel configuration chosen by the tuner, -1 = default
-    uint32_t exportAdjacency = 0;
-    bool resident = true;               // false: held in host memory only (serving eviction)
-    uint32_t capacity = 0;              // bubble slots of its device range (>= bubbles), set by pack
-    int packedClass = -1;               // kernel configuration of the last pack (-1 = not on the device)
-};
-
-bool class_fits(int k, const HostWorld &w) {
-    return class_fits(k, w.bubbles.size(), w.segments.size(), w.bodies.size(), w.vertices.size() / 2, w.triangles.size() / 4);
-}
-
-// Default kernel configuration of a world: the cooperative kernel for worlds above the widest
-// LDS configuration (or on request, as the reference's multi-dispatch path), else the narrowest
-// LDS configuration whose bubble and geometry capacities hold it (classes 0-9 are the LDS
-// configurations of bp_world_lds.hip), else the single-workgroup global-state kernel.
-int default_class_for(const HostWorld &w, bool cooperative) {
-    if (cooperative && (w.config.force_cooperative || w.bubbles.size() > kLdsLargest)) return kCoopClass;
-    static const int kBySize[] = {0, 1, 2, 3, 4, 7, 9};
-    for (int k : kBySize)
-        if (class_fits(k, w)) return k;
-    return kGlobalClass;
-}
-
-uint32_t next_pow2(uint32_t v, uint32_t lowest) {
-    uint32_t p = lowest;
-    while (p < v && p < (1u << 24)) p <<= 1;
-    return p;
-}
-
-template <class T> struct DevBuf {
-    T *ptr = nullptr;
-    size_t count = 0;
-};
-
-}  // namespace
-
-struct bp_batch {
-    std::vector<HostWorld> worlds;
-    bool packed = false;
-    bool deviceValid = false;          // device state holds the published state of every world
-    std::vector<void *> allocations;
-    bpd::Arena arena{};
-    bpd::Arena *arenaDev = nullptr;
-    bpd::WorldDesc *descs = nullptr;
-    std::vector<bpd::WorldDesc> hostDescs;
-    uint32_t *orders[kClassCount] = {};
-    std::vector<uint32_t> classWorlds[kClassCount];
-    std::vector<uint32_t> coopGroups;   // per world: workgroups of its cooperative launch (0 = not cooperative)
-    hipStream_t streams[kStreams] = {};
-    hipEvent_t startEvent = nullptr, endEvents[kStreams] = {};
-    hipEvent_t classStart[kClassCount] = {}, classEnd[kClassCount] = {};
-    bpd::WorldStatus *stats = nullptr;
-    size_t statsCapacity = 0;          // in WorldStatus records
-    std::vector<bpd::WorldStatus> hostStats;
-    float lastGpuMs = 0.0f;
-    float lastClassMs[kClassCount] = {};
-    bool inFlight = false;
-    bool pending = false;              // launches queued by launch() and not yet collected
-    uint32_t pendingSteps = 0;
-    std::vector<uint32_t> pendingList; // worlds of the pending launches, in the caller's order
-    std::vector<uint32_t> subsetHost;  // per class, the listed worlds (longest first)
-    uint32_t *subsetOrders = nullptr;  // device copy of subsetHost
-    bp_bubble *xferBubbles = nullptr;  // one world's records in transit (transactions)
-    int32_t *xferIndices = nullptr;    // their motor and band indices
-    uint32_t xferCapacity = 0;
-    uint32_t launchOffset[kClassCount] = {}, launchCount[kClassCount] = {};
-    uint32_t totalBubbles = 0;
-    std::mutex mutex;
-
-    ~bp_batch() { release(); }
-
-    void release() {
-        for (void *p : allocations) (void)hipFree(p);
-        allocations.clear();
-        for (int k = 0; k < kStreams; ++k) {
-            if (streams[k]) (void)hipStreamDestroy(streams[k]);
-            if (endEvents[k]) (void)hipEventDestroy(endEvents[k]);
-            streams[k] = nullptr; endEvents[k] = nullptr;
-        }
-        for (int k = 0; k < kClassCount; ++k) {
-            if (classStart[k]) (void)hipEventDestroy(classStart[k]);
-            if (classEnd[k]) (void)hipEventDestroy(classEnd[k]);
-            classStart[k] = classEnd[k] = nullptr;
-        }
-        if (startEvent) (void)hipEventDestroy(startEvent);
-        startEvent = nullptr;
-        stats = nullptr; statsCapacity = 0; descs = nullptr; arenaDev = nullptr; subsetOrders = nullptr;
-        xferBubbles = nullptr; xferIndices = nullptr; xferCapacity = 0;
-        pending = false;
-        for (auto &o : orders) o = nullptr;
-        packed = false;
-    }
-
-    template <class T> bp_status alloc(T *&out, size_t count) {
-        void *p = nullptr;
-        size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
-        hipError_t e = hipMalloc(&p, bytes);
-        if (e != hipSuccess) { set_error(std::string("hipMalloc: ") + hipGetErrorString(e)); return BP_ERR_ALLOCATION_FAILED; }
-        allocations.push_back(p);
-        out = static_cast<T *>(p);
-        return BP_OK;
-    }
-    template <class T> bp_status upload(T *dst, const std::vector<T> &src) {
-        if (src.empty()) return BP_OK;
-        BP_HIP(hipMemcpy(dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
-        return BP_OK;
-    }
-
-    bp_status pack();
-    bp_status download_state();
-    bp_status download_world(uint32_t wi);
-    bp_status write_bubbles(uint32_t wi, const bp_bubble *bubbles, uint32_t count);
-    uint64_t footprint(uint32_t wi) const;
-    bp_status launch(const uint32_t *subset, uint32_t subsetCount, uint32_t steps);
-    bp_status finish(bp_frame_info *infos, bool wait);
-    bp_status step(uint32_t steps, bp_frame_info *infos);
-};
 
 namespace {
 
@@ -319,6 +112,8 @@ bp_status bp_batch::pack() {
     BP_ALLOC(A.motor, nb); BP_ALLOC(A.band, nb);
     BP_ALLOC(A.acc, nb); BP_ALLOC(A.a0, nb); BP_ALLOC(A.ref, nb); BP_ALLOC(A.margin, nb);
     BP_ALLOC(A.rowStart, nb); BP_ALLOC(A.rowLen, nb); BP_ALLOC(A.grid, nb);
+    BP_HIP(hipMemset(A.rowStart, 0, std::max<size_t>(nb, 1) * sizeof(uint32_t)));
+    BP_HIP(hipMemset(A.rowLen, 0, std::max<size_t>(nb, 1) * sizeof(uint32_t)));
     BP_ALLOC(A.omega2, 1); BP_ALLOC(A.bound, 1); BP_ALLOC(A.bodyHits, nbod);
     BP_ALLOC(A.adj, adj);
     BP_ALLOC(A.segAB, ns); BP_ALLOC(A.segRegion, ns); BP_ALLOC(A.segFlags, ns); BP_ALLOC(A.segCat, ns);
@@ -388,7 +183,8 @@ bp_status bp_batch::pack() {
         d.injectionObject = c.injection_object;
         d.injectionPosition[0] = c.injection_position[0]; d.injectionPosition[1] = c.injection_position[1];
         d.failed = 0;
-        d.exportAdjacency = w.exportAdjacency;
+        d.exportAdjacency = (w.exportAdjacency || w.config.export_adjacency) ? 1u : 0u;
+        d.bubbleCapacity = w.capacity;
         const uint32_t resident = w.resident ? n : 0u;       // bubbles staged in the device range
         for (uint32_t i = 0; i < resident; ++i) {
             const bp_bubble &b = w.bubbles[i];
@@ -500,10 +296,26 @@ bp_status bp_batch::pack() {
 }
 
 // Copies the published state of every world back into the host mirror.
-bp_status bp_batch::download_state() {
+bp_status bp_batch::download_state(bool full) {
     if (!packed || !deviceValid) return BP_OK;
     BP_HIP(hipMemcpy(hostDescs.data(), descs, hostDescs.size() * sizeof(bpd::WorldDesc), hipMemcpyDeviceToHost));
     uint32_t nb = totalBubbles;
+    std::vector<int64_t> ids;
+    std::vector<float> mass, invm, radius, Kc, gamma;
+    std::vector<uint32_t> flags, cat, mask;
+    if (full && nb) {
+        ids.resize(nb); mass.resize(nb); invm.resize(nb); radius.resize(nb); Kc.resize(nb); gamma.resize(nb);
+        flags.resize(nb); cat.resize(nb); mask.resize(nb);
+        BP_HIP(hipMemcpy(ids.data(), arena.id, nb * sizeof(int64_t), hipMemcpyDeviceToHost));
+        BP_HIP(hipMemcpy(mass.data(), arena.mass, nb * sizeof(float), hipMemcpyDeviceToHost));
+        BP_HIP(hipMemcpy(invm.data(), arena.invm, nb * sizeof(float), hipMemcpyDeviceToHost));
+        BP_HIP(hipMemcpy(radius.data(), arena.radius, nb * sizeof(float), hipMemcpyDeviceToHost));
+        BP_HIP(hipMemcpy(Kc.data(), arena.Kc, nb * sizeof(float), hipMemcpyDeviceToHost));
+        BP_HIP(hipMemcpy(gamma.data(), arena.gamma, nb * sizeof(float), hipMemcpyDeviceToHost));
+        BP_HIP(hipMemcpy(flags.data(), arena.flags, nb * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        BP_HIP(hipMemcpy(cat.data(), arena.cat, nb * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        BP_HIP(hipMemcpy(mask.data(), arena.mask, nb * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    }
     std::vector<float2> pos[2], start[2], vel[2];
     std::vector<float> rot[2], om[2];
     size_t nbod = 0;
@@ -532,9 +344,14 @@ bp_status bp_batch::download_state() {
         if (!w.resident) continue;                         // its host copy is the state
         const bpd::WorldDesc &d = hostDescs[wi];
         uint32_t p = d.pub;
+        if (full) w.bubbles.resize(std::min(d.bubbleCount, w.capacity));
         for (uint32_t i = 0; i < w.bubbles.size(); ++i) {
             uint32_t g = d.bubbleOffset + i;
             bp_bubble &b = w.bubbles[i];
+            if (full) {
+                b.id = ids[g]; b.mass = mass[g]; b.inverse_mass = invm[g]; b.radius = radius[g];
+                b.contact_stiffness = Kc[g]; b.damping = gamma[g]; b.flags = flags[g]; b.category = cat[g]; b.mask = mask[g];
+            }
             b.center[0] = pos[p][g].x; b.center[1] = pos[p][g].y;
             b.previous_center[0] = start[p][g].x; b.previous_center[1] = start[p][g].y;
             b.velocity[0] = vel[p][g].x; b.velocity[1] = vel[p][g].y;
@@ -663,6 +480,7 @@ bp_status bp_batch::write_bubbles(uint32_t wi, const bp_bubble *bubbles, uint32_
                       return hostDescs[a].bubbleCount > hostDescs[b].bubbleCount;
                   }), wi);
         w.packedClass = cls;
+        release_graph();                              // its launches were captured with the old class lists
         bp_status s2 = BP_OK;
         if ((s2 = upload(orders[old], from)) != BP_OK || (s2 = upload(orders[cls], to)) != BP_OK) return s2;
     }
@@ -799,15 +617,21 @@ bp_status bp_batch::finish(bp_frame_info *infos, bool wait) {
         if (!launchCount[k]) continue;
         BP_HIP(hipEventElapsedTime(&lastClassMs[k], classStart[k], classEnd[k]));
     }
+    return collect(steps, infos, pendingList, ms);
+}
+
+// Result records of the last `steps` steps into infos[step * listed + j] for the j-th listed world.
+bp_status bp_batch::collect(uint32_t steps, bp_frame_info *infos, const std::vector<uint32_t> &listedWorlds, float ms) {
+    const size_t W = worlds.size();
     lastGpuMs = ms;
     const size_t need = (size_t)steps * W;
     hostStats.resize(need);
     BP_HIP(hipMemcpy(hostStats.data(), stats, need * sizeof(bpd::WorldStatus), hipMemcpyDeviceToHost));
     if (!infos) return BP_OK;
-    const size_t listed = pendingList.size();
+    const size_t listed = listedWorlds.size();
     for (uint32_t st = 0; st < steps; ++st)
         for (size_t j = 0; j < listed; ++j) {
-            const uint32_t wi = pendingList[j];
+            const uint32_t wi = listedWorlds[j];
             bp_frame_info &out = infos[(size_t)st * listed + j];
             fill_info(hostStats[(size_t)st * W + wi], &out);
             const int cls = worlds[wi].packedClass;
@@ -820,10 +644,114 @@ bp_status bp_batch::finish(bp_frame_info *infos, bool wait) {
     return BP_OK;
 }
 
+// FP64 audit of every world's published state (audit_kernel, bp_audit.hip).
+bp_status bp_batch::audit(bp_audit *out, uint32_t count) {
+    if (count != worlds.size() || (count && !out)) return BP_ERR_INVALID_ARGUMENT;
+    if (count == 0) return BP_OK;
+    if (!packed) {
+        bp_status s = pack();
+        if (s != BP_OK) return s;
+    }
+    void *dev = nullptr;
+    BP_HIP(hipMalloc(&dev, count * sizeof(bp_audit)));
+    hipError_t e = bp_launch_audit(arenaDev, descs, count, dev, streams[0]);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, dev, count * sizeof(bp_audit), hipMemcpyDeviceToHost, streams[0]);
+    if (e == hipSuccess) e = hipStreamSynchronize(streams[0]);
+    (void)hipFree(dev);
+    if (e != hipSuccess) { set_error(std::string("audit: ") + hipGetErrorString(e)); return BP_ERR_COMMAND_FAILED; }
+    for (uint32_t wi = 0; wi < count; ++wi)
+        if (!worlds[wi].resident) {                    // held in host memory: nothing on the device to audit
+            const double nan = std::nan("");
+            out[wi] = bp_audit{{nan, nan}, nan, nan, nan, 0u, 0u};
+        }
+    return BP_OK;
+}
+
 bp_status bp_batch::step(uint32_t steps, bp_frame_info *infos) {
     bp_status s = launch(nullptr, 0, steps);
     if (s != BP_OK) return s;
     return finish(infos, true);
+}
+
+// Captures `steps` steps of every resident world into a hipGraph: a fill kernel clears the
+// result records (no memset node), the capture forks from the first stream to the others, each
+// kernel class is launched on its stream as launch() does, and the streams join back.
+bp_status bp_batch::capture(uint32_t steps) {
+    if (steps == 0) return BP_ERR_INVALID_ARGUMENT;
+    if (!packed) {
+        bp_status s = pack();
+        if (s != BP_OK) return s;
+    }
+    release_graph();
+    const size_t W = worlds.size();
+    const size_t need = (size_t)steps * W;
+    if (statsCapacity < need) {                       // allocations happen before the capture
+        bp_status s = alloc(stats, need);
+        if (s != BP_OK) return s;
+        statsCapacity = need;
+    }
+    if (!forkEvent) {
+        BP_HIP(hipEventCreateWithFlags(&forkEvent, hipEventDisableTiming));
+        for (auto &e : joinEvents) BP_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+    hipStream_t origin = streams[0];
+    BP_HIP(hipStreamBeginCapture(origin, hipStreamCaptureModeThreadLocal));
+    hipError_t e = bp_launch_fill_u32(reinterpret_cast<uint32_t *>(stats), need * sizeof(bpd::WorldStatus) / 4, 0u, origin);
+    if (e == hipSuccess) e = hipEventRecord(forkEvent, origin);
+    for (int k = 1; k < kStreams && e == hipSuccess; ++k) e = hipStreamWaitEvent(streams[k], forkEvent, 0);
+    for (int k = kClassCount - 1; k >= 0 && e == hipSuccess; --k) {
+        const uint32_t count = (uint32_t)classWorlds[k].size();
+        if (!count || k == kCoopClass) continue;
+        hipStream_t stream = streams[k % kStreams];
+        if (k == kGlobalClass) {
+            for (uint32_t st = 0; st < steps && e == hipSuccess; ++st)
+                e = bp_launch_world_step(1024, arenaDev, descs, stats + (size_t)st * W, orders[k], count, stream);
+        } else {
+            e = bp_launch_world_lds(k, arenaDev, descs, stats, orders[k], count, steps, (uint32_t)W, stream);
+        }
+    }
+    for (int k = 1; k < kStreams && e == hipSuccess; ++k) {
+        e = hipEventRecord(joinEvents[k], streams[k]);
+        if (e == hipSuccess) e = hipStreamWaitEvent(origin, joinEvents[k], 0);
+    }
+    if (!classWorlds[kCoopClass].empty())
+        for (uint32_t st = 0; st < steps && e == hipSuccess; ++st)
+            for (uint32_t j = 0; j < classWorlds[kCoopClass].size() && e == hipSuccess; ++j) {
+                const uint32_t wi = classWorlds[kCoopClass][j];
+                e = bp_launch_world_coop(arenaDev, descs, stats + (size_t)st * W, orders[kCoopClass] + j, wi, coopGroups[wi], origin);
+            }
+    hipGraph_t g = nullptr;
+    hipError_t end = hipStreamEndCapture(origin, &g);
+    if (e != hipSuccess || end != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        set_error(std::string("graph capture: ") + hipGetErrorString(e != hipSuccess ? e : end));
+        return BP_ERR_COMMAND_SETUP_FAILED;
+    }
+    graph = g;
+    hipError_t ie = hipGraphInstantiate(&graphExec, graph, nullptr, nullptr, 0);
+    if (ie != hipSuccess) {
+        release_graph();
+        set_error(std::string("graph instantiate: ") + hipGetErrorString(ie));
+        return BP_ERR_COMMAND_SETUP_FAILED;
+    }
+    graphSteps = steps;
+    return BP_OK;
+}
+
+// One launch of the captured graph, then the result records of its steps for every world.
+bp_status bp_batch::replay(bp_frame_info *infos) {
+    if (!graphExec || !packed) { set_error("no captured graph (capture again after the batch changed)"); return BP_ERR_INVALID_ARGUMENT; }
+    hipStream_t origin = streams[0];
+    BP_HIP(hipEventRecord(startEvent, origin));
+    BP_HIP(hipGraphLaunch(graphExec, origin));
+    BP_HIP(hipEventRecord(endEvents[0], origin));
+    BP_HIP(hipEventSynchronize(endEvents[0]));
+    float ms = 0.0f;
+    BP_HIP(hipEventElapsedTime(&ms, startEvent, endEvents[0]));
+    for (int k = 0; k < kClassCount; ++k) lastClassMs[k] = 0.0f;
+    std::vector<uint32_t> all(worlds.size());
+    for (uint32_t wi = 0; wi < all.size(); ++wi) all[wi] = wi;
+    return collect(graphSteps, infos, all, ms);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -920,7 +848,7 @@ static bp_status world_from_desc(HostWorld &w, const bp_batch_world_desc *d) {
 bp_status bp_batch_add_world(bp_batch *b, const bp_batch_world_desc *desc, uint32_t *out_index) {
     if (!b) return BP_ERR_INVALID_ARGUMENT;
     std::lock_guard<std::mutex> lock(b->mutex);
-    if (b->inFlight || b->pending) return BP_ERR_FRAME_IN_FLIGHT;
+    if (b->inFlight || b->pending || b->serving) return BP_ERR_FRAME_IN_FLIGHT;
     bp_status s = b->download_state();
     if (s != BP_OK) return s;
     HostWorld w;
@@ -934,10 +862,14 @@ bp_status bp_batch_add_world(bp_batch *b, const bp_batch_world_desc *desc, uint3
 
 uint32_t bp_batch_world_count(const bp_batch *b) { return b ? (uint32_t)b->worlds.size() : 0u; }
 
+uint32_t bp_batch_bubble_count(const bp_batch *b, uint32_t world) {
+    return b && world < b->worlds.size() ? (uint32_t)b->worlds[world].bubbles.size() : 0u;
+}
+
 bp_status bp_batch_step(bp_batch *b, uint32_t steps, bp_frame_info *infos) {
     if (!b) return BP_ERR_INVALID_ARGUMENT;
     std::unique_lock<std::mutex> lock(b->mutex, std::try_to_lock);
-    if (!lock.owns_lock() || b->inFlight || b->pending) return BP_ERR_FRAME_IN_FLIGHT;
+    if (!lock.owns_lock() || b->inFlight || b->pending || b->serving) return BP_ERR_FRAME_IN_FLIGHT;
     b->inFlight = true;
     bp_status s = b->step(steps, infos);
     b->inFlight = false;
@@ -947,7 +879,7 @@ bp_status bp_batch_step(bp_batch *b, uint32_t steps, bp_frame_info *infos) {
 bp_status bp_batch_step_worlds(bp_batch *b, const uint32_t *worlds, uint32_t count, uint32_t steps, bp_frame_info *infos) {
     if (!b || (count && !worlds)) return BP_ERR_INVALID_ARGUMENT;
     std::unique_lock<std::mutex> lock(b->mutex, std::try_to_lock);
-    if (!lock.owns_lock() || b->inFlight || b->pending) return BP_ERR_FRAME_IN_FLIGHT;
+    if (!lock.owns_lock() || b->inFlight || b->pending || b->serving) return BP_ERR_FRAME_IN_FLIGHT;
     b->inFlight = true;
     bp_status s = b->launch(worlds, count, steps);
     if (s == BP_OK) s = b->finish(infos, true);
@@ -958,14 +890,14 @@ bp_status bp_batch_step_worlds(bp_batch *b, const uint32_t *worlds, uint32_t cou
 bp_status bp_batch_submit_worlds(bp_batch *b, const uint32_t *worlds, uint32_t count, uint32_t steps) {
     if (!b || (count && !worlds)) return BP_ERR_INVALID_ARGUMENT;
     std::unique_lock<std::mutex> lock(b->mutex, std::try_to_lock);
-    if (!lock.owns_lock() || b->inFlight || b->pending) return BP_ERR_FRAME_IN_FLIGHT;
+    if (!lock.owns_lock() || b->inFlight || b->pending || b->serving) return BP_ERR_FRAME_IN_FLIGHT;
     return b->launch(worlds, count, steps);
 }
 
 bp_status bp_batch_poll(bp_batch *b, bp_frame_info *infos) {
     if (!b) return BP_ERR_INVALID_ARGUMENT;
     std::unique_lock<std::mutex> lock(b->mutex, std::try_to_lock);
-    if (!lock.owns_lock() || b->inFlight) return BP_ERR_FRAME_IN_FLIGHT;
+    if (!lock.owns_lock() || b->inFlight || b->serving) return BP_ERR_FRAME_IN_FLIGHT;
     if (!b->pending) { set_error("nothing was submitted"); return BP_ERR_INVALID_ARGUMENT; }
     return b->finish(infos, false);
 }
@@ -973,14 +905,14 @@ bp_status bp_batch_poll(bp_batch *b, bp_frame_info *infos) {
 bp_status bp_batch_write_bubbles(bp_batch *b, uint32_t world, const bp_bubble *bubbles, uint32_t count) {
     if (!b || world >= b->worlds.size() || (count && !bubbles)) return BP_ERR_INVALID_ARGUMENT;
     std::lock_guard<std::mutex> lock(b->mutex);
-    if (b->inFlight || b->pending) return BP_ERR_FRAME_IN_FLIGHT;
+    if (b->inFlight || b->pending || b->serving) return BP_ERR_FRAME_IN_FLIGHT;
     return b->write_bubbles(world, bubbles, count);
 }
 
 bp_status bp_batch_set_resident(bp_batch *b, uint32_t world, uint32_t resident) {
     if (!b || world >= b->worlds.size()) return BP_ERR_INVALID_ARGUMENT;
     std::lock_guard<std::mutex> lock(b->mutex);
-    if (b->inFlight || b->pending) return BP_ERR_FRAME_IN_FLIGHT;
+    if (b->inFlight || b->pending || b->serving) return BP_ERR_FRAME_IN_FLIGHT;
     HostWorld &w = b->worlds[world];
     if (w.resident == (resident != 0)) return BP_OK;
     bp_status s = b->download_state();            // every world's newest state, before the layout changes
@@ -999,7 +931,7 @@ bp_status bp_batch_world_footprint(const bp_batch *b, uint32_t world, uint64_t *
 bp_status bp_batch_read_bubbles(bp_batch *b, uint32_t world, bp_bubble *out, uint32_t count) {
     if (!b || world >= b->worlds.size()) return BP_ERR_INVALID_ARGUMENT;
     std::lock_guard<std::mutex> lock(b->mutex);
-    if (b->pending) return BP_ERR_FRAME_IN_FLIGHT;
+    if (b->pending || b->serving) return BP_ERR_FRAME_IN_FLIGHT;
     bp_status s = b->download_world(world);
     if (s != BP_OK) return s;
     const HostWorld &w = b->worlds[world];
@@ -1023,10 +955,34 @@ bp_status bp_batch_last_class_timing(const bp_batch *b, float *ms, uint32_t *wor
 
 uint32_t bp_batch_class_count(void) { return kClassCount; }
 
+bp_status bp_batch_capture(bp_batch *b, uint32_t steps) {
+    if (!b) return BP_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(b->mutex);
+    if (b->inFlight || b->pending || b->serving) return BP_ERR_FRAME_IN_FLIGHT;
+    return b->capture(steps);
+}
+
+bp_status bp_batch_replay(bp_batch *b, bp_frame_info *infos) {
+    if (!b) return BP_ERR_INVALID_ARGUMENT;
+    std::unique_lock<std::mutex> lock(b->mutex, std::try_to_lock);
+    if (!lock.owns_lock() || b->inFlight || b->pending || b->serving) return BP_ERR_FRAME_IN_FLIGHT;
+    b->inFlight = true;
+    bp_status s = b->replay(infos);
+    b->inFlight = false;
+    return s;
+}
+
+bp_status bp_batch_audit(bp_batch *b, bp_audit *out, uint32_t count) {
+    if (!b) return BP_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(b->mutex);
+    if (b->inFlight || b->pending || b->serving) return BP_ERR_FRAME_IN_FLIGHT;
+    return b->audit(out, count);
+}
+
 bp_status bp_batch_set_world_class(bp_batch *b, uint32_t world, int32_t cls) {
     if (!b || world >= b->worlds.size() || cls < -1 || cls >= kClassCount) return BP_ERR_INVALID_ARGUMENT;
     std::lock_guard<std::mutex> lock(b->mutex);
-    if (b->inFlight || b->pending) return BP_ERR_FRAME_IN_FLIGHT;
+    if (b->inFlight || b->pending || b->serving) return BP_ERR_FRAME_IN_FLIGHT;
     bp_status s = b->download_state();
     if (s != BP_OK) return s;
     if (cls >= 0 && !class_fits(cls, b->worlds[world])) return BP_ERR_INVALID_ARGUMENT;
@@ -1249,6 +1205,12 @@ bp_status bp_world_read_step_history(bp_world *w, float *out, uint32_t capacity,
     BP_HIP(hipMemcpy(out, w->batch.arena.history + d.historyOffset, count * sizeof(float), hipMemcpyDeviceToHost));
     if (out_count) *out_count = w->batch.hostStats.empty() ? 0u : std::min(w->batch.hostStats[0].substeps, count);
     return BP_OK;
+}
+
+bp_status bp_world_audit(bp_world *w, bp_audit *out) {
+    if (!w || !out) return BP_ERR_INVALID_ARGUMENT;
+    if (w->inFlight) return BP_ERR_FRAME_IN_FLIGHT;
+    return w->batch.audit(out, 1);
 }
 
 bp_status bp_world_debug_fail_next_command(bp_world *w) {

@@ -1,0 +1,241 @@
+// bp_batch.h -- internal: the batch of worlds that every host entry point drives (the arena,
+// the kernel classes, the streams) and the helpers shared by bp_api.cpp and bp_server.cpp.
+#ifndef BP_BATCH_H
+#define BP_BATCH_H
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/bubblephysics.h"
+#include "bp_device.h"
+
+extern "C" hipError_t bp_launch_world_step(int block, const bpd::Arena *arena, bpd::WorldDesc *descs, bpd::WorldStatus *stats,
+                                           const uint32_t *order, uint32_t count, hipStream_t stream);
+extern "C" hipError_t bp_launch_world_lds(int id, const bpd::Arena *arena, bpd::WorldDesc *descs, bpd::WorldStatus *stats,
+                                          const uint32_t *order, uint32_t count, uint32_t steps, uint32_t statsStride,
+                                          hipStream_t stream);
+extern "C" int bp_lds_config_count();
+extern "C" void bp_lds_config(int id, uint32_t *threads, uint32_t *per, uint32_t *rowcap, uint32_t *caps);
+extern "C" uint32_t bp_coop_max_groups();
+extern "C" hipError_t bp_launch_gather_world(const bpd::Arena *arena, const bpd::WorldDesc *desc, bp_bubble *out, uint32_t count,
+                                             hipStream_t stream);
+extern "C" hipError_t bp_launch_scatter_world(const bpd::Arena *arena, uint32_t offset, uint32_t count, const bp_bubble *in,
+                                              const int32_t *motorIdx, const int32_t *bandIdx, hipStream_t stream);
+extern "C" uint32_t bp_coop_block();
+extern "C" hipError_t bp_launch_fill_u32(uint32_t *p, size_t words, uint32_t value, hipStream_t stream);
+extern "C" hipError_t bp_launch_audit(const bpd::Arena *arena, const bpd::WorldDesc *descs, uint32_t worlds, void *out,
+                                      hipStream_t stream);
+extern "C" hipError_t bp_launch_world_coop(const bpd::Arena *arena, bpd::WorldDesc *descs, bpd::WorldStatus *stats,
+                                           const uint32_t *orderEntry, uint32_t world, uint32_t groups, hipStream_t stream);
+
+namespace bph {
+
+inline thread_local std::string g_lastError;
+
+inline void set_error(const std::string &text) { g_lastError = text; }
+
+#define BP_HIP(expr)                                                                                   \
+    do {                                                                                               \
+        hipError_t bp_e = (expr);                                                                      \
+        if (bp_e != hipSuccess) {                                                                      \
+            bph::set_error(std::string(#expr) + ": " + hipGetErrorString(bp_e));                         \
+            return BP_ERR_COMMAND_FAILED;                                                              \
+        }                                                                                              \
+    } while (0)
+
+constexpr uint32_t kAdjPerBubble = 64;       // CSR capacity per bubble (grows on overflow)
+constexpr int kClassCount = 12;              // ten LDS configurations, the global-state kernel, the cooperative kernel
+constexpr int kGlobalClass = 10;
+constexpr int kCoopClass = 11;
+constexpr uint32_t kLdsLargest = 4096;       // bubbles of the widest LDS configuration
+constexpr int kStreams = 4;
+
+inline uint32_t class_threads(int k) {
+    if (k == kGlobalClass) return 1024;
+    if (k == kCoopClass) return bp_coop_block();
+    uint32_t t, p, r;
+    bp_lds_config(k, &t, &p, &r, nullptr);
+    return t;
+}
+
+inline uint32_t class_capacity(int k) {
+    if (k == kGlobalClass || k == kCoopClass) return 0xffffffffu;
+    uint32_t t, p, r;
+    bp_lds_config(k, &t, &p, &r, nullptr);
+    return t * p;
+}
+
+// Whether a world of n bubbles with the given geometry fits configuration k (the global-state
+// kernel takes every world).
+inline bool class_fits(int k, size_t n, size_t segments, size_t bodies, size_t vertices, size_t triangles) {
+    if (k == kGlobalClass || k == kCoopClass) return true;
+    uint32_t t, p, r, caps[4];
+    bp_lds_config(k, &t, &p, &r, caps);
+    return n <= (size_t)t * p && segments <= caps[0] && bodies <= caps[1] && vertices <= caps[2] && triangles <= caps[3];
+}
+
+inline uint32_t auto_buckets(uint32_t n) {
+    uint32_t b = 16;
+    while (b < n && b < bpd::kMaxBuckets) b <<= 1;
+    return b;
+}
+
+struct HostWorld {
+    bp_world_config config{};
+    std::vector<bp_bubble> bubbles;
+    std::vector<bp_segment> segments;
+    std::vector<bp_body> bodies;
+    std::vector<float> vertices;        // float2 pairs
+    std::vector<uint32_t> triangles;    // 4 per triangle
+    std::vector<bp_motor> motors;
+    std::vector<bp_band> bands;
+    std::vector<bp_field> fields;
+    std::vector<float> fieldVertices;   // float2 pairs
+    std::vector<int64_t> fieldIds;      // filter ids, ascending within each field's range
+    uint32_t adjPerBubble = kAdjPerBubble;
+    int forcedClass = -1;               // kernel configuration chosen by the tuner, -1 = default
+    uint32_t exportAdjacency = 0;
+    bool resident = true;               // false: held in host memory only (serving eviction)
+    uint32_t capacity = 0;              // bubble slots of its device range (>= bubbles), set by pack
+    int packedClass = -1;               // kernel configuration of the last pack (-1 = not on the device)
+};
+
+inline bool class_fits(int k, const HostWorld &w) {
+    return class_fits(k, w.bubbles.size(), w.segments.size(), w.bodies.size(), w.vertices.size() / 2, w.triangles.size() / 4);
+}
+
+// Default kernel configuration of a world: the cooperative kernel for worlds above the widest
+// LDS configuration (or on request, as the reference's multi-dispatch path), else the narrowest
+// LDS configuration whose bubble and geometry capacities hold it (classes 0-9 are the LDS
+// configurations of bp_world_lds.hip), else the single-workgroup global-state kernel.
+inline int default_class_for(const HostWorld &w, bool cooperative) {
+    if (cooperative && (w.config.force_cooperative || w.bubbles.size() > kLdsLargest)) return kCoopClass;
+    static const int kBySize[] = {0, 1, 2, 3, 4, 7, 9};
+    for (int k : kBySize)
+        if (class_fits(k, w)) return k;
+    return kGlobalClass;
+}
+
+inline uint32_t next_pow2(uint32_t v, uint32_t lowest) {
+    uint32_t p = lowest;
+    while (p < v && p < (1u << 24)) p <<= 1;
+    return p;
+}
+
+template <class T> struct DevBuf {
+    T *ptr = nullptr;
+    size_t count = 0;
+};
+
+}  // namespace bph
+
+struct bp_batch {
+    std::vector<bph::HostWorld> worlds;
+    bool packed = false;
+    bool deviceValid = false;          // device state holds the published state of every world
+    std::vector<void *> allocations;
+    bpd::Arena arena{};
+    bpd::Arena *arenaDev = nullptr;
+    bpd::WorldDesc *descs = nullptr;
+    std::vector<bpd::WorldDesc> hostDescs;
+    uint32_t *orders[bph::kClassCount] = {};
+    std::vector<uint32_t> classWorlds[bph::kClassCount];
+    std::vector<uint32_t> coopGroups;   // per world: workgroups of its cooperative launch (0 = not cooperative)
+    hipStream_t streams[bph::kStreams] = {};
+    hipEvent_t startEvent = nullptr, endEvents[bph::kStreams] = {};
+    hipEvent_t classStart[bph::kClassCount] = {}, classEnd[bph::kClassCount] = {};
+    bpd::WorldStatus *stats = nullptr;
+    size_t statsCapacity = 0;          // in WorldStatus records
+    std::vector<bpd::WorldStatus> hostStats;
+    float lastGpuMs = 0.0f;
+    float lastClassMs[bph::kClassCount] = {};
+    bool inFlight = false;
+    bool serving = false;              // a bp_server drives the batch; its own entry points refuse
+    bool pending = false;              // launches queued by launch() and not yet collected
+    uint32_t pendingSteps = 0;
+    std::vector<uint32_t> pendingList; // worlds of the pending launches, in the caller's order
+    std::vector<uint32_t> subsetHost;  // per class, the listed worlds (longest first)
+    uint32_t *subsetOrders = nullptr;  // device copy of subsetHost
+    bp_bubble *xferBubbles = nullptr;  // one world's records in transit (transactions)
+    int32_t *xferIndices = nullptr;    // their motor and band indices
+    uint32_t xferCapacity = 0;
+    uint32_t launchOffset[bph::kClassCount] = {}, launchCount[bph::kClassCount] = {};
+    uint32_t totalBubbles = 0;
+    std::mutex mutex;
+    // Captured step chain (bp_batch_capture): the launches of `graphSteps` steps of every world.
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graphExec = nullptr;
+    uint32_t graphSteps = 0;
+    hipEvent_t forkEvent = nullptr, joinEvents[bph::kStreams] = {};
+
+    ~bp_batch() { release(); }
+
+    void release_graph() {
+        if (graphExec) (void)hipGraphExecDestroy(graphExec);
+        if (graph) (void)hipGraphDestroy(graph);
+        graphExec = nullptr; graph = nullptr; graphSteps = 0;
+    }
+
+    void release() {
+        release_graph();
+        if (forkEvent) (void)hipEventDestroy(forkEvent);
+        forkEvent = nullptr;
+        for (auto &e : joinEvents) { if (e) (void)hipEventDestroy(e); e = nullptr; }
+        for (void *p : allocations) (void)hipFree(p);
+        allocations.clear();
+        for (int k = 0; k < bph::kStreams; ++k) {
+            if (streams[k]) (void)hipStreamDestroy(streams[k]);
+            if (endEvents[k]) (void)hipEventDestroy(endEvents[k]);
+            streams[k] = nullptr; endEvents[k] = nullptr;
+        }
+        for (int k = 0; k < bph::kClassCount; ++k) {
+            if (classStart[k]) (void)hipEventDestroy(classStart[k]);
+            if (classEnd[k]) (void)hipEventDestroy(classEnd[k]);
+            classStart[k] = classEnd[k] = nullptr;
+        }
+        if (startEvent) (void)hipEventDestroy(startEvent);
+        startEvent = nullptr;
+        stats = nullptr; statsCapacity = 0; descs = nullptr; arenaDev = nullptr; subsetOrders = nullptr;
+        xferBubbles = nullptr; xferIndices = nullptr; xferCapacity = 0;
+        pending = false;
+        for (auto &o : orders) o = nullptr;
+        packed = false;
+    }
+
+    template <class T> bp_status alloc(T *&out, size_t count) {
+        void *p = nullptr;
+        size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e != hipSuccess) { bph::set_error(std::string("hipMalloc: ") + hipGetErrorString(e)); return BP_ERR_ALLOCATION_FAILED; }
+        allocations.push_back(p);
+        out = static_cast<T *>(p);
+        return BP_OK;
+    }
+    template <class T> bp_status upload(T *dst, const std::vector<T> &src) {
+        if (src.empty()) return BP_OK;
+        BP_HIP(hipMemcpy(dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+        return BP_OK;
+    }
+
+    bp_status pack();
+    // Copies the published state of every resident world into its host mirror; `full` also
+    // copies the properties and the device's bubble count (transactions applied on the device).
+    bp_status download_state(bool full = false);
+    bp_status download_world(uint32_t wi);
+    bp_status write_bubbles(uint32_t wi, const bp_bubble *bubbles, uint32_t count);
+    uint64_t footprint(uint32_t wi) const;
+    bp_status launch(const uint32_t *subset, uint32_t subsetCount, uint32_t steps);
+    bp_status finish(bp_frame_info *infos, bool wait);
+    bp_status step(uint32_t steps, bp_frame_info *infos);
+    bp_status audit(bp_audit *out, uint32_t count);
+    bp_status capture(uint32_t steps);
+    bp_status replay(bp_frame_info *infos);
+    bp_status collect(uint32_t steps, bp_frame_info *infos, const std::vector<uint32_t> &listed, float ms);
+};
+
+#endif  // BP_BATCH_H

@@ -119,6 +119,15 @@ struct WorldDesc {
     float injectionPosition[2];
     uint32_t failed;          // sticky: a world that failed a step is not advanced again
     uint32_t exportAdjacency; // 1 = leave the rows of the last grid build in rowStart/rowLen/adj
+    uint32_t bubbleCapacity;  // bubble slots of the world's range (transactions never grow past it)
+};
+
+// A serving transaction on one world, applied on the device before the world's step
+// (bp_request of include/bubblephysics.h; `buckets` is the grid bucket count for the bubble
+// count after the transaction, computed by the host, which tracks the count).
+struct TxRecord {
+    uint32_t world, kind, index, buckets;
+    float impulse[2], direction[2];
 };
 
 // Per-world result of the last step (bp_frame_info fields).

@@ -1,0 +1,697 @@
+// bp_server.cpp -- native request loop over the worlds of a batch (continuous batching).
+//
+// One dispatcher thread owns every device call of the server. Requests (one step of one world,
+// optionally after a transaction) queue per world. A world with a waiting request and nothing
+// on the device is "ready" in the kernel class its bubble count selects. Whenever a class has
+// ready worlds and fewer than `max_inflight_per_class` launches in flight, one launch takes all
+// of them: an upload of the world list and transaction records, the transaction kernel, the
+// class's step kernel, a gather of the worlds' result records and their copy to pinned host
+// memory, then an event. Launches spread over several streams, so different classes and
+// consecutive launches of one class run concurrently, and requests that arrive while others
+// are on the device join the next launch of their class. Results are posted as each launch's
+// event completes, without waiting for other launches.
+//
+// Residency: worlds are admitted against a device-memory budget in index order; a request for
+// a world held in host memory waits until the launches in flight drain, then the least recently
+// used idle worlds are evicted to make room and the batch is repacked.
+//
+// The reference has no counterpart: its game loop steps one world per call
+// (BubbleWorld.submit/advance, Sources/BubblePhysicsPublic/BubbleWorld.swift:252-283).
+#include "bp_batch.h"
+
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <thread>
+
+using namespace bph;
+
+extern "C" hipError_t bp_launch_transactions(const bpd::Arena *arena, bpd::WorldDesc *descs, const bpd::TxRecord *txs,
+                                             uint32_t count, hipStream_t stream);
+extern "C" hipError_t bp_launch_gather_stats(const bpd::WorldStatus *stats, const uint32_t *order, uint32_t count,
+                                             bpd::WorldStatus *out, hipStream_t stream);
+extern "C" hipError_t bp_launch_gather_full_world(const bpd::Arena *arena, const bpd::WorldDesc *desc, bp_bubble *out,
+                                                  uint32_t count, hipStream_t stream);
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+struct Pending {
+    bp_request req;
+    double arrival;          // server clock
+    int64_t sink;            // index into the trace results, or -1 for the poll queue
+};
+
+struct Launch {
+    int cls = -1;
+    int stream = -1;
+    hipEvent_t done = nullptr;
+    uint32_t count = 0, txCount = 0;
+    std::vector<uint32_t> worlds;
+    std::vector<Pending> reqs;
+    std::vector<uint32_t> bubbles;
+    uint32_t *hOrder = nullptr, *dOrder = nullptr;            // pinned / device world lists
+    bpd::TxRecord *hTx = nullptr, *dTx = nullptr;
+    bpd::WorldStatus *hStats = nullptr, *dStats = nullptr;
+};
+
+constexpr int kMaxLaunches = 64;
+// Classes a served world may run in: the LDS configurations and the one-workgroup global kernel
+// (a cooperative grid needs the whole device and would stall every other launch).
+const int kServeClasses[] = {0, 1, 2, 3, 4, 7, 9};
+
+}  // namespace
+
+struct bp_server {
+    bp_batch *batch = nullptr;
+    bp_server_config cfg{};
+    Clock::time_point t0;
+    int device = 0;
+    std::thread worker;
+    std::mutex mu;
+    std::condition_variable cvWork, cvDone;
+    std::deque<Pending> incoming;               // submitted, not yet taken by the dispatcher
+    std::vector<bp_server_result> results;      // completed, not yet polled
+    bp_server_result *sinkResults = nullptr;    // results of the trace being replayed
+    uint64_t submitted = 0, completed = 0;      // guarded by mu
+    bool stop = false;
+    bp_status fatal = BP_OK;
+    std::string fatalText;
+    bp_server_stats st{};                       // published counters, guarded by mu
+    bp_server_stats local{};                    // the dispatcher's own counters, published under mu
+
+    // Dispatcher state.
+    uint32_t W = 0;
+    std::vector<std::deque<Pending>> queue;
+    std::vector<uint8_t> busy, queued, resident, wantAdmission;
+    std::vector<uint32_t> count, headCount;
+    std::vector<int> headClass;
+    std::vector<uint64_t> lastUse, footprintOf;
+    uint64_t useClock = 0, used = 0;
+    std::deque<uint32_t> ready[kClassCount];
+    std::vector<uint32_t> admissions;
+    bool launchAfterAdmission = false;          // the worlds ready after an admission round launch first
+    int inflightPerClass[kClassCount] = {};
+    Launch launches[kMaxLaunches];
+    std::vector<int> freeLaunch, inflight;
+    std::vector<hipStream_t> streams;
+    std::vector<int> streamLoad;
+    bpd::WorldStatus *stats = nullptr;          // one record per world (each world is in one launch at a time)
+    hipStream_t readStream = nullptr;
+    bp_bubble *readBuffer = nullptr;
+    uint32_t readCapacity = 0;
+    std::vector<void *> deviceAllocs, hostAllocs;
+
+    double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
+
+    bp_status init();
+    void release();
+    void run();
+    void enqueue(const Pending &p);
+    void prepare_head(uint32_t w);
+    void finish_request(const Pending &p, bp_status status, uint32_t substeps, uint32_t bubbles, uint32_t launchWorlds,
+                        int cls, float deviceMs);
+    bool poll_launches();
+    bool form_launches();
+    bp_status start_launch(int cls);
+    bp_status admit_waiting();
+    int class_for(uint32_t w, uint32_t n) const;
+    uint32_t buckets_for(uint32_t w, uint32_t n) const;
+    void fail(bp_status s);
+    void publish_counters() {          // with mu held
+        st.launches = local.launches; st.launched_worlds = local.launched_worlds; st.transactions = local.transactions;
+        st.rejected = local.rejected; st.evictions = local.evictions; st.admissions = local.admissions;
+        st.repacks = local.repacks; st.resident_bytes = local.resident_bytes;
+    }
+};
+
+namespace {
+
+template <class T> bp_status dev_alloc(bp_server *s, T *&out, size_t count) {
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T));
+    if (e != hipSuccess) { set_error(std::string("hipMalloc: ") + hipGetErrorString(e)); return BP_ERR_ALLOCATION_FAILED; }
+    s->deviceAllocs.push_back(p);
+    out = static_cast<T *>(p);
+    return BP_OK;
+}
+
+template <class T> bp_status host_alloc(bp_server *s, T *&out, size_t count) {
+    void *p = nullptr;
+    hipError_t e = hipHostMalloc(&p, std::max<size_t>(count, 1) * sizeof(T), hipHostMallocDefault);
+    if (e != hipSuccess) { set_error(std::string("hipHostMalloc: ") + hipGetErrorString(e)); return BP_ERR_ALLOCATION_FAILED; }
+    s->hostAllocs.push_back(p);
+    out = static_cast<T *>(p);
+    return BP_OK;
+}
+
+}  // namespace
+
+int bp_server::class_for(uint32_t w, uint32_t n) const {
+    const HostWorld &hw = batch->worlds[w];
+    const size_t ns = hw.segments.size(), nb = hw.bodies.size(), nv = hw.vertices.size() / 2, nt = hw.triangles.size() / 4;
+    if (hw.forcedClass >= 0 && hw.forcedClass != kCoopClass && class_fits(hw.forcedClass, n, ns, nb, nv, nt)) return hw.forcedClass;
+    if (n > cfg.big_world) return kGlobalClass;
+    for (int k : kServeClasses)
+        if (class_fits(k, n, ns, nb, nv, nt)) return k;
+    return kGlobalClass;
+}
+
+uint32_t bp_server::buckets_for(uint32_t w, uint32_t n) const {
+    const bp_world_config &c = batch->worlds[w].config;
+    return c.grid_bucket_count == 0 ? auto_buckets(n) : std::min(c.grid_bucket_count, bpd::kMaxBuckets);
+}
+
+void bp_server::fail(bp_status s) {
+    std::lock_guard<std::mutex> lock(mu);
+    if (fatal == BP_OK) { fatal = s; fatalText = g_lastError; }
+    cvDone.notify_all();
+}
+
+bp_status bp_server::init() {
+    bp_batch &b = *batch;
+    W = (uint32_t)b.worlds.size();
+    // Residency against the budget, in index order; the rest wait in host memory.
+    resident.assign(W, 0);
+    footprintOf.assign(W, 0);
+    bool changed = !b.packed;
+    for (uint32_t w = 0; w < W; ++w) {
+        footprintOf[w] = b.footprint(w);
+        const bool fits = cfg.memory_budget == 0 || used + footprintOf[w] <= cfg.memory_budget;
+        if (fits) { resident[w] = 1; used += footprintOf[w]; }
+        if (b.worlds[w].resident != (fits != 0)) {
+            if (!changed) {
+                bp_status s = b.download_state();
+                if (s != BP_OK) return s;
+            }
+            b.worlds[w].resident = fits;
+            changed = true;
+        }
+    }
+    if (changed || !b.packed) {
+        b.packed = false;
+        bp_status s = b.pack();
+        if (s != BP_OK) return s;
+    }
+    for (uint32_t w = 0; w < W; ++w) if (resident[w]) footprintOf[w] = b.footprint(w);
+    queue.assign(W, {});
+    busy.assign(W, 0); queued.assign(W, 0); wantAdmission.assign(W, 0);
+    count.resize(W); headCount.assign(W, 0); headClass.assign(W, -1); lastUse.assign(W, 0);
+    for (uint32_t w = 0; w < W; ++w) count[w] = (uint32_t)b.worlds[w].bubbles.size();
+    bp_status s = dev_alloc(this, stats, W);
+    if (s != BP_OK) return s;
+    const uint32_t cap = std::max<uint32_t>(1, std::min(cfg.max_worlds_per_launch, W));
+    for (int k = 0; k < kMaxLaunches; ++k) {
+        Launch &L = launches[k];
+        if ((s = host_alloc(this, L.hOrder, cap)) != BP_OK || (s = dev_alloc(this, L.dOrder, cap)) != BP_OK ||
+            (s = host_alloc(this, L.hTx, cap)) != BP_OK || (s = dev_alloc(this, L.dTx, cap)) != BP_OK ||
+            (s = host_alloc(this, L.hStats, cap)) != BP_OK || (s = dev_alloc(this, L.dStats, cap)) != BP_OK)
+            return s;
+        BP_HIP(hipEventCreateWithFlags(&L.done, hipEventDisableTiming));
+        L.worlds.reserve(cap); L.reqs.reserve(cap); L.bubbles.reserve(cap);
+        freeLaunch.push_back(k);
+    }
+    streams.resize(std::max<uint32_t>(1, cfg.streams));
+    streamLoad.assign(streams.size(), 0);
+    for (auto &st_ : streams) BP_HIP(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
+    BP_HIP(hipStreamCreateWithFlags(&readStream, hipStreamNonBlocking));
+    local.resident_bytes = used;
+    st.resident_bytes = used;
+    return BP_OK;
+}
+
+void bp_server::release() {
+    for (int k = 0; k < kMaxLaunches; ++k)
+        if (launches[k].done) { (void)hipEventDestroy(launches[k].done); launches[k].done = nullptr; }
+    for (auto &s : streams) if (s) (void)hipStreamDestroy(s);
+    streams.clear();
+    if (readStream) (void)hipStreamDestroy(readStream);
+    readStream = nullptr;
+    for (void *p : deviceAllocs) (void)hipFree(p);
+    for (void *p : hostAllocs) (void)hipHostFree(p);
+    deviceAllocs.clear(); hostAllocs.clear();
+}
+
+// A new request: it waits behind the world's earlier requests.
+void bp_server::enqueue(const Pending &p) {
+    const uint32_t w = p.req.world;
+    if (w >= W) {
+        set_error("request names an unknown world");
+        finish_request(p, BP_ERR_INVALID_ARGUMENT, 0, 0, 0, -1, 0.0f);
+        return;
+    }
+    queue[w].push_back(p);
+    if (!busy[w] && !queued[w]) prepare_head(w);
+}
+
+// Makes the world's oldest request ready: decides the transaction's effect on the bubble count
+// (rejecting one that needs more slots than the world holds) and the class of the step.
+void bp_server::prepare_head(uint32_t w) {
+    while (!queue[w].empty()) {
+        if (!resident[w]) {
+            if (!wantAdmission[w]) { wantAdmission[w] = 1; admissions.push_back(w); }
+            return;
+        }
+        const Pending &p = queue[w].front();
+        const uint32_t n = count[w];
+        uint32_t next = n;
+        switch (p.req.transaction) {
+            case BP_TX_ADD: case BP_TX_SPLIT: next = n == 0 ? 0 : n + 1; break;
+            case BP_TX_REMOVE: next = n > 1 ? n - 1 : n; break;
+            case BP_TX_MERGE: next = n > 1 ? n - 1 : n; break;
+            default: break;
+        }
+        if (p.req.transaction > BP_TX_SPLIT) {
+            set_error("unknown transaction kind");
+            Pending done = p; queue[w].pop_front();
+            finish_request(done, BP_ERR_INVALID_ARGUMENT, 0, n, 0, -1, 0.0f);
+            continue;
+        }
+        if (next > batch->worlds[w].capacity) {
+            Pending done = p; queue[w].pop_front();
+            ++local.rejected;
+            finish_request(done, BP_ERR_CAPACITY_OVERFLOW, 0, n, 0, -1, 0.0f);
+            continue;
+        }
+        headCount[w] = next;
+        headClass[w] = class_for(w, next);
+        ready[headClass[w]].push_back(w);
+        queued[w] = 1;
+        return;
+    }
+}
+
+void bp_server::finish_request(const Pending &p, bp_status status, uint32_t substeps, uint32_t bubbles, uint32_t launchWorlds,
+                               int cls, float deviceMs) {
+    bp_server_result r{};
+    r.tag = p.req.tag;
+    r.world = p.req.world;
+    r.status = status;
+    r.substeps = substeps;
+    r.bubbles = bubbles;
+    r.arrival = p.arrival;
+    r.completed = now();
+    r.launch_worlds = launchWorlds;
+    r.kernel_class = cls < 0 ? 0xffffffffu : (uint32_t)cls;
+    r.device_milliseconds = deviceMs;
+    std::lock_guard<std::mutex> lock(mu);
+    if (p.sink >= 0 && sinkResults) sinkResults[p.sink] = r;
+    else results.push_back(r);
+    ++completed;
+    st.completed = completed;
+    publish_counters();
+    cvDone.notify_all();
+}
+
+// Posts the results of every launch whose event has completed.
+bool bp_server::poll_launches() {
+    bool any = false;
+    for (size_t k = 0; k < inflight.size();) {
+        Launch &L = launches[inflight[k]];
+        hipError_t e = hipEventQuery(L.done);
+        if (e == hipErrorNotReady) { ++k; continue; }
+        if (e != hipSuccess) { set_error(std::string("hipEventQuery: ") + hipGetErrorString(e)); fail(BP_ERR_COMMAND_FAILED); return false; }
+        for (uint32_t j = 0; j < L.count; ++j) {
+            const uint32_t w = L.worlds[j];
+            const bpd::WorldStatus &s = L.hStats[j];
+            const bp_status status = s.executed ? (bp_status)s.status : BP_ERR_UNAVAILABLE;
+            finish_request(L.reqs[j], status, s.executed ? s.substeps : 0, L.bubbles[j], L.count, L.cls,
+                           s.executed ? (float)s.ticks * 1e-5f : 0.0f);
+            busy[w] = 0;
+            prepare_head(w);
+        }
+        --inflightPerClass[L.cls];
+        --streamLoad[L.stream];
+        freeLaunch.push_back(inflight[k]);
+        inflight[k] = inflight.back();
+        inflight.pop_back();
+        any = true;
+    }
+    return any;
+}
+
+// Queues one launch of class `cls` with its ready worlds.
+bp_status bp_server::start_launch(int cls) {
+    const int li = freeLaunch.back();
+    freeLaunch.pop_back();
+    Launch &L = launches[li];
+    L.cls = cls;
+    L.worlds.clear(); L.reqs.clear(); L.bubbles.clear();
+    L.txCount = 0;
+    const uint32_t cap = std::max<uint32_t>(1, std::min(cfg.max_worlds_per_launch, W));
+    auto &rq = ready[cls];
+    while (!rq.empty() && L.worlds.size() < cap) {
+        const uint32_t w = rq.front();
+        rq.pop_front();
+        queued[w] = 0;
+        busy[w] = 1;
+        Pending p = queue[w].front();
+        queue[w].pop_front();
+        if (p.req.transaction != BP_TX_NONE && count[w] > 0) {
+            bpd::TxRecord &t = L.hTx[L.txCount++];
+            t.world = w; t.kind = p.req.transaction; t.index = p.req.index; t.buckets = buckets_for(w, headCount[w]);
+            t.impulse[0] = p.req.impulse[0]; t.impulse[1] = p.req.impulse[1];
+            t.direction[0] = p.req.direction[0]; t.direction[1] = p.req.direction[1];
+            ++local.transactions;
+        }
+        count[w] = headCount[w];
+        lastUse[w] = ++useClock;
+        L.worlds.push_back(w);
+        L.reqs.push_back(p);
+        L.bubbles.push_back(count[w]);
+    }
+    // Longest world first inside the launch (its workgroup starts first).
+    std::vector<uint32_t> idx(L.worlds.size());
+    for (uint32_t j = 0; j < idx.size(); ++j) idx[j] = j;
+    std::stable_sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) { return L.bubbles[a] > L.bubbles[b]; });
+    std::vector<uint32_t> ws(idx.size()), nb(idx.size());
+    std::vector<Pending> rs(idx.size());
+    for (uint32_t j = 0; j < idx.size(); ++j) { ws[j] = L.worlds[idx[j]]; nb[j] = L.bubbles[idx[j]]; rs[j] = L.reqs[idx[j]]; }
+    L.worlds.swap(ws); L.bubbles.swap(nb); L.reqs.swap(rs);
+    L.count = (uint32_t)L.worlds.size();
+    for (uint32_t j = 0; j < L.count; ++j) L.hOrder[j] = L.worlds[j];
+    // The least loaded stream.
+    int si = 0;
+    for (int k = 1; k < (int)streams.size(); ++k) if (streamLoad[k] < streamLoad[si]) si = k;
+    L.stream = si;
+    hipStream_t s = streams[si];
+    bp_batch &b = *batch;
+    BP_HIP(hipMemcpyAsync(L.dOrder, L.hOrder, L.count * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    if (L.txCount) {
+        BP_HIP(hipMemcpyAsync(L.dTx, L.hTx, L.txCount * sizeof(bpd::TxRecord), hipMemcpyHostToDevice, s));
+        hipError_t e = bp_launch_transactions(b.arenaDev, b.descs, L.dTx, L.txCount, s);
+        if (e != hipSuccess) { set_error(std::string("transaction launch: ") + hipGetErrorString(e)); return BP_ERR_COMMAND_FAILED; }
+    }
+    hipError_t e = cls == kGlobalClass ? bp_launch_world_step(1024, b.arenaDev, b.descs, stats, L.dOrder, L.count, s)
+                                       : bp_launch_world_lds(cls, b.arenaDev, b.descs, stats, L.dOrder, L.count, 1, W, s);
+    if (e != hipSuccess) { set_error(std::string("kernel launch: ") + hipGetErrorString(e)); return BP_ERR_COMMAND_FAILED; }
+    e = bp_launch_gather_stats(stats, L.dOrder, L.count, L.dStats, s);
+    if (e != hipSuccess) { set_error(std::string("gather launch: ") + hipGetErrorString(e)); return BP_ERR_COMMAND_FAILED; }
+    BP_HIP(hipMemcpyAsync(L.hStats, L.dStats, L.count * sizeof(bpd::WorldStatus), hipMemcpyDeviceToHost, s));
+    BP_HIP(hipEventRecord(L.done, s));
+    ++inflightPerClass[cls];
+    ++streamLoad[si];
+    inflight.push_back(li);
+    ++local.launches;
+    local.launched_worlds += L.count;
+    return BP_OK;
+}
+
+// Admits the worlds whose requests wait in host memory. Runs with nothing in flight: the
+// newest state of every world is read back, idle worlds are evicted least recently used first
+// until the admitted ones fit the budget, and the batch is repacked.
+bp_status bp_server::admit_waiting() {
+    bp_batch &b = *batch;
+    bp_status s = b.download_state(true);
+    if (s != BP_OK) return s;
+    std::vector<uint8_t> admitted(W, 0);
+    for (uint32_t w : admissions) {
+        if (resident[w]) continue;
+        const uint64_t need = b.footprint(w);
+        while (cfg.memory_budget && used + need > cfg.memory_budget) {
+            // Least recently used resident world with no waiting request and not admitted now.
+            int64_t victim = -1;
+            for (uint32_t v = 0; v < W; ++v)
+                if (resident[v] && !admitted[v] && queue[v].empty() && (victim < 0 || lastUse[v] < lastUse[victim])) victim = v;
+            if (victim < 0)
+                for (uint32_t v = 0; v < W; ++v)
+                    if (resident[v] && !admitted[v] && (victim < 0 || lastUse[v] < lastUse[victim])) victim = v;
+            if (victim < 0) { set_error("memory budget too small for the waiting worlds"); return BP_ERR_ALLOCATION_FAILED; }
+            resident[victim] = 0;
+            b.worlds[victim].resident = false;
+            used -= footprintOf[victim];
+            // Requests already prepared for the victim go back to waiting for admission.
+            if (queued[victim]) {
+                auto &rq = ready[headClass[victim]];
+                rq.erase(std::remove(rq.begin(), rq.end(), victim), rq.end());
+                queued[victim] = 0;
+            }
+            ++local.evictions;
+        }
+        resident[w] = 1;
+        admitted[w] = 1;
+        b.worlds[w].resident = true;
+        footprintOf[w] = need;
+        used += need;
+        ++local.admissions;
+    }
+    b.packed = false;
+    if ((s = b.pack()) != BP_OK) return s;
+    ++local.repacks;
+    for (uint32_t w = 0; w < W; ++w) {
+        count[w] = (uint32_t)b.worlds[w].bubbles.size();
+        if (resident[w]) footprintOf[w] = b.footprint(w);
+    }
+    used = 0;
+    for (uint32_t w = 0; w < W; ++w) if (resident[w]) used += footprintOf[w];
+    local.resident_bytes = used;
+    std::vector<uint32_t> waiting;
+    waiting.swap(admissions);
+    for (uint32_t w : waiting) wantAdmission[w] = 0;
+    // Ready lists hold class choices made with the old layout; rebuild them.
+    for (auto &rq : ready) rq.clear();
+    for (uint32_t w = 0; w < W; ++w) {
+        queued[w] = 0;
+        if (!queue[w].empty()) prepare_head(w);
+    }
+    return BP_OK;
+}
+
+// Launches the ready worlds. While worlds wait for admission, no new launches start (the repack
+// needs an idle device) except right after an admission round: then every world that is ready
+// is launched once before the next round, so each round makes progress even when the budget
+// forces the next admission to evict a world that still has requests.
+bool bp_server::form_launches() {
+    if (!admissions.empty() && !launchAfterAdmission) {
+        if (!inflight.empty()) return false;        // drain first: the repack moves every world
+        bp_status s = admit_waiting();
+        if (s != BP_OK) { fail(s); return false; }
+        launchAfterAdmission = true;
+        return true;
+    }
+    bool any = false;
+    for (int k = 0; k < kClassCount; ++k) {
+        while (!ready[k].empty() && inflightPerClass[k] < (int)cfg.max_inflight_per_class && !freeLaunch.empty()) {
+            bp_status s = start_launch(k);
+            if (s != BP_OK) { fail(s); return false; }
+            any = true;
+        }
+    }
+    bool readyLeft = false;
+    for (int k = 0; k < kClassCount && !readyLeft; ++k) readyLeft = !ready[k].empty();
+    if (!readyLeft) launchAfterAdmission = false;
+    return any;
+}
+
+void bp_server::run() {
+    (void)hipSetDevice(device);
+    std::vector<Pending> taken;
+    for (;;) {
+        {
+            std::unique_lock<std::mutex> lock(mu);
+            if (fatal != BP_OK) break;
+            bool idle = inflight.empty();
+            if (idle)
+                for (int k = 0; k < kClassCount && idle; ++k) idle = ready[k].empty();
+            idle = idle && admissions.empty();
+            if (stop && inflight.empty()) break;
+            if (incoming.empty() && idle && !stop) cvWork.wait_for(lock, std::chrono::milliseconds(2));
+            taken.assign(incoming.begin(), incoming.end());
+            incoming.clear();
+            publish_counters();
+        }
+        for (const Pending &p : taken) enqueue(p);
+        bool progressed = poll_launches();
+        {
+            std::lock_guard<std::mutex> lock(mu);
+            if (fatal != BP_OK) break;
+            if (stop) {                             // no new launches once stopping
+                if (!progressed) std::this_thread::yield();
+                continue;
+            }
+        }
+        progressed |= form_launches();
+        if (!progressed && !inflight.empty()) std::this_thread::yield();
+    }
+    // Drain whatever is still on the device before the dispatcher ends.
+    for (int id : inflight) (void)hipEventSynchronize(launches[id].done);
+    poll_launches();
+    std::lock_guard<std::mutex> lock(mu);
+    publish_counters();
+}
+
+// ---------------------------------------------------------------------------------------
+// C ABI.
+// ---------------------------------------------------------------------------------------
+extern "C" {
+
+void bp_server_record_sizes(uint32_t *out) {
+    if (!out) return;
+    out[0] = sizeof(bp_request); out[1] = sizeof(bp_server_result); out[2] = sizeof(bp_server_config);
+    out[3] = sizeof(bp_server_stats);
+}
+
+void bp_server_config_default(bp_server_config *c) {
+    if (!c) return;
+    std::memset(c, 0, sizeof *c);
+    c->memory_budget = 0;
+    c->streams = 8;
+    c->max_inflight_per_class = 2;
+    c->max_worlds_per_launch = 1024;
+    c->big_world = 4096;
+}
+
+bp_status bp_server_create(bp_batch *batch, const bp_server_config *config, bp_server **out) {
+    if (!batch || !out) return BP_ERR_INVALID_ARGUMENT;
+    {
+        std::lock_guard<std::mutex> lock(batch->mutex);
+        if (batch->inFlight || batch->pending || batch->serving) return BP_ERR_FRAME_IN_FLIGHT;
+        batch->serving = true;
+    }
+    bp_server *s = new bp_server();
+    s->batch = batch;
+    bp_server_config_default(&s->cfg);
+    if (config) {
+        s->cfg = *config;
+        if (s->cfg.streams == 0) s->cfg.streams = 8;
+        if (s->cfg.max_inflight_per_class == 0) s->cfg.max_inflight_per_class = 2;
+        if (s->cfg.max_worlds_per_launch == 0) s->cfg.max_worlds_per_launch = 1024;
+        if (s->cfg.big_world == 0) s->cfg.big_world = 4096;
+        s->cfg.streams = std::min<uint32_t>(s->cfg.streams, 32);
+    }
+    (void)hipGetDevice(&s->device);
+    s->t0 = Clock::now();
+    bp_status st = s->init();
+    if (st != BP_OK) {
+        s->release();
+        batch->serving = false;
+        delete s;
+        return st;
+    }
+    s->worker = std::thread([s] { s->run(); });
+    *out = s;
+    return BP_OK;
+}
+
+void bp_server_destroy(bp_server *s) {
+    if (!s) return;
+    {
+        std::lock_guard<std::mutex> lock(s->mu);
+        s->stop = true;
+    }
+    s->cvWork.notify_all();
+    if (s->worker.joinable()) s->worker.join();
+    (void)hipDeviceSynchronize();
+    bp_batch &b = *s->batch;
+    // Hand back host mirrors with the device's counts and properties (transactions ran on the
+    // device); the next batch step repacks with the classes the new counts select.
+    if (b.download_state(true) == BP_OK) b.packed = false;
+    s->release();
+    b.serving = false;
+    delete s;
+}
+
+bp_status bp_server_submit(bp_server *s, const bp_request *r) {
+    if (!s || !r) return BP_ERR_INVALID_ARGUMENT;
+    {
+        std::lock_guard<std::mutex> lock(s->mu);
+        if (s->fatal != BP_OK) { set_error(s->fatalText); return s->fatal; }
+        if (s->stop) return BP_ERR_UNAVAILABLE;
+        s->incoming.push_back(Pending{*r, s->now(), -1});
+        ++s->submitted;
+        s->st.submitted = s->submitted;
+    }
+    s->cvWork.notify_one();
+    return BP_OK;
+}
+
+bp_status bp_server_poll(bp_server *s, bp_server_result *out, uint32_t capacity, uint32_t *out_count) {
+    if (!s || (capacity && !out)) return BP_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(s->mu);
+    const uint32_t n = (uint32_t)std::min<size_t>(capacity, s->results.size());
+    std::copy(s->results.begin(), s->results.begin() + n, out);
+    s->results.erase(s->results.begin(), s->results.begin() + n);
+    if (out_count) *out_count = n;
+    if (s->fatal != BP_OK) { set_error(s->fatalText); return s->fatal; }
+    return BP_OK;
+}
+
+bp_status bp_server_wait_idle(bp_server *s) {
+    if (!s) return BP_ERR_INVALID_ARGUMENT;
+    std::unique_lock<std::mutex> lock(s->mu);
+    s->cvDone.wait(lock, [&] { return s->completed >= s->submitted || s->fatal != BP_OK; });
+    if (s->fatal != BP_OK) { set_error(s->fatalText); return s->fatal; }
+    return BP_OK;
+}
+
+bp_status bp_server_run_trace(bp_server *s, const bp_request *trace, uint32_t count, bp_server_result *results) {
+    if (!s || (count && (!trace || !results))) return BP_ERR_INVALID_ARGUMENT;
+    {
+        std::unique_lock<std::mutex> lock(s->mu);
+        if (s->sinkResults) return BP_ERR_FRAME_IN_FLIGHT;
+        s->cvDone.wait(lock, [&] { return s->completed >= s->submitted || s->fatal != BP_OK; });
+        s->sinkResults = results;
+    }
+    const double base = s->now();
+    for (uint32_t i = 0; i < count; ++i) {
+        const double due = base + trace[i].arrival;
+        for (;;) {
+            const double left = due - s->now();
+            if (left <= 0.0) break;
+            if (left > 3e-4) std::this_thread::sleep_for(std::chrono::duration<double>(left - 2e-4));
+            else std::this_thread::yield();
+        }
+        {
+            std::lock_guard<std::mutex> lock(s->mu);
+            if (s->fatal != BP_OK) break;
+            s->incoming.push_back(Pending{trace[i], due, (int64_t)i});
+            ++s->submitted;
+            s->st.submitted = s->submitted;
+        }
+        s->cvWork.notify_one();
+    }
+    std::unique_lock<std::mutex> lock(s->mu);
+    s->cvDone.wait(lock, [&] { return s->completed >= s->submitted || s->fatal != BP_OK; });
+    s->sinkResults = nullptr;
+    if (s->fatal != BP_OK) { set_error(s->fatalText); return s->fatal; }
+    return BP_OK;
+}
+
+bp_status bp_server_read_bubbles(bp_server *s, uint32_t world, bp_bubble *out, uint32_t capacity, uint32_t *out_count) {
+    if (!s || world >= s->W) return BP_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(s->mu);
+    if (s->completed < s->submitted || !s->incoming.empty()) return BP_ERR_FRAME_IN_FLIGHT;
+    bp_batch &b = *s->batch;
+    const uint32_t n = s->count[world];
+    if (out_count) *out_count = n;
+    if (!out) return BP_OK;
+    if (capacity < n) return BP_ERR_CAPACITY_OVERFLOW;
+    if (!s->resident[world]) {
+        std::copy(b.worlds[world].bubbles.begin(), b.worlds[world].bubbles.end(), out);
+        return BP_OK;
+    }
+    if (n == 0) return BP_OK;
+    if (s->readCapacity < n) {
+        bp_status st = dev_alloc(s, s->readBuffer, n);
+        if (st != BP_OK) return st;
+        s->readCapacity = n;
+    }
+    hipError_t e = bp_launch_gather_full_world(b.arenaDev, b.descs + world, s->readBuffer, n, s->readStream);
+    if (e != hipSuccess) { set_error(std::string("gather: ") + hipGetErrorString(e)); return BP_ERR_COMMAND_FAILED; }
+    BP_HIP(hipMemcpyAsync(out, s->readBuffer, n * sizeof(bp_bubble), hipMemcpyDeviceToHost, s->readStream));
+    BP_HIP(hipStreamSynchronize(s->readStream));
+    return BP_OK;
+}
+
+bp_status bp_server_get_stats(bp_server *s, bp_server_stats *out) {
+    if (!s || !out) return BP_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(s->mu);
+    *out = s->st;
+    return BP_OK;
+}
+
+double bp_server_clock(const bp_server *s) { return s ? s->now() : 0.0; }
+
+}  // extern "C"

@@ -1,109 +1,151 @@
 """Serving: step and transaction requests for many resident worlds, by continuous batching.
 
-Requests arrive as a seeded Poisson process; each asks for one step of one world, some carry a
-transaction applied before that step (impulse, add, remove, merge or split of bubbles). The loop
-keeps one launch in flight: every world with a waiting request contributes its oldest request to
-the next launch, and requests that arrive meanwhile wait for the launch after it. Worlds are
-admitted against a device-memory budget; the least recently used idle worlds are evicted to host
-memory when an admission needs room, and come back on their next request.
+The request loop is native (bp_server in the HIP library): a dispatcher thread takes requests
+as they arrive, launches per kernel class the waiting worlds that are not already on the
+device, keeps several launches in flight on several streams and posts each result when its
+launch completes. Transactions run on the device before the step they come with. Worlds are
+admitted against a device-memory budget; the least recently used idle worlds are evicted to
+host memory when an admission needs room.
 
-This has no counterpart in the reference, which steps one world per call. The transaction
-semantics follow the reference's staging in simplified form (TransactionStager.swift:312-324 for
-impulses, StructuralOperations.swift:10-151 for merge and split: area, mass and momentum kept).
+This module builds request traces (a seeded Poisson process), states the transactions in numpy
+(``apply_transaction``, the host-side statement the device kernel is tested against) and wraps
+the server. The reference has no serving form: its game loop steps one world per call
+(BubbleWorld.submit/advance, Sources/BubblePhysicsPublic/BubbleWorld.swift:252-283). The
+transactions are simplified forms of its staging (TransactionStager.swift:312-324 for impulses,
+StructuralOperations.swift:10-151 for merge and split): mass, momentum and area are kept.
 """
 from __future__ import annotations
 
-import collections
+import ctypes
 import math
-import time
-from dataclasses import dataclass, field
+from dataclasses import dataclass
 
 import numpy as np
 
 from . import abi, scenes, solver
 
 KINDS = ("impulse", "add", "remove", "merge", "split")
-BIG_WORLD = 4096          # above this a world runs in the one-workgroup global kernel while serving
+TX_CODE = {None: abi.BP_TX_NONE, "impulse": abi.BP_TX_IMPULSE, "add": abi.BP_TX_ADD, "remove": abi.BP_TX_REMOVE,
+           "merge": abi.BP_TX_MERGE, "split": abi.BP_TX_SPLIT}
+HALF_RADIUS = np.float32(0.70710678)      # 1/sqrt(2) rounded to FP32: radius of each half of a split
 
 
 @dataclass
 class Request:
-    arrival: float         # seconds after the start of the run
+    arrival: float                   # seconds after the start of the trace
     world: int
-    transaction: str | None
-    seed: int
+    transaction: str | None = None
+    index: int = 0                   # bubble of the transaction: index % bubble count
+    impulse: tuple = (0.0, 0.0)      # velocity change of an impulse (J = impulse * m)
+    direction: tuple = (1.0, 0.0)    # unit vector of add and split
+    # Filled by the server.
     done: float = math.nan
     status: int = -1
     substeps: int = 0
+    bubbles: int = 0
+
+
+def transaction_params(rng, kind: str | None) -> dict:
+    """Random parameters of one transaction, rounded to FP32 as the device receives them."""
+    angle = float(rng.uniform(0.0, 2.0 * math.pi))
+    return {"transaction": kind, "index": int(rng.integers(0, 2**31 - 1)),
+            "impulse": tuple(float(x) for x in rng.normal(0.0, 50.0, 2).astype(np.float32)),
+            "direction": (float(np.float32(math.cos(angle))), float(np.float32(math.sin(angle))))}
 
 
 def poisson_requests(worlds: int, rate: float, duration: float, seed: int, transaction_fraction: float = 0.1):
-    """Arrivals of a Poisson process of `rate` requests/s over `duration` s, worlds uniform."""
+    """Arrivals of a Poisson process of `rate` requests/s over `duration` s, worlds uniform; a
+    `transaction_fraction` of them carry a transaction, the five kinds equally likely."""
     rng = np.random.Generator(np.random.PCG64(seed))
     count = int(rng.poisson(rate * duration))
     arrivals = np.sort(rng.uniform(0.0, duration, count))
     world = rng.integers(0, worlds, count)
     with_tx = rng.uniform(0.0, 1.0, count) < transaction_fraction
     kind = rng.integers(0, len(KINDS), count)
-    seeds = rng.integers(0, 2**31 - 1, count)
-    return [Request(float(arrivals[k]), int(world[k]), KINDS[int(kind[k])] if with_tx[k] else None, int(seeds[k]))
-            for k in range(count)]
+    out = []
+    for k in range(count):
+        p = transaction_params(rng, KINDS[int(kind[k])] if with_tx[k] else None)
+        out.append(Request(float(arrivals[k]), int(world[k]), **p))
+    return out
 
 
-def apply_transaction(bubbles: np.ndarray, kind: str, seed: int) -> np.ndarray:
-    """The world's bubbles after one transaction (float32 arithmetic, deterministic in `seed`)."""
+def apply_transaction(bubbles: np.ndarray, req: Request) -> np.ndarray:
+    """The world's bubbles after one transaction: the host-side statement of the device kernel
+    (transaction_kernel in csrc/bp_transfer.hip), FP32 with one rounding per operation."""
     b = bubbles.copy()
     n = len(b)
-    if n == 0:
+    if n == 0 or req.transaction is None:
         return b
-    rng = np.random.Generator(np.random.PCG64(seed))
-    i = int(rng.integers(0, n))
     f32 = np.float32
-    if kind == "impulse":                                   # v += J / m
+    i = int(req.index) % n
+    kind = req.transaction
+    if kind == "impulse":                                            # v += (J m) / m
         if b["inverse_mass"][i] > 0:
-            j = rng.normal(0.0, 50.0, 2).astype(f32) * b["mass"][i]
-            b["velocity"][i] = (b["velocity"][i] + j * b["inverse_mass"][i]).astype(f32)
+            m = b["mass"][i]
+            j = np.array(req.impulse, f32) * m
+            b["velocity"][i] = b["velocity"][i] + j * b["inverse_mass"][i]
         return b
-    if kind == "remove" and n > 1:
-        return np.delete(b, i)
+    if kind == "remove":
+        return np.delete(b, i) if n > 1 else b
+    direction = np.array(req.direction, f32)
     if kind == "add":
         new = b[i:i + 1].copy()
-        new["id"] = int(b["id"].max()) + 1
-        angle = f32(rng.uniform(0.0, 2.0 * math.pi))
+        new["id"] = b["id"][n - 1] + 1
         offset = f32(2.0) * b["radius"][i]
-        new["center"][0] = b["center"][i] + offset * np.array([math.cos(angle), math.sin(angle)], f32)
+        new["center"][0] = b["center"][i] + offset * direction
         new["previous_center"] = new["center"]
         return np.concatenate([b, new])
-    if kind == "merge" and n > 1:                            # two neighbours by index become one
+    if kind == "merge":                                              # index neighbours a < c become one in slot a
+        if n < 2:
+            return b
         k = i + 1 if i + 1 < n else i - 1
-        a, c = b[min(i, k)], b[max(i, k)]
-        m = f32(a["mass"] + c["mass"])
-        merged = b[min(i, k):min(i, k) + 1].copy()
-        merged["radius"] = f32(math.sqrt(float(a["radius"]) ** 2 + float(c["radius"]) ** 2))
-        merged["mass"] = m
-        merged["inverse_mass"] = f32(1.0) / m if m > 0 else f32(0.0)
-        merged["center"] = ((a["center"] * a["mass"] + c["center"] * c["mass"]) / m).astype(f32)
-        merged["previous_center"] = merged["center"]
-        merged["velocity"] = ((a["velocity"] * a["mass"] + c["velocity"] * c["mass"]) / m).astype(f32)
-        keep = np.delete(b, [min(i, k), max(i, k)])
-        return np.sort(np.concatenate([keep, merged]), order="id")
-    if kind == "split":                                      # one bubble becomes two halves
-        a = b[i]
-        half = b[i:i + 1].copy()
-        half["radius"] = f32(a["radius"] / math.sqrt(2.0))
-        half["mass"] = f32(a["mass"] * f32(0.5))
-        half["inverse_mass"] = f32(1.0) / half["mass"] if half["mass"][0] > 0 else f32(0.0)
-        angle = f32(rng.uniform(0.0, 2.0 * math.pi))
-        d = (np.array([math.cos(angle), math.sin(angle)], f32) * half["radius"][0]).astype(f32)
-        one, two = half.copy(), half.copy()
-        one["center"] = (a["center"] + d).astype(f32)
-        two["center"] = (a["center"] - d).astype(f32)
-        two["id"] = int(b["id"].max()) + 1
-        for x in (one, two):
-            x["previous_center"] = x["center"]
-        keep = np.delete(b, i)
-        return np.sort(np.concatenate([keep, one, two]), order="id")
-    return b
+        a, c = min(i, k), max(i, k)
+        ma, mc = b["mass"][a], b["mass"][c]
+        m = f32(ma + mc)
+        ra, rc = b["radius"][a], b["radius"][c]
+        out = b.copy()
+        out["radius"][a] = np.sqrt(f32(ra * ra + rc * rc))
+        out["mass"][a] = m
+        if m > 0:                                                    # a massless pair keeps a's centre and velocity
+            out["inverse_mass"][a] = f32(1.0) / m
+            out["center"][a] = (b["center"][a] * ma + b["center"][c] * mc) / m
+            out["velocity"][a] = (b["velocity"][a] * ma + b["velocity"][c] * mc) / m
+        else:
+            out["inverse_mass"][a] = f32(0.0)
+        out["previous_center"][a] = out["center"][a]
+        return np.delete(out, c)
+    if kind == "split":                                              # halves at c +- r_half * direction
+        r = b["radius"][i] * HALF_RADIUS
+        m = b["mass"][i] * f32(0.5)
+        invm = f32(1.0) / m if m > 0 else f32(0.0)
+        d = direction * r
+        centre = b["center"][i].copy()
+        two = b[i:i + 1].copy()
+        for rec in (b[i:i + 1], two):
+            rec["radius"] = r
+            rec["mass"] = m
+            rec["inverse_mass"] = invm
+        b["center"][i] = centre + d
+        b["previous_center"][i] = b["center"][i]
+        two["center"][0] = centre - d
+        two["previous_center"] = two["center"]
+        two["id"] = b["id"][n - 1] + 1
+        return np.concatenate([b, two])
+    raise ValueError(f"unknown transaction {kind!r}")
+
+
+def to_trace(requests) -> np.ndarray:
+    """bp_request records of a list of Request (tag = position)."""
+    t = np.zeros(len(requests), abi.REQUEST_DTYPE)
+    for k, r in enumerate(requests):
+        t[k]["arrival"] = r.arrival
+        t[k]["tag"] = k
+        t[k]["world"] = r.world
+        t[k]["transaction"] = TX_CODE[r.transaction]
+        t[k]["index"] = r.index
+        t[k]["impulse"] = r.impulse
+        t[k]["direction"] = r.direction
+    return t
 
 
 def serving_worlds(count: int, seed: int, size=(32, 16384)):
@@ -115,119 +157,106 @@ def serving_worlds(count: int, seed: int, size=(32, 16384)):
 
 @dataclass
 class ServeStats:
-    served: int = 0
-    bubble_substeps: int = 0
-    elapsed: float = 0.0
-    launches: int = 0
-    evictions: int = 0
-    admissions: int = 0
-    failed: int = 0
-    transaction_seconds: float = 0.0   # host time spent applying transactions
-    launch_sizes: list = field(default_factory=list)
-    latencies_ms: list = field(default_factory=list)
+    results: np.ndarray              # bp_server_result records, in trace order
+    elapsed: float                   # seconds from the start of the trace to the last result
+    counters: dict
 
     def summary(self) -> dict:
-        lat = np.asarray(self.latencies_ms, np.float64)
+        r = self.results
+        lat = (r["completed"] - r["arrival"]) * 1e3
+        ok = r["status"] == abi.BP_OK
+        statuses = {abi.STATUS_NAMES.get(int(s), int(s)): int(c) for s, c in zip(*np.unique(r["status"], return_counts=True))}
+        work = int((r["bubbles"][ok].astype(np.int64) * r["substeps"][ok].astype(np.int64)).sum())
         return {
-            "served": self.served, "launches": self.launches, "elapsed_s": self.elapsed,
-            "requests_per_s": self.served / self.elapsed if self.elapsed else 0.0,
-            "bubble_substeps_per_s": self.bubble_substeps / self.elapsed if self.elapsed else 0.0,
+            "served": int(len(r)), "elapsed_s": self.elapsed,
+            "requests_per_s": len(r) / self.elapsed if self.elapsed else 0.0,
+            "bubble_substeps_per_s": work / self.elapsed if self.elapsed else 0.0,
             "p50_ms": float(np.percentile(lat, 50)) if len(lat) else math.nan,
             "p99_ms": float(np.percentile(lat, 99)) if len(lat) else math.nan,
-            "evictions": self.evictions, "admissions": self.admissions, "failed_steps": self.failed,
-            "mean_worlds_per_launch": float(np.mean(self.launch_sizes)) if self.launch_sizes else 0.0,
-            "transaction_seconds": self.transaction_seconds,
+            "p999_ms": float(np.percentile(lat, 99.9)) if len(lat) else math.nan,
+            "p99_ms_ok_steps": float(np.percentile(lat[ok], 99)) if ok.any() else math.nan,
+            "statuses": statuses,
+            "mean_worlds_per_launch": self.counters["launched_worlds"] / max(self.counters["launches"], 1),
+            **{k: self.counters[k] for k in ("launches", "transactions", "rejected", "evictions", "admissions", "repacks",
+                                              "resident_bytes")},
         }
 
 
 class Server:
-    """Continuous batching over a bp_batch of worlds with a device-memory budget (bytes)."""
+    """bp_server over a batch of worlds with a device-memory budget (bytes, 0 = no limit)."""
 
-    def __init__(self, worlds, *, memory_budget: int):
-        self.batch = solver.Batch(worlds)
-        self.counts = [len(w.bubbles) for w in worlds]
-        for w, n in enumerate(self.counts):
-            if n > BIG_WORLD:
-                self.batch.set_world_class(w, 10)   # concurrent with the other classes
-        self.budget = memory_budget
-        self.resident = collections.OrderedDict()   # world -> footprint, least recently used first
-        self.stats = ServeStats()
-        self.used = 0
-        # Worlds start resident while they fit the budget, in index order; the rest wait in host memory.
-        for w in range(len(worlds)):
-            need = self.batch.footprint(w)
-            if self.used + need <= self.budget:
-                self.resident[w] = need
-                self.used += need
-            else:
-                self.batch.set_resident(w, False)
+    def __init__(self, worlds, *, memory_budget: int = 0, streams: int = 0, max_inflight_per_class: int = 0,
+                 max_worlds_per_launch: int = 0, big_world: int = 0):
+        self._lib = abi.load_library()
+        self.batch = worlds if isinstance(worlds, solver.Batch) else solver.Batch(worlds)
+        cfg = abi.ServerConfig()
+        self._lib.bp_server_config_default(ctypes.byref(cfg))
+        cfg.memory_budget = memory_budget
+        for name, value in (("streams", streams), ("max_inflight_per_class", max_inflight_per_class),
+                            ("max_worlds_per_launch", max_worlds_per_launch), ("big_world", big_world)):
+            if value:
+                setattr(cfg, name, value)
+        self._handle = ctypes.c_void_p()
+        abi.check(self._lib.bp_server_create(self.batch._handle, ctypes.byref(cfg), ctypes.byref(self._handle)),
+                  "bp_server_create")
 
-    def _admit(self, world: int, busy: set):
-        if world in self.resident:
-            self.resident.move_to_end(world)
-            return
-        need = self.batch.footprint(world)
-        while self.used + need > self.budget:
-            victim = next((w for w in self.resident if w not in busy), None)
-            if victim is None:
-                raise RuntimeError("memory budget too small for the worlds of one launch")
-            self.used -= self.resident.pop(victim)
-            self.batch.set_resident(victim, False)
-            self.stats.evictions += 1
-        self.batch.set_resident(world, True)
-        self.resident[world] = need
-        self.used += need
-        self.stats.admissions += 1
+    def clock(self) -> float:
+        return float(self._lib.bp_server_clock(self._handle))
 
-    def run(self, requests, *, on_done=None) -> ServeStats:
-        """Serves every request (arrival times relative to the call) and returns the statistics."""
-        queues = collections.defaultdict(collections.deque)
-        pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
-        in_flight = None
-        t0 = time.perf_counter()
-        while pending or in_flight is not None or any(queues.values()):
-            now = time.perf_counter() - t0
-            while pending and pending[0].arrival <= now:
-                r = pending.popleft()
-                queues[r.world].append(r)
-            if in_flight is None:
-                ready = [w for w, q in queues.items() if q]
-                if not ready:
-                    if pending:
-                        time.sleep(min(max(pending[0].arrival - now, 0.0), 0.001))
-                    continue
-                ready.sort(key=lambda w: queues[w][0].arrival)
-                batch_list = [(w, queues[w].popleft()) for w in ready]
-                busy = {w for w, _ in batch_list}
-                for w, r in batch_list:
-                    self._admit(w, busy)
-                t_tx = time.perf_counter()
-                for w, r in batch_list:
-                    if r.transaction:
-                        nb = apply_transaction(self.batch.read_bubbles(w), r.transaction, r.seed)
-                        self.batch.write_bubbles(w, nb)
-                        self.counts[w] = len(nb)
-                self.stats.transaction_seconds += time.perf_counter() - t_tx
-                self.batch.submit_worlds([w for w, _ in batch_list], 1)
-                self.stats.launch_sizes.append(len(batch_list))
-                in_flight = batch_list
-                self.stats.launches += 1
-                continue
-            infos = self.batch.poll()
-            if infos is None:
-                time.sleep(0.0001)
-                continue
-            done = time.perf_counter() - t0
-            for j, (w, r) in enumerate(in_flight):
-                info = infos[0, j]
-                r.done, r.status, r.substeps = done, int(info["status"]), int(info["substep_count"])
-                self.stats.latencies_ms.append((done - r.arrival) * 1e3)
-                if r.status == abi.BP_OK:                      # work of failed steps is not counted
-                    self.stats.bubble_substeps += self.counts[w] * r.substeps
-                self.stats.served += 1
-                self.stats.failed += r.status != abi.BP_OK
-                if on_done:
-                    on_done(r)
-            in_flight = None
-        self.stats.elapsed = time.perf_counter() - t0
-        return self.stats
+    def run(self, requests) -> ServeStats:
+        """Replays the requests at their arrival times (natively) and fills their results."""
+        trace = to_trace(requests)
+        results = np.zeros(len(trace), abi.RESULT_DTYPE)
+        base = self.clock()
+        abi.check(self._lib.bp_server_run_trace(self._handle, abi.ptr(trace), len(trace), abi.ptr(results)),
+                  "bp_server_run_trace")
+        for r, res in zip(requests, results):
+            r.done, r.status, r.substeps, r.bubbles = float(res["completed"]), int(res["status"]), int(res["substeps"]), \
+                int(res["bubbles"])
+        elapsed = float(results["completed"].max() - base) if len(results) else 0.0
+        return ServeStats(results, elapsed, self.stats())
+
+    def submit(self, req: Request, tag: int = 0):
+        rec = to_trace([req])
+        rec[0]["tag"] = tag
+        abi.check(self._lib.bp_server_submit(self._handle, abi.ptr(rec)), "bp_server_submit")
+
+    def poll(self) -> np.ndarray:
+        out = np.zeros(4096, abi.RESULT_DTYPE)
+        n = ctypes.c_uint32(0)
+        abi.check(self._lib.bp_server_poll(self._handle, abi.ptr(out), len(out), ctypes.byref(n)), "bp_server_poll")
+        return out[:n.value].copy()
+
+    def wait_idle(self):
+        abi.check(self._lib.bp_server_wait_idle(self._handle), "bp_server_wait_idle")
+
+    def read_bubbles(self, world: int) -> np.ndarray:
+        n = ctypes.c_uint32(0)
+        abi.check(self._lib.bp_server_read_bubbles(self._handle, world, None, 0, ctypes.byref(n)), "bp_server_read_bubbles")
+        out = np.zeros(n.value, abi.BUBBLE_DTYPE)
+        abi.check(self._lib.bp_server_read_bubbles(self._handle, world, abi.ptr(out), len(out), ctypes.byref(n)),
+                  "bp_server_read_bubbles")
+        return out
+
+    def stats(self) -> dict:
+        s = abi.ServerStats()
+        abi.check(self._lib.bp_server_get_stats(self._handle, ctypes.byref(s)), "bp_server_get_stats")
+        return {name: int(getattr(s, name)) for name, _ in abi.ServerStats._fields_}
+
+    def close(self):
+        """Stops the server; the batch keeps every world's newest state."""
+        if self._handle:
+            self._lib.bp_server_destroy(self._handle)
+            self._handle = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

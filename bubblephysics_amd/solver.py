@@ -174,6 +174,12 @@ class NativeWorld:
                                                     ctypes.byref(total)), "bp_world_read_adjacency")
         return offsets, entries[: total.value]
 
+    def audit(self) -> np.ndarray:
+        """FP64 audit (bp_audit record) of the published state."""
+        out = np.zeros(1, abi.AUDIT_DTYPE)
+        abi.check(self._lib.bp_world_audit(self._handle, abi.ptr(out)), "bp_world_audit")
+        return out[0]
+
     def read_step_history(self) -> np.ndarray:
         out = np.zeros(1024, np.float32)
         count = ctypes.c_uint32(0)
@@ -210,9 +216,28 @@ class Batch:
         return infos.reshape(steps, len(self._counts))
 
     def read_bubbles(self, world: int) -> np.ndarray:
+        self._counts[world] = int(self._lib.bp_batch_bubble_count(self._handle, world))
         out = np.zeros(self._counts[world], abi.BUBBLE_DTYPE)
         abi.check(self._lib.bp_batch_read_bubbles(self._handle, world, abi.ptr(out), len(out)), "bp_batch_read_bubbles")
         return out
+
+    def audit(self) -> np.ndarray:
+        """FP64 audit (bp_audit records) of every world's published state."""
+        out = np.zeros(len(self._counts), abi.AUDIT_DTYPE)
+        abi.check(self._lib.bp_batch_audit(self._handle, abi.ptr(out), len(out)), "bp_batch_audit")
+        return out
+
+    def capture(self, steps: int = 1):
+        """Captures `steps` steps of every world into a hipGraph (bp_batch_capture)."""
+        abi.check(self._lib.bp_batch_capture(self._handle, steps), "bp_batch_capture")
+        self._graph_steps = steps
+
+    def replay(self) -> np.ndarray:
+        """One replay of the captured steps; FrameInfo records shaped (steps, worlds)."""
+        steps = self._graph_steps
+        infos = np.zeros(steps * len(self._counts), abi.FRAME_DTYPE)
+        abi.check(self._lib.bp_batch_replay(self._handle, abi.ptr(infos)), "bp_batch_replay")
+        return infos.reshape(steps, len(self._counts))
 
     def last_class_timing(self):
         """Per kernel class: (device milliseconds, worlds, threads per workgroup, bubble capacity)."""

@@ -209,7 +209,9 @@ typedef struct bp_world_config {
     int64_t injection_object;
     float injection_position[2];
     uint32_t force_cooperative;   /* 1 = multi-workgroup path even for small worlds        */
-    uint32_t reserved;
+    uint32_t export_adjacency;    /* 1 = keep the rows of the last grid build readable after a
+                                     step (adjacency reads, contact energy of the audit); a plain
+                                     bp_world always keeps them                               */
 } bp_world_config;
 
 /* Details of a failure: NativeMetalFailureDetail (NativeSolverInput.swift:280-290). */
@@ -291,6 +293,22 @@ bp_status bp_world_read_adjacency(bp_world *world, uint32_t *row_offsets, uint32
                                   uint32_t *entries, uint32_t entries_capacity, uint32_t *out_entry_count);
 /* h of each substep of the last step (stepHistory), up to 1024 values. */
 bp_status bp_world_read_step_history(bp_world *world, float *out, uint32_t capacity, uint32_t *out_count);
+/* FP64 audit of a world's published state: momentum, angular momentum and kinetic energy of
+ * the movable bubbles, and the elastic energy of the contacts in the rows of the last grid build,
+ * U = K r (eps / (1 - eps) + ln(1 - eps)) per deflected bubble (ContactLawReference.swift:26-31;
+ * a pair of different materials splits its overlap as r_a eps(F/K_a) + r_b eps(F/K_b) = overlap,
+ * :68-82). Sums are FP64 on the device. No reference counterpart: the reference audits energy and
+ * momentum only in its tests (NativeMetalSubstepKernelTests.swift:957-978). */
+typedef struct bp_audit {
+    double momentum[2];         /* sum of m v                                                   */
+    double angular_momentum;    /* sum of m (x cross v), about the origin                        */
+    double kinetic_energy;      /* sum of m |v|^2 / 2                                            */
+    double contact_energy;      /* NaN when the world's rows are not kept (export_adjacency = 0) */
+    uint32_t contacts;          /* contacts with a deflection in contact_energy                  */
+    uint32_t bubbles;
+} bp_audit;
+bp_status bp_world_audit(bp_world *world, bp_audit *out);
+
 /* Test switch mirroring the injected NativeMetalCommandExecuting of the reference
  * (NativeMetalSolverLifecycleTests.swift:292-308): the next step fails with BP_ERR_COMMAND_FAILED
  * before touching the device. */
@@ -344,10 +362,20 @@ void bp_batch_destroy(bp_batch *batch);
 /* Adds a world; returns its index. Worlds are packed on the next step. */
 bp_status bp_batch_add_world(bp_batch *batch, const bp_batch_world_desc *desc, uint32_t *out_index);
 uint32_t bp_batch_world_count(const bp_batch *batch);
+/* Bubbles of one world (transactions applied by a bp_server may have changed it). */
+uint32_t bp_batch_bubble_count(const bp_batch *batch, uint32_t world);
 /* Advances every world by `steps` steps; infos[world_count * steps] (step-major) may be null.
  * A failed world stops at its failing step and keeps its last published state. */
 bp_status bp_batch_step(bp_batch *batch, uint32_t steps, bp_frame_info *infos);
 bp_status bp_batch_read_bubbles(bp_batch *batch, uint32_t world, bp_bubble *out, uint32_t count);
+/* Captures `steps` steps of every resident world (the launches of every kernel class on the
+ * batch's streams, forked from and joined back to one stream) into a hipGraph, replayed by
+ * bp_batch_replay with one graph launch. A change of the batch's layout (new worlds, residency,
+ * a world moving to another kernel class) drops the graph; replay then returns
+ * BP_ERR_INVALID_ARGUMENT until the next capture. */
+bp_status bp_batch_capture(bp_batch *batch, uint32_t steps);
+/* One replay of the captured steps; infos[world_count * steps] (step-major) may be null. */
+bp_status bp_batch_replay(bp_batch *batch, bp_frame_info *infos);
 /* Device time of the last bp_batch_step (HIP events around the launches). */
 float bp_batch_last_gpu_milliseconds(const bp_batch *batch);
 /* Kernel configurations ("classes") a world can run in: LDS-resident configurations for small
@@ -357,6 +385,27 @@ float bp_batch_last_gpu_milliseconds(const bp_batch *batch);
 uint32_t bp_batch_class_count(void);
 bp_status bp_batch_last_class_timing(const bp_batch *batch, float *milliseconds, uint32_t *worlds, uint32_t *threads,
                                      uint32_t *capacity);
+/* FP64 audit (bp_world_audit) of every world: out[world_count]. */
+bp_status bp_batch_audit(bp_batch *batch, bp_audit *out, uint32_t count);
+/* Autotuner: for each bin of world sizes (bin b: 2^b to 2^(b+1) - 1 bubbles), times one step of
+ * the bin's worlds in every kernel configuration that holds them all (the LDS-resident
+ * configurations and the one-workgroup global-state kernel: workgroup shape and the LDS-to-HBM
+ * cutover), then the best configuration with half and twice the automatic grid bucket count.
+ * Every trial starts from the batch's published state, which the call leaves unchanged; the
+ * batch adopts the choices (the next step repacks). Results do not depend on the choices.
+ * No reference counterpart (its single-dispatch path has a fixed width, NativeMetalBackend.swift:17-21). */
+typedef struct bp_tune_bin {
+    uint32_t min_bubbles, max_bubbles;
+    uint32_t worlds;               /* worlds of the batch in the bin                              */
+    int32_t best_class;            /* chosen kernel configuration, -1 = none                       */
+    int32_t best_bucket_shift;     /* grid buckets: -1 half, 0 automatic, +1 twice the automatic count */
+    float best_rate;               /* bubble-substeps/s of the bin's trial step with the choices   */
+    float class_rate[12];          /* per kernel configuration; 0 = not tried (does not hold the bin) */
+    float bucket_rate[2];          /* best configuration with half and with twice the buckets      */
+} bp_tune_bin;
+bp_status bp_batch_autotune(bp_batch *batch, uint32_t repeats, bp_tune_bin *out, uint32_t capacity, uint32_t *out_count);
+/* Applies a table of bp_batch_autotune (for instance one cached for this GPU) without trials. */
+bp_status bp_batch_apply_tuning(bp_batch *batch, const bp_tune_bin *bins, uint32_t count);
 /* Pins a world to a class (-1 = automatic); used by the autotuner. */
 bp_status bp_batch_set_world_class(bp_batch *batch, uint32_t world, int32_t cls);
 
@@ -380,6 +429,97 @@ bp_status bp_batch_write_bubbles(bp_batch *batch, uint32_t world, const bp_bubbl
 bp_status bp_batch_set_resident(bp_batch *batch, uint32_t world, uint32_t resident);
 /* Device bytes a world holds (or would hold) while resident. */
 bp_status bp_batch_world_footprint(const bp_batch *batch, uint32_t world, uint64_t *bytes);
+
+/* ------------------------------------------------------------------------ */
+/* Server: a native request loop over the worlds of a batch. No reference     */
+/* counterpart: the reference steps one world per call from its game loop     */
+/* (BubbleWorld.submit / advance, BubbleWorld.swift:252-283).                  */
+/* A dispatcher thread takes step requests as they arrive and launches, per   */
+/* kernel class, the waiting worlds that are not already on the device;       */
+/* launches of different classes and consecutive launches of one class run    */
+/* concurrently on several streams. Every world keeps its state on the device; */
+/* worlds are admitted against a device-memory budget and the least recently  */
+/* used idle worlds are evicted to host memory when an admission needs room.  */
+/* ------------------------------------------------------------------------ */
+typedef struct bp_server bp_server;
+
+/* Transaction carried by a request, applied on the device before the step. The bubble it
+ * names is `index % bubble count`. Structural changes keep ids sorted (new ids are the largest
+ * id + 1) and keep mass, momentum and area. */
+enum {
+    BP_TX_NONE = 0,
+    BP_TX_IMPULSE = 1,  /* v_i += impulse (as J = impulse * m_i, applied as J / m_i); movable bubbles only */
+    BP_TX_ADD = 2,      /* copy of bubble i at c_i + 2 r_i * direction, new id                        */
+    BP_TX_REMOVE = 3,   /* bubble i leaves (not the last bubble of a world)                            */
+    BP_TX_MERGE = 4,    /* bubble i and its index neighbour (i + 1, or i - 1 for the last) become one  */
+    BP_TX_SPLIT = 5     /* bubble i becomes two halves at c_i +- (r_i / sqrt 2) * direction            */
+};
+
+typedef struct bp_request {
+    double arrival;             /* bp_server_run_trace: seconds after the start of the trace  */
+    uint64_t tag;               /* caller's id, returned with the result                       */
+    uint32_t world;
+    uint32_t transaction;       /* BP_TX_*                                                     */
+    uint32_t index;             /* bubble of the transaction: index % bubble count             */
+    float impulse[2];           /* BP_TX_IMPULSE                                               */
+    float direction[2];         /* BP_TX_ADD, BP_TX_SPLIT: unit vector                         */
+    uint32_t reserved;
+} bp_request;
+
+typedef struct bp_server_result {
+    uint64_t tag;
+    uint32_t world;
+    bp_status status;           /* status of the step; BP_ERR_CAPACITY_OVERFLOW when a transaction
+                                   needs more bubble slots than the world was packed with (no step);
+                                   BP_ERR_UNAVAILABLE for a world that failed an earlier step      */
+    uint32_t substeps;
+    uint32_t bubbles;           /* bubbles of the world in the step                            */
+    double arrival;             /* seconds on the server clock (bp_server_clock)               */
+    double completed;           /* seconds on the server clock when the result reached the host */
+    uint32_t launch_worlds;     /* worlds in the launch that carried the step                  */
+    uint32_t kernel_class;
+    float device_milliseconds;  /* time the world's workgroup spent on the step                */
+    uint32_t reserved;
+} bp_server_result;
+
+typedef struct bp_server_config {
+    uint64_t memory_budget;         /* device bytes of resident worlds; 0 = no limit           */
+    uint32_t streams;               /* HIP streams the launches spread over (default 8)         */
+    uint32_t max_inflight_per_class;/* launches of one kernel class in flight at once (default 2) */
+    uint32_t max_worlds_per_launch; /* default 1024                                             */
+    uint32_t big_world;             /* worlds above this many bubbles run in the one-workgroup
+                                       global-state kernel (default 4096)                       */
+} bp_server_config;
+
+typedef struct bp_server_stats {
+    uint64_t submitted, completed, launches, launched_worlds, transactions, rejected;
+    uint64_t evictions, admissions, repacks;
+    uint64_t resident_bytes;
+} bp_server_stats;
+
+/* sizeof of bp_request, bp_server_result, bp_server_config, bp_server_stats, in that order. */
+void bp_server_record_sizes(uint32_t *out4);
+void bp_server_config_default(bp_server_config *config);
+/* Starts a server over every world of `batch`. While it lives the batch's own entry points
+ * return BP_ERR_FRAME_IN_FLIGHT. */
+bp_status bp_server_create(bp_batch *batch, const bp_server_config *config, bp_server **out_server);
+/* Finishes the launches in flight (requests not yet launched are dropped), stops the
+ * dispatcher and hands the batch back with every world's newest state. */
+void bp_server_destroy(bp_server *server);
+/* Queues one step request (non-blocking, thread-safe); its arrival time is the call time. */
+bp_status bp_server_submit(bp_server *server, const bp_request *request);
+/* Results completed since the last poll (at most `capacity`). */
+bp_status bp_server_poll(bp_server *server, bp_server_result *out, uint32_t capacity, uint32_t *out_count);
+/* Blocks until every submitted request has a result. */
+bp_status bp_server_wait_idle(bp_server *server);
+/* Replays a trace: submits trace[i] at its arrival time (seconds after the call) and returns
+ * when all have completed; results[i] belongs to trace[i] (and is not returned by poll). */
+bp_status bp_server_run_trace(bp_server *server, const bp_request *trace, uint32_t count, bp_server_result *results);
+/* A world's bubbles (state and properties) while the server is idle (after wait_idle). */
+bp_status bp_server_read_bubbles(bp_server *server, uint32_t world, bp_bubble *out, uint32_t capacity, uint32_t *out_count);
+bp_status bp_server_get_stats(bp_server *server, bp_server_stats *out);
+/* Seconds since the server started (the clock of bp_server_result). */
+double bp_server_clock(const bp_server *server);
 
 #ifdef __cplusplus
 }

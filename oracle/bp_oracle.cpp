@@ -1307,6 +1307,81 @@ int bpo_world_run(const bp_world_config *config, bp_bubble *bubbles, uint32_t n,
     return run(w);
 }
 
+// FP64 audit of a world state by brute force over every pair, segment and body: momentum,
+// angular momentum, kinetic energy and the elastic energy of every contact the step's force
+// law acts on. The contact energy per deflected bubble is U = K r (eps/(1-eps) + ln(1-eps))
+// (ContactLawReference.swift:26-31); a pair of different materials splits its overlap by
+// r_a eps(F/K_a) + r_b eps(F/K_b) = overlap, solved for F by bisection (:68-82). Distances are
+// the FP32 ones of the step's contact law (pair centre distance in FP64 from FP32 centres);
+// which pairs, segments and bodies count follows the candidate predicates (filters, body
+// kinds: Geometry:262-298, 350-392).
+namespace {
+double audit_epsilon(double q) { return q > 0 ? 2 * q / ((2 * q + 1) + std::sqrt(4 * q + 1)) : 0.0; }
+double audit_spring(double eps, double K, double r) { return eps > 0 ? K * r * (eps / (1 - eps) + std::log1p(-eps)) : 0.0; }
+double audit_wall(double K, double r, double d, uint32_t &contacts) {
+    double dd = std::fmax(d, r * 1e-6);
+    if (!(K > 0) || !(r > 0) || !(dd < r)) return 0.0;
+    ++contacts;
+    return audit_spring((r - dd) / r, K, r);
+}
+double audit_pair(double Ka, double ra, double Kb, double rb, double d, uint32_t &contacts) {
+    double R = ra + rb, dd = std::fmax(d, R * 1e-6), overlap = R - dd;
+    if (!(Ka > 0) || !(Kb > 0) || !(overlap > 0)) return 0.0;
+    ++contacts;
+    if (Ka == Kb) return audit_spring(overlap / R, Ka, R);
+    double lo = 0, hi = std::fmax(Ka, Kb);
+    while (ra * audit_epsilon(hi / Ka) + rb * audit_epsilon(hi / Kb) < overlap) hi *= 2;
+    for (int k = 0; k < 100; ++k) {
+        double mid = 0.5 * (lo + hi);
+        if (ra * audit_epsilon(mid / Ka) + rb * audit_epsilon(mid / Kb) < overlap) lo = mid; else hi = mid;
+    }
+    double f = 0.5 * (lo + hi);
+    return audit_spring(audit_epsilon(f / Ka), Ka, ra) + audit_spring(audit_epsilon(f / Kb), Kb, rb);
+}
+}  // namespace
+
+void bpo_audit(const bp_world_config *config, const bp_bubble *bubbles, uint32_t n, const bp_segment *segments, uint32_t ns,
+               const bp_body *bodies, uint32_t nb, const float *vertices, uint32_t nv, const uint32_t *triangles, uint32_t nt,
+               bp_audit *out) {
+    bpo::World<float> w;
+    bpo::load(w, config, bubbles, n, segments, ns, bodies, nb, vertices, nv, triangles, nt, nullptr, 0, nullptr, 0, nullptr, 0,
+              nullptr, 0, nullptr, 0);
+    double px = 0, py = 0, ang = 0, kin = 0, pot = 0;
+    uint32_t contacts = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        const auto &a = w.b[i];
+        if (!w.is_static(a)) {
+            double m = a.mass, vx = a.vel.x, vy = a.vel.y;
+            px += m * vx; py += m * vy;
+            ang += m * ((double)a.pos.x * vy - (double)a.pos.y * vx);
+            kin += 0.5 * m * (vx * vx + vy * vy);
+        }
+        if (!bpo::finite2(a.pos)) continue;
+        for (uint32_t j = i + 1; j < n; ++j) {
+            const auto &o = w.b[j];
+            if (!bpo::finite2(o.pos) || !bpo::filter_accepts(a.cat, a.mask, o.cat, o.mask)) continue;
+            double dx = (double)o.pos.x - (double)a.pos.x, dy = (double)o.pos.y - (double)a.pos.y;
+            pot += audit_pair(a.Kc, a.radius, o.Kc, o.radius, std::sqrt(dx * dx + dy * dy), contacts);
+        }
+        for (uint32_t j = 0; j < ns; ++j) {
+            const auto &s = w.s[j];
+            if (!bpo::filter_accepts(a.cat, a.mask, s.cat, s.mask)) continue;
+            bpo::V2<float> normal; float d;
+            if (!bpo::wall_region(s, a.pos, normal, d)) continue;
+            pot += audit_wall(a.Kc, a.radius, d, contacts);
+        }
+        for (uint32_t k = 0; k < nb; ++k) {
+            const auto &body = w.bodies[k];
+            bool obstacle = body.isStatic ? bpo::rigid_is_round(body) : (body.kinematic != 0);
+            if (!obstacle || !bpo::filter_accepts(a.cat, a.mask, body.cat, body.mask)) continue;
+            auto ob = bpo::body_obstacle(body, 1.0f, w.verts, w.tris, a.pos);
+            pot += audit_wall(a.Kc, a.radius, ob.distanceValue, contacts);
+        }
+    }
+    out->momentum[0] = px; out->momentum[1] = py; out->angular_momentum = ang; out->kinetic_energy = kin;
+    out->contact_energy = pot; out->contacts = contacts; out->bubbles = n;
+}
+
 // Candidate rows of one grid build with a given cell width and margins (the kernel
 // nativeEmitCandidates, Geometry:300-340, and its hashed twin, Force:1010-1035).
 // rows[n * limit * 2] receives (kind, other index) per slot, row_counts[2n] the bubble and

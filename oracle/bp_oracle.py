@@ -51,6 +51,7 @@ def load():
         lib.bpo_pair_law_f32.argtypes = [P, P, P, U32]
         lib.bpo_half_impulse_factor_f32.argtypes = [P, P, P, U32]
         lib.bpo_detmath_f32.argtypes = [ctypes.c_int, P, P, U32]
+        lib.bpo_audit.argtypes = [P, P, U32, P, U32, P, U32, P, U32, P, U32, P]
         _lib = lib
     return _lib
 
@@ -103,6 +104,25 @@ def run(scene, steps: int | None = None, *, precision: int = 32, assert_mode: in
         extras["terms"] = terms[:count].copy()
         extras["objects"] = objects[:count].copy()
     return bubbles, bodies, frames[:executed], extras
+
+
+AUDIT_FIELDS = [("momentum", np.float64, (2,)), ("angular_momentum", np.float64), ("kinetic_energy", np.float64),
+                ("contact_energy", np.float64), ("contacts", np.uint32), ("bubbles", np.uint32)]
+AUDIT_DTYPE = np.dtype(AUDIT_FIELDS)
+
+
+def audit(scene, bubbles=None, bodies=None):
+    """FP64 audit by brute force of a world state (the scene's records, or the given bubbles and
+    bodies on the scene's geometry): momentum, angular momentum, kinetic and contact energy."""
+    lib = load()
+    b = np.ascontiguousarray(scene.bubbles if bubbles is None else bubbles)
+    bod = np.ascontiguousarray(scene.bodies if bodies is None else bodies)
+    verts = np.ascontiguousarray(scene.vertices, np.float32)
+    tris = np.ascontiguousarray(scene.triangles, np.uint32)
+    out = np.zeros(1, AUDIT_DTYPE)
+    lib.bpo_audit(ctypes.byref(scene.config), _p(b), len(b), _p(scene.segments), len(scene.segments), _p(bod), len(bod),
+                  _p(verts), len(verts), _p(tris), len(tris), _p(out))
+    return out[0]
 
 
 def noncontact_acceleration(scene):

@@ -37,6 +37,16 @@ def test_record_layouts_match_the_header(hip_library):
     assert list(sizes) == expected
     assert abi.BUBBLE_DTYPE.itemsize == 72 and abi.SEGMENT_DTYPE.itemsize == 64 and abi.BODY_DTYPE.itemsize == 96
     assert abi.FIELD_DTYPE.itemsize == 64
+    server = (ctypes.c_uint32 * 4)()
+    hip_library.bp_server_record_sizes(server)
+    assert list(server) == [ctypes.sizeof(t) for t in (abi.Request, abi.ServerResult, abi.ServerConfig, abi.ServerStats)]
+    assert abi.REQUEST_DTYPE.itemsize == 48 and abi.RESULT_DTYPE.itemsize == 56
+
+
+def test_server_defaults(hip_library):
+    cfg = abi.ServerConfig()
+    hip_library.bp_server_config_default(ctypes.byref(cfg))
+    assert cfg.memory_budget == 0 and cfg.streams >= 1 and cfg.max_inflight_per_class >= 1 and cfg.big_world == 4096
 
 
 def test_defaults_are_the_reference_defaults(hip_library):

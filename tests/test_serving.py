@@ -1,58 +1,89 @@
-"""Serving: subset launches, asynchronous completion, transactions, residency and continuous
-batching. The GPU tests hold the served worlds to the oracle and to the same world served alone."""
+"""Serving: subset launches, asynchronous completion, device-side transactions, residency and the
+native continuous-batching server. The GPU tests hold served worlds to the oracle (after the
+host-side statement of each transaction) and to the same world served alone."""
 import numpy as np
 import pytest
 
 from bubblephysics_amd import abi, scenes, serving, solver
 
 STATE = ("center", "previous_center", "velocity", "rotation", "angular_velocity")
+PROPS = ("id", "mass", "inverse_mass", "radius", "contact_stiffness", "damping", "flags", "category", "mask")
 
 
-def same_bits(a, b):
-    return all(np.array_equal(np.ascontiguousarray(a[f]).view(np.uint32), np.ascontiguousarray(b[f]).view(np.uint32))
-               for f in STATE) and np.array_equal(a["id"], b["id"])
+def same_bits(a, b, props=False):
+    if len(a) != len(b):
+        return False
+    fields = STATE + (PROPS if props else ("id",))
+    return all(np.array_equal(np.ascontiguousarray(a[f]).view(np.uint8), np.ascontiguousarray(b[f]).view(np.uint8))
+               for f in fields)
 
 
-def worlds(seed, sizes):
+def worlds(seed, sizes, bodies_max=4):
     rng = np.random.Generator(np.random.PCG64(seed))
-    return [scenes.batch_world(rng, n, bodies_max=4) for n in sizes]
+    return [scenes.batch_world(rng, n, bodies_max=bodies_max) for n in sizes]
 
 
-# ---- CPU: request traces and transactions -------------------------------------------------
+def request(world, kind, seed, arrival=0.0):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    return serving.Request(arrival, world, **serving.transaction_params(rng, kind))
+
+
+# ---- CPU: request traces and the host-side statement of the transactions ------------------
 def test_poisson_trace_is_seeded_and_has_the_rate():
     a = serving.poisson_requests(100, 2000.0, 2.0, seed=3)
     b = serving.poisson_requests(100, 2000.0, 2.0, seed=3)
-    assert [(r.arrival, r.world, r.transaction, r.seed) for r in a] == [(r.arrival, r.world, r.transaction, r.seed) for r in b]
+    key = lambda r: (r.arrival, r.world, r.transaction, r.index, r.impulse, r.direction)
+    assert [key(r) for r in a] == [key(r) for r in b]
     assert abs(len(a) - 4000) < 4 * np.sqrt(4000)
     assert all(x.arrival <= y.arrival for x, y in zip(a, a[1:]))
     frac = np.mean([r.transaction is not None for r in a])
     assert 0.07 < frac < 0.13
+    for r in a:
+        assert abs(np.hypot(*r.direction) - 1.0) < 1e-6
+    trace = serving.to_trace(a)
+    assert trace.dtype == abi.REQUEST_DTYPE and np.array_equal(trace["tag"], np.arange(len(a)))
 
 
 @pytest.mark.parametrize("kind", serving.KINDS)
 def test_transactions_keep_ids_sorted_and_conserve_what_they_should(kind):
     b = scenes.make_bubbles([(0, 0), (30, 0), (60, 0), (90, 5)], [10, 12, 8, 9], [(1, 2), (3, 4), (5, 6), (7, 8)])
-    out = serving.apply_transaction(b, kind, seed=11)
-    assert np.all(np.diff(out["id"]) > 0)
-    mv = lambda x: (x["mass"][:, None].astype(np.float64) * x["velocity"].astype(np.float64)).sum(axis=0)
-    if kind in ("merge", "split"):
-        # Mass, momentum and area are kept (StructuralOperations.swift:10-151).
-        assert np.isclose(out["mass"].sum(dtype=np.float64), b["mass"].sum(dtype=np.float64), rtol=1e-6)
-        assert np.allclose(mv(out), mv(b), rtol=1e-5)
-        assert np.isclose((out["radius"].astype(np.float64) ** 2).sum(), (b["radius"].astype(np.float64) ** 2).sum(), rtol=1e-5)
-    if kind == "merge":
-        assert len(out) == len(b) - 1
-    if kind == "split":
-        assert len(out) == len(b) + 1
-    if kind == "add":
-        assert len(out) == len(b) + 1 and out["id"][-1] == b["id"].max() + 1
-    if kind == "remove":
-        assert len(out) == len(b) - 1
-    if kind == "impulse":
-        assert len(out) == len(b) and (out["velocity"] != b["velocity"]).sum() >= 1
+    for seed in range(6):
+        req = request(0, kind, seed)
+        out = serving.apply_transaction(b, req)
+        assert np.all(np.diff(out["id"]) > 0)
+        mv = lambda x: (x["mass"][:, None].astype(np.float64) * x["velocity"].astype(np.float64)).sum(axis=0)
+        i = req.index % len(b)
+        if kind in ("merge", "split"):
+            # Mass, momentum and area are kept (StructuralOperations.swift:10-151).
+            assert np.isclose(out["mass"].sum(dtype=np.float64), b["mass"].sum(dtype=np.float64), rtol=1e-6)
+            assert np.allclose(mv(out), mv(b), rtol=1e-5)
+            assert np.isclose((out["radius"].astype(np.float64) ** 2).sum(), (b["radius"].astype(np.float64) ** 2).sum(), rtol=1e-5)
+        if kind == "merge":
+            k = i + 1 if i + 1 < len(b) else i - 1
+            assert len(out) == len(b) - 1 and b["id"][max(i, k)] not in out["id"]
+        if kind == "split":
+            assert len(out) == len(b) + 1 and out["id"][-1] == b["id"].max() + 1
+            # The halves sit symmetrically about the old centre, one half-radius away.
+            c = b["center"][i].astype(np.float64)
+            d1, d2 = out["center"][i] - c, out["center"][-1] - c
+            assert np.allclose(d1, -d2, atol=1e-4) and np.isclose(np.hypot(*d1), b["radius"][i] / np.sqrt(2), rtol=1e-5)
+        if kind == "add":
+            assert len(out) == len(b) + 1 and out["id"][-1] == b["id"].max() + 1
+            assert np.isclose(np.hypot(*(out["center"][-1] - b["center"][i])), 2 * b["radius"][i], rtol=1e-5)
+        if kind == "remove":
+            assert len(out) == len(b) - 1 and b["id"][i] not in out["id"]
+        if kind == "impulse":
+            dv = out["velocity"][i] - b["velocity"][i]
+            assert len(out) == len(b) and np.allclose(dv, req.impulse, rtol=1e-5, atol=1e-4)
 
 
-# ---- GPU ------------------------------------------------------------------------------
+def test_transactions_leave_a_single_bubble_world_whole():
+    b = scenes.make_bubbles([(0, 0)], [10], [(1, 2)])
+    for kind in ("remove", "merge"):
+        assert len(serving.apply_transaction(b, request(0, kind, 1))) == 1
+
+
+# ---- GPU: batch subset launches -------------------------------------------------------------
 @pytest.mark.gpu
 def test_listed_worlds_advance_and_the_others_do_not(oracle):
     ws = worlds(21, [40, 300, 70, 1200, 20])
@@ -83,23 +114,6 @@ def test_submitted_step_is_collected_by_polling(oracle):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("kind", serving.KINDS)
-def test_transaction_between_steps_matches_oracle(oracle, kind):
-    ws = worlds(23, [60, 900])
-    batch = solver.Batch(ws)
-    batch.step(1)
-    cur = ws[1].copy()
-    cur.bubbles, _, _, _ = oracle.run(ws[1], 1)
-    changed = serving.apply_transaction(batch.read_bubbles(1), kind, seed=5)
-    batch.write_bubbles(1, changed)
-    cur.bubbles = changed
-    infos = batch.step_worlds([1], 2)
-    cb, _, cf, _ = oracle.run(cur, 2)
-    assert np.array_equal(infos[:, 0]["substep_count"], cf["substeps"])
-    assert same_bits(batch.read_bubbles(1), cb)
-
-
-@pytest.mark.gpu
 def test_evicted_world_keeps_its_state_and_resumes(oracle):
     ws = worlds(24, [80, 400, 30])
     batch = solver.Batch(ws)
@@ -118,26 +132,108 @@ def test_evicted_world_keeps_its_state_and_resumes(oracle):
     assert same_bits(batch.read_bubbles(1), cb)
 
 
+# ---- GPU: the native server -------------------------------------------------------------------
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", serving.KINDS)
+def test_served_transaction_and_step_match_the_oracle(oracle, kind):
+    # A step, then a served request carrying the transaction, then a plain served request: the
+    # device transaction kernel must equal the host statement, and the steps the oracle's.
+    ws = worlds(23, [60, 900])
+    batch = solver.Batch(ws)
+    batch.step(1)
+    with serving.Server(batch) as server:
+        req = request(1, kind, 5)
+        server.submit(req, tag=7)
+        server.wait_idle()
+        r = server.poll()
+        assert len(r) == 1 and r[0]["tag"] == 7 and r[0]["world"] == 1
+        cur = ws[1].copy()
+        cur.bubbles, _, _, _ = oracle.run(ws[1], 1)
+        cur.bubbles = serving.apply_transaction(cur.bubbles, req)
+        assert r[0]["bubbles"] == len(cur.bubbles)
+        cb, _, cf, _ = oracle.run(cur, 1)
+        assert r[0]["status"] == cf[0]["status"] and r[0]["substeps"] == cf[0]["substeps"]
+        assert same_bits(server.read_bubbles(1), cb, props=True)
+        server.submit(serving.Request(0.0, 1), tag=8)
+        server.wait_idle()
+        r = server.poll()
+        cur.bubbles = cb
+        cb, _, cf, _ = oracle.run(cur, 1)
+        assert r[0]["substeps"] == cf[0]["substeps"]
+        assert same_bits(server.read_bubbles(1), cb, props=True)
+    # The batch takes the worlds back with their new counts.
+    assert same_bits(batch.read_bubbles(1), cb, props=True)
+    cur.bubbles = cb
+    cb, _, cf, _ = oracle.run(cur, 1)
+    assert batch.step(1)[0, 1]["substep_count"] == cf[0]["substeps"]
+    assert same_bits(batch.read_bubbles(1), cb, props=True)
+
+
+@pytest.mark.gpu
+def test_growing_world_moves_to_a_wider_kernel_class(oracle):
+    # 64 bubbles fill the narrowest configuration (64 threads x 1); an add moves the world up.
+    rng = np.random.Generator(np.random.PCG64(31))
+    w = scenes.batch_world(rng, 64, bodies_max=0, packing=(0.3, 0.3))
+    assert len(w.bubbles) == 64
+    with serving.Server([w]) as server:
+        server.submit(serving.Request(0.0, 0), tag=0)
+        req = request(0, "add", 3)
+        server.submit(req, tag=1)
+        server.wait_idle()
+        r = np.sort(server.poll(), order="tag")
+        assert list(r["bubbles"]) == [64, 65] and r[0]["kernel_class"] != r[1]["kernel_class"]
+        cur = w.copy()
+        cur.bubbles, _, _, _ = oracle.run(w, 1)
+        cur.bubbles = serving.apply_transaction(cur.bubbles, req)
+        cb, _, _, _ = oracle.run(cur, 1)
+        assert same_bits(server.read_bubbles(0), cb, props=True)
+
+
+@pytest.mark.gpu
+def test_transaction_beyond_the_packed_slots_is_rejected():
+    ws = worlds(26, [16])
+    n = len(ws[0].bubbles)
+    slots = n + max(8, n // 8)
+    with serving.Server(ws) as server:
+        for k in range(slots - n + 2):
+            server.submit(request(0, "add", 100 + k), tag=k)
+        server.wait_idle()
+        r = np.sort(server.poll(), order="tag")
+        grown = r["status"] != abi.BP_ERR_CAPACITY_OVERFLOW
+        assert grown.sum() == slots - n and not grown[-2:].any()
+        assert server.stats()["rejected"] == 2
+        assert len(server.read_bubbles(0)) == slots
+
+
 @pytest.mark.gpu
 def test_each_served_world_gets_the_result_it_gets_alone():
     # Continuous batching with a budget that forces evictions: every world ends in the same
-    # state as when its own requests (and transactions) are replayed on it alone.
-    ws = worlds(25, [30, 200, 60, 700, 45, 120])
-    requests = serving.poisson_requests(len(ws), 400.0, 0.25, seed=9, transaction_fraction=0.3)
+    # state as when its own requests (and transactions) are replayed on it alone, with the
+    # transactions applied on the host by their statement in numpy.
+    ws = worlds(25, [30, 200, 60, 700, 45, 120, 2500])
+    requests = serving.poisson_requests(len(ws), 600.0, 0.4, seed=9, transaction_fraction=0.3)
     probe = solver.Batch(ws)
     budget = sum(sorted(probe.footprint(w) for w in range(len(ws)))[-3:])   # about three worlds at a time
     probe.close()
     server = serving.Server(ws, memory_budget=budget)
-    order = {w: [] for w in range(len(ws))}
-    stats = server.run(requests, on_done=lambda r: order[r.world].append(r))
-    assert stats.served == len(requests) and stats.evictions > 0
-    for w, served in order.items():
-        if not served:
-            continue
+    stats = server.run(requests)
+    counters = server.stats()
+    assert counters["completed"] == len(requests) and counters["evictions"] > 0 and counters["repacks"] > 0
+    final = []
+    for w in range(len(ws)):
+        final.append(server.read_bubbles(w))
+    server.close()
+    s = stats.summary()
+    assert s["served"] == len(requests) and s["p99_ms"] >= s["p50_ms"] > 0
+    assert np.all(stats.results["completed"] >= stats.results["arrival"])
+    for w in range(len(ws)):
+        served = sorted((r for r in requests if r.world == w), key=lambda r: r.done)
         alone = solver.Batch([ws[w]])
-        for r in sorted(served, key=lambda r: r.done):
+        for r in served:
+            if r.status == abi.BP_ERR_CAPACITY_OVERFLOW:    # refused by the server: no transaction, no step
+                continue
             if r.transaction:
-                alone.write_bubbles(0, serving.apply_transaction(alone.read_bubbles(0), r.transaction, r.seed))
-            alone.step_worlds([0], 1)
-        server.batch.set_resident(w, True)
-        assert same_bits(server.batch.read_bubbles(w), alone.read_bubbles(0)), w
+                alone.write_bubbles(0, serving.apply_transaction(alone.read_bubbles(0), r))
+            info = alone.step_worlds([0], 1)[0, 0]
+            assert info["status"] == r.status and info["substep_count"] == r.substeps, (w, info["status"], r.status)
+        assert same_bits(final[w], alone.read_bubbles(0), props=True), w
