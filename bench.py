@@ -25,6 +25,13 @@ import numpy as np
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
+# Hardware queues of this process, read by the HIP runtime when it starts (before any device
+# call below). The batched step runs each kernel class on its own stream and the server keeps one
+# launch on each of its 32 streams; with HIP's default of 4 queues those streams would share
+# queues and their launches would run one after another.
+HW_QUEUES = 32
+os.environ["GPU_MAX_HW_QUEUES"] = str(HW_QUEUES)
+
 HBM_PEAK_GBPS = 8000.0
 
 
@@ -228,8 +235,8 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=30.0)
     ap.add_argument("--serve-rate", type=float, default=20000.0)
     ap.add_argument("--serve-worlds", type=int, default=8192)
-    ap.add_argument("--serve-streams", type=int, default=8)
-    ap.add_argument("--serve-inflight", type=int, default=2)
+    ap.add_argument("--serve-streams", type=int, default=32)
+    ap.add_argument("--serve-inflight", type=int, default=8)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=min(os.cpu_count() or 1, 16))
     ap.add_argument("--no-cpu", action="store_true")

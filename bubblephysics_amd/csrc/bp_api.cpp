@@ -261,7 +261,11 @@ bp_status bp_batch::pack() {
         of += (uint32_t)w.fields.size(); ofv += (uint32_t)(w.fieldVertices.size() / 2); ofi += (uint32_t)w.fieldIds.size();
         oadj += (uint64_t)d.adjCapacity;
     }
-    // Longest-processing-time order inside each class: the largest worlds start first.
+    // Longest-processing-time order inside each class: the largest worlds start first (until a
+    // step has measured each world's substeps, see launch()).
+    workEstimate.assign(worlds.size(), 0);
+    for (size_t wi = 0; wi < worlds.size(); ++wi) workEstimate[wi] = hostDescs[wi].bubbleCount;
+    orderDirty = false;
     for (auto &k : classWorlds)
         std::stable_sort(k.begin(), k.end(), [&](uint32_t a, uint32_t b) { return hostDescs[a].bubbleCount > hostDescs[b].bubbleCount; });
     if ((s = upload(A.pos[0], pos)) || (s = upload(A.start[0], start)) || (s = upload(A.vel[0], vel)) ||
@@ -534,6 +538,18 @@ bp_status bp_batch::launch(const uint32_t *subset, uint32_t subsetCount, uint32_
     } else {
         for (size_t wi = 0; wi < W; ++wi) pendingList.push_back((uint32_t)wi);
         for (int k = 0; k < kClassCount; ++k) { launchOffset[k] = 0; launchCount[k] = (uint32_t)classWorlds[k].size(); }
+        // Longest-processing-time order by measured work (bubbles x substeps of the last step):
+        // a small world with hundreds of substeps per step starts first instead of last.
+        if (orderDirty) {
+            for (int k = 0; k < kClassCount; ++k) {
+                auto &list = classWorlds[k];
+                if (list.size() < 2) continue;
+                std::stable_sort(list.begin(), list.end(), [&](uint32_t a, uint32_t b) { return workEstimate[a] > workEstimate[b]; });
+                bp_status s = upload(orders[k], list);
+                if (s != BP_OK) return s;
+            }
+            orderDirty = false;
+        }
     }
     auto orderOf = [&](int k) -> const uint32_t * { return subset ? subsetOrders + launchOffset[k] : orders[k]; };
     auto worldOf = [&](int k, uint32_t j) -> uint32_t { return subset ? subsetHost[launchOffset[k] + j] : classWorlds[k][j]; };
@@ -627,6 +643,16 @@ bp_status bp_batch::collect(uint32_t steps, bp_frame_info *infos, const std::vec
     const size_t need = (size_t)steps * W;
     hostStats.resize(need);
     BP_HIP(hipMemcpy(hostStats.data(), stats, need * sizeof(bpd::WorldStatus), hipMemcpyDeviceToHost));
+    if (workEstimate.size() == W)
+        for (uint32_t wi : listedWorlds) {
+            uint64_t sub = 0;
+            for (uint32_t st = 0; st < steps; ++st) {
+                const bpd::WorldStatus &r = hostStats[(size_t)st * W + wi];
+                if (r.executed) sub += r.substeps;
+            }
+            const uint64_t est = (uint64_t)hostDescs[wi].bubbleCount * std::max<uint64_t>(1, sub / std::max(steps, 1u));
+            if (est != workEstimate[wi]) { workEstimate[wi] = est; orderDirty = true; }
+        }
     if (!infos) return BP_OK;
     const size_t listed = listedWorlds.size();
     for (uint32_t st = 0; st < steps; ++st)

@@ -53,7 +53,7 @@ constexpr int kClassCount = 12;              // ten LDS configurations, the glob
 constexpr int kGlobalClass = 10;
 constexpr int kCoopClass = 11;
 constexpr uint32_t kLdsLargest = 4096;       // bubbles of the widest LDS configuration
-constexpr int kStreams = 4;
+constexpr int kStreams = kClassCount;          // one stream per kernel class: classes never queue behind each other
 
 inline uint32_t class_threads(int k) {
     if (k == kGlobalClass) return 1024;
@@ -146,6 +146,8 @@ struct bp_batch {
     uint32_t *orders[bph::kClassCount] = {};
     std::vector<uint32_t> classWorlds[bph::kClassCount];
     std::vector<uint32_t> coopGroups;   // per world: workgroups of its cooperative launch (0 = not cooperative)
+    std::vector<uint64_t> workEstimate; // per world: bubbles x substeps of its last step (launch order key)
+    bool orderDirty = false;            // estimates changed since the class order lists were uploaded
     hipStream_t streams[bph::kStreams] = {};
     hipEvent_t startEvent = nullptr, endEvents[bph::kStreams] = {};
     hipEvent_t classStart[bph::kClassCount] = {}, classEnd[bph::kClassCount] = {};

@@ -3,10 +3,10 @@
 // One dispatcher thread owns every device call of the server. Requests (one step of one world,
 // optionally after a transaction) queue per world. A world with a waiting request and nothing
 // on the device is "ready" in the kernel class its bubble count selects. Whenever a class has
-// ready worlds and fewer than `max_inflight_per_class` launches in flight, one launch takes all
-// of them: an upload of the world list and transaction records, the transaction kernel, the
+// ready worlds, fewer than `max_inflight_per_class` launches in flight and a stream is idle, one
+// launch takes all of them: an upload of the world list and transaction records, the transaction kernel, the
 // class's step kernel, a gather of the worlds' result records and their copy to pinned host
-// memory, then an event. Launches spread over several streams, so different classes and
+// memory, then an event. Each launch takes a stream of its own, so different classes and
 // consecutive launches of one class run concurrently, and requests that arrive while others
 // are on the device join the next launch of their class. Results are posted as each launch's
 // event completes, without waiting for other launches.
@@ -97,7 +97,8 @@ struct bp_server {
     Launch launches[kMaxLaunches];
     std::vector<int> freeLaunch, inflight;
     std::vector<hipStream_t> streams;
-    std::vector<int> streamLoad;
+    std::vector<int> streamLoad;                // launches in flight per stream (0 or 1)
+    int idleStreams = 0;
     bpd::WorldStatus *stats = nullptr;          // one record per world (each world is in one launch at a time)
     hipStream_t readStream = nullptr;
     bp_bubble *readBuffer = nullptr;
@@ -215,6 +216,7 @@ bp_status bp_server::init() {
     }
     streams.resize(std::max<uint32_t>(1, cfg.streams));
     streamLoad.assign(streams.size(), 0);
+    idleStreams = (int)streams.size();
     for (auto &st_ : streams) BP_HIP(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
     BP_HIP(hipStreamCreateWithFlags(&readStream, hipStreamNonBlocking));
     local.resident_bytes = used;
@@ -324,6 +326,7 @@ bool bp_server::poll_launches() {
         }
         --inflightPerClass[L.cls];
         --streamLoad[L.stream];
+        ++idleStreams;
         freeLaunch.push_back(inflight[k]);
         inflight[k] = inflight.back();
         inflight.pop_back();
@@ -372,9 +375,10 @@ bp_status bp_server::start_launch(int cls) {
     L.worlds.swap(ws); L.bubbles.swap(nb); L.reqs.swap(rs);
     L.count = (uint32_t)L.worlds.size();
     for (uint32_t j = 0; j < L.count; ++j) L.hOrder[j] = L.worlds[j];
-    // The least loaded stream.
+    // An idle stream (form_launches checked that one exists): a launch never queues behind
+    // another one, so a world with a very long step holds up only the worlds of its own launch.
     int si = 0;
-    for (int k = 1; k < (int)streams.size(); ++k) if (streamLoad[k] < streamLoad[si]) si = k;
+    while (streamLoad[si] != 0) ++si;
     L.stream = si;
     hipStream_t s = streams[si];
     bp_batch &b = *batch;
@@ -393,6 +397,7 @@ bp_status bp_server::start_launch(int cls) {
     BP_HIP(hipEventRecord(L.done, s));
     ++inflightPerClass[cls];
     ++streamLoad[si];
+    --idleStreams;
     inflight.push_back(li);
     ++local.launches;
     local.launched_worlds += L.count;
@@ -472,8 +477,17 @@ bool bp_server::form_launches() {
         return true;
     }
     bool any = false;
-    for (int k = 0; k < kClassCount; ++k) {
-        while (!ready[k].empty() && inflightPerClass[k] < (int)cfg.max_inflight_per_class && !freeLaunch.empty()) {
+    // Classes in the order of their oldest ready request, each while an idle stream is left.
+    int order[kClassCount];
+    int classes = 0;
+    for (int k = 0; k < kClassCount; ++k) if (!ready[k].empty()) order[classes++] = k;
+    std::sort(order, order + classes, [&](int a, int b) {
+        return queue[ready[a].front()].front().arrival < queue[ready[b].front()].front().arrival;
+    });
+    for (int x = 0; x < classes; ++x) {
+        const int k = order[x];
+        while (!ready[k].empty() && inflightPerClass[k] < (int)cfg.max_inflight_per_class && !freeLaunch.empty() &&
+               idleStreams > 0) {
             bp_status s = start_launch(k);
             if (s != BP_OK) { fail(s); return false; }
             any = true;
@@ -537,8 +551,8 @@ void bp_server_config_default(bp_server_config *c) {
     if (!c) return;
     std::memset(c, 0, sizeof *c);
     c->memory_budget = 0;
-    c->streams = 8;
-    c->max_inflight_per_class = 2;
+    c->streams = 32;
+    c->max_inflight_per_class = 8;
     c->max_worlds_per_launch = 1024;
     c->big_world = 4096;
 }
@@ -555,8 +569,8 @@ bp_status bp_server_create(bp_batch *batch, const bp_server_config *config, bp_s
     bp_server_config_default(&s->cfg);
     if (config) {
         s->cfg = *config;
-        if (s->cfg.streams == 0) s->cfg.streams = 8;
-        if (s->cfg.max_inflight_per_class == 0) s->cfg.max_inflight_per_class = 2;
+        if (s->cfg.streams == 0) s->cfg.streams = 32;
+        if (s->cfg.max_inflight_per_class == 0) s->cfg.max_inflight_per_class = 8;
         if (s->cfg.max_worlds_per_launch == 0) s->cfg.max_worlds_per_launch = 1024;
         if (s->cfg.big_world == 0) s->cfg.big_world = 4096;
         s->cfg.streams = std::min<uint32_t>(s->cfg.streams, 32);
