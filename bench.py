@@ -236,7 +236,7 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=20000.0)
     ap.add_argument("--serve-worlds", type=int, default=8192)
     ap.add_argument("--serve-streams", type=int, default=32)
-    ap.add_argument("--serve-inflight", type=int, default=8)
+    ap.add_argument("--serve-inflight", type=int, default=32)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=min(os.cpu_count() or 1, 16))
     ap.add_argument("--no-cpu", action="store_true")

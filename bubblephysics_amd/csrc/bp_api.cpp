@@ -115,6 +115,7 @@ bp_status bp_batch::pack() {
     BP_HIP(hipMemset(A.rowStart, 0, std::max<size_t>(nb, 1) * sizeof(uint32_t)));
     BP_HIP(hipMemset(A.rowLen, 0, std::max<size_t>(nb, 1) * sizeof(uint32_t)));
     BP_ALLOC(A.omega2, 1); BP_ALLOC(A.bound, 1); BP_ALLOC(A.bodyHits, nbod);
+    BP_ALLOC(A.nbrA, nb); BP_ALLOC(A.nbrB, nb); BP_ALLOC(A.invmEff, nb);
     BP_ALLOC(A.adj, adj);
     BP_ALLOC(A.segAB, ns); BP_ALLOC(A.segRegion, ns); BP_ALLOC(A.segFlags, ns); BP_ALLOC(A.segCat, ns);
     BP_ALLOC(A.segMask, ns); BP_ALLOC(A.segId, ns); BP_ALLOC(A.segOwner, ns);
@@ -573,11 +574,11 @@ bp_status bp_batch::launch(const uint32_t *subset, uint32_t subsetCount, uint32_
         // through them without waiting for the slowest world of the class at every step.
         if (k == kGlobalClass) {
             for (uint32_t st = 0; st < steps; ++st) {
-                hipError_t e = bp_launch_world_step(1024, arenaDev, descs, stats + (size_t)st * W, orderOf(k), count, stream);
+                hipError_t e = bp_launch_world_step(1024, arenaDev, descs, stats + (size_t)st * W, orderOf(k), count, 0u, stream);
                 if (e != hipSuccess) { set_error(std::string("kernel launch: ") + hipGetErrorString(e)); return BP_ERR_COMMAND_FAILED; }
             }
         } else {
-            hipError_t e = bp_launch_world_lds(k, arenaDev, descs, stats, orderOf(k), count, steps, (uint32_t)W, stream);
+            hipError_t e = bp_launch_world_lds(k, arenaDev, descs, stats, orderOf(k), count, steps, (uint32_t)W, 0u, stream);
             if (e != hipSuccess) { set_error(std::string("kernel launch: ") + hipGetErrorString(e)); return BP_ERR_COMMAND_FAILED; }
         }
         BP_HIP(hipEventRecord(classEnd[k], stream));
@@ -731,9 +732,9 @@ bp_status bp_batch::capture(uint32_t steps) {
         hipStream_t stream = streams[k % kStreams];
         if (k == kGlobalClass) {
             for (uint32_t st = 0; st < steps && e == hipSuccess; ++st)
-                e = bp_launch_world_step(1024, arenaDev, descs, stats + (size_t)st * W, orders[k], count, stream);
+                e = bp_launch_world_step(1024, arenaDev, descs, stats + (size_t)st * W, orders[k], count, 0u, stream);
         } else {
-            e = bp_launch_world_lds(k, arenaDev, descs, stats, orders[k], count, steps, (uint32_t)W, stream);
+            e = bp_launch_world_lds(k, arenaDev, descs, stats, orders[k], count, steps, (uint32_t)W, 0u, stream);
         }
     }
     for (int k = 1; k < kStreams && e == hipSuccess; ++k) {

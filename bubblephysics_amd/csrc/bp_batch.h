@@ -15,10 +15,10 @@
 #include "bp_device.h"
 
 extern "C" hipError_t bp_launch_world_step(int block, const bpd::Arena *arena, bpd::WorldDesc *descs, bpd::WorldStatus *stats,
-                                           const uint32_t *order, uint32_t count, hipStream_t stream);
+                                           const uint32_t *order, uint32_t count, uint32_t notify, hipStream_t stream);
 extern "C" hipError_t bp_launch_world_lds(int id, const bpd::Arena *arena, bpd::WorldDesc *descs, bpd::WorldStatus *stats,
                                           const uint32_t *order, uint32_t count, uint32_t steps, uint32_t statsStride,
-                                          hipStream_t stream);
+                                          uint32_t notify, hipStream_t stream);
 extern "C" int bp_lds_config_count();
 extern "C" void bp_lds_config(int id, uint32_t *threads, uint32_t *per, uint32_t *rowcap, uint32_t *caps);
 extern "C" uint32_t bp_coop_max_groups();

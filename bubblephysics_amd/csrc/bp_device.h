@@ -52,6 +52,11 @@ struct Arena {
     float *margin;
     uint32_t *rowStart, *rowLen, *grid;
     float *omega2, *bound;    // per-bubble reduction inputs (debug / cooperative path)
+    // What neighbours read, packed per bubble for the global-state passes (one 16-byte load each):
+    // nbrA = (x, y, r, K_c) of the working state, nbrB = (effective velocity, |a|, 0), and the
+    // effective inverse mass of the step.
+    float4 *nbrA, *nbrB;
+    float *invmEff;
     uint32_t *bodyHits;       // per body: bubbles in its candidate row at the last grid build
     // Adjacency entries: (kind << 30) | neighbour, CSR per world.
     uint32_t *adj;
@@ -146,7 +151,15 @@ struct WorldStatus {
     uint32_t groupWidth;
     uint32_t executed;        // 1 when the kernel ran this world's step
     uint32_t ticks;           // device wall-clock ticks (100 MHz) the workgroup spent on the step
+    uint32_t seq;             // set to 1 last, with a system-scope release, when the launch asks for it
 };
+
+// Marks a world's result record complete (the thread that wrote the record calls it). With
+// `notify`, the release store at system scope makes the record visible to the host while the
+// launch is still running (the record then lives in pinned host memory, polled by the server).
+BPD void post_status(WorldStatus *st, uint32_t notify) {
+    if (notify) __hip_atomic_store(&st->seq, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
 
 // ---------------------------------------------------------------------------------------
 // float2 helpers (componentwise, one rounding per operation).

@@ -13,11 +13,11 @@
 namespace {
 template <int ID, int B, int P, int R, int SC, int BC, int VC, int TC>
 hipError_t launch_if(const bpd::Arena *arena, bpd::WorldDesc *descs, bpd::WorldStatus *stats, const uint32_t *order,
-                     uint32_t count, uint32_t steps, uint32_t statsStride, hipStream_t stream, bool &done) {
+                     uint32_t count, uint32_t steps, uint32_t statsStride, uint32_t notify, hipStream_t stream, bool &done) {
     if constexpr (ID == BP_LDS_CFG) {
         done = true;
         hipLaunchKernelGGL((bpd::world_lds_kernel<B, P, R, SC, BC, VC, TC>), dim3(count), dim3(B), 0, stream, arena, descs,
-                           stats, order, count, steps, statsStride);
+                           stats, order, count, steps, statsStride, notify);
         return hipGetLastError();
     } else {
         return hipSuccess;
@@ -27,11 +27,11 @@ hipError_t launch_if(const bpd::Arena *arena, bpd::WorldDesc *descs, bpd::WorldS
 
 extern "C" __attribute__((visibility("hidden"))) hipError_t BP_CAT(bp_launch_lds_cfg_, BP_LDS_CFG)(
     const bpd::Arena *arena, bpd::WorldDesc *descs, bpd::WorldStatus *stats, const uint32_t *order, uint32_t count,
-    uint32_t steps, uint32_t statsStride, hipStream_t stream) {
+    uint32_t steps, uint32_t statsStride, uint32_t notify, hipStream_t stream) {
     bool done = false;
     hipError_t e = hipSuccess;
 #define BP_LDS_ONE(ID, B, P, R, SC, BC, VC, TC) \
-    if (!done) e = launch_if<ID, B, P, R, SC, BC, VC, TC>(arena, descs, stats, order, count, steps, statsStride, stream, done);
+    if (!done) e = launch_if<ID, B, P, R, SC, BC, VC, TC>(arena, descs, stats, order, count, steps, statsStride, notify, stream, done);
     BP_LDS_CONFIGS(BP_LDS_ONE)
 #undef BP_LDS_ONE
     return done ? e : hipErrorInvalidValue;

@@ -5,11 +5,13 @@
 // on the device is "ready" in the kernel class its bubble count selects. Whenever a class has
 // ready worlds, fewer than `max_inflight_per_class` launches in flight and a stream is idle, one
 // launch takes all of them: an upload of the world list and transaction records, the transaction kernel, the
-// class's step kernel, a gather of the worlds' result records and their copy to pinned host
-// memory, then an event. Each launch takes a stream of its own, so different classes and
+// class's step kernel, then an event. Each launch takes a stream of its own, so different classes and
 // consecutive launches of one class run concurrently, and requests that arrive while others
-// are on the device join the next launch of their class. Results are posted as each launch's
-// event completes, without waiting for other launches.
+// are on the device join the next launch of their class. Each world's result is posted as soon
+// as its own workgroup has finished: the step kernels write the result record into pinned host
+// memory and mark it complete with a system-scope release, and the dispatcher polls the marks.
+// A world takes part in a new launch only once its previous launch has completed (its event),
+// so a step always reads the state that the previous step published.
 //
 // Residency: worlds are admitted against a device-memory budget in index order; a request for
 // a world held in host memory waits until the launches in flight drain, then the least recently
@@ -29,8 +31,6 @@ using namespace bph;
 
 extern "C" hipError_t bp_launch_transactions(const bpd::Arena *arena, bpd::WorldDesc *descs, const bpd::TxRecord *txs,
                                              uint32_t count, hipStream_t stream);
-extern "C" hipError_t bp_launch_gather_stats(const bpd::WorldStatus *stats, const uint32_t *order, uint32_t count,
-                                             bpd::WorldStatus *out, hipStream_t stream);
 extern "C" hipError_t bp_launch_gather_full_world(const bpd::Arena *arena, const bpd::WorldDesc *desc, bp_bubble *out,
                                                   uint32_t count, hipStream_t stream);
 
@@ -52,9 +52,10 @@ struct Launch {
     std::vector<uint32_t> worlds;
     std::vector<Pending> reqs;
     std::vector<uint32_t> bubbles;
+    std::vector<uint8_t> posted;                              // result already handed to the caller
+    uint32_t postedCount = 0;
     uint32_t *hOrder = nullptr, *dOrder = nullptr;            // pinned / device world lists
     bpd::TxRecord *hTx = nullptr, *dTx = nullptr;
-    bpd::WorldStatus *hStats = nullptr, *dStats = nullptr;
 };
 
 constexpr int kMaxLaunches = 64;
@@ -99,7 +100,8 @@ struct bp_server {
     std::vector<hipStream_t> streams;
     std::vector<int> streamLoad;                // launches in flight per stream (0 or 1)
     int idleStreams = 0;
-    bpd::WorldStatus *stats = nullptr;          // one record per world (each world is in one launch at a time)
+    bpd::WorldStatus *hostStats = nullptr;      // one record per world in pinned host memory (each world is
+    bpd::WorldStatus *stats = nullptr;          // in one launch at a time); `stats` is its device address
     hipStream_t readStream = nullptr;
     bp_bubble *readBuffer = nullptr;
     uint32_t readCapacity = 0;
@@ -201,17 +203,26 @@ bp_status bp_server::init() {
     busy.assign(W, 0); queued.assign(W, 0); wantAdmission.assign(W, 0);
     count.resize(W); headCount.assign(W, 0); headClass.assign(W, -1); lastUse.assign(W, 0);
     for (uint32_t w = 0; w < W; ++w) count[w] = (uint32_t)b.worlds[w].bubbles.size();
-    bp_status s = dev_alloc(this, stats, W);
-    if (s != BP_OK) return s;
+    bp_status s = BP_OK;
+    {
+        void *p = nullptr;
+        hipError_t e = hipHostMalloc(&p, std::max<size_t>(W, 1) * sizeof(bpd::WorldStatus), hipHostMallocCoherent | hipHostMallocMapped);
+        if (e != hipSuccess) { set_error(std::string("hipHostMalloc: ") + hipGetErrorString(e)); return BP_ERR_ALLOCATION_FAILED; }
+        hostAllocs.push_back(p);
+        hostStats = static_cast<bpd::WorldStatus *>(p);
+        std::memset(hostStats, 0, std::max<size_t>(W, 1) * sizeof(bpd::WorldStatus));
+        void *dp = nullptr;
+        BP_HIP(hipHostGetDevicePointer(&dp, p, 0));
+        stats = static_cast<bpd::WorldStatus *>(dp);
+    }
     const uint32_t cap = std::max<uint32_t>(1, std::min(cfg.max_worlds_per_launch, W));
     for (int k = 0; k < kMaxLaunches; ++k) {
         Launch &L = launches[k];
         if ((s = host_alloc(this, L.hOrder, cap)) != BP_OK || (s = dev_alloc(this, L.dOrder, cap)) != BP_OK ||
-            (s = host_alloc(this, L.hTx, cap)) != BP_OK || (s = dev_alloc(this, L.dTx, cap)) != BP_OK ||
-            (s = host_alloc(this, L.hStats, cap)) != BP_OK || (s = dev_alloc(this, L.dStats, cap)) != BP_OK)
+            (s = host_alloc(this, L.hTx, cap)) != BP_OK || (s = dev_alloc(this, L.dTx, cap)) != BP_OK)
             return s;
         BP_HIP(hipEventCreateWithFlags(&L.done, hipEventDisableTiming));
-        L.worlds.reserve(cap); L.reqs.reserve(cap); L.bubbles.reserve(cap);
+        L.worlds.reserve(cap); L.reqs.reserve(cap); L.bubbles.reserve(cap); L.posted.reserve(cap);
         freeLaunch.push_back(k);
     }
     streams.resize(std::max<uint32_t>(1, cfg.streams));
@@ -307,20 +318,29 @@ void bp_server::finish_request(const Pending &p, bp_status status, uint32_t subs
     cvDone.notify_all();
 }
 
-// Posts the results of every launch whose event has completed.
+// Posts the result of every world whose record is marked complete, and frees the worlds of every
+// launch whose event has completed.
 bool bp_server::poll_launches() {
     bool any = false;
     for (size_t k = 0; k < inflight.size();) {
         Launch &L = launches[inflight[k]];
+        auto post = [&](uint32_t j) {
+            const bpd::WorldStatus &s = hostStats[L.worlds[j]];
+            const bp_status status = s.executed ? (bp_status)s.status : BP_ERR_UNAVAILABLE;
+            finish_request(L.reqs[j], status, s.executed ? s.substeps : 0, L.bubbles[j], L.count, L.cls,
+                           s.executed ? (float)s.ticks * 1e-5f : 0.0f);
+            L.posted[j] = 1;
+            ++L.postedCount;
+            any = true;
+        };
+        for (uint32_t j = 0; j < L.count && L.postedCount < L.count; ++j)
+            if (!L.posted[j] && __atomic_load_n(&hostStats[L.worlds[j]].seq, __ATOMIC_ACQUIRE) == 1u) post(j);
         hipError_t e = hipEventQuery(L.done);
         if (e == hipErrorNotReady) { ++k; continue; }
         if (e != hipSuccess) { set_error(std::string("hipEventQuery: ") + hipGetErrorString(e)); fail(BP_ERR_COMMAND_FAILED); return false; }
         for (uint32_t j = 0; j < L.count; ++j) {
+            if (!L.posted[j]) post(j);                       // the launch is over: every record is final
             const uint32_t w = L.worlds[j];
-            const bpd::WorldStatus &s = L.hStats[j];
-            const bp_status status = s.executed ? (bp_status)s.status : BP_ERR_UNAVAILABLE;
-            finish_request(L.reqs[j], status, s.executed ? s.substeps : 0, L.bubbles[j], L.count, L.cls,
-                           s.executed ? (float)s.ticks * 1e-5f : 0.0f);
             busy[w] = 0;
             prepare_head(w);
         }
@@ -374,7 +394,12 @@ bp_status bp_server::start_launch(int cls) {
     for (uint32_t j = 0; j < idx.size(); ++j) { ws[j] = L.worlds[idx[j]]; nb[j] = L.bubbles[idx[j]]; rs[j] = L.reqs[idx[j]]; }
     L.worlds.swap(ws); L.bubbles.swap(nb); L.reqs.swap(rs);
     L.count = (uint32_t)L.worlds.size();
-    for (uint32_t j = 0; j < L.count; ++j) L.hOrder[j] = L.worlds[j];
+    L.posted.assign(L.count, 0);
+    L.postedCount = 0;
+    for (uint32_t j = 0; j < L.count; ++j) {
+        L.hOrder[j] = L.worlds[j];
+        __atomic_store_n(&hostStats[L.worlds[j]].seq, 0u, __ATOMIC_RELEASE);   // the kernel sets it to 1
+    }
     // An idle stream (form_launches checked that one exists): a launch never queues behind
     // another one, so a world with a very long step holds up only the worlds of its own launch.
     int si = 0;
@@ -388,12 +413,9 @@ bp_status bp_server::start_launch(int cls) {
         hipError_t e = bp_launch_transactions(b.arenaDev, b.descs, L.dTx, L.txCount, s);
         if (e != hipSuccess) { set_error(std::string("transaction launch: ") + hipGetErrorString(e)); return BP_ERR_COMMAND_FAILED; }
     }
-    hipError_t e = cls == kGlobalClass ? bp_launch_world_step(1024, b.arenaDev, b.descs, stats, L.dOrder, L.count, s)
-                                       : bp_launch_world_lds(cls, b.arenaDev, b.descs, stats, L.dOrder, L.count, 1, W, s);
+    hipError_t e = cls == kGlobalClass ? bp_launch_world_step(1024, b.arenaDev, b.descs, stats, L.dOrder, L.count, 1u, s)
+                                       : bp_launch_world_lds(cls, b.arenaDev, b.descs, stats, L.dOrder, L.count, 1, W, 1u, s);
     if (e != hipSuccess) { set_error(std::string("kernel launch: ") + hipGetErrorString(e)); return BP_ERR_COMMAND_FAILED; }
-    e = bp_launch_gather_stats(stats, L.dOrder, L.count, L.dStats, s);
-    if (e != hipSuccess) { set_error(std::string("gather launch: ") + hipGetErrorString(e)); return BP_ERR_COMMAND_FAILED; }
-    BP_HIP(hipMemcpyAsync(L.hStats, L.dStats, L.count * sizeof(bpd::WorldStatus), hipMemcpyDeviceToHost, s));
     BP_HIP(hipEventRecord(L.done, s));
     ++inflightPerClass[cls];
     ++streamLoad[si];
@@ -552,7 +574,7 @@ void bp_server_config_default(bp_server_config *c) {
     std::memset(c, 0, sizeof *c);
     c->memory_budget = 0;
     c->streams = 32;
-    c->max_inflight_per_class = 8;
+    c->max_inflight_per_class = 32;
     c->max_worlds_per_launch = 1024;
     c->big_world = 4096;
 }
@@ -570,7 +592,7 @@ bp_status bp_server_create(bp_batch *batch, const bp_server_config *config, bp_s
     if (config) {
         s->cfg = *config;
         if (s->cfg.streams == 0) s->cfg.streams = 32;
-        if (s->cfg.max_inflight_per_class == 0) s->cfg.max_inflight_per_class = 8;
+        if (s->cfg.max_inflight_per_class == 0) s->cfg.max_inflight_per_class = 32;
         if (s->cfg.max_worlds_per_launch == 0) s->cfg.max_worlds_per_launch = 1024;
         if (s->cfg.big_world == 0) s->cfg.big_world = 4096;
         s->cfg.streams = std::min<uint32_t>(s->cfg.streams, 32);

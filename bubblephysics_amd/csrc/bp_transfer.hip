@@ -63,7 +63,7 @@ extern "C" __attribute__((visibility("hidden"))) hipError_t bp_launch_scatter_wo
 }
 
 // ---------------------------------------------------------------------------------------
-// Serving: transactions applied on the device, result records of one launch, full records.
+// Serving: transactions applied on the device, and full records of one world.
 // ---------------------------------------------------------------------------------------
 namespace bpd {
 
@@ -206,13 +206,6 @@ __global__ void transaction_kernel(const Arena *__restrict__ A, WorldDesc *descs
     }
 }
 
-// Result records of one launch: out[j] = stats[order[j]].
-__global__ void gather_stats_kernel(const WorldStatus *__restrict__ stats, const uint32_t *order, uint32_t count,
-                                    WorldStatus *out) {
-    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j < count) out[j] = stats[order[j]];
-}
-
 // Complete records (state and properties) of one world's published side.
 __global__ void gather_full_world_kernel(const Arena *__restrict__ A, const WorldDesc *d, bp_bubble *out) {
     const uint32_t n = d->bubbleCount, o = d->bubbleOffset, p = d->pub;
@@ -240,14 +233,6 @@ extern "C" __attribute__((visibility("hidden"))) hipError_t bp_launch_transactio
                                                                                hipStream_t stream) {
     if (count == 0) return hipSuccess;
     hipLaunchKernelGGL(bpd::transaction_kernel, dim3(count), dim3(256), 0, stream, arena, descs, txs);
-    return hipGetLastError();
-}
-
-extern "C" __attribute__((visibility("hidden"))) hipError_t bp_launch_gather_stats(const bpd::WorldStatus *stats, const uint32_t *order,
-                                                                               uint32_t count, bpd::WorldStatus *out,
-                                                                               hipStream_t stream) {
-    if (count == 0) return hipSuccess;
-    hipLaunchKernelGGL(bpd::gather_stats_kernel, dim3((count + 255) / 256), dim3(256), 0, stream, stats, order, count, out);
     return hipGetLastError();
 }
 

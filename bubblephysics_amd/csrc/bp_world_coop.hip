@@ -521,6 +521,7 @@ __global__ __launch_bounds__(BLOCK) void world_coop_kernel(const Arena *__restri
         if ((f & kEnabled) && (f & kStatic)) { v = f2(0.0f, 0.0f); om = 0.0f; }
         W.pos[i] = p; W.start[i] = p; W.vel[i] = v; W.rot[i] = A->rot[pub][o + i]; W.omega[i] = om;
         W.region[i] = BP_REGION_UNCHANGED;
+        W.publish_step(i);
     }
     if (gtid == 0) {
         s.counters[kCoSafety] = 0; s.counters[kCoRemovals] = 0; s.counters[kCoWraps] = 0;
@@ -579,7 +580,10 @@ __global__ __launch_bounds__(BLOCK) void world_coop_kernel(const Arena *__restri
                 grid_maxmin<BLOCK>(grid, red, s, w2Max, w2Own, mbMin, mbOwn);
                 if (d->injectionEnabled && c.c1 == d->injectionAfter + 1) {
                     for (uint32_t i = R.first; i < n; i += R.stride)
-                        if (W.id[i] == d->injectionObject) W.pos[i] = f2(d->injectionPosition[0], d->injectionPosition[1]);
+                        if (W.id[i] == d->injectionObject) {
+                            W.pos[i] = f2(d->injectionPosition[0], d->injectionPosition[1]);
+                            W.publish_a(i);
+                        }
                     grid.sync();
                     pass_bounds<BLOCK>(W, red, R, c, w2Max, w2Own, mbMin, mbOwn);
                     grid_maxmin<BLOCK>(grid, red, s, w2Max, w2Own, mbMin, mbOwn);

@@ -31,6 +31,8 @@ struct World {
     float2 *acc, *a0, *ref;
     float *margin;
     uint32_t *rowStart, *rowLen, *grid, *adj;
+    float4 *nbrA, *nbrB;         // packed neighbour reads (Arena::nbrA, nbrB)
+    float *invmEff;
     const float4 *motors, *bands;
     const float2 *verts;
     const uint4 *tris;
@@ -43,6 +45,14 @@ struct World {
     }
     BPD float inv_mass(uint32_t i) const { return is_static(i) ? 0.0f : invmRaw[i]; }
     BPD float2 eff_vel(uint32_t i) const { return is_static(i) ? f2(0.0f, 0.0f) : vel[i]; }
+    // The packed records of bubble i: A after its centre moved, B after its velocity and
+    // acceleration changed, and the step's effective inverse mass at the start of the step.
+    BPD void publish_a(uint32_t i) const { const float2 p = pos[i]; nbrA[i] = make_float4(p.x, p.y, radius[i], Kc[i]); }
+    BPD void publish_b(uint32_t i) const {
+        const float2 v = eff_vel(i);
+        nbrB[i] = make_float4(v.x, v.y, len2(acc[i]), 0.0f);
+    }
+    BPD void publish_step(uint32_t i) const { invmEff[i] = inv_mass(i); publish_a(i); }
 
     BPD Seg segment(uint32_t j) const {
         Seg s;
@@ -175,6 +185,7 @@ BPD World make_world(const Arena *A, const WorldDesc *d, uint32_t wside) {
     W.motor = A->motor + o; W.band = A->band + o;
     W.acc = A->acc + o; W.a0 = A->a0 + o; W.ref = A->ref + o; W.margin = A->margin + o;
     W.rowStart = A->rowStart + o; W.rowLen = A->rowLen + o; W.grid = A->grid + o; W.adj = A->adj + d->adjOffset;
+    W.nbrA = A->nbrA + o; W.nbrB = A->nbrB + o; W.invmEff = A->invmEff + o;
     W.motors = A->motors + d->motorOffset; W.bands = A->bands + d->bandOffset;
     W.verts = A->verts + d->vertexOffset; W.tris = A->tris + d->triangleOffset;
     W.cellWidth = 0.0f;
@@ -211,13 +222,15 @@ BPD void pass_forces(World &W, Red<BLOCK> &red, Range r, const Ctl &c, uint32_t 
         for (uint32_t e = 0; e < len; ++e) {
             uint32_t entry = row[e], kind = entry >> kKindShift, j = entry & kIndexMask;
             if (kind == 0u) {
-                uint32_t lo = min(i, j), hi = max(i, j);
-                float2 delta = sub(W.pos[hi], W.pos[lo]);
+                const float4 q = W.nbrA[j];
+                const float invmJ = W.invmEff[j];
+                const bool low = i < j;
+                const float2 delta = low ? sub(f2(q.x, q.y), p) : sub(p, f2(q.x, q.y));
                 float dd = len2(delta);
                 float2 normal = dd > 0.0f ? dvs(delta, dd) : f2(1.0f, 0.0f);
-                Law law = pair_law(W.Kc[lo], W.radius[lo], W.Kc[hi], W.radius[hi], dd);
-                total = add(total, mul(i == lo ? -law.force : law.force, normal));
-                if (invm > 0.0f) w2 = w2 + law.stiffness * (invm + __builtin_sqrtf(invm * W.inv_mass(j)));
+                Law law = low ? pair_law(K, ri, q.w, q.z, dd) : pair_law(q.w, q.z, K, ri, dd);
+                total = add(total, mul(low ? -law.force : law.force, normal));
+                if (invm > 0.0f) w2 = w2 + law.stiffness * (invm + __builtin_sqrtf(invm * invmJ));
             } else if (kind == 1u) {
                 float2 normal; float dd;
                 if (!wall_region(W.segment(j), p, normal, dd) || !(dd < ri)) continue;
@@ -247,6 +260,7 @@ BPD void pass_forces(World &W, Red<BLOCK> &red, Range r, const Ctl &c, uint32_t 
                 W.vel[i] = v;
             }
         }
+        W.nbrB[i] = make_float4(invm > 0.0f ? v.x : 0.0f, invm > 0.0f ? v.y : 0.0f, len2(a), 0.0f);
         // omega^2 (Force:510-541) and margin bound (Force:604-614) for the next reduction.
         if (w2 > bestW) { bestW = w2; ownW = i; }
         float mb;
@@ -287,10 +301,11 @@ BPD void pass_bounds(World &W, Red<BLOCK> &red, Range r, const Ctl &c, float &w2
             for (uint32_t e = 0; e < len; ++e) {
                 uint32_t entry = row[e], kind = entry >> kKindShift, j = entry & kIndexMask;
                 if (kind == 0u) {
-                    uint32_t lo = min(i, j), hi = max(i, j);
-                    float dd = len2(sub(W.pos[hi], W.pos[lo]));
-                    Law law = pair_law(W.Kc[lo], W.radius[lo], W.Kc[hi], W.radius[hi], dd);
-                    w2 = w2 + law.stiffness * (invm + __builtin_sqrtf(invm * W.inv_mass(j)));
+                    const float4 q = W.nbrA[j];
+                    const bool low = i < j;
+                    float dd = len2(low ? sub(f2(q.x, q.y), p) : sub(p, f2(q.x, q.y)));
+                    Law law = low ? pair_law(K, ri, q.w, q.z, dd) : pair_law(q.w, q.z, K, ri, dd);
+                    w2 = w2 + law.stiffness * (invm + __builtin_sqrtf(invm * W.invmEff[j]));
                 } else if (kind == 1u) {
                     float2 normal; float dd;
                     if (!wall_region(W.segment(j), p, normal, dd) || !(dd < ri)) continue;
@@ -339,12 +354,13 @@ BPD void pass_distance(World &W, Red<BLOCK> &red, Range r, const Ctl &c, float h
             uint32_t entry = row[e], kind = entry >> kKindShift, j = entry & kIndexMask;
             float dd, approach, accSum, reach, bodySpeed = 0.0f;
             if (kind == 0u) {
-                float2 delta = sub(W.pos[j], p);
+                const float4 q = W.nbrA[j], qb = W.nbrB[j];
+                float2 delta = sub(f2(q.x, q.y), p);
                 dd = len2(delta);
                 float2 normal = dd > 0.0f ? dvs(delta, dd) : f2(1.0f, 0.0f);
-                approach = dot2(sub(v, W.eff_vel(j)), normal);
-                accSum = accI + len2(W.acc[j]);
-                reach = ri + W.radius[j];
+                approach = dot2(sub(v, f2(qb.x, qb.y)), normal);
+                accSum = accI + qb.z;
+                reach = ri + q.z;
             } else if (kind == 1u) {
                 float2 normal;
                 if (!wall_region(W.segment(j), p, normal, dd)) continue;
@@ -399,6 +415,7 @@ BPD void pass_drift(World &W, Range r, const Ctl &c) {
         }
         W.pos[i] = add(W.pos[i], mul(v, h));
         W.vel[i] = v;
+        W.publish_a(i);
         W.rot[i] = W.rot[i] + om * h;
         W.omega[i] = angular;
     }

@@ -268,7 +268,7 @@ BPD bool assert_contacts(World &W, Shared<BLOCK> &sh, const Ctl &c, WorldStatus 
 // ---------------------------------------------------------------------------------------
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void world_step_kernel(const Arena *__restrict__ A, WorldDesc *descs, WorldStatus *stats,
-                                                           const uint32_t *order, uint32_t count) {
+                                                           const uint32_t *order, uint32_t count, uint32_t notify) {
     __shared__ Shared<BLOCK> sh;
     if (blockIdx.x >= count) return;
     const uint32_t wi = order[blockIdx.x];
@@ -276,7 +276,7 @@ __global__ __launch_bounds__(BLOCK) void world_step_kernel(const Arena *__restri
     WorldStatus *st = stats + wi;
     const uint64_t ticks0 = wall_clock64();
     if (d->failed) {
-        if (threadIdx.x == 0) st->executed = 0;
+        if (threadIdx.x == 0) { st->executed = 0; post_status(st, notify); }
         return;
     }
     const uint32_t pub = d->pub, wside = 1u - pub;
@@ -294,6 +294,7 @@ __global__ __launch_bounds__(BLOCK) void world_step_kernel(const Arena *__restri
         if ((f & kEnabled) && (f & kStatic)) { v = f2(0.0f, 0.0f); om = 0.0f; }
         W.pos[i] = p; W.start[i] = p; W.vel[i] = v; W.rot[i] = A->rot[pub][o + i]; W.omega[i] = om;
         W.region[i] = BP_REGION_UNCHANGED;
+        W.publish_step(i);
     }
     // Kinematic and static bodies (PostSolve:66-82 for the bodies a bubble world may hold).
     uint32_t safety = 0;
@@ -376,8 +377,10 @@ __global__ __launch_bounds__(BLOCK) void world_step_kernel(const Arena *__restri
                 pass_forces<BLOCK>(W, sh.red, Range{threadIdx.x, BLOCK}, c, 1u, w2Max, w2Own, mbMin, mbOwn);
                 if (d->injectionEnabled && c.c1 == d->injectionAfter + 1) {
                     for (uint32_t i = threadIdx.x; i < n; i += BLOCK)
-                        if (W.id[i] == d->injectionObject)
+                        if (W.id[i] == d->injectionObject) {
                             W.pos[i] = f2(d->injectionPosition[0], d->injectionPosition[1]);
+                            W.publish_a(i);
+                        }
                     __syncthreads();
                     pass_bounds<BLOCK>(W, sh.red, Range{threadIdx.x, BLOCK}, c, w2Max, w2Own, mbMin, mbOwn);
                 }
@@ -476,6 +479,7 @@ __global__ __launch_bounds__(BLOCK) void world_step_kernel(const Arena *__restri
         st->ticks = (uint32_t)(wall_clock64() - ticks0);
         if (status == BP_OK) d->pub = wside;
         else if (status != BP_ERR_CAPACITY_OVERFLOW) d->failed = 1;
+        post_status(st, notify);
     }
 }
 
@@ -483,13 +487,13 @@ __global__ __launch_bounds__(BLOCK) void world_step_kernel(const Arena *__restri
 
 // Host launchers (used by bp_api.cpp).
 extern "C" __attribute__((visibility("hidden"))) hipError_t bp_launch_world_step(int block, const bpd::Arena *arena, bpd::WorldDesc *descs, bpd::WorldStatus *stats,
-                                           const uint32_t *order, uint32_t count, hipStream_t stream) {
+                                           const uint32_t *order, uint32_t count, uint32_t notify, hipStream_t stream) {
     if (count == 0) return hipSuccess;
     switch (block) {
-        case 256: hipLaunchKernelGGL(bpd::world_step_kernel<256>, dim3(count), dim3(256), 0, stream, arena, descs, stats, order, count); break;
-        case 512: hipLaunchKernelGGL(bpd::world_step_kernel<512>, dim3(count), dim3(512), 0, stream, arena, descs, stats, order, count); break;
-        case 1024: hipLaunchKernelGGL(bpd::world_step_kernel<1024>, dim3(count), dim3(1024), 0, stream, arena, descs, stats, order, count); break;
-        default: hipLaunchKernelGGL(bpd::world_step_kernel<64>, dim3(count), dim3(64), 0, stream, arena, descs, stats, order, count); break;
+        case 256: hipLaunchKernelGGL(bpd::world_step_kernel<256>, dim3(count), dim3(256), 0, stream, arena, descs, stats, order, count, notify); break;
+        case 512: hipLaunchKernelGGL(bpd::world_step_kernel<512>, dim3(count), dim3(512), 0, stream, arena, descs, stats, order, count, notify); break;
+        case 1024: hipLaunchKernelGGL(bpd::world_step_kernel<1024>, dim3(count), dim3(1024), 0, stream, arena, descs, stats, order, count, notify); break;
+        default: hipLaunchKernelGGL(bpd::world_step_kernel<64>, dim3(count), dim3(64), 0, stream, arena, descs, stats, order, count, notify); break;
     }
     return hipGetLastError();
 }

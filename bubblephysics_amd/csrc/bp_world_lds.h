@@ -76,7 +76,7 @@ constexpr uint32_t kHasMotor = 1u << 31, kHasTorque = 1u << 30;   // register-on
 template <int BLOCK, int PER, int ROWCAP, int SEGCAP, int BODYCAP, int VERTCAP, int TRICAP>
 __global__ __launch_bounds__(BLOCK) void world_lds_kernel(const Arena *__restrict__ A, WorldDesc *descs, WorldStatus *stats,
                                                           const uint32_t *order, uint32_t count, uint32_t steps,
-                                                          uint32_t statsStride) {
+                                                          uint32_t statsStride, uint32_t notify) {
     using L = LdsWorld<BLOCK, PER, ROWCAP, SEGCAP, BODYCAP, VERTCAP, TRICAP>;
     __shared__ L S;
     if (blockIdx.x >= count) return;
@@ -85,13 +85,13 @@ __global__ __launch_bounds__(BLOCK) void world_lds_kernel(const Arena *__restric
     WorldStatus *st = stats + wi;
     const uint32_t lane = threadIdx.x;
     if (d->failed) {
-        if (lane == 0) st->executed = 0;
+        if (lane == 0) { st->executed = 0; post_status(st, notify); }
         return;
     }
     if (d->segmentCount > SEGCAP || d->bodyCount > BODYCAP || d->vertexCount > VERTCAP || d->triangleCount > TRICAP ||
         d->bubbleCount > L::N) {
         // The host never routes such a world here; refuse it rather than overrun LDS.
-        if (lane == 0) { st->status = BP_ERR_COMMAND_FAILED; st->substeps = 0; st->executed = 1; }
+        if (lane == 0) { st->status = BP_ERR_COMMAND_FAILED; st->substeps = 0; st->executed = 1; post_status(st, notify); }
         return;
     }
     uint32_t pubNow = d->pub;                                 // published side, tracked in registers
@@ -792,6 +792,7 @@ __global__ __launch_bounds__(BLOCK) void world_lds_kernel(const Arena *__restric
             st->ticks = (uint32_t)(wall_clock64() - ticks0);
             if (status == BP_OK) d->pub = ws;
             else if (status != BP_ERR_CAPACITY_OVERFLOW) d->failed = 1;
+            post_status(st, notify);
         }
         if (status != BP_OK) break;                           // a failed world stops; later steps stay unexecuted
         pubNow = ws;

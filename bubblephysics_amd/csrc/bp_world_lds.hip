@@ -10,7 +10,7 @@
 #define BP_LDS_DECLARE(ID, B, P, R, SC, BC, VC, TC)                                                                 \
     extern "C" hipError_t bp_launch_lds_cfg_##ID(const bpd::Arena *arena, bpd::WorldDesc *descs, bpd::WorldStatus *stats, \
                                                  const uint32_t *order, uint32_t count, uint32_t steps,         \
-                                                 uint32_t statsStride, hipStream_t stream);
+                                                 uint32_t statsStride, uint32_t notify, hipStream_t stream);
 BP_LDS_CONFIGS(BP_LDS_DECLARE)
 #undef BP_LDS_DECLARE
 
@@ -32,13 +32,14 @@ extern "C" __attribute__((visibility("hidden"))) void bp_lds_config(int id, uint
 }
 
 // `steps` steps of every world in one launch; the status of step k of world w is stats[k * statsStride + w].
+// With `notify`, each status record is marked complete with a system-scope release (post_status).
 extern "C" __attribute__((visibility("hidden"))) hipError_t bp_launch_world_lds(int id, const bpd::Arena *arena, bpd::WorldDesc *descs,
                                                                            bpd::WorldStatus *stats, const uint32_t *order,
                                                                            uint32_t count, uint32_t steps, uint32_t statsStride,
-                                                                           hipStream_t stream) {
+                                                                           uint32_t notify, hipStream_t stream) {
     if (count == 0) return hipSuccess;
 #define BP_LDS_LAUNCH(ID, B, P, R, SC, BC, VC, TC) \
-    if (id == ID) return bp_launch_lds_cfg_##ID(arena, descs, stats, order, count, steps, statsStride, stream);
+    if (id == ID) return bp_launch_lds_cfg_##ID(arena, descs, stats, order, count, steps, statsStride, notify, stream);
     BP_LDS_CONFIGS(BP_LDS_LAUNCH)
 #undef BP_LDS_LAUNCH
     return hipErrorInvalidValue;

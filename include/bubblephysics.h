@@ -487,7 +487,7 @@ typedef struct bp_server_config {
     uint32_t streams;               /* HIP streams, one launch in flight on each (default 32; the
                                        process should have as many hardware queues:
                                        GPU_MAX_HW_QUEUES)                                       */
-    uint32_t max_inflight_per_class;/* launches of one kernel class in flight at once (default 8) */
+    uint32_t max_inflight_per_class;/* launches of one kernel class in flight at once (default 32) */
     uint32_t max_worlds_per_launch; /* default 1024                                             */
     uint32_t big_world;             /* worlds above this many bubbles run in the one-workgroup
                                        global-state kernel (default 4096)                       */
