@@ -116,6 +116,7 @@ bp_status bp_batch::pack() {
     BP_HIP(hipMemset(A.rowLen, 0, std::max<size_t>(nb, 1) * sizeof(uint32_t)));
     BP_ALLOC(A.omega2, 1); BP_ALLOC(A.bound, 1); BP_ALLOC(A.bodyHits, nbod);
     BP_ALLOC(A.nbrA, nb); BP_ALLOC(A.nbrB, nb); BP_ALLOC(A.invmEff, nb);
+    BP_ALLOC(A.cellA, nb); BP_ALLOC(A.cellB, nb);
     BP_ALLOC(A.adj, adj);
     BP_ALLOC(A.segAB, ns); BP_ALLOC(A.segRegion, ns); BP_ALLOC(A.segFlags, ns); BP_ALLOC(A.segCat, ns);
     BP_ALLOC(A.segMask, ns); BP_ALLOC(A.segId, ns); BP_ALLOC(A.segOwner, ns);

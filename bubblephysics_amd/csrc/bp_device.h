@@ -57,6 +57,11 @@ struct Arena {
     // effective inverse mass of the step.
     float4 *nbrA, *nbrB;
     float *invmEff;
+    // The same bubbles in the order of the last counting sort by cell (slot order of `grid`), so a
+    // scan of a cell reads consecutive records: (x, y, r, margin) and (index, category, mask,
+    // flags: 1 = finite step start, 2 = removed by the region, 4 = movable).
+    float4 *cellA;
+    uint4 *cellB;
     uint32_t *bodyHits;       // per body: bubbles in its candidate row at the last grid build
     // Adjacency entries: (kind << 30) | neighbour, CSR per world.
     uint32_t *adj;

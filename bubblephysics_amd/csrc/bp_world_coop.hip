@@ -311,6 +311,7 @@ BPD void coop_hash(cg::grid_group &grid, World &W, Red<BLOCK> &red, const CoopSc
         if (!finite2(p)) continue;
         const uint32_t slot = atomicAdd(&cells[bucket_of(W.cell_x(p), W.cell_y(p), B)], 1u);
         W.grid[slot] = i;
+        W.publish_cell(slot, i);
     }
     grid.sync();
 }
@@ -426,6 +427,7 @@ BPD bool coop_assert(cg::grid_group &grid, World &W, Red<BLOCK> &red, const Coop
         const float2 center = W.pos[i], start = W.start[i];
         if (!finite2(center) || W.region[i] == BP_REGION_REMOVED) continue;
         const bool movable = W.inv_mass(i) > 0.0f;
+        const uint32_t ci = W.cat[i], ki = W.mask[i];
         uint32_t bestJ = 0xffffffffu;
         float bestD = 0.0f;
         const int64_t cx = W.cell_x(center), cy = W.cell_y(center);
@@ -435,12 +437,14 @@ BPD bool coop_assert(cg::grid_group &grid, World &W, Red<BLOCK> &red, const Coop
                 const uint32_t b = bucket_of(tx, ty, B);
                 const uint32_t lo = b == 0 ? 0u : cells[b - 1], hi = cells[b];
                 for (uint32_t k = lo; k < hi; ++k) {
-                    const uint32_t j = W.grid[k];
+                    const uint4 cb = W.cellB[k];
+                    const uint32_t j = cb.x;
                     if (j <= i || j >= bestJ) continue;
-                    const float2 q = W.pos[j];
+                    const float4 ca = W.cellA[k];
+                    const float2 q = f2(ca.x, ca.y);
                     if (W.cell_x(q) != tx || W.cell_y(q) != ty) continue;
-                    if (W.region[j] == BP_REGION_REMOVED || !filter_accepts(W.cat[i], W.mask[i], W.cat[j], W.mask[j])) continue;
-                    if (!movable && !(W.inv_mass(j) > 0.0f)) continue;
+                    if ((cb.w & 2u) || !filter_accepts(ci, ki, cb.y, cb.z)) continue;
+                    if (!movable && !(cb.w & 4u)) continue;
                     const float dd = len2(sub(q, center));
                     if (dd <= 0.0f) { bestJ = j; bestD = dd; }
                 }

@@ -33,6 +33,8 @@ struct World {
     uint32_t *rowStart, *rowLen, *grid, *adj;
     float4 *nbrA, *nbrB;         // packed neighbour reads (Arena::nbrA, nbrB)
     float *invmEff;
+    float4 *cellA;               // cell-sorted records (Arena::cellA, cellB)
+    uint4 *cellB;
     const float4 *motors, *bands;
     const float2 *verts;
     const uint4 *tris;
@@ -53,6 +55,15 @@ struct World {
         nbrB[i] = make_float4(v.x, v.y, len2(acc[i]), 0.0f);
     }
     BPD void publish_step(uint32_t i) const { invmEff[i] = inv_mass(i); publish_a(i); }
+    // Bubble i's record at its slot of the cell-sorted order (written where the counting sort
+    // places it in `grid`).
+    BPD void publish_cell(uint32_t slot, uint32_t i) const {
+        const float2 p = pos[i];
+        cellA[slot] = make_float4(p.x, p.y, radius[i], margin[i]);
+        cellB[slot] = make_uint4(i, cat[i], mask[i],
+                                 (finite2(start[i]) ? 1u : 0u) | (region[i] == BP_REGION_REMOVED ? 2u : 0u) |
+                                     (inv_mass(i) > 0.0f ? 4u : 0u));
+    }
 
     BPD Seg segment(uint32_t j) const {
         Seg s;
@@ -144,6 +155,8 @@ BPD void enumerate_row(const World &W, const uint32_t *ends, uint32_t buckets, u
         return;
     }
     bool ownRow = finite2(W.start[i]);                        // Geometry:305-307
+    const float ri = W.radius[i], mi = W.margin[i];
+    const uint32_t ci = W.cat[i], ki = W.mask[i];
     int64_t cx = W.cell_x(p), cy = W.cell_y(p);
     for (int dx = -1; dx <= 1; ++dx)
         for (int dy = -1; dy <= 1; ++dy) {
@@ -151,17 +164,24 @@ BPD void enumerate_row(const World &W, const uint32_t *ends, uint32_t buckets, u
             uint32_t b = bucket_of(tx, ty, buckets);
             uint32_t lo = b == 0 ? 0u : ends[b - 1], hi = ends[b];
             for (uint32_t k = lo; k < hi; ++k) {
-                uint32_t j = W.grid[k];
+                // The cell-sorted record of slot k: W.pair_candidate's operands, read in one go.
+                const uint4 cb = W.cellB[k];
+                const uint32_t j = cb.x;
                 if (j == i) continue;
-                float2 q = W.pos[j];
+                const float4 ca = W.cellA[k];
+                const float2 q = f2(ca.x, ca.y);
                 if (W.cell_x(q) != tx || W.cell_y(q) != ty) continue;
                 if (j > i) {
                     if (!ownRow) continue;
-                    if (!W.pair_candidate(i, j)) continue;
+                    if (!filter_accepts(ci, ki, cb.y, cb.z)) continue;               // pair_candidate(i, j)
+                    const float reach = ri + ca.z + mi + ca.w + W.sepTol;
+                    if (!(fabsf(p.x - q.x) <= reach && fabsf(p.y - q.y) <= reach)) continue;
                     ++forward;
                 } else {
-                    if (!finite2(W.start[j])) continue;
-                    if (!W.pair_candidate(j, i)) continue;
+                    if (!(cb.w & 1u)) continue;                                      // step start not finite
+                    if (!filter_accepts(cb.y, cb.z, ci, ki)) continue;               // pair_candidate(j, i)
+                    const float reach = ca.z + ri + ca.w + mi + W.sepTol;
+                    if (!(fabsf(q.x - p.x) <= reach && fabsf(q.y - p.y) <= reach)) continue;
                 }
                 emit(j);
             }
@@ -186,6 +206,7 @@ BPD World make_world(const Arena *A, const WorldDesc *d, uint32_t wside) {
     W.acc = A->acc + o; W.a0 = A->a0 + o; W.ref = A->ref + o; W.margin = A->margin + o;
     W.rowStart = A->rowStart + o; W.rowLen = A->rowLen + o; W.grid = A->grid + o; W.adj = A->adj + d->adjOffset;
     W.nbrA = A->nbrA + o; W.nbrB = A->nbrB + o; W.invmEff = A->invmEff + o;
+    W.cellA = A->cellA + o; W.cellB = A->cellB + o;
     W.motors = A->motors + d->motorOffset; W.bands = A->bands + d->bandOffset;
     W.verts = A->verts + d->vertexOffset; W.tris = A->tris + d->triangleOffset;
     W.cellWidth = 0.0f;

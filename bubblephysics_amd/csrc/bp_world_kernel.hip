@@ -79,6 +79,7 @@ BPD bool build_grid(World &W, Shared<BLOCK> &sh, float remaining, uint32_t limit
         if (!finite2(p)) continue;
         uint32_t slot = atomicAdd(&sh.buckets[bucket_of(W.cell_x(p), W.cell_y(p), buckets)], 1u);
         W.grid[slot] = i;
+        W.publish_cell(slot, i);
     }
     __syncthreads();
     // Pass 1: row lengths and the reference's overflow counters.
@@ -183,6 +184,7 @@ BPD bool assert_contacts(World &W, Shared<BLOCK> &sh, const Ctl &c, WorldStatus 
         if (!finite2(p)) continue;
         uint32_t slot = atomicAdd(&sh.buckets[bucket_of(W.cell_x(p), W.cell_y(p), buckets)], 1u);
         W.grid[slot] = i;
+        W.publish_cell(slot, i);
     }
     __syncthreads();
     float fraction = c.t0 > 0.0f ? clampf(c.t1 / c.t0, 0.0f, 1.0f) : 1.0f;
@@ -192,6 +194,7 @@ BPD bool assert_contacts(World &W, Shared<BLOCK> &sh, const Ctl &c, WorldStatus 
         float2 center = W.pos[i], start = W.start[i];
         if (!finite2(center) || W.region[i] == BP_REGION_REMOVED) continue;
         bool movable = W.inv_mass(i) > 0.0f;
+        const uint32_t ci = W.cat[i], ki = W.mask[i];
         uint32_t bestJ = 0xffffffffu;
         float bestD = 0.0f;
         int64_t cx = W.cell_x(center), cy = W.cell_y(center);
@@ -201,12 +204,14 @@ BPD bool assert_contacts(World &W, Shared<BLOCK> &sh, const Ctl &c, WorldStatus 
                 uint32_t b = bucket_of(tx, ty, buckets);
                 uint32_t lo = b == 0 ? 0u : sh.buckets[b - 1], hi = sh.buckets[b];
                 for (uint32_t k = lo; k < hi; ++k) {
-                    uint32_t j = W.grid[k];
+                    const uint4 cb = W.cellB[k];
+                    uint32_t j = cb.x;
                     if (j <= i || j >= bestJ) continue;
-                    float2 q = W.pos[j];
+                    const float4 ca = W.cellA[k];
+                    float2 q = f2(ca.x, ca.y);
                     if (W.cell_x(q) != tx || W.cell_y(q) != ty) continue;
-                    if (W.region[j] == BP_REGION_REMOVED || !filter_accepts(W.cat[i], W.mask[i], W.cat[j], W.mask[j])) continue;
-                    if (!movable && !(W.inv_mass(j) > 0.0f)) continue;
+                    if ((cb.w & 2u) || !filter_accepts(ci, ki, cb.y, cb.z)) continue;
+                    if (!movable && !(cb.w & 4u)) continue;
                     float dd = len2(sub(q, center));
                     if (dd <= 0.0f) { bestJ = j; bestD = dd; }
                 }
