@@ -133,6 +133,20 @@ def run_served(args):
     return stats
 
 
+def run_served_batch(args, scenes_list):
+    """Serving the worlds of the batched workload: Poisson step requests (10% with transactions) at
+    `serve_c4_rate` over `serve_c4_seconds`, from a fresh copy of the worlds."""
+    from bubblephysics_amd import serving, solver
+    batch = solver.Batch(scenes_list)
+    batch.step(1)
+    server = serving.Server(batch, streams=args.serve_streams, max_inflight_per_class=args.serve_inflight)
+    requests = serving.poisson_requests(len(scenes_list), args.serve_c4_rate, args.serve_c4_seconds, seed=20261208)
+    stats = server.run(requests).summary()
+    server.close()
+    batch.close()
+    return stats
+
+
 def run_plain(steps: int):
     from bubblephysics_amd import scenes, solver
     scene = scenes.config2()
@@ -235,6 +249,8 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=30.0)
     ap.add_argument("--serve-rate", type=float, default=20000.0)
     ap.add_argument("--serve-worlds", type=int, default=8192)
+    ap.add_argument("--serve-c4-seconds", type=float, default=5.0)
+    ap.add_argument("--serve-c4-rate", type=float, default=20000.0)
     ap.add_argument("--serve-streams", type=int, default=32)
     ap.add_argument("--serve-inflight", type=int, default=32)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -261,6 +277,9 @@ def main():
     large = run_large(args.large_steps) if args.large_steps > 0 else None
     if large:
         print(f"[bench] large done: {large['value']:.3g} bubble-substeps/s", file=sys.stderr, flush=True)
+    served_c4 = run_served_batch(args, scenes_list) if args.serve_c4_seconds > 0 else None
+    if served_c4:
+        print(f"[bench] served batch worlds: {served_c4}", file=sys.stderr, flush=True)
     served = run_served(args) if args.serve_seconds > 0 else None
     if served:
         print(f"[bench] served: {served}", file=sys.stderr, flush=True)
@@ -305,6 +324,11 @@ def main():
             "workload": f"config 3: 1 world, {large['bubbles']} bubbles + {large['bodies']} bodies (kinematic and walls), "
                         "cooperative grid", "value": large["value"], "ms_per_step": large["ms_per_step"],
             "gpu_ms_per_step": large["gpu_ms_per_step"], "statuses": large["statuses"]},
+        "served_batch_worlds": None if served_c4 is None else {
+            "workload": f"serving the {res['worlds']} batched worlds above: Poisson {args.serve_c4_rate:.0f} step "
+                        f"requests/s for {args.serve_c4_seconds:.0f}s, 10% with transactions",
+            "over_plain": served_c4["bubble_substeps_per_s"] / plain["value"] if plain["value"] > 0 else None,
+            **served_c4},
         "served": None if served is None else {
             "workload": f"serving: {served['worlds']} resident worlds, bubbles log-uniform in [32, 16384] "
                         f"({served['bubbles']} total), Poisson {args.serve_rate:.0f} step requests/s for "

@@ -242,10 +242,18 @@ BPD Law pair_law(float Ka, float ra, float Kb, float rb, float distanceValue) {
             force = fmaxf(force - residual / slope, lowest);
         }
     }
-    float qa = force / Ka, qb = force / Kb;
-    float ea = epsilon_from_force_ratio(qa), eb = epsilon_from_force_ratio(qb);
-    float sa = rest_from_force_ratio(qa), sb = rest_from_force_ratio(qb);
-    float compliance = ra * sa * sa * sa / ((1.0f + ea) * Ka) + rb * sb * sb * sb / ((1.0f + eb) * Kb);
+    // Equal stiffnesses give equal force ratios, so b's strain, rest factor and compliance
+    // denominator are a's, bit for bit: computed once.
+    const float qa = force / Ka;
+    const float ea = epsilon_from_force_ratio(qa), sa = rest_from_force_ratio(qa);
+    const float da = (1.0f + ea) * Ka;
+    float eb = ea, sb = sa, db = da;
+    if (Ka != Kb) {
+        const float qb = force / Kb;
+        eb = epsilon_from_force_ratio(qb); sb = rest_from_force_ratio(qb);
+        db = (1.0f + eb) * Kb;
+    }
+    float compliance = ra * sa * sa * sa / da + rb * sb * sb * sb / db;
     law.force = force; law.epsA = ea; law.epsB = eb;
     law.stiffness = compliance > kFloatMin ? 1.0f / compliance : 0.0f;
     return law;

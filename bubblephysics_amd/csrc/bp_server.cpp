@@ -4,9 +4,10 @@
 // optionally after a transaction) queue per world. A world with a waiting request and nothing
 // on the device is "ready" in the kernel class its bubble count selects. Whenever a class has
 // ready worlds, fewer than `max_inflight_per_class` launches in flight and a stream is idle, one
-// launch takes all of them: an upload of the world list and transaction records, the transaction kernel, the
-// class's step kernel, then an event. Each launch takes a stream of its own, so different classes and
-// consecutive launches of one class run concurrently, and requests that arrive while others
+// launch takes all of them: the transaction kernel, the class's step kernel (both read the world
+// list and the transaction records from pinned host memory), then an event. Each launch takes a
+// stream of its own, so different classes and consecutive launches of one class run
+// concurrently, and requests that arrive while others
 // are on the device join the next launch of their class. Each world's result is posted as soon
 // as its own workgroup has finished: the step kernels write the result record into pinned host
 // memory and mark it complete with a system-scope release, and the dispatcher polls the marks.
@@ -54,7 +55,9 @@ struct Launch {
     std::vector<uint32_t> bubbles;
     std::vector<uint8_t> posted;                              // result already handed to the caller
     uint32_t postedCount = 0;
-    uint32_t *hOrder = nullptr, *dOrder = nullptr;            // pinned / device world lists
+    // World list and transaction records in coherent pinned memory, read by the kernels through
+    // their device addresses (no copy per launch).
+    uint32_t *hOrder = nullptr, *dOrder = nullptr;
     bpd::TxRecord *hTx = nullptr, *dTx = nullptr;
 };
 
@@ -141,12 +144,16 @@ template <class T> bp_status dev_alloc(bp_server *s, T *&out, size_t count) {
     return BP_OK;
 }
 
-template <class T> bp_status host_alloc(bp_server *s, T *&out, size_t count) {
-    void *p = nullptr;
-    hipError_t e = hipHostMalloc(&p, std::max<size_t>(count, 1) * sizeof(T), hipHostMallocDefault);
+// Coherent pinned host memory and its device address.
+template <class T> bp_status host_alloc(bp_server *s, T *&out, T *&device, size_t count) {
+    void *p = nullptr, *dp = nullptr;
+    hipError_t e = hipHostMalloc(&p, std::max<size_t>(count, 1) * sizeof(T), hipHostMallocCoherent | hipHostMallocMapped);
     if (e != hipSuccess) { set_error(std::string("hipHostMalloc: ") + hipGetErrorString(e)); return BP_ERR_ALLOCATION_FAILED; }
     s->hostAllocs.push_back(p);
+    e = hipHostGetDevicePointer(&dp, p, 0);
+    if (e != hipSuccess) { set_error(std::string("hipHostGetDevicePointer: ") + hipGetErrorString(e)); return BP_ERR_ALLOCATION_FAILED; }
     out = static_cast<T *>(p);
+    device = static_cast<T *>(dp);
     return BP_OK;
 }
 
@@ -218,8 +225,7 @@ bp_status bp_server::init() {
     const uint32_t cap = std::max<uint32_t>(1, std::min(cfg.max_worlds_per_launch, W));
     for (int k = 0; k < kMaxLaunches; ++k) {
         Launch &L = launches[k];
-        if ((s = host_alloc(this, L.hOrder, cap)) != BP_OK || (s = dev_alloc(this, L.dOrder, cap)) != BP_OK ||
-            (s = host_alloc(this, L.hTx, cap)) != BP_OK || (s = dev_alloc(this, L.dTx, cap)) != BP_OK)
+        if ((s = host_alloc(this, L.hOrder, L.dOrder, cap)) != BP_OK || (s = host_alloc(this, L.hTx, L.dTx, cap)) != BP_OK)
             return s;
         BP_HIP(hipEventCreateWithFlags(&L.done, hipEventDisableTiming));
         L.worlds.reserve(cap); L.reqs.reserve(cap); L.bubbles.reserve(cap); L.posted.reserve(cap);
@@ -407,9 +413,7 @@ bp_status bp_server::start_launch(int cls) {
     L.stream = si;
     hipStream_t s = streams[si];
     bp_batch &b = *batch;
-    BP_HIP(hipMemcpyAsync(L.dOrder, L.hOrder, L.count * sizeof(uint32_t), hipMemcpyHostToDevice, s));
     if (L.txCount) {
-        BP_HIP(hipMemcpyAsync(L.dTx, L.hTx, L.txCount * sizeof(bpd::TxRecord), hipMemcpyHostToDevice, s));
         hipError_t e = bp_launch_transactions(b.arenaDev, b.descs, L.dTx, L.txCount, s);
         if (e != hipSuccess) { set_error(std::string("transaction launch: ") + hipGetErrorString(e)); return BP_ERR_COMMAND_FAILED; }
     }

@@ -73,6 +73,62 @@ struct RowRef {
 
 constexpr uint32_t kHasMotor = 1u << 31, kHasTorque = 1u << 30;   // register-only flag bits
 
+// Neighbour index of row entry e of the block [start, end): every fourth entry loads the next
+// four indices together (one memory round trip for four entries, which matters most for rows
+// in global memory), the others select from those registers.
+BPD uint32_t row_entry4(const RowRef &rows, uint32_t e, uint32_t start, uint32_t end, uint32_t (&jn)[4]) {
+    const uint32_t u = (e - start) & 3u;
+    if (u == 0u) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) jn[q] = rows.index(min(e + q, end - 1));
+    }
+    return u == 0u ? jn[0] : u == 1u ? jn[1] : u == 2u ? jn[2] : jn[3];
+}
+
+// Sorts the first `count` of N register values ascending with a bitonic network (every index is a
+// compile-time constant once unrolled, so the values stay in registers). Slots past `count`
+// hold the largest value and stay at the end.
+template <int N>
+BPD void bitonic_sort(uint32_t (&v)[N]) {
+#pragma unroll
+    for (int k = 2; k <= N; k <<= 1)
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1)
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                const int l = i ^ j;
+                if (l > i) {
+                    const uint32_t lo = min(v[i], v[l]), hi = max(v[i], v[l]);
+                    if ((i & k) == 0) { v[i] = lo; v[l] = hi; } else { v[i] = hi; v[l] = lo; }
+                }
+            }
+}
+template <int N, typename T>
+BPD void sort_small_block(T *row, uint32_t count) {
+    uint32_t v[N];
+#pragma unroll
+    for (int u = 0; u < N; ++u) v[u] = (uint32_t)u < count ? (uint32_t)row[u] : 0xffffffffu;
+    bitonic_sort<N>(v);
+#pragma unroll
+    for (int u = 0; u < N; ++u)
+        if ((uint32_t)u < count) row[u] = (T)v[u];
+}
+// Orders the pair block of an adjacency row (entries of kind 0, so the value is the neighbour
+// index) ascending: blocks of up to 16 entries are loaded together, sorted in registers and
+// stored together; longer blocks are insertion-sorted in place.
+template <typename T>
+BPD void sort_pair_block(T *row, uint32_t count) {
+    if (count < 2) return;
+    if (count <= 8) { sort_small_block<8>(row, count); return; }
+    if (count <= 16) { sort_small_block<16>(row, count); return; }
+    for (uint32_t x = 1; x < count; ++x) {
+        const T val = row[x];
+        uint32_t loc = x;
+        while (loc > 0 && val < row[loc - 1]) { row[loc] = row[loc - 1]; --loc; }
+        row[loc] = val;
+    }
+}
+
 template <int BLOCK, int PER, int ROWCAP, int SEGCAP, int BODYCAP, int VERTCAP, int TRICAP>
 __global__ __launch_bounds__(BLOCK) void world_lds_kernel(const Arena *__restrict__ A, WorldDesc *descs, WorldStatus *stats,
                                                           const uint32_t *order, uint32_t count, uint32_t steps,
@@ -153,6 +209,20 @@ __global__ __launch_bounds__(BLOCK) void world_lds_kernel(const Arena *__restric
                 if (driven && md.w > 0.0f) { fl[s] |= kHasTorque; mq[s] = md.z / md.w; }
             }
         }
+        // World-uniform shortcuts of the candidate tests: when every bubble has a nonzero category
+        // and a full mask, every pair passes the filter test; when every start is finite, every
+        // neighbour's row exists. Either lets candidate enumeration skip the per-candidate global
+        // loads of categories, masks and starts (the starts do not change until the end of step).
+        uint32_t localAccept = 1u, localStarts = 1u;
+    #pragma unroll
+        for (int s = 0; s < PER; ++s) {
+            if (!valid[s]) continue;
+            const uint32_t gi = o + lane + s * BLOCK;
+            if (A->cat[gi] == 0u || A->mask[gi] != 0xffffffffu) localAccept = 0u;
+            if (!finite2(p[s])) localStarts = 0u;
+        }
+        const bool allAccept = __syncthreads_and((int)localAccept) != 0;
+        const bool allStarts = __syncthreads_and((int)localStarts) != 0;
         uint32_t safety = predict_bodies<BLOCK>(A, d, pub, ws);
         // Geometry into LDS. Bodies are read back by the lane that predicted them.
         for (uint32_t k = lane; k < g.nb; k += BLOCK) {
@@ -286,34 +356,52 @@ __global__ __launch_bounds__(BLOCK) void world_lds_kernel(const Arena *__restric
             const float ri = S.r[i];
             const uint32_t ci = catG[i], ki = maskG[i];
             if (finite2(pi)) {
-                const bool ownRow = finite2(startG[i]);
+                const bool ownRow = allStarts || finite2(startG[i]);
                 const int64_t cx = cell_coord(pi.x, cellWidth), cy = cell_coord(pi.y, cellWidth);
                 for (int dx = -1; dx <= 1; ++dx)
                     for (int dy = -1; dy <= 1; ++dy) {
                         const int64_t tx = cx + dx, ty = cy + dy;
                         const uint32_t b = bucket_of(tx, ty, buckets);
                         const uint32_t lo = b == 0 ? 0u : S.buckets[b - 1], hi = S.buckets[b];
-                        for (uint32_t k = lo; k < hi; ++k) {
-                            const uint32_t j = S.grid[k];
-                            if (j == i) continue;
-                            const float2 q = S.pos[j];
-                            // The reach test first: it is cheaper than the cell test and most
-                            // entries fail it (the conditions are independent, so the result is
-                            // the same in either order).
-                            const uint32_t l = min(i, j), h = max(i, j);
-                            const float2 pl = l == i ? pi : q, ph = l == i ? q : pi;
-                            const float ml = l == i ? mi : S.aux[j], mh = l == i ? S.aux[j] : mi;
-                            const float reach = S.r[l] + S.r[h] + ml + mh + sepTol;
-                            if (!(fabsf(pl.x - ph.x) <= reach && fabsf(pl.y - ph.y) <= reach)) continue;
-                            if (cell_coord(q.x, cellWidth) != tx || cell_coord(q.y, cellWidth) != ty) continue;
-                            if (!filter_accepts(catG[l], maskG[l], catG[h], maskG[h])) continue;
-                            if (j > i) {
-                                if (!ownRow) continue;
-                                ++forward;
-                            } else if (!finite2(startG[j])) {
-                                continue;
+                        // Entries in groups of four: the group's index loads, then its centre,
+                        // radius and margin loads, are issued together, so one LDS round trip
+                        // serves four entries. The entries are still tested in bucket order.
+                        for (uint32_t k0 = lo; k0 < hi; k0 += 4) {
+                            uint32_t jj[4];
+                            float2 qq[4];
+                            float rr[4], mm[4];
+    #pragma unroll
+                            for (int u = 0; u < 4; ++u) jj[u] = S.grid[min(k0 + u, hi - 1)];
+    #pragma unroll
+                            for (int u = 0; u < 4; ++u) { qq[u] = S.pos[jj[u]]; rr[u] = S.r[jj[u]]; mm[u] = S.aux[jj[u]]; }
+    #pragma unroll
+                            for (int u = 0; u < 4; ++u) {
+                                if (k0 + u >= hi) break;
+                                const uint32_t j = jj[u];
+                                if (j == i) continue;
+                                const float2 q = qq[u];
+                                // The reach test first: it is cheaper than the cell test and most
+                                // entries fail it (the conditions are independent, so the result
+                                // is the same in either order). The reach sums the lower index's
+                                // radius and margin first (r_l + r_h is exact in either order,
+                                // the margins are not), and |x_l - x_h| = |x_i - x_j| exactly.
+                                const bool iLow = i < j;
+                                const float ml = iLow ? mi : mm[u], mh = iLow ? mm[u] : mi;
+                                const float reach = ri + rr[u] + ml + mh + sepTol;
+                                if (!(fabsf(pi.x - q.x) <= reach && fabsf(pi.y - q.y) <= reach)) continue;
+                                if (cell_coord(q.x, cellWidth) != tx || cell_coord(q.y, cellWidth) != ty) continue;
+                                if (!allAccept) {
+                                    const uint32_t l = min(i, j), h = max(i, j);
+                                    if (!filter_accepts(catG[l], maskG[l], catG[h], maskG[h])) continue;
+                                }
+                                if (j > i) {
+                                    if (!ownRow) continue;
+                                    ++forward;
+                                } else if (!allStarts && !finite2(startG[j])) {
+                                    continue;
+                                }
+                                emit(j);
                             }
-                            emit(j);
                         }
                     }
                 if (ownRow)
@@ -384,38 +472,27 @@ __global__ __launch_bounds__(BLOCK) void world_lds_kernel(const Arena *__restric
     #pragma unroll
             for (int s = 0; s < PER; ++s) {
                 if (!valid[s]) continue;
-                uint32_t forward, cnt = 0;
+                // Enumeration emits the pairs, then the segments, then the bodies, so the block
+                // lengths are the counts of each kind.
+                uint32_t forward, cnt = 0, pc = 0, sc = 0;
                 if (rows.inLds) {
                     uint16_t *row = S.rows + rs[s];
                     enumerate(lane + s * BLOCK, p[s], mg[s], [&](uint32_t e) __attribute__((always_inline)) {
                         row[cnt++] = (uint16_t)(((e >> 30) << kLdsKindShift) | (e & kIndexMask));
+                        pc += (e >> 30) == 0u;
+                        sc += (e >> 30) == 1u;
                     }, forward);
-                    uint32_t pc = 0;
-                    while (pc < cnt && (row[pc] >> kLdsKindShift) == 0u) ++pc;
-                    uint32_t qc = pc;
-                    while (qc < cnt && (row[qc] >> kLdsKindShift) == 1u) ++qc;
-                    rp[s] = pc; rq[s] = qc;
-                    for (uint32_t x = 1; x < pc; ++x) {
-                        uint16_t val = row[x];
-                        uint32_t loc = x;
-                        while (loc > 0 && val < row[loc - 1]) { row[loc] = row[loc - 1]; --loc; }
-                        row[loc] = val;
-                    }
+                    sort_pair_block(row, pc);
                 } else {
                     uint32_t *row = A->adj + d->adjOffset + rs[s];
-                    enumerate(lane + s * BLOCK, p[s], mg[s], [&](uint32_t e) __attribute__((always_inline)) { row[cnt++] = e; }, forward);
-                    uint32_t pc = 0;
-                    while (pc < cnt && (row[pc] >> kKindShift) == 0u) ++pc;
-                    uint32_t qc = pc;
-                    while (qc < cnt && (row[qc] >> kKindShift) == 1u) ++qc;
-                    rp[s] = pc; rq[s] = qc;
-                    for (uint32_t x = 1; x < pc; ++x) {
-                        uint32_t val = row[x];
-                        uint32_t loc = x;
-                        while (loc > 0 && val < row[loc - 1]) { row[loc] = row[loc - 1]; --loc; }
-                        row[loc] = val;
-                    }
+                    enumerate(lane + s * BLOCK, p[s], mg[s], [&](uint32_t e) __attribute__((always_inline)) {
+                        row[cnt++] = e;
+                        pc += (e >> 30) == 0u;
+                        sc += (e >> 30) == 1u;
+                    }, forward);
+                    sort_pair_block(row, pc);
                 }
+                rp[s] = pc; rq[s] = pc + sc;
             }
             __syncthreads();
             return true;
@@ -441,8 +518,9 @@ __global__ __launch_bounds__(BLOCK) void world_lds_kernel(const Arena *__restric
                 if (invm > 0.0f && bi[s] >= 0) w2 = w2 + g.band(bi[s]).z * invm;
                 // The row is (pairs, segments, bodies): the reference's summation order (Force:216-255).
                 const uint32_t start = rs[s], pairEnd = start + rp[s], segEnd = start + rq[s], end = start + rl[s];
+                uint32_t jn[4] = {0u, 0u, 0u, 0u};
                 for (uint32_t e = start; e < pairEnd; ++e) {
-                    const uint32_t j = rows.index(e);
+                    const uint32_t j = row_entry4(rows, e, start, pairEnd, jn);
                     const uint32_t lo = min(i, j), hi = max(i, j);
                     const float2 delta = sub(S.pos[hi], S.pos[lo]);
                     const float dd = len2(delta);
@@ -525,8 +603,9 @@ __global__ __launch_bounds__(BLOCK) void world_lds_kernel(const Arena *__restric
                 const float ri = S.r[i], accI = S.aux[i];
                 float best = inff();
                 const uint32_t start = rs[s], pairEnd = start + rp[s], segEnd = start + rq[s], end = start + rl[s];
+                uint32_t jn[4] = {0u, 0u, 0u, 0u};
                 for (uint32_t e = start; e < pairEnd; ++e) {
-                    const uint32_t j = rows.index(e);
+                    const uint32_t j = row_entry4(rows, e, start, pairEnd, jn);
                     const float2 delta = sub(S.pos[j], pi);
                     const float dd = len2(delta);
                     const float2 normal = dd > 0.0f ? dvs(delta, dd) : f2(1.0f, 0.0f);
