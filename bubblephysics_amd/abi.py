@@ -187,6 +187,13 @@ class ServerConfig(ctypes.Structure):
     ]
 
 
+class Contact(ctypes.Structure):          # bp_contact
+    _fields_ = [("id", ctypes.c_uint64), ("first", ctypes.c_int64), ("second", ctypes.c_int64), ("kind", ctypes.c_uint32),
+                ("feature", ctypes.c_uint32), ("normal", ctypes.c_float * 2), ("point", ctypes.c_float * 2),
+                ("penetration", ctypes.c_float), ("allowed_side", ctypes.c_float), ("impulse", ctypes.c_float),
+                ("index_b", ctypes.c_uint32)]
+
+
 class ServerStats(ctypes.Structure):
     _fields_ = [(name, ctypes.c_uint64) for name in (
         "submitted", "completed", "launches", "launched_worlds", "transactions", "rejected", "evictions", "admissions",
@@ -202,6 +209,8 @@ BAND_DTYPE = np.dtype(Band)
 FIELD_DTYPE = np.dtype(Field)
 FRAME_DTYPE = np.dtype(FrameInfo)
 AUDIT_DTYPE = np.dtype(Audit)
+CONTACT_DTYPE = np.dtype(Contact)
+BP_CONTACT_PAIR, BP_CONTACT_SEGMENT = 0, 1
 TUNE_DTYPE = np.dtype(TuneBin)
 REQUEST_DTYPE = np.dtype(Request)
 RESULT_DTYPE = np.dtype(ServerResult)
@@ -233,6 +242,8 @@ SIGNATURES = {
     "bp_world_debug_fail_next_command": ([_P], _I32),
     "bp_world_audit": ([_P, _P], _I32),
     "bp_batch_audit": ([_P, _P, _U32], _I32),
+    "bp_world_contacts": ([_P, _P, _U32, _P], _I32),
+    "bp_batch_contacts": ([_P, _U32, _P, _U32, _P], _I32),
     "bp_static_polygon_edges": ([_P, _U32, _I64, _I64, _U32, _P, _U32, _P], _I32),
     "bp_polygon_body_geometry": ([_P, _U32, _P, _P, _P, _P, _P, _P], _I32),
     "bp_place_discs": ([_P, _U32, _P, _F, _F, _P, _U32, _U64, _U32, _P, _P], _I32),

@@ -695,6 +695,61 @@ bp_status bp_batch::audit(bp_audit *out, uint32_t count) {
     return BP_OK;
 }
 
+// End-of-step contact records of world `wi` (contacts_kernel, bp_contacts.hip).
+bp_status bp_batch::contacts(uint32_t wi, bp_contact *out, uint32_t capacity, uint32_t *count) {
+    if (wi >= worlds.size() || (capacity && !out) || !count) return BP_ERR_INVALID_ARGUMENT;
+    if (!packed) {
+        bp_status s = pack();
+        if (s != BP_OK) return s;
+    }
+    if (!worlds[wi].resident) { set_error("world is not resident"); return BP_ERR_UNAVAILABLE; }
+    bpd::WorldDesc d;
+    BP_HIP(hipMemcpy(&d, descs + wi, sizeof(d), hipMemcpyDeviceToHost));
+    if (!d.exportAdjacency) {
+        set_error("contact records read the rows of the last grid build: set export_adjacency");
+        return BP_ERR_UNAVAILABLE;
+    }
+    *count = 0;
+    if (d.bubbleCount == 0) return BP_OK;
+    void *dev = nullptr;
+    const size_t offsetBytes = (size_t)2 * d.bubbleCount * sizeof(uint32_t), recordBytes = (size_t)capacity * sizeof(bpd::ContactRecord);
+    BP_HIP(hipMalloc(&dev, offsetBytes + recordBytes + 256));
+    uint32_t *offsets = static_cast<uint32_t *>(dev);
+    auto *records = reinterpret_cast<bpd::ContactRecord *>(static_cast<char *>(dev) + ((offsetBytes + 255) / 256) * 256);
+    uint32_t *total = reinterpret_cast<uint32_t *>(static_cast<char *>(dev) + ((offsetBytes + 255) / 256) * 256 + recordBytes);
+    hipError_t e = bp_launch_contacts(arenaDev, descs, wi, offsets, records, capacity, total, streams[0]);
+    uint32_t kept = 0;
+    std::vector<bpd::ContactRecord> host;
+    if (e == hipSuccess) e = hipMemcpyAsync(&kept, total, sizeof(kept), hipMemcpyDeviceToHost, streams[0]);
+    if (e == hipSuccess) e = hipStreamSynchronize(streams[0]);
+    if (e == hipSuccess && kept) {
+        host.resize(std::min(kept, capacity));
+        e = hipMemcpy(host.data(), records, host.size() * sizeof(bpd::ContactRecord), hipMemcpyDeviceToHost);
+    }
+    (void)hipFree(dev);
+    if (e != hipSuccess) { set_error(std::string("contacts: ") + hipGetErrorString(e)); return BP_ERR_COMMAND_FAILED; }
+    for (size_t k = 0; k < host.size(); ++k) {
+        const bpd::ContactRecord &r = host[k];
+        bp_contact &c = out[k];
+        // The contact id of NativeMetalSolver.swift:1047-1052 (FNV-1a steps over the key).
+        uint64_t h = 14695981039346656037ull;
+        h = (h ^ (uint64_t)r.kind) * 1099511628211ull;
+        h = (h ^ (uint64_t)r.first) * 1099511628211ull;
+        h = (h ^ (uint64_t)r.second) * 1099511628211ull;
+        c.id = h;
+        c.first = r.first; c.second = r.second;
+        c.kind = r.kind; c.feature = r.feature;
+        c.normal[0] = r.normal[0]; c.normal[1] = r.normal[1];
+        c.point[0] = r.point[0]; c.point[1] = r.point[1];
+        c.penetration = r.penetration;
+        c.allowed_side = r.allowedSide;
+        c.impulse = r.impulse;
+        c.index_b = r.indexB;
+    }
+    *count = kept;
+    return BP_OK;
+}
+
 bp_status bp_batch::step(uint32_t steps, bp_frame_info *infos) {
     bp_status s = launch(nullptr, 0, steps);
     if (s != BP_OK) return s;
@@ -788,6 +843,9 @@ bp_status bp_batch::replay(bp_frame_info *infos) {
 extern "C" {
 
 uint32_t bp_abi_version(void) { return BP_ABI_VERSION; }
+
+static_assert(sizeof(bp_contact) == 64, "bp_contact is 64 bytes");
+static_assert(sizeof(bpd::ContactRecord) == 56, "device contact record");
 
 void bp_abi_record_sizes(uint32_t *out) {
     if (!out) return;
@@ -1005,6 +1063,13 @@ bp_status bp_batch_audit(bp_batch *b, bp_audit *out, uint32_t count) {
     std::lock_guard<std::mutex> lock(b->mutex);
     if (b->inFlight || b->pending || b->serving) return BP_ERR_FRAME_IN_FLIGHT;
     return b->audit(out, count);
+}
+
+bp_status bp_batch_contacts(bp_batch *b, uint32_t world, bp_contact *out, uint32_t capacity, uint32_t *out_count) {
+    if (!b) return BP_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(b->mutex);
+    if (b->inFlight || b->pending || b->serving) return BP_ERR_FRAME_IN_FLIGHT;
+    return b->contacts(world, out, capacity, out_count);
 }
 
 bp_status bp_batch_set_world_class(bp_batch *b, uint32_t world, int32_t cls) {
@@ -1239,6 +1304,12 @@ bp_status bp_world_audit(bp_world *w, bp_audit *out) {
     if (!w || !out) return BP_ERR_INVALID_ARGUMENT;
     if (w->inFlight) return BP_ERR_FRAME_IN_FLIGHT;
     return w->batch.audit(out, 1);
+}
+
+bp_status bp_world_contacts(bp_world *w, bp_contact *out, uint32_t capacity, uint32_t *out_count) {
+    if (!w || !out_count) return BP_ERR_INVALID_ARGUMENT;
+    if (w->inFlight) return BP_ERR_FRAME_IN_FLIGHT;
+    return w->batch.contacts(0, out, capacity, out_count);
 }
 
 bp_status bp_world_debug_fail_next_command(bp_world *w) {

@@ -159,6 +159,15 @@ struct WorldStatus {
     uint32_t seq;             // set to 1 last, with a system-scope release, when the launch asks for it
 };
 
+// One end-of-step contact record (bp_contact without its id, which the host hashes from the key).
+struct ContactRecord {
+    int64_t first, second;    // bubble A id; bubble B id or segment id
+    uint32_t kind, feature;   // 0 pair, 1 segment; polygon edge feature
+    float normal[2], point[2];
+    float penetration, allowedSide, impulse;
+    uint32_t indexB;          // bubble B index or segment index
+};
+
 // Marks a world's result record complete (the thread that wrote the record calls it). With
 // `notify`, the release store at system scope makes the record visible to the host while the
 // launch is still running (the record then lives in pinned host memory, polled by the server).

@@ -180,6 +180,11 @@ class NativeWorld:
         abi.check(self._lib.bp_world_audit(self._handle, abi.ptr(out)), "bp_world_audit")
         return out[0]
 
+    def contacts(self, capacity: int | None = None) -> np.ndarray:
+        """End-of-step contact records (bp_contact) of the published state."""
+        return _read_contacts(lambda out, cap, count: self._lib.bp_world_contacts(self._handle, out, cap, count),
+                              "bp_world_contacts", capacity)
+
     def read_step_history(self) -> np.ndarray:
         out = np.zeros(1024, np.float32)
         count = ctypes.c_uint32(0)
@@ -189,6 +194,17 @@ class NativeWorld:
 
     def fail_next_command(self):
         abi.check(self._lib.bp_world_debug_fail_next_command(self._handle), "bp_world_debug_fail_next_command")
+
+
+def _read_contacts(call, name: str, capacity: int | None) -> np.ndarray:
+    """Runs a contact-record call; without a capacity, asks for the count first."""
+    count = ctypes.c_uint32(0)
+    if capacity is None:
+        abi.check(call(None, 0, ctypes.byref(count)), name)
+        capacity = count.value
+    out = np.zeros(max(capacity, 1), abi.CONTACT_DTYPE)
+    abi.check(call(abi.ptr(out), capacity, ctypes.byref(count)), name)
+    return out[: min(count.value, capacity)]
 
 
 class Batch:
@@ -226,6 +242,11 @@ class Batch:
         out = np.zeros(len(self._counts), abi.AUDIT_DTYPE)
         abi.check(self._lib.bp_batch_audit(self._handle, abi.ptr(out), len(out)), "bp_batch_audit")
         return out
+
+    def contacts(self, world: int, capacity: int | None = None) -> np.ndarray:
+        """End-of-step contact records (bp_contact) of one world's published state."""
+        return _read_contacts(lambda out, cap, count: self._lib.bp_batch_contacts(self._handle, world, out, cap, count),
+                              "bp_batch_contacts", capacity)
 
     def capture(self, steps: int = 1):
         """Captures `steps` steps of every world into a hipGraph (bp_batch_capture)."""

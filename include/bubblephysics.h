@@ -309,6 +309,33 @@ typedef struct bp_audit {
 } bp_audit;
 bp_status bp_world_audit(bp_world *world, bp_audit *out);
 
+/* End-of-step contact records of a world's published state: NativeSolverContact
+ * (NativeSolverInput.swift:188-200) as the reference fills it from its contact cache
+ * (NativeMetalSolver.swift:1041-1060). Built from the candidate list of the last grid build at the
+ * final centres (nativeBuildContacts, NativeGeometryKernels.metal-source:683-728), kept iff
+ * penetration > -separation tolerance (NativeContactKernels.metal-source:146-160), in candidate
+ * order: every bubble's pairs (the lower index first, partner ascending), then every bubble's
+ * segments; `impulse` is the contact law's force at the record's geometry times dt^2
+ * (nativeContactImpulses, NativeForceKernels.metal-source:261-280). The world's rows must be kept
+ * (export_adjacency, always on for bp_world) and its last step must have succeeded. 64 bytes. */
+enum { BP_CONTACT_PAIR = 0, BP_CONTACT_SEGMENT = 1 };
+typedef struct bp_contact {
+    uint64_t id;                /* FNV-1a of (kind, first, second), NativeMetalSolver.swift:1047-1052 */
+    int64_t first;              /* bubble A id                                                     */
+    int64_t second;             /* bubble B id, or the segment id                                  */
+    uint32_t kind;              /* BP_CONTACT_*                                                    */
+    uint32_t feature;           /* polygon edge: 1 edge interior, 2 convex start vertex; else 0   */
+    float normal[2];
+    float point[2];             /* pointQ                                                          */
+    float penetration;
+    float allowed_side;         /* 0: the segment has no allowed side (the reference's nil)        */
+    float impulse;              /* accumulatedCompression: F dt^2                                  */
+    uint32_t index_b;           /* bubble B index, or the segment index                            */
+} bp_contact;
+/* Writes up to `capacity` records; *out_count receives the number of records the step has
+ * (more than `capacity` means the list was cut, as the reference cuts at its contact capacity). */
+bp_status bp_world_contacts(bp_world *world, bp_contact *out, uint32_t capacity, uint32_t *out_count);
+
 /* Test switch mirroring the injected NativeMetalCommandExecuting of the reference
  * (NativeMetalSolverLifecycleTests.swift:292-308): the next step fails with BP_ERR_COMMAND_FAILED
  * before touching the device. */
@@ -387,6 +414,8 @@ bp_status bp_batch_last_class_timing(const bp_batch *batch, float *milliseconds,
                                      uint32_t *capacity);
 /* FP64 audit (bp_world_audit) of every world: out[world_count]. */
 bp_status bp_batch_audit(bp_batch *batch, bp_audit *out, uint32_t count);
+/* End-of-step contact records (bp_world_contacts) of one world of a batch. */
+bp_status bp_batch_contacts(bp_batch *batch, uint32_t world, bp_contact *out, uint32_t capacity, uint32_t *out_count);
 /* Autotuner: for each bin of world sizes (bin b: 2^b to 2^(b+1) - 1 bubbles), times one step of
  * the bin's worlds in every kernel configuration that holds them all (the LDS-resident
  * configurations and the one-workgroup global-state kernel: workgroup shape and the LDS-to-HBM

@@ -1382,6 +1382,118 @@ void bpo_audit(const bp_world_config *config, const bp_bubble *bubbles, uint32_t
     out->contact_energy = pot; out->contacts = contacts; out->bubbles = n;
 }
 
+// End-of-step contact records of a world state (the records NativeMetalSolver.swift:846-871 builds
+// after the region pass and returns as NativeSolverContact, :1041-1060):
+//   nativeBuildContacts (Geometry:683-728) per candidate, with nativePolygonEdgeFeature (:674-681);
+//   kept iff penetration > -separation tolerance (Contact:146-153), compacted in candidate order
+//   (Contact:155-160), the candidate order being every bubble's pair candidates, then every
+//   bubble's segment candidates (Geometry:625-651);
+//   impulse = the law's force at the distance the record implies times dt^2 (Force:261-280);
+//   id = the FNV-1a steps of NativeMetalSolver.swift:1047-1052 over (kind, first, second).
+// The candidates are read from the rows of the last grid build (row_offsets[n + 1], entries
+// (kind << 30) | index): bubble i's pair candidates are its pair entries above i, its segment
+// candidates its segment entries. Returns the number of kept records, writing up to `capacity`.
+uint32_t bpo_contacts(const bp_world_config *config, const bp_bubble *bubbles, uint32_t n, const bp_segment *segments,
+                      uint32_t ns, const uint32_t *row_offsets, const uint32_t *entries, bp_contact *out, uint32_t capacity) {
+    using P = bpo::V2<float>;
+    const float sepTol = config->separation_tolerance, dt = config->time_step;
+    uint32_t kept = 0;
+    for (uint32_t kind = 0; kind < 2; ++kind) {
+        for (uint32_t i = 0; i < n; ++i) {
+            const bp_bubble &a = bubbles[i];
+            const P pa = bpo::v2(a.center[0], a.center[1]);
+            const float ra = a.radius;
+            for (uint32_t e = row_offsets[i]; e < row_offsets[i + 1]; ++e) {
+                const uint32_t entryKind = entries[e] >> 30, j = entries[e] & ((1u << 30) - 1u);
+                if (entryKind != kind || (kind == 0 && j <= i)) continue;
+                P normal, point;
+                float penetration, allowedSide = 0.0f, force;
+                uint32_t feature = 0;
+                int64_t second;
+                if (kind == 0) {
+                    const bp_bubble &b = bubbles[j];
+                    const P delta = bpo::v2(b.center[0], b.center[1]) - pa;
+                    const float dist = bpo::length(delta);
+                    normal = dist > 0.0f ? delta / dist : bpo::v2(1.0f, 0.0f);
+                    point = pa + normal * ra;
+                    penetration = ra + b.radius - dist;
+                    second = b.id;
+                } else {
+                    if (j >= ns) continue;
+                    const bp_segment &sg = segments[j];
+                    const P sa = bpo::v2(sg.a[0], sg.a[1]), sb = bpo::v2(sg.b[0], sg.b[1]);
+                    const P edge = sb - sa;
+                    const float lengthSquared = bpo::dot(edge, edge);
+                    const float t = lengthSquared > 0.0f ? bpo::vclamp(bpo::dot(pa - sa, edge) / lengthSquared, 0.0f, 1.0f) : 0.0f;
+                    point = sa + t * edge;
+                    P delta = pa - point;
+                    float dist = bpo::length(delta);
+                    const float edgeLength = bpo::length(edge);
+                    const P base = edgeLength > 0.0f ? bpo::v2(-edge.y, edge.x) / edgeLength : bpo::v2(0.0f, 1.0f);
+                    const float sign = (((uint64_t)a.id ^ (uint64_t)sg.id) & 1u) == 0u ? 1.0f : -1.0f;
+                    const bool sided = (sg.flags & BP_SEGMENT_ALLOWED_SIDE) != 0u;
+                    normal = sided ? base * sg.allowed_side : (dist > 0.0f ? delta / dist : base * sign);
+                    allowedSide = sided ? sg.allowed_side : 0.0f;
+                    penetration = ra - dist;
+                    if (sg.flags & BP_SEGMENT_POLYGON_EDGE) {
+                        const P outward = base * sg.outward_sign;
+                        const float along = bpo::dot(pa - sa, sb - sa);
+                        const bool convexStart = (sg.flags & BP_SEGMENT_CONVEX_START) != 0u;
+                        const P previous = bpo::v2(sg.previous_start[0], sg.previous_start[1]);
+                        if (convexStart && along <= 0.0f && bpo::past_segment_end(pa, previous, sa)) feature = 2;
+                        else feature = bpo::dot(pa - sa, outward) >= 0.0f && along > 0.0f && !bpo::past_segment_end(pa, sa, sb) ? 1u : 0u;
+                        if (feature == 1) {
+                            normal = outward;
+                            const float depth = bpo::dot(pa - sa, outward);
+                            point = pa - outward * depth;
+                            penetration = ra - depth;
+                        } else if (feature == 2) {
+                            point = sa;
+                            delta = pa - point;
+                            dist = bpo::length(delta);
+                            normal = dist > 0.0f ? delta / dist : outward;
+                            penetration = ra - dist;
+                        } else {
+                            normal = outward;
+                            point = sa;
+                            penetration = -FLT_MAX;
+                        }
+                    }
+                    second = sg.id;
+                }
+                if (!(penetration > -sepTol)) continue;
+                if (kind == 0) {
+                    const bp_bubble &b = bubbles[j];
+                    force = bpo::pair_contact_law<float>(a.contact_stiffness, ra, b.contact_stiffness, b.radius,
+                                                         ra + b.radius - penetration).force;
+                } else {
+                    force = bpo::wall_contact_law<float>(a.contact_stiffness, ra, ra - penetration).force;
+                }
+                if (kept < capacity) {
+                    bp_contact &c = out[kept];
+                    uint64_t h = 14695981039346656037ull;
+                    h = (h ^ (uint64_t)kind) * 1099511628211ull;
+                    h = (h ^ (uint64_t)a.id) * 1099511628211ull;
+                    h = (h ^ (uint64_t)second) * 1099511628211ull;
+                    c.id = h;
+                    c.first = a.id;
+                    c.second = second;
+                    c.kind = kind;
+                    c.feature = feature;
+                    c.normal[0] = normal.x; c.normal[1] = normal.y;
+                    c.point[0] = point.x; c.point[1] = point.y;
+                    c.penetration = penetration;
+                    c.allowed_side = allowedSide;
+                    c.impulse = force * dt * dt;
+                    c.index_b = j;
+                }
+                ++kept;
+            }
+        }
+    }
+    return kept;
+}
+
 // Candidate rows of one grid build with a given cell width and margins (the kernel
 // nativeEmitCandidates, Geometry:300-340, and its hashed twin, Force:1010-1035).
 // rows[n * limit * 2] receives (kind, other index) per slot, row_counts[2n] the bubble and

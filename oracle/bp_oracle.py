@@ -52,6 +52,8 @@ def load():
         lib.bpo_half_impulse_factor_f32.argtypes = [P, P, P, U32]
         lib.bpo_detmath_f32.argtypes = [ctypes.c_int, P, P, U32]
         lib.bpo_audit.argtypes = [P, P, U32, P, U32, P, U32, P, U32, P, U32, P]
+        lib.bpo_contacts.argtypes = [P, P, U32, P, U32, P, P, P, U32]
+        lib.bpo_contacts.restype = U32
         _lib = lib
     return _lib
 
@@ -123,6 +125,26 @@ def audit(scene, bubbles=None, bodies=None):
     lib.bpo_audit(ctypes.byref(scene.config), _p(b), len(b), _p(scene.segments), len(scene.segments), _p(bod), len(bod),
                   _p(verts), len(verts), _p(tris), len(tris), _p(out))
     return out[0]
+
+
+CONTACT_FIELDS = [("id", np.uint64), ("first", np.int64), ("second", np.int64), ("kind", np.uint32), ("feature", np.uint32),
+                  ("normal", np.float32, (2,)), ("point", np.float32, (2,)), ("penetration", np.float32),
+                  ("allowed_side", np.float32), ("impulse", np.float32), ("index_b", np.uint32)]
+CONTACT_DTYPE = np.dtype(CONTACT_FIELDS)
+
+
+def contacts(scene, bubbles, row_offsets, entries, capacity=None):
+    """End-of-step contact records (bp_contact) of a world state, from the rows of its last grid
+    build (run(..., adjacency=True) extras) on the scene's configuration and segments."""
+    lib = load()
+    b = np.ascontiguousarray(bubbles)
+    offs = np.ascontiguousarray(row_offsets, np.uint32)
+    ents = np.ascontiguousarray(entries, np.uint32)
+    cap = int(offs[-1]) if capacity is None else capacity
+    out = np.zeros(max(cap, 1), CONTACT_DTYPE)
+    k = lib.bpo_contacts(ctypes.byref(scene.config), _p(b), len(b), _p(scene.segments), len(scene.segments), _p(offs),
+                         _p(ents), _p(out), cap)
+    return out[: min(k, cap)], int(k)
 
 
 def noncontact_acceleration(scene):

@@ -36,7 +36,7 @@ def test_record_layouts_match_the_header(hip_library):
                                           abi.FrameInfo, abi.BatchWorldDesc, abi.Field)]
     assert list(sizes) == expected
     assert abi.BUBBLE_DTYPE.itemsize == 72 and abi.SEGMENT_DTYPE.itemsize == 64 and abi.BODY_DTYPE.itemsize == 96
-    assert abi.FIELD_DTYPE.itemsize == 64
+    assert abi.FIELD_DTYPE.itemsize == 64 and abi.CONTACT_DTYPE.itemsize == 64
     server = (ctypes.c_uint32 * 4)()
     hip_library.bp_server_record_sizes(server)
     assert list(server) == [ctypes.sizeof(t) for t in (abi.Request, abi.ServerResult, abi.ServerConfig, abi.ServerStats)]
