@@ -64,6 +64,13 @@ bp_status validate_world(const HostWorld &w) {
 }  // namespace
 
 bp_status bp_batch::pack() {
+    // A failed world stays failed across a repack (it is not advanced again until it receives
+    // new input): keep the device's sticky flags of the worlds packed last time.
+    if (descs && !hostDescs.empty()) {
+        std::vector<bpd::WorldDesc> cur(std::min(hostDescs.size(), worlds.size()));
+        if (!cur.empty() && hipMemcpy(cur.data(), descs, cur.size() * sizeof(bpd::WorldDesc), hipMemcpyDeviceToHost) == hipSuccess)
+            for (size_t w = 0; w < cur.size(); ++w) worlds[w].failed = cur[w].failed != 0;
+    }
     release();
     uint32_t nb = 0, ns = 0, nbod = 0, nv = 0, nt = 0, nm = 0, nband = 0, nf = 0, nfv = 0, nfi = 0;
     uint64_t adj = 0, coopCells = 0;
@@ -184,7 +191,7 @@ bp_status bp_batch::pack() {
         d.injectionEnabled = c.injection_enabled; d.injectionAfter = c.injection_after_substep;
         d.injectionObject = c.injection_object;
         d.injectionPosition[0] = c.injection_position[0]; d.injectionPosition[1] = c.injection_position[1];
-        d.failed = 0;
+        d.failed = w.failed ? 1u : 0u;
         d.exportAdjacency = (w.exportAdjacency || w.config.export_adjacency) ? 1u : 0u;
         d.bubbleCapacity = w.capacity;
         const uint32_t resident = w.resident ? n : 0u;       // bubbles staged in the device range
@@ -439,6 +446,13 @@ bp_status bp_batch::write_bubbles(uint32_t wi, const bp_bubble *bubbles, uint32_
         bp_status s = download_state();
         if (s != BP_OK) return s;
         w.bubbles.assign(bubbles, bubbles + count);
+        // New input: the world may be stepped again (the repack keeps the other worlds' flags).
+        w.failed = false;
+        if (packed && descs && wi < hostDescs.size()) {
+            bpd::WorldDesc d;
+            BP_HIP(hipMemcpy(&d, descs + wi, sizeof d, hipMemcpyDeviceToHost));
+            if (d.failed) { d.failed = 0; BP_HIP(hipMemcpy(descs + wi, &d, sizeof d, hipMemcpyHostToDevice)); }
+        }
         packed = false;
         return BP_OK;
     }
@@ -469,6 +483,7 @@ bp_status bp_batch::write_bubbles(uint32_t wi, const bp_bubble *bubbles, uint32_
     d.bubbleCount = count;
     d.buckets = w.config.grid_bucket_count == 0 ? auto_buckets(count) : std::min(w.config.grid_bucket_count, bpd::kMaxBuckets);
     d.failed = 0;
+    w.failed = false;
     BP_HIP(hipMemcpy(descs + wi, &d, sizeof d, hipMemcpyHostToDevice));
     w.bubbles.assign(bubbles, bubbles + count);
     // A world that no longer fits its kernel configuration moves to another one: only the order
@@ -544,8 +559,10 @@ bp_status bp_batch::launch(const uint32_t *subset, uint32_t subsetCount, uint32_
         // a small world with hundreds of substeps per step starts first instead of last.
         if (orderDirty) {
             for (int k = 0; k < kClassCount; ++k) {
+                // Every non-empty list is uploaded: a class's membership may have changed (the
+                // autotuner moves worlds between classes and marks the order dirty).
                 auto &list = classWorlds[k];
-                if (list.size() < 2) continue;
+                if (list.empty()) continue;
                 std::stable_sort(list.begin(), list.end(), [&](uint32_t a, uint32_t b) { return workEstimate[a] > workEstimate[b]; });
                 bp_status s = upload(orders[k], list);
                 if (s != BP_OK) return s;

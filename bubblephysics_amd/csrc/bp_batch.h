@@ -106,6 +106,7 @@ struct HostWorld {
     bool resident = true;               // false: held in host memory only (serving eviction)
     uint32_t capacity = 0;              // bubble slots of its device range (>= bubbles), set by pack
     int packedClass = -1;               // kernel configuration of the last pack (-1 = not on the device)
+    bool failed = false;                // failed a step: not advanced again until new input (kept across packs)
 };
 
 inline bool class_fits(int k, const HostWorld &w) {

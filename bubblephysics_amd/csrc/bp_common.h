@@ -163,6 +163,16 @@ BPD int64_t cell_coord(float x, float width) {
     return fabsf(f) < 2147483648.0f ? (int64_t)(int32_t)f : (int64_t)f;
 }
 
+// The cell offsets along one axis (within -1..1 of the centre's cell `c`) that can hold another
+// centre at FP32 distance 0 from a centre with coordinate `v`: the contact assertion's pair test
+// (Force:880-881). sqrt(dx^2 + dy^2) is 0 only when both squares flush to zero, so
+// |dx|, |dy| < 2.7e-23, and floor(x / w) is monotonic in x, so the cells of v - 1e-20 and
+// v + 1e-20 bound the cell of every such centre. Away from a cell boundary this is the own cell.
+BPD void zero_distance_span(float v, float width, int64_t c, int &lo, int &hi) {
+    lo = cell_coord(v - 1e-20f, width) < c ? -1 : 0;
+    hi = cell_coord(v + 1e-20f, width) > c ? 1 : 0;
+}
+
 // ---------------------------------------------------------------------------------------
 // Geometry, force records and configuration of one world (working side of the body poses).
 // ---------------------------------------------------------------------------------------

@@ -198,8 +198,11 @@ BPD bool assert_contacts(World &W, Shared<BLOCK> &sh, const Ctl &c, WorldStatus 
         uint32_t bestJ = 0xffffffffu;
         float bestD = 0.0f;
         int64_t cx = W.cell_x(center), cy = W.cell_y(center);
-        for (int dx = -1; dx <= 1; ++dx)
-            for (int dy = -1; dy <= 1; ++dy) {
+        int x0, x1, y0, y1;
+        zero_distance_span(center.x, W.cellWidth, cx, x0, x1);
+        zero_distance_span(center.y, W.cellWidth, cy, y0, y1);
+        for (int dx = x0; dx <= x1; ++dx)
+            for (int dy = y0; dy <= y1; ++dy) {
                 int64_t tx = cx + dx, ty = cy + dy;
                 uint32_t b = bucket_of(tx, ty, buckets);
                 uint32_t lo = b == 0 ? 0u : sh.buckets[b - 1], hi = sh.buckets[b];

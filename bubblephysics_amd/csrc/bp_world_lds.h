@@ -697,8 +697,11 @@ __global__ __launch_bounds__(BLOCK) void world_lds_kernel(const Arena *__restric
                 uint32_t bestJ = 0xffffffffu;
                 float bestD = 0.0f;
                 const int64_t cx = cell_coord(center.x, cellWidth), cy = cell_coord(center.y, cellWidth);
-                for (int dx = -1; dx <= 1; ++dx)
-                    for (int dy = -1; dy <= 1; ++dy) {
+                int x0, x1, y0, y1;
+                zero_distance_span(center.x, cellWidth, cx, x0, x1);
+                zero_distance_span(center.y, cellWidth, cy, y0, y1);
+                for (int dx = x0; dx <= x1; ++dx)
+                    for (int dy = y0; dy <= y1; ++dy) {
                         const int64_t tx = cx + dx, ty = cy + dy;
                         const uint32_t b = bucket_of(tx, ty, buckets);
                         const uint32_t lo = b == 0 ? 0u : S.buckets[b - 1], hi = S.buckets[b];

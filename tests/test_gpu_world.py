@@ -179,6 +179,35 @@ def test_injected_centre_is_a_contact_violation(oracle):
         assert frame.detail.object_ids == (int(c["violation_objects"][0]), int(c["violation_objects"][1]))
 
 
+@pytest.mark.parametrize("cls", [None, 10])
+def test_centre_injected_onto_a_static_bubble_is_a_pair_violation(oracle, cls):
+    # ContactAssertionTests.swift:35-83 with the target held still, so the centres coincide
+    # exactly after the injection: the pair test d <= 0 (Force:880-881) must fire, in the
+    # LDS-resident kernel and in the global-state kernel alike.
+    scene = scenes.config1()
+    target = 20
+    scene.bubbles["inverse_mass"][target] = 0.0
+    scene.bubbles["flags"][target] |= abi.BP_BUBBLE_STATIC
+    scene.config.assert_every_substep = 1
+    scene.config.injection_enabled = 1
+    scene.config.injection_after_substep = 0
+    scene.config.injection_object = int(scene.bubbles["id"][10])
+    scene.config.injection_position[:] = [float(v) for v in scene.bubbles["center"][target]]
+    _, _, cf, _ = oracle.run(scene, 1)
+    c = cf[0]
+    assert c["status"] == abi.BP_ERR_CONTACT_VIOLATION and c["violation_kind"] == abi.BP_VIOLATION_PAIR
+    batch = solver.Batch([scene])
+    if cls is not None:
+        batch.set_world_class(0, cls)
+    info = batch.step(1)[0][0]
+    assert info["status"] == c["status"]
+    assert info["detail"]["violation_kind"] == c["violation_kind"]
+    assert (int(info["detail"]["object_ids"][0]), int(info["detail"]["object_ids"][1])) == (
+        int(c["violation_objects"][0]), int(c["violation_objects"][1]))
+    if cls is not None:
+        assert info["kernel_class"] == cls
+
+
 def test_command_failure_switch_leaves_the_world_usable(oracle):
     scene = scenes.config1()
     world = solver.NativeWorld(scene)

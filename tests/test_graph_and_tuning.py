@@ -65,9 +65,12 @@ def test_autotune_keeps_state_and_results():
     plain.step(1)
     before = [tuned.read_bubbles(w) for w in range(len(ws))]
     table = autotune.tune(tuned, repeats=1)
-    assert len(table) >= 5 and np.all(table["best_class"] >= 0) and np.all(table["best_rate"] > 0)
+    # best_class -1 keeps the automatic classes (no configuration beat them by more than 2 %).
+    assert len(table) >= 5 and np.all(table["best_class"] >= -1) and np.all(table["best_rate"] > 0)
     for r in table:
-        assert r["class_rate"][r["best_class"]] > 0
+        assert (r["class_rate"] > 0).any()
+        if r["best_class"] >= 0:
+            assert r["class_rate"][r["best_class"]] > 0
     for w in range(len(ws)):
         assert same_bits(tuned.read_bubbles(w), before[w]), w
     ti, pi = tuned.step(3), plain.step(3)
