@@ -271,9 +271,10 @@ bp_status bp_batch::pack() {
         oadj += (uint64_t)d.adjCapacity;
     }
     // Longest-processing-time order inside each class: the largest worlds start first (until a
-    // step has measured each world's substeps, see launch()).
+    // step has measured each world's device time, see launch()).
     workEstimate.assign(worlds.size(), 0);
     for (size_t wi = 0; wi < worlds.size(); ++wi) workEstimate[wi] = hostDescs[wi].bubbleCount;
+    for (int k = 0; k < kClassCount; ++k) classOrder[k] = kClassCount - 1 - k;
     orderDirty = false;
     for (auto &k : classWorlds)
         std::stable_sort(k.begin(), k.end(), [&](uint32_t a, uint32_t b) { return hostDescs[a].bubbleCount > hostDescs[b].bubbleCount; });
@@ -555,8 +556,9 @@ bp_status bp_batch::launch(const uint32_t *subset, uint32_t subsetCount, uint32_
     } else {
         for (size_t wi = 0; wi < W; ++wi) pendingList.push_back((uint32_t)wi);
         for (int k = 0; k < kClassCount; ++k) { launchOffset[k] = 0; launchCount[k] = (uint32_t)classWorlds[k].size(); }
-        // Longest-processing-time order by measured work (bubbles x substeps of the last step):
-        // a small world with hundreds of substeps per step starts first instead of last.
+        // Longest-processing-time order by measured device time of the last step: a small world
+        // with hundreds of substeps per step starts first instead of last, and its class is
+        // launched first.
         if (orderDirty) {
             for (int k = 0; k < kClassCount; ++k) {
                 // Every non-empty list is uploaded: a class's membership may have changed (the
@@ -567,6 +569,12 @@ bp_status bp_batch::launch(const uint32_t *subset, uint32_t subsetCount, uint32_
                 bp_status s = upload(orders[k], list);
                 if (s != BP_OK) return s;
             }
+            uint64_t longest[kClassCount];
+            for (int k = 0; k < kClassCount; ++k) {
+                classOrder[k] = k;
+                longest[k] = classWorlds[k].empty() ? 0 : workEstimate[classWorlds[k].front()];
+            }
+            std::stable_sort(classOrder, classOrder + kClassCount, [&](int a, int b) { return longest[a] > longest[b]; });
             orderDirty = false;
         }
     }
@@ -582,8 +590,8 @@ bp_status bp_batch::launch(const uint32_t *subset, uint32_t subsetCount, uint32_
     BP_HIP(hipMemsetAsync(stats, 0, need * sizeof(bpd::WorldStatus), streams[0]));
     BP_HIP(hipEventRecord(startEvent, streams[0]));
     for (int k = 1; k < kStreams; ++k) BP_HIP(hipStreamWaitEvent(streams[k], startEvent, 0));
-    // Largest classes first so that their long-running workgroups start early on their stream.
-    for (int k = kClassCount - 1; k >= 0; --k) {
+    for (int idx = 0; idx < kClassCount; ++idx) {
+        const int k = classOrder[idx];
         const uint32_t count = launchCount[k];
         if (!count || k == kCoopClass) continue;
         hipStream_t stream = streams[k % kStreams];
@@ -664,13 +672,16 @@ bp_status bp_batch::collect(uint32_t steps, bp_frame_info *infos, const std::vec
     BP_HIP(hipMemcpy(hostStats.data(), stats, need * sizeof(bpd::WorldStatus), hipMemcpyDeviceToHost));
     if (workEstimate.size() == W)
         for (uint32_t wi : listedWorlds) {
-            uint64_t sub = 0;
+            // Device ticks per executed step; the order changes only when an estimate moves by
+            // more than a quarter (the ticks of a world vary a little from step to step).
+            uint64_t ticks = 0, executed = 0;
             for (uint32_t st = 0; st < steps; ++st) {
                 const bpd::WorldStatus &r = hostStats[(size_t)st * W + wi];
-                if (r.executed) sub += r.substeps;
+                if (r.executed) { ticks += r.ticks; ++executed; }
             }
-            const uint64_t est = (uint64_t)hostDescs[wi].bubbleCount * std::max<uint64_t>(1, sub / std::max(steps, 1u));
-            if (est != workEstimate[wi]) { workEstimate[wi] = est; orderDirty = true; }
+            if (!executed) continue;
+            const uint64_t est = std::max<uint64_t>(1, ticks / executed), old = workEstimate[wi];
+            if (4 * est > 5 * old || 5 * est < 4 * old) { workEstimate[wi] = est; orderDirty = true; }
         }
     if (!infos) return BP_OK;
     const size_t listed = listedWorlds.size();
@@ -799,7 +810,8 @@ bp_status bp_batch::capture(uint32_t steps) {
     hipError_t e = bp_launch_fill_u32(reinterpret_cast<uint32_t *>(stats), need * sizeof(bpd::WorldStatus) / 4, 0u, origin);
     if (e == hipSuccess) e = hipEventRecord(forkEvent, origin);
     for (int k = 1; k < kStreams && e == hipSuccess; ++k) e = hipStreamWaitEvent(streams[k], forkEvent, 0);
-    for (int k = kClassCount - 1; k >= 0 && e == hipSuccess; --k) {
+    for (int idx = 0; idx < kClassCount && e == hipSuccess; ++idx) {
+        const int k = classOrder[idx];
         const uint32_t count = (uint32_t)classWorlds[k].size();
         if (!count || k == kCoopClass) continue;
         hipStream_t stream = streams[k % kStreams];

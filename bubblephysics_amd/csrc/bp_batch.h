@@ -150,7 +150,10 @@ struct bp_batch {
     uint32_t *orders[bph::kClassCount] = {};
     std::vector<uint32_t> classWorlds[bph::kClassCount];
     std::vector<uint32_t> coopGroups;   // per world: workgroups of its cooperative launch (0 = not cooperative)
-    std::vector<uint64_t> workEstimate; // per world: bubbles x substeps of its last step (launch order key)
+    std::vector<uint64_t> workEstimate; // per world: device ticks of its last step (launch order key)
+    // Launch order of the classes: the class holding the longest-running world first, so that
+    // the world on the critical path of the batch gets a compute unit before the others.
+    int classOrder[bph::kClassCount] = {11, 10, 9, 8, 7, 6, 5, 4, 3, 2, 1, 0};
     bool orderDirty = false;            // estimates changed since the class order lists were uploaded
     hipStream_t streams[bph::kStreams] = {};
     hipEvent_t startEvent = nullptr, endEvents[bph::kStreams] = {};

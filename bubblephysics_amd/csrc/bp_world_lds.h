@@ -71,7 +71,7 @@ struct RowRef {
     BPD uint32_t index(uint32_t k) const { return inLds ? (uint32_t)lds[k] & kLdsIndexMask : global[k] & kIndexMask; }
 };
 
-constexpr uint32_t kHasMotor = 1u << 31, kHasTorque = 1u << 30;   // register-only flag bits
+constexpr uint32_t kHasMotor = 1u << 31, kHasTorque = 1u << 30, kHasBand = 1u << 29;   // register-only flag bits
 
 // Neighbour index of row entry e of the block [start, end): every fourth entry loads the next
 // four indices together (one memory round trip for four entries, which matters most for rows
@@ -167,19 +167,21 @@ __global__ __launch_bounds__(BLOCK) void world_lds_kernel(const Arena *__restric
         const uint32_t buckets = min(d->buckets, L::NB);
 
         // ---- 1. Begin step (Force:285-293): own state into registers, neighbour fields into LDS.
-        float2 p[PER], v[PER], a[PER], ref[PER], mf[PER];
-        float rot[PER], om[PER], mg[PER], gm[PER], mq[PER];
-        uint32_t fl[PER], rs[PER], rl[PER], rp[PER], rq[PER];
-        int32_t bi[PER];
+        // Drag, motor and band records stay in memory and are read, for the bubbles whose flags
+        // say they have them, where they are used: registers hold only what every bubble needs.
+        // The row bounds of a bubble are its start, its length and, packed in one word, the
+        // lengths of its pair block and of its pair and segment blocks (rows are below 2^16).
+        float2 p[PER], v[PER], a[PER], ref[PER];
+        float rot[PER], om[PER], mg[PER];
+        uint32_t fl[PER], rs[PER], rl[PER], rpq[PER];
         bool valid[PER];
     #pragma unroll
         for (int s = 0; s < PER; ++s) {
             uint32_t i = lane + s * BLOCK;
             valid[s] = i < n;
-            p[s] = v[s] = a[s] = ref[s] = mf[s] = f2(0.0f, 0.0f);
-            rot[s] = om[s] = mg[s] = gm[s] = mq[s] = 0.0f;
-            fl[s] = 0; rs[s] = rl[s] = rp[s] = rq[s] = 0;
-            bi[s] = -1;
+            p[s] = v[s] = a[s] = ref[s] = f2(0.0f, 0.0f);
+            rot[s] = om[s] = mg[s] = 0.0f;
+            fl[s] = 0; rs[s] = rl[s] = rpq[s] = 0;
             if (!valid[s]) continue;
             uint32_t gi = o + i;
             p[s] = A->pos[pub][gi];
@@ -197,16 +199,14 @@ __global__ __launch_bounds__(BLOCK) void world_lds_kernel(const Arena *__restric
             S.K[i] = A->Kc[gi];
             S.invm[i] = isStatic ? 0.0f : invmRaw;
             // Force records (Force:423-438, resolved to indices on the host) and drag, per step.
-            bi[s] = A->band[gi];
             const int32_t mi = A->motor[gi];
             const bool driven = (fl[s] & kForceDriven) != 0;
-            gm[s] = driven ? fmaxf(A->gamma[gi], 0.0f) : 0.0f;
-            fl[s] &= ~(kHasMotor | kHasTorque);
+            fl[s] &= ~(kHasMotor | kHasTorque | kHasBand);
+            if (A->band[gi] >= 0) fl[s] |= kHasBand;
             if (mi >= 0) {
                 const float4 md = g.motor(mi);
                 fl[s] |= kHasMotor;
-                mf[s] = f2(md.x, md.y);
-                if (driven && md.w > 0.0f) { fl[s] |= kHasTorque; mq[s] = md.z / md.w; }
+                if (driven && md.w > 0.0f) fl[s] |= kHasTorque;
             }
         }
         // World-uniform shortcuts of the candidate tests: when every bubble has a nonzero category
@@ -299,8 +299,20 @@ __global__ __launch_bounds__(BLOCK) void world_lds_kernel(const Arena *__restric
             f.round = S.bodyKind[k].x != 0u;
             return f;
         };
+        // The per-bubble records read on demand (drag only for force-driven bubbles, as the step
+        // applies it; the motor's torque term is (torque / inertia), computed as at the start).
+        auto gamma_of = [&](int s) __attribute__((always_inline)) -> float {
+            return (fl[s] & kForceDriven) ? fmaxf(A->gamma[o + lane + s * BLOCK], 0.0f) : 0.0f;
+        };
+        auto motor_of = [&](int s) __attribute__((always_inline)) -> float4 { return g.motor(A->motor[o + lane + s * BLOCK]); };
+        auto band_of = [&](int s) __attribute__((always_inline)) -> int32_t {
+            return (fl[s] & kHasBand) ? A->band[o + lane + s * BLOCK] : -1;
+        };
         auto a0_of = [&](int s, float invm, float2 pi) __attribute__((always_inline)) -> float2 {
-            return g.non_contact_with(fl[s], invm, ids + lane + s * BLOCK, bi[s], (fl[s] & kHasMotor) != 0u, mf[s], pi);
+            if (!(fl[s] & kForceDriven) || !(invm > 0.0f)) return f2(0.0f, 0.0f);
+            const bool motor = (fl[s] & kHasMotor) != 0u;
+            const float4 md = motor ? motor_of(s) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            return g.non_contact_with(fl[s], invm, ids + lane + s * BLOCK, band_of(s), motor, f2(md.x, md.y), pi);
         };
         // Candidate tests against the staged geometry (Geometry:279-290, 361-379).
         auto segment_candidate = [&](float2 pc, float radius, float margin, uint32_t cat, uint32_t mask, uint32_t j)
@@ -492,7 +504,7 @@ __global__ __launch_bounds__(BLOCK) void world_lds_kernel(const Arena *__restric
                     }, forward);
                     sort_pair_block(row, pc);
                 }
-                rp[s] = pc; rq[s] = pc + sc;
+                rpq[s] = pc | ((pc + sc) << 16);
             }
             __syncthreads();
             return true;
@@ -515,9 +527,9 @@ __global__ __launch_bounds__(BLOCK) void world_lds_kernel(const Arena *__restric
                 const float invm = S.invm[i], ri = S.r[i], Ki = S.K[i];
                 float2 total = f2(0.0f, 0.0f);
                 float w2 = 0.0f;
-                if (invm > 0.0f && bi[s] >= 0) w2 = w2 + g.band(bi[s]).z * invm;
+                if (invm > 0.0f && (fl[s] & kHasBand)) w2 = w2 + g.band(band_of(s)).z * invm;
                 // The row is (pairs, segments, bodies): the reference's summation order (Force:216-255).
-                const uint32_t start = rs[s], pairEnd = start + rp[s], segEnd = start + rq[s], end = start + rl[s];
+                const uint32_t start = rs[s], pairEnd = start + (rpq[s] & 0xffffu), segEnd = start + (rpq[s] >> 16), end = start + rl[s];
                 uint32_t jn[4] = {0u, 0u, 0u, 0u};
                 for (uint32_t e = start; e < pairEnd; ++e) {
                     const uint32_t j = row_entry4(rows, e, start, pairEnd, jn);
@@ -546,7 +558,7 @@ __global__ __launch_bounds__(BLOCK) void world_lds_kernel(const Arena *__restric
                     if (phase == 2) {
                         a[s] = add(mul(total, invm), a0_of(s, invm, pi));
                     } else if (invm > 0.0f) {
-                        const float gamma = gm[s];
+                        const float gamma = gamma_of(s);
                         const float decay = bpm::expf_det(-gamma * halfStep), factor = half_impulse_factor(gamma, halfStep);
                         a[s] = add(mul(total, invm), a0_of(s, invm, pi));
                         v[s] = add(mul(v[s], decay), mul(a[s], factor));
@@ -602,7 +614,7 @@ __global__ __launch_bounds__(BLOCK) void world_lds_kernel(const Arena *__restric
                 const float2 pi = p[s], vi = v[s];
                 const float ri = S.r[i], accI = S.aux[i];
                 float best = inff();
-                const uint32_t start = rs[s], pairEnd = start + rp[s], segEnd = start + rq[s], end = start + rl[s];
+                const uint32_t start = rs[s], pairEnd = start + (rpq[s] & 0xffffu), segEnd = start + (rpq[s] >> 16), end = start + rl[s];
                 uint32_t jn[4] = {0u, 0u, 0u, 0u};
                 for (uint32_t e = start; e < pairEnd; ++e) {
                     const uint32_t j = row_entry4(rows, e, start, pairEnd, jn);
@@ -634,11 +646,14 @@ __global__ __launch_bounds__(BLOCK) void world_lds_kernel(const Arena *__restric
                 if (!valid[s]) continue;
                 const uint32_t i = lane + s * BLOCK;
                 if (!(S.invm[i] > 0.0f)) continue;
-                const float gamma = gm[s];
+                const float gamma = gamma_of(s);
                 const float decay = bpm::expf_det(-gamma * halfStep), factor = half_impulse_factor(gamma, halfStep);
                 const float2 vn = add(mul(v[s], decay), mul(a[s], factor));
                 float angular = om[s] * bpm::expf_det(-gamma * h);
-                if (fl[s] & kHasTorque) angular = angular + mq[s] * half_impulse_factor(gamma, h);
+                if (fl[s] & kHasTorque) {
+                    const float4 md = motor_of(s);
+                    angular = angular + (md.z / md.w) * half_impulse_factor(gamma, h);
+                }
                 p[s] = add(p[s], mul(vn, h));
                 v[s] = vn;
                 rot[s] = rot[s] + om[s] * h;
