@@ -165,9 +165,10 @@ def run_plain(steps: int):
     return {"value": total / elapsed, "p50_ms": percentile(lat, 50), "p99_ms": percentile(lat, 99), "bubbles": n}
 
 
-def cpu_baseline(scenes_list, budget_s: float, threads: int):
-    """The oracle (FP32) on the worlds of the same batch, one step each, spread over `threads` host
-    threads (the oracle library runs outside the interpreter lock), until the wall budget is used."""
+def cpu_baseline(scenes_list, budget_s: float, threads: int, steps: int = 3):
+    """The oracle (FP32) on the worlds of the same batch, `steps` steps each from their initial
+    state, spread over `threads` host threads (the oracle library runs outside the interpreter
+    lock), until the wall budget is used. Executed bubble-substeps of successful steps / wall time."""
     from concurrent.futures import ThreadPoolExecutor
     from oracle import bp_oracle
     bp_oracle.load()
@@ -176,15 +177,16 @@ def cpu_baseline(scenes_list, budget_s: float, threads: int):
     def one(s):
         if time.perf_counter() >= deadline:
             return None
-        _, _, frames, _ = bp_oracle.run(s, 1, assert_mode=1)
-        return len(s.bubbles) * int(frames[0]["substeps"])
+        _, _, frames, _ = bp_oracle.run(s, steps, assert_mode=1)
+        ok = frames["status"] == 0
+        return len(s.bubbles) * int(frames["substeps"][ok].sum())
 
     t0 = time.perf_counter()
     with ThreadPoolExecutor(max_workers=threads) as pool:
         done = [r for r in pool.map(one, scenes_list) if r is not None]
     used = time.perf_counter() - t0
     return {"value": sum(done) / used if used > 0 else float("nan"), "worlds": len(done), "seconds": used,
-            "threads": threads}
+            "threads": threads, "steps": steps}
 
 
 def fp64_drift(checkpoints=(1, 10, 30, 60)):
@@ -342,8 +344,9 @@ def main():
         },
         "cpu_baseline": None if cpu is None else {
             "value": cpu["value"], "unit": "bubble-substeps/s", "cores": cpu["threads"], "kind": "port",
-            "sample": f"oracle FP32, one step of each of the first {cpu['worlds']} worlds of the same batch, worlds "
-                      f"spread over {cpu['threads']} host threads ({cpu['seconds']:.1f}s wall)",
+            "sample": f"oracle FP32, {cpu['steps']} steps of each of the first {cpu['worlds']} worlds of the same "
+                      f"batch from their initial state, worlds spread over {cpu['threads']} host threads "
+                      f"({cpu['seconds']:.1f}s wall, about {cpu['seconds'] * cpu['threads']:.0f} thread-seconds)",
             "fp64_drift": cpu["drift"],
         },
     }
