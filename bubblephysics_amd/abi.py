@@ -194,6 +194,12 @@ class Contact(ctypes.Structure):          # bp_contact
                 ("index_b", ctypes.c_uint32)]
 
 
+class ContactEvent(ctypes.Structure):     # bp_contact_event
+    _fields_ = [("kind", ctypes.c_uint32), ("reserved", ctypes.c_uint32), ("first", ctypes.c_uint64),
+                ("second", ctypes.c_uint64), ("point", ctypes.c_float * 2), ("normal", ctypes.c_float * 2),
+                ("impulse", ctypes.c_float), ("reserved2", ctypes.c_float)]
+
+
 class ServerStats(ctypes.Structure):
     _fields_ = [(name, ctypes.c_uint64) for name in (
         "submitted", "completed", "launches", "launched_worlds", "transactions", "rejected", "evictions", "admissions",
@@ -210,6 +216,8 @@ FIELD_DTYPE = np.dtype(Field)
 FRAME_DTYPE = np.dtype(FrameInfo)
 AUDIT_DTYPE = np.dtype(Audit)
 CONTACT_DTYPE = np.dtype(Contact)
+EVENT_DTYPE = np.dtype(ContactEvent)
+BP_EVENT_CONTACT_BEGAN, BP_EVENT_CONTACT_ENDED = 1, 2
 BP_CONTACT_PAIR, BP_CONTACT_SEGMENT = 0, 1
 TUNE_DTYPE = np.dtype(TuneBin)
 REQUEST_DTYPE = np.dtype(Request)
@@ -244,6 +252,7 @@ SIGNATURES = {
     "bp_batch_audit": ([_P, _P, _U32], _I32),
     "bp_world_contacts": ([_P, _P, _U32, _P], _I32),
     "bp_batch_contacts": ([_P, _U32, _P, _U32, _P], _I32),
+    "bp_contact_transitions": ([_P, _U32, _P, _U32, _P, _U32, ctypes.c_float, _P, _U32, _U32, _P, _P, _U32, _P], _I32),
     "bp_static_polygon_edges": ([_P, _U32, _I64, _I64, _U32, _P, _U32, _P], _I32),
     "bp_polygon_body_geometry": ([_P, _U32, _P, _P, _P, _P, _P, _P], _I32),
     "bp_place_discs": ([_P, _U32, _P, _F, _F, _P, _U32, _U64, _U32, _P, _P], _I32),

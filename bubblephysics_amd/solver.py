@@ -207,6 +207,26 @@ def _read_contacts(call, name: str, capacity: int | None) -> np.ndarray:
     return out[: min(count.value, capacity)]
 
 
+def contact_transitions(contacts: np.ndarray, segments: np.ndarray, interested, step_duration: float,
+                        history: np.ndarray | None = None):
+    """Contact began/ended events of one step (bp_contact_transitions) from its contact records and
+    the previous step's pair set; returns (events, new pair set as an (n, 2) uint64 array)."""
+    lib = abi.load_library()
+    c = np.ascontiguousarray(contacts, abi.CONTACT_DTYPE)
+    seg = np.ascontiguousarray(segments, abi.SEGMENT_DTYPE)
+    ids = np.ascontiguousarray(np.asarray(list(interested), np.uint64))
+    prev = np.zeros((0, 2), np.uint64) if history is None else np.ascontiguousarray(history, np.uint64).reshape(-1, 2)
+    cap = len(c) + len(prev) + 1
+    hist = np.zeros((cap, 2), np.uint64)
+    hist[: len(prev)] = prev
+    events = np.zeros(cap, abi.EVENT_DTYPE)
+    hc, ec = ctypes.c_uint32(0), ctypes.c_uint32(0)
+    abi.check(lib.bp_contact_transitions(abi.ptr(c), len(c), abi.ptr(seg), len(seg), abi.ptr(ids), len(ids),
+                                         ctypes.c_float(step_duration), abi.ptr(hist), len(prev), cap, ctypes.byref(hc),
+                                         abi.ptr(events), cap, ctypes.byref(ec)), "bp_contact_transitions")
+    return events[: ec.value].copy(), hist[: hc.value].copy()
+
+
 class Batch:
     """Many independent worlds advanced together (bp_batch)."""
 

@@ -336,6 +336,32 @@ typedef struct bp_contact {
  * (more than `capacity` means the list was cut, as the reference cuts at its contact capacity). */
 bp_status bp_world_contacts(bp_world *world, bp_contact *out, uint32_t capacity, uint32_t *out_count);
 
+/* Contact transitions of one step: the contactBegan / contactEnded events of
+ * NativePublicWorld.swift:340-391 derived from the step's contact records (bp_world_contacts) and
+ * the previous step's set of contact pairs. Keys are (bubble A, bubble B) or (bubble A, segment
+ * owner), ordered by raw ObjectID; only records with penetration > 0 or impulse > 0 whose key has an
+ * object of interest count. Began events (key order) carry the penetration-weighted mean point,
+ * the normalised normal sum from the key's first object to its second and the impulse F dt;
+ * ended events follow in key order. `history` holds pairs as (first, second) u64 couples: the
+ * previous set on input (history_count pairs), the current set on output (*out_history_count; more
+ * than history_capacity returns BP_ERR_CAPACITY_OVERFLOW). Host-only: no device work. The
+ * reference's rigid-body contact records are not part of the sums. */
+enum { BP_EVENT_CONTACT_BEGAN = 1, BP_EVENT_CONTACT_ENDED = 2 };
+typedef struct bp_contact_event {
+    uint32_t kind;              /* BP_EVENT_CONTACT_*                                             */
+    uint32_t reserved;
+    uint64_t first, second;     /* ObjectID raw values, first <= second                          */
+    float point[2];             /* began: penetration-weighted mean point                         */
+    float normal[2];            /* began: unit normal from first to second (0 when the sum is 0)  */
+    float impulse;              /* began: sum of F dt (record impulse / dt)                       */
+    float reserved2;
+} bp_contact_event;
+bp_status bp_contact_transitions(const bp_contact *contacts, uint32_t count, const bp_segment *segments,
+                                 uint32_t segment_count, const uint64_t *interested, uint32_t interested_count,
+                                 float step_duration, uint64_t *history, uint32_t history_count,
+                                 uint32_t history_capacity, uint32_t *out_history_count,
+                                 bp_contact_event *events, uint32_t event_capacity, uint32_t *out_event_count);
+
 /* Test switch mirroring the injected NativeMetalCommandExecuting of the reference
  * (NativeMetalSolverLifecycleTests.swift:292-308): the next step fails with BP_ERR_COMMAND_FAILED
  * before touching the device. */
