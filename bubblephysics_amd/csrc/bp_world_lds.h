@@ -28,6 +28,7 @@ namespace bpd {
 constexpr uint32_t kLdsKindShift = 14;
 constexpr uint32_t kLdsIndexMask = (1u << 14) - 1;
 constexpr uint32_t kLdsIndexLimit = 1u << 14;
+constexpr uint32_t kStageStride = 64;      // staged row entries per bubble in the global adjacency area
 
 template <int BLOCK, int PER, int ROWCAP, int SEGCAP, int BODYCAP, int VERTCAP, int TRICAP>
 struct LdsWorld {
@@ -445,16 +446,29 @@ __global__ __launch_bounds__(BLOCK) void world_lds_kernel(const Arena *__restric
             if (lane < 8) S.red.counters[lane] = 0;
             build_hash();
             // Pass 1: row lengths and the reference's overflow counters (Geometry:291-297, 410-414).
-            uint32_t droppedLocal = 0, maxLocal = 0, pairsLocal = 0, mine = 0;
+            // Each row is also staged in the world's global adjacency area at a fixed stride
+            // (kStageStride entries per bubble, when the area holds n strides), so that pass 2
+            // copies it instead of enumerating the candidates again. The block lengths of the
+            // row (pairs, then segments, then bodies: the order enumeration emits them in) are
+            // counted here as well.
+            const bool stageOk = (uint64_t)n * kStageStride <= d->adjCapacity;
+            uint32_t *const stageArea = A->adj + d->adjOffset;
+            uint32_t droppedLocal = 0, maxLocal = 0, pairsLocal = 0, mine = 0, longLocal = 0;
     #pragma unroll
             for (int s = 0; s < PER; ++s) {
                 if (!valid[s]) continue;
-                uint32_t cnt = 0, forward = 0;
+                uint32_t cnt = 0, forward = 0, pc = 0, sc = 0;
+                uint32_t *stage = stageArea + (size_t)(lane + s * BLOCK) * kStageStride;
                 enumerate(lane + s * BLOCK, p[s], mg[s], [&](uint32_t e) __attribute__((always_inline)) {
+                    if (stageOk && cnt < kStageStride) stage[cnt] = e;
                     ++cnt;
+                    pc += (e >> 30) == 0u;
+                    sc += (e >> 30) == 1u;
                     if ((e >> 30) == 2u) atomicAdd(&g.bodyHits()[e & kIndexMask], 1u);
                 }, forward);
                 rl[s] = cnt;
+                rpq[s] = pc | ((pc + sc) << 16);
+                longLocal |= cnt > kStageStride ? 1u : 0u;
                 mine += cnt;
                 if (forward > limit) droppedLocal += forward - limit;
                 if (cnt > limit) droppedLocal += cnt - limit;
@@ -470,6 +484,7 @@ __global__ __launch_bounds__(BLOCK) void world_lds_kernel(const Arena *__restric
             atomicAdd(&S.red.counters[0], droppedLocal);
             atomicMax(&S.red.counters[1], maxLocal);
             atomicAdd(&S.red.counters[2], pairsLocal);
+            if (longLocal) atomicOr(&S.red.counters[3], 1u);
             uint32_t total;
             uint32_t base = block_exclusive_scan<BLOCK>(S.red, mine, total);
     #pragma unroll
@@ -479,32 +494,45 @@ __global__ __launch_bounds__(BLOCK) void world_lds_kernel(const Arena *__restric
             ov1 = max(ov1, S.red.counters[1]);
             pairs = S.red.counters[2];
             rows.inLds = ldsRowsAllowed && total <= (uint32_t)ROWCAP;
-            if (!rows.inLds && total > d->adjCapacity) return false;
+            // Rows that stay in global memory stay where pass 1 staged them when every row fit
+            // its stride (rows need not be contiguous: each bubble keeps its start and length).
+            const bool strideRows = !rows.inLds && stageOk && S.red.counters[3] == 0u;
+            if (strideRows) {
+    #pragma unroll
+                for (int s = 0; s < PER; ++s) rs[s] = (lane + s * BLOCK) * kStageStride;
+            }
+            if (!rows.inLds && !strideRows && total > d->adjCapacity) return false;
             // Pass 2: fill the rows and order the pair block (Contact:222-238).
     #pragma unroll
             for (int s = 0; s < PER; ++s) {
                 if (!valid[s]) continue;
-                // Enumeration emits the pairs, then the segments, then the bodies, so the block
-                // lengths are the counts of each kind.
-                uint32_t forward, cnt = 0, pc = 0, sc = 0;
+                const uint32_t i = lane + s * BLOCK, cnt = rl[s], pc = rpq[s] & 0xffffu;
+                uint32_t forward, filled = 0;
                 if (rows.inLds) {
                     uint16_t *row = S.rows + rs[s];
-                    enumerate(lane + s * BLOCK, p[s], mg[s], [&](uint32_t e) __attribute__((always_inline)) {
-                        row[cnt++] = (uint16_t)(((e >> 30) << kLdsKindShift) | (e & kIndexMask));
-                        pc += (e >> 30) == 0u;
-                        sc += (e >> 30) == 1u;
-                    }, forward);
+                    if (stageOk && cnt <= kStageStride) {
+                        const uint32_t *src = stageArea + (size_t)i * kStageStride;
+                        for (uint32_t x = 0; x < cnt; x += 4) {
+                            uint32_t ee[4];
+    #pragma unroll
+                            for (int u = 0; u < 4; ++u) ee[u] = src[min(x + u, cnt - 1)];
+    #pragma unroll
+                            for (int u = 0; u < 4; ++u)
+                                if (x + u < cnt) row[x + u] = (uint16_t)(((ee[u] >> 30) << kLdsKindShift) | (ee[u] & kIndexMask));
+                        }
+                    } else {
+                        enumerate(i, p[s], mg[s], [&](uint32_t e) __attribute__((always_inline)) {
+                            row[filled++] = (uint16_t)(((e >> 30) << kLdsKindShift) | (e & kIndexMask));
+                        }, forward);
+                    }
                     sort_pair_block(row, pc);
+                } else if (strideRows) {
+                    sort_pair_block(stageArea + rs[s], pc);
                 } else {
                     uint32_t *row = A->adj + d->adjOffset + rs[s];
-                    enumerate(lane + s * BLOCK, p[s], mg[s], [&](uint32_t e) __attribute__((always_inline)) {
-                        row[cnt++] = e;
-                        pc += (e >> 30) == 0u;
-                        sc += (e >> 30) == 1u;
-                    }, forward);
+                    enumerate(i, p[s], mg[s], [&](uint32_t e) __attribute__((always_inline)) { row[filled++] = e; }, forward);
                     sort_pair_block(row, pc);
                 }
-                rpq[s] = pc | ((pc + sc) << 16);
             }
             __syncthreads();
             return true;
