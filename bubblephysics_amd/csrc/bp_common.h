@@ -163,6 +163,52 @@ BPD int64_t cell_coord(float x, float width) {
     return fabsf(f) < 2147483648.0f ? (int64_t)(int32_t)f : (int64_t)f;
 }
 
+constexpr uint32_t kStageStride = 64;      // staged row entries per bubble in a world's global adjacency area
+
+// Sorts the first `count` of N register values ascending with a bitonic network (every index is a
+// compile-time constant once unrolled, so the values stay in registers). Slots past `count`
+// hold the largest value and stay at the end.
+template <int N>
+BPD void bitonic_sort(uint32_t (&v)[N]) {
+#pragma unroll
+    for (int k = 2; k <= N; k <<= 1)
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1)
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                const int l = i ^ j;
+                if (l > i) {
+                    const uint32_t lo = min(v[i], v[l]), hi = max(v[i], v[l]);
+                    if ((i & k) == 0) { v[i] = lo; v[l] = hi; } else { v[i] = hi; v[l] = lo; }
+                }
+            }
+}
+template <int N, typename T>
+BPD void sort_small_block(T *row, uint32_t count) {
+    uint32_t v[N];
+#pragma unroll
+    for (int u = 0; u < N; ++u) v[u] = (uint32_t)u < count ? (uint32_t)row[u] : 0xffffffffu;
+    bitonic_sort<N>(v);
+#pragma unroll
+    for (int u = 0; u < N; ++u)
+        if ((uint32_t)u < count) row[u] = (T)v[u];
+}
+// Orders the pair block of an adjacency row (entries of kind 0, so the value is the neighbour
+// index) ascending: blocks of up to 16 entries are loaded together, sorted in registers and
+// stored together; longer blocks are insertion-sorted in place.
+template <typename T>
+BPD void sort_pair_block(T *row, uint32_t count) {
+    if (count < 2) return;
+    if (count <= 8) { sort_small_block<8>(row, count); return; }
+    if (count <= 16) { sort_small_block<16>(row, count); return; }
+    for (uint32_t x = 1; x < count; ++x) {
+        const T val = row[x];
+        uint32_t loc = x;
+        while (loc > 0 && val < row[loc - 1]) { row[loc] = row[loc - 1]; --loc; }
+        row[loc] = val;
+    }
+}
+
 // The cell offsets along one axis (within -1..1 of the centre's cell `c`) that can hold another
 // centre at FP32 distance 0 from a centre with coordinate `v`: the contact assertion's pair test
 // (Force:880-881). sqrt(dx^2 + dy^2) is 0 only when both squares flush to zero, so

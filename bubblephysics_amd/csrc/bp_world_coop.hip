@@ -350,23 +350,33 @@ BPD bool coop_build(cg::grid_group &grid, World &W, Red<BLOCK> &red, const CoopS
     for (uint32_t b = gtid; b < B; b += R.stride) cells[b] = 0;
     uint32_t *bodyHits = W.A->bodyHits + d.bodyOffset;
     for (uint32_t k = gtid; k < W.nb; k += R.stride) bodyHits[k] = 0;
-    if (gtid == 0) for (uint32_t k = kCoDropped; k <= kCoOverflowRows; ++k) s.counters[k] = 0;
+    if (gtid == 0) {
+        for (uint32_t k = kCoDropped; k <= kCoOverflowRows; ++k) s.counters[k] = 0;
+        s.counters[kCoLongRows] = 0;
+    }
     grid.sync();
     width = W.sepTol;
     for (uint32_t w = threadIdx.x; w < gridDim.x; w += BLOCK) width = fmaxf(width, s.width[w]);
     width = block_maxf<BLOCK>(red, width);
     W.cellWidth = fmaxf(width, kFloatMin);
     coop_hash<BLOCK>(grid, W, red, s, R, cells, B);
-    // Row lengths and the reference's overflow counters (Geometry:291-297, 410-414).
-    uint32_t droppedLocal = 0, maxLocal = 0, pairsLocal = 0;
+    // Row lengths and the reference's overflow counters (Geometry:291-297, 410-414). Each row
+    // is also staged at a fixed stride of kStageStride entries per bubble in the world's adjacency
+    // area (when the area holds n strides); when every row fits, the rows stay there and the fill
+    // pass only orders their pair and body runs instead of enumerating the candidates again.
+    const bool stageOk = (uint64_t)n * kStageStride <= d.adjCapacity;
+    uint32_t droppedLocal = 0, maxLocal = 0, pairsLocal = 0, longLocal = 0;
     for (uint32_t i = R.first; i < n; i += R.stride) {
         uint32_t count = 0, forward = 0;
+        uint32_t *stage = W.adj + (size_t)i * kStageStride;
         auto tally = [&](uint32_t e) {
+            if (stageOk && count < kStageStride) stage[count] = e;
             ++count;
             if ((e >> kKindShift) == 2u) atomicAdd(&bodyHits[e & kIndexMask], 1u);
         };
         enumerate_row(W, cells, B, i, tally, forward, bodies);
         W.rowLen[i] = count;
+        longLocal |= count > kStageStride ? 1u : 0u;
         if (forward > limit) droppedLocal += forward - limit;
         if (count > limit) droppedLocal += count - limit;
         maxLocal = max(maxLocal, max(forward, count));
@@ -386,11 +396,30 @@ BPD bool coop_build(cg::grid_group &grid, World &W, Red<BLOCK> &red, const CoopS
         atomicAdd(&s.counters[kCoPairs], pairsLocal);
         atomicMax(&s.counters[kCoMaxDemand], maxLocal);
     }
+    if (longLocal) atomicOr(&s.counters[kCoLongRows], 1u);
     entries = grid_scan<BLOCK>(grid, red, s, W.rowLen, W.rowStart, n);
     grid.sync();
     dropped = s.counters[kCoDropped];
     maxDemand = s.counters[kCoMaxDemand];
     pairs = s.counters[kCoPairs];
+    if (stageOk && s.counters[kCoLongRows] == 0u) {
+        // The staged rows are the rows (a start and a length per bubble).
+        for (uint32_t i = R.first; i < n; i += R.stride) {
+            W.rowStart[i] = i * kStageStride;
+            uint32_t *row = W.adj + (size_t)i * kStageStride;
+            const uint32_t count = W.rowLen[i];
+            uint32_t pairEnd = 0, segs = 0;
+            for (uint32_t x = 0; x < count; ++x) {
+                const uint32_t kind = row[x] >> kKindShift;
+                pairEnd += kind == 0u ? 1u : 0u;
+                segs += kind == 1u ? 1u : 0u;
+            }
+            sort_pair_block(row, pairEnd);
+            sort_run(row, pairEnd + segs, count);
+        }
+        grid.sync();
+        return true;
+    }
     if (entries > d.adjCapacity) return false;
     // Fill the rows, then order the pair run and the body run (Contact:222-238).
     for (uint32_t i = R.first; i < n; i += R.stride) {

@@ -82,18 +82,25 @@ BPD bool build_grid(World &W, Shared<BLOCK> &sh, float remaining, uint32_t limit
         W.publish_cell(slot, i);
     }
     __syncthreads();
-    // Pass 1: row lengths and the reference's overflow counters.
+    // Pass 1: row lengths and the reference's overflow counters. Each row is also staged at a
+    // fixed stride of kStageStride entries per bubble in the world's adjacency area (when the
+    // area holds n strides); when every row fits its stride, the rows stay there and pass 2 only
+    // orders their pair blocks instead of enumerating the candidates again.
     const uint32_t *ends = sh.buckets;
-    uint32_t droppedLocal = 0, maxLocal = 0, pairsLocal = 0, overflowLocal = 0;
+    uint32_t droppedLocal = 0, maxLocal = 0, pairsLocal = 0, overflowLocal = 0, longLocal = 0;
     uint32_t *bodyHits = W.A->bodyHits + d.bodyOffset;
+    const bool stageOk = (uint64_t)n * kStageStride <= d.adjCapacity;
     for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
         uint32_t count = 0, forward = 0;
+        uint32_t *stage = W.adj + (size_t)i * kStageStride;
         auto tally = [&](uint32_t e) {
+            if (stageOk && count < kStageStride) stage[count] = e;
             ++count;
             if ((e >> kKindShift) == 2u) atomicAdd(&bodyHits[e & kIndexMask], 1u);
         };
         enumerate_row(W, ends, buckets, i, tally, forward);
         W.rowLen[i] = count;
+        longLocal |= count > kStageStride ? 1u : 0u;
         if (forward > limit) { droppedLocal += forward - limit; overflowLocal = 1; }
         maxLocal = max(maxLocal, max(forward, count));
         if (count > limit) { droppedLocal += count - limit; overflowLocal = 1; }
@@ -110,6 +117,7 @@ BPD bool build_grid(World &W, Shared<BLOCK> &sh, float remaining, uint32_t limit
     atomicMax(&sh.red.counters[1], maxLocal);
     atomicAdd(&sh.red.counters[2], pairsLocal);
     atomicOr(&sh.red.counters[3], overflowLocal);
+    if (longLocal) atomicOr(&sh.red.counters[4], 1u);
     __syncthreads();
     // Row starts: tiled exclusive scan of the row lengths in bubble order.
     uint32_t carry = 0;
@@ -126,6 +134,21 @@ BPD bool build_grid(World &W, Shared<BLOCK> &sh, float remaining, uint32_t limit
     stats.maxDemand = sh.red.counters[1];
     stats.pairs = sh.red.counters[2];
     stats.overflowRows = sh.red.counters[3];
+    const bool strideRows = stageOk && sh.red.counters[4] == 0u;
+    if (strideRows) {
+        // The staged rows are the rows: each bubble's start is its stride (rows need only a start
+        // and a length), and only the pair block is out of order (hash-bucket order).
+        for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
+            W.rowStart[i] = i * kStageStride;
+            uint32_t *row = W.adj + (size_t)i * kStageStride;
+            const uint32_t count = W.rowLen[i];
+            uint32_t pairs = 0;
+            for (uint32_t x = 0; x < count; ++x) pairs += (row[x] >> kKindShift) == 0u ? 1u : 0u;
+            sort_pair_block(row, pairs);
+        }
+        __syncthreads();
+        return true;
+    }
     if (carry > d.adjCapacity) return false;
     __syncthreads();
     // Pass 2: fill rows, then order each row by (kind, neighbour) (Contact:222-238).
@@ -137,12 +160,7 @@ BPD bool build_grid(World &W, Shared<BLOCK> &sh, float remaining, uint32_t limit
         // Only the pair block can be out of order (hash-bucket order); segments and bodies follow in order.
         uint32_t pairs = 0;
         while (pairs < count && (row[pairs] >> kKindShift) == 0u) ++pairs;
-        for (uint32_t a = 1; a < pairs; ++a) {
-            uint32_t v = row[a];
-            uint32_t loc = a;
-            while (loc > 0 && v < row[loc - 1]) { row[loc] = row[loc - 1]; --loc; }
-            row[loc] = v;
-        }
+        sort_pair_block(row, pairs);
     }
     __syncthreads();
     return true;
