@@ -652,6 +652,11 @@ __global__ __launch_bounds__(BLOCK) void world_lds_kernel(const Arena *__restric
             for (uint32_t k = 0; k < g.nb; ++k) {
                 const uint4 kind = S.bodyKind[k];
                 if (!(kind.y & (kBodyStatic | kBodyKinematic)) || !filter_accepts(cat, mask, kind.z, kind.w)) continue;
+                // A centre outside the body's bounding circle (with a relative slack far above the
+                // rounding of the inside test) is not inside it: the triangle test is skipped.
+                const float2 q = sub(center, S.frameAt[k]);
+                const float reach = S.bodyExtent[k].z * 1.0001f + 1e-3f;
+                if (q.x * q.x + q.y * q.y > reach * reach) continue;
                 if (frame_inside(center, frame_at(k), S.verts, S.tris)) {
                     out.kind = BP_VIOLATION_INSIDE_BODY; out.other = g.bodyId(k); out.segment = 0; out.detail = 0.0f;
                     return true;
@@ -663,7 +668,9 @@ __global__ __launch_bounds__(BLOCK) void world_lds_kernel(const Arena *__restric
                 const float2 a = f2(ab.x, ab.y), edge = sub(f2(ab.z, ab.w), a);
                 const float lengthSquared = dot2(edge, edge);
                 if (!(lengthSquared > 0.0f)) continue;
-                const float2 outward = mul(dvs(f2(-edge.y, edge.x), __builtin_sqrtf(lengthSquared)), S.segOutward[j]);
+                // The staged unit left normal is (-e.y, e.x) / sqrt(e . e), the same operations.
+                const float4 rg = S.segRegion[j];
+                const float2 outward = mul(f2(rg.z, rg.w), S.segOutward[j]);
                 const float before = dot2(sub(start, a), outward), after = dot2(sub(center, a), outward);
                 if (!(before > 0.0f) || !(after <= 0.0f)) continue;
                 const float2 crossing = add(start, mul(sub(center, start), before / (before - after)));
