@@ -261,8 +261,9 @@ def main():
     ap.add_argument("--copy-bytes", type=int, default=0, help="only run the copy calibration kernel of this size")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU: the worlds are independent and this build measures one MI355X "
-              f"(--gpus {args.gpus} is not supported)", file=sys.stderr)
+        print(f"bench.py runs on one GPU: the worlds are independent, so a multi-GPU run would shard them by "
+              f"world with no exchange; only one MI355X is measured (--gpus {args.gpus} is not supported)",
+              file=sys.stderr)
         sys.exit(2)
     if args.copy_bytes:
         copy_calibration(args.copy_bytes)
