@@ -209,6 +209,106 @@ BPD void sort_pair_block(T *row, uint32_t count) {
     }
 }
 
+// ---------------------------------------------------------------------------------------
+// Rows cut to the reference's k candidates (Geometry:230-251 and 276-298, 350-415;
+// Contact:190-238).
+//
+// A build here lists every candidate of a bubble in one row, both directions of each pair,
+// sorted by (kind, neighbour). The reference keeps less when a row overflows:
+//   - the candidate row of bubble i holds the k smallest keys of its pairs j > i, then its
+//     segments: its k smallest forward neighbours, and k - min(forward pairs, k) segments;
+//   - the candidate row of a body holds its k bubbles of smallest index;
+//   - the adjacency of bubble i takes the surviving candidates in arrival order (pairs (a, i)
+//     by ascending a, then the pairs (i, j) by ascending j, segments, bodies: the row's own
+//     order) and keeps the first k.
+// Bubble ids ascend with the index, so key order is index order. When a build drops anything,
+// the rows are cut to exactly that in two passes over the bubbles with a barrier between:
+//   1. forward_cut gives each bubble the largest forward neighbour its candidate row keeps
+//      (kNoCut when it keeps every one), stored where every bubble can read it, and
+//      mark_body_cuts marks, in the bubble's own row, the bodies that do not keep it;
+//   2. cut_row compacts each bubble's own row in place: pair (l, h) stays iff h <= cut(l).
+// Nothing of this runs for a build that drops nothing.
+// ---------------------------------------------------------------------------------------
+constexpr uint32_t kNoCut = 0xffffffffu;
+
+template <typename T> struct RowCodec;
+template <> struct RowCodec<uint16_t> {        // LDS rows: kind << 14 | index
+    static constexpr uint32_t kMask = (1u << 14) - 1;
+    static constexpr uint16_t kDrop = 0xffff;  // kind 3: a marked entry
+};
+template <> struct RowCodec<uint32_t> {        // global rows: kind << 30 | index
+    static constexpr uint32_t kMask = (1u << 30) - 1;
+    static constexpr uint32_t kDrop = 0xffffffffu;
+};
+
+// Forward pair count of bubble i (its pairs j > i) and the largest neighbour its candidate row
+// keeps, from the sorted pair run row[0, pairEnd).
+template <typename T>
+BPD uint32_t forward_cut(const T *row, uint32_t pairEnd, uint32_t i, uint32_t limit, uint32_t &forwardPairs) {
+    uint32_t lo = 0, hi = pairEnd;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (((uint32_t)row[mid] & RowCodec<T>::kMask) < i) lo = mid + 1; else hi = mid;
+    }
+    forwardPairs = pairEnd - lo;
+    return forwardPairs > limit ? ((uint32_t)row[lo + limit - 1] & RowCodec<T>::kMask) : kNoCut;
+}
+
+// Marks the entries of the body run row[segEnd, end) whose body keeps k bubbles of lower index
+// than i. `hits[k]` is the length of body k's full row; `below(k)` counts, stopping at `limit`,
+// the bubbles of index < i in it.
+template <typename T, class Below>
+BPD void mark_body_cuts(T *row, uint32_t segEnd, uint32_t end, uint32_t limit, const uint32_t *hits, Below below) {
+    for (uint32_t e = segEnd; e < end; ++e) {
+        const uint32_t k = (uint32_t)row[e] & RowCodec<T>::kMask;
+        if (hits[k] > limit && below(k) >= limit) row[e] = RowCodec<T>::kDrop;
+    }
+}
+
+// Block ends of a global row ordered (pairs, segments, bodies).
+BPD void row_blocks(const uint32_t *row, uint32_t len, uint32_t &pairEnd, uint32_t &segEnd) {
+    pairEnd = 0;
+    while (pairEnd < len && (row[pairEnd] >> kKindShift) == 0u) ++pairEnd;
+    segEnd = pairEnd;
+    while (segEnd < len && (row[segEnd] >> kKindShift) == 1u) ++segEnd;
+}
+
+struct RowCut {
+    uint32_t len, pairEnd, segEnd;   // the cut row and its block ends
+    uint32_t demand;                 // adjacency entries the reference's candidates ask for
+};
+
+// Compacts the row of bubble i in place. `cutOf(l)` is bubble l's forward cut.
+template <typename T, class CutOf>
+BPD RowCut cut_row(T *row, uint32_t pairEnd, uint32_t segEnd, uint32_t end, uint32_t i, uint32_t limit,
+                   uint32_t forwardPairs, CutOf cutOf) {
+    RowCut r{0u, 0u, 0u, 0u};
+    const uint32_t own = cutOf(i);
+    for (uint32_t e = 0; e < pairEnd; ++e) {
+        const T v = row[e];
+        const uint32_t j = (uint32_t)v & RowCodec<T>::kMask;
+        if (j < i ? i <= cutOf(j) : j <= own) {
+            if (r.len < limit) row[r.len++] = v;
+            ++r.demand;
+        }
+    }
+    r.pairEnd = r.len;
+    const uint32_t segKeep = min(segEnd - pairEnd, limit - min(forwardPairs, limit));
+    for (uint32_t e = pairEnd; e < pairEnd + segKeep; ++e) {
+        const T v = row[e];
+        if (r.len < limit) row[r.len++] = v;
+        ++r.demand;
+    }
+    r.segEnd = r.len;
+    for (uint32_t e = segEnd; e < end; ++e) {
+        const T v = row[e];
+        if (v == RowCodec<T>::kDrop) continue;
+        if (r.len < limit) row[r.len++] = v;
+        ++r.demand;
+    }
+    return r;
+}
+
 // The cell offsets along one axis (within -1..1 of the centre's cell `c`) that can hold another
 // centre at FP32 distance 0 from a centre with coordinate `v`: the contact assertion's pair test
 // (Force:880-881). sqrt(dx^2 + dy^2) is 0 only when both squares flush to zero, so

@@ -101,7 +101,7 @@ constexpr uint32_t kCoopCounters = 32;
 constexpr uint32_t kCoopScratchWords = 8 * kCoopMaxGroups + kCoopCounters;
 enum CoopCounter : uint32_t {
     kCoDropped = 0, kCoMaxDemand, kCoPairs, kCoOverflowRows, kCoSafety, kCoRemovals, kCoWraps, kCoViolator,
-    kCoBvhBuilt, kCoBodySafety, kCoEntries, kCoLongRows
+    kCoBvhBuilt, kCoBodySafety, kCoEntries, kCoLongRows, kCoCutDropped, kCoCutWidest, kCoCutEntries
 };
 
 

@@ -325,6 +325,61 @@ BPD void sort_run(uint32_t *row, uint32_t from, uint32_t to) {        // inserti
     }
 }
 
+// Cuts the sorted rows to the reference's k candidates (forward_cut, mark_body_cuts, cut_row)
+// after a build that dropped candidates, and replaces the build's counters by those of the
+// reference's truncated rows (Geometry:291-297, 410-414; Contact:190-195). The grid slots hold
+// the forward cuts (the contact assertion hashes the grid again).
+template <int BLOCK>
+BPD void coop_cut_rows(cg::grid_group &grid, World &W, Red<BLOCK> &red, const CoopScratch &s, Range R, uint32_t limit,
+                       uint32_t &dropped, uint32_t &maxDemand, uint32_t &entries) {
+    const uint32_t n = W.n;
+    const uint32_t *hits = W.A->bodyHits + W.d->bodyOffset;
+    for (uint32_t i = R.first; i < n; i += R.stride) {
+        uint32_t *row = W.adj + W.rowStart[i];
+        const uint32_t len = W.rowLen[i];
+        uint32_t pairEnd, segEnd, forwardPairs;
+        row_blocks(row, len, pairEnd, segEnd);
+        W.grid[i] = forward_cut(row, pairEnd, i, limit, forwardPairs);
+        mark_body_cuts(row, segEnd, len, limit, hits, [&](uint32_t k) {
+            uint32_t below = 0;
+            for (uint32_t a = 0; a < i && below < limit; ++a) below += W.body_candidate(a, k) ? 1u : 0u;
+            return below;
+        });
+    }
+    grid.sync();
+    uint32_t droppedLocal = 0, widest = 0, kept = 0;
+    for (uint32_t i = R.first; i < n; i += R.stride) {
+        uint32_t *row = W.adj + W.rowStart[i];
+        const uint32_t len = W.rowLen[i];
+        uint32_t pairEnd, segEnd, forwardPairs;
+        row_blocks(row, len, pairEnd, segEnd);
+        forward_cut(row, pairEnd, i, limit, forwardPairs);
+        const RowCut r = cut_row(row, pairEnd, segEnd, len, i, limit, forwardPairs, [&](uint32_t l) { return W.grid[l]; });
+        W.rowLen[i] = r.len;
+        const uint32_t forward = forwardPairs + (segEnd - pairEnd);
+        droppedLocal += (forward > limit ? forward - limit : 0u) + (r.demand > limit ? r.demand - limit : 0u);
+        widest = max(widest, max(forward, r.demand));
+        kept += r.len;
+    }
+    for (uint32_t k = R.first; k < W.nb; k += R.stride) {
+        const uint32_t h = hits[k];
+        droppedLocal += h > limit ? h - limit : 0u;
+        widest = max(widest, h);
+    }
+    droppedLocal = block_sum_u32<BLOCK>(red, droppedLocal);
+    kept = block_sum_u32<BLOCK>(red, kept);
+    widest = block_max_u32<BLOCK>(red, widest);
+    if (threadIdx.x == 0) {
+        atomicAdd(&s.counters[kCoCutDropped], droppedLocal);
+        atomicMax(&s.counters[kCoCutWidest], widest);
+        atomicAdd(&s.counters[kCoCutEntries], kept);
+    }
+    grid.sync();
+    dropped = s.counters[kCoCutDropped];
+    maxDemand = s.counters[kCoCutWidest];
+    entries = s.counters[kCoCutEntries];
+}
+
 // Grid build of a cooperative world (Force:1040-1134 and Geometry:300-426 with CSR rows).
 template <int BLOCK>
 BPD bool coop_build(cg::grid_group &grid, World &W, Red<BLOCK> &red, const CoopScratch &s, Range R, uint32_t *cells,
@@ -352,7 +407,7 @@ BPD bool coop_build(cg::grid_group &grid, World &W, Red<BLOCK> &red, const CoopS
     for (uint32_t k = gtid; k < W.nb; k += R.stride) bodyHits[k] = 0;
     if (gtid == 0) {
         for (uint32_t k = kCoDropped; k <= kCoOverflowRows; ++k) s.counters[k] = 0;
-        s.counters[kCoLongRows] = 0;
+        for (uint32_t k = kCoLongRows; k <= kCoCutEntries; ++k) s.counters[k] = 0;
     }
     grid.sync();
     width = W.sepTol;
@@ -418,6 +473,7 @@ BPD bool coop_build(cg::grid_group &grid, World &W, Red<BLOCK> &red, const CoopS
             sort_run(row, pairEnd + segs, count);
         }
         grid.sync();
+        if (dropped != 0u) coop_cut_rows<BLOCK>(grid, W, red, s, R, limit, dropped, maxDemand, entries);
         return true;
     }
     if (entries > d.adjCapacity) return false;
@@ -435,6 +491,7 @@ BPD bool coop_build(cg::grid_group &grid, World &W, Red<BLOCK> &red, const CoopS
         sort_run(row, segEnd, count);
     }
     grid.sync();
+    if (dropped != 0u) coop_cut_rows<BLOCK>(grid, W, red, s, R, limit, dropped, maxDemand, entries);
     return true;
 }
 

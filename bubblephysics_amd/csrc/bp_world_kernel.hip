@@ -29,6 +29,56 @@ template <int BLOCK> struct Shared {
 
 struct RowStats { uint32_t dropped, maxDemand, pairs, overflowRows; };
 
+// Cuts the sorted rows to the reference's k candidates (forward_cut, mark_body_cuts, cut_row)
+// after a build that dropped candidates, and replaces the build's counters by those of the
+// reference's truncated rows (Geometry:291-297, 410-414; Contact:190-195). The grid slots hold
+// the forward cuts (the contact assertion builds its own grid).
+template <int BLOCK>
+BPD void cut_rows(World &W, Shared<BLOCK> &sh, uint32_t limit, RowStats &stats, uint32_t &entries) {
+    const uint32_t n = W.n;
+    const uint32_t *hits = W.A->bodyHits + W.d->bodyOffset;
+    for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
+        uint32_t *row = W.adj + W.rowStart[i];
+        const uint32_t len = W.rowLen[i];
+        uint32_t pairEnd, segEnd, forwardPairs;
+        row_blocks(row, len, pairEnd, segEnd);
+        W.grid[i] = forward_cut(row, pairEnd, i, limit, forwardPairs);
+        mark_body_cuts(row, segEnd, len, limit, hits, [&](uint32_t k) {
+            uint32_t below = 0;
+            for (uint32_t a = 0; a < i && below < limit; ++a) below += W.body_candidate(a, k) ? 1u : 0u;
+            return below;
+        });
+    }
+    __syncthreads();
+    uint32_t dropped = 0, widest = 0, kept = 0;
+    for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
+        uint32_t *row = W.adj + W.rowStart[i];
+        const uint32_t len = W.rowLen[i];
+        uint32_t pairEnd, segEnd, forwardPairs;
+        row_blocks(row, len, pairEnd, segEnd);
+        forward_cut(row, pairEnd, i, limit, forwardPairs);
+        const RowCut r = cut_row(row, pairEnd, segEnd, len, i, limit, forwardPairs, [&](uint32_t l) { return W.grid[l]; });
+        W.rowLen[i] = r.len;
+        const uint32_t forward = forwardPairs + (segEnd - pairEnd);
+        dropped += (forward > limit ? forward - limit : 0u) + (r.demand > limit ? r.demand - limit : 0u);
+        widest = max(widest, max(forward, r.demand));
+        kept += r.len;
+    }
+    for (uint32_t k = threadIdx.x; k < W.nb; k += BLOCK) {
+        const uint32_t h = hits[k];
+        dropped += h > limit ? h - limit : 0u;
+        widest = max(widest, h);
+    }
+    atomicAdd(&sh.red.counters[5], dropped);
+    atomicMax(&sh.red.counters[6], widest);
+    atomicAdd(&sh.red.counters[7], kept);
+    __syncthreads();
+    stats.dropped = sh.red.counters[5];
+    stats.maxDemand = sh.red.counters[6];
+    entries = sh.red.counters[7];
+    __syncthreads();
+}
+
 // Grid build of one workgroup: Force:1040-1134 with a CSR adjacency. Returns false when the
 // adjacency does not fit the world's capacity.
 template <int BLOCK>
@@ -147,6 +197,7 @@ BPD bool build_grid(World &W, Shared<BLOCK> &sh, float remaining, uint32_t limit
             sort_pair_block(row, pairs);
         }
         __syncthreads();
+        if (stats.dropped != 0u) cut_rows<BLOCK>(W, sh, limit, stats, entries);
         return true;
     }
     if (carry > d.adjCapacity) return false;
@@ -163,6 +214,7 @@ BPD bool build_grid(World &W, Shared<BLOCK> &sh, float remaining, uint32_t limit
         sort_pair_block(row, pairs);
     }
     __syncthreads();
+    if (stats.dropped != 0u) cut_rows<BLOCK>(W, sh, limit, stats, entries);
     return true;
 }
 

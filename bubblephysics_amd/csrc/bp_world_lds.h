@@ -380,6 +380,74 @@ __global__ __launch_bounds__(BLOCK) void world_lds_kernel(const Arena *__restric
                 if (body_candidate(pi, ri, mi, ci, ki, k)) emit((2u << 30) | k);
         };
 
+        // Rows cut to the reference's k candidates after a build that dropped some (forward_cut,
+        // mark_body_cuts, cut_row), with the counters of the reference's truncated rows
+        // (Geometry:291-297, 410-414; Contact:190-195). The grid slots hold the forward cuts
+        // (the contact assertion hashes the grid again); the margins are still in S.aux.
+        auto cut_rows = [&](uint32_t *globalRows) __attribute__((always_inline)) {
+            const uint32_t *hits = g.bodyHits();
+    #pragma unroll
+            for (int s = 0; s < PER; ++s) {
+                if (!valid[s]) continue;
+                const uint32_t i = lane + s * BLOCK, pc = rpq[s] & 0xffffu, segEnd = rpq[s] >> 16;
+                auto below = [&](uint32_t k) {
+                    uint32_t count = 0;
+                    for (uint32_t a = 0; a < i && count < limit; ++a)
+                        count += body_candidate(S.pos[a], S.r[a], S.aux[a], catG[a], maskG[a], k) ? 1u : 0u;
+                    return count;
+                };
+                uint32_t fp, cut;
+                if (rows.inLds) {
+                    uint16_t *row = S.rows + rs[s];
+                    cut = forward_cut(row, pc, i, limit, fp);
+                    mark_body_cuts(row, segEnd, rl[s], limit, hits, below);
+                } else {
+                    uint32_t *row = globalRows + rs[s];
+                    cut = forward_cut(row, pc, i, limit, fp);
+                    mark_body_cuts(row, segEnd, rl[s], limit, hits, below);
+                }
+                S.grid[i] = cut == kNoCut ? (uint16_t)0xffffu : (uint16_t)cut;
+            }
+            __syncthreads();
+            auto cutOf = [&](uint32_t l) { const uint32_t v = S.grid[l]; return v == 0xffffu ? kNoCut : v; };
+            uint32_t dropped = 0, widest = 0, kept = 0;
+    #pragma unroll
+            for (int s = 0; s < PER; ++s) {
+                if (!valid[s]) continue;
+                const uint32_t i = lane + s * BLOCK, pc = rpq[s] & 0xffffu, segEnd = rpq[s] >> 16;
+                uint32_t fp;
+                RowCut r;
+                if (rows.inLds) {
+                    uint16_t *row = S.rows + rs[s];
+                    forward_cut(row, pc, i, limit, fp);
+                    r = cut_row(row, pc, segEnd, rl[s], i, limit, fp, cutOf);
+                } else {
+                    uint32_t *row = globalRows + rs[s];
+                    forward_cut(row, pc, i, limit, fp);
+                    r = cut_row(row, pc, segEnd, rl[s], i, limit, fp, cutOf);
+                }
+                rl[s] = r.len;
+                rpq[s] = r.pairEnd | (r.segEnd << 16);
+                const uint32_t forward = fp + (segEnd - pc);
+                dropped += (forward > limit ? forward - limit : 0u) + (r.demand > limit ? r.demand - limit : 0u);
+                widest = max(widest, max(forward, r.demand));
+                kept += r.len;
+            }
+            for (uint32_t k = lane; k < g.nb; k += BLOCK) {
+                const uint32_t h = hits[k];
+                dropped += h > limit ? h - limit : 0u;
+                widest = max(widest, h);
+            }
+            atomicAdd(&S.red.counters[4], dropped);
+            atomicMax(&S.red.counters[5], widest);
+            atomicAdd(&S.red.counters[6], kept);
+            __syncthreads();
+            ov0 += S.red.counters[4];
+            ov1 = max(ov1, S.red.counters[5]);
+            entriesTotal = S.red.counters[6];
+            __syncthreads();
+        };
+
         // Grid build (Force:1040-1134) with CSR rows. Returns false when the rows do not fit.
         auto build = [&]() __attribute__((always_inline)) -> bool {
             float remaining = fmaxf(c.t0 - c.t1, 0.0f);
@@ -445,8 +513,7 @@ __global__ __launch_bounds__(BLOCK) void world_lds_kernel(const Arena *__restric
     #pragma unroll
             for (int s = 0; s < PER; ++s) { rs[s] = base; base += rl[s]; }
             entriesTotal = total;
-            ov0 += S.red.counters[0];
-            ov1 = max(ov1, S.red.counters[1]);
+            const uint32_t buildDropped = S.red.counters[0], buildWidest = S.red.counters[1];
             pairs = S.red.counters[2];
             rows.inLds = ldsRowsAllowed && total <= (uint32_t)ROWCAP;
             // Rows that stay in global memory stay where pass 1 staged them when every row fit
@@ -456,7 +523,10 @@ __global__ __launch_bounds__(BLOCK) void world_lds_kernel(const Arena *__restric
     #pragma unroll
                 for (int s = 0; s < PER; ++s) rs[s] = (lane + s * BLOCK) * kStageStride;
             }
-            if (!rows.inLds && !strideRows && total > d->adjCapacity) return false;
+            if (!rows.inLds && !strideRows && total > d->adjCapacity) {
+                ov0 += buildDropped; ov1 = max(ov1, buildWidest);
+                return false;
+            }
             // Pass 2: fill the rows and order the pair block (Contact:222-238).
     #pragma unroll
             for (int s = 0; s < PER; ++s) {
@@ -490,6 +560,8 @@ __global__ __launch_bounds__(BLOCK) void world_lds_kernel(const Arena *__restric
                 }
             }
             __syncthreads();
+            if (buildDropped != 0u) cut_rows(stageArea);
+            else { ov0 += buildDropped; ov1 = max(ov1, buildWidest); }
             return true;
         };
 
