@@ -206,6 +206,21 @@ class ServerStats(ctypes.Structure):
         "repacks", "resident_bytes")]
 
 
+class StageRejection(ctypes.Structure):
+    _fields_ = [("reason", ctypes.c_uint32), ("issue", ctypes.c_uint32), ("object_id", ctypes.c_int64)]
+
+
+class StageScene(ctypes.Structure):
+    _fields_ = [("bubbles", ctypes.c_void_p), ("bubble_count", ctypes.c_uint32), ("body_count", ctypes.c_uint32),
+                ("bodies", ctypes.c_void_p), ("vertices", ctypes.c_void_p), ("vertex_count", ctypes.c_uint32),
+                ("region", ctypes.c_float * 4)]
+
+
+BP_PLACEMENT_OK, BP_PLACEMENT_OUTSIDE_REGION, BP_PLACEMENT_INSIDE_RIGID_BODY, BP_PLACEMENT_COINCIDENT_CENTER = 0, 1, 2, 3
+(BP_REJECT_NONE, BP_REJECT_INVALID_STRUCTURAL_OPERATION, BP_REJECT_INVALID_PLACEMENT, BP_REJECT_DUPLICATE_ID,
+ BP_REJECT_UNKNOWN_OBJECT) = 0, 1, 2, 3, 4
+
+
 # numpy views of the record arrays (same layout as the C structures).
 BUBBLE_DTYPE = np.dtype(Bubble)
 SEGMENT_DTYPE = np.dtype(Segment)
@@ -253,6 +268,8 @@ SIGNATURES = {
     "bp_world_contacts": ([_P, _P, _U32, _P], _I32),
     "bp_batch_contacts": ([_P, _U32, _P, _U32, _P], _I32),
     "bp_contact_transitions": ([_P, _U32, _P, _U32, _P, _U32, ctypes.c_float, _P, _U32, _U32, _P, _P, _U32, _P], _I32),
+    "bp_stage_merge": ([_P, _P, _U32, _P, ctypes.c_int32, _P, _U32, _P, _P], _I32),
+    "bp_stage_split": ([_P, ctypes.c_int64, _P, ctypes.c_float, _P, ctypes.c_float, _P, _U32, _P, _P], _I32),
     "bp_static_polygon_edges": ([_P, _U32, _I64, _I64, _U32, _P, _U32, _P], _I32),
     "bp_polygon_body_geometry": ([_P, _U32, _P, _P, _P, _P, _P, _P], _I32),
     "bp_place_discs": ([_P, _U32, _P, _F, _F, _P, _U32, _U64, _U32, _P, _P], _I32),

@@ -227,6 +227,51 @@ def contact_transitions(contacts: np.ndarray, segments: np.ndarray, interested, 
     return events[: ec.value].copy(), hist[: hc.value].copy()
 
 
+class StageRejected(abi.BubblePhysicsError):
+    """A merge or split the staging refused (TransactionRejectionReason): reason, issue, object id."""
+
+    def __init__(self, why: abi.StageRejection, where: str):
+        super().__init__(abi.BP_ERR_INVALID_ARGUMENT, where, f"reason {why.reason} issue {why.issue} object {why.object_id}")
+        self.reason, self.issue, self.object_id = int(why.reason), int(why.issue), int(why.object_id)
+
+
+def _stage(fn, name, bubbles, bodies, vertices, region, extra_out, *args):
+    lib = abi.load_library()
+    b = np.ascontiguousarray(bubbles, abi.BUBBLE_DTYPE)
+    bod = np.ascontiguousarray(np.zeros(0, abi.BODY_DTYPE) if bodies is None else bodies, abi.BODY_DTYPE)
+    v = np.ascontiguousarray(np.zeros((0, 2), np.float32) if vertices is None else vertices, np.float32).reshape(-1, 2)
+    sc = abi.StageScene(abi.ptr(b), len(b), len(bod), abi.ptr(bod), abi.ptr(v), len(v), (ctypes.c_float * 4)(*region))
+    out = np.zeros(len(b) + extra_out, abi.BUBBLE_DTYPE)
+    count, why = ctypes.c_uint32(0), abi.StageRejection()
+    status = getattr(lib, fn)(ctypes.byref(sc), *args, abi.ptr(out), len(out), ctypes.byref(count), ctypes.byref(why))
+    if status != abi.BP_OK and why.reason != abi.BP_REJECT_NONE:
+        raise StageRejected(why, name)
+    abi.check(status, name)
+    return out[: count.value].copy()
+
+
+def stage_merge(bubbles, sources, result, *, result_mass=None, bodies=None, vertices=None, region=(-1e30, -1e30, 1e30, 1e30)):
+    """World bubbles after merging `sources` into `result` (a BUBBLE_DTYPE record: id, filter,
+    material, flags) -- bp_stage_merge, TransactionStager.merge. Raises StageRejected."""
+    res = np.zeros(1, abi.BUBBLE_DTYPE)
+    res[0] = result
+    if result_mass is not None:
+        res[0]["mass"] = result_mass
+    ids = np.ascontiguousarray(np.asarray(list(sources), np.int64))
+    return _stage("bp_stage_merge", "bp_stage_merge", bubbles, bodies, vertices, region, 1, abi.ptr(ids), len(ids),
+                  abi.ptr(res), 1 if result_mass is not None else 0)
+
+
+def stage_split(bubbles, source, first, first_share, second, second_share, *, bodies=None, vertices=None,
+                region=(-1e30, -1e30, 1e30, 1e30)):
+    """World bubbles after splitting `source` into `first` and `second` with area shares --
+    bp_stage_split, TransactionStager.split. Raises StageRejected."""
+    parts = np.zeros(2, abi.BUBBLE_DTYPE)
+    parts[0], parts[1] = first, second
+    return _stage("bp_stage_split", "bp_stage_split", bubbles, bodies, vertices, region, 2, ctypes.c_int64(int(source)),
+                  abi.ptr(parts[0:1]), ctypes.c_float(first_share), abi.ptr(parts[1:2]), ctypes.c_float(second_share))
+
+
 class Batch:
     """Many independent worlds advanced together (bp_batch)."""
 

@@ -362,6 +362,52 @@ bp_status bp_contact_transitions(const bp_contact *contacts, uint32_t count, con
                                  uint32_t history_capacity, uint32_t *out_history_count,
                                  bp_contact_event *events, uint32_t event_capacity, uint32_t *out_event_count);
 
+/* Structural staging of merge and split on the host: TransactionStager.merge / .split
+ * (TransactionStager.swift:630-735) with StructuralOperations (StructuralOperations.swift:10-151)
+ * and the placement rules of PlacementGeometry (PlacementGeometry.swift:42-159). Host-only: the
+ * caller passes the world's published bubbles (sorted by id), its bodies at their current poses
+ * with their local vertices, and the simulation region, and receives the world's new bubble list
+ * (sorted by id) to write back. A refused transaction returns BP_ERR_INVALID_ARGUMENT with the
+ * reason in `why` (TransactionRejectionReason); the world is unchanged. */
+enum {
+    BP_PLACEMENT_OK = 0,
+    BP_PLACEMENT_OUTSIDE_REGION = 1,       /* PlacementIssue.outsideRegion    */
+    BP_PLACEMENT_INSIDE_RIGID_BODY = 2,    /* PlacementIssue.insideRigidBody  */
+    BP_PLACEMENT_COINCIDENT_CENTER = 3     /* PlacementIssue.coincidentCenter */
+};
+enum {
+    BP_REJECT_NONE = 0,
+    BP_REJECT_INVALID_STRUCTURAL_OPERATION = 1,
+    BP_REJECT_INVALID_PLACEMENT = 2,       /* issue and object_id say where and for which result */
+    BP_REJECT_DUPLICATE_ID = 3,
+    BP_REJECT_UNKNOWN_OBJECT = 4
+};
+typedef struct bp_stage_rejection {
+    uint32_t reason;            /* BP_REJECT_*    */
+    uint32_t issue;             /* BP_PLACEMENT_* */
+    int64_t object_id;
+} bp_stage_rejection;
+typedef struct bp_stage_scene {
+    const bp_bubble *bubbles;   /* sorted by id */
+    uint32_t bubble_count;
+    uint32_t body_count;
+    const bp_body *bodies;      /* pose = the body's current position and rotation */
+    const float *vertices;      /* local polygon vertices (x, y), indexed by vertex_offset */
+    uint32_t vertex_count;
+    float region[4];            /* simulationRegion: min.x, min.y, max.x, max.y (inclusive) */
+} bp_stage_scene;
+/* Merge: `sources` (two or more distinct ids) become `result` (its id, filter, material and flags;
+ * its mass when result_mass_given, else the sources' total). Centre: the mean contact point of
+ * the source pairs, or the first candidate placement that passes. */
+bp_status bp_stage_merge(const bp_stage_scene *scene, const int64_t *sources, uint32_t source_count,
+                         const bp_bubble *result, int32_t result_mass_given, bp_bubble *out,
+                         uint32_t out_capacity, uint32_t *out_count, bp_stage_rejection *why);
+/* Split: `source` becomes `first` and `second` with area shares summing to 1 (within 1e-6); the
+ * parts' masses and radii follow the shares. */
+bp_status bp_stage_split(const bp_stage_scene *scene, int64_t source, const bp_bubble *first, float first_share,
+                         const bp_bubble *second, float second_share, bp_bubble *out, uint32_t out_capacity,
+                         uint32_t *out_count, bp_stage_rejection *why);
+
 /* Test switch mirroring the injected NativeMetalCommandExecuting of the reference
  * (NativeMetalSolverLifecycleTests.swift:292-308): the next step fails with BP_ERR_COMMAND_FAILED
  * before touching the device. */
