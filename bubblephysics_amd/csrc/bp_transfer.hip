@@ -108,7 +108,8 @@ __device__ void remove_slot(const Arena *__restrict__ A, uint32_t p, uint32_t o,
 
 // One workgroup per transaction (each names a different world). The arithmetic is FP32, one
 // rounding per operation, in the order of the host-side statement of the same transactions:
-//   impulse: v_i += (J * m_i) * (1/m_i) for a movable bubble;
+//   impulse: v_i = float(double(v_i) + double(J) / double(m_i)) for a movable bubble (the
+//            reference's FP64 form, TransactionStager.swift:312-324);
 //   add:     a copy of bubble i with a new id (largest + 1) at c_i + (2 r_i) * direction;
 //   remove:  bubble i leaves;
 //   merge:   bubbles a < b (i and its next index, or previous for the last) become one bubble in
@@ -133,10 +134,10 @@ __global__ void transaction_kernel(const Arena *__restrict__ A, WorldDesc *descs
             const uint32_t g = o + i;
             const float invm = A->invm[g];
             if (invm > 0.0f) {
-                const float m = A->mass[g];
-                const float jx = tx.impulse[0] * m, jy = tx.impulse[1] * m;
+                const double m = (double)A->mass[g];
                 const float2 v = A->vel[p][g];
-                A->vel[p][g] = make_float2(v.x + jx * invm, v.y + jy * invm);
+                A->vel[p][g] = make_float2((float)((double)v.x + (double)tx.impulse[0] / m),
+                                           (float)((double)v.y + (double)tx.impulse[1] / m));
             }
         }
     } else if (tx.kind == BP_TX_ADD) {

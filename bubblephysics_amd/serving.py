@@ -11,8 +11,9 @@ This module builds request traces (a seeded Poisson process), states the transac
 (``apply_transaction``, the host-side statement the device kernel is tested against) and wraps
 the server. The reference has no serving form: its game loop steps one world per call
 (BubbleWorld.submit/advance, Sources/BubblePhysicsPublic/BubbleWorld.swift:252-283). The
-transactions are simplified forms of its staging (TransactionStager.swift:312-324 for impulses,
-StructuralOperations.swift:10-151 for merge and split): mass, momentum and area are kept.
+impulse is the reference's (TransactionStager.swift:312-324, in FP64); the structural transactions
+are simplified forms of its staging (StructuralOperations.swift:10-151 for merge and split, whose
+full rules are bp_stage_merge and bp_stage_split): mass, momentum and area are kept.
 """
 from __future__ import annotations
 
@@ -36,7 +37,7 @@ class Request:
     world: int
     transaction: str | None = None
     index: int = 0                   # bubble of the transaction: index % bubble count
-    impulse: tuple = (0.0, 0.0)      # velocity change of an impulse (J = impulse * m)
+    impulse: tuple = (0.0, 0.0)      # impulse J of an impulse transaction (velocity change J / m)
     direction: tuple = (1.0, 0.0)    # unit vector of add and split
     # Filled by the server.
     done: float = math.nan
@@ -79,11 +80,10 @@ def apply_transaction(bubbles: np.ndarray, req: Request) -> np.ndarray:
     f32 = np.float32
     i = int(req.index) % n
     kind = req.transaction
-    if kind == "impulse":                                            # v += (J m) / m
+    if kind == "impulse":                       # v = Float(Double(v) + Double(J) / Double(m)), TransactionStager.swift:312-324
         if b["inverse_mass"][i] > 0:
-            m = b["mass"][i]
-            j = np.array(req.impulse, f32) * m
-            b["velocity"][i] = b["velocity"][i] + j * b["inverse_mass"][i]
+            m = np.float64(b["mass"][i])
+            b["velocity"][i] = (b["velocity"][i].astype(np.float64) + np.array(req.impulse, f32).astype(np.float64) / m).astype(f32)
         return b
     if kind == "remove":
         return np.delete(b, i) if n > 1 else b

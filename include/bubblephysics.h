@@ -549,7 +549,7 @@ typedef struct bp_server bp_server;
  * id + 1) and keep mass, momentum and area. */
 enum {
     BP_TX_NONE = 0,
-    BP_TX_IMPULSE = 1,  /* v_i += impulse (as J = impulse * m_i, applied as J / m_i); movable bubbles only */
+    BP_TX_IMPULSE = 1,  /* v_i = float(double(v_i) + double(J) / double(m_i)), J = impulse; movable bubbles only */
     BP_TX_ADD = 2,      /* copy of bubble i at c_i + 2 r_i * direction, new id                        */
     BP_TX_REMOVE = 3,   /* bubble i leaves (not the last bubble of a world)                            */
     BP_TX_MERGE = 4,    /* bubble i and its index neighbour (i + 1, or i - 1 for the last) become one  */
