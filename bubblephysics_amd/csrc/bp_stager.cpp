@@ -13,6 +13,8 @@
 
 #include "../../include/bubblephysics.h"
 
+static_assert(sizeof(bp_stage_scene) == 56 && sizeof(bp_stage_rejection) == 16, "staging records changed size");
+
 namespace {
 
 struct P2 { float x, y; };

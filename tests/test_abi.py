@@ -43,6 +43,17 @@ def test_record_layouts_match_the_header(hip_library):
     assert abi.REQUEST_DTYPE.itemsize == 48 and abi.RESULT_DTYPE.itemsize == 56
 
 
+def test_staging_records_and_codes_match_the_header():
+    # bp_stage_scene and bp_stage_rejection: the sizes bp_stager.cpp asserts at compile time.
+    assert ctypes.sizeof(abi.StageScene) == 56 and ctypes.sizeof(abi.StageRejection) == 16
+    text = (ROOT / "include" / "bubblephysics.h").read_text()
+    for name in ("BP_PLACEMENT_OUTSIDE_REGION", "BP_PLACEMENT_INSIDE_RIGID_BODY", "BP_PLACEMENT_COINCIDENT_CENTER",
+                 "BP_REJECT_INVALID_STRUCTURAL_OPERATION", "BP_REJECT_INVALID_PLACEMENT", "BP_REJECT_DUPLICATE_ID",
+                 "BP_REJECT_UNKNOWN_OBJECT"):
+        (value,) = re.findall(rf"\b{name} = (\d+)", text)
+        assert int(value) == getattr(abi, name), name
+
+
 def test_server_defaults(hip_library):
     cfg = abi.ServerConfig()
     hip_library.bp_server_config_default(ctypes.byref(cfg))
