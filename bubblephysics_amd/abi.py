@@ -31,6 +31,7 @@ BP_ERR_GUARD_FAILURE = 10
 BP_ERR_CONTACT_VIOLATION = 11
 BP_ERR_SUBSTEP_LIMIT_EXCEEDED = 12
 BP_ERR_UNAVAILABLE = 13
+BP_ERR_INVALID_FRAME = 14
 
 STATUS_NAMES = {
     0: "ok", 1: "invalidArgument", 2: "resourceUnavailable", 3: "allocationFailed", 4: "commandSetupFailed",
@@ -176,7 +177,7 @@ class ServerResult(ctypes.Structure):
         ("tag", ctypes.c_uint64), ("world", ctypes.c_uint32), ("status", ctypes.c_int32), ("substeps", ctypes.c_uint32),
         ("bubbles", ctypes.c_uint32), ("arrival", ctypes.c_double), ("completed", ctypes.c_double),
         ("launch_worlds", ctypes.c_uint32), ("kernel_class", ctypes.c_uint32), ("device_milliseconds", ctypes.c_float),
-        ("reserved", ctypes.c_uint32),
+        ("transaction_result", ctypes.c_uint32),
     ]
 
 
@@ -218,7 +219,7 @@ class StageScene(ctypes.Structure):
 
 BP_PLACEMENT_OK, BP_PLACEMENT_OUTSIDE_REGION, BP_PLACEMENT_INSIDE_RIGID_BODY, BP_PLACEMENT_COINCIDENT_CENTER = 0, 1, 2, 3
 (BP_REJECT_NONE, BP_REJECT_INVALID_STRUCTURAL_OPERATION, BP_REJECT_INVALID_PLACEMENT, BP_REJECT_DUPLICATE_ID,
- BP_REJECT_UNKNOWN_OBJECT) = 0, 1, 2, 3, 4
+ BP_REJECT_UNKNOWN_OBJECT, BP_REJECT_INCOMPATIBLE_MOTION_ROLE, BP_REJECT_INVALID_VALUE) = 0, 1, 2, 3, 4, 5, 6
 
 
 # numpy views of the record arrays (same layout as the C structures).
@@ -251,7 +252,11 @@ SIGNATURES = {
     "bp_world_destroy": ([_P], None),
     "bp_world_set_config": ([_P, _P], _I32),
     "bp_world_set_bubbles": ([_P, _P, _U32], _I32),
-    "bp_world_update_bubbles": ([_P, _P, _P, _U32], _I32),
+    "bp_world_update_bubbles": ([_P, _U64, _P, _P, _U32], _I32),
+    "bp_world_publication": ([_P], _U64),
+    "bp_world_region_results": ([_P, _P, _U32, _P, _P, _U32, _P], _I32),
+    "bp_world_remove_bubbles": ([_P, _P, _U32], _I32),
+    "bp_batch_region_results": ([_P, _U32, _P, _U32, _P, _P, _U32, _P], _I32),
     "bp_world_set_segments": ([_P, _P, _U32], _I32),
     "bp_world_set_bodies": ([_P, _P, _U32, _P, _U32, _P, _U32], _I32),
     "bp_world_set_forces": ([_P, _P, _U32, _P, _U32], _I32),
@@ -269,6 +274,7 @@ SIGNATURES = {
     "bp_batch_contacts": ([_P, _U32, _P, _U32, _P], _I32),
     "bp_contact_transitions": ([_P, _U32, _P, _U32, _P, _U32, ctypes.c_float, _P, _U32, _U32, _P, _P, _U32, _P], _I32),
     "bp_stage_merge": ([_P, _P, _U32, _P, ctypes.c_int32, _P, _U32, _P, _P], _I32),
+    "bp_stage_add": ([_P, _P, _P, _U32, _P, _P], _I32),
     "bp_stage_split": ([_P, ctypes.c_int64, _P, ctypes.c_float, _P, ctypes.c_float, _P, _U32, _P, _P], _I32),
     "bp_static_polygon_edges": ([_P, _U32, _I64, _I64, _U32, _P, _U32, _P], _I32),
     "bp_polygon_body_geometry": ([_P, _U32, _P, _P, _P, _P, _P, _P], _I32),
@@ -280,6 +286,7 @@ SIGNATURES = {
     "bp_batch_bubble_count": ([_P, _U32], _U32),
     "bp_batch_step": ([_P, _U32, _P], _I32),
     "bp_batch_read_bubbles": ([_P, _U32, _P, _U32], _I32),
+    "bp_batch_read_bodies": ([_P, _U32, _P, _U32], _I32),
     "bp_batch_last_gpu_milliseconds": ([_P], _F),
     "bp_batch_last_class_timing": ([_P, _P, _P, _P, _P], _I32),
     "bp_batch_class_count": ([], _U32),

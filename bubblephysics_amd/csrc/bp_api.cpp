@@ -69,7 +69,26 @@ bp_status bp_batch::pack() {
     if (descs && !hostDescs.empty()) {
         std::vector<bpd::WorldDesc> cur(std::min(hostDescs.size(), worlds.size()));
         if (!cur.empty() && hipMemcpy(cur.data(), descs, cur.size() * sizeof(bpd::WorldDesc), hipMemcpyDeviceToHost) == hipSuccess)
-            for (size_t w = 0; w < cur.size(); ++w) worlds[w].failed = cur[w].failed != 0;
+            for (size_t w = 0; w < cur.size(); ++w) {
+                worlds[w].failed = cur[w].failed == bpd::kWorldFailed;
+                if (cur[w].failed == bpd::kWorldHeld) {
+                    // A held world's removed bubbles are still in its published range; the repack
+                    // reads the state back first (download_state), so delete them from the mirror.
+                    std::vector<uint32_t> region(cur[w].bubbleCount);
+                    std::vector<int64_t> ids(cur[w].bubbleCount);
+                    if (!region.empty() &&
+                        hipMemcpy(region.data(), arena.region[cur[w].pub] + cur[w].bubbleOffset, region.size() * 4,
+                                  hipMemcpyDeviceToHost) == hipSuccess &&
+                        hipMemcpy(ids.data(), arena.id + cur[w].bubbleOffset, ids.size() * 8, hipMemcpyDeviceToHost) == hipSuccess) {
+                        auto &bs = worlds[w].bubbles;
+                        for (size_t i = 0; i < region.size(); ++i)
+                            if (region[i] == BP_REGION_REMOVED) {
+                                const int64_t id = ids[i];
+                                bs.erase(std::remove_if(bs.begin(), bs.end(), [&](const bp_bubble &b) { return b.id == id; }), bs.end());
+                            }
+                    }
+                }
+            }
     }
     release();
     uint32_t nb = 0, ns = 0, nbod = 0, nv = 0, nt = 0, nm = 0, nband = 0, nf = 0, nfv = 0, nfi = 0;
@@ -578,6 +597,8 @@ bp_status bp_batch::launch(const uint32_t *subset, uint32_t subsetCount, uint32_
             orderDirty = false;
         }
     }
+    if (!settling)
+        for (uint32_t wi : pendingList) worlds[wi].removedIds.clear();
     auto orderOf = [&](int k) -> const uint32_t * { return subset ? subsetOrders + launchOffset[k] : orders[k]; };
     auto worldOf = [&](int k, uint32_t j) -> uint32_t { return subset ? subsetHost[launchOffset[k] + j] : classWorlds[k][j]; };
     if (W == 0 || steps == 0) return BP_OK;
@@ -660,7 +681,136 @@ bp_status bp_batch::finish(bp_frame_info *infos, bool wait) {
         if (!launchCount[k]) continue;
         BP_HIP(hipEventElapsedTime(&lastClassMs[k], classStart[k], classEnd[k]));
     }
-    return collect(steps, infos, pendingList, ms);
+    bp_status s = collect(steps, infos, pendingList, ms);
+    if (s != BP_OK || settling) return s;
+    const std::vector<uint32_t> listed = pendingList;
+    return settle_held(steps, infos, listed);
+}
+
+// Deletes the region-removed bubbles of the held worlds (one workgroup each) and records their ids.
+// removals[j]: bubbles the last step of held[j] removed.
+bp_status bp_batch::drop_removed(const std::vector<uint32_t> &held, const std::vector<uint32_t> &removals) {
+    const uint32_t count = (uint32_t)held.size();
+    if (!count) return BP_OK;
+    std::vector<bpd::DropRecord> recs(count);
+    uint32_t total = 0;
+    for (uint32_t j = 0; j < count; ++j) {
+        const uint32_t wi = held[j];
+        const HostWorld &w = worlds[wi];
+        const uint32_t n = hostDescs[wi].bubbleCount >= removals[j] ? hostDescs[wi].bubbleCount - removals[j] : 0u;
+        recs[j].world = wi;
+        recs[j].buckets = w.config.grid_bucket_count == 0 ? auto_buckets(n) : std::min(w.config.grid_bucket_count, bpd::kMaxBuckets);
+        recs[j].removedOffset = total;
+        recs[j].reserved = 0;
+        total += removals[j];
+    }
+    if (dropCapacity < count) { bp_status s = alloc(dropRecs, count); if (s != BP_OK) return s; dropCapacity = count; }
+    if (dropIdCapacity < std::max<uint32_t>(total, 1)) {
+        bp_status s = alloc(dropIds, std::max<uint32_t>(total, 1));
+        if (s != BP_OK) return s;
+        dropIdCapacity = std::max<uint32_t>(total, 1);
+    }
+    BP_HIP(hipMemcpyAsync(dropRecs, recs.data(), count * sizeof(bpd::DropRecord), hipMemcpyHostToDevice, streams[0]));
+    hipError_t e = bp_launch_drop_removed(arenaDev, descs, dropRecs, count, dropIds, streams[0]);
+    if (e != hipSuccess) { set_error(std::string("drop kernel: ") + hipGetErrorString(e)); return BP_ERR_COMMAND_FAILED; }
+    std::vector<int64_t> ids(total);
+    if (total) BP_HIP(hipMemcpyAsync(ids.data(), dropIds, total * sizeof(int64_t), hipMemcpyDeviceToHost, streams[0]));
+    BP_HIP(hipStreamSynchronize(streams[0]));
+    for (uint32_t j = 0; j < count; ++j) {
+        const uint32_t wi = held[j];
+        HostWorld &w = worlds[wi];
+        auto &bs = w.bubbles;
+        for (uint32_t k = 0; k < removals[j]; ++k) {
+            const int64_t id = ids[recs[j].removedOffset + k];
+            w.removedIds.push_back(id);
+            auto it = std::lower_bound(bs.begin(), bs.end(), id, [](const bp_bubble &b, int64_t v) { return b.id < v; });
+            if (it != bs.end() && it->id == id) bs.erase(it);
+        }
+        hostDescs[wi].bubbleCount = (uint32_t)bs.size();
+        hostDescs[wi].buckets = recs[j].buckets;
+        hostDescs[wi].failed = bpd::kWorldRunnable;
+    }
+    return BP_OK;
+}
+
+// After a launch: worlds whose last executed step removed bubbles from the region are held on the
+// device. A batch deletes their removed bubbles and runs the launch's remaining steps for them,
+// writing their results into infos (step-major over `listed`, may be null); a plain world only
+// makes them runnable again.
+bp_status bp_batch::settle_held(uint32_t steps, bp_frame_info *infos, const std::vector<uint32_t> &listed) {
+    const size_t W = worlds.size();
+    std::vector<uint32_t> done(listed.size(), 0), held, removals;
+    std::vector<size_t> slot;                         // position of a held world in `listed`
+    for (size_t j = 0; j < listed.size(); ++j) {
+        const uint32_t wi = listed[j];
+        for (uint32_t st = 0; st < steps; ++st) {
+            const bpd::WorldStatus &r = hostStats[(size_t)st * W + wi];
+            if (!r.executed) break;
+            done[j] = st + 1;
+        }
+        if (!done[j]) continue;
+        const bpd::WorldStatus &last = hostStats[(size_t)(done[j] - 1) * W + wi];
+        if (last.status == BP_OK && last.removals) { held.push_back(wi); removals.push_back(last.removals); slot.push_back(j); }
+    }
+    if (held.empty()) return BP_OK;
+    if (!dropOnHold) {
+        for (uint32_t wi : held) {
+            bpd::WorldDesc d;
+            BP_HIP(hipMemcpy(&d, descs + wi, sizeof d, hipMemcpyDeviceToHost));
+            if (d.failed == bpd::kWorldHeld) { d.failed = bpd::kWorldRunnable; BP_HIP(hipMemcpy(descs + wi, &d, sizeof d, hipMemcpyHostToDevice)); }
+        }
+        return BP_OK;
+    }
+    settling = true;
+    bp_status s = BP_OK;
+    const size_t L = listed.size();
+    while (!held.empty() && s == BP_OK) {
+        s = drop_removed(held, removals);
+        if (s != BP_OK) break;
+        // Worlds with steps left, grouped by how many.
+        std::vector<uint32_t> nextHeld, nextRemovals;
+        std::vector<size_t> nextSlot;
+        std::vector<size_t> order(held.size());
+        for (size_t k = 0; k < order.size(); ++k) order[k] = k;
+        std::sort(order.begin(), order.end(), [&](size_t a, size_t b) { return done[slot[a]] < done[slot[b]]; });
+        for (size_t a = 0; a < order.size() && s == BP_OK;) {
+            const uint32_t already = done[slot[order[a]]];
+            size_t b = a;
+            std::vector<uint32_t> group;
+            std::vector<size_t> groupSlot;
+            while (b < order.size() && done[slot[order[b]]] == already) {
+                group.push_back(held[order[b]]);
+                groupSlot.push_back(slot[order[b]]);
+                ++b;
+            }
+            a = b;
+            if (already >= steps) continue;
+            const uint32_t rest = steps - already;
+            std::vector<bp_frame_info> tmp((size_t)rest * group.size());
+            s = launch(group.data(), (uint32_t)group.size(), rest);
+            if (s == BP_OK) s = finish(tmp.data(), true);
+            if (s != BP_OK) break;
+            for (size_t g = 0; g < group.size(); ++g) {
+                const size_t j = groupSlot[g];
+                uint32_t ran = 0;
+                for (uint32_t st = 0; st < rest; ++st) {
+                    const bp_frame_info &r = tmp[(size_t)st * group.size() + g];
+                    if (infos) infos[(size_t)(already + st) * L + j] = r;
+                    if (hostStats[(size_t)st * W + group[g]].executed) ran = st + 1;
+                }
+                done[j] = already + ran;
+                if (ran && tmp[(size_t)(ran - 1) * group.size() + g].status == BP_OK &&
+                    tmp[(size_t)(ran - 1) * group.size() + g].region_removals) {
+                    nextHeld.push_back(group[g]);
+                    nextRemovals.push_back(tmp[(size_t)(ran - 1) * group.size() + g].region_removals);
+                    nextSlot.push_back(j);
+                }
+            }
+        }
+        held.swap(nextHeld); removals.swap(nextRemovals); slot.swap(nextSlot);
+    }
+    settling = false;
+    return s;
 }
 
 // Result records of the last `steps` steps into infos[step * listed + j] for the j-th listed world.
@@ -695,8 +845,67 @@ bp_status bp_batch::collect(uint32_t steps, bp_frame_info *infos, const std::vec
             if (cls == kGlobalClass) out.execution_path = BP_PATH_WORKGROUP;
             if (cls == kCoopClass) out.execution_path = BP_PATH_COOPERATIVE;
             out.gpu_milliseconds = ms / steps;
-            if (!hostStats[(size_t)st * W + wi].executed) out.status = BP_ERR_UNAVAILABLE;
+            const bpd::WorldStatus &rec = hostStats[(size_t)st * W + wi];
+            if (!rec.executed) out.status = BP_ERR_UNAVAILABLE;
+            // A record that claims success must describe a finished step within the substep limit
+            // (the read-back consistency check of the reference's frame runner): otherwise the frame
+            // is invalid and nothing it reports may be used.
+            else if (rec.status == BP_OK && (rec.substeps > bpd::kFrameLimit || rec.time[0] - rec.time[1] > 0.0f ||
+                                             rec.removals + rec.wraps > hostDescs[wi].bubbleCapacity))
+                out.status = BP_ERR_INVALID_FRAME;
         }
+    return BP_OK;
+}
+
+// Region results of a world: the ids the region removed (a batch: during its last stepping call,
+// after which they were deleted; a plain world: in its last published step, whose frame still
+// holds them) and the ids it wrapped in the world's last published step, ascending.
+bp_status bp_batch::region_results(uint32_t wi, int64_t *removed, uint32_t removedCapacity, uint32_t *removedCount,
+                                   int64_t *wrapped, uint32_t wrappedCapacity, uint32_t *wrappedCount) {
+    std::vector<int64_t> rm, wr;
+    const HostWorld &w = worlds[wi];
+    if (packed && deviceValid && w.resident) {
+        bpd::WorldDesc d;
+        BP_HIP(hipMemcpy(&d, descs + wi, sizeof d, hipMemcpyDeviceToHost));
+        std::vector<uint32_t> region(d.bubbleCount);
+        std::vector<int64_t> ids(d.bubbleCount);
+        if (d.bubbleCount) {
+            BP_HIP(hipMemcpy(region.data(), arena.region[d.pub] + d.bubbleOffset, d.bubbleCount * sizeof(uint32_t), hipMemcpyDeviceToHost));
+            BP_HIP(hipMemcpy(ids.data(), arena.id + d.bubbleOffset, d.bubbleCount * sizeof(int64_t), hipMemcpyDeviceToHost));
+        }
+        for (uint32_t i = 0; i < d.bubbleCount; ++i) {
+            if (region[i] == BP_REGION_REMOVED) rm.push_back(ids[i]);
+            if (region[i] == BP_REGION_WRAPPED) wr.push_back(ids[i]);
+        }
+    }
+    if (dropOnHold) rm = w.removedIds;
+    std::sort(rm.begin(), rm.end());
+    if (removedCount) *removedCount = (uint32_t)rm.size();
+    if (wrappedCount) *wrappedCount = (uint32_t)wr.size();
+    if ((rm.size() && removed && removedCapacity < rm.size()) || (wr.size() && wrapped && wrappedCapacity < wr.size()))
+        return BP_ERR_CAPACITY_OVERFLOW;
+    if (removed) std::copy(rm.begin(), rm.end(), removed);
+    if (wrapped) std::copy(wr.begin(), wr.end(), wrapped);
+    return BP_OK;
+}
+
+// Deletes bubbles by id from a world's published state (the next snapshot of a public world that
+// dropped them, NativePublicWorld.swift:264-267), with their motor and band records.
+bp_status bp_batch::remove_bubbles(uint32_t wi, const int64_t *ids, uint32_t count) {
+    bp_status s = download_state();
+    if (s != BP_OK) return s;
+    HostWorld &w = worlds[wi];
+    std::vector<int64_t> gone(ids, ids + count);
+    std::sort(gone.begin(), gone.end());
+    for (int64_t id : gone) {
+        auto it = std::lower_bound(w.bubbles.begin(), w.bubbles.end(), id, [](const bp_bubble &b, int64_t v) { return b.id < v; });
+        if (it == w.bubbles.end() || it->id != id) { set_error("removal names a bubble the world does not hold"); return BP_ERR_INVALID_SNAPSHOT; }
+    }
+    auto isGone = [&](int64_t id) { return std::binary_search(gone.begin(), gone.end(), id); };
+    w.bubbles.erase(std::remove_if(w.bubbles.begin(), w.bubbles.end(), [&](const bp_bubble &b) { return isGone(b.id); }), w.bubbles.end());
+    w.motors.erase(std::remove_if(w.motors.begin(), w.motors.end(), [&](const bp_motor &m) { return isGone(m.bubble_id); }), w.motors.end());
+    w.bands.erase(std::remove_if(w.bands.begin(), w.bands.end(), [&](const bp_band &m) { return isGone(m.bubble_id); }), w.bands.end());
+    packed = false;
     return BP_OK;
 }
 
@@ -863,7 +1072,13 @@ bp_status bp_batch::replay(bp_frame_info *infos) {
     for (int k = 0; k < kClassCount; ++k) lastClassMs[k] = 0.0f;
     std::vector<uint32_t> all(worlds.size());
     for (uint32_t wi = 0; wi < all.size(); ++wi) all[wi] = wi;
-    return collect(graphSteps, infos, all, ms);
+    for (auto &w : worlds) w.removedIds.clear();
+    bp_status s = collect(graphSteps, infos, all, ms);
+    if (s != BP_OK) return s;
+    // A world held for region removals leaves the graph's launches: the deletion moves it to a
+    // smaller bubble count, which the captured launches still cover (counts are read from the
+    // descriptors), so the graph stays valid.
+    return settle_held(graphSteps, infos, all);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -910,6 +1125,7 @@ const char *bp_status_string(bp_status s) {
         case BP_ERR_CONTACT_VIOLATION: return "contact violation";
         case BP_ERR_SUBSTEP_LIMIT_EXCEEDED: return "substep limit exceeded";
         case BP_ERR_UNAVAILABLE: return "unavailable";
+        case BP_ERR_INVALID_FRAME: return "invalid frame";
         default: return "unknown status";
     }
 }
@@ -1024,6 +1240,14 @@ bp_status bp_batch_write_bubbles(bp_batch *b, uint32_t world, const bp_bubble *b
     return b->write_bubbles(world, bubbles, count);
 }
 
+bp_status bp_batch_region_results(bp_batch *b, uint32_t world, int64_t *removed, uint32_t removed_capacity,
+                                  uint32_t *out_removed, int64_t *wrapped, uint32_t wrapped_capacity, uint32_t *out_wrapped) {
+    if (!b || world >= b->worlds.size()) return BP_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(b->mutex);
+    if (b->inFlight || b->pending || b->serving) return BP_ERR_FRAME_IN_FLIGHT;
+    return b->region_results(world, removed, removed_capacity, out_removed, wrapped, wrapped_capacity, out_wrapped);
+}
+
 bp_status bp_batch_set_resident(bp_batch *b, uint32_t world, uint32_t resident) {
     if (!b || world >= b->worlds.size()) return BP_ERR_INVALID_ARGUMENT;
     std::lock_guard<std::mutex> lock(b->mutex);
@@ -1052,6 +1276,18 @@ bp_status bp_batch_read_bubbles(bp_batch *b, uint32_t world, bp_bubble *out, uin
     const HostWorld &w = b->worlds[world];
     if (count != w.bubbles.size() || (count && !out)) return BP_ERR_INVALID_ARGUMENT;
     std::copy(w.bubbles.begin(), w.bubbles.end(), out);
+    return BP_OK;
+}
+
+bp_status bp_batch_read_bodies(bp_batch *b, uint32_t world, bp_body *out, uint32_t count) {
+    if (!b || world >= b->worlds.size()) return BP_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(b->mutex);
+    if (b->inFlight || b->pending || b->serving) return BP_ERR_FRAME_IN_FLIGHT;
+    bp_status s = b->download_state();
+    if (s != BP_OK) return s;
+    const HostWorld &hw = b->worlds[world];
+    if (count != hw.bodies.size() || (count && !out)) return BP_ERR_INVALID_ARGUMENT;
+    std::copy(hw.bodies.begin(), hw.bodies.end(), out);
     return BP_OK;
 }
 
@@ -1120,6 +1356,7 @@ bp_status bp_batch_set_world_class(bp_batch *b, uint32_t world, int32_t cls) {
 // ---------------------------------------------------------------------------------------
 struct bp_world {
     bp_batch batch;
+    uint64_t publication = 1;          // counts published snapshots: full uploads and successful steps
     bool failNext = false;
     bool inFlight = false;
     std::vector<float> history;
@@ -1137,6 +1374,7 @@ bp_status bp_world_create(const bp_world_config *config, bp_world **out) {
     hw.config = *config;
     hw.exportAdjacency = 1;
     w->batch.worlds.push_back(hw);
+    w->batch.dropOnHold = false;       // the solver's frame keeps removed bubbles; the caller deletes them
     *out = w;
     return BP_OK;
 }
@@ -1168,12 +1406,21 @@ bp_status bp_world_set_bubbles(bp_world *w, const bp_bubble *bubbles, uint32_t c
     for (uint32_t i = 1; i < count; ++i)
         if (!(bubbles[i - 1].id < bubbles[i].id)) { set_error("bubbles must be sorted by strictly increasing id"); return BP_ERR_INVALID_SNAPSHOT; }
     host_of(w).bubbles.assign(bubbles, bubbles + count);
+    ++w->publication;
     return BP_OK;
 }
 
-bp_status bp_world_update_bubbles(bp_world *w, const uint32_t *indices, const bp_bubble *bubbles, uint32_t count) {
+uint64_t bp_world_publication(const bp_world *w) { return w ? w->publication : 0u; }
+
+bp_status bp_world_update_bubbles(bp_world *w, uint64_t base_publication, const uint32_t *indices, const bp_bubble *bubbles,
+                                  uint32_t count) {
     if (!w || (count && (!indices || !bubbles))) return BP_ERR_INVALID_ARGUMENT;
     if (w->inFlight) return BP_ERR_FRAME_IN_FLIGHT;
+    if (base_publication != w->publication) {
+        // NativeMetalSolver.swift:416-419: a delta must be taken against the last published snapshot.
+        set_error("the delta is not based on the last published snapshot");
+        return BP_ERR_INVALID_SNAPSHOT;
+    }
     bp_status s = sync_host(w);
     if (s != BP_OK) return s;
     HostWorld &hw = host_of(w);
@@ -1268,8 +1515,24 @@ bp_status bp_world_step(bp_world *w, int32_t scenario_step, bp_frame_info *out) 
             (void)hipMemcpy(w->batch.descs, &d, sizeof d, hipMemcpyHostToDevice);
         }
     }
+    if (info.status == BP_OK) ++w->publication;
     if (out) *out = info;
     return info.status;
+}
+
+bp_status bp_world_region_results(bp_world *w, int64_t *removed, uint32_t removed_capacity, uint32_t *out_removed,
+                                  int64_t *wrapped, uint32_t wrapped_capacity, uint32_t *out_wrapped) {
+    if (!w) return BP_ERR_INVALID_ARGUMENT;
+    if (w->inFlight) return BP_ERR_FRAME_IN_FLIGHT;
+    return w->batch.region_results(0, removed, removed_capacity, out_removed, wrapped, wrapped_capacity, out_wrapped);
+}
+
+bp_status bp_world_remove_bubbles(bp_world *w, const int64_t *ids, uint32_t count) {
+    if (!w || (count && !ids)) return BP_ERR_INVALID_ARGUMENT;
+    if (w->inFlight) return BP_ERR_FRAME_IN_FLIGHT;
+    bp_status s = w->batch.remove_bubbles(0, ids, count);
+    if (s == BP_OK) ++w->publication;
+    return s;
 }
 
 bp_status bp_world_read_bubbles(bp_world *w, bp_bubble *out, uint32_t count) {

@@ -28,6 +28,8 @@ extern "C" hipError_t bp_launch_scatter_world(const bpd::Arena *arena, uint32_t 
                                               const int32_t *motorIdx, const int32_t *bandIdx, hipStream_t stream);
 extern "C" uint32_t bp_coop_block();
 extern "C" hipError_t bp_launch_fill_u32(uint32_t *p, size_t words, uint32_t value, hipStream_t stream);
+extern "C" hipError_t bp_launch_drop_removed(const bpd::Arena *arena, bpd::WorldDesc *descs, const bpd::DropRecord *recs,
+                                             uint32_t count, int64_t *removedIds, hipStream_t stream);
 extern "C" hipError_t bp_launch_audit(const bpd::Arena *arena, const bpd::WorldDesc *descs, uint32_t worlds, void *out,
                                       hipStream_t stream);
 extern "C" hipError_t bp_launch_contacts(const bpd::Arena *arena, const bpd::WorldDesc *descs, uint32_t world,
@@ -107,6 +109,7 @@ struct HostWorld {
     uint32_t capacity = 0;              // bubble slots of its device range (>= bubbles), set by pack
     int packedClass = -1;               // kernel configuration of the last pack (-1 = not on the device)
     bool failed = false;                // failed a step: not advanced again until new input (kept across packs)
+    std::vector<int64_t> removedIds;    // bubbles the region removed during the last stepping call
 };
 
 inline bool class_fits(int k, const HostWorld &w) {
@@ -181,6 +184,15 @@ struct bp_batch {
     hipGraphExec_t graphExec = nullptr;
     uint32_t graphSteps = 0;
     hipEvent_t forkEvent = nullptr, joinEvents[bph::kStreams] = {};
+    // Region removals: a world whose step removed bubbles is held on the device until they are
+    // deleted. A batch deletes them and runs the rest of the call's steps (the public world's
+    // bookkeeping, NativePublicWorld.swift:264-267); a plain world (bp_world) keeps them in its
+    // published frame, as the reference's solver does, and leaves the deletion to the caller.
+    bool dropOnHold = true;
+    bool settling = false;
+    bpd::DropRecord *dropRecs = nullptr;
+    int64_t *dropIds = nullptr;
+    size_t dropCapacity = 0, dropIdCapacity = 0;
 
     ~bp_batch() { release(); }
 
@@ -211,6 +223,7 @@ struct bp_batch {
         startEvent = nullptr;
         stats = nullptr; statsCapacity = 0; descs = nullptr; arenaDev = nullptr; subsetOrders = nullptr;
         xferBubbles = nullptr; xferIndices = nullptr; xferCapacity = 0;
+        dropRecs = nullptr; dropIds = nullptr; dropCapacity = dropIdCapacity = 0;
         pending = false;
         for (auto &o : orders) o = nullptr;
         packed = false;
@@ -246,6 +259,11 @@ struct bp_batch {
     bp_status capture(uint32_t steps);
     bp_status replay(bp_frame_info *infos);
     bp_status collect(uint32_t steps, bp_frame_info *infos, const std::vector<uint32_t> &listed, float ms);
+    bp_status settle_held(uint32_t steps, bp_frame_info *infos, const std::vector<uint32_t> &listed);
+    bp_status drop_removed(const std::vector<uint32_t> &held, const std::vector<uint32_t> &removals);
+    bp_status region_results(uint32_t wi, int64_t *removed, uint32_t removedCapacity, uint32_t *removedCount,
+                             int64_t *wrapped, uint32_t wrappedCapacity, uint32_t *wrappedCount);
+    bp_status remove_bubbles(uint32_t wi, const int64_t *ids, uint32_t count);
 };
 
 #endif  // BP_BATCH_H

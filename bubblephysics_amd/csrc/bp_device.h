@@ -127,10 +127,14 @@ struct WorldDesc {
     uint32_t assertEverySubstep, injectionEnabled, injectionAfter;
     int64_t injectionObject;
     float injectionPosition[2];
-    uint32_t failed;          // sticky: a world that failed a step is not advanced again
+    uint32_t failed;          // kWorldFailed: failed a step, sticky, not advanced again; kWorldHeld: a
+                              // step removed bubbles from the region, the world waits until they are
+                              // deleted (kWorldRunnable = 0 otherwise)
     uint32_t exportAdjacency; // 1 = leave the rows of the last grid build in rowStart/rowLen/adj
     uint32_t bubbleCapacity;  // bubble slots of the world's range (transactions never grow past it)
 };
+
+enum : uint32_t { kWorldRunnable = 0u, kWorldFailed = 1u, kWorldHeld = 2u };
 
 // A serving transaction on one world, applied on the device before the world's step
 // (bp_request of include/bubblephysics.h; `buckets` is the grid bucket count for the bubble
@@ -138,6 +142,13 @@ struct WorldDesc {
 struct TxRecord {
     uint32_t world, kind, index, buckets;
     float impulse[2], direction[2];
+};
+
+// A world held after a step that removed bubbles from the region (kWorldHeld): its removed
+// bubbles are deleted on the device before it steps again. `buckets` is the grid bucket count for
+// the bubble count that remains; the ids of the removed bubbles go to removedIds[removedOffset...].
+struct DropRecord {
+    uint32_t world, buckets, removedOffset, reserved;
 };
 
 // Per-world result of the last step (bp_frame_info fields).
@@ -157,6 +168,7 @@ struct WorldStatus {
     uint32_t executed;        // 1 when the kernel ran this world's step
     uint32_t ticks;           // device wall-clock ticks (100 MHz) the workgroup spent on the step
     uint32_t seq;             // set to 1 last, with a system-scope release, when the launch asks for it
+    uint32_t txResult;        // BP_REJECT_* of a serving transaction applied on the device (0 = applied)
 };
 
 // One end-of-step contact record (bp_contact without its id, which the host hashes from the key).

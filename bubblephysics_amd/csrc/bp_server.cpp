@@ -23,6 +23,7 @@
 #include "bp_batch.h"
 
 #include <atomic>
+#include <cmath>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -31,7 +32,7 @@
 using namespace bph;
 
 extern "C" hipError_t bp_launch_transactions(const bpd::Arena *arena, bpd::WorldDesc *descs, const bpd::TxRecord *txs,
-                                             uint32_t count, hipStream_t stream);
+                                             uint32_t count, bpd::WorldStatus *stats, hipStream_t stream);
 extern "C" hipError_t bp_launch_gather_full_world(const bpd::Arena *arena, const bpd::WorldDesc *desc, bp_bubble *out,
                                                   uint32_t count, hipStream_t stream);
 
@@ -43,6 +44,8 @@ struct Pending {
     bp_request req;
     double arrival;          // server clock
     int64_t sink;            // index into the trace results, or -1 for the poll queue
+    bool staged = false;     // the transaction was decided on the host (applied or refused)
+    uint32_t txResult = 0;   // its outcome (BP_REJECT_*) when staged
 };
 
 struct Launch {
@@ -90,6 +93,9 @@ struct bp_server {
     uint32_t W = 0;
     std::vector<std::deque<Pending>> queue;
     std::vector<uint8_t> busy, queued, resident, wantAdmission;
+    std::vector<uint8_t> worldFailed;           // a step failed: the world takes no more steps or transactions
+    std::vector<uint32_t> dropPending;          // bubbles the world's last step removed from the region
+    bpd::DropRecord *dropDev = nullptr;
     std::vector<uint32_t> count, headCount;
     std::vector<int> headClass;
     std::vector<uint64_t> lastUse, footprintOf;
@@ -118,7 +124,9 @@ struct bp_server {
     void enqueue(const Pending &p);
     void prepare_head(uint32_t w);
     void finish_request(const Pending &p, bp_status status, uint32_t substeps, uint32_t bubbles, uint32_t launchWorlds,
-                        int cls, float deviceMs);
+                        int cls, float deviceMs, uint32_t txResult = 0);
+    bp_status stage_structural(uint32_t w, Pending &p);
+    bp_status drop_removed(const std::vector<uint32_t> &ws);
     bool poll_launches();
     bool form_launches();
     bp_status start_launch(int cls);
@@ -208,6 +216,8 @@ bp_status bp_server::init() {
     for (uint32_t w = 0; w < W; ++w) if (resident[w]) footprintOf[w] = b.footprint(w);
     queue.assign(W, {});
     busy.assign(W, 0); queued.assign(W, 0); wantAdmission.assign(W, 0);
+    worldFailed.assign(W, 0); dropPending.assign(W, 0);
+    for (uint32_t w = 0; w < W; ++w) worldFailed[w] = b.worlds[w].failed ? 1 : 0;
     count.resize(W); headCount.assign(W, 0); headClass.assign(W, -1); lastUse.assign(W, 0);
     for (uint32_t w = 0; w < W; ++w) count[w] = (uint32_t)b.worlds[w].bubbles.size();
     bp_status s = BP_OK;
@@ -231,6 +241,7 @@ bp_status bp_server::init() {
         L.worlds.reserve(cap); L.reqs.reserve(cap); L.bubbles.reserve(cap); L.posted.reserve(cap);
         freeLaunch.push_back(k);
     }
+    if ((s = dev_alloc(this, dropDev, std::max<uint32_t>(W, 1))) != BP_OK) return s;
     streams.resize(std::max<uint32_t>(1, cfg.streams));
     streamLoad.assign(streams.size(), 0);
     idleStreams = (int)streams.size();
@@ -273,14 +284,13 @@ void bp_server::prepare_head(uint32_t w) {
             if (!wantAdmission[w]) { wantAdmission[w] = 1; admissions.push_back(w); }
             return;
         }
-        const Pending &p = queue[w].front();
+        Pending &p = queue[w].front();
         const uint32_t n = count[w];
-        uint32_t next = n;
-        switch (p.req.transaction) {
-            case BP_TX_ADD: case BP_TX_SPLIT: next = n == 0 ? 0 : n + 1; break;
-            case BP_TX_REMOVE: next = n > 1 ? n - 1 : n; break;
-            case BP_TX_MERGE: next = n > 1 ? n - 1 : n; break;
-            default: break;
+        if (worldFailed[w]) {
+            // worldFailed is final (BubbleWorld.swift:296-299, 424-430): no transaction, no step.
+            Pending done = p; queue[w].pop_front();
+            finish_request(done, BP_ERR_UNAVAILABLE, 0, n, 0, -1, 0.0f);
+            continue;
         }
         if (p.req.transaction > BP_TX_SPLIT) {
             set_error("unknown transaction kind");
@@ -288,11 +298,22 @@ void bp_server::prepare_head(uint32_t w) {
             finish_request(done, BP_ERR_INVALID_ARGUMENT, 0, n, 0, -1, 0.0f);
             continue;
         }
-        if (next > batch->worlds[w].capacity) {
-            Pending done = p; queue[w].pop_front();
-            ++local.rejected;
-            finish_request(done, BP_ERR_CAPACITY_OVERFLOW, 0, n, 0, -1, 0.0f);
-            continue;
+        uint32_t next = n;
+        const uint32_t kind = p.req.transaction;
+        if ((kind == BP_TX_ADD || kind == BP_TX_MERGE || kind == BP_TX_SPLIT) && !p.staged && n > 0) {
+            // Structural transactions are staged here, while the world is idle: its published state
+            // is final, and the staged list is written back before the world's next launch.
+            bp_status s = stage_structural(w, p);
+            if (s == BP_ERR_CAPACITY_OVERFLOW) {
+                Pending done = p; queue[w].pop_front();
+                ++local.rejected;
+                finish_request(done, BP_ERR_CAPACITY_OVERFLOW, 0, n, 0, -1, 0.0f);
+                continue;
+            }
+            if (s != BP_OK) { fail(s); return; }
+            next = count[w];
+        } else if (kind == BP_TX_REMOVE) {
+            next = n > 1 ? n - 1 : n;
         }
         headCount[w] = next;
         headClass[w] = class_for(w, next);
@@ -303,7 +324,7 @@ void bp_server::prepare_head(uint32_t w) {
 }
 
 void bp_server::finish_request(const Pending &p, bp_status status, uint32_t substeps, uint32_t bubbles, uint32_t launchWorlds,
-                               int cls, float deviceMs) {
+                               int cls, float deviceMs, uint32_t txResult) {
     bp_server_result r{};
     r.tag = p.req.tag;
     r.world = p.req.world;
@@ -315,6 +336,7 @@ void bp_server::finish_request(const Pending &p, bp_status status, uint32_t subs
     r.launch_worlds = launchWorlds;
     r.kernel_class = cls < 0 ? 0xffffffffu : (uint32_t)cls;
     r.device_milliseconds = deviceMs;
+    r.transaction_result = txResult;
     std::lock_guard<std::mutex> lock(mu);
     if (p.sink >= 0 && sinkResults) sinkResults[p.sink] = r;
     else results.push_back(r);
@@ -324,6 +346,108 @@ void bp_server::finish_request(const Pending &p, bp_status status, uint32_t subs
     cvDone.notify_all();
 }
 
+// Stages an add, merge or split on the host with the reference's rules (bp_stage_add,
+// bp_stage_merge, bp_stage_split: TransactionStager.swift:406-421, 630-735) against the world's
+// published records and its bodies at their current poses, then writes the staged list back in
+// place. A refused transaction leaves the world as it is (p.txResult holds the reason). Returns
+// BP_ERR_CAPACITY_OVERFLOW, changing nothing, when the result needs more slots than the world has.
+bp_status bp_server::stage_structural(uint32_t w, Pending &p) {
+    bp_batch &b = *batch;
+    HostWorld &hw = b.worlds[w];
+    const uint32_t n = count[w];
+    if (readCapacity < n) {
+        bp_status st = dev_alloc(this, readBuffer, n);
+        if (st != BP_OK) return st;
+        readCapacity = n;
+    }
+    std::vector<bp_bubble> recs(n);
+    hipError_t e = bp_launch_gather_full_world(b.arenaDev, b.descs + w, readBuffer, n, readStream);
+    if (e != hipSuccess) { set_error(std::string("gather: ") + hipGetErrorString(e)); return BP_ERR_COMMAND_FAILED; }
+    BP_HIP(hipMemcpyAsync(recs.data(), readBuffer, n * sizeof(bp_bubble), hipMemcpyDeviceToHost, readStream));
+    bpd::WorldDesc d;
+    BP_HIP(hipMemcpyAsync(&d, b.descs + w, sizeof d, hipMemcpyDeviceToHost, readStream));
+    BP_HIP(hipStreamSynchronize(readStream));
+    std::vector<bp_body> bodies = hw.bodies;
+    if (!bodies.empty()) {
+        std::vector<float4> pose(bodies.size());
+        BP_HIP(hipMemcpy(pose.data(), b.arena.bodyPose[d.pub] + d.bodyOffset, pose.size() * sizeof(float4), hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < bodies.size(); ++k) {
+            bodies[k].pose[0] = pose[k].x; bodies[k].pose[1] = pose[k].y; bodies[k].pose[2] = pose[k].z;
+        }
+    }
+    bp_stage_scene scene{};
+    scene.bubbles = recs.data();
+    scene.bubble_count = n;
+    scene.bodies = bodies.empty() ? nullptr : bodies.data();
+    scene.body_count = (uint32_t)bodies.size();
+    scene.vertices = hw.vertices.empty() ? nullptr : hw.vertices.data();
+    scene.vertex_count = (uint32_t)(hw.vertices.size() / 2);
+    for (int k = 0; k < 4; ++k) scene.region[k] = hw.config.region[k];
+    const uint32_t i = p.req.index % n;
+    const int64_t newId = recs[n - 1].id + 1;
+    std::vector<bp_bubble> out(n + 2);
+    uint32_t outCount = 0;
+    bp_stage_rejection why{};
+    bp_status st = BP_OK;
+    ++local.transactions;
+    if (p.req.transaction == BP_TX_ADD) {
+        bp_bubble nb = recs[i];
+        nb.id = newId;
+        const float offset = 2.0f * recs[i].radius;
+        nb.center[0] = nb.previous_center[0] = recs[i].center[0] + offset * p.req.direction[0];
+        nb.center[1] = nb.previous_center[1] = recs[i].center[1] + offset * p.req.direction[1];
+        nb.velocity[0] = nb.velocity[1] = 0.0f;
+        nb.rotation = nb.angular_velocity = 0.0f;
+        st = bp_stage_add(&scene, &nb, out.data(), (uint32_t)out.size(), &outCount, &why);
+    } else if (p.req.transaction == BP_TX_MERGE) {
+        if (n < 2) {
+            st = BP_ERR_INVALID_ARGUMENT;
+            why.reason = BP_REJECT_INVALID_STRUCTURAL_OPERATION;
+        } else {
+            const uint32_t k = i + 1 < n ? i + 1 : i - 1, a = std::min(i, k), c = std::max(i, k);
+            const int64_t sources[2] = {recs[a].id, recs[c].id};
+            st = bp_stage_merge(&scene, sources, 2, &recs[a], 0, out.data(), (uint32_t)out.size(), &outCount, &why);
+        }
+    } else {
+        bp_bubble first = recs[i], second = recs[i];
+        second.id = newId;
+        st = bp_stage_split(&scene, recs[i].id, &first, 0.5f, &second, 0.5f, out.data(), (uint32_t)out.size(), &outCount, &why);
+    }
+    p.staged = true;
+    if (st == BP_ERR_INVALID_ARGUMENT && why.reason != BP_REJECT_NONE) {
+        p.txResult = why.reason;                       // refused: the world is unchanged, the step runs
+        return BP_OK;
+    }
+    if (st != BP_OK) return st;
+    if (outCount > hw.capacity) { p.staged = false; --local.transactions; return BP_ERR_CAPACITY_OVERFLOW; }
+    st = b.write_bubbles(w, out.data(), outCount);
+    if (st != BP_OK) return st;
+    count[w] = outCount;
+    p.txResult = BP_REJECT_NONE;
+    return BP_OK;
+}
+
+// Deletes, on the device, the bubbles the last step of each listed world removed from the region
+// (the public world deletes them before the next frame, NativePublicWorld.swift:264-267).
+bp_status bp_server::drop_removed(const std::vector<uint32_t> &ws) {
+    if (ws.empty()) return BP_OK;
+    std::vector<bpd::DropRecord> recs(ws.size());
+    for (size_t j = 0; j < ws.size(); ++j) {
+        const uint32_t w = ws[j];
+        const uint32_t n = count[w] >= dropPending[w] ? count[w] - dropPending[w] : 0u;
+        recs[j] = bpd::DropRecord{w, buckets_for(w, n), 0u, 0u};
+    }
+    BP_HIP(hipMemcpyAsync(dropDev, recs.data(), recs.size() * sizeof(bpd::DropRecord), hipMemcpyHostToDevice, readStream));
+    hipError_t e = bp_launch_drop_removed(batch->arenaDev, batch->descs, dropDev, (uint32_t)ws.size(), nullptr, readStream);
+    if (e != hipSuccess) { set_error(std::string("drop kernel: ") + hipGetErrorString(e)); return BP_ERR_COMMAND_FAILED; }
+    BP_HIP(hipStreamSynchronize(readStream));
+    for (uint32_t w : ws) {
+        count[w] = count[w] >= dropPending[w] ? count[w] - dropPending[w] : 0u;
+        dropPending[w] = 0;
+    }
+    return BP_OK;
+}
+
 // Posts the result of every world whose record is marked complete, and frees the worlds of every
 // launch whose event has completed.
 bool bp_server::poll_launches() {
@@ -331,10 +455,14 @@ bool bp_server::poll_launches() {
     for (size_t k = 0; k < inflight.size();) {
         Launch &L = launches[inflight[k]];
         auto post = [&](uint32_t j) {
-            const bpd::WorldStatus &s = hostStats[L.worlds[j]];
+            const uint32_t w = L.worlds[j];
+            const bpd::WorldStatus &s = hostStats[w];
             const bp_status status = s.executed ? (bp_status)s.status : BP_ERR_UNAVAILABLE;
+            if (s.executed && status != BP_OK && status != BP_ERR_CAPACITY_OVERFLOW) worldFailed[w] = 1;
+            if (s.executed && status == BP_OK && s.removals) dropPending[w] = s.removals;
+            const uint32_t txr = L.reqs[j].staged ? L.reqs[j].txResult : s.txResult;
             finish_request(L.reqs[j], status, s.executed ? s.substeps : 0, L.bubbles[j], L.count, L.cls,
-                           s.executed ? (float)s.ticks * 1e-5f : 0.0f);
+                           s.executed ? (float)s.ticks * 1e-5f : 0.0f, txr);
             L.posted[j] = 1;
             ++L.postedCount;
             any = true;
@@ -344,8 +472,13 @@ bool bp_server::poll_launches() {
         hipError_t e = hipEventQuery(L.done);
         if (e == hipErrorNotReady) { ++k; continue; }
         if (e != hipSuccess) { set_error(std::string("hipEventQuery: ") + hipGetErrorString(e)); fail(BP_ERR_COMMAND_FAILED); return false; }
+        std::vector<uint32_t> held;
         for (uint32_t j = 0; j < L.count; ++j) {
             if (!L.posted[j]) post(j);                       // the launch is over: every record is final
+            if (dropPending[L.worlds[j]]) held.push_back(L.worlds[j]);
+        }
+        if (bp_status ds = drop_removed(held); ds != BP_OK) { fail(ds); return false; }
+        for (uint32_t j = 0; j < L.count; ++j) {
             const uint32_t w = L.worlds[j];
             busy[w] = 0;
             prepare_head(w);
@@ -378,12 +511,19 @@ bp_status bp_server::start_launch(int cls) {
         busy[w] = 1;
         Pending p = queue[w].front();
         queue[w].pop_front();
-        if (p.req.transaction != BP_TX_NONE && count[w] > 0) {
-            bpd::TxRecord &t = L.hTx[L.txCount++];
-            t.world = w; t.kind = p.req.transaction; t.index = p.req.index; t.buckets = buckets_for(w, headCount[w]);
-            t.impulse[0] = p.req.impulse[0]; t.impulse[1] = p.req.impulse[1];
-            t.direction[0] = p.req.direction[0]; t.direction[1] = p.req.direction[1];
+        hostStats[w].txResult = BP_REJECT_NONE;
+        const uint32_t kind = p.req.transaction;
+        if ((kind == BP_TX_IMPULSE || kind == BP_TX_REMOVE) && count[w] > 0) {
             ++local.transactions;
+            if (kind == BP_TX_IMPULSE && !(std::isfinite(p.req.impulse[0]) && std::isfinite(p.req.impulse[1]))) {
+                p.staged = true;                             // requireFinite(impulse): refused on the host
+                p.txResult = BP_REJECT_INVALID_VALUE;
+            } else {
+                bpd::TxRecord &t = L.hTx[L.txCount++];
+                t.world = w; t.kind = kind; t.index = p.req.index; t.buckets = buckets_for(w, headCount[w]);
+                t.impulse[0] = p.req.impulse[0]; t.impulse[1] = p.req.impulse[1];
+                t.direction[0] = p.req.direction[0]; t.direction[1] = p.req.direction[1];
+            }
         }
         count[w] = headCount[w];
         lastUse[w] = ++useClock;
@@ -414,7 +554,7 @@ bp_status bp_server::start_launch(int cls) {
     hipStream_t s = streams[si];
     bp_batch &b = *batch;
     if (L.txCount) {
-        hipError_t e = bp_launch_transactions(b.arenaDev, b.descs, L.dTx, L.txCount, s);
+        hipError_t e = bp_launch_transactions(b.arenaDev, b.descs, L.dTx, L.txCount, stats, s);
         if (e != hipSuccess) { set_error(std::string("transaction launch: ") + hipGetErrorString(e)); return BP_ERR_COMMAND_FAILED; }
     }
     hipError_t e = cls == kGlobalClass ? bp_launch_world_step(1024, b.arenaDev, b.descs, stats, L.dOrder, L.count, 1u, s)
@@ -473,6 +613,7 @@ bp_status bp_server::admit_waiting() {
     ++local.repacks;
     for (uint32_t w = 0; w < W; ++w) {
         count[w] = (uint32_t)b.worlds[w].bubbles.size();
+        dropPending[w] = 0;                          // the repack deleted the region-removed bubbles
         if (resident[w]) footprintOf[w] = b.footprint(w);
     }
     used = 0;

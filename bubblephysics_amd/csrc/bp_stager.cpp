@@ -138,6 +138,19 @@ bp_status assemble(const bp_stage_scene &s, const std::vector<int64_t> &removed,
 
 }  // namespace
 
+extern "C" bp_status bp_stage_add(const bp_stage_scene *scene, const bp_bubble *bubble, bp_bubble *out, uint32_t out_capacity,
+                                  uint32_t *out_count, bp_stage_rejection *why) {
+    if (!scene || !bubble || (scene->bubble_count && !scene->bubbles) || (scene->body_count && !scene->bodies))
+        return BP_ERR_INVALID_ARGUMENT;
+    if (find(*scene, bubble->id)) return reject(why, BP_REJECT_DUPLICATE_ID, BP_PLACEMENT_OK, bubble->id);   // :407
+    Scene sc{scene, {}};
+    for (uint32_t i = 0; i < scene->bubble_count; ++i) sc.bubbles.push_back(&scene->bubbles[i]);
+    const uint32_t issue = placement_issue(sc, P2{bubble->center[0], bubble->center[1]}, bubble->radius, bubble->category,
+                                           bubble->mask);                                                   // :415-420
+    if (issue != BP_PLACEMENT_OK) return reject(why, BP_REJECT_INVALID_PLACEMENT, issue, bubble->id);
+    return assemble(*scene, {}, {*bubble}, out, out_capacity, out_count, why);
+}
+
 extern "C" bp_status bp_stage_merge(const bp_stage_scene *scene, const int64_t *sources, uint32_t source_count,
                                     const bp_bubble *result, int32_t result_mass_given, bp_bubble *out,
                                     uint32_t out_capacity, uint32_t *out_count, bp_stage_rejection *why) {

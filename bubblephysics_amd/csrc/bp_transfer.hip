@@ -67,21 +67,6 @@ extern "C" __attribute__((visibility("hidden"))) hipError_t bp_launch_scatter_wo
 // ---------------------------------------------------------------------------------------
 namespace bpd {
 
-// Copies bubble slot `src` to slot `dst` of a world (published side and properties).
-__device__ void copy_slot(const Arena *__restrict__ A, uint32_t p, uint32_t src, uint32_t dst) {
-    A->pos[p][dst] = A->pos[p][src];
-    A->start[p][dst] = A->start[p][src];
-    A->vel[p][dst] = A->vel[p][src];
-    A->rot[p][dst] = A->rot[p][src];
-    A->omega[p][dst] = A->omega[p][src];
-    A->region[p][dst] = A->region[p][src];
-    A->id[dst] = A->id[src];
-    A->mass[dst] = A->mass[src]; A->invm[dst] = A->invm[src]; A->radius[dst] = A->radius[src];
-    A->Kc[dst] = A->Kc[src]; A->gamma[dst] = A->gamma[src]; A->flags[dst] = A->flags[src];
-    A->cat[dst] = A->cat[src]; A->mask[dst] = A->mask[src];
-    A->motor[dst] = A->motor[src]; A->band[dst] = A->band[src];
-}
-
 // Moves elements [from + 1, n) of one array to [from, n - 1), a workgroup-wide chunk at a time:
 // a chunk is read whole before it is written, and no chunk writes what a later chunk reads.
 template <class T> __device__ void shift_down(T *base, uint32_t from, uint32_t n) {
@@ -106,104 +91,107 @@ __device__ void remove_slot(const Arena *__restrict__ A, uint32_t p, uint32_t o,
     shift_down(A->mask, f, e); shift_down(A->motor, f, e); shift_down(A->band, f, e);
 }
 
-// One workgroup per transaction (each names a different world). The arithmetic is FP32, one
-// rounding per operation, in the order of the host-side statement of the same transactions:
-//   impulse: v_i = float(double(v_i) + double(J) / double(m_i)) for a movable bubble (the
-//            reference's FP64 form, TransactionStager.swift:312-324);
-//   add:     a copy of bubble i with a new id (largest + 1) at c_i + (2 r_i) * direction;
-//   remove:  bubble i leaves;
-//   merge:   bubbles a < b (i and its next index, or previous for the last) become one bubble in
-//            slot a: mass m_a + m_b, radius sqrt(r_a^2 + r_b^2), centre and velocity the
-//            mass-weighted means;
-//   split:   bubble i becomes two halves (mass / 2, radius / sqrt 2) at c_i +- r_half * direction;
-//            the first keeps the id and slot, the second takes a new id (largest + 1).
-// The index is `index % n`. Structural changes keep ids sorted, so no reordering is needed.
-// Like a host write of the world's bubbles, a transaction clears the world's failed mark.
-__global__ void transaction_kernel(const Arena *__restrict__ A, WorldDesc *descs, const TxRecord *txs) {
+// One workgroup per transaction (each names a different world). Only the transactions that
+// need no host staging run here; add, merge and split are staged on the host with the reference's
+// placement rules (bp_stage_add, bp_stage_merge, bp_stage_split) and written back before the launch.
+//   impulse: applyImpulse (TransactionStager.swift:312-324): only a .dynamic or .motorDriven bubble
+//            (force-driven without a band drive) takes it, else incompatibleMotionRole;
+//            v = float(double(v) + double(J) / double(m)), refused (invalidValue) when not finite;
+//   remove:  bubble i leaves (not the last bubble of a world).
+// The index is `index % n`. A world that failed a step takes no transaction: its failure is final.
+// The outcome (BP_REJECT_*, 0 = applied) goes to the world's result record.
+__global__ void transaction_kernel(const Arena *__restrict__ A, WorldDesc *descs, const TxRecord *txs, WorldStatus *stats) {
     const TxRecord tx = txs[blockIdx.x];
     WorldDesc *d = descs + tx.world;
     const uint32_t n = d->bubbleCount, o = d->bubbleOffset, p = d->pub;
-    if (n == 0 || tx.kind == BP_TX_NONE) return;
-    // The host refuses growth past the world's slots before it launches; this keeps the kernel
-    // inside the range whatever it is given.
-    if ((tx.kind == BP_TX_ADD || tx.kind == BP_TX_SPLIT) && n + 1 > d->bubbleCapacity) return;
+    if (n == 0 || tx.kind == BP_TX_NONE || d->failed == kWorldFailed) return;
     const uint32_t i = tx.index % n;
-    uint32_t newCount = n;
+    uint32_t newCount = n, outcome = BP_REJECT_NONE;
     if (tx.kind == BP_TX_IMPULSE) {
         if (threadIdx.x == 0) {
             const uint32_t g = o + i;
-            const float invm = A->invm[g];
-            if (invm > 0.0f) {
+            const uint32_t fl = A->flags[g];
+            const bool movable = (fl & BP_BUBBLE_FORCE_DRIVEN) != 0u && (fl & BP_BUBBLE_STATIC) == 0u && A->band[g] < 0;
+            if (!movable) {
+                outcome = BP_REJECT_INCOMPATIBLE_MOTION_ROLE;
+            } else {
                 const double m = (double)A->mass[g];
                 const float2 v = A->vel[p][g];
-                A->vel[p][g] = make_float2((float)((double)v.x + (double)tx.impulse[0] / m),
-                                           (float)((double)v.y + (double)tx.impulse[1] / m));
+                const float2 nv = make_float2((float)((double)v.x + (double)tx.impulse[0] / m),
+                                              (float)((double)v.y + (double)tx.impulse[1] / m));
+                if (isfinite(nv.x) && isfinite(nv.y)) A->vel[p][g] = nv;
+                else outcome = BP_REJECT_INVALID_VALUE;
             }
         }
-    } else if (tx.kind == BP_TX_ADD) {
-        if (threadIdx.x == 0) {
-            const uint32_t g = o + i, t = o + n;
-            copy_slot(A, p, g, t);
-            A->id[t] = A->id[o + n - 1] + 1;
-            const float offset = 2.0f * A->radius[g];
-            const float2 c = A->pos[p][g];
-            const float2 nc = make_float2(c.x + offset * tx.direction[0], c.y + offset * tx.direction[1]);
-            A->pos[p][t] = nc; A->start[p][t] = nc; A->region[p][t] = 0u;
-            A->motor[t] = -1; A->band[t] = -1;      // the new id has no motor or band record
-        }
-        newCount = n + 1;
     } else if (tx.kind == BP_TX_REMOVE) {
         if (n > 1) { remove_slot(A, p, o, i, n); newCount = n - 1; }
-    } else if (tx.kind == BP_TX_MERGE) {
-        if (n > 1) {
-            const uint32_t k = i + 1 < n ? i + 1 : i - 1;
-            const uint32_t a = min(i, k), b = max(i, k), ga = o + a, gb = o + b;
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                const float ma = A->mass[ga], mb = A->mass[gb];
-                const float m = ma + mb;
-                const float ra = A->radius[ga], rb = A->radius[gb];
-                A->radius[ga] = __builtin_sqrtf(ra * ra + rb * rb);
-                A->mass[ga] = m;
-                if (m > 0.0f) {                  // a massless pair keeps a's centre and velocity
-                    const float2 ca = A->pos[p][ga], cb = A->pos[p][gb], va = A->vel[p][ga], vb = A->vel[p][gb];
-                    const float2 c = make_float2((ca.x * ma + cb.x * mb) / m, (ca.y * ma + cb.y * mb) / m);
-                    A->invm[ga] = 1.0f / m;
-                    A->pos[p][ga] = c;
-                    A->vel[p][ga] = make_float2((va.x * ma + vb.x * mb) / m, (va.y * ma + vb.y * mb) / m);
-                } else {
-                    A->invm[ga] = 0.0f;
-                }
-                A->start[p][ga] = A->pos[p][ga]; A->region[p][ga] = 0u;
-            }
-            __syncthreads();
-            remove_slot(A, p, o, b, n);
-            newCount = n - 1;
-        }
-    } else if (tx.kind == BP_TX_SPLIT) {
-        if (threadIdx.x == 0) {
-            const uint32_t g = o + i, t = o + n;
-            const float r = A->radius[g] * 0.70710678f;
-            const float m = A->mass[g] * 0.5f;
-            const float invm = m > 0.0f ? 1.0f / m : 0.0f;
-            const float2 c = A->pos[p][g];
-            const float dx = tx.direction[0] * r, dy = tx.direction[1] * r;
-            copy_slot(A, p, g, t);
-            const float2 c1 = make_float2(c.x + dx, c.y + dy), c2 = make_float2(c.x - dx, c.y - dy);
-            A->radius[g] = r; A->mass[g] = m; A->invm[g] = invm;
-            A->pos[p][g] = c1; A->start[p][g] = c1; A->region[p][g] = 0u;
-            A->radius[t] = r; A->mass[t] = m; A->invm[t] = invm;
-            A->pos[p][t] = c2; A->start[p][t] = c2; A->region[p][t] = 0u;
-            A->id[t] = A->id[o + n - 1] + 1;
-            A->motor[t] = -1; A->band[t] = -1;
-        }
-        newCount = n + 1;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         d->bubbleCount = newCount;
         d->buckets = tx.buckets;
-        d->failed = 0;              // new input: a world that failed a step is stepped again
+        stats[tx.world].txResult = outcome;
+    }
+}
+
+// Deletes the bubbles that a held world's last step removed from the region (region result
+// BP_REGION_REMOVED on the published side), keeping the order of the others, and makes the world
+// runnable again: the public world deletes such objects before its next frame
+// (NativePublicWorld.swift:264-267). One workgroup per world. The range is compacted in place a
+// chunk of blockDim bubbles at a time: a chunk is read whole before it is written, and its
+// bubbles only move down, below the next chunk.
+__global__ void drop_removed_kernel(const Arena *__restrict__ A, WorldDesc *descs, const DropRecord *recs,
+                                    int64_t *removedIds) {
+    __shared__ uint32_t scanKeep[256], scanDrop[256];
+    const DropRecord r = recs[blockIdx.x];
+    WorldDesc *d = descs + r.world;
+    if (d->failed != kWorldHeld) return;
+    const uint32_t n = d->bubbleCount, o = d->bubbleOffset, p = d->pub, t = threadIdx.x;
+    uint32_t kept = 0, dropped = 0;
+    for (uint32_t c = 0; c < n; c += blockDim.x) {
+        const uint32_t i = c + t;
+        const bool ok = i < n;
+        const uint32_t g = o + i;
+        const bool keep = ok && A->region[p][g] != BP_REGION_REMOVED;
+        float2 pos{}, start{}, vel{};
+        float rot = 0.0f, om = 0.0f, mass = 0.0f, invm = 0.0f, radius = 0.0f, Kc = 0.0f, gamma = 0.0f;
+        uint32_t region = 0, flags = 0, cat = 0, mask = 0;
+        int32_t motor = -1, band = -1;
+        int64_t id = 0;
+        if (ok) {
+            pos = A->pos[p][g]; start = A->start[p][g]; vel = A->vel[p][g]; rot = A->rot[p][g]; om = A->omega[p][g];
+            region = A->region[p][g]; id = A->id[g]; mass = A->mass[g]; invm = A->invm[g]; radius = A->radius[g];
+            Kc = A->Kc[g]; gamma = A->gamma[g]; flags = A->flags[g]; cat = A->cat[g]; mask = A->mask[g];
+            motor = A->motor[g]; band = A->band[g];
+        }
+        // Inclusive scans of the keep and drop marks over the chunk (blockDim <= 256).
+        scanKeep[t] = keep ? 1u : 0u;
+        scanDrop[t] = (ok && !keep) ? 1u : 0u;
+        __syncthreads();
+        for (uint32_t step = 1; step < blockDim.x; step <<= 1) {
+            const uint32_t ak = t >= step ? scanKeep[t - step] : 0u, ad = t >= step ? scanDrop[t - step] : 0u;
+            __syncthreads();
+            scanKeep[t] += ak; scanDrop[t] += ad;
+            __syncthreads();
+        }
+        if (keep) {
+            const uint32_t q = o + kept + scanKeep[t] - 1;
+            A->pos[p][q] = pos; A->start[p][q] = start; A->vel[p][q] = vel; A->rot[p][q] = rot; A->omega[p][q] = om;
+            A->region[p][q] = region; A->id[q] = id; A->mass[q] = mass; A->invm[q] = invm; A->radius[q] = radius;
+            A->Kc[q] = Kc; A->gamma[q] = gamma; A->flags[q] = flags; A->cat[q] = cat; A->mask[q] = mask;
+            A->motor[q] = motor; A->band[q] = band;
+        } else if (ok && removedIds) {
+            removedIds[r.removedOffset + dropped + scanDrop[t] - 1] = id;
+        }
+        const uint32_t chunkKept = scanKeep[blockDim.x - 1], chunkDropped = scanDrop[blockDim.x - 1];
+        __syncthreads();
+        kept += chunkKept;
+        dropped += chunkDropped;
+    }
+    if (t == 0) {
+        d->bubbleCount = kept;
+        d->buckets = r.buckets;
+        d->failed = kWorldRunnable;
     }
 }
 
@@ -231,9 +219,17 @@ __global__ void gather_full_world_kernel(const Arena *__restrict__ A, const Worl
 
 extern "C" __attribute__((visibility("hidden"))) hipError_t bp_launch_transactions(const bpd::Arena *arena, bpd::WorldDesc *descs,
                                                                                const bpd::TxRecord *txs, uint32_t count,
-                                                                               hipStream_t stream) {
+                                                                               bpd::WorldStatus *stats, hipStream_t stream) {
     if (count == 0) return hipSuccess;
-    hipLaunchKernelGGL(bpd::transaction_kernel, dim3(count), dim3(256), 0, stream, arena, descs, txs);
+    hipLaunchKernelGGL(bpd::transaction_kernel, dim3(count), dim3(256), 0, stream, arena, descs, txs, stats);
+    return hipGetLastError();
+}
+
+extern "C" __attribute__((visibility("hidden"))) hipError_t bp_launch_drop_removed(const bpd::Arena *arena, bpd::WorldDesc *descs,
+                                                                               const bpd::DropRecord *recs, uint32_t count,
+                                                                               int64_t *removedIds, hipStream_t stream) {
+    if (count == 0) return hipSuccess;
+    hipLaunchKernelGGL(bpd::drop_removed_kernel, dim3(count), dim3(256), 0, stream, arena, descs, recs, removedIds);
     return hipGetLastError();
 }
 

@@ -746,8 +746,12 @@ __global__ __launch_bounds__(BLOCK) void world_coop_kernel(const Arena *__restri
         st->groupWidth = BLOCK;
         st->executed = 1;
         st->ticks = (uint32_t)(wall_clock64() - ticks0);
-        if (status == BP_OK) d->pub = wside;
-        else if (status != BP_ERR_CAPACITY_OVERFLOW) d->failed = 1;
+        if (status == BP_OK) {
+            d->pub = wside;
+            if (s.counters[kCoRemovals] != 0) d->failed = kWorldHeld;   // removed bubbles leave before the next step
+        } else if (status != BP_ERR_CAPACITY_OVERFLOW) {
+            d->failed = kWorldFailed;
+        }
     }
 }
 

@@ -949,11 +949,17 @@ __global__ __launch_bounds__(BLOCK) void world_lds_kernel(const Arena *__restric
             st->groupWidth = BLOCK;
             st->executed = 1;
             st->ticks = (uint32_t)(wall_clock64() - ticks0);
-            if (status == BP_OK) d->pub = ws;
-            else if (status != BP_ERR_CAPACITY_OVERFLOW) d->failed = 1;
+            if (status == BP_OK) {
+                d->pub = ws;
+                if (S.red.counters[6] != 0) d->failed = kWorldHeld;
+            } else if (status != BP_ERR_CAPACITY_OVERFLOW) {
+                d->failed = kWorldFailed;
+            }
             post_status(st, notify);
         }
-        if (status != BP_OK) break;                           // a failed world stops; later steps stay unexecuted
+        // A failed world stops, and so does one whose step removed bubbles from the region: the
+        // removed bubbles leave the world before its next step. Later steps stay unexecuted.
+        if (status != BP_OK || S.red.counters[6] != 0) break;
         pubNow = ws;
         __syncthreads();                                      // LDS is reused by the next step
     }

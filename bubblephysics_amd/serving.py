@@ -7,13 +7,13 @@ launch completes. Transactions run on the device before the step they come with.
 admitted against a device-memory budget; the least recently used idle worlds are evicted to
 host memory when an admission needs room.
 
-This module builds request traces (a seeded Poisson process), states the transactions in numpy
-(``apply_transaction``, the host-side statement the device kernel is tested against) and wraps
-the server. The reference has no serving form: its game loop steps one world per call
+This module builds request traces (a seeded Poisson process), restates how a served transaction
+changes a world (``apply_transaction``: the same rules, for replaying a world's requests alone) and
+wraps the server. The reference has no serving form: its game loop steps one world per call
 (BubbleWorld.submit/advance, Sources/BubblePhysicsPublic/BubbleWorld.swift:252-283). The
-impulse is the reference's (TransactionStager.swift:312-324, in FP64); the structural transactions
-are simplified forms of its staging (StructuralOperations.swift:10-151 for merge and split, whose
-full rules are bp_stage_merge and bp_stage_split): mass, momentum and area are kept.
+transactions follow its staging: the impulse with its role and finiteness checks
+(TransactionStager.swift:312-324, in FP64), add with the placement rules (:406-421,
+PlacementGeometry.swift:42-94), merge and split with StructuralOperations.swift:10-151.
 """
 from __future__ import annotations
 
@@ -28,7 +28,6 @@ from . import abi, scenes, solver
 KINDS = ("impulse", "add", "remove", "merge", "split")
 TX_CODE = {None: abi.BP_TX_NONE, "impulse": abi.BP_TX_IMPULSE, "add": abi.BP_TX_ADD, "remove": abi.BP_TX_REMOVE,
            "merge": abi.BP_TX_MERGE, "split": abi.BP_TX_SPLIT}
-HALF_RADIUS = np.float32(0.70710678)      # 1/sqrt(2) rounded to FP32: radius of each half of a split
 
 
 @dataclass
@@ -44,6 +43,7 @@ class Request:
     status: int = -1
     substeps: int = 0
     bubbles: int = 0
+    tx_result: int = 0               # BP_REJECT_* of its transaction (0 = applied or none)
 
 
 def transaction_params(rng, kind: str | None) -> dict:
@@ -70,67 +70,61 @@ def poisson_requests(worlds: int, rate: float, duration: float, seed: int, trans
     return out
 
 
-def apply_transaction(bubbles: np.ndarray, req: Request) -> np.ndarray:
-    """The world's bubbles after one transaction: the host-side statement of the device kernel
-    (transaction_kernel in csrc/bp_transfer.hip), FP32 with one rounding per operation."""
+def apply_transaction(bubbles: np.ndarray, req: Request, *, bodies=None, vertices=None, region=None, bands=None):
+    """(bubbles, reason) after one served transaction under the reference's rules, as the server
+    applies them: the impulse and the removal as the device kernel states them, add, merge and
+    split through the library's host staging (bp_stage_add, bp_stage_merge, bp_stage_split) with
+    the world's bodies at their current poses. `reason` is BP_REJECT_* (0 = applied); a refused
+    transaction returns the bubbles unchanged."""
     b = bubbles.copy()
     n = len(b)
     if n == 0 or req.transaction is None:
-        return b
+        return b, abi.BP_REJECT_NONE
     f32 = np.float32
     i = int(req.index) % n
     kind = req.transaction
-    if kind == "impulse":                       # v = Float(Double(v) + Double(J) / Double(m)), TransactionStager.swift:312-324
-        if b["inverse_mass"][i] > 0:
-            m = np.float64(b["mass"][i])
-            b["velocity"][i] = (b["velocity"][i].astype(np.float64) + np.array(req.impulse, f32).astype(np.float64) / m).astype(f32)
-        return b
+    if kind == "impulse":                       # TransactionStager.swift:312-324
+        J = np.array(req.impulse, f32)
+        if not np.all(np.isfinite(J)):
+            return b, abi.BP_REJECT_INVALID_VALUE
+        banded = bands is not None and int(b["id"][i]) in set(int(x) for x in bands["bubble_id"])
+        if not (b["flags"][i] & abi.BP_BUBBLE_FORCE_DRIVEN) or (b["flags"][i] & abi.BP_BUBBLE_STATIC) or banded:
+            return b, abi.BP_REJECT_INCOMPATIBLE_MOTION_ROLE
+        m = np.float64(b["mass"][i])
+        with np.errstate(over="ignore", invalid="ignore"):
+            v = (b["velocity"][i].astype(np.float64) + J.astype(np.float64) / m).astype(f32)
+        if not np.all(np.isfinite(v)):
+            return b, abi.BP_REJECT_INVALID_VALUE
+        b["velocity"][i] = v
+        return b, abi.BP_REJECT_NONE
     if kind == "remove":
-        return np.delete(b, i) if n > 1 else b
-    direction = np.array(req.direction, f32)
-    if kind == "add":
-        new = b[i:i + 1].copy()
-        new["id"] = b["id"][n - 1] + 1
-        offset = f32(2.0) * b["radius"][i]
-        new["center"][0] = b["center"][i] + offset * direction
-        new["previous_center"] = new["center"]
-        return np.concatenate([b, new])
-    if kind == "merge":                                              # index neighbours a < c become one in slot a
-        if n < 2:
-            return b
-        k = i + 1 if i + 1 < n else i - 1
-        a, c = min(i, k), max(i, k)
-        ma, mc = b["mass"][a], b["mass"][c]
-        m = f32(ma + mc)
-        ra, rc = b["radius"][a], b["radius"][c]
-        out = b.copy()
-        out["radius"][a] = np.sqrt(f32(ra * ra + rc * rc))
-        out["mass"][a] = m
-        if m > 0:                                                    # a massless pair keeps a's centre and velocity
-            out["inverse_mass"][a] = f32(1.0) / m
-            out["center"][a] = (b["center"][a] * ma + b["center"][c] * mc) / m
-            out["velocity"][a] = (b["velocity"][a] * ma + b["velocity"][c] * mc) / m
-        else:
-            out["inverse_mass"][a] = f32(0.0)
-        out["previous_center"][a] = out["center"][a]
-        return np.delete(out, c)
-    if kind == "split":                                              # halves at c +- r_half * direction
-        r = b["radius"][i] * HALF_RADIUS
-        m = b["mass"][i] * f32(0.5)
-        invm = f32(1.0) / m if m > 0 else f32(0.0)
-        d = direction * r
-        centre = b["center"][i].copy()
-        two = b[i:i + 1].copy()
-        for rec in (b[i:i + 1], two):
-            rec["radius"] = r
-            rec["mass"] = m
-            rec["inverse_mass"] = invm
-        b["center"][i] = centre + d
-        b["previous_center"][i] = b["center"][i]
-        two["center"][0] = centre - d
-        two["previous_center"] = two["center"]
-        two["id"] = b["id"][n - 1] + 1
-        return np.concatenate([b, two])
+        return (np.delete(b, i) if n > 1 else b), abi.BP_REJECT_NONE
+    kw = dict(bodies=bodies, vertices=vertices, region=tuple(region) if region is not None else (-1e30, -1e30, 1e30, 1e30))
+    new_id = int(b["id"][n - 1]) + 1
+    try:
+        if kind == "add":
+            rec = b[i].copy()
+            rec["id"] = new_id
+            offset = f32(2.0) * b["radius"][i]
+            direction = np.array(req.direction, f32)
+            rec["center"] = b["center"][i] + offset * direction
+            rec["previous_center"] = rec["center"]
+            rec["velocity"] = 0.0
+            rec["rotation"] = 0.0
+            rec["angular_velocity"] = 0.0
+            return solver.stage_add(b, rec, **kw), abi.BP_REJECT_NONE
+        if kind == "merge":
+            if n < 2:
+                return b, abi.BP_REJECT_INVALID_STRUCTURAL_OPERATION
+            k = i + 1 if i + 1 < n else i - 1
+            a, c = min(i, k), max(i, k)
+            return solver.stage_merge(b, [b["id"][a], b["id"][c]], b[a], **kw), abi.BP_REJECT_NONE
+        if kind == "split":
+            second = b[i].copy()
+            second["id"] = new_id
+            return solver.stage_split(b, b["id"][i], b[i], 0.5, second, 0.5, **kw), abi.BP_REJECT_NONE
+    except solver.StageRejected as e:
+        return b, e.reason
     raise ValueError(f"unknown transaction {kind!r}")
 
 
@@ -213,6 +207,7 @@ class Server:
         for r, res in zip(requests, results):
             r.done, r.status, r.substeps, r.bubbles = float(res["completed"]), int(res["status"]), int(res["substeps"]), \
                 int(res["bubbles"])
+            r.tx_result = int(res["transaction_result"])
         elapsed = float(results["completed"].max() - base) if len(results) else 0.0
         return ServeStats(results, elapsed, self.stats())
 

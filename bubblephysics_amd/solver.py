@@ -106,10 +106,28 @@ class NativeWorld:
         abi.check(self._lib.bp_world_set_bubbles(self._handle, abi.ptr(b), len(b)), "bp_world_set_bubbles")
         self._count = len(b)
 
-    def update_bubbles(self, indices, bubbles: np.ndarray):
+    @property
+    def publication(self) -> int:
+        """Number of the last published snapshot (the base a delta upload must name)."""
+        return int(self._lib.bp_world_publication(self._handle))
+
+    def update_bubbles(self, indices, bubbles: np.ndarray, base_publication: int | None = None):
+        """Delta upload against the snapshot `base_publication` (default: the last published one)."""
         idx = np.ascontiguousarray(indices, np.uint32)
         b = np.ascontiguousarray(bubbles, abi.BUBBLE_DTYPE)
-        abi.check(self._lib.bp_world_update_bubbles(self._handle, abi.ptr(idx), abi.ptr(b), len(b)), "bp_world_update_bubbles")
+        base = self.publication if base_publication is None else int(base_publication)
+        abi.check(self._lib.bp_world_update_bubbles(self._handle, base, abi.ptr(idx), abi.ptr(b), len(b)),
+                  "bp_world_update_bubbles")
+
+    def region_results(self):
+        """(removed ids, wrapped ids) of the last published step, ascending."""
+        return _region_results(lambda *a: self._lib.bp_world_region_results(self._handle, *a), "bp_world_region_results")
+
+    def remove_bubbles(self, ids):
+        """Deletes bubbles by id, as the public world deletes the ones the region removed."""
+        arr = np.ascontiguousarray(ids, np.int64)
+        abi.check(self._lib.bp_world_remove_bubbles(self._handle, abi.ptr(arr), len(arr)), "bp_world_remove_bubbles")
+        self._count -= len(arr)
 
     def set_segments(self, segments: np.ndarray):
         s = np.ascontiguousarray(segments, abi.SEGMENT_DTYPE)
@@ -144,12 +162,16 @@ class NativeWorld:
 
     def _frame(self, info: abi.FrameInfo) -> SolverFrame:
         bubbles = self.read_bubbles()
+        removed, wrapped = (set(), set())
+        if info.status == abi.BP_OK and (info.region_removals or info.region_wraps):
+            r, w = self.region_results()
+            removed, wrapped = set(int(x) for x in r), set(int(x) for x in w)
         return SolverFrame(
             status=int(info.status), bubbles=bubbles, substep_count=int(info.substep_count),
             grid_rebuild_count=int(info.grid_rebuild_count), candidate_pair_count=int(info.candidate_pair_count),
             contact_candidate_overflow_count=int(info.candidate_overflow),
             maximum_contact_candidates_per_object=int(info.max_candidates_per_object), group_width=int(info.group_width),
-            gpu_milliseconds=float(info.gpu_milliseconds), region_removals=set(), region_wraps=set(),
+            gpu_milliseconds=float(info.gpu_milliseconds), region_removals=removed, region_wraps=wrapped,
             limiter_object=int(info.limiter_object), limiter_term=int(info.limiter_term), detail=_detail(info),
             execution_path=int(info.execution_path))
 
@@ -194,6 +216,16 @@ class NativeWorld:
 
     def fail_next_command(self):
         abi.check(self._lib.bp_world_debug_fail_next_command(self._handle), "bp_world_debug_fail_next_command")
+
+
+def _region_results(call, name: str):
+    """Runs a region-results call: counts first, then the ids."""
+    nr, nw = ctypes.c_uint32(0), ctypes.c_uint32(0)
+    abi.check(call(None, 0, ctypes.byref(nr), None, 0, ctypes.byref(nw)), name)
+    removed = np.zeros(max(nr.value, 1), np.int64)
+    wrapped = np.zeros(max(nw.value, 1), np.int64)
+    abi.check(call(abi.ptr(removed), len(removed), ctypes.byref(nr), abi.ptr(wrapped), len(wrapped), ctypes.byref(nw)), name)
+    return removed[: nr.value], wrapped[: nw.value]
 
 
 def _read_contacts(call, name: str, capacity: int | None) -> np.ndarray:
@@ -262,6 +294,14 @@ def stage_merge(bubbles, sources, result, *, result_mass=None, bodies=None, vert
                   abi.ptr(res), 1 if result_mass is not None else 0)
 
 
+def stage_add(bubbles, bubble, *, bodies=None, vertices=None, region=(-1e30, -1e30, 1e30, 1e30)):
+    """World bubbles after adding `bubble` (a BUBBLE_DTYPE record) -- bp_stage_add,
+    TransactionStager.add with the placement rules. Raises StageRejected."""
+    rec = np.zeros(1, abi.BUBBLE_DTYPE)
+    rec[0] = bubble
+    return _stage("bp_stage_add", "bp_stage_add", bubbles, bodies, vertices, region, 1, abi.ptr(rec))
+
+
 def stage_split(bubbles, source, first, first_share, second, second_share, *, bodies=None, vertices=None,
                 region=(-1e30, -1e30, 1e30, 1e30)):
     """World bubbles after splitting `source` into `first` and `second` with area shares --
@@ -295,6 +335,17 @@ class Batch:
         infos = np.zeros(steps * len(self._counts), abi.FRAME_DTYPE)
         abi.check(self._lib.bp_batch_step(self._handle, steps, abi.ptr(infos)), "bp_batch_step")
         return infos.reshape(steps, len(self._counts))
+
+    def region_results(self, world: int):
+        """(ids removed during the last stepping call, ids wrapped in the last published step)."""
+        return _region_results(lambda *a: self._lib.bp_batch_region_results(self._handle, world, *a),
+                               "bp_batch_region_results")
+
+    def read_bodies(self, world: int, count: int) -> np.ndarray:
+        """Bodies of one world at their published poses."""
+        out = np.zeros(count, abi.BODY_DTYPE)
+        abi.check(self._lib.bp_batch_read_bodies(self._handle, world, abi.ptr(out), count), "bp_batch_read_bodies")
+        return out
 
     def read_bubbles(self, world: int) -> np.ndarray:
         self._counts[world] = int(self._lib.bp_batch_bubble_count(self._handle, world))

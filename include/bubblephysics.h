@@ -45,7 +45,10 @@ enum {
     BP_ERR_GUARD_FAILURE = 10,         /* NativeMetalWorldError.guardFailure                  */
     BP_ERR_CONTACT_VIOLATION = 11,     /* NativeMetalWorldError.contactViolation              */
     BP_ERR_SUBSTEP_LIMIT_EXCEEDED = 12,/* NativeMetalWorldError.substepLimitExceeded          */
-    BP_ERR_UNAVAILABLE = 13            /* NativeMetalWorldError.unavailable                   */
+    BP_ERR_UNAVAILABLE = 13,           /* NativeMetalWorldError.unavailable                   */
+    BP_ERR_INVALID_FRAME = 14          /* NativeMetalWorldError.invalidFrame: the step's read-back
+                                          record is inconsistent (claims success for an unfinished
+                                          step or more than 1024 substeps); nothing is published  */
 };
 
 /* Term that fixed h in the last substep: NativeMetalSubstep.Term
@@ -270,8 +273,15 @@ void bp_world_destroy(bp_world *world);
 bp_status bp_world_set_config(bp_world *world, const bp_world_config *config);
 /* Full snapshot of the bubbles (replaces the published state; ids sorted ascending). */
 bp_status bp_world_set_bubbles(bp_world *world, const bp_bubble *bubbles, uint32_t count);
-/* Delta upload: bubbleChanges of NativeMetalSnapshot (NativeMetalBufferLayout.swift:209-289). */
-bp_status bp_world_update_bubbles(bp_world *world, const uint32_t *indices, const bp_bubble *bubbles, uint32_t count);
+/* Number of the last published snapshot: it changes with every full upload, every deletion by
+ * bp_world_remove_bubbles and every successful step (a failed step publishes nothing). */
+uint64_t bp_world_publication(const bp_world *world);
+/* Delta upload: bubbleChanges of NativeMetalSnapshot (NativeMetalBufferLayout.swift:209-289),
+ * taken against the snapshot `base_publication` names. A delta against any other snapshot than the
+ * last published one returns BP_ERR_INVALID_SNAPSHOT and changes nothing (the previous snapshot
+ * must be the last published one, NativeMetalSolver.swift:404-419). */
+bp_status bp_world_update_bubbles(bp_world *world, uint64_t base_publication, const uint32_t *indices,
+                                  const bp_bubble *bubbles, uint32_t count);
 bp_status bp_world_set_segments(bp_world *world, const bp_segment *segments, uint32_t count);
 bp_status bp_world_set_bodies(bp_world *world, const bp_body *bodies, uint32_t count,
                               const float *vertices, uint32_t vertex_count,
@@ -285,6 +295,18 @@ bp_status bp_world_set_fields(bp_world *world, const bp_field *fields, uint32_t 
 /* One step of dt (executeFrame). On any error the published state is unchanged. */
 bp_status bp_world_step(bp_world *world, int32_t scenario_step, bp_frame_info *out_info);
 bp_status bp_world_read_bubbles(bp_world *world, bp_bubble *out, uint32_t count);
+/* Region results of the last published step: NativeSolverFrame.regionRemovals and .regionWraps
+ * (NativeSolverInput.swift:243-244; nativeApplyPublicRegion, NativePostSolveKernels.metal-source:536-566),
+ * bubble ids ascending. As in the reference's frame, removed bubbles are still in the published
+ * state; the public world deletes them before its next frame (NativePublicWorld.swift:264-267),
+ * here with bp_world_remove_bubbles. *out_removed / *out_wrapped receive the counts; an array
+ * smaller than its count returns BP_ERR_CAPACITY_OVERFLOW (null arrays ask for counts only). */
+bp_status bp_world_region_results(bp_world *world, int64_t *removed, uint32_t removed_capacity, uint32_t *out_removed,
+                                  int64_t *wrapped, uint32_t wrapped_capacity, uint32_t *out_wrapped);
+/* Deletes bubbles by id from the published state, with their motor and band records: the next
+ * snapshot of a public world that dropped them (NativePublicWorld.swift:264-267). An id the world
+ * does not hold returns BP_ERR_INVALID_SNAPSHOT and changes nothing. */
+bp_status bp_world_remove_bubbles(bp_world *world, const int64_t *ids, uint32_t count);
 bp_status bp_world_read_bodies(bp_world *world, bp_body *out, uint32_t count);
 uint32_t bp_world_bubble_count(const bp_world *world);
 /* Adjacency of the last grid build (test seam, like NativeMetalSolver.candidateKeys):
@@ -380,7 +402,10 @@ enum {
     BP_REJECT_INVALID_STRUCTURAL_OPERATION = 1,
     BP_REJECT_INVALID_PLACEMENT = 2,       /* issue and object_id say where and for which result */
     BP_REJECT_DUPLICATE_ID = 3,
-    BP_REJECT_UNKNOWN_OBJECT = 4
+    BP_REJECT_UNKNOWN_OBJECT = 4,
+    BP_REJECT_INCOMPATIBLE_MOTION_ROLE = 5, /* incompatibleMotionRole: an impulse on a bubble that is
+                                               neither .dynamic nor .motorDriven                */
+    BP_REJECT_INVALID_VALUE = 6            /* invalidValue: a non-finite impulse or velocity after it */
 };
 typedef struct bp_stage_rejection {
     uint32_t reason;            /* BP_REJECT_*    */
@@ -402,6 +427,12 @@ typedef struct bp_stage_scene {
 bp_status bp_stage_merge(const bp_stage_scene *scene, const int64_t *sources, uint32_t source_count,
                          const bp_bubble *result, int32_t result_mass_given, bp_bubble *out,
                          uint32_t out_capacity, uint32_t *out_count, bp_stage_rejection *why);
+/* Add: TransactionStager.add (TransactionStager.swift:406-421): refused when the id is taken
+ * (BP_REJECT_DUPLICATE_ID) or when the bubble's centre fails the placement rules: outside the
+ * region, inside a rigid body it collides with, or on the centre of a bubble it collides with
+ * (BP_REJECT_INVALID_PLACEMENT with the issue; PlacementGeometry.swift:42-94). */
+bp_status bp_stage_add(const bp_stage_scene *scene, const bp_bubble *bubble, bp_bubble *out, uint32_t out_capacity,
+                       uint32_t *out_count, bp_stage_rejection *why);
 /* Split: `source` becomes `first` and `second` with area shares summing to 1 (within 1e-6); the
  * parts' masses and radii follow the shares. */
 bp_status bp_stage_split(const bp_stage_scene *scene, int64_t source, const bp_bubble *first, float first_share,
@@ -464,9 +495,18 @@ uint32_t bp_batch_world_count(const bp_batch *batch);
 /* Bubbles of one world (transactions applied by a bp_server may have changed it). */
 uint32_t bp_batch_bubble_count(const bp_batch *batch, uint32_t world);
 /* Advances every world by `steps` steps; infos[world_count * steps] (step-major) may be null.
- * A failed world stops at its failing step and keeps its last published state. */
+ * A failed world stops at its failing step and keeps its last published state. Bubbles a step
+ * removes from the region leave their world before its next step, as the public world deletes
+ * them (NativePublicWorld.swift:264-267). */
 bp_status bp_batch_step(bp_batch *batch, uint32_t steps, bp_frame_info *infos);
+/* Region results of a world of a batch: the ids removed during the last stepping call (already
+ * deleted from the world) and the ids wrapped in its last published step, ascending; arrays as
+ * in bp_world_region_results. */
+bp_status bp_batch_region_results(bp_batch *batch, uint32_t world, int64_t *removed, uint32_t removed_capacity,
+                                  uint32_t *out_removed, int64_t *wrapped, uint32_t wrapped_capacity, uint32_t *out_wrapped);
 bp_status bp_batch_read_bubbles(bp_batch *batch, uint32_t world, bp_bubble *out, uint32_t count);
+/* Bodies of one world at their published poses (count = the world's body count). */
+bp_status bp_batch_read_bodies(bp_batch *batch, uint32_t world, bp_body *out, uint32_t count);
 /* Captures `steps` steps of every resident world (the launches of every kernel class on the
  * batch's streams, forked from and joined back to one stream) into a hipGraph, replayed by
  * bp_batch_replay with one graph launch. A change of the batch's layout (new worlds, residency,
@@ -544,16 +584,25 @@ bp_status bp_batch_world_footprint(const bp_batch *batch, uint32_t world, uint64
 /* ------------------------------------------------------------------------ */
 typedef struct bp_server bp_server;
 
-/* Transaction carried by a request, applied on the device before the step. The bubble it
- * names is `index % bubble count`. Structural changes keep ids sorted (new ids are the largest
- * id + 1) and keep mass, momentum and area. */
+/* Transaction carried by a request, applied before the step with the reference's rules
+ * (TransactionStager). The bubble it names is `index % bubble count`; new ids are the largest id + 1.
+ * A refused transaction leaves the world unchanged, the step still runs, and the result's
+ * transaction_result gives the reason (BP_REJECT_*), as the public world reports a
+ * transactionRejected event and advances. */
 enum {
     BP_TX_NONE = 0,
-    BP_TX_IMPULSE = 1,  /* v_i = float(double(v_i) + double(J) / double(m_i)), J = impulse; movable bubbles only */
-    BP_TX_ADD = 2,      /* copy of bubble i at c_i + 2 r_i * direction, new id                        */
-    BP_TX_REMOVE = 3,   /* bubble i leaves (not the last bubble of a world)                            */
-    BP_TX_MERGE = 4,    /* bubble i and its index neighbour (i + 1, or i - 1 for the last) become one  */
-    BP_TX_SPLIT = 5     /* bubble i becomes two halves at c_i +- (r_i / sqrt 2) * direction            */
+    BP_TX_IMPULSE = 1,  /* applyImpulse (TransactionStager.swift:312-324), on the device:
+                           v = float(double(v) + double(J) / double(m)) for a .dynamic or .motorDriven
+                           bubble; other roles: BP_REJECT_INCOMPATIBLE_MOTION_ROLE; a non-finite impulse
+                           or result: BP_REJECT_INVALID_VALUE                                           */
+    BP_TX_ADD = 2,      /* add (:406-413) of a bubble with bubble i's properties at c_i + 2 r_i * direction,
+                           at rest, staged by bp_stage_add (placement rules)                           */
+    BP_TX_REMOVE = 3,   /* remove (:429-436) of bubble i (not the last bubble of a world), on the device  */
+    BP_TX_MERGE = 4,    /* mergeBubbles (:630-674) of bubble i and its index neighbour (i + 1, or i - 1
+                           for the last) into a bubble with the lower id and the first source's
+                           properties, staged by bp_stage_merge                                         */
+    BP_TX_SPLIT = 5     /* splitBubble (:675-735) of bubble i into halves (shares 1/2): the first keeps
+                           the id, the second takes a new one; staged by bp_stage_split                */
 };
 
 typedef struct bp_request {
@@ -572,7 +621,9 @@ typedef struct bp_server_result {
     uint32_t world;
     bp_status status;           /* status of the step; BP_ERR_CAPACITY_OVERFLOW when a transaction
                                    needs more bubble slots than the world was packed with (no step);
-                                   BP_ERR_UNAVAILABLE for a world that failed an earlier step      */
+                                   BP_ERR_UNAVAILABLE for a world that failed an earlier step: a
+                                   world failure is final (BubbleWorld.swift:296-299, 424-430), so
+                                   such a request runs neither its transaction nor a step          */
     uint32_t substeps;
     uint32_t bubbles;           /* bubbles of the world in the step                            */
     double arrival;             /* seconds on the server clock (bp_server_clock)               */
@@ -580,7 +631,7 @@ typedef struct bp_server_result {
     uint32_t launch_worlds;     /* worlds in the launch that carried the step                  */
     uint32_t kernel_class;
     float device_milliseconds;  /* time the world's workgroup spent on the step                */
-    uint32_t reserved;
+    uint32_t transaction_result;/* BP_REJECT_* of the request's transaction (0: applied or none) */
 } bp_server_result;
 
 typedef struct bp_server_config {

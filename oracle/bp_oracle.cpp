@@ -33,6 +33,14 @@
 
 namespace bpo {
 
+// Planted mistakes, switched on only by the tests that show the pins catch them (0 = none):
+//   1 round core: distance to the core axis without subtracting the core radius;
+//   2 polygon obstacle: the nearest vertex taken for the nearest point of the edge;
+//   3 region wrap: the period without the collider's diameter;
+//   4 edge-side assertion: the side after the move taken at the step's start point;
+//   5 body prediction: the angular velocity ignored.
+int g_plant = 0;
+
 // ---------------------------------------------------------------------------------------
 // Scalar helpers. Float uses the deterministic transcendentals shared with the kernels;
 // double uses libm (the FP64 run is a drift reference, not a bitwise one).
@@ -307,14 +315,14 @@ inline Obstacle<T> body_obstacle(const Body<T> &source, T fraction, const std::v
         T axisLength = length(axis);
         V2<T> fallback = axisLength > T(1e-6) ? v2(-axis.y, axis.x) / length(v2(-axis.y, axis.x)) : v2(T(0), T(1));
         normal = away > T(1e-7) ? delta / away : fallback;
-        distanceValue = away - core.radius;
+        distanceValue = g_plant == 1 ? away : away - core.radius;
         point = point + normal * core.radius;
     } else {
         T best = inf<T>();
         V2<T> nearest = v2(T(0), T(0)), outward = v2(T(0), T(1));
         for (uint32_t e = 0; e < body.vCnt; ++e) {
             V2<T> p = rigid_vertex(body, verts, e), q = rigid_vertex(body, verts, (e + 1) % body.vCnt);
-            V2<T> candidate = closest_point(center, p, q);
+            V2<T> candidate = g_plant == 2 ? p : closest_point(center, p, q);
             T d = distance(center, candidate);
             if (d < best) {
                 best = d; nearest = candidate;
@@ -388,6 +396,8 @@ template <class T> struct World {
     Assertion assertion;
     T cellWidth = 0;
     uint32_t limit = 128;
+    bool captureOn = false;                   // record the state after every substep (captureSubsteps)
+    std::vector<V2<T>> captured;              // per substep: (position, velocity) of every bubble
 
     bool is_static(const Bubble<T> &x) const { return !(x.invm > 0) || (x.enabled && x.isStatic); }   // Force:381
     T inverse_mass(const Bubble<T> &x) const { return is_static(x) ? T(0) : x.invm; }                   // :382
@@ -475,7 +485,7 @@ template <class T> struct World {
             if (body.maxAng > 0) body.motion[2] = vclamp(body.motion[2], -body.maxAng, body.maxAng);
             body.pose[0] = body.pose[0] + body.motion[0] * dt;
             body.pose[1] = body.pose[1] + body.motion[1] * dt;
-            body.pose[2] = body.pose[2] + body.motion[2] * dt;
+            if (g_plant != 5) body.pose[2] = body.pose[2] + body.motion[2] * dt;
             if (!std::isfinite(body.pose[0]) || !std::isfinite(body.pose[1]) || !std::isfinite(body.pose[2]) ||
                 !std::isfinite(body.pose[3]) || !std::isfinite(body.motion[0]) || !std::isfinite(body.motion[1]) ||
                 !std::isfinite(body.motion[2]))
@@ -954,7 +964,7 @@ template <class T> struct World {
             T lengthSquared = dot(edge, edge);
             if (!(lengthSquared > 0)) continue;
             V2<T> outward = v2(-edge.y, edge.x) / std::sqrt(lengthSquared) * seg.outward;
-            T before = dot(start - seg.a, outward), after = dot(center - seg.a, outward);
+            T before = dot(start - seg.a, outward), after = dot((g_plant == 4 ? start : center) - seg.a, outward);
             if (!(before > 0) || !(after <= 0)) continue;
             V2<T> crossing = start + (center - start) * (before / (before - after));
             T along = dot(crossing - seg.a, edge) / lengthSquared;
@@ -1006,7 +1016,7 @@ template <class T> struct World {
             if (length(x.pos - nearest) <= x.radius) continue;
             if (!x.wrap) { x.region = 1; continue; }
             V2<T> delta = v2(T(0), T(0));
-            V2<T> span = hi - lo + 2 * x.radius;
+            V2<T> span = g_plant == 3 ? hi - lo : hi - lo + 2 * x.radius;
             T *dp[2] = {&delta.x, &delta.y};
             T pos[2] = {x.pos.x, x.pos.y}, rlo[2] = {lo.x, lo.y}, rhi[2] = {hi.x, hi.y}, sp[2] = {span.x, span.y};
             for (int axis = 0; axis < 2; ++axis) {
@@ -1057,7 +1067,7 @@ template <class T> struct World {
         stepTerms.assign(kFrameLimit, 0u);
         stepObjects.assign(kFrameLimit, 0u);
         std::memset(overflow, 0, sizeof overflow);
-        safety = 0; assertion = Assertion{}; candidateCount = 0; candidates.clear();
+        safety = 0; assertion = Assertion{}; candidateCount = 0; candidates.clear(); captured.clear();
         sc = SubstepControl<T>{};
         sc.time[0] = T(cfg.dt);
         sc.time[1] = n == 0 ? T(cfg.dt) : T(0);
@@ -1094,6 +1104,8 @@ template <class T> struct World {
                             x.pos = v2(T(cfg.injectionPosition[0]), T(cfg.injectionPosition[1]));
                 }
                 if (cfg.assertEverySubstep) assert_contacts();
+                if (captureOn)
+                    for (const auto &x : b) { captured.push_back(x.pos); captured.push_back(x.vel); }
             }
             overflow[2] = std::max(overflow[2], overflow[0]);
         }
@@ -1260,11 +1272,13 @@ int bpo_world_run(const bp_world_config *config, bp_bubble *bubbles, uint32_t n,
                   const int64_t *field_ids, uint32_t nfi, uint32_t steps,
                   int precision, uint32_t assert_mode, uint32_t hashed_override, bpo_frame *frames,
                   uint32_t *row_offsets, uint32_t *entries, uint32_t entries_capacity, float *step_history,
-                  uint32_t *step_terms, uint64_t *step_objects) {
+                  uint32_t *step_terms, uint64_t *step_objects, uint32_t *region_out, float *capture_out,
+                  uint32_t capture_capacity, uint32_t *capture_count) {
     auto run = [&](auto &w) -> int {
         bpo::load(w, config, bubbles, n, segments, ns, bodies, nb, vertices, nv, triangles, nt, motors, nm, bands, nbands,
                   fields, nf, field_vertices, nfv, field_ids, nfi);
         w.cfg.assertMode = assert_mode;
+        w.captureOn = capture_out != nullptr;
         if (hashed_override == 1) w.cfg.hashedGrid = 1;
         if (hashed_override == 2) w.cfg.hashedGrid = 0;
         int executed = 0;
@@ -1283,6 +1297,17 @@ int bpo_world_run(const bp_world_config *config, bp_bubble *bubbles, uint32_t n,
             if (f.status != BP_OK) break;
         }
         bpo::store(w, bubbles, bodies);
+        if (region_out)
+            for (uint32_t i = 0; i < n; ++i) region_out[i] = w.b[i].region;
+        if (capture_out) {
+            // Substeps of the last executed step, each as n (x, y, vx, vy) records.
+            uint32_t floats = (uint32_t)std::min<size_t>(2 * w.captured.size(), capture_capacity);
+            for (uint32_t k = 0; k < floats / 2; ++k) {
+                capture_out[2 * k] = (float)w.captured[k].x;
+                capture_out[2 * k + 1] = (float)w.captured[k].y;
+            }
+            if (capture_count) *capture_count = (uint32_t)(w.captured.size() / 2);
+        }
         if (row_offsets) {
             uint32_t total = 0;
             for (uint32_t i = 0; i < n; ++i) {
@@ -1526,6 +1551,9 @@ void bpo_candidate_rows(const bp_bubble *bubbles, const float *margins, uint32_t
     overflow[0] = w.overflow[0];
     overflow[1] = w.overflow[1];
 }
+
+// Planted mistake for the tests that show a pin catches it (0 switches every plant off).
+void bpo_set_plant(int code) { bpo::g_plant = code; }
 
 // Contact-law entry points (the harness kernels testEpsilon / testWallLaw / testPairLaw of
 // Tests/BubblePhysicsNativeMetalTests/NativeMetalContactLawTests.swift:15-44).

@@ -43,7 +43,8 @@ def load():
         lib = ctypes.CDLL(str(LIB_PATH))
         P, U32, I32 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int
         lib.bpo_world_run.argtypes = [P, P, U32, P, U32, P, U32, P, U32, P, U32, P, U32, P, U32, P, U32, P, U32, P, U32,
-                                      U32, I32, U32, U32, P, P, P, U32, P, P, P]
+                                      U32, I32, U32, U32, P, P, P, U32, P, P, P, P, P, U32, P]
+        lib.bpo_set_plant.argtypes = [I32]
         lib.bpo_noncontact_acceleration.argtypes = [P, P, U32, P, U32, P, U32, P, U32, P, U32, P, U32, P]
         lib.bpo_world_run.restype = ctypes.c_int
         lib.bpo_epsilon_f32.argtypes = [P, P, U32]
@@ -62,12 +63,19 @@ def _p(a):
     return None if a is None or a.size == 0 else a.ctypes.data
 
 
+def set_plant(code: int):
+    """Switches on a planted mistake of the oracle (0 = none); only for tests that show a pin fails."""
+    load().bpo_set_plant(int(code))
+
+
 def run(scene, steps: int | None = None, *, precision: int = 32, assert_mode: int = 1, grid: str = "auto",
-        adjacency: bool = False, history: bool = False):
+        adjacency: bool = False, history: bool = False, region: bool = False, capture: bool = False):
     """Runs `steps` steps of a scene in the oracle; returns (bubbles, bodies, frames, extras).
 
     The scene is not modified. grid: "auto" (hashed for <= 1024 bubbles like the reference's
-    single-dispatch path, sorted otherwise), "hashed" or "sorted".
+    single-dispatch path, sorted otherwise), "hashed" or "sorted". region: extras["region"] holds
+    each bubble's region result of the last step (BP_REGION_*). capture: extras["substeps"] holds
+    the (x, y, vx, vy) of every bubble after each substep of the last step, shape (substeps, n, 4).
     """
     lib = load()
     steps = scene.steps if steps is None else steps
@@ -89,13 +97,21 @@ def run(scene, steps: int | None = None, *, precision: int = 32, assert_mode: in
     fverts = np.ascontiguousarray(scene.field_vertices, np.float32).reshape(-1, 2)
     fids = np.ascontiguousarray(scene.field_filter_ids, np.int64)
     override = {"auto": 0, "hashed": 1, "sorted": 2}[grid]
+    region_out = np.zeros(max(n, 1), np.uint32) if region else None
+    cap = np.zeros(4 * max(n, 1) * 1024, np.float32) if capture else None
+    cap_count = ctypes.c_uint32(0)
     executed = lib.bpo_world_run(ctypes.byref(scene.config), _p(bubbles), n, _p(scene.segments), len(scene.segments),
                                  _p(bodies), len(bodies), _p(verts), len(verts), _p(tris), len(tris), _p(scene.motors),
                                  len(scene.motors), _p(scene.bands), len(scene.bands), _p(scene.fields), len(scene.fields),
                                  _p(fverts), len(fverts), _p(fids), len(fids), steps, precision, assert_mode,
                                  override, _p(frames), _p(row_offsets), _p(entries), 0 if entries is None else len(entries),
-                                 _p(hist), _p(terms), _p(objects))
+                                 _p(hist), _p(terms), _p(objects), _p(region_out), _p(cap), 0 if cap is None else len(cap),
+                                 ctypes.byref(cap_count))
     extras = {"executed": executed}
+    if region:
+        extras["region"] = region_out[:n].copy()
+    if capture:
+        extras["substeps"] = cap[: 4 * cap_count.value].reshape(-1, max(n, 1), 4).copy() if n else np.zeros((0, 0, 4))
     if adjacency:
         extras["row_offsets"] = row_offsets
         extras["entries"] = entries[: row_offsets[-1]].copy()
