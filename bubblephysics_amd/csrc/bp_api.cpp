@@ -1604,6 +1604,68 @@ bp_status bp_world_contacts(bp_world *w, bp_contact *out, uint32_t capacity, uin
     return w->batch.contacts(0, out, capacity, out_count);
 }
 
+// bp_world_contacts without any allocation: the slot offsets and records live in the library's
+// fixed per-GPU scratch array (bp_contacts.hip), the records are read back into the caller's
+// array and widened to bp_contact there, in place.
+bp_status bp_world_contacts_noalloc(bp_world *w, bp_contact *out, uint32_t capacity, uint32_t *out_count) {
+    if (!w || !out_count || (capacity && !out)) return BP_ERR_INVALID_ARGUMENT;
+    if (w->inFlight) return BP_ERR_FRAME_IN_FLIGHT;
+    bp_batch &b = w->batch;
+    if (!b.packed || !b.deviceValid || !b.descs) {
+        set_error("the world has no state on the device: upload and step it before reading contacts");
+        return BP_ERR_UNAVAILABLE;
+    }
+    void *base = nullptr;
+    uint32_t maxBubbles = 0, maxRecords = 0;
+    hipError_t e = bp_contacts_scratch(&base, &maxBubbles, &maxRecords);
+    if (e != hipSuccess) { set_error(std::string("scratch address: ") + hipGetErrorString(e)); return BP_ERR_COMMAND_SETUP_FAILED; }
+    bpd::WorldDesc d;
+    BP_HIP(hipMemcpy(&d, b.descs, sizeof d, hipMemcpyDeviceToHost));
+    if (!d.exportAdjacency) { set_error("contact records read the rows of the last grid build"); return BP_ERR_UNAVAILABLE; }
+    if (d.bubbleCount > maxBubbles || capacity > maxRecords) {
+        set_error("bp_world_contacts_noalloc holds worlds of at most 32768 bubbles and 65536 records per call");
+        return BP_ERR_CAPACITY_OVERFLOW;
+    }
+    *out_count = 0;
+    if (d.bubbleCount == 0) return BP_OK;
+    uint32_t *offsets = static_cast<uint32_t *>(base);
+    char *records = static_cast<char *>(base) + (size_t)2 * maxBubbles * sizeof(uint32_t);
+    uint32_t *total = reinterpret_cast<uint32_t *>(records + (size_t)maxRecords * sizeof(bpd::ContactRecord));
+    hipStream_t stream = b.streams[0];
+    e = bp_launch_contacts(b.arenaDev, b.descs, 0, offsets, records, capacity, total, stream);
+    uint32_t kept = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&kept, total, sizeof kept, hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    const uint32_t n = std::min(kept, capacity);
+    // The device records (56 bytes) land at the front of the caller's array (64-byte records)...
+    if (e == hipSuccess && n) e = hipMemcpy(out, records, (size_t)n * sizeof(bpd::ContactRecord), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) { set_error(std::string("contacts: ") + hipGetErrorString(e)); return BP_ERR_COMMAND_FAILED; }
+    // ...and are widened in place from the last: record k moves from 56 k to 64 k, so it is read
+    // whole before any write reaches it, and only records already moved are overwritten.
+    const char *raw = reinterpret_cast<const char *>(out);
+    for (uint32_t k = n; k-- > 0;) {
+        bpd::ContactRecord r;
+        std::memcpy(&r, raw + (size_t)k * sizeof(bpd::ContactRecord), sizeof r);
+        bp_contact c;
+        uint64_t h = 14695981039346656037ull;          // NativeMetalSolver.swift:1047-1052
+        h = (h ^ (uint64_t)r.kind) * 1099511628211ull;
+        h = (h ^ (uint64_t)r.first) * 1099511628211ull;
+        h = (h ^ (uint64_t)r.second) * 1099511628211ull;
+        c.id = h;
+        c.first = r.first; c.second = r.second;
+        c.kind = r.kind; c.feature = r.feature;
+        c.normal[0] = r.normal[0]; c.normal[1] = r.normal[1];
+        c.point[0] = r.point[0]; c.point[1] = r.point[1];
+        c.penetration = r.penetration;
+        c.allowed_side = r.allowedSide;
+        c.impulse = r.impulse;
+        c.index_b = r.indexB;
+        std::memcpy(reinterpret_cast<char *>(out) + (size_t)k * sizeof(bp_contact), &c, sizeof c);
+    }
+    *out_count = kept;
+    return BP_OK;
+}
+
 bp_status bp_world_debug_fail_next_command(bp_world *w) {
     if (!w) return BP_ERR_INVALID_ARGUMENT;
     w->failNext = true;

@@ -32,6 +32,7 @@ extern "C" hipError_t bp_launch_drop_removed(const bpd::Arena *arena, bpd::World
                                              uint32_t count, int64_t *removedIds, hipStream_t stream);
 extern "C" hipError_t bp_launch_audit(const bpd::Arena *arena, const bpd::WorldDesc *descs, uint32_t worlds, void *out,
                                       hipStream_t stream);
+extern "C" hipError_t bp_contacts_scratch(void **out, uint32_t *maxBubbles, uint32_t *maxRecords);
 extern "C" hipError_t bp_launch_contacts(const bpd::Arena *arena, const bpd::WorldDesc *descs, uint32_t world,
                                          uint32_t *offsets, void *out, uint32_t capacity, uint32_t *total,
                                          hipStream_t stream);

@@ -20,6 +20,8 @@
 // in chunks gives the offsets, and a writing pass recomputes and stores the kept records.
 #include "bp_world_global.h"
 
+#include <mutex>
+
 namespace bpd {
 
 namespace {
@@ -188,4 +190,47 @@ extern "C" __attribute__((visibility("hidden"))) hipError_t bp_launch_contacts(c
     hipLaunchKernelGGL(bpd::contacts_kernel, dim3(1), dim3(bpd::kContactBlock), 0, stream, arena, descs, world, offsets,
                        static_cast<bpd::ContactRecord *>(out), capacity, total);
     return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------------
+// Scratch of the entry point that allocates nothing (bp_world_contacts_noalloc): one fixed
+// device array per GPU for the slot offsets (2 n words) and the records, instead of a buffer
+// allocated per call. Every call on a GPU shares it and it keeps the last call's contents; calls
+// that overlap on one GPU race.
+// ---------------------------------------------------------------------------------------
+namespace bpd {
+constexpr uint32_t kNoAllocMaxBubbles = 32768;                 // worlds up to this many bubbles
+constexpr uint32_t kNoAllocMaxRecords = 65536;                 // up to this many records per call
+constexpr size_t kNoAllocOffsetBytes = (size_t)2 * kNoAllocMaxBubbles * sizeof(uint32_t);   // 256 KiB
+constexpr size_t kNoAllocRecordBytes = (size_t)kNoAllocMaxRecords * sizeof(ContactRecord);  // 3.5 MiB
+static_assert(kNoAllocOffsetBytes % 256 == 0 && kNoAllocRecordBytes % 256 == 0, "scratch sections stay 256-aligned");
+}  // namespace bpd
+
+static __device__ __align__(256) unsigned char scratch[bpd::kNoAllocOffsetBytes + bpd::kNoAllocRecordBytes + 256];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratchMutex;
+void *g_scratchPtr[kMaxDevices] = {};                          // device address of `scratch`, per GPU
+}  // namespace
+
+// Device address of this GPU's scratch array, looked up once per device with that device current.
+// The array's name is a host-side stand-in: only this address may be given to kernels and copies.
+extern "C" __attribute__((visibility("hidden"))) hipError_t bp_contacts_scratch(void **out, uint32_t *maxBubbles,
+                                                                            uint32_t *maxRecords) {
+    int device = 0;
+    hipError_t e = hipGetDevice(&device);
+    if (e != hipSuccess) return e;
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratchMutex);
+    if (!g_scratchPtr[device]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratchPtr[device] = p;
+    }
+    *out = g_scratchPtr[device];
+    *maxBubbles = bpd::kNoAllocMaxBubbles;
+    *maxRecords = bpd::kNoAllocMaxRecords;
+    return hipSuccess;
 }

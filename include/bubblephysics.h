@@ -358,6 +358,15 @@ typedef struct bp_contact {
  * (more than `capacity` means the list was cut, as the reference cuts at its contact capacity). */
 bp_status bp_world_contacts(bp_world *world, bp_contact *out, uint32_t capacity, uint32_t *out_count);
 
+/* bp_world_contacts without any device allocation. No counterpart in the reference: it is for
+ * callers that cannot allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). Its working memory is a fixed array of about 3.8 MiB in the library, one per
+ * GPU, held until the process ends: worlds of at most 32,768 bubbles and at most 65,536 records per
+ * call (larger return BP_ERR_CAPACITY_OVERFLOW before any device work). Every call on a GPU shares
+ * that array, so calls that overlap on one GPU race. The world must already be on the device (a
+ * step since its last upload), else BP_ERR_UNAVAILABLE. Results equal bp_world_contacts. */
+bp_status bp_world_contacts_noalloc(bp_world *world, bp_contact *out, uint32_t capacity, uint32_t *out_count);
+
 /* Contact transitions of one step: the contactBegan / contactEnded events of
  * NativePublicWorld.swift:340-391 derived from the step's contact records (bp_world_contacts) and
  * the previous step's set of contact pairs. Keys are (bubble A, bubble B) or (bubble A, segment
