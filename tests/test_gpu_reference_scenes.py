@@ -141,3 +141,51 @@ def test_region_ids_through_the_c_abi():
     assert list(world.read_bubbles()["id"]) == [2, 3]
     frame = world.step(1)
     assert frame.status == abi.BP_OK and frame.region_removals == set() and list(frame.bubbles["id"]) == [2, 3]
+
+
+# ---- StaticPolygonEdgeTests scenes (tests/test_oracle_polygon_edges.py) --------------------------
+import test_oracle_polygon_edges as edges_
+
+
+def polygon_world(kind):
+    if kind == "thin wall":
+        w = edges_.world(1000.0, [edges_.rectangle(-100.0, 48.0)])
+        return w.bubble(1, (48.0, 0.0), radius=148.0, mass=1.0).motor(1, (-2500.0, 0.0))
+    if kind == "collinear":
+        two = [(-148.0, -400.0), (-100.0, -400.0), (-100.0, 0.0), (-100.0, 400.0), (-148.0, 400.0)]
+        return edges_.world(1000.0, [two]).bubble(1, (-76.0, 0.0), radius=24.0, mass=1.36).motor(1, (-600.0, 0.0))
+    if kind == "winding":
+        return edges_.winding_worlds(True)
+    if kind == "fast":
+        v = (float(np.float32(-math.cos(math.pi / 6) * 6000.0)), float(np.float32(math.sin(math.pi / 6) * 6000.0)))
+        return edges_.world(2500.0, [edges_.rectangle(0.0, 48.0)]).bubble(1, (200.0, 0.0), radius=12.0,
+                                                                         mass=edges_.crowd_mass(12.0), velocity=v)
+    return edges_.world(2500.0, [edges_.rectangle(-100.0, 48.0)]).bubble(1, (-220.0, 510.0), radius=12.0,
+                                                                        mass=edges_.crowd_mass(12.0),
+                                                                        velocity=(1200.0, -1800.0))
+
+
+@pytest.mark.parametrize("family", FAMILIES)
+@pytest.mark.parametrize("kind", ["thin wall", "collinear", "winding", "fast", "oblique"])
+def test_polygon_edge_scenes_match_the_oracle(oracle, family, kind):
+    frames = {"thin wall": 60, "collinear": 60, "winding": 60, "fast": 40, "oblique": 6}[kind]
+    compare(polygon_world(kind), oracle, family, frames, kind)
+
+
+@pytest.mark.parametrize("family", ["workgroup", "cooperative"])
+def test_convex_vertex_with_a_drive_set_every_frame_matches_the_oracle(oracle, family):
+    # StaticPolygonEdgeTests.swift:166-185: the motor force changes before every frame.
+    w = edges_.world(1000.0, [edges_.wedge(90.0)]).bubble(1, (-24.0, 0.0), radius=24.0, mass=1.36)
+    scene = w.build()
+    cpu, gpu = rs.OracleRunner(scene, oracle), gpu_runner(scene, family)
+    for k in range(120):
+        position = cpu.position(1)
+        force = tuple(np.float32(edges_.unit(-position) * 600.0)) if k else (0.0, 0.0)
+        if k:
+            cpu.set_motor(1, force)
+            gpu.set_motor(1, force)
+        c, g = cpu.step(), gpu.step()
+        assert rs.frame_summary(g) == rs.frame_summary(c), k
+        for f in STATE:
+            assert np.array_equal(np.ascontiguousarray(gpu.scene.bubbles[f]).view(np.uint32),
+                                  np.ascontiguousarray(cpu.scene.bubbles[f]).view(np.uint32)), (k, f)
