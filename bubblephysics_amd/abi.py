@@ -288,6 +288,7 @@ SIGNATURES = {
     "bp_batch_step": ([_P, _U32, _P], _I32),
     "bp_batch_read_bubbles": ([_P, _U32, _P, _U32], _I32),
     "bp_batch_read_bodies": ([_P, _U32, _P, _U32], _I32),
+    "bp_batch_read_all_bubbles": ([_P, _P, _U32, _P, _U32], _I32),
     "bp_batch_last_gpu_milliseconds": ([_P], _F),
     "bp_batch_last_class_timing": ([_P, _P, _P, _P, _P], _I32),
     "bp_batch_class_count": ([], _U32),

@@ -1279,6 +1279,22 @@ bp_status bp_batch_read_bubbles(bp_batch *b, uint32_t world, bp_bubble *out, uin
     return BP_OK;
 }
 
+bp_status bp_batch_read_all_bubbles(bp_batch *b, bp_bubble *out, uint32_t capacity, uint32_t *offsets, uint32_t offsets_count) {
+    if (!b || !offsets || offsets_count != b->worlds.size() + 1) return BP_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(b->mutex);
+    if (b->inFlight || b->pending || b->serving) return BP_ERR_FRAME_IN_FLIGHT;
+    bp_status s = b->download_state();
+    if (s != BP_OK) return s;
+    uint32_t total = 0;
+    for (size_t wi = 0; wi < b->worlds.size(); ++wi) { offsets[wi] = total; total += (uint32_t)b->worlds[wi].bubbles.size(); }
+    offsets[b->worlds.size()] = total;
+    if (!out) return BP_OK;
+    if (capacity < total) return BP_ERR_CAPACITY_OVERFLOW;
+    for (size_t wi = 0; wi < b->worlds.size(); ++wi)
+        std::copy(b->worlds[wi].bubbles.begin(), b->worlds[wi].bubbles.end(), out + offsets[wi]);
+    return BP_OK;
+}
+
 bp_status bp_batch_read_bodies(bp_batch *b, uint32_t world, bp_body *out, uint32_t count) {
     if (!b || world >= b->worlds.size()) return BP_ERR_INVALID_ARGUMENT;
     std::lock_guard<std::mutex> lock(b->mutex);

@@ -341,6 +341,16 @@ class Batch:
         return _region_results(lambda *a: self._lib.bp_batch_region_results(self._handle, world, *a),
                                "bp_batch_region_results")
 
+    def read_all_bubbles(self):
+        """(records of every world concatenated, offsets[worlds + 1]) in one read."""
+        offsets = np.zeros(len(self._counts) + 1, np.uint32)
+        abi.check(self._lib.bp_batch_read_all_bubbles(self._handle, None, 0, abi.ptr(offsets), len(offsets)),
+                  "bp_batch_read_all_bubbles")
+        out = np.zeros(max(int(offsets[-1]), 1), abi.BUBBLE_DTYPE)
+        abi.check(self._lib.bp_batch_read_all_bubbles(self._handle, abi.ptr(out), len(out), abi.ptr(offsets), len(offsets)),
+                  "bp_batch_read_all_bubbles")
+        return out[: int(offsets[-1])], offsets
+
     def read_bodies(self, world: int, count: int) -> np.ndarray:
         """Bodies of one world at their published poses."""
         out = np.zeros(count, abi.BODY_DTYPE)

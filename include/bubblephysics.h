@@ -514,6 +514,10 @@ bp_status bp_batch_step(bp_batch *batch, uint32_t steps, bp_frame_info *infos);
 bp_status bp_batch_region_results(bp_batch *batch, uint32_t world, int64_t *removed, uint32_t removed_capacity,
                                   uint32_t *out_removed, int64_t *wrapped, uint32_t wrapped_capacity, uint32_t *out_wrapped);
 bp_status bp_batch_read_bubbles(bp_batch *batch, uint32_t world, bp_bubble *out, uint32_t count);
+/* Published bubbles of every world in one read: offsets[world_count + 1] receives each world's
+ * first record (a null `out` asks for the offsets only). */
+bp_status bp_batch_read_all_bubbles(bp_batch *batch, bp_bubble *out, uint32_t capacity, uint32_t *offsets,
+                                    uint32_t offsets_count);
 /* Bodies of one world at their published poses (count = the world's body count). */
 bp_status bp_batch_read_bodies(bp_batch *batch, uint32_t world, bp_body *out, uint32_t count);
 /* Captures `steps` steps of every resident world (the launches of every kernel class on the

@@ -55,7 +55,7 @@ def run_both(scene, steps, oracle, *, check_adjacency=False):
 
 
 def test_config1_plain_world_matches_oracle_bitwise(oracle):
-    run_both(scenes.config1(), 60, oracle, check_adjacency=True)
+    run_both(scenes.config1(), 120, oracle, check_adjacency=True)
 
 
 def test_config2_bodies_match_oracle_bitwise(oracle):
