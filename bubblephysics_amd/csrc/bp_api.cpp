@@ -144,6 +144,7 @@ bp_status bp_batch::pack() {
     BP_ALLOC(A.nbrA, nb); BP_ALLOC(A.nbrB, nb); BP_ALLOC(A.invmEff, nb);
     BP_ALLOC(A.cellA, nb); BP_ALLOC(A.cellB, nb);
     BP_ALLOC(A.adj, adj);
+    BP_ALLOC(A.adjGeo, adj);
     BP_ALLOC(A.segAB, ns); BP_ALLOC(A.segRegion, ns); BP_ALLOC(A.segFlags, ns); BP_ALLOC(A.segCat, ns);
     BP_ALLOC(A.segMask, ns); BP_ALLOC(A.segId, ns); BP_ALLOC(A.segOwner, ns);
     BP_ALLOC(A.bodyMotion, nbod); BP_ALLOC(A.bodyShape, nbod); BP_ALLOC(A.bodyLimits, nbod);
@@ -537,7 +538,7 @@ uint64_t bp_batch::footprint(uint32_t wi) const {
     const uint64_t perSlot = 2 * (3 * sizeof(float2) + 2 * sizeof(float) + sizeof(uint32_t))   // two-sided state
                              + sizeof(int64_t) + 6 * sizeof(float) + 3 * sizeof(uint32_t) + 2 * sizeof(int32_t)
                              + 3 * sizeof(float2) + sizeof(float) + 3 * sizeof(uint32_t);     // properties, scratch
-    const uint64_t adjacency = std::max<uint64_t>(slots, 16) * w.adjPerBubble * sizeof(uint32_t);
+    const uint64_t adjacency = std::max<uint64_t>(slots, 16) * w.adjPerBubble * (sizeof(uint32_t) + 4 * sizeof(float));
     const uint64_t geometry = w.segments.size() * 64 + w.bodies.size() * 160 + w.vertices.size() * sizeof(float) +
                               w.triangles.size() * sizeof(uint32_t) + (w.motors.size() + w.bands.size()) * sizeof(float4);
     return slots * perSlot + adjacency + geometry + bpd::kFrameLimit * sizeof(float) + sizeof(bpd::WorldDesc);

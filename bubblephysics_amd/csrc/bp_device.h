@@ -65,6 +65,10 @@ struct Arena {
     uint32_t *bodyHits;       // per body: bubbles in its candidate row at the last grid build
     // Adjacency entries: (kind << 30) | neighbour, CSR per world.
     uint32_t *adj;
+    // Per adjacency slot, the contact geometry the last force pass computed for that row entry
+    // (normal, distance and, for a body, its surface speed bound; for a segment whether the centre
+    // is in its region), read back by the distance pass that follows at the same state.
+    float4 *adjGeo;
     // Segments.
     float4 *segAB;            // a.xy, b.xy
     float4 *segRegion;        // previous_start.xy, outward sign, allowed side
@@ -280,6 +284,27 @@ BPD Law pair_law(float Ka, float ra, float Kb, float rb, float distanceValue) {
     return law;
 }
 
+// pair_law for two bubbles of equal stiffness (Ka == Kb), without branches: the same operations
+// in the same order as pair_law's closed form (no Newton steps are taken for equal stiffnesses),
+// so the result is the same bit for bit; the zero law is selected at the end.
+BPD Law pair_law_same(float K, float ra, float rb, float distanceValue) {
+    const float R = ra + rb;
+    const float d = fmaxf(distanceValue, R * 1e-6f);
+    const float overlap = R - d;
+    const bool none = !(K > 0.0f) || !(overlap > 0.0f);
+    const float force = fminf(K, K) * overlap * R / (d * d);
+    const float qa = force / K;
+    const float ea = epsilon_from_force_ratio(qa), sa = rest_from_force_ratio(qa);
+    const float da = (1.0f + ea) * K;
+    const float compliance = ra * sa * sa * sa / da + rb * sa * sa * sa / da;
+    Law law;
+    law.force = none ? 0.0f : force;
+    law.epsA = none ? 0.0f : ea;
+    law.epsB = law.epsA;
+    law.stiffness = none ? 0.0f : (compliance > kFloatMin ? 1.0f / compliance : 0.0f);
+    return law;
+}
+
 // ---------------------------------------------------------------------------------------
 // Segments, Geometry:436-440, 665-668; Force:112-140.
 // ---------------------------------------------------------------------------------------
@@ -392,7 +417,7 @@ BPD float2 frame_vertex(const BodyFrame &f, V verts, uint32_t i) {
 
 // Centre inside the body (Geometry:134-143, 467-490): round core distance, else the union of
 // the triangles in body coordinates.
-template <class V, class T>
+template <int EB = 4, class V, class T>
 BPD bool frame_inside(float2 p, const BodyFrame &f, V verts, T tris) {
     if (f.round) {
         float2 a = frame_vertex(f, verts, 0);
@@ -401,12 +426,19 @@ BPD bool frame_inside(float2 p, const BodyFrame &f, V verts, T tris) {
     }
     float2 q = sub(p, f.at);
     float2 local = f2(q.x * f.cn - q.y * f.sn, q.x * f.sn + q.y * f.cn);
-    for (uint32_t i = 0; i < f.ranges.w; ++i) {
-        uint3 t = triangle_of(tris, f.ranges.z + i);
-        float2 a = verts[f.ranges.x + t.x], bb = verts[f.ranges.x + t.y], c = verts[f.ranges.x + t.z];
-        if (cross2(sub(bb, a), sub(local, a)) >= 0.0f && cross2(sub(c, bb), sub(local, bb)) >= 0.0f &&
-            cross2(sub(a, c), sub(local, c)) >= 0.0f)
-            return true;
+    // EB triangles at a time (independent tests); the union is the same in any order.
+    const uint32_t count = f.ranges.w;
+    for (uint32_t i = 0; i < count; i += EB) {
+        bool hit = false;
+#pragma unroll
+        for (int k = 0; k < EB; ++k) {
+            const uint32_t ti = i + k < count ? i + k : i;
+            uint3 t = triangle_of(tris, f.ranges.z + ti);
+            float2 a = verts[f.ranges.x + t.x], bb = verts[f.ranges.x + t.y], c = verts[f.ranges.x + t.z];
+            hit = hit || (cross2(sub(bb, a), sub(local, a)) >= 0.0f && cross2(sub(c, bb), sub(local, bb)) >= 0.0f &&
+                          cross2(sub(a, c), sub(local, c)) >= 0.0f);
+        }
+        if (hit) return true;
     }
     return false;
 }
@@ -415,7 +447,7 @@ struct Obstacle { float2 normal; float distanceValue; float speedBound; };
 
 // Body as an obstacle of a bubble centre (Force:162-210): normal, signed distance to the
 // surface and the speed bound of the nearest surface point.
-template <class V, class T>
+template <int EB = 4, class V, class T>
 BPD Obstacle frame_obstacle(const BodyFrame &body, V verts, T tris, float2 center) {
     Obstacle result;
     float2 point, normal;
@@ -436,20 +468,36 @@ BPD Obstacle frame_obstacle(const BodyFrame &body, V verts, T tris, float2 cente
         float best = inff();
         float2 nearest = f2(0.0f, 0.0f), outward = f2(0.0f, 1.0f);
         const uint32_t count = body.ranges.y;
+        uint32_t bestEdge = 0;
+        bool found = false;
         float2 p = count > 0 ? frame_vertex(body, verts, 0) : f2(0.0f, 0.0f);
-        for (uint32_t e = 0; e < count; ++e) {
-            float2 q = frame_vertex(body, verts, e + 1 == count ? 0u : e + 1);
-            float2 candidate = closest_point(center, p, q);
-            float d = dist2(center, candidate);
-            if (d < best) {
-                best = d; nearest = candidate;
-                float2 edge = sub(q, p);
-                float size = len2(edge);
-                outward = size > kFloatMin ? dvs(f2(edge.y, -edge.x), size) : f2(0.0f, 1.0f);
+        // EB edges at a time: their closest points are independent and computed together; the
+        // nearest is then taken in edge order with the strict comparison, as one edge at a time.
+        for (uint32_t e = 0; e < count; e += EB) {
+            float2 v[EB + 1];
+            v[0] = p;
+#pragma unroll
+            for (int k = 1; k <= EB; ++k) v[k] = frame_vertex(body, verts, e + k < count ? e + k : 0u);
+            float2 cp[EB];
+            float cd[EB];
+#pragma unroll
+            for (int k = 0; k < EB; ++k) {
+                cp[k] = closest_point(center, v[k], v[k + 1]);
+                cd[k] = dist2(center, cp[k]);
             }
-            p = q;
+#pragma unroll
+            for (int k = 0; k < EB; ++k)
+                if (e + k < count && cd[k] < best) { best = cd[k]; nearest = cp[k]; bestEdge = e + k; found = true; }
+            p = v[EB];
         }
-        bool inside = frame_inside(center, body, verts, tris);
+        if (found) {
+            const float2 a = frame_vertex(body, verts, bestEdge);
+            const float2 b = frame_vertex(body, verts, bestEdge + 1 == count ? 0u : bestEdge + 1);
+            const float2 edge = sub(b, a);
+            const float size = len2(edge);
+            outward = size > kFloatMin ? dvs(f2(edge.y, -edge.x), size) : f2(0.0f, 1.0f);
+        }
+        bool inside = frame_inside<EB>(center, body, verts, tris);
         normal = inside ? outward : (best > 1e-7f ? dvs(sub(center, nearest), best) : outward);
         distanceValue = inside ? -best : best;
         point = nearest;

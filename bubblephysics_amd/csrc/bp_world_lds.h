@@ -73,23 +73,16 @@ struct RowRef {
 
 constexpr uint32_t kHasMotor = 1u << 31, kHasTorque = 1u << 30, kHasBand = 1u << 29;   // register-only flag bits
 
-// Neighbour index of row entry e of the block [start, end): every fourth entry loads the next
-// four indices together (one memory round trip for four entries, which matters most for rows
-// in global memory), the others select from those registers.
-BPD uint32_t row_entry4(const RowRef &rows, uint32_t e, uint32_t start, uint32_t end, uint32_t (&jn)[4]) {
-    const uint32_t u = (e - start) & 3u;
-    if (u == 0u) {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) jn[q] = rows.index(min(e + q, end - 1));
-    }
-    return u == 0u ? jn[0] : u == 1u ? jn[1] : u == 2u ? jn[2] : jn[3];
-}
 
 template <int BLOCK, int PER, int ROWCAP, int SEGCAP, int BODYCAP, int VERTCAP, int TRICAP>
 __global__ __launch_bounds__(BLOCK) void world_lds_kernel(const Arena *__restrict__ A, WorldDesc *descs, WorldStatus *stats,
                                                           const uint32_t *order, uint32_t count, uint32_t steps,
                                                           uint32_t statsStride, uint32_t notify) {
     using L = LdsWorld<BLOCK, PER, ROWCAP, SEGCAP, BODYCAP, VERTCAP, TRICAP>;
+    // Row entries, polygon edges and triangles evaluated side by side: QB at a time where a lane
+    // has 256 registers (workgroups of at most 512 threads), one at a time in the 1,024-thread
+    // configurations, whose 128 registers per lane leave no room for more in flight.
+    constexpr int QB = BLOCK <= 512 ? 4 : 1;
     __shared__ L S;
     if (blockIdx.x >= count) return;
     const uint32_t wi = order[blockIdx.x];
@@ -169,16 +162,20 @@ __global__ __launch_bounds__(BLOCK) void world_lds_kernel(const Arena *__restric
         // and a full mask, every pair passes the filter test; when every start is finite, every
         // neighbour's row exists. Either lets candidate enumeration skip the per-candidate global
         // loads of categories, masks and starts (the starts do not change until the end of step).
-        uint32_t localAccept = 1u, localStarts = 1u;
+        // Likewise, when no force-driven bubble has drag (its drag as the step applies it,
+        // max(gamma, 0), is +0), the drag is +0 for every bubble and is not read per substep.
+        uint32_t localAccept = 1u, localStarts = 1u, localNoDrag = 1u;
     #pragma unroll
         for (int s = 0; s < PER; ++s) {
             if (!valid[s]) continue;
             const uint32_t gi = o + lane + s * BLOCK;
             if (A->cat[gi] == 0u || A->mask[gi] != 0xffffffffu) localAccept = 0u;
             if (!finite2(p[s])) localStarts = 0u;
+            if ((fl[s] & kForceDriven) && __float_as_uint(fmaxf(A->gamma[gi], 0.0f)) != 0u) localNoDrag = 0u;
         }
         const bool allAccept = __syncthreads_and((int)localAccept) != 0;
         const bool allStarts = __syncthreads_and((int)localStarts) != 0;
+        const bool allNoDrag = __syncthreads_and((int)localNoDrag) != 0;
         uint32_t safety = predict_bodies<BLOCK>(A, d, pub, ws);
         // Geometry into LDS. Bodies are read back by the lane that predicted them.
         for (uint32_t k = lane; k < g.nb; k += BLOCK) {
@@ -215,6 +212,7 @@ __global__ __launch_bounds__(BLOCK) void world_lds_kernel(const Arena *__restric
         RowRef rows;
         rows.lds = S.rows;
         rows.global = A->adj + d->adjOffset;
+        float4 *const geoG = A->adjGeo + d->adjOffset;
         rows.inLds = true;
         const bool ldsRowsAllowed = ROWCAP > 0 && g.ns < kLdsIndexLimit && g.nb < kLdsIndexLimit;
 
@@ -258,6 +256,7 @@ __global__ __launch_bounds__(BLOCK) void world_lds_kernel(const Arena *__restric
         // The per-bubble records read on demand (drag only for force-driven bubbles, as the step
         // applies it; the motor's torque term is (torque / inertia), computed as at the start).
         auto gamma_of = [&](int s) __attribute__((always_inline)) -> float {
+            if (allNoDrag) return 0.0f;
             return (fl[s] & kForceDriven) ? fmaxf(A->gamma[o + lane + s * BLOCK], 0.0f) : 0.0f;
         };
         auto motor_of = [&](int s) __attribute__((always_inline)) -> float4 { return g.motor(A->motor[o + lane + s * BLOCK]); };
@@ -523,7 +522,9 @@ __global__ __launch_bounds__(BLOCK) void world_lds_kernel(const Arena *__restric
     #pragma unroll
                 for (int s = 0; s < PER; ++s) rs[s] = (lane + s * BLOCK) * kStageStride;
             }
-            if (!rows.inLds && !strideRows && total > d->adjCapacity) {
+            // Rows in LDS or packed in global memory must also fit the world's adjacency slots:
+            // the force pass keeps each entry's geometry in the slot of the same index.
+            if (!strideRows && total > d->adjCapacity) {
                 ov0 += buildDropped; ov1 = max(ov1, buildWidest);
                 return false;
             }
@@ -585,26 +586,54 @@ __global__ __launch_bounds__(BLOCK) void world_lds_kernel(const Arena *__restric
                 if (invm > 0.0f && (fl[s] & kHasBand)) w2 = w2 + g.band(band_of(s)).z * invm;
                 // The row is (pairs, segments, bodies): the reference's summation order (Force:216-255).
                 const uint32_t start = rs[s], pairEnd = start + (rpq[s] & 0xffffu), segEnd = start + (rpq[s] >> 16), end = start + rl[s];
-                uint32_t jn[4] = {0u, 0u, 0u, 0u};
-                for (uint32_t e = start; e < pairEnd; ++e) {
-                    const uint32_t j = row_entry4(rows, e, start, pairEnd, jn);
-                    const uint32_t lo = min(i, j), hi = max(i, j);
-                    const float2 delta = sub(S.pos[hi], S.pos[lo]);
-                    const float dd = len2(delta);
-                    const float2 normal = dd > 0.0f ? dvs(delta, dd) : f2(1.0f, 0.0f);
-                    const Law law = pair_law(S.K[lo], S.r[lo], S.K[hi], S.r[hi], dd);
-                    total = add(total, mul(i == lo ? -law.force : law.force, normal));
-                    if (invm > 0.0f) w2 = w2 + law.stiffness * (invm + __builtin_sqrtf(invm * S.invm[j]));
+                // QB pair entries at a time: their geometry and (for equal stiffnesses) their laws are
+                // independent and evaluated together; the sums then take them in row order.
+                for (uint32_t e = start; e < pairEnd; e += QB) {
+                    uint32_t jq[QB];
+                    bool okq[QB];
+                    float2 nq[QB];
+                    float ddq[QB], Klo[QB], Khi[QB], rlo[QB], rhi[QB];
+                    bool same = true;
+#pragma unroll
+                    for (int q = 0; q < QB; ++q) {
+                        okq[q] = e + q < pairEnd;
+                        const uint32_t j = rows.index(okq[q] ? e + q : start);
+                        jq[q] = j;
+                        const uint32_t lo = min(i, j), hi = max(i, j);
+                        const float2 delta = sub(S.pos[hi], S.pos[lo]);
+                        ddq[q] = len2(delta);
+                        nq[q] = ddq[q] > 0.0f ? dvs(delta, ddq[q]) : f2(1.0f, 0.0f);
+                        if (okq[q]) geoG[e + q] = make_float4(nq[q].x, nq[q].y, ddq[q], 0.0f);
+                        Klo[q] = S.K[lo]; Khi[q] = S.K[hi]; rlo[q] = S.r[lo]; rhi[q] = S.r[hi];
+                        same = same && (!okq[q] || Klo[q] == Khi[q]);
+                    }
+                    Law lq[QB];
+                    if (same) {
+#pragma unroll
+                        for (int q = 0; q < QB; ++q) lq[q] = pair_law_same(Klo[q], rlo[q], rhi[q], ddq[q]);
+                    } else {
+                        for (int q = 0; q < QB; ++q)
+                            lq[q] = okq[q] ? pair_law(Klo[q], rlo[q], Khi[q], rhi[q], ddq[q]) : Law{0.0f, 0.0f, 0.0f, 0.0f};
+                    }
+#pragma unroll
+                    for (int q = 0; q < QB; ++q) {
+                        if (!okq[q]) continue;
+                        total = add(total, mul(i < jq[q] ? -lq[q].force : lq[q].force, nq[q]));
+                        if (invm > 0.0f) w2 = w2 + lq[q].stiffness * (invm + __builtin_sqrtf(invm * S.invm[jq[q]]));
+                    }
                 }
                 for (uint32_t e = pairEnd; e < segEnd; ++e) {
-                    float2 normal; float dd;
-                    if (!wall_region(segment_at(rows.index(e)), pi, normal, dd) || !(dd < ri)) continue;
+                    float2 normal = f2(0.0f, 0.0f); float dd = 0.0f;
+                    const bool inRegion = wall_region(segment_at(rows.index(e)), pi, normal, dd);
+                    geoG[e] = make_float4(normal.x, normal.y, dd, inRegion ? 1.0f : 0.0f);
+                    if (!inRegion || !(dd < ri)) continue;
                     const Law law = wall_law(Ki, ri, dd);
                     total = add(total, mul(law.force, normal));
                     if (invm > 0.0f) w2 = w2 + law.stiffness * invm;
                 }
                 for (uint32_t e = segEnd; e < end; ++e) {
-                    const Obstacle ob = frame_obstacle(frame_at(rows.index(e)), S.verts, S.tris, pi);
+                    const Obstacle ob = frame_obstacle<QB>(frame_at(rows.index(e)), S.verts, S.tris, pi);
+                    geoG[e] = make_float4(ob.normal.x, ob.normal.y, ob.distanceValue, ob.speedBound);
                     const Law law = wall_law(Ki, ri, ob.distanceValue);
                     total = add(total, mul(law.force, ob.normal));
                     if (invm > 0.0f) w2 = w2 + law.stiffness * invm;
@@ -647,19 +676,16 @@ __global__ __launch_bounds__(BLOCK) void world_lds_kernel(const Arena *__restric
             float bestD = inff();
             uint32_t ownD = 0;
             // One candidate's bound (Force:563-597); returns +inf for an exempt or degenerate one.
+            // Written without branches (the exempt and degenerate cases are selected at the end), so
+            // that the bounds of several entries can be evaluated side by side.
             auto bound = [&](float dd, float approach, float accSum, float reach, float bodySpeed) __attribute__((always_inline)) -> float {
                 const bool active = dd < reach;
                 const float dc = fmaxf(dd, reach * 1e-6f);
-                float speed;
-                if (active) speed = fabsf(approach) + bodySpeed;
-                else {
-                    if (approach + bodySpeed + accSum * hMax * 0.5f <= 0.0f) return inff();
-                    speed = approach + bodySpeed;
-                }
+                const bool exempt = !active && approach + bodySpeed + accSum * hMax * 0.5f <= 0.0f;
+                const float speed = active ? fabsf(approach) + bodySpeed : approach + bodySpeed;
                 const float sq = __builtin_sqrtf(speed * speed + accSum * dc * 0.5f);
                 const float den = sq + speed;
-                if (!(den > 0.0f)) return inff();
-                return dc / (2.0f * den);
+                return (exempt || !(den > 0.0f)) ? inff() : dc / (2.0f * den);
             };
     #pragma unroll
             for (int s = 0; s < PER; ++s) {
@@ -670,22 +696,33 @@ __global__ __launch_bounds__(BLOCK) void world_lds_kernel(const Arena *__restric
                 const float ri = S.r[i], accI = S.aux[i];
                 float best = inff();
                 const uint32_t start = rs[s], pairEnd = start + (rpq[s] & 0xffffu), segEnd = start + (rpq[s] >> 16), end = start + rl[s];
-                uint32_t jn[4] = {0u, 0u, 0u, 0u};
-                for (uint32_t e = start; e < pairEnd; ++e) {
-                    const uint32_t j = row_entry4(rows, e, start, pairEnd, jn);
-                    const float2 delta = sub(S.pos[j], pi);
-                    const float dd = len2(delta);
-                    const float2 normal = dd > 0.0f ? dvs(delta, dd) : f2(1.0f, 0.0f);
-                    best = fminf(best, bound(dd, dot2(sub(vi, S.vel[j]), normal), accI + S.aux[j], ri + S.r[j], 0.0f));
+                // QB pair entries at a time (independent bounds; the minimum does not depend on order).
+                for (uint32_t e = start; e < pairEnd; e += QB) {
+                    float bq[QB];
+#pragma unroll
+                    for (int q = 0; q < QB; ++q) {
+                        const bool ok = e + q < pairEnd;
+                        const uint32_t j = rows.index(ok ? e + q : start);
+                        // The force pass stored the pair's geometry from its lower to its higher
+                        // index: the same bits as (x_j - x_i) / d for i < j, their negation (exact)
+                        // for i > j; (1, 0) at d = 0 either way.
+                        const float4 gq = geoG[ok ? e + q : start];
+                        const float dd = gq.z;
+                        const float2 normal = !(dd > 0.0f) ? f2(1.0f, 0.0f) : (i < j ? f2(gq.x, gq.y) : f2(-gq.x, -gq.y));
+                        const float b = bound(dd, dot2(sub(vi, S.vel[j]), normal), accI + S.aux[j], ri + S.r[j], 0.0f);
+                        bq[q] = ok ? b : inff();
+                    }
+#pragma unroll
+                    for (int q = 0; q < QB; ++q) best = fminf(best, bq[q]);
                 }
                 for (uint32_t e = pairEnd; e < segEnd; ++e) {
-                    float2 normal; float dd;
-                    if (!wall_region(segment_at(rows.index(e)), pi, normal, dd)) continue;
-                    best = fminf(best, bound(dd, -dot2(vi, normal), accI, ri, 0.0f));
+                    const float4 gs = geoG[e];                    // wall_region of the force pass
+                    if (gs.w == 0.0f) continue;
+                    best = fminf(best, bound(gs.z, -dot2(vi, f2(gs.x, gs.y)), accI, ri, 0.0f));
                 }
                 for (uint32_t e = segEnd; e < end; ++e) {
-                    const Obstacle ob = frame_obstacle(frame_at(rows.index(e)), S.verts, S.tris, pi);
-                    best = fminf(best, bound(ob.distanceValue, -dot2(vi, ob.normal), accI, ri, ob.speedBound));
+                    const float4 gb = geoG[e];                    // frame_obstacle of the force pass
+                    best = fminf(best, bound(gb.z, -dot2(vi, f2(gb.x, gb.y)), accI, ri, gb.w));
                 }
                 if (best < bestD) { bestD = best; ownD = i; }
             }
@@ -729,7 +766,7 @@ __global__ __launch_bounds__(BLOCK) void world_lds_kernel(const Arena *__restric
                 const float2 q = sub(center, S.frameAt[k]);
                 const float reach = S.bodyExtent[k].z * 1.0001f + 1e-3f;
                 if (q.x * q.x + q.y * q.y > reach * reach) continue;
-                if (frame_inside(center, frame_at(k), S.verts, S.tris)) {
+                if (frame_inside<QB>(center, frame_at(k), S.verts, S.tris)) {
                     out.kind = BP_VIOLATION_INSIDE_BODY; out.other = g.bodyId(k); out.segment = 0; out.detail = 0.0f;
                     return true;
                 }
