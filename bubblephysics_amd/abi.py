@@ -234,7 +234,7 @@ AUDIT_DTYPE = np.dtype(Audit)
 CONTACT_DTYPE = np.dtype(Contact)
 EVENT_DTYPE = np.dtype(ContactEvent)
 BP_EVENT_CONTACT_BEGAN, BP_EVENT_CONTACT_ENDED = 1, 2
-BP_CONTACT_PAIR, BP_CONTACT_SEGMENT = 0, 1
+BP_CONTACT_PAIR, BP_CONTACT_SEGMENT, BP_CONTACT_BODY = 0, 1, 2
 TUNE_DTYPE = np.dtype(TuneBin)
 REQUEST_DTYPE = np.dtype(Request)
 RESULT_DTYPE = np.dtype(ServerResult)
@@ -273,6 +273,8 @@ SIGNATURES = {
     "bp_world_contacts": ([_P, _P, _U32, _P], _I32),
     "bp_world_contacts_noalloc": ([_P, _P, _U32, _P], _I32),
     "bp_batch_contacts": ([_P, _U32, _P, _U32, _P], _I32),
+    "bp_world_body_contacts": ([_P, _P, _U32, _P], _I32),
+    "bp_batch_body_contacts": ([_P, _U32, _P, _U32, _P], _I32),
     "bp_contact_transitions": ([_P, _U32, _P, _U32, _P, _U32, ctypes.c_float, _P, _U32, _U32, _P, _P, _U32, _P], _I32),
     "bp_stage_merge": ([_P, _P, _U32, _P, ctypes.c_int32, _P, _U32, _P, _P], _I32),
     "bp_stage_add": ([_P, _P, _P, _U32, _P, _P], _I32),

@@ -933,8 +933,9 @@ bp_status bp_batch::audit(bp_audit *out, uint32_t count) {
     return BP_OK;
 }
 
-// End-of-step contact records of world `wi` (contacts_kernel, bp_contacts.hip).
-bp_status bp_batch::contacts(uint32_t wi, bp_contact *out, uint32_t capacity, uint32_t *count) {
+// End-of-step contact records of world `wi` (contacts_kernel, bp_contacts.hip): the pair and
+// segment records, or with `bodies` the bubble-body records, read back and put body by body.
+bp_status bp_batch::contacts(uint32_t wi, bp_contact *out, uint32_t capacity, uint32_t *count, bool bodies) {
     if (wi >= worlds.size() || (capacity && !out) || !count) return BP_ERR_INVALID_ARGUMENT;
     if (!packed) {
         bp_status s = pack();
@@ -949,23 +950,35 @@ bp_status bp_batch::contacts(uint32_t wi, bp_contact *out, uint32_t capacity, ui
     }
     *count = 0;
     if (d.bubbleCount == 0) return BP_OK;
-    void *dev = nullptr;
-    const size_t offsetBytes = (size_t)2 * d.bubbleCount * sizeof(uint32_t), recordBytes = (size_t)capacity * sizeof(bpd::ContactRecord);
-    BP_HIP(hipMalloc(&dev, offsetBytes + recordBytes + 256));
-    uint32_t *offsets = static_cast<uint32_t *>(dev);
-    auto *records = reinterpret_cast<bpd::ContactRecord *>(static_cast<char *>(dev) + ((offsetBytes + 255) / 256) * 256);
-    uint32_t *total = reinterpret_cast<uint32_t *>(static_cast<char *>(dev) + ((offsetBytes + 255) / 256) * 256 + recordBytes);
-    hipError_t e = bp_launch_contacts(arenaDev, descs, wi, offsets, records, capacity, total, streams[0]);
-    uint32_t kept = 0;
+    // Body records are reordered on the host, so all of them are read: a second launch with room
+    // for every record when the first finds more than `capacity`.
+    uint32_t room = capacity, kept = 0;
     std::vector<bpd::ContactRecord> host;
-    if (e == hipSuccess) e = hipMemcpyAsync(&kept, total, sizeof(kept), hipMemcpyDeviceToHost, streams[0]);
-    if (e == hipSuccess) e = hipStreamSynchronize(streams[0]);
-    if (e == hipSuccess && kept) {
-        host.resize(std::min(kept, capacity));
-        e = hipMemcpy(host.data(), records, host.size() * sizeof(bpd::ContactRecord), hipMemcpyDeviceToHost);
+    for (int pass = 0; pass < 2; ++pass) {
+        void *dev = nullptr;
+        const size_t offsetBytes = (size_t)(bodies ? 1 : 2) * d.bubbleCount * sizeof(uint32_t);
+        const size_t recordBytes = (size_t)room * sizeof(bpd::ContactRecord);
+        BP_HIP(hipMalloc(&dev, offsetBytes + recordBytes + 256));
+        uint32_t *offsets = static_cast<uint32_t *>(dev);
+        auto *records = reinterpret_cast<bpd::ContactRecord *>(static_cast<char *>(dev) + ((offsetBytes + 255) / 256) * 256);
+        uint32_t *total = reinterpret_cast<uint32_t *>(static_cast<char *>(dev) + ((offsetBytes + 255) / 256) * 256 + recordBytes);
+        hipError_t e = bodies ? bp_launch_body_contacts(arenaDev, descs, wi, offsets, records, room, total, streams[0])
+                              : bp_launch_contacts(arenaDev, descs, wi, offsets, records, room, total, streams[0]);
+        if (e == hipSuccess) e = hipMemcpyAsync(&kept, total, sizeof(kept), hipMemcpyDeviceToHost, streams[0]);
+        if (e == hipSuccess) e = hipStreamSynchronize(streams[0]);
+        if (e == hipSuccess && kept) {
+            host.resize(std::min(kept, room));
+            e = hipMemcpy(host.data(), records, host.size() * sizeof(bpd::ContactRecord), hipMemcpyDeviceToHost);
+        }
+        (void)hipFree(dev);
+        if (e != hipSuccess) { set_error(std::string("contacts: ") + hipGetErrorString(e)); return BP_ERR_COMMAND_FAILED; }
+        if (!bodies || kept <= room) break;
+        room = kept;
     }
-    (void)hipFree(dev);
-    if (e != hipSuccess) { set_error(std::string("contacts: ") + hipGetErrorString(e)); return BP_ERR_COMMAND_FAILED; }
+    if (bodies)
+        std::stable_sort(host.begin(), host.end(),
+                         [](const bpd::ContactRecord &a, const bpd::ContactRecord &b) { return a.indexB < b.indexB; });
+    if (host.size() > capacity) host.resize(capacity);
     for (size_t k = 0; k < host.size(); ++k) {
         const bpd::ContactRecord &r = host[k];
         bp_contact &c = out[k];
@@ -1354,6 +1367,13 @@ bp_status bp_batch_contacts(bp_batch *b, uint32_t world, bp_contact *out, uint32
     return b->contacts(world, out, capacity, out_count);
 }
 
+bp_status bp_batch_body_contacts(bp_batch *b, uint32_t world, bp_contact *out, uint32_t capacity, uint32_t *out_count) {
+    if (!b) return BP_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(b->mutex);
+    if (b->inFlight || b->pending || b->serving) return BP_ERR_FRAME_IN_FLIGHT;
+    return b->contacts(world, out, capacity, out_count, true);
+}
+
 bp_status bp_batch_set_world_class(bp_batch *b, uint32_t world, int32_t cls) {
     if (!b || world >= b->worlds.size() || cls < -1 || cls >= kClassCount) return BP_ERR_INVALID_ARGUMENT;
     std::lock_guard<std::mutex> lock(b->mutex);
@@ -1619,6 +1639,12 @@ bp_status bp_world_contacts(bp_world *w, bp_contact *out, uint32_t capacity, uin
     if (!w || !out_count) return BP_ERR_INVALID_ARGUMENT;
     if (w->inFlight) return BP_ERR_FRAME_IN_FLIGHT;
     return w->batch.contacts(0, out, capacity, out_count);
+}
+
+bp_status bp_world_body_contacts(bp_world *w, bp_contact *out, uint32_t capacity, uint32_t *out_count) {
+    if (!w || !out_count) return BP_ERR_INVALID_ARGUMENT;
+    if (w->inFlight) return BP_ERR_FRAME_IN_FLIGHT;
+    return w->batch.contacts(0, out, capacity, out_count, true);
 }
 
 // bp_world_contacts without any allocation: the slot offsets and records live in the library's

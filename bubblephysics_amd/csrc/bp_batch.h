@@ -36,6 +36,9 @@ extern "C" hipError_t bp_contacts_scratch(void **out, uint32_t *maxBubbles, uint
 extern "C" hipError_t bp_launch_contacts(const bpd::Arena *arena, const bpd::WorldDesc *descs, uint32_t world,
                                          uint32_t *offsets, void *out, uint32_t capacity, uint32_t *total,
                                          hipStream_t stream);
+extern "C" hipError_t bp_launch_body_contacts(const bpd::Arena *arena, const bpd::WorldDesc *descs, uint32_t world,
+                                              uint32_t *offsets, void *out, uint32_t capacity, uint32_t *total,
+                                              hipStream_t stream);
 extern "C" hipError_t bp_launch_world_coop(const bpd::Arena *arena, bpd::WorldDesc *descs, bpd::WorldStatus *stats,
                                            const uint32_t *orderEntry, uint32_t world, uint32_t groups, hipStream_t stream);
 
@@ -256,7 +259,7 @@ struct bp_batch {
     bp_status finish(bp_frame_info *infos, bool wait);
     bp_status step(uint32_t steps, bp_frame_info *infos);
     bp_status audit(bp_audit *out, uint32_t count);
-    bp_status contacts(uint32_t world, bp_contact *out, uint32_t capacity, uint32_t *count);
+    bp_status contacts(uint32_t world, bp_contact *out, uint32_t capacity, uint32_t *count, bool bodies = false);
     bp_status capture(uint32_t steps);
     bp_status replay(bp_frame_info *infos);
     bp_status collect(uint32_t steps, bp_frame_info *infos, const std::vector<uint32_t> &listed, float ms);

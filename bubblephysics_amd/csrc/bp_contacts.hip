@@ -14,7 +14,8 @@
 // 625-651): every bubble's pair candidates (partner index ascending), then every bubble's segment
 // candidates (segment ascending, which is id order). Here the candidates are read from the
 // exported adjacency rows of the last build: the pair entries above the bubble's own index and the
-// segment entries. Body entries belong to the rigid-body records, not to this list.
+// segment entries. Body entries feed the bubble-body records (nativeBubbleBodyContacts), a list of
+// their own (bp_world_body_contacts).
 //
 // One workgroup per world: a counting pass writes each list slot's kept-record count, a block scan
 // in chunks gives the offsets, and a writing pass recomputes and stores the kept records.
@@ -103,6 +104,46 @@ __device__ Contact segment_contact(float2 pa, float ra, int64_t idA, const World
     return c;
 }
 
+// nativeBubbleBodyContacts (NativeForceKernels.metal-source:337-358): bubble i against its body
+// candidates at the bodies' end-of-step poses (fraction 1); a record iff distance < radius and the
+// normal is not zero. Records of one bubble come in its row's order (body ascending); the host
+// orders them body by body.
+template <bool WRITE>
+__device__ uint32_t walk_bodies(const World &W, uint32_t i, float dt, const uint32_t *offsets, ContactRecord *out,
+                                uint32_t capacity) {
+    const float2 pa = W.pos[i];
+    const float ra = W.radius[i], Ka = W.Kc[i];
+    const uint32_t *row = W.adj + W.rowStart[i];
+    const uint32_t len = W.rowLen[i];
+    uint32_t kept = 0, base = 0;
+    if (WRITE) base = i == 0 ? 0u : offsets[i - 1];
+    for (uint32_t e = 0; e < len; ++e) {
+        const uint32_t entry = row[e], kind = entry >> kKindShift, k = entry & kIndexMask;
+        if (kind != 2u) continue;
+        const Obstacle ob = body_obstacle(W.body(k), 1.0f, W.verts, W.tris, pa);
+        if (!(ob.distanceValue < ra)) continue;
+        const float2 normal = f2(-ob.normal.x, -ob.normal.y);
+        if (normal.x == 0.0f && normal.y == 0.0f) continue;
+        if (WRITE && base + kept < capacity) {
+            const float2 point = sub(pa, mul(ob.normal, ob.distanceValue));
+            ContactRecord r;
+            r.first = W.id[i];
+            r.second = (int64_t)W.A->bodyId[W.d->bodyOffset + k];
+            r.kind = 2u;
+            r.feature = 0u;
+            r.normal[0] = normal.x; r.normal[1] = normal.y;
+            r.point[0] = point.x; r.point[1] = point.y;
+            r.penetration = ra - ob.distanceValue;
+            r.allowedSide = 0.0f;
+            r.impulse = wall_law(Ka, ra, ob.distanceValue).force * dt * dt;
+            r.indexB = k;
+            out[base + kept] = r;
+        }
+        ++kept;
+    }
+    return kept;
+}
+
 // The list slot of bubble i's pair candidates is i, of its segment candidates n + i.
 template <bool WRITE>
 __device__ uint32_t walk_slot(const World &W, uint32_t slot, float sepTol, float dt, const uint32_t *offsets,
@@ -144,15 +185,18 @@ __device__ uint32_t walk_slot(const World &W, uint32_t slot, float sepTol, float
     return kept;
 }
 
+// BODIES = false: pair and segment records (2 n slots); true: bubble-body records (n slots).
+template <bool BODIES>
 __global__ __launch_bounds__(kContactBlock) void contacts_kernel(const Arena *__restrict__ A, const WorldDesc *descs,
                                                                  uint32_t world, uint32_t *offsets, ContactRecord *out,
                                                                  uint32_t capacity, uint32_t *total) {
     const WorldDesc *d = descs + world;
     const World W = make_world(A, d, d->pub);
-    const uint32_t slots = 2 * W.n;
+    const uint32_t slots = BODIES ? W.n : 2 * W.n;
     const float sepTol = d->sepTol, dt = d->dt;
     for (uint32_t s = threadIdx.x; s < slots; s += kContactBlock)
-        offsets[s] = walk_slot<false>(W, s, sepTol, dt, nullptr, nullptr, 0);
+        offsets[s] = BODIES ? walk_bodies<false>(W, s, dt, nullptr, nullptr, 0)
+                            : walk_slot<false>(W, s, sepTol, dt, nullptr, nullptr, 0);
     __syncthreads();
     // Inclusive scan over the slots in chunks of the workgroup width (nativeScanInGroup).
     __shared__ uint32_t part[kContactBlock];
@@ -172,7 +216,10 @@ __global__ __launch_bounds__(kContactBlock) void contacts_kernel(const Arena *__
         carry += part[kContactBlock - 1];
         __syncthreads();
     }
-    for (uint32_t s = threadIdx.x; s < slots; s += kContactBlock) walk_slot<true>(W, s, sepTol, dt, offsets, out, capacity);
+    for (uint32_t s = threadIdx.x; s < slots; s += kContactBlock) {
+        if (BODIES) walk_bodies<true>(W, s, dt, offsets, out, capacity);
+        else walk_slot<true>(W, s, sepTol, dt, offsets, out, capacity);
+    }
     if (threadIdx.x == 0) *total = carry;
 }
 
@@ -187,7 +234,18 @@ extern "C" __attribute__((visibility("hidden"))) hipError_t bp_launch_contacts(c
                                                                            uint32_t world, uint32_t *offsets, void *out,
                                                                            uint32_t capacity, uint32_t *total,
                                                                            hipStream_t stream) {
-    hipLaunchKernelGGL(bpd::contacts_kernel, dim3(1), dim3(bpd::kContactBlock), 0, stream, arena, descs, world, offsets,
+    hipLaunchKernelGGL(bpd::contacts_kernel<false>, dim3(1), dim3(bpd::kContactBlock), 0, stream, arena, descs, world, offsets,
+                       static_cast<bpd::ContactRecord *>(out), capacity, total);
+    return hipGetLastError();
+}
+
+// Bubble-body records of one world, as bp_launch_contacts with n offset words.
+extern "C" __attribute__((visibility("hidden"))) hipError_t bp_launch_body_contacts(const bpd::Arena *arena,
+                                                                                const bpd::WorldDesc *descs, uint32_t world,
+                                                                                uint32_t *offsets, void *out,
+                                                                                uint32_t capacity, uint32_t *total,
+                                                                                hipStream_t stream) {
+    hipLaunchKernelGGL(bpd::contacts_kernel<true>, dim3(1), dim3(bpd::kContactBlock), 0, stream, arena, descs, world, offsets,
                        static_cast<bpd::ContactRecord *>(out), capacity, total);
     return hipGetLastError();
 }

@@ -13,8 +13,8 @@
 // normalised (zero when the sum is zero) and the impulse sum; ended events follow, in key order,
 // for keys of the previous set that are not current. The current keys become the new set. With
 // no object of interest and an empty previous set there are no events and the set stays empty.
-// The rigid-body contact records the reference adds to the same sums are not produced here
-// (bubble worlds carry kinematic bodies only as obstacles).
+// Bubble-body records (BP_CONTACT_BODY) are the reference's rigid records (:377-380): key (bubble,
+// body), weight 1, their normal from the bubble to the body, impulse / dt, and no penetration test.
 #include <algorithm>
 #include <cmath>
 #include <map>
@@ -59,14 +59,15 @@ extern "C" bp_status bp_contact_transitions(const bp_contact *contacts, uint32_t
     for (uint32_t k = 0; k < count; ++k) {
         const bp_contact &c = contacts[k];
         const uint64_t first = (uint64_t)c.first;
+        const bool rigid = c.kind == BP_CONTACT_BODY;
         uint64_t second;
-        if (c.kind == BP_CONTACT_PAIR) second = (uint64_t)c.second;
+        if (c.kind == BP_CONTACT_PAIR || rigid) second = (uint64_t)c.second;
         else if (!owner_of(c.second, second)) continue;
-        if (!(c.penetration > 0.0f || c.impulse > 0.0f)) continue;
+        if (!rigid && !(c.penetration > 0.0f || c.impulse > 0.0f)) continue;
         if (!is_interesting(first) && !is_interesting(second)) continue;
-        const float w = std::fmax(c.penetration, 0.0f) + 1e-4f;
+        const float w = rigid ? 1.0f : std::fmax(c.penetration, 0.0f) + 1e-4f;
         float nx = c.normal[0], ny = c.normal[1];
-        if (c.kind != BP_CONTACT_PAIR) { nx = -nx; ny = -ny; }          // from the segment to A: A to the owner
+        if (c.kind == BP_CONTACT_SEGMENT) { nx = -nx; ny = -ny; }       // from the segment to A: A to the owner
         const Key key = first <= second ? Key(first, second) : Key(second, first);
         if (key.first != first) { nx = -nx; ny = -ny; }
         Sum &s = sums[key];

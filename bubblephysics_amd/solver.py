@@ -207,6 +207,11 @@ class NativeWorld:
         return _read_contacts(lambda out, cap, count: self._lib.bp_world_contacts(self._handle, out, cap, count),
                               "bp_world_contacts", capacity)
 
+    def body_contacts(self, capacity: int | None = None) -> np.ndarray:
+        """End-of-step bubble-body contact records (bp_contact, kind BP_CONTACT_BODY)."""
+        return _read_contacts(lambda out, cap, count: self._lib.bp_world_body_contacts(self._handle, out, cap, count),
+                              "bp_world_body_contacts", capacity)
+
     def read_step_history(self) -> np.ndarray:
         out = np.zeros(1024, np.float32)
         count = ctypes.c_uint32(0)
@@ -373,6 +378,11 @@ class Batch:
         """End-of-step contact records (bp_contact) of one world's published state."""
         return _read_contacts(lambda out, cap, count: self._lib.bp_batch_contacts(self._handle, world, out, cap, count),
                               "bp_batch_contacts", capacity)
+
+    def body_contacts(self, world: int, capacity: int | None = None) -> np.ndarray:
+        """End-of-step bubble-body contact records (bp_contact, kind BP_CONTACT_BODY) of one world."""
+        return _read_contacts(lambda out, cap, count: self._lib.bp_batch_body_contacts(self._handle, world, out, cap, count),
+                              "bp_batch_body_contacts", capacity)
 
     def capture(self, steps: int = 1):
         """Captures `steps` steps of every world into a hipGraph (bp_batch_capture)."""

@@ -340,23 +340,36 @@ bp_status bp_world_audit(bp_world *world, bp_audit *out);
  * segments; `impulse` is the contact law's force at the record's geometry times dt^2
  * (nativeContactImpulses, NativeForceKernels.metal-source:261-280). The world's rows must be kept
  * (export_adjacency, always on for bp_world) and its last step must have succeeded. 64 bytes. */
-enum { BP_CONTACT_PAIR = 0, BP_CONTACT_SEGMENT = 1 };
+enum { BP_CONTACT_PAIR = 0, BP_CONTACT_SEGMENT = 1, BP_CONTACT_BODY = 2 };
 typedef struct bp_contact {
     uint64_t id;                /* FNV-1a of (kind, first, second), NativeMetalSolver.swift:1047-1052 */
     int64_t first;              /* bubble A id                                                     */
-    int64_t second;             /* bubble B id, or the segment id                                  */
+    int64_t second;             /* bubble B id, the segment id, or the body id                     */
     uint32_t kind;              /* BP_CONTACT_*                                                    */
     uint32_t feature;           /* polygon edge: 1 edge interior, 2 convex start vertex; else 0   */
     float normal[2];
     float point[2];             /* pointQ                                                          */
-    float penetration;
+    float penetration;          /* body records: radius - distance                                */
     float allowed_side;         /* 0: the segment has no allowed side (the reference's nil)        */
     float impulse;              /* accumulatedCompression: F dt^2                                  */
-    uint32_t index_b;           /* bubble B index, or the segment index                            */
+    uint32_t index_b;           /* bubble B index, the segment index, or the body index            */
 } bp_contact;
 /* Writes up to `capacity` records; *out_count receives the number of records the step has
  * (more than `capacity` means the list was cut, as the reference cuts at its contact capacity). */
 bp_status bp_world_contacts(bp_world *world, bp_contact *out, uint32_t capacity, uint32_t *out_count);
+
+/* End-of-step bubble-body contact records (nativeBubbleBodyContacts,
+ * NativeForceKernels.metal-source:333-358, read back as the rigid records of
+ * NativeRigidPolygonState.swift:390-402): per bubble-body candidate of the last grid build (the body
+ * entries of the bubbles' rows), the body's obstacle at its end-of-step pose against the bubble's
+ * final centre; kept iff distance < radius (and a non-zero normal). kind BP_CONTACT_BODY, first = the
+ * bubble id, second = the body id, normal from the bubble to the body, point = centre - obstacle
+ * normal * distance on the body's surface, penetration = radius - distance, impulse = the wall law's
+ * force at that distance times dt^2, index_b = the body index. Ordered body by body, bubbles
+ * ascending, as the reference's body rows. Rows of a bubble that overflowed its row capacity hold
+ * only the entries the cut kept, where the reference's body rows keep every candidate. Writes up to
+ * `capacity` records; *out_count receives the number the step has. */
+bp_status bp_world_body_contacts(bp_world *world, bp_contact *out, uint32_t capacity, uint32_t *out_count);
 
 /* bp_world_contacts without any device allocation. No counterpart in the reference: it is for
  * callers that cannot allocate (code under HIP graph capture, tight serving loops, harnesses that
@@ -375,8 +388,10 @@ bp_status bp_world_contacts_noalloc(bp_world *world, bp_contact *out, uint32_t c
  * the normalised normal sum from the key's first object to its second and the impulse F dt;
  * ended events follow in key order. `history` holds pairs as (first, second) u64 couples: the
  * previous set on input (history_count pairs), the current set on output (*out_history_count; more
- * than history_capacity returns BP_ERR_CAPACITY_OVERFLOW). Host-only: no device work. The
- * reference's rigid-body contact records are not part of the sums. */
+ * than history_capacity returns BP_ERR_CAPACITY_OVERFLOW). Host-only: no device work. Bubble-body
+ * records (BP_CONTACT_BODY, bp_world_body_contacts) enter the sums as the reference's rigid records
+ * (NativePublicWorld.swift:377-380): key (bubble, body), weight 1, their normal, impulse / dt, and no
+ * penetration test; pass them after the pair and segment records, the reference's order. */
 enum { BP_EVENT_CONTACT_BEGAN = 1, BP_EVENT_CONTACT_ENDED = 2 };
 typedef struct bp_contact_event {
     uint32_t kind;              /* BP_EVENT_CONTACT_*                                             */
@@ -541,6 +556,8 @@ bp_status bp_batch_last_class_timing(const bp_batch *batch, float *milliseconds,
 bp_status bp_batch_audit(bp_batch *batch, bp_audit *out, uint32_t count);
 /* End-of-step contact records (bp_world_contacts) of one world of a batch. */
 bp_status bp_batch_contacts(bp_batch *batch, uint32_t world, bp_contact *out, uint32_t capacity, uint32_t *out_count);
+/* End-of-step bubble-body contact records (bp_world_body_contacts) of one world of a batch. */
+bp_status bp_batch_body_contacts(bp_batch *batch, uint32_t world, bp_contact *out, uint32_t capacity, uint32_t *out_count);
 /* Autotuner: for each bin of world sizes (bin b: 2^b to 2^(b+1) - 1 bubbles), times one step of
  * the bin's worlds in every kernel configuration that holds them all (the LDS-resident
  * configurations and the one-workgroup global-state kernel: workgroup shape and the LDS-to-HBM

@@ -1519,6 +1519,58 @@ uint32_t bpo_contacts(const bp_world_config *config, const bp_bubble *bubbles, u
     return kept;
 }
 
+// End-of-step bubble-body contact records (nativeBubbleBodyContacts, NativeForceKernels.metal-source:
+// 333-358, read back by NativeRigidPolygonState.swift:390-402): per candidate (body, bubble) of the
+// body rows, the obstacle at the body's end-of-step pose (fraction 1) and the bubble's final centre;
+// kept iff distance < radius and the record's normal is not zero. Record: normal = -obstacle normal
+// (from the bubble to the body), point = centre - obstacle normal * distance, impulse = the wall
+// law's force at that distance times dt^2; penetration carries radius - distance. The candidates
+// are the body entries of the bubbles' rows of the last grid build (entries (2 << 30) | body), taken
+// body by body with the bubbles ascending, the order of the reference's body rows.
+uint32_t bpo_body_contacts(const bp_world_config *config, const bp_bubble *bubbles, uint32_t n, const bp_body *bodies,
+                           uint32_t nb, const float *vertices, uint32_t nv, const uint32_t *triangles, uint32_t nt,
+                           const uint32_t *row_offsets, const uint32_t *entries, bp_contact *out, uint32_t capacity) {
+    bpo::World<float> w;
+    bpo::load(w, config, bubbles, n, nullptr, 0, bodies, nb, vertices, nv, triangles, nt, nullptr, 0, nullptr, 0, nullptr, 0,
+              nullptr, 0, nullptr, 0);
+    const float dt = config->time_step;
+    uint32_t kept = 0;
+    for (uint32_t k = 0; k < nb; ++k) {
+        for (uint32_t i = 0; i < n; ++i) {
+            bool candidate = false;
+            for (uint32_t e = row_offsets[i]; e < row_offsets[i + 1]; ++e)
+                if ((entries[e] >> 30) == 2u && (entries[e] & ((1u << 30) - 1u)) == k) candidate = true;
+            if (!candidate) continue;
+            const auto &a = w.b[i];
+            const auto ob = bpo::body_obstacle(w.bodies[k], 1.0f, w.verts, w.tris, a.pos);
+            if (!(ob.distanceValue < a.radius)) continue;
+            const bpo::V2<float> normal = -ob.normal;
+            if (normal.x == 0.0f && normal.y == 0.0f) continue;
+            if (kept < capacity) {
+                const bpo::V2<float> point = a.pos - ob.normal * ob.distanceValue;
+                const float force = bpo::wall_contact_law<float>(a.Kc, a.radius, ob.distanceValue).force;
+                bp_contact &c = out[kept];
+                c = bp_contact{};
+                uint64_t h = 14695981039346656037ull;
+                h = (h ^ (uint64_t)BP_CONTACT_BODY) * 1099511628211ull;
+                h = (h ^ (uint64_t)a.id) * 1099511628211ull;
+                h = (h ^ (uint64_t)bodies[k].id) * 1099511628211ull;
+                c.id = h;
+                c.first = a.id;
+                c.second = (int64_t)bodies[k].id;
+                c.kind = BP_CONTACT_BODY;
+                c.normal[0] = normal.x; c.normal[1] = normal.y;
+                c.point[0] = point.x; c.point[1] = point.y;
+                c.penetration = a.radius - ob.distanceValue;
+                c.impulse = force * dt * dt;
+                c.index_b = k;
+            }
+            ++kept;
+        }
+    }
+    return kept;
+}
+
 // Candidate rows of one grid build with a given cell width and margins (the kernel
 // nativeEmitCandidates, Geometry:300-340, and its hashed twin, Force:1010-1035).
 // rows[n * limit * 2] receives (kind, other index) per slot, row_counts[2n] the bubble and

@@ -55,6 +55,8 @@ def load():
         lib.bpo_audit.argtypes = [P, P, U32, P, U32, P, U32, P, U32, P, U32, P]
         lib.bpo_contacts.argtypes = [P, P, U32, P, U32, P, P, P, U32]
         lib.bpo_contacts.restype = U32
+        lib.bpo_body_contacts.argtypes = [P, P, U32, P, U32, P, U32, P, U32, P, P, P, U32]
+        lib.bpo_body_contacts.restype = U32
         _lib = lib
     return _lib
 
@@ -160,6 +162,23 @@ def contacts(scene, bubbles, row_offsets, entries, capacity=None):
     out = np.zeros(max(cap, 1), CONTACT_DTYPE)
     k = lib.bpo_contacts(ctypes.byref(scene.config), _p(b), len(b), _p(scene.segments), len(scene.segments), _p(offs),
                          _p(ents), _p(out), cap)
+    return out[: min(k, cap)], int(k)
+
+
+def body_contacts(scene, bubbles, bodies, row_offsets, entries, capacity=None):
+    """End-of-step bubble-body contact records (bp_contact, kind 2) of a world state: the body
+    entries of its last build's rows against the bodies at their end-of-step poses."""
+    lib = load()
+    b = np.ascontiguousarray(bubbles)
+    bd = np.ascontiguousarray(bodies)
+    verts = np.ascontiguousarray(scene.vertices, np.float32).reshape(-1, 2)
+    tris = np.ascontiguousarray(scene.triangles, np.uint32).reshape(-1, 4)
+    offs = np.ascontiguousarray(row_offsets, np.uint32)
+    ents = np.ascontiguousarray(entries, np.uint32)
+    cap = int(offs[-1]) if capacity is None else capacity
+    out = np.zeros(max(cap, 1), CONTACT_DTYPE)
+    k = lib.bpo_body_contacts(ctypes.byref(scene.config), _p(b), len(b), _p(bd), len(bd), _p(verts), len(verts), _p(tris),
+                              len(tris), _p(offs), _p(ents), _p(out), cap)
     return out[: min(k, cap)], int(k)
 
 
