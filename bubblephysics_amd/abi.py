@@ -275,6 +275,7 @@ SIGNATURES = {
     "bp_batch_contacts": ([_P, _U32, _P, _U32, _P], _I32),
     "bp_world_body_contacts": ([_P, _P, _U32, _P], _I32),
     "bp_batch_body_contacts": ([_P, _U32, _P, _U32, _P], _I32),
+    "bp_batch_set_pooling": ([_P, _U32], _I32),
     "bp_contact_transitions": ([_P, _U32, _P, _U32, _P, _U32, ctypes.c_float, _P, _U32, _U32, _P, _P, _U32, _P], _I32),
     "bp_stage_merge": ([_P, _P, _U32, _P, ctypes.c_int32, _P, _U32, _P, _P], _I32),
     "bp_stage_add": ([_P, _P, _P, _U32, _P, _P], _I32),

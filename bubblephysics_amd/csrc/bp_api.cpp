@@ -159,6 +159,7 @@ bp_status bp_batch::pack() {
     BP_ALLOC(arenaDev, 1);
     for (int k = 0; k < kClassCount; ++k) BP_ALLOC(orders[k], worlds.size());
     BP_ALLOC(subsetOrders, worlds.size());
+    BP_ALLOC(cursors, kClassCount + 1);
     xferCapacity = 1;
     for (const auto &w : worlds) xferCapacity = std::max(xferCapacity, w.capacity);
     BP_ALLOC(xferBubbles, xferCapacity);
@@ -324,9 +325,45 @@ bp_status bp_batch::pack() {
         BP_HIP(hipEventCreate(&classEnd[k]));
     }
     BP_HIP(hipEventCreate(&startEvent));
+    BP_HIP(hipEventCreateWithFlags(&gateEvent, hipEventDisableTiming));
+    if (!deviceCUs) {
+        int device = 0;
+        BP_HIP(hipGetDevice(&device));
+        BP_HIP(hipDeviceGetAttribute(&deviceCUs, hipDeviceAttributeMultiprocessorCount, device));
+        for (int k = 0; k < bp_lds_config_count(); ++k) residency[k] = bp_lds_config_residency(k);
+    }
+    for (auto &g : poolGroups) g = 0;
+    estimatesMeasured = false;
     packed = true;
     deviceValid = true;
     return BP_OK;
+}
+
+// Workgroups of the pooled launches. The classes whose workgroup needs a whole CU (one resident
+// per CU) share out the CUs in proportion to their measured work (the summed device time of
+// their worlds' last steps), leaving the small classes a share in proportion to theirs, counted
+// as if 1.5 of their workgroups ran on a CU at full speed (a small world's step is bound by
+// latency, so company on a CU slows each one down; 1.5 is the measured best on config-4 batches).
+// The small classes are launched one workgroup per world and fill the CUs the pools leave. A
+// class whose share is at least its world count is launched one workgroup per world too.
+void bp_batch::plan_pools() {
+    for (auto &g : poolGroups) g = 0;
+    if (!pooling || !estimatesMeasured || deviceCUs <= 0) return;
+    double work[kClassCount] = {}, heavy = 0.0, small = 0.0;
+    for (int k = 0; k < bp_lds_config_count(); ++k) {
+        if (classWorlds[k].empty() || residency[k] <= 0) continue;
+        for (uint32_t wi : classWorlds[k]) work[k] += (double)workEstimate[wi];
+        if (residency[k] == 1) heavy += work[k];
+        else small += work[k] / std::min((double)residency[k], 1.5);
+    }
+    if (!(heavy > 0.0)) return;
+    const double cus = std::floor(deviceCUs * heavy / (heavy + small));
+    for (int k = 0; k < bp_lds_config_count(); ++k) {
+        const uint32_t count = (uint32_t)classWorlds[k].size();
+        if (!count || residency[k] != 1) continue;
+        const uint32_t g = (uint32_t)std::max(1.0, std::floor(cus * work[k] / heavy));
+        poolGroups[k] = g < count ? g : 0;
+    }
 }
 
 // Copies the published state of every world back into the host mirror.
@@ -596,6 +633,7 @@ bp_status bp_batch::launch(const uint32_t *subset, uint32_t subsetCount, uint32_
             }
             std::stable_sort(classOrder, classOrder + kClassCount, [&](int a, int b) { return longest[a] > longest[b]; });
             orderDirty = false;
+            plan_pools();
         }
     }
     if (!settling)
@@ -610,13 +648,43 @@ bp_status bp_batch::launch(const uint32_t *subset, uint32_t subsetCount, uint32_
         statsCapacity = need;
     }
     BP_HIP(hipMemsetAsync(stats, 0, need * sizeof(bpd::WorldStatus), streams[0]));
+    const bool pooled = !subset && std::any_of(poolGroups, poolGroups + kClassCount, [](uint32_t g) { return g != 0; });
+    if (pooled) {
+        hipError_t e = bp_launch_fill_u32(cursors, kClassCount + 1, 0u, streams[0]);
+        if (e != hipSuccess) { set_error(std::string("kernel launch: ") + hipGetErrorString(e)); return BP_ERR_COMMAND_FAILED; }
+    }
     BP_HIP(hipEventRecord(startEvent, streams[0]));
     for (int k = 1; k < kStreams; ++k) BP_HIP(hipStreamWaitEvent(streams[k], startEvent, 0));
+    // With pools, the pooled classes go first and the others wait behind a gate until every
+    // pooled workgroup has started (or 2 ms have passed): the pools' workgroups need whole CUs,
+    // and the small workgroups of the other classes would otherwise be scattered over the CUs
+    // before the pools are placed.
+    int sequence[kClassCount], pooledClasses = 0, listed = 0;
+    uint32_t poolTarget = 0;
+    for (int idx = 0; idx < kClassCount; ++idx)
+        if (pooled && poolGroups[classOrder[idx]] && launchCount[classOrder[idx]]) {
+            sequence[listed++] = classOrder[idx];
+            poolTarget += poolGroups[classOrder[idx]];
+        }
+    pooledClasses = listed;
+    for (int idx = 0; idx < kClassCount; ++idx)
+        if (!(pooled && poolGroups[classOrder[idx]] && launchCount[classOrder[idx]])) sequence[listed++] = classOrder[idx];
+    bool gated = false;
     for (int idx = 0; idx < kClassCount; ++idx) {
-        const int k = classOrder[idx];
+        const int k = sequence[idx];
         const uint32_t count = launchCount[k];
         if (!count || k == kCoopClass) continue;
         hipStream_t stream = streams[k % kStreams];
+        if (pooledClasses && idx >= pooledClasses) {
+            if (!gated) {
+                hipError_t e = bp_launch_gate(cursors + kClassCount, poolTarget, 200000u, stream);
+                if (e != hipSuccess) { set_error(std::string("kernel launch: ") + hipGetErrorString(e)); return BP_ERR_COMMAND_FAILED; }
+                BP_HIP(hipEventRecord(gateEvent, stream));
+                gated = true;
+            } else {
+                BP_HIP(hipStreamWaitEvent(stream, gateEvent, 0));
+            }
+        }
         BP_HIP(hipEventRecord(classStart[k], stream));
         // An LDS configuration runs all the steps in one launch: each workgroup carries its world
         // through them without waiting for the slowest world of the class at every step.
@@ -626,7 +694,8 @@ bp_status bp_batch::launch(const uint32_t *subset, uint32_t subsetCount, uint32_
                 if (e != hipSuccess) { set_error(std::string("kernel launch: ") + hipGetErrorString(e)); return BP_ERR_COMMAND_FAILED; }
             }
         } else {
-            hipError_t e = bp_launch_world_lds(k, arenaDev, descs, stats, orderOf(k), count, steps, (uint32_t)W, 0u, stream);
+            hipError_t e = bp_launch_world_lds(k, arenaDev, descs, stats, orderOf(k), count, steps, (uint32_t)W, 0u,
+                                               pooled ? poolGroups[k] : 0u, cursors + k, cursors + kClassCount, stream);
             if (e != hipSuccess) { set_error(std::string("kernel launch: ") + hipGetErrorString(e)); return BP_ERR_COMMAND_FAILED; }
         }
         BP_HIP(hipEventRecord(classEnd[k], stream));
@@ -834,6 +903,7 @@ bp_status bp_batch::collect(uint32_t steps, bp_frame_info *infos, const std::vec
             const uint64_t est = std::max<uint64_t>(1, ticks / executed), old = workEstimate[wi];
             if (4 * est > 5 * old || 5 * est < 4 * old) { workEstimate[wi] = est; orderDirty = true; }
         }
+    if (workEstimate.size() == W && listedWorlds.size() == W) estimatesMeasured = true;
     if (!infos) return BP_OK;
     const size_t listed = listedWorlds.size();
     for (uint32_t st = 0; st < steps; ++st)
@@ -1031,6 +1101,7 @@ bp_status bp_batch::capture(uint32_t steps) {
     hipStream_t origin = streams[0];
     BP_HIP(hipStreamBeginCapture(origin, hipStreamCaptureModeThreadLocal));
     hipError_t e = bp_launch_fill_u32(reinterpret_cast<uint32_t *>(stats), need * sizeof(bpd::WorldStatus) / 4, 0u, origin);
+    if (e == hipSuccess) e = bp_launch_fill_u32(cursors, kClassCount + 1, 0u, origin);
     if (e == hipSuccess) e = hipEventRecord(forkEvent, origin);
     for (int k = 1; k < kStreams && e == hipSuccess; ++k) e = hipStreamWaitEvent(streams[k], forkEvent, 0);
     for (int idx = 0; idx < kClassCount && e == hipSuccess; ++idx) {
@@ -1042,7 +1113,8 @@ bp_status bp_batch::capture(uint32_t steps) {
             for (uint32_t st = 0; st < steps && e == hipSuccess; ++st)
                 e = bp_launch_world_step(1024, arenaDev, descs, stats + (size_t)st * W, orders[k], count, 0u, stream);
         } else {
-            e = bp_launch_world_lds(k, arenaDev, descs, stats, orders[k], count, steps, (uint32_t)W, 0u, stream);
+            e = bp_launch_world_lds(k, arenaDev, descs, stats, orders[k], count, steps, (uint32_t)W, 0u, poolGroups[k],
+                                    cursors + k, cursors + kClassCount, stream);
         }
     }
     for (int k = 1; k < kStreams && e == hipSuccess; ++k) {
@@ -1383,6 +1455,15 @@ bp_status bp_batch_set_world_class(bp_batch *b, uint32_t world, int32_t cls) {
     if (cls >= 0 && !class_fits(cls, b->worlds[world])) return BP_ERR_INVALID_ARGUMENT;
     b->worlds[world].forcedClass = cls;
     b->packed = false;
+    return BP_OK;
+}
+
+bp_status bp_batch_set_pooling(bp_batch *b, uint32_t enabled) {
+    if (!b) return BP_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(b->mutex);
+    if (b->inFlight || b->pending || b->serving) return BP_ERR_FRAME_IN_FLIGHT;
+    b->pooling = enabled != 0;
+    b->plan_pools();
     return BP_OK;
 }
 

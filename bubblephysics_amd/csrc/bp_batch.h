@@ -18,8 +18,11 @@ extern "C" hipError_t bp_launch_world_step(int block, const bpd::Arena *arena, b
                                            const uint32_t *order, uint32_t count, uint32_t notify, hipStream_t stream);
 extern "C" hipError_t bp_launch_world_lds(int id, const bpd::Arena *arena, bpd::WorldDesc *descs, bpd::WorldStatus *stats,
                                           const uint32_t *order, uint32_t count, uint32_t steps, uint32_t statsStride,
-                                          uint32_t notify, hipStream_t stream);
+                                          uint32_t notify, uint32_t groups, uint32_t *cursor, uint32_t *placed,
+                                          hipStream_t stream);
+extern "C" hipError_t bp_launch_gate(const uint32_t *placed, uint32_t target, uint32_t maxTicks, hipStream_t stream);
 extern "C" int bp_lds_config_count();
+extern "C" int bp_lds_config_residency(int id);
 extern "C" void bp_lds_config(int id, uint32_t *threads, uint32_t *per, uint32_t *rowcap, uint32_t *caps);
 extern "C" uint32_t bp_coop_max_groups();
 extern "C" hipError_t bp_launch_gather_world(const bpd::Arena *arena, const bpd::WorldDesc *desc, bp_bubble *out, uint32_t count,
@@ -162,8 +165,20 @@ struct bp_batch {
     // the world on the critical path of the batch gets a compute unit before the others.
     int classOrder[bph::kClassCount] = {11, 10, 9, 8, 7, 6, 5, 4, 3, 2, 1, 0};
     bool orderDirty = false;            // estimates changed since the class order lists were uploaded
+    bool estimatesMeasured = false;     // workEstimate holds device ticks (after a whole-batch step)
+    // Workgroup pools: a class launched with fewer workgroups than worlds, each workgroup taking
+    // the next world of the class's longest-first order when it finishes one. The pools divide
+    // the CUs among the classes in proportion to their measured work, so that the large
+    // workgroups, which need a whole CU, never wait behind small ones scattered over the CUs.
+    bool pooling = true;
+    uint32_t *cursors = nullptr;        // per class: the next-world word of a pooled launch; then the
+                                        // count of pooled workgroups that have started
+    uint32_t poolGroups[bph::kClassCount] = {};   // per class: workgroups of a pooled launch (0 = one per world)
+    int residency[bph::kClassCount] = {};         // per class: workgroups one CU holds
+    int deviceCUs = 0;
     hipStream_t streams[bph::kStreams] = {};
     hipEvent_t startEvent = nullptr, endEvents[bph::kStreams] = {};
+    hipEvent_t gateEvent = nullptr;     // the gate of the classes launched after the pools
     hipEvent_t classStart[bph::kClassCount] = {}, classEnd[bph::kClassCount] = {};
     bpd::WorldStatus *stats = nullptr;
     size_t statsCapacity = 0;          // in WorldStatus records
@@ -224,8 +239,10 @@ struct bp_batch {
             classStart[k] = classEnd[k] = nullptr;
         }
         if (startEvent) (void)hipEventDestroy(startEvent);
+        if (gateEvent) (void)hipEventDestroy(gateEvent);
+        gateEvent = nullptr;
         startEvent = nullptr;
-        stats = nullptr; statsCapacity = 0; descs = nullptr; arenaDev = nullptr; subsetOrders = nullptr;
+        stats = nullptr; statsCapacity = 0; descs = nullptr; arenaDev = nullptr; subsetOrders = nullptr; cursors = nullptr;
         xferBubbles = nullptr; xferIndices = nullptr; xferCapacity = 0;
         dropRecs = nullptr; dropIds = nullptr; dropCapacity = dropIdCapacity = 0;
         pending = false;
@@ -256,6 +273,7 @@ struct bp_batch {
     bp_status write_bubbles(uint32_t wi, const bp_bubble *bubbles, uint32_t count);
     uint64_t footprint(uint32_t wi) const;
     bp_status launch(const uint32_t *subset, uint32_t subsetCount, uint32_t steps);
+    void plan_pools();
     bp_status finish(bp_frame_info *infos, bool wait);
     bp_status step(uint32_t steps, bp_frame_info *infos);
     bp_status audit(bp_audit *out, uint32_t count);

@@ -558,7 +558,7 @@ bp_status bp_server::start_launch(int cls) {
         if (e != hipSuccess) { set_error(std::string("transaction launch: ") + hipGetErrorString(e)); return BP_ERR_COMMAND_FAILED; }
     }
     hipError_t e = cls == kGlobalClass ? bp_launch_world_step(1024, b.arenaDev, b.descs, stats, L.dOrder, L.count, 1u, s)
-                                       : bp_launch_world_lds(cls, b.arenaDev, b.descs, stats, L.dOrder, L.count, 1, W, 1u, s);
+                                       : bp_launch_world_lds(cls, b.arenaDev, b.descs, stats, L.dOrder, L.count, 1, W, 1u, 0u, nullptr, nullptr, s);
     if (e != hipSuccess) { set_error(std::string("kernel launch: ") + hipGetErrorString(e)); return BP_ERR_COMMAND_FAILED; }
     BP_HIP(hipEventRecord(L.done, s));
     ++inflightPerClass[cls];

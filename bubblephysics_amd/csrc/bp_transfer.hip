@@ -242,6 +242,25 @@ extern "C" __attribute__((visibility("hidden"))) hipError_t bp_launch_gather_ful
     return hipGetLastError();
 }
 
+// Gate of the launches that follow it on a stream: one lane waits until `target` pooled
+// workgroups have started (they count themselves in *placed), or `maxTicks` ticks of the
+// 100 MHz wall clock have passed, whichever comes first, so the wait always ends.
+namespace bpd {
+__global__ void gate_kernel(const uint32_t *placed, uint32_t target, uint32_t maxTicks) {
+    if (threadIdx.x != 0) return;
+    const uint64_t t0 = wall_clock64();
+    while (__hip_atomic_load(placed, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target &&
+           wall_clock64() - t0 < (uint64_t)maxTicks)
+        __builtin_amdgcn_s_sleep(4);
+}
+}  // namespace bpd
+
+extern "C" __attribute__((visibility("hidden"))) hipError_t bp_launch_gate(const uint32_t *placed, uint32_t target,
+                                                                       uint32_t maxTicks, hipStream_t stream) {
+    hipLaunchKernelGGL(bpd::gate_kernel, dim3(1), dim3(64), 0, stream, placed, target, maxTicks);
+    return hipGetLastError();
+}
+
 // Fill of a u32 array by a kernel: graph captures record kernels, not memset nodes.
 namespace bpd {
 __global__ void fill_u32_kernel(uint32_t *p, size_t words, uint32_t value) {

@@ -74,18 +74,17 @@ struct RowRef {
 constexpr uint32_t kHasMotor = 1u << 31, kHasTorque = 1u << 30, kHasBand = 1u << 29;   // register-only flag bits
 
 
+// All the steps of world `wi` by one workgroup, with the world's LDS state in S.
 template <int BLOCK, int PER, int ROWCAP, int SEGCAP, int BODYCAP, int VERTCAP, int TRICAP>
-__global__ __launch_bounds__(BLOCK) void world_lds_kernel(const Arena *__restrict__ A, WorldDesc *descs, WorldStatus *stats,
-                                                          const uint32_t *order, uint32_t count, uint32_t steps,
-                                                          uint32_t statsStride, uint32_t notify) {
+__device__ __attribute__((always_inline)) void lds_world_steps(LdsWorld<BLOCK, PER, ROWCAP, SEGCAP, BODYCAP, VERTCAP, TRICAP> &S,
+                                                               const Arena *__restrict__ A, WorldDesc *descs,
+                                                               WorldStatus *stats, const uint32_t wi, uint32_t steps,
+                                                               uint32_t statsStride, uint32_t notify) {
     using L = LdsWorld<BLOCK, PER, ROWCAP, SEGCAP, BODYCAP, VERTCAP, TRICAP>;
     // Row entries, polygon edges and triangles evaluated side by side: QB at a time where a lane
     // has 256 registers (workgroups of at most 512 threads), one at a time in the 1,024-thread
     // configurations, whose 128 registers per lane leave no room for more in flight.
     constexpr int QB = BLOCK <= 512 ? 4 : 1;
-    __shared__ L S;
-    if (blockIdx.x >= count) return;
-    const uint32_t wi = order[blockIdx.x];
     WorldDesc *d = descs + wi;
     WorldStatus *st = stats + wi;
     const uint32_t lane = threadIdx.x;
@@ -1004,6 +1003,32 @@ __global__ __launch_bounds__(BLOCK) void world_lds_kernel(const Arena *__restric
 #undef catG
 #undef maskG
 #undef startG
+}
+
+// The worlds order[0..count) of one configuration. Without `cursor` each workgroup steps the
+// world of its own index. With `cursor` (a zeroed device word) the launch has fewer workgroups
+// than worlds, sized by the host to the CUs the class may hold: each workgroup steps the world of
+// its index, then takes the next unclaimed index from the cursor, so the worlds run in the
+// order's longest-first sequence on a fixed set of resident workgroups. Each such workgroup
+// counts itself in `placed` when it starts (the host's gate for the other classes waits on it).
+template <int BLOCK, int PER, int ROWCAP, int SEGCAP, int BODYCAP, int VERTCAP, int TRICAP>
+__global__ __launch_bounds__(BLOCK) void world_lds_kernel(const Arena *__restrict__ A, WorldDesc *descs, WorldStatus *stats,
+                                                          const uint32_t *order, uint32_t count, uint32_t steps,
+                                                          uint32_t statsStride, uint32_t notify, uint32_t *cursor,
+                                                          uint32_t *placed) {
+    __shared__ LdsWorld<BLOCK, PER, ROWCAP, SEGCAP, BODYCAP, VERTCAP, TRICAP> S;
+    __shared__ uint32_t nextSlot;
+    if (placed && threadIdx.x == 0) atomicAdd(placed, 1u);
+    uint32_t slot = blockIdx.x;
+    while (slot < count) {
+        lds_world_steps<BLOCK, PER, ROWCAP, SEGCAP, BODYCAP, VERTCAP, TRICAP>(S, A, descs, stats, order[slot], steps,
+                                                                              statsStride, notify);
+        if (!cursor) break;
+        __syncthreads();                                      // the next world reuses LDS
+        if (threadIdx.x == 0) nextSlot = gridDim.x + atomicAdd(cursor, 1u);
+        __syncthreads();
+        slot = nextSlot;
+    }
 }
 
 }  // namespace bpd

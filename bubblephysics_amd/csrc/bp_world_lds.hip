@@ -10,7 +10,9 @@
 #define BP_LDS_DECLARE(ID, B, P, R, SC, BC, VC, TC)                                                                 \
     extern "C" hipError_t bp_launch_lds_cfg_##ID(const bpd::Arena *arena, bpd::WorldDesc *descs, bpd::WorldStatus *stats, \
                                                  const uint32_t *order, uint32_t count, uint32_t steps,         \
-                                                 uint32_t statsStride, uint32_t notify, hipStream_t stream);
+                                                 uint32_t statsStride, uint32_t notify, uint32_t groups,        \
+                                                 uint32_t *cursor, uint32_t *placed, hipStream_t stream);        \
+    extern "C" int bp_residency_lds_cfg_##ID();
 BP_LDS_CONFIGS(BP_LDS_DECLARE)
 #undef BP_LDS_DECLARE
 
@@ -31,15 +33,28 @@ extern "C" __attribute__((visibility("hidden"))) void bp_lds_config(int id, uint
     if (caps) caps[0] = caps[1] = caps[2] = caps[3] = 0;
 }
 
+// Workgroups of configuration `id` one CU holds at once (0 when unknown).
+extern "C" __attribute__((visibility("hidden"))) int bp_lds_config_residency(int id) {
+#define BP_LDS_RESIDENCY(ID, B, P, R, SC, BC, VC, TC) \
+    if (id == ID) return bp_residency_lds_cfg_##ID();
+    BP_LDS_CONFIGS(BP_LDS_RESIDENCY)
+#undef BP_LDS_RESIDENCY
+    return 0;
+}
+
 // `steps` steps of every world in one launch; the status of step k of world w is stats[k * statsStride + w].
 // With `notify`, each status record is marked complete with a system-scope release (post_status).
+// With `groups` below `count` and a zeroed `cursor` word, `groups` workgroups share the worlds and
+// count themselves in `placed` (may be null) as they start.
 extern "C" __attribute__((visibility("hidden"))) hipError_t bp_launch_world_lds(int id, const bpd::Arena *arena, bpd::WorldDesc *descs,
                                                                            bpd::WorldStatus *stats, const uint32_t *order,
                                                                            uint32_t count, uint32_t steps, uint32_t statsStride,
-                                                                           uint32_t notify, hipStream_t stream) {
+                                                                           uint32_t notify, uint32_t groups, uint32_t *cursor,
+                                                                           uint32_t *placed, hipStream_t stream) {
     if (count == 0) return hipSuccess;
 #define BP_LDS_LAUNCH(ID, B, P, R, SC, BC, VC, TC) \
-    if (id == ID) return bp_launch_lds_cfg_##ID(arena, descs, stats, order, count, steps, statsStride, notify, stream);
+    if (id == ID) return bp_launch_lds_cfg_##ID(arena, descs, stats, order, count, steps, statsStride, notify, groups, cursor, placed, \
+                                          stream);
     BP_LDS_CONFIGS(BP_LDS_LAUNCH)
 #undef BP_LDS_LAUNCH
     return hipErrorInvalidValue;

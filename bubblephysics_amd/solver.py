@@ -408,6 +408,10 @@ class Batch:
     def set_world_class(self, world: int, cls: int):
         abi.check(self._lib.bp_batch_set_world_class(self._handle, world, cls), "bp_batch_set_world_class")
 
+    def set_pooling(self, enabled: bool):
+        """Workgroup pools of the batched launch (bp_batch_set_pooling); results do not change."""
+        abi.check(self._lib.bp_batch_set_pooling(self._handle, 1 if enabled else 0), "bp_batch_set_pooling")
+
     # ---- serving -------------------------------------------------------------------------
     def step_worlds(self, worlds, steps: int = 1) -> np.ndarray:
         """Advances only the listed worlds; returns FrameInfo records shaped (steps, len(worlds))."""

@@ -579,6 +579,12 @@ bp_status bp_batch_autotune(bp_batch *batch, uint32_t repeats, bp_tune_bin *out,
 bp_status bp_batch_apply_tuning(bp_batch *batch, const bp_tune_bin *bins, uint32_t count);
 /* Pins a world to a class (-1 = automatic); used by the autotuner. */
 bp_status bp_batch_set_world_class(bp_batch *batch, uint32_t world, int32_t cls);
+/* Workgroup pools of bp_batch_step (on by default). No reference counterpart. Once a step of every
+ * world has measured their device times, each LDS kernel class is launched with a share of the
+ * CUs in proportion to its measured work instead of one workgroup per world: its workgroups take
+ * the class's worlds longest first, the next one when they finish one. Results are the same bit
+ * for bit either way; only the placement of worlds on CUs changes. 0 turns pools off. */
+bp_status bp_batch_set_pooling(bp_batch *batch, uint32_t enabled);
 
 /* ------------------------------------------------------------------------ */
 /* Serving: continuous batching over resident worlds. No reference counterpart. */
