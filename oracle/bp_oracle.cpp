@@ -40,6 +40,8 @@ namespace bpo {
 //   4 edge-side assertion: the side after the move taken at the step's start point;
 //   5 body prediction: the angular velocity ignored.
 int g_plant = 0;
+// Host threads of the per-bubble phases (1 = the plain sequential restatement).
+int g_threads = 1;
 
 // ---------------------------------------------------------------------------------------
 // Scalar helpers. Float uses the deterministic transcendentals shared with the kernels;
@@ -383,6 +385,7 @@ template <class T> struct World {
     // Working state of the step (NativeMetalPersistentState buffers).
     std::vector<T> margin;
     std::vector<V2<T>> ref, a0, acc, cforce;
+    std::vector<T> reduceTerms;                 // per-bubble terms of one h reduction pass
     std::vector<std::vector<Edge>> adjacency;
     std::vector<uint32_t> demanded;
     std::vector<Candidate> candidates;        // compacted candidate list
@@ -529,8 +532,10 @@ template <class T> struct World {
             insert_candidate(row, dropped, limit, Candidate{a.id, o.id, 0u, index, j});
     }
 
-    // Rest of the row: segments, split of the row, overflow counters, Geometry:276-298.
-    void finish_bubble_row(uint32_t index, std::vector<Candidate> &row, uint32_t dropped, std::vector<uint32_t> &rowCounts) {
+    // Rest of the row: segments, split of the row, overflow counters, Geometry:276-298. The
+    // counters are added by the caller (sums and maxima, so the order of rows does not matter).
+    void finish_bubble_row(uint32_t index, std::vector<Candidate> &row, uint32_t dropped, std::vector<uint32_t> &rowCounts,
+                           uint32_t &droppedTotal, uint32_t &widest) {
         const Bubble<T> &a = b[index];
         for (uint32_t j = 0; j < s.size(); ++j) {
             const Segment<T> &seg = s[j];
@@ -545,8 +550,8 @@ template <class T> struct World {
         while (bubbleCount < count && row[bubbleCount].kind == 0) ++bubbleCount;
         rowCounts[index * 2] = bubbleCount;
         rowCounts[index * 2 + 1] = count - bubbleCount;
-        overflow[0] += dropped;
-        overflow[1] = std::max(overflow[1], count + dropped);
+        droppedTotal += dropped;
+        widest = std::max(widest, count + dropped);
     }
 
     static V2<T> floor_cell_pos(V2<T> p) { return p; }
@@ -573,6 +578,7 @@ template <class T> struct World {
             if (a.cy != c.cy) return a.cy < c.cy;
             return a.index < c.index;
         });
+        uint32_t droppedTotal = 0, widest = 0;
         for (uint32_t index = 0; index < n; ++index) {
             const Bubble<T> &a = b[index];
             rows[index].clear();
@@ -593,8 +599,10 @@ template <class T> struct World {
                     bubble_pair_candidate(index, j, rows[index], dropped);
                 }
             }
-            finish_bubble_row(index, rows[index], dropped, rowCounts);
+            finish_bubble_row(index, rows[index], dropped, rowCounts, droppedTotal, widest);
         }
+        overflow[0] += droppedTotal;
+        overflow[1] = std::max(overflow[1], widest);
     }
 
     // Hashed grid of the single-workgroup kernel, Force:999-1035, 1066-1108.
@@ -611,7 +619,12 @@ template <class T> struct World {
             cells[i] = cell_of(b[i].pos);
             bucket[group_bucket(cells[i].x, cells[i].y, buckets)].push_back(i);
         }
-        for (uint32_t index = 0; index < n; ++index) {
+        uint32_t droppedTotal = 0, widest = 0;
+        const int ni = (int)n;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(g_threads) if (g_threads > 1) \
+    reduction(+ : droppedTotal) reduction(max : widest)
+        for (int ii = 0; ii < ni; ++ii) {
+            const uint32_t index = (uint32_t)ii;
             const Bubble<T> &a = b[index];
             rows[index].clear();
             if (!finite2(a.pos) || !finite2(a.start)) { rowCounts[index * 2] = rowCounts[index * 2 + 1] = 0; continue; }
@@ -626,8 +639,10 @@ template <class T> struct World {
                         bubble_pair_candidate(index, j, rows[index], dropped);
                     }
                 }
-            finish_bubble_row(index, rows[index], dropped, rowCounts);
+            finish_bubble_row(index, rows[index], dropped, rowCounts, droppedTotal, widest);
         }
+        overflow[0] += droppedTotal;
+        overflow[1] = std::max(overflow[1], widest);
     }
 
     // Bubble-body candidate rows (mode "bubbles"), Geometry:350-392 and 410-415.
@@ -845,16 +860,27 @@ template <class T> struct World {
         }
         uint32_t n = (uint32_t)b.size();
         T fraction = step_fraction(sc);
+        // Each pass evaluates every bubble's term (in parallel with g_threads), then takes the
+        // extreme in bubble order: the lowest index wins ties, whatever the thread count.
+        std::vector<T> &terms = reduceTerms;
+        terms.resize(n);
+        const int ni = (int)n;
+#pragma omp parallel for schedule(static) num_threads(g_threads) if (g_threads > 1)
+        for (int i = 0; i < ni; ++i) terms[i] = omega_squared((uint32_t)i, fraction);
         T best = -1; uint32_t owner = 0;                                       // pass 1: max omega^2
-        for (uint32_t i = 0; i < n; ++i) { T v = omega_squared(i, fraction); if (v > best) { best = v; owner = i; } }
+        for (uint32_t i = 0; i < n; ++i) { T v = terms[i]; if (v > best) { best = v; owner = i; } }
         T omegaSquared = best; uint32_t omegaOwner = owner;
         T stiffnessBound = omegaSquared > 0 ? 1 / std::sqrt(omegaSquared) : inf<T>();
         T hMax = vmin(remaining, stiffnessBound);
+#pragma omp parallel for schedule(static) num_threads(g_threads) if (g_threads > 1)
+        for (int i = 0; i < ni; ++i) terms[i] = distance_bound((uint32_t)i, hMax, fraction);
         best = inf<T>(); owner = 0;                                            // pass 2: min d/4 term
-        for (uint32_t i = 0; i < n; ++i) { T v = distance_bound(i, hMax, fraction); if (v < best) { best = v; owner = i; } }
+        for (uint32_t i = 0; i < n; ++i) { T v = terms[i]; if (v < best) { best = v; owner = i; } }
         T distanceBoundV = best; uint32_t distanceOwner = owner;
+#pragma omp parallel for schedule(static) num_threads(g_threads) if (g_threads > 1)
+        for (int i = 0; i < ni; ++i) terms[i] = margin_bound((uint32_t)i);
         best = inf<T>(); owner = 0;                                            // pass 3: min margin term
-        for (uint32_t i = 0; i < n; ++i) { T v = margin_bound(i); if (v < best) { best = v; owner = i; } }
+        for (uint32_t i = 0; i < n; ++i) { T v = terms[i]; if (v < best) { best = v; owner = i; } }
         T marginBoundV = best; uint32_t marginOwner = owner;
         if (marginBoundV <= 0) {
             sc.counts[2] = 1;
@@ -1049,8 +1075,18 @@ template <class T> struct World {
         }
     }
 
+    // The per-bubble phases below run on g_threads host threads when it is above 1 (the per-phase
+    // CPU baseline): each iteration writes only its own bubble's slots and reads state no other
+    // iteration of the same loop writes, so any thread count gives the same bits.
     void contact_forces_all(T fraction) {
-        for (uint32_t i = 0; i < b.size(); ++i) cforce[i] = bubble_contact_force(i, fraction);
+        const int n = (int)b.size();
+#pragma omp parallel for schedule(static) num_threads(g_threads) if (g_threads > 1)
+        for (int i = 0; i < n; ++i) cforce[i] = bubble_contact_force((uint32_t)i, fraction);
+    }
+    void integrate_all(uint32_t phase) {
+        const int n = (int)b.size();
+#pragma omp parallel for schedule(static) num_threads(g_threads) if (g_threads > 1)
+        for (int i = 0; i < n; ++i) integrate_phase((uint32_t)i, phase);
     }
 
     // One step: executeSnapshot prologue (NativeMetalSolver.swift:467-498), step prefix
@@ -1079,7 +1115,7 @@ template <class T> struct World {
             broadphase();
             T fraction = step_fraction(sc);
             contact_forces_all(fraction);
-            for (uint32_t i = 0; i < n; ++i) integrate_phase(i, 2);
+            integrate_all(2);
             for (uint32_t iteration = 0; iteration < 2 * kFrameLimit + 2; ++iteration) {
                 reduce_substep();
                 T h = sc.time[2];
@@ -1092,12 +1128,12 @@ template <class T> struct World {
                     overflow[3] += 1;
                     broadphase();
                     contact_forces_all(step_fraction(sc));
-                    for (uint32_t i = 0; i < n; ++i) integrate_phase(i, 2);
+                    integrate_all(2);
                     continue;
                 }
-                for (uint32_t i = 0; i < n; ++i) integrate_phase(i, 0);
+                integrate_all(0);
                 contact_forces_all(step_fraction(sc));
-                for (uint32_t i = 0; i < n; ++i) integrate_phase(i, 1);
+                integrate_all(1);
                 if (cfg.injectionEnabled) {
                     for (auto &x : b)
                         if (sc.counts[1] == cfg.injectionAfter + 1 && x.id == cfg.injectionObject)
@@ -1606,6 +1642,7 @@ void bpo_candidate_rows(const bp_bubble *bubbles, const float *margins, uint32_t
 
 // Planted mistake for the tests that show a pin catches it (0 switches every plant off).
 void bpo_set_plant(int code) { bpo::g_plant = code; }
+void bpo_set_threads(int threads) { bpo::g_threads = threads < 1 ? 1 : threads; }
 
 // Contact-law entry points (the harness kernels testEpsilon / testWallLaw / testPairLaw of
 // Tests/BubblePhysicsNativeMetalTests/NativeMetalContactLawTests.swift:15-44).

@@ -65,6 +65,11 @@ def _p(a):
     return None if a is None or a.size == 0 else a.ctypes.data
 
 
+def set_threads(threads: int):
+    """Host threads of the oracle's per-bubble phases (1 = sequential); the results do not change."""
+    load().bpo_set_threads(int(threads))
+
+
 def set_plant(code: int):
     """Switches on a planted mistake of the oracle (0 = none); only for tests that show a pin fails."""
     load().bpo_set_plant(int(code))

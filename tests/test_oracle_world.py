@@ -383,3 +383,19 @@ def test_planted_field_rotation_mistake_is_caught():
     right = np.array([c * 30 - s * -40, s * 30 + c * -40])
     wrong = np.array([c * 30 + s * -40, -s * 30 + c * -40])
     assert np.linalg.norm(right - wrong) / np.linalg.norm(right) > 1e-4
+
+
+def test_per_phase_threads_give_the_same_bits(oracle):
+    # The per-phase CPU baseline runs the per-bubble phases and the candidate rows on several host
+    # threads; every thread count must give the sequential run's bits (state, frames, rows).
+    from bubblephysics_amd import scenes
+    for scene, steps in ((scenes.config1(), 12), (scenes.config2(), 6)):
+        runs = []
+        for threads in (1, 4):
+            oracle.set_threads(threads)
+            try:
+                cb, cbod, cf, ex = oracle.run(scene, steps, adjacency=True)
+            finally:
+                oracle.set_threads(1)
+            runs.append((cb.tobytes(), cbod.tobytes(), cf.tobytes(), ex["row_offsets"].tobytes(), ex["entries"].tobytes()))
+        assert runs[0] == runs[1]
