@@ -316,16 +316,7 @@ bp_status bp_batch::pack() {
     for (int k = 0; k < kClassCount; ++k)
         if ((s = upload(orders[k], classWorlds[k])) != BP_OK) return s;
     BP_HIP(hipMemcpy(arenaDev, &arena, sizeof arena, hipMemcpyHostToDevice));
-    for (int k = 0; k < kStreams; ++k) {
-        BP_HIP(hipStreamCreateWithFlags(&streams[k], hipStreamNonBlocking));
-        BP_HIP(hipEventCreate(&endEvents[k]));
-    }
-    for (int k = 0; k < kClassCount; ++k) {
-        BP_HIP(hipEventCreate(&classStart[k]));
-        BP_HIP(hipEventCreate(&classEnd[k]));
-    }
-    BP_HIP(hipEventCreate(&startEvent));
-    BP_HIP(hipEventCreateWithFlags(&gateEvent, hipEventDisableTiming));
+    if ((s = create_streams()) != BP_OK) return s;
     if (!deviceCUs) {
         int device = 0;
         BP_HIP(hipGetDevice(&device));
@@ -514,11 +505,29 @@ bp_status bp_batch::write_bubbles(uint32_t wi, const bp_bubble *bubbles, uint32_
         packed = false;
         return BP_OK;
     }
-    bpd::WorldDesc &d = hostDescs[wi];
+    bpd::WorldDesc d = hostDescs[wi];
     BP_HIP(hipMemcpy(&d, descs + wi, sizeof d, hipMemcpyDeviceToHost));   // the device side holds `pub`
-    // Motor and band records keep their ids; their bubble indices are resolved again (the
-    // device holds the records sorted by bubble id, as pack() wrote them).
     std::vector<int32_t> indices(2 * (size_t)count, -1);
+    motor_band_indices(wi, bubbles, count, indices.data());
+    if (count) {
+        BP_HIP(hipMemcpyAsync(xferBubbles, bubbles, count * sizeof(bp_bubble), hipMemcpyHostToDevice, streams[0]));
+        BP_HIP(hipMemcpyAsync(xferIndices, indices.data(), indices.size() * sizeof(int32_t), hipMemcpyHostToDevice, streams[0]));
+        hipError_t e = bp_launch_scatter_world(arenaDev, d.bubbleOffset, count, xferBubbles, xferIndices, xferIndices + count,
+                                               streams[0]);
+        if (e != hipSuccess) { set_error(std::string("scatter: ") + hipGetErrorString(e)); return BP_ERR_COMMAND_FAILED; }
+        BP_HIP(hipStreamSynchronize(streams[0]));
+    }
+    written_desc(wi, count, d);
+    BP_HIP(hipMemcpy(descs + wi, &d, sizeof d, hipMemcpyHostToDevice));
+    return adopt_written(wi, bubbles, count, d);
+}
+
+// Motor and band indices of a world's new bubble list (indices[i] and indices[count + i], -1 for
+// none): the records keep their bubble ids, and the device holds them sorted by bubble id, as
+// pack() wrote them.
+void bp_batch::motor_band_indices(uint32_t wi, const bp_bubble *bubbles, uint32_t count, int32_t *indices) const {
+    const HostWorld &w = worlds[wi];
+    for (size_t k = 0; k < 2 * (size_t)count; ++k) indices[k] = -1;
     std::vector<bp_motor> ms = w.motors;
     std::sort(ms.begin(), ms.end(), [](const bp_motor &a, const bp_motor &b) { return a.bubble_id < b.bubble_id; });
     std::vector<bp_band> bs = w.bands;
@@ -530,19 +539,22 @@ bp_status bp_batch::write_bubbles(uint32_t wi, const bp_bubble *bubbles, uint32_
         auto b = std::lower_bound(bs.begin(), bs.end(), id, [](const bp_band &r, int64_t v) { return r.bubble_id < v; });
         if (b != bs.end() && b->bubble_id == id) indices[count + i] = (int32_t)(b - bs.begin());
     }
-    if (count) {
-        BP_HIP(hipMemcpyAsync(xferBubbles, bubbles, count * sizeof(bp_bubble), hipMemcpyHostToDevice, streams[0]));
-        BP_HIP(hipMemcpyAsync(xferIndices, indices.data(), indices.size() * sizeof(int32_t), hipMemcpyHostToDevice, streams[0]));
-        hipError_t e = bp_launch_scatter_world(arenaDev, d.bubbleOffset, count, xferBubbles, xferIndices, xferIndices + count,
-                                               streams[0]);
-        if (e != hipSuccess) { set_error(std::string("scatter: ") + hipGetErrorString(e)); return BP_ERR_COMMAND_FAILED; }
-        BP_HIP(hipStreamSynchronize(streams[0]));
-    }
+}
+
+// The descriptor of a world whose list was rewritten in place with `count` bubbles.
+void bp_batch::written_desc(uint32_t wi, uint32_t count, bpd::WorldDesc &d) const {
+    const HostWorld &w = worlds[wi];
     d.bubbleCount = count;
     d.buckets = w.config.grid_bucket_count == 0 ? auto_buckets(count) : std::min(w.config.grid_bucket_count, bpd::kMaxBuckets);
     d.failed = 0;
+}
+
+// Host bookkeeping of a world whose list was rewritten in place on the device (its scatter and
+// descriptor `d` are complete).
+bp_status bp_batch::adopt_written(uint32_t wi, const bp_bubble *bubbles, uint32_t count, const bpd::WorldDesc &d) {
+    HostWorld &w = worlds[wi];
+    hostDescs[wi] = d;
     w.failed = false;
-    BP_HIP(hipMemcpy(descs + wi, &d, sizeof d, hipMemcpyHostToDevice));
     w.bubbles.assign(bubbles, bubbles + count);
     // A world that no longer fits its kernel configuration moves to another one: only the order
     // lists of the two configurations change. A world without cooperative storage takes the

@@ -213,7 +213,10 @@ struct bp_batch {
     int64_t *dropIds = nullptr;
     size_t dropCapacity = 0, dropIdCapacity = 0;
 
-    ~bp_batch() { release(); }
+    ~bp_batch() {
+        release();
+        destroy_streams();
+    }
 
     void release_graph() {
         if (graphExec) (void)hipGraphExecDestroy(graphExec);
@@ -221,13 +224,22 @@ struct bp_batch {
         graphExec = nullptr; graph = nullptr; graphSteps = 0;
     }
 
-    void release() {
-        release_graph();
-        if (forkEvent) (void)hipEventDestroy(forkEvent);
-        forkEvent = nullptr;
-        for (auto &e : joinEvents) { if (e) (void)hipEventDestroy(e); e = nullptr; }
-        for (void *p : allocations) (void)hipFree(p);
-        allocations.clear();
+    // Streams and events live as long as the batch (a repack keeps them, so a server that launches
+    // on the batch's streams keeps valid handles across its repacks).
+    bp_status create_streams() {
+        for (int k = 0; k < bph::kStreams; ++k) {
+            if (!streams[k]) BP_HIP(hipStreamCreateWithFlags(&streams[k], hipStreamNonBlocking));
+            if (!endEvents[k]) BP_HIP(hipEventCreate(&endEvents[k]));
+        }
+        for (int k = 0; k < bph::kClassCount; ++k) {
+            if (!classStart[k]) BP_HIP(hipEventCreate(&classStart[k]));
+            if (!classEnd[k]) BP_HIP(hipEventCreate(&classEnd[k]));
+        }
+        if (!startEvent) BP_HIP(hipEventCreate(&startEvent));
+        if (!gateEvent) BP_HIP(hipEventCreateWithFlags(&gateEvent, hipEventDisableTiming));
+        return BP_OK;
+    }
+    void destroy_streams() {
         for (int k = 0; k < bph::kStreams; ++k) {
             if (streams[k]) (void)hipStreamDestroy(streams[k]);
             if (endEvents[k]) (void)hipEventDestroy(endEvents[k]);
@@ -242,6 +254,15 @@ struct bp_batch {
         if (gateEvent) (void)hipEventDestroy(gateEvent);
         gateEvent = nullptr;
         startEvent = nullptr;
+    }
+
+    void release() {
+        release_graph();
+        if (forkEvent) (void)hipEventDestroy(forkEvent);
+        forkEvent = nullptr;
+        for (auto &e : joinEvents) { if (e) (void)hipEventDestroy(e); e = nullptr; }
+        for (void *p : allocations) (void)hipFree(p);
+        allocations.clear();
         stats = nullptr; statsCapacity = 0; descs = nullptr; arenaDev = nullptr; subsetOrders = nullptr; cursors = nullptr;
         xferBubbles = nullptr; xferIndices = nullptr; xferCapacity = 0;
         dropRecs = nullptr; dropIds = nullptr; dropCapacity = dropIdCapacity = 0;
@@ -271,6 +292,9 @@ struct bp_batch {
     bp_status download_state(bool full = false);
     bp_status download_world(uint32_t wi);
     bp_status write_bubbles(uint32_t wi, const bp_bubble *bubbles, uint32_t count);
+    void motor_band_indices(uint32_t wi, const bp_bubble *bubbles, uint32_t count, int32_t *indices) const;
+    void written_desc(uint32_t wi, uint32_t count, bpd::WorldDesc &d) const;
+    bp_status adopt_written(uint32_t wi, const bp_bubble *bubbles, uint32_t count, const bpd::WorldDesc &d);
     uint64_t footprint(uint32_t wi) const;
     bp_status launch(const uint32_t *subset, uint32_t subsetCount, uint32_t steps);
     void plan_pools();

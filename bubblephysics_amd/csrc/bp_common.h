@@ -568,4 +568,37 @@ BPD int32_t final_status(int32_t status, const Ctl &c, uint32_t safetyAll, bool 
 
 }  // namespace bpd
 
+namespace bpd {
+
+// Record i of a served world's published state into its mirror in pinned host memory (the server
+// reads it once the launch has completed, for the host staging of add, merge and split).
+BPD void mirror_record(const Arena *A, const WorldDesc *d, uint32_t side, uint32_t i, float2 pos, float2 vel, float rot,
+                       float om) {
+    const uint32_t g = d->bubbleOffset + i;
+    bp_bubble r;
+    r.id = A->id[g];
+    r.center[0] = pos.x; r.center[1] = pos.y;
+    const float2 st = A->start[side][g];
+    r.previous_center[0] = st.x; r.previous_center[1] = st.y;
+    r.velocity[0] = vel.x; r.velocity[1] = vel.y;
+    r.rotation = rot;
+    r.angular_velocity = om;
+    r.mass = A->mass[g];
+    r.inverse_mass = A->invm[g];
+    r.radius = A->radius[g];
+    r.contact_stiffness = A->Kc[g];
+    r.damping = A->gamma[g];
+    r.flags = A->flags[g];
+    r.category = A->cat[g];
+    r.mask = A->mask[g];
+    static_cast<bp_bubble *>(d->mirror)[i] = r;
+}
+
+// The bodies' published poses into the mirror (lanes `lane`, `lane + width`, ...).
+BPD void mirror_bodies(const Arena *A, const WorldDesc *d, uint32_t side, uint32_t lane, uint32_t width) {
+    for (uint32_t k = lane; k < d->bodyCount; k += width) d->bodyMirror[k] = A->bodyPose[side][d->bodyOffset + k];
+}
+
+}  // namespace bpd
+
 #endif  // BP_COMMON_H

@@ -136,6 +136,15 @@ struct WorldDesc {
                               // deleted (kWorldRunnable = 0 otherwise)
     uint32_t exportAdjacency; // 1 = leave the rows of the last grid build in rowStart/rowLen/adj
     uint32_t bubbleCapacity;  // bubble slots of the world's range (transactions never grow past it)
+    // Serving mirror in pinned host memory (device addresses; null outside a server): after every
+    // published step the kernel writes the world's records (bp_bubble) and its bodies' poses there,
+    // and a replacement transaction reads the world's new records and their motor and band indices
+    // (2 x mirrorCapacity) from there.
+    void *mirror;
+    float4 *bodyMirror;
+    int32_t *mirrorIdx;
+    uint32_t mirrorCapacity;
+    uint32_t reserved0;
 };
 
 enum : uint32_t { kWorldRunnable = 0u, kWorldFailed = 1u, kWorldHeld = 2u };
@@ -147,6 +156,9 @@ struct TxRecord {
     uint32_t world, kind, index, buckets;
     float impulse[2], direction[2];
 };
+// TxRecord.kind of a replacement: the world's records become the `index` records of its mirror
+// (an add, merge or split the host staged on the mirror of the published state).
+constexpr uint32_t kTxReplace = 100u;
 
 // A world held after a step that removed bubbles from the region (kWorldHeld): its removed
 // bubbles are deleted on the device before it steps again. `buckets` is the grid bucket count for

@@ -46,6 +46,8 @@ struct Pending {
     int64_t sink;            // index into the trace results, or -1 for the poll queue
     bool staged = false;     // the transaction was decided on the host (applied or refused)
     uint32_t txResult = 0;   // its outcome (BP_REJECT_*) when staged
+    bool replace = false;    // staged on the mirror: the launch replaces the world's records
+    uint32_t replaceCount = 0;
 };
 
 struct Launch {
@@ -64,7 +66,25 @@ struct Launch {
     bpd::TxRecord *hTx = nullptr, *dTx = nullptr;
 };
 
+// An add, merge or split being staged on the host: the world's published records are read back
+// (phase 1), the reference's staging runs on the host, and the staged list is written back in
+// place (phase 2), each phase ending with an event the dispatcher polls, so a staging world never
+// holds up the launches of the others. Buffers grow with the largest world staged.
+struct StageJob {
+    uint32_t world = 0;
+    int phase = 0;                      // 0 free, 1 reading the world, 2 writing the staged list
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    bp_bubble *dRecs = nullptr, *hRecs = nullptr;        // device and pinned host records
+    int32_t *dIdx = nullptr, *hIdx = nullptr;            // motor and band indices of the staged list
+    bpd::WorldDesc *hDesc = nullptr;                     // pinned host descriptor
+    float4 *hPose = nullptr;                             // body poses of both published sides
+    uint32_t capacity = 0, poseCapacity = 0;
+    std::vector<bp_bubble> staged;
+};
+
 constexpr int kMaxLaunches = 64;
+constexpr int kStageJobs = 16;
 // Classes a served world may run in: the LDS configurations and the one-workgroup global kernel
 // (a cooperative grid needs the whole device and would stall every other launch).
 const int kServeClasses[] = {0, 1, 2, 3, 4, 7, 9};
@@ -111,9 +131,27 @@ struct bp_server {
     int idleStreams = 0;
     bpd::WorldStatus *hostStats = nullptr;      // one record per world in pinned host memory (each world is
     bpd::WorldStatus *stats = nullptr;          // in one launch at a time); `stats` is its device address
+    size_t ownedFrom = 0;                       // streams[ownedFrom..] were created by the server
     hipStream_t readStream = nullptr;
-    bp_bubble *readBuffer = nullptr;
+    hipStream_t stageStream = nullptr;          // the staging jobs' copies and kernels
+    bp_bubble *readBuffer = nullptr;            // bp_server_read_bubbles
     uint32_t readCapacity = 0;
+    StageJob stageJobs[kStageJobs];
+    std::vector<int> freeStage, activeStage;
+    std::deque<uint32_t> stageWaiting;          // worlds whose staging waits for a free job
+    std::vector<uint8_t> staging;               // per world: its head transaction is being staged
+    // Mirrors of the worlds' published records and body poses in pinned, mapped host memory: the
+    // step kernels write them after every published step, so an add, merge or split is staged on
+    // the host from the mirror, written into the mirror and applied by the launch's transaction
+    // kernel, with no device round trip of its own. A world whose mirror is not current (after a
+    // repack deleted region-removed bubbles, or beyond its mirror's slots) is staged through the
+    // device instead (begin_stage).
+    bp_bubble *mirrorHost = nullptr, *mirrorDev = nullptr;
+    int32_t *mirrorIdxHost = nullptr, *mirrorIdxDev = nullptr;
+    float4 *bodyMirrorHost = nullptr, *bodyMirrorDev = nullptr;
+    std::vector<uint64_t> mirrorOffset, bodyMirrorOffset;
+    std::vector<uint32_t> mirrorCap;
+    std::vector<uint8_t> mirrorValid, mirrorAttached;
     std::vector<void *> deviceAllocs, hostAllocs;
 
     double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
@@ -125,7 +163,15 @@ struct bp_server {
     void prepare_head(uint32_t w);
     void finish_request(const Pending &p, bp_status status, uint32_t substeps, uint32_t bubbles, uint32_t launchWorlds,
                         int cls, float deviceMs, uint32_t txResult = 0);
-    bp_status stage_structural(uint32_t w, Pending &p);
+    bp_status attach_mirrors();
+    void detach_mirrors();
+    bp_status stage_from_mirror(uint32_t w, Pending &p);
+    bp_status stage_scene(uint32_t w, const Pending &p, const bp_bubble *recs, uint32_t n, const float4 *poses,
+                          std::vector<bp_bubble> &out, uint32_t &outCount, bp_stage_rejection &why);
+    bp_status begin_stage(uint32_t w);
+    bp_status stage_on_host(StageJob &job);
+    bool poll_stages();
+    void free_stage_job(int j);
     bp_status drop_removed(const std::vector<uint32_t> &ws);
     bool poll_launches();
     bool form_launches();
@@ -190,6 +236,7 @@ void bp_server::fail(bp_status s) {
 
 bp_status bp_server::init() {
     bp_batch &b = *batch;
+    bp_status s = BP_OK;
     W = (uint32_t)b.worlds.size();
     // Residency against the budget, in index order; the rest wait in host memory.
     resident.assign(W, 0);
@@ -210,17 +257,34 @@ bp_status bp_server::init() {
     }
     if (changed || !b.packed) {
         b.packed = false;
-        bp_status s = b.pack();
-        if (s != BP_OK) return s;
+        if ((s = b.pack()) != BP_OK) return s;
+    } else if ((s = b.download_state(true)) != BP_OK) {    // the host mirrors seed the serving mirrors
+        return s;
     }
     for (uint32_t w = 0; w < W; ++w) if (resident[w]) footprintOf[w] = b.footprint(w);
+    {
+        // Mirror slots per world: its device slots, or the slots a pack would give it.
+        mirrorOffset.assign(W, 0); bodyMirrorOffset.assign(W, 0); mirrorCap.assign(W, 0); mirrorValid.assign(W, 0);
+        mirrorAttached.assign(W, 0);
+        uint64_t records = 0, poses = 0;
+        for (uint32_t w = 0; w < W; ++w) {
+            const uint32_t n = (uint32_t)b.worlds[w].bubbles.size();
+            const uint32_t cap = std::max<uint32_t>(resident[w] ? b.hostDescs[w].bubbleCapacity : 0u, n + std::max<uint32_t>(8, n / 8));
+            mirrorOffset[w] = records; mirrorCap[w] = cap; records += cap;
+            bodyMirrorOffset[w] = poses; poses += b.worlds[w].bodies.size();
+        }
+        if ((s = host_alloc(this, mirrorHost, mirrorDev, records)) != BP_OK ||
+            (s = host_alloc(this, mirrorIdxHost, mirrorIdxDev, 2 * records)) != BP_OK ||
+            (s = host_alloc(this, bodyMirrorHost, bodyMirrorDev, std::max<uint64_t>(poses, 1))) != BP_OK)
+            return s;
+    }
     queue.assign(W, {});
     busy.assign(W, 0); queued.assign(W, 0); wantAdmission.assign(W, 0);
     worldFailed.assign(W, 0); dropPending.assign(W, 0);
     for (uint32_t w = 0; w < W; ++w) worldFailed[w] = b.worlds[w].failed ? 1 : 0;
     count.resize(W); headCount.assign(W, 0); headClass.assign(W, -1); lastUse.assign(W, 0);
     for (uint32_t w = 0; w < W; ++w) count[w] = (uint32_t)b.worlds[w].bubbles.size();
-    bp_status s = BP_OK;
+    if ((s = attach_mirrors()) != BP_OK) return s;
     {
         void *p = nullptr;
         hipError_t e = hipHostMalloc(&p, std::max<size_t>(W, 1) * sizeof(bpd::WorldStatus), hipHostMallocCoherent | hipHostMallocMapped);
@@ -242,23 +306,54 @@ bp_status bp_server::init() {
         freeLaunch.push_back(k);
     }
     if ((s = dev_alloc(this, dropDev, std::max<uint32_t>(W, 1))) != BP_OK) return s;
-    streams.resize(std::max<uint32_t>(1, cfg.streams));
+    // Streams: `cfg.streams` in all, each on a hardware queue of its own when the process has that
+    // many (GPU_MAX_HW_QUEUES): two for reads and staging, the rest for launches, starting with the
+    // batch's own streams (idle while the server drives the batch). Two streams on one hardware
+    // queue run in order, so a staging copy would otherwise wait behind a long step launch.
+    const uint32_t total = std::max<uint32_t>(3, cfg.streams), launchStreams = total - 2;
+    streams.clear();
+    for (int k = 0; k < kStreams && streams.size() < launchStreams; ++k) streams.push_back(b.streams[k]);
+    ownedFrom = streams.size();
+    while (streams.size() < launchStreams) {
+        hipStream_t st_ = nullptr;
+        BP_HIP(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
+        streams.push_back(st_);
+    }
     streamLoad.assign(streams.size(), 0);
     idleStreams = (int)streams.size();
-    for (auto &st_ : streams) BP_HIP(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
     BP_HIP(hipStreamCreateWithFlags(&readStream, hipStreamNonBlocking));
+    BP_HIP(hipStreamCreateWithFlags(&stageStream, hipStreamNonBlocking));
+    staging.assign(W, 0);
+    for (int j = 0; j < kStageJobs; ++j) {
+        stageJobs[j].stream = stageStream;
+        BP_HIP(hipEventCreateWithFlags(&stageJobs[j].done, hipEventDisableTiming));
+        freeStage.push_back(j);
+    }
     local.resident_bytes = used;
     st.resident_bytes = used;
     return BP_OK;
 }
 
 void bp_server::release() {
+    detach_mirrors();
     for (int k = 0; k < kMaxLaunches; ++k)
         if (launches[k].done) { (void)hipEventDestroy(launches[k].done); launches[k].done = nullptr; }
-    for (auto &s : streams) if (s) (void)hipStreamDestroy(s);
+    for (size_t k = ownedFrom; k < streams.size(); ++k) if (streams[k]) (void)hipStreamDestroy(streams[k]);
     streams.clear();
     if (readStream) (void)hipStreamDestroy(readStream);
     readStream = nullptr;
+    if (stageStream) (void)hipStreamDestroy(stageStream);
+    stageStream = nullptr;
+    for (StageJob &j : stageJobs) {
+        if (j.done) (void)hipEventDestroy(j.done);
+        if (j.dRecs) (void)hipFree(j.dRecs);
+        if (j.dIdx) (void)hipFree(j.dIdx);
+        if (j.hRecs) (void)hipHostFree(j.hRecs);
+        if (j.hIdx) (void)hipHostFree(j.hIdx);
+        if (j.hDesc) (void)hipHostFree(j.hDesc);
+        if (j.hPose) (void)hipHostFree(j.hPose);
+        j = StageJob{};
+    }
     for (void *p : deviceAllocs) (void)hipFree(p);
     for (void *p : hostAllocs) (void)hipHostFree(p);
     deviceAllocs.clear(); hostAllocs.clear();
@@ -273,7 +368,7 @@ void bp_server::enqueue(const Pending &p) {
         return;
     }
     queue[w].push_back(p);
-    if (!busy[w] && !queued[w]) prepare_head(w);
+    if (!busy[w] && !queued[w] && !staging[w]) prepare_head(w);
 }
 
 // Makes the world's oldest request ready: decides the transaction's effect on the bubble count
@@ -301,16 +396,30 @@ void bp_server::prepare_head(uint32_t w) {
         uint32_t next = n;
         const uint32_t kind = p.req.transaction;
         if ((kind == BP_TX_ADD || kind == BP_TX_MERGE || kind == BP_TX_SPLIT) && !p.staged && n > 0) {
-            // Structural transactions are staged here, while the world is idle: its published state
-            // is final, and the staged list is written back before the world's next launch.
-            bp_status s = stage_structural(w, p);
-            if (s == BP_ERR_CAPACITY_OVERFLOW) {
-                Pending done = p; queue[w].pop_front();
-                ++local.rejected;
-                finish_request(done, BP_ERR_CAPACITY_OVERFLOW, 0, n, 0, -1, 0.0f);
-                continue;
+            // Structural transactions are staged while the world is idle: its published state is
+            // final, and the staged list is written back before the world's next launch. The world
+            // comes back here when its staging has finished (poll_stages).
+            if (mirrorValid[w]) {
+                bp_status s = stage_from_mirror(w, p);
+                if (s == BP_ERR_CAPACITY_OVERFLOW) {
+                    Pending done = p; queue[w].pop_front();
+                    ++local.rejected;
+                    finish_request(done, BP_ERR_CAPACITY_OVERFLOW, 0, n, 0, -1, 0.0f);
+                    continue;
+                }
+                if (s != BP_OK) { fail(s); return; }
+            } else {
+                if (!staging[w]) {
+                    staging[w] = 1;
+                    bp_status s = begin_stage(w);
+                    if (s != BP_OK) { fail(s); return; }
+                }
+                return;
             }
-            if (s != BP_OK) { fail(s); return; }
+        }
+        if (p.replace) {
+            next = p.replaceCount;
+        } else if (kind == BP_TX_ADD || kind == BP_TX_MERGE || kind == BP_TX_SPLIT) {
             next = count[w];
         } else if (kind == BP_TX_REMOVE) {
             next = n > 1 ? n - 1 : n;
@@ -346,37 +455,113 @@ void bp_server::finish_request(const Pending &p, bp_status status, uint32_t subs
     cvDone.notify_all();
 }
 
-// Stages an add, merge or split on the host with the reference's rules (bp_stage_add,
-// bp_stage_merge, bp_stage_split: TransactionStager.swift:406-421, 630-735) against the world's
-// published records and its bodies at their current poses, then writes the staged list back in
-// place. A refused transaction leaves the world as it is (p.txResult holds the reason). Returns
-// BP_ERR_CAPACITY_OVERFLOW, changing nothing, when the result needs more slots than the world has.
-bp_status bp_server::stage_structural(uint32_t w, Pending &p) {
+// Phase 1 of staging an add, merge or split: the world's published records, its descriptor and
+// its bodies' poses are read back on a stream of the staging job (a job is taken when one is free;
+// otherwise the world waits its turn). Nothing here waits for the device.
+bp_status bp_server::begin_stage(uint32_t w) {
+    if (freeStage.empty()) { stageWaiting.push_back(w); return BP_OK; }
+    const int ji = freeStage.back();
+    freeStage.pop_back();
+    StageJob &job = stageJobs[ji];
     bp_batch &b = *batch;
-    HostWorld &hw = b.worlds[w];
-    const uint32_t n = count[w];
-    if (readCapacity < n) {
-        bp_status st = dev_alloc(this, readBuffer, n);
-        if (st != BP_OK) return st;
-        readCapacity = n;
+    const HostWorld &hw = b.worlds[w];
+    const uint32_t n = count[w], need = n + 2, nb = (uint32_t)hw.bodies.size();
+    if (job.capacity < need) {
+        if (job.dRecs) (void)hipFree(job.dRecs);
+        if (job.dIdx) (void)hipFree(job.dIdx);
+        if (job.hRecs) (void)hipHostFree(job.hRecs);
+        if (job.hIdx) (void)hipHostFree(job.hIdx);
+        job.dRecs = job.hRecs = nullptr; job.dIdx = job.hIdx = nullptr; job.capacity = 0;
+        const uint32_t cap = std::max<uint32_t>(need, 1024);
+        BP_HIP(hipMalloc(&job.dRecs, cap * sizeof(bp_bubble)));
+        BP_HIP(hipMalloc(&job.dIdx, 2 * (size_t)cap * sizeof(int32_t)));
+        BP_HIP(hipHostMalloc(&job.hRecs, cap * sizeof(bp_bubble), hipHostMallocDefault));
+        BP_HIP(hipHostMalloc(&job.hIdx, 2 * (size_t)cap * sizeof(int32_t), hipHostMallocDefault));
+        job.capacity = cap;
     }
-    std::vector<bp_bubble> recs(n);
-    hipError_t e = bp_launch_gather_full_world(b.arenaDev, b.descs + w, readBuffer, n, readStream);
+    if (!job.hDesc) BP_HIP(hipHostMalloc(&job.hDesc, sizeof(bpd::WorldDesc), hipHostMallocDefault));
+    if (job.poseCapacity < nb) {
+        if (job.hPose) (void)hipHostFree(job.hPose);
+        job.hPose = nullptr; job.poseCapacity = 0;
+        BP_HIP(hipHostMalloc(&job.hPose, 2 * (size_t)std::max<uint32_t>(nb, 64) * sizeof(float4), hipHostMallocDefault));
+        job.poseCapacity = std::max<uint32_t>(nb, 64);
+    }
+    job.world = w;
+    job.phase = 1;
+    hipError_t e = bp_launch_gather_full_world(b.arenaDev, b.descs + w, job.dRecs, n, job.stream);
     if (e != hipSuccess) { set_error(std::string("gather: ") + hipGetErrorString(e)); return BP_ERR_COMMAND_FAILED; }
-    BP_HIP(hipMemcpyAsync(recs.data(), readBuffer, n * sizeof(bp_bubble), hipMemcpyDeviceToHost, readStream));
-    bpd::WorldDesc d;
-    BP_HIP(hipMemcpyAsync(&d, b.descs + w, sizeof d, hipMemcpyDeviceToHost, readStream));
-    BP_HIP(hipStreamSynchronize(readStream));
+    BP_HIP(hipMemcpyAsync(job.hRecs, job.dRecs, n * sizeof(bp_bubble), hipMemcpyDeviceToHost, job.stream));
+    BP_HIP(hipMemcpyAsync(job.hDesc, b.descs + w, sizeof(bpd::WorldDesc), hipMemcpyDeviceToHost, job.stream));
+    if (nb) {
+        const uint32_t off = b.hostDescs[w].bodyOffset;
+        BP_HIP(hipMemcpyAsync(job.hPose, b.arena.bodyPose[0] + off, nb * sizeof(float4), hipMemcpyDeviceToHost, job.stream));
+        BP_HIP(hipMemcpyAsync(job.hPose + nb, b.arena.bodyPose[1] + off, nb * sizeof(float4), hipMemcpyDeviceToHost, job.stream));
+    }
+    BP_HIP(hipEventRecord(job.done, job.stream));
+    activeStage.push_back(ji);
+    return BP_OK;
+}
+
+// Phase 2: the reference's staging on the host (bp_stage_add, bp_stage_merge, bp_stage_split:
+// TransactionStager.swift:406-421, 630-735) against the world's published records and its bodies
+// at their current poses; a refused transaction leaves the world as it is (the request's
+// txResult holds the reason), one that needs more slots than the world has is rejected, and an
+// applied one is written back in place (scatter kernel and descriptor) on the job's stream.
+bp_status bp_server::stage_on_host(StageJob &job) {
+    bp_batch &b = *batch;
+    const uint32_t w = job.world;
+    HostWorld &hw = b.worlds[w];
+    Pending &p = queue[w].front();
+    const uint32_t n = count[w];
+    const uint32_t nb = (uint32_t)hw.bodies.size(), pub = job.hDesc->pub;
+    uint32_t outCount = 0;
+    bp_stage_rejection why{};
+    bp_status st = stage_scene(w, p, job.hRecs, n, job.hPose + (size_t)pub * nb, job.staged, outCount, why);
+    if (st == BP_ERR_INVALID_ARGUMENT && why.reason != BP_REJECT_NONE) {
+        p.staged = true;                               // refused: the world is unchanged, the step runs
+        p.txResult = why.reason;
+        job.phase = 0;
+        return BP_OK;
+    }
+    if (st != BP_OK) return st;
+    if (outCount > hw.capacity) {
+        --local.transactions;
+        ++local.rejected;
+        Pending done = p; queue[w].pop_front();
+        finish_request(done, BP_ERR_CAPACITY_OVERFLOW, 0, n, 0, -1, 0.0f);
+        job.phase = 0;
+        return BP_OK;
+    }
+    job.staged.resize(outCount);
+    std::memcpy(job.hRecs, job.staged.data(), outCount * sizeof(bp_bubble));
+    b.motor_band_indices(w, job.staged.data(), outCount, job.hIdx);
+    b.written_desc(w, outCount, *job.hDesc);
+    if (outCount) {
+        BP_HIP(hipMemcpyAsync(job.dRecs, job.hRecs, outCount * sizeof(bp_bubble), hipMemcpyHostToDevice, job.stream));
+        BP_HIP(hipMemcpyAsync(job.dIdx, job.hIdx, 2 * (size_t)outCount * sizeof(int32_t), hipMemcpyHostToDevice, job.stream));
+        hipError_t e = bp_launch_scatter_world(b.arenaDev, job.hDesc->bubbleOffset, outCount, job.dRecs, job.dIdx,
+                                               job.dIdx + outCount, job.stream);
+        if (e != hipSuccess) { set_error(std::string("scatter: ") + hipGetErrorString(e)); return BP_ERR_COMMAND_FAILED; }
+    }
+    BP_HIP(hipMemcpyAsync(b.descs + w, job.hDesc, sizeof(bpd::WorldDesc), hipMemcpyHostToDevice, job.stream));
+    BP_HIP(hipEventRecord(job.done, job.stream));
+    job.phase = 2;
+    return BP_OK;
+}
+
+// The reference's staging of the head request's add, merge or split (bp_stage_add,
+// bp_stage_merge, bp_stage_split: TransactionStager.swift:406-421, 630-735) against the world's
+// published records `recs` (n) and its bodies at the poses `poses` (x, y, angle per body); the
+// staged list goes to `out`. Counts the transaction.
+bp_status bp_server::stage_scene(uint32_t w, const Pending &p, const bp_bubble *recs, uint32_t n, const float4 *poses,
+                                 std::vector<bp_bubble> &out, uint32_t &outCount, bp_stage_rejection &why) {
+    const HostWorld &hw = batch->worlds[w];
     std::vector<bp_body> bodies = hw.bodies;
-    if (!bodies.empty()) {
-        std::vector<float4> pose(bodies.size());
-        BP_HIP(hipMemcpy(pose.data(), b.arena.bodyPose[d.pub] + d.bodyOffset, pose.size() * sizeof(float4), hipMemcpyDeviceToHost));
-        for (size_t k = 0; k < bodies.size(); ++k) {
-            bodies[k].pose[0] = pose[k].x; bodies[k].pose[1] = pose[k].y; bodies[k].pose[2] = pose[k].z;
-        }
+    for (size_t k = 0; k < bodies.size(); ++k) {
+        bodies[k].pose[0] = poses[k].x; bodies[k].pose[1] = poses[k].y; bodies[k].pose[2] = poses[k].z;
     }
     bp_stage_scene scene{};
-    scene.bubbles = recs.data();
+    scene.bubbles = recs;
     scene.bubble_count = n;
     scene.bodies = bodies.empty() ? nullptr : bodies.data();
     scene.body_count = (uint32_t)bodies.size();
@@ -385,10 +570,8 @@ bp_status bp_server::stage_structural(uint32_t w, Pending &p) {
     for (int k = 0; k < 4; ++k) scene.region[k] = hw.config.region[k];
     const uint32_t i = p.req.index % n;
     const int64_t newId = recs[n - 1].id + 1;
-    std::vector<bp_bubble> out(n + 2);
-    uint32_t outCount = 0;
-    bp_stage_rejection why{};
-    bp_status st = BP_OK;
+    out.resize(n + 2);
+    outCount = 0;
     ++local.transactions;
     if (p.req.transaction == BP_TX_ADD) {
         bp_bubble nb = recs[i];
@@ -398,33 +581,147 @@ bp_status bp_server::stage_structural(uint32_t w, Pending &p) {
         nb.center[1] = nb.previous_center[1] = recs[i].center[1] + offset * p.req.direction[1];
         nb.velocity[0] = nb.velocity[1] = 0.0f;
         nb.rotation = nb.angular_velocity = 0.0f;
-        st = bp_stage_add(&scene, &nb, out.data(), (uint32_t)out.size(), &outCount, &why);
-    } else if (p.req.transaction == BP_TX_MERGE) {
-        if (n < 2) {
-            st = BP_ERR_INVALID_ARGUMENT;
-            why.reason = BP_REJECT_INVALID_STRUCTURAL_OPERATION;
-        } else {
-            const uint32_t k = i + 1 < n ? i + 1 : i - 1, a = std::min(i, k), c = std::max(i, k);
-            const int64_t sources[2] = {recs[a].id, recs[c].id};
-            st = bp_stage_merge(&scene, sources, 2, &recs[a], 0, out.data(), (uint32_t)out.size(), &outCount, &why);
-        }
-    } else {
-        bp_bubble first = recs[i], second = recs[i];
-        second.id = newId;
-        st = bp_stage_split(&scene, recs[i].id, &first, 0.5f, &second, 0.5f, out.data(), (uint32_t)out.size(), &outCount, &why);
+        return bp_stage_add(&scene, &nb, out.data(), (uint32_t)out.size(), &outCount, &why);
     }
+    if (p.req.transaction == BP_TX_MERGE) {
+        if (n < 2) {
+            why.reason = BP_REJECT_INVALID_STRUCTURAL_OPERATION;
+            return BP_ERR_INVALID_ARGUMENT;
+        }
+        const uint32_t k = i + 1 < n ? i + 1 : i - 1, a = std::min(i, k), c = std::max(i, k);
+        const int64_t sources[2] = {recs[a].id, recs[c].id};
+        return bp_stage_merge(&scene, sources, 2, &recs[a], 0, out.data(), (uint32_t)out.size(), &outCount, &why);
+    }
+    bp_bubble first = recs[i], second = recs[i];
+    second.id = newId;
+    return bp_stage_split(&scene, recs[i].id, &first, 0.5f, &second, 0.5f, out.data(), (uint32_t)out.size(), &outCount, &why);
+}
+
+// Stages the head request's add, merge or split from the world's mirror (its published state,
+// current once its last launch has completed) and writes the staged list and its motor and band
+// indices into the mirror, for the launch's transaction kernel to apply before the step.
+bp_status bp_server::stage_from_mirror(uint32_t w, Pending &p) {
+    const HostWorld &hw = batch->worlds[w];
+    const uint32_t n = count[w], cap = mirrorCap[w];
+    std::vector<bp_bubble> out;
+    uint32_t outCount = 0;
+    bp_stage_rejection why{};
+    bp_status st = stage_scene(w, p, mirrorHost + mirrorOffset[w], n, bodyMirrorHost + bodyMirrorOffset[w], out, outCount, why);
     p.staged = true;
     if (st == BP_ERR_INVALID_ARGUMENT && why.reason != BP_REJECT_NONE) {
         p.txResult = why.reason;                       // refused: the world is unchanged, the step runs
         return BP_OK;
     }
     if (st != BP_OK) return st;
-    if (outCount > hw.capacity) { p.staged = false; --local.transactions; return BP_ERR_CAPACITY_OVERFLOW; }
-    st = b.write_bubbles(w, out.data(), outCount);
-    if (st != BP_OK) return st;
-    count[w] = outCount;
+    if (outCount > hw.capacity || outCount > cap) {
+        p.staged = false;
+        --local.transactions;
+        return BP_ERR_CAPACITY_OVERFLOW;
+    }
+    std::memcpy(mirrorHost + mirrorOffset[w], out.data(), outCount * sizeof(bp_bubble));
+    std::vector<int32_t> idx(2 * (size_t)outCount);
+    batch->motor_band_indices(w, out.data(), outCount, idx.data());
+    int32_t *dst = mirrorIdxHost + 2 * mirrorOffset[w];
+    for (uint32_t k = 0; k < outCount; ++k) { dst[k] = idx[k]; dst[cap + k] = idx[outCount + k]; }
     p.txResult = BP_REJECT_NONE;
+    p.replace = true;
+    p.replaceCount = outCount;
     return BP_OK;
+}
+
+// Points every resident world's descriptor at its mirror and fills the mirror from the batch's
+// host copy of the world (current after a pack or a read-back); a world beyond its mirror's
+// slots gets none. Runs with nothing in flight.
+bp_status bp_server::attach_mirrors() {
+    bp_batch &b = *batch;
+    BP_HIP(hipMemcpy(b.hostDescs.data(), b.descs, W * sizeof(bpd::WorldDesc), hipMemcpyDeviceToHost));
+    for (uint32_t w = 0; w < W; ++w) {
+        bpd::WorldDesc &d = b.hostDescs[w];
+        const HostWorld &hw = b.worlds[w];
+        const uint32_t n = (uint32_t)hw.bubbles.size();
+        mirrorValid[w] = 0;
+        mirrorAttached[w] = 0;
+        d.mirror = nullptr; d.bodyMirror = nullptr; d.mirrorIdx = nullptr; d.mirrorCapacity = 0;
+        // A resident world's head request staged on its mirror loses its staged list here (the
+        // mirror is refilled or dropped): it is staged again from the new state.
+        if (resident[w] && !queue.empty() && !queue[w].empty() && queue[w].front().replace) {
+            Pending &p = queue[w].front();
+            p.staged = false; p.replace = false; p.replaceCount = 0;
+            --local.transactions;
+        }
+        if (!resident[w] || n > mirrorCap[w] || n != d.bubbleCount) continue;
+        mirrorAttached[w] = 1;
+        d.mirror = mirrorDev + mirrorOffset[w];
+        d.bodyMirror = bodyMirrorDev + bodyMirrorOffset[w];
+        d.mirrorIdx = mirrorIdxDev + 2 * mirrorOffset[w];
+        d.mirrorCapacity = mirrorCap[w];
+        if (n) std::memcpy(mirrorHost + mirrorOffset[w], hw.bubbles.data(), n * sizeof(bp_bubble));
+        for (size_t k = 0; k < hw.bodies.size(); ++k) {
+            const bp_body &bd = hw.bodies[k];
+            bodyMirrorHost[bodyMirrorOffset[w] + k] = make_float4(bd.pose[0], bd.pose[1], bd.pose[2], bd.pose[2]);
+        }
+        mirrorValid[w] = 1;
+    }
+    BP_HIP(hipMemcpy(b.descs, b.hostDescs.data(), W * sizeof(bpd::WorldDesc), hipMemcpyHostToDevice));
+    return BP_OK;
+}
+
+// Clears the mirror addresses from the descriptors before the mirrors are freed.
+void bp_server::detach_mirrors() {
+    if (!mirrorHost || !batch || !batch->packed || !batch->descs) return;
+    bp_batch &b = *batch;
+    if (hipMemcpy(b.hostDescs.data(), b.descs, W * sizeof(bpd::WorldDesc), hipMemcpyDeviceToHost) != hipSuccess) return;
+    for (auto &d : b.hostDescs) { d.mirror = nullptr; d.bodyMirror = nullptr; d.mirrorIdx = nullptr; d.mirrorCapacity = 0; }
+    (void)hipMemcpy(b.descs, b.hostDescs.data(), W * sizeof(bpd::WorldDesc), hipMemcpyHostToDevice);
+}
+
+void bp_server::free_stage_job(int j) {
+    StageJob &job = stageJobs[j];
+    staging[job.world] = 0;
+    job.phase = 0;
+    freeStage.push_back(j);
+}
+
+// Advances the staging jobs whose event has completed; a world whose staging ended goes back to
+// prepare_head (its request then runs as an ordinary step). Starts the staging of waiting worlds
+// as jobs free up.
+bool bp_server::poll_stages() {
+    bool any = false;
+    for (size_t k = 0; k < activeStage.size();) {
+        const int ji = activeStage[k];
+        StageJob &job = stageJobs[ji];
+        hipError_t e = hipEventQuery(job.done);
+        if (e == hipErrorNotReady) { ++k; continue; }
+        if (e != hipSuccess) { set_error(std::string("hipEventQuery: ") + hipGetErrorString(e)); fail(BP_ERR_COMMAND_FAILED); return false; }
+        const uint32_t w = job.world;
+        if (job.phase == 1) {
+            bp_status s = stage_on_host(job);
+            if (s != BP_OK) { fail(s); return false; }
+            if (job.phase == 2) { ++k; any = true; continue; }           // written back next
+        } else {
+            const uint32_t outCount = (uint32_t)job.staged.size();
+            bp_status s = batch->adopt_written(w, job.staged.data(), outCount, *job.hDesc);
+            if (s != BP_OK) { fail(s); return false; }
+            count[w] = outCount;
+            Pending &p = queue[w].front();
+            p.staged = true;
+            p.txResult = BP_REJECT_NONE;
+        }
+        activeStage[k] = activeStage.back();
+        activeStage.pop_back();
+        free_stage_job(ji);
+        prepare_head(w);
+        any = true;
+    }
+    while (!freeStage.empty() && !stageWaiting.empty()) {
+        const uint32_t w = stageWaiting.front();
+        stageWaiting.pop_front();
+        if (!resident[w] || queue[w].empty()) { staging[w] = 0; prepare_head(w); continue; }
+        bp_status s = begin_stage(w);
+        if (s != BP_OK) { fail(s); return false; }
+        any = true;
+    }
+    return any;
 }
 
 // Deletes, on the device, the bubbles the last step of each listed world removed from the region
@@ -444,6 +741,7 @@ bp_status bp_server::drop_removed(const std::vector<uint32_t> &ws) {
     for (uint32_t w : ws) {
         count[w] = count[w] >= dropPending[w] ? count[w] - dropPending[w] : 0u;
         dropPending[w] = 0;
+        mirrorValid[w] = 0;                          // the mirror still holds the deleted bubbles
     }
     return BP_OK;
 }
@@ -481,6 +779,11 @@ bool bp_server::poll_launches() {
         for (uint32_t j = 0; j < L.count; ++j) {
             const uint32_t w = L.worlds[j];
             busy[w] = 0;
+            // A published step without region removals leaves the mirror current (the kernels write
+            // it when the world's descriptor holds one and the world fits it); any other outcome
+            // leaves it stale (an unpublished state, or bubbles the drop deleted).
+            const bpd::WorldStatus &r = hostStats[w];
+            mirrorValid[w] = mirrorAttached[w] && r.executed && r.status == BP_OK && r.removals == 0 && count[w] <= mirrorCap[w];
             prepare_head(w);
         }
         --inflightPerClass[L.cls];
@@ -513,7 +816,11 @@ bp_status bp_server::start_launch(int cls) {
         queue[w].pop_front();
         hostStats[w].txResult = BP_REJECT_NONE;
         const uint32_t kind = p.req.transaction;
-        if ((kind == BP_TX_IMPULSE || kind == BP_TX_REMOVE) && count[w] > 0) {
+        if (p.replace) {
+            bpd::TxRecord &t = L.hTx[L.txCount++];
+            t.world = w; t.kind = bpd::kTxReplace; t.index = p.replaceCount; t.buckets = buckets_for(w, p.replaceCount);
+            t.impulse[0] = t.impulse[1] = 0.0f; t.direction[0] = t.direction[1] = 0.0f;
+        } else if ((kind == BP_TX_IMPULSE || kind == BP_TX_REMOVE) && count[w] > 0) {
             ++local.transactions;
             if (kind == BP_TX_IMPULSE && !(std::isfinite(p.req.impulse[0]) && std::isfinite(p.req.impulse[1]))) {
                 p.staged = true;                             // requireFinite(impulse): refused on the host
@@ -611,6 +918,7 @@ bp_status bp_server::admit_waiting() {
     b.packed = false;
     if ((s = b.pack()) != BP_OK) return s;
     ++local.repacks;
+    if ((s = attach_mirrors()) != BP_OK) return s;
     for (uint32_t w = 0; w < W; ++w) {
         count[w] = (uint32_t)b.worlds[w].bubbles.size();
         dropPending[w] = 0;                          // the repack deleted the region-removed bubbles
@@ -637,7 +945,7 @@ bp_status bp_server::admit_waiting() {
 // forces the next admission to evict a world that still has requests.
 bool bp_server::form_launches() {
     if (!admissions.empty() && !launchAfterAdmission) {
-        if (!inflight.empty()) return false;        // drain first: the repack moves every world
+        if (!inflight.empty() || !activeStage.empty()) return false;   // drain first: the repack moves every world
         bp_status s = admit_waiting();
         if (s != BP_OK) { fail(s); return false; }
         launchAfterAdmission = true;
@@ -673,7 +981,7 @@ void bp_server::run() {
         {
             std::unique_lock<std::mutex> lock(mu);
             if (fatal != BP_OK) break;
-            bool idle = inflight.empty();
+            bool idle = inflight.empty() && activeStage.empty();
             if (idle)
                 for (int k = 0; k < kClassCount && idle; ++k) idle = ready[k].empty();
             idle = idle && admissions.empty();
@@ -685,6 +993,7 @@ void bp_server::run() {
         }
         for (const Pending &p : taken) enqueue(p);
         bool progressed = poll_launches();
+        progressed |= poll_stages();
         {
             std::lock_guard<std::mutex> lock(mu);
             if (fatal != BP_OK) break;
@@ -694,10 +1003,11 @@ void bp_server::run() {
             }
         }
         progressed |= form_launches();
-        if (!progressed && !inflight.empty()) std::this_thread::yield();
+        if (!progressed && (!inflight.empty() || !activeStage.empty())) std::this_thread::yield();
     }
     // Drain whatever is still on the device before the dispatcher ends.
     for (int id : inflight) (void)hipEventSynchronize(launches[id].done);
+    for (int j : activeStage) (void)hipEventSynchronize(stageJobs[j].done);
     poll_launches();
     std::lock_guard<std::mutex> lock(mu);
     publish_counters();

@@ -125,6 +125,29 @@ __global__ void transaction_kernel(const Arena *__restrict__ A, WorldDesc *descs
         }
     } else if (tx.kind == BP_TX_REMOVE) {
         if (n > 1) { remove_slot(A, p, o, i, n); newCount = n - 1; }
+    } else if (tx.kind == kTxReplace && d->mirror && tx.index <= d->mirrorCapacity && tx.index <= d->bubbleCapacity) {
+        // The host staged an add, merge or split on the mirror: its records replace the world's,
+        // on both sides, with their motor and band indices.
+        const bp_bubble *in = static_cast<const bp_bubble *>(d->mirror);
+        const uint32_t count = tx.index, cap = d->mirrorCapacity;
+        for (uint32_t k = threadIdx.x; k < count; k += blockDim.x) {
+            const bp_bubble b = in[k];
+            const uint32_t g = o + k;
+            for (int side = 0; side < 2; ++side) {
+                A->pos[side][g] = make_float2(b.center[0], b.center[1]);
+                A->start[side][g] = make_float2(b.previous_center[0], b.previous_center[1]);
+                A->vel[side][g] = make_float2(b.velocity[0], b.velocity[1]);
+                A->rot[side][g] = b.rotation;
+                A->omega[side][g] = b.angular_velocity;
+                A->region[side][g] = 0u;
+            }
+            A->id[g] = b.id;
+            A->mass[g] = b.mass; A->invm[g] = b.inverse_mass; A->radius[g] = b.radius; A->Kc[g] = b.contact_stiffness;
+            A->gamma[g] = b.damping; A->flags[g] = b.flags; A->cat[g] = b.category; A->mask[g] = b.mask;
+            A->motor[g] = d->mirrorIdx[k];
+            A->band[g] = d->mirrorIdx[cap + k];
+        }
+        newCount = count;
     }
     __syncthreads();
     if (threadIdx.x == 0) {

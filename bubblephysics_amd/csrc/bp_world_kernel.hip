@@ -535,6 +535,10 @@ __global__ __launch_bounds__(BLOCK) void world_step_kernel(const Arena *__restri
         else if (claimed) status = BP_ERR_CONTACT_VIOLATION;
         else if (n > 0 && !finished) status = BP_ERR_COMMAND_FAILED;   // loop ended without finishing
     }
+    if (d->mirror && status == BP_OK && n <= d->mirrorCapacity) {
+        for (uint32_t i = threadIdx.x; i < n; i += BLOCK) mirror_record(A, d, wside, i, W.pos[i], W.vel[i], W.rot[i], W.omega[i]);
+        mirror_bodies(A, d, wside, threadIdx.x, BLOCK);
+    }
     if (threadIdx.x == 0) {
         st->status = status;
         st->substeps = c.c1;

@@ -691,7 +691,7 @@ __device__ __attribute__((always_inline)) void lds_world_steps(LdsWorld<BLOCK, P
                 if (!valid[s]) continue;
                 const uint32_t i = lane + s * BLOCK;
                 if (!(S.invm[i] > 0.0f)) continue;
-                const float2 pi = p[s], vi = v[s];
+                const float2 vi = v[s];
                 const float ri = S.r[i], accI = S.aux[i];
                 float best = inff();
                 const uint32_t start = rs[s], pairEnd = start + (rpq[s] & 0xffffu), segEnd = start + (rpq[s] >> 16), end = start + rl[s];
@@ -933,8 +933,11 @@ __device__ __attribute__((always_inline)) void lds_world_steps(LdsWorld<BLOCK, P
             if (!claimed && n > 0 && assert_contacts()) claimed = true;
         }
 
-        // ---- Write-back of the working side, counters and publication.
+        // ---- Write-back of the working side, counters and publication. A served world's mirror
+        // takes the same records; it is read only after a published step (a failed world takes no
+        // further requests, and the server reads the mirror of a world whose last step succeeded).
         uint32_t rmLocal = 0, wrLocal = 0;
+        const bool mirrored = d->mirror != nullptr && n <= d->mirrorCapacity;
     #pragma unroll
         for (int s = 0; s < PER; ++s) {
             if (!valid[s]) continue;
@@ -943,6 +946,7 @@ __device__ __attribute__((always_inline)) void lds_world_steps(LdsWorld<BLOCK, P
             A->vel[ws][gi] = v[s];
             A->rot[ws][gi] = rot[s];
             A->omega[ws][gi] = om[s];
+            if (mirrored) mirror_record(A, d, ws, lane + s * BLOCK, p[s], v[s], rot[s], om[s]);
             if (!(status == BP_OK && finished)) A->region[ws][gi] = BP_REGION_UNCHANGED;
             rmLocal += region[s] == BP_REGION_REMOVED;
             wrLocal += region[s] == BP_REGION_WRAPPED;
@@ -957,6 +961,7 @@ __device__ __attribute__((always_inline)) void lds_world_steps(LdsWorld<BLOCK, P
                     }
             }
         }
+        if (mirrored) mirror_bodies(A, d, ws, lane, BLOCK);
         __syncthreads();
         if (lane < 8) S.red.counters[lane] = 0;
         __syncthreads();

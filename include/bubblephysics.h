@@ -672,9 +672,11 @@ typedef struct bp_server_result {
 
 typedef struct bp_server_config {
     uint64_t memory_budget;         /* device bytes of resident worlds; 0 = no limit           */
-    uint32_t streams;               /* HIP streams, one launch in flight on each (default 32; the
-                                       process should have as many hardware queues:
-                                       GPU_MAX_HW_QUEUES)                                       */
+    uint32_t streams;               /* HIP streams in all (default 32, at least 3): one for reads,
+                                       one for host staging of add, merge and split, the rest
+                                       for launches (one in flight on each, the batch's own
+                                       streams first); the process should have as many hardware
+                                       queues (GPU_MAX_HW_QUEUES)                               */
     uint32_t max_inflight_per_class;/* launches of one kernel class in flight at once (default 32) */
     uint32_t max_worlds_per_launch; /* default 1024                                             */
     uint32_t big_world;             /* worlds above this many bubbles run in the one-workgroup

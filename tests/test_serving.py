@@ -298,3 +298,42 @@ def test_each_served_world_gets_the_result_it_gets_alone():
             info = alone.step_worlds([0], 1)[0, 0]
             assert info["status"] == r.status and info["substep_count"] == r.substeps, (w, info["status"], r.status)
         assert same_bits(final[w], alone.read_bubbles(0), props=True), w
+
+
+@pytest.mark.gpu
+def test_structural_transactions_after_region_removals_match_alone():
+    # A step that removes bubbles from the region leaves the world's serving mirror stale (the
+    # server deletes the bubbles on the device afterwards): the next add, merge or split is
+    # staged from a read-back of the device state instead. Results equal the world served alone.
+    rng = np.random.Generator(np.random.PCG64(41))
+    w = scenes.batch_world(rng, 150, bodies_max=3)
+    w.segments = w.segments[:0]                          # no walls: bubbles may leave
+    side = float(w.config.region[2])
+    w.config.region[2] = side - 60.0                     # the right strip lies outside the region
+    kinds = [None, "merge", None, "add", "split", None, "merge", "add", None, "split", "merge", None]
+    reqs = []
+    for k, kind in enumerate(kinds * 2):
+        r = request(0, kind, 100 + k, arrival=0.002 * k)
+        reqs.append(r)
+    server = serving.Server([w])
+    stats = server.run(reqs)
+    final = server.read_bubbles(0)
+    server.close()
+    assert stats.summary()["served"] == len(reqs)
+    alone = solver.Batch([w])
+    removed_any = False
+    for r in sorted(reqs, key=lambda r: r.done):
+        if r.status == abi.BP_ERR_UNAVAILABLE:
+            assert alone.step_worlds([0], 1)[0, 0]["status"] == abi.BP_ERR_UNAVAILABLE
+            continue
+        if r.transaction:
+            kw = stage_kw(w, bodies=alone.read_bodies(0, len(w.bodies)))
+            staged, reason = serving.apply_transaction(alone.read_bubbles(0), r, **kw)
+            assert reason == r.tx_result, (r.transaction, reason, r.tx_result)
+            if reason == abi.BP_REJECT_NONE:
+                alone.write_bubbles(0, staged)
+        info = alone.step_worlds([0], 1)[0, 0]
+        removed_any |= int(info["region_removals"]) > 0
+        assert info["status"] == r.status and info["substep_count"] == r.substeps
+    assert removed_any
+    assert same_bits(final, alone.read_bubbles(0), props=True)
