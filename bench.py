@@ -1,11 +1,19 @@
 """Benchmark: bubble-substeps/sec of the world step on one MI355X.
 
-Workload (default "batched"): 4,096 independent worlds with bubble counts log-uniform in
+Workload of `value` ("batched"): 4,096 independent worlds with bubble counts log-uniform in
 [16, 4096] (about 3.0M bubbles), radii U[6, 20], 0-16 kinematic bodies each, walls, advanced
-together: the timed call advances every world by `--steps` frames of dt = 1/60, one launch per
-kernel class, each workgroup carrying its world through all the frames. Beside it: the plain
-single-world step (1,024 bubbles + 64 kinematic bodies, one launch per frame), the 262,144-bubble
-world of config 3 on the cooperative grid, and the CPU oracle on the host cores.
+together: the timed call replays hipGraphs that advance every world by `--steps` frames of
+dt = 1/60, captured after `--warmup` untimed frames: one launch per kernel class (workgroup pools
+for the classes whose workgroups fill a CU), each workgroup carrying its world through all the
+frames. Beside it, on the same box:
+  * the same launches made directly, without pools, and after the autotuner;
+  * the plain single-world step (one launch per frame) of the 256-bubble world and of the
+    1,024-bubble + 64-body world;
+  * the 262,144-bubble world on the cooperative grid;
+  * serving the batched worlds at rising offered loads, and the 8,192-world serving workload
+    (60 s), then again with a device-memory budget that forces evictions;
+  * the CPU oracle on the host cores: the batch, and the single worlds on one thread and with its
+    per-phase threads.
 
 One JSON line on stdout. value = bubble-substeps/s = sum over worlds and frames of
 (bubbles x substeps) / wall time of the timed call, with every world resident on the GPU first.
@@ -16,6 +24,7 @@ import argparse
 import json
 import math
 import os
+import platform
 import sys
 import time
 from pathlib import Path
@@ -33,6 +42,11 @@ HW_QUEUES = 32
 os.environ["GPU_MAX_HW_QUEUES"] = str(HW_QUEUES)
 
 HBM_PEAK_GBPS = 8000.0
+HOST_THREADS = 16          # host threads a one-GPU job may keep busy
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def bytes_per_bubble_substep(mean_degree: float) -> float:
@@ -42,57 +56,127 @@ def bytes_per_bubble_substep(mean_degree: float) -> float:
     return 168.0 + 8.0 * mean_degree
 
 
-def kernel_name(k, res):
+def kernel_name(k, threads, capacity):
     """Kernel of class k: the LDS configurations, the single-workgroup global-state kernel (class 10)
     and the cooperative kernel (class 11)."""
     if k == 10:
         return "world_step_kernel<1024> (global state, one workgroup)"
     if k == 11:
         return "world_coop_kernel<256> (cooperative grid)"
-    return f"world_lds_kernel<{int(res['class_threads'][k])} threads, {int(res['class_capacity'][k])} bubbles>"
+    return f"world_lds_kernel<{int(threads[k])} threads, {int(capacity[k])} bubbles>"
 
 
 def percentile(values, q):
     return float(np.percentile(np.asarray(values, np.float64), q)) if len(values) else float("nan")
 
 
-def build_batched(worlds: int):
-    from bubblephysics_amd import scenes
-    return scenes.config4(worlds=worlds)
+def cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
 
 
-def run_batched(args, scenes_list):
-    """Throughput: one call advances every world `steps` steps (one launch per kernel class, each
-    workgroup carrying its world through all the steps). Latency: `latency_steps` calls of one step."""
+# ---------------------------------------------------------------------------------------------
+# GPU measurements
+# ---------------------------------------------------------------------------------------------
+def summarize(infos, sizes, elapsed):
+    ok = infos["status"] == 0                      # failed and unexecuted steps do not count
+    substeps = np.where(ok, infos["substep_count"], 0).astype(np.int64)
+    return int((substeps * sizes[None, :]).sum()), substeps
+
+
+def run_batched(args, scenes_list, *, mode="graph", label="batched"):
+    """Throughput: every world advances `steps` steps in one timed call. mode "graph": the steps
+    captured into hipGraphs after the warm-up (one graph per kernel class, each replayed on its
+    class's stream, pools planned from the warm-up's measured times) and replayed once; "direct":
+    the same launches made directly; "unpooled": direct launches of one workgroup per world.
+    Latency: `latency_steps` direct calls of one step."""
     from bubblephysics_amd import solver
     batch = solver.Batch(scenes_list)
+    batch.set_pooling(mode != "unpooled")
     sizes = np.array([len(s.bubbles) for s in scenes_list], np.int64)
     if args.warmup:
         batch.step(args.warmup)
     classes = int(batch._lib.bp_batch_class_count())
-    t0 = time.perf_counter()
-    infos = batch.step(args.steps)
-    elapsed = time.perf_counter() - t0
+    if mode == "graph":
+        batch.capture(args.steps)
+        t0 = time.perf_counter()
+        infos = batch.replay()
+        elapsed = time.perf_counter() - t0
+    else:
+        t0 = time.perf_counter()
+        infos = batch.step(args.steps)
+        elapsed = time.perf_counter() - t0
     ms, _, threads, capacity = batch.last_class_timing()
-    ok = infos["status"] == 0                      # failed and unexecuted steps do not count
-    substeps = np.where(ok, infos["substep_count"], 0).astype(np.int64)
-    substep_bubbles = int((substeps * sizes[None, :]).sum())
+    work, substeps = summarize(infos, sizes, elapsed)
     deg = infos["adjacency_entries"].astype(np.float64) / np.maximum(sizes[None, :], 1)
     per = substeps * sizes[None, :] * bytes_per_bubble_substep(deg)
     class_bytes = np.zeros(classes)
     for k in range(classes):
         class_bytes[k] = float(per[:, infos[0]["kernel_class"] == k].sum())
-    failed = int((infos[-1]["status"] != 0).sum())
     latencies = []
     for _ in range(args.latency_steps):
         s0 = time.perf_counter()
         batch.step(1)
         latencies.append((time.perf_counter() - s0) * 1e3)
+    batch.close()
+    log(f"{label}: {elapsed * 1e3:.1f} ms for {args.steps} steps, {work / elapsed:.4g} bubble-substeps/s")
     return {
-        "class_threads": threads, "class_capacity": capacity, "elapsed": elapsed, "substep_bubbles": substep_bubbles,
-        "latencies": latencies, "class_ms": ms, "class_bytes": class_bytes, "failed_worlds": failed,
+        "class_threads": threads, "class_capacity": capacity, "elapsed": elapsed, "substep_bubbles": work,
+        "latencies": latencies, "class_ms": ms, "class_bytes": class_bytes, "failed_worlds": int((infos[-1]["status"] != 0).sum()),
         "bubbles": int(sizes.sum()), "worlds": len(sizes), "overflow": int(infos["candidate_overflow"].max()),
     }
+
+
+def run_autotuned(args, scenes_list):
+    """The autotuner's trials on the warmed-up batch (whole-batch steps per kernel configuration and
+    bucket count), then the same timed steps with its choices."""
+    from bubblephysics_amd import autotune, solver
+    batch = solver.Batch(scenes_list)
+    sizes = np.array([len(s.bubbles) for s in scenes_list], np.int64)
+    if args.warmup:
+        batch.step(args.warmup)
+    t = time.perf_counter()
+    table = autotune.tune(batch, repeats=args.tune_repeats)
+    tune_s = time.perf_counter() - t
+    batch.step(1)                                  # repacks with the choices and measures the worlds again
+    t0 = time.perf_counter()
+    infos = batch.step(args.steps)
+    elapsed = time.perf_counter() - t0
+    work, _ = summarize(infos, sizes, elapsed)
+    batch.close()
+    moved = [{"bubbles": [int(r["min_bubbles"]), int(r["max_bubbles"])], "class": int(r["best_class"]),
+              "bucket_shift": int(r["best_bucket_shift"])} for r in table if r["worlds"]]
+    log(f"autotuned: {elapsed * 1e3:.1f} ms, {work / elapsed:.4g} (trials {tune_s:.1f}s)")
+    return {"value": work / elapsed, "ms_per_step": elapsed / args.steps * 1e3, "trials_s": tune_s, "choices": moved,
+            "note": "timed one step later than the other lines (the step that adopts the choices)"}
+
+
+def run_plain(scene, steps, warmup=3):
+    """One world, one launch per frame, frames timed one by one on the host clock."""
+    from bubblephysics_amd import solver
+    world = solver.NativeWorld(scene)
+    n = len(scene.bubbles)
+    for k in range(warmup):
+        world.step(k)
+    lat, total, statuses = [], 0, set()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        s0 = time.perf_counter()
+        f = world.step(warmup + k)
+        lat.append((time.perf_counter() - s0) * 1e3)
+        statuses.add(int(f.status))
+        if f.status == 0:
+            total += n * f.substep_count
+        else:
+            break
+    elapsed = time.perf_counter() - t0
+    return {"value": total / elapsed, "p50_step_ms": percentile(lat, 50), "p99_step_ms": percentile(lat, 99),
+            "bubbles": n, "steps": len(lat), "statuses": sorted(statuses)}
 
 
 def run_large(steps: int):
@@ -114,79 +198,125 @@ def run_large(steps: int):
             "bubbles": n, "bodies": len(scene.bodies), "statuses": sorted(set(statuses))}
 
 
+def run_served_sweep(args, scenes_list, plain_value):
+    """Serving the batched worlds at rising offered loads (Poisson step requests, 10% with
+    transactions), each for `serve_c4_seconds`, one server over the same resident worlds. The
+    highest throughput is the server's ceiling on this workload; the line with p99 <= 4 ms is the
+    highest load the latency target allows."""
+    from bubblephysics_amd import serving, solver
+    batch = solver.Batch(scenes_list)
+    batch.step(1)
+    server = serving.Server(batch, streams=args.serve_streams, max_inflight_per_class=args.serve_inflight)
+    out = []
+    for j, rate in enumerate(args.serve_c4_rates):
+        requests = serving.poisson_requests(len(scenes_list), rate, args.serve_c4_seconds, seed=20261208 + j)
+        stats = server.run(requests).summary()
+        stats["offered_per_s"] = rate
+        stats["over_plain"] = stats["bubble_substeps_per_s"] / plain_value if plain_value > 0 else None
+        out.append(stats)
+        log(f"served batch worlds at {rate:.0f}/s: {stats['bubble_substeps_per_s']:.4g} bubble-substeps/s, "
+            f"p50 {stats['p50_ms']:.2f} ms, p99 {stats['p99_ms']:.2f} ms")
+    server.close()
+    batch.close()
+    best = max(out, key=lambda s: s["bubble_substeps_per_s"])
+    within = [s for s in out if s["p99_ms"] <= 4.0]
+    return {"sweep": out, "ceiling_bubble_substeps_per_s": best["bubble_substeps_per_s"],
+            "ceiling_over_plain": best["over_plain"],
+            "best_within_p99_4ms": max(within, key=lambda s: s["bubble_substeps_per_s"])["offered_per_s"] if within else None}
+
+
 def run_served(args):
-    """Serving: Poisson step requests (10% with transactions) over resident worlds, continuous batching."""
+    """Config 5: Poisson step requests (10% with transactions) over 8,192 resident worlds for
+    `serve_seconds`; then a shorter run with a device-memory budget below the resident footprint
+    (`serve_evict_budget` of it), which forces admissions and LRU evictions to host memory."""
     from bubblephysics_amd import serving, solver
     t = time.perf_counter()
     worlds = serving.serving_worlds(args.serve_worlds, seed=20261206)
-    print(f"[bench] built {len(worlds)} serving worlds in {time.perf_counter() - t:.1f}s", file=sys.stderr, flush=True)
+    log(f"built {len(worlds)} serving worlds in {time.perf_counter() - t:.1f}s")
     batch = solver.Batch(worlds)
     batch.step(1)                                          # packs and warms every kernel class
     server = serving.Server(batch, memory_budget=int(200e9), streams=args.serve_streams,
                             max_inflight_per_class=args.serve_inflight)
     requests = serving.poisson_requests(len(worlds), args.serve_rate, args.serve_seconds, seed=20261207)
-    print(f"[bench] serving {len(requests)} requests", file=sys.stderr, flush=True)
+    log(f"serving {len(requests)} requests")
     stats = server.run(requests).summary()
     server.close()
+    resident = int(stats["resident_bytes"])
     stats["bubbles"] = int(sum(len(w.bubbles) for w in worlds))
     stats["worlds"] = len(worlds)
-    return stats
-
-
-def run_served_batch(args, scenes_list):
-    """Serving the worlds of the batched workload: Poisson step requests (10% with transactions) at
-    `serve_c4_rate` over `serve_c4_seconds`, from a fresh copy of the worlds."""
-    from bubblephysics_amd import serving, solver
-    batch = solver.Batch(scenes_list)
-    batch.step(1)
-    server = serving.Server(batch, streams=args.serve_streams, max_inflight_per_class=args.serve_inflight)
-    requests = serving.poisson_requests(len(scenes_list), args.serve_c4_rate, args.serve_c4_seconds, seed=20261208)
-    stats = server.run(requests).summary()
-    server.close()
+    log(f"served: {stats['bubble_substeps_per_s']:.4g} bubble-substeps/s, p99 {stats['p99_ms']:.2f} ms")
+    evict = None
+    if args.serve_evict_seconds > 0 and resident > 0:
+        budget = int(resident * args.serve_evict_budget)
+        server = serving.Server(batch, memory_budget=budget, streams=args.serve_streams,
+                                max_inflight_per_class=args.serve_inflight)
+        requests = serving.poisson_requests(len(worlds), args.serve_rate, args.serve_evict_seconds, seed=20261209)
+        evict = server.run(requests).summary()
+        server.close()
+        evict["memory_budget_bytes"] = budget
+        log(f"served under a {budget / 1e9:.1f} GB budget: {evict['evictions']} evictions, p99 {evict['p99_ms']:.2f} ms")
     batch.close()
-    return stats
+    return stats, evict
 
 
-def run_plain(steps: int):
-    from bubblephysics_amd import scenes, solver
-    scene = scenes.config2()
-    world = solver.NativeWorld(scene)
-    n = len(scene.bubbles)
-    for k in range(3):
-        world.step(k)
-    lat, total = [], 0
-    t0 = time.perf_counter()
-    for k in range(steps):
-        s0 = time.perf_counter()
-        f = world.step(k)
-        lat.append((time.perf_counter() - s0) * 1e3)
-        total += n * f.substep_count
-    elapsed = time.perf_counter() - t0
-    return {"value": total / elapsed, "p50_ms": percentile(lat, 50), "p99_ms": percentile(lat, 99), "bubbles": n}
-
-
-def cpu_baseline(scenes_list, budget_s: float, threads: int, steps: int = 3):
-    """The oracle (FP32) on the worlds of the same batch, `steps` steps each from their initial
-    state, spread over `threads` host threads (the oracle library runs outside the interpreter
-    lock), until the wall budget is used. Executed bubble-substeps of successful steps / wall time."""
+# ---------------------------------------------------------------------------------------------
+# CPU baselines (the oracle; test infrastructure used here only as the timed CPU restatement)
+# ---------------------------------------------------------------------------------------------
+def cpu_batch(scenes_list, budget_s, threads, warmup, steps):
+    """The oracle (FP32) on a random sample of the batch's worlds with the GPU line's schedule:
+    first every sampled world's `warmup` untimed steps (worlds are added to the sample until the
+    warm-up share of the wall budget is used), then the timed `steps` steps of all of them, the
+    worlds spread over `threads` host threads (the oracle runs outside the interpreter lock).
+    value = bubble-substeps of successful timed steps / wall time of the timed part."""
     from concurrent.futures import ThreadPoolExecutor
     from oracle import bp_oracle
     bp_oracle.load()
-    deadline = time.perf_counter() + budget_s
+    bp_oracle.set_threads(1)
+    order = np.random.Generator(np.random.PCG64(7)).permutation(len(scenes_list))
+    deadline = time.perf_counter() + budget_s * warmup / max(warmup + steps, 1)
 
-    def one(s):
+    def warm(w):
         if time.perf_counter() >= deadline:
             return None
+        s = scenes_list[w].copy()
+        if warmup:
+            s.bubbles, s.bodies, _, _ = bp_oracle.run(s, warmup, assert_mode=1)
+        return s
+
+    with ThreadPoolExecutor(max_workers=threads) as pool:
+        sample = [s for s in pool.map(warm, order) if s is not None]
+
+    def timed(s):
         _, _, frames, _ = bp_oracle.run(s, steps, assert_mode=1)
         ok = frames["status"] == 0
         return len(s.bubbles) * int(frames["substeps"][ok].sum())
 
     t0 = time.perf_counter()
     with ThreadPoolExecutor(max_workers=threads) as pool:
-        done = [r for r in pool.map(one, scenes_list) if r is not None]
-    used = time.perf_counter() - t0
-    return {"value": sum(done) / used if used > 0 else float("nan"), "worlds": len(done), "seconds": used,
-            "threads": threads, "steps": steps}
+        work = sum(pool.map(timed, sample))
+    wall = time.perf_counter() - t0
+    return {"value": work / wall if wall > 0 else float("nan"), "worlds": len(sample), "seconds": wall,
+            "bubbles": int(sum(len(s.bubbles) for s in sample)), "threads": threads, "warmup": warmup, "steps": steps}
+
+
+def cpu_single(scene, steps, threads, warmup=3):
+    """One world in the oracle: `warmup` untimed steps, then `steps` timed steps with `threads`
+    host threads for its per-bubble phases and candidate rows (1 = sequential)."""
+    from oracle import bp_oracle
+    b, bod, _, _ = bp_oracle.run(scene, warmup, assert_mode=1)
+    start = scene.copy()
+    start.bubbles, start.bodies = b, bod
+    bp_oracle.set_threads(threads)
+    try:
+        t = time.perf_counter()
+        _, _, frames, _ = bp_oracle.run(start, steps, assert_mode=1)
+        dt = time.perf_counter() - t
+    finally:
+        bp_oracle.set_threads(1)
+    ok = frames["status"] == 0
+    work = len(scene.bubbles) * int(frames["substeps"][ok].sum())
+    return {"value": work / dt, "steps": int(len(frames)), "threads": threads, "seconds": dt,
+            "ms_per_step": dt / max(len(frames), 1) * 1e3}
 
 
 def fp64_drift(checkpoints=(1, 10, 30, 60)):
@@ -245,18 +375,24 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--worlds", type=int, default=4096)
-    ap.add_argument("--plain-steps", type=int, default=60)
+    ap.add_argument("--plain-steps", type=int, default=120, help="timed frames of the 256-bubble plain world")
+    ap.add_argument("--plain2-steps", type=int, default=300, help="timed frames of the 1,024-bubble plain world")
     ap.add_argument("--latency-steps", type=int, default=10)
     ap.add_argument("--large-steps", type=int, default=5)
-    ap.add_argument("--serve-seconds", type=float, default=30.0)
+    ap.add_argument("--no-ab", action="store_true", help="skip the unpooled, graph and autotuned lines")
+    ap.add_argument("--tune-repeats", type=int, default=3)
+    ap.add_argument("--serve-seconds", type=float, default=60.0)
+    ap.add_argument("--serve-evict-seconds", type=float, default=10.0)
+    ap.add_argument("--serve-evict-budget", type=float, default=0.9, help="memory budget of the eviction run, "
+                    "as a fraction of the resident footprint")
     ap.add_argument("--serve-rate", type=float, default=20000.0)
     ap.add_argument("--serve-worlds", type=int, default=8192)
-    ap.add_argument("--serve-c4-seconds", type=float, default=5.0)
-    ap.add_argument("--serve-c4-rate", type=float, default=20000.0)
+    ap.add_argument("--serve-c4-seconds", type=float, default=4.0)
+    ap.add_argument("--serve-c4-rates", type=float, nargs="*", default=[20000.0, 40000.0, 80000.0, 160000.0])
     ap.add_argument("--serve-streams", type=int, default=32)
     ap.add_argument("--serve-inflight", type=int, default=32)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--cpu-threads", type=int, default=min(os.cpu_count() or 1, 16))
+    ap.add_argument("--cpu-threads", type=int, default=min(os.cpu_count() or 1, HOST_THREADS))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-bytes", type=int, default=0, help="only run the copy calibration kernel of this size")
     args = ap.parse_args()
@@ -269,35 +405,47 @@ def main():
         copy_calibration(args.copy_bytes)
         return
 
+    from bubblephysics_amd import scenes
     t_build = time.perf_counter()
-    scenes_list = build_batched(args.worlds)
-    print(f"[bench] built {len(scenes_list)} worlds in {time.perf_counter() - t_build:.1f}s", file=sys.stderr, flush=True)
-    res = run_batched(args, scenes_list)
-    print(f"[bench] batched done: {res['elapsed']:.2f}s for {args.steps} steps", file=sys.stderr, flush=True)
-    plain = run_plain(args.plain_steps) if args.plain_steps > 0 else {"value": float("nan"), "p50_ms": float("nan"),
-                                                                        "p99_ms": float("nan"), "bubbles": 0}
-    print(f"[bench] plain done: {plain['value']:.3g} bubble-substeps/s", file=sys.stderr, flush=True)
+    scenes_list = scenes.config4(worlds=args.worlds)
+    log(f"built {len(scenes_list)} worlds in {time.perf_counter() - t_build:.1f}s")
+    res = run_batched(args, scenes_list, mode="graph", label="batched (graph replay)")
+    ab = None
+    if not args.no_ab:
+        line = lambda r: {"value": r["substep_bubbles"] / r["elapsed"], "ms_per_step": r["elapsed"] / args.steps * 1e3}
+        ab = {"direct_launches": line(run_batched(args, scenes_list, mode="direct", label="direct launches")),
+              "unpooled": line(run_batched(args, scenes_list, mode="unpooled", label="unpooled")),
+              "autotuned": run_autotuned(args, scenes_list)}
+    c1, c2 = scenes.config1(), scenes.config2()
+    plain = run_plain(c1, args.plain_steps) if args.plain_steps > 0 else None
+    plain2 = run_plain(c2, args.plain2_steps) if args.plain2_steps > 0 else None
+    plain_value = plain["value"] if plain else float("nan")
+    log(f"plain: {plain_value:.4g} (config 1), {plain2['value'] if plain2 else float('nan'):.4g} (config 2)")
     large = run_large(args.large_steps) if args.large_steps > 0 else None
     if large:
-        print(f"[bench] large done: {large['value']:.3g} bubble-substeps/s", file=sys.stderr, flush=True)
-    served_c4 = run_served_batch(args, scenes_list) if args.serve_c4_seconds > 0 else None
-    if served_c4:
-        print(f"[bench] served batch worlds: {served_c4}", file=sys.stderr, flush=True)
-    served = run_served(args) if args.serve_seconds > 0 else None
-    if served:
-        print(f"[bench] served: {served}", file=sys.stderr, flush=True)
+        log(f"large done: {large['value']:.3g} bubble-substeps/s")
+    served_c4 = run_served_sweep(args, scenes_list, plain_value) if args.serve_c4_seconds > 0 and args.serve_c4_rates else None
+    served, evict = run_served(args) if args.serve_seconds > 0 else (None, None)
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(scenes_list, args.cpu_seconds, args.cpu_threads)
+        cpu = cpu_batch(scenes_list, args.cpu_seconds, args.cpu_threads, args.warmup, args.steps)
+        log(f"cpu batch: {cpu['value']:.3g} bubble-substeps/s on {cpu['threads']} threads")
+        cpu["single"] = {
+            "config1_1_thread": cpu_single(c1, args.plain_steps, 1),
+            "config1_per_phase": cpu_single(c1, args.plain_steps, args.cpu_threads),
+            "config2_1_thread": cpu_single(c2, 60, 1),
+            "config2_per_phase": cpu_single(c2, 60, args.cpu_threads),
+        }
         cpu["drift"] = fp64_drift()
-        print(f"[bench] cpu done: {cpu['value']:.3g} bubble-substeps/s on {cpu['threads']} threads", file=sys.stderr,
-              flush=True)
+        log("cpu single-world and drift done")
 
     value = res["substep_bubbles"] / res["elapsed"]
-    k = int(np.argmax(res["class_ms"]))
+    # The dominant kernel: the class that moves the most algorithmic bytes in the timed launch.
+    k = int(np.argmax(res["class_bytes"]))
     dur_s = float(res["class_ms"][k]) / 1e3          # one launch of the class covers all the timed steps
     achieved = res["class_bytes"][k] / dur_s / 1e9 if dur_s > 0 else 0.0
-    traffic = recorded_traffic(kernel_name(k, res), args.steps, res["worlds"])
+    kname = kernel_name(k, res["class_threads"], res["class_capacity"])
+    traffic = recorded_traffic(kname, args.steps, res["worlds"])
     line = {
         "metric": "bubble-substeps/sec on 1 MI355X (batched/served vs plain) + p50/p99 step latency",
         "value": value,
@@ -314,40 +462,48 @@ def main():
         "config": {
             "workload": f"batched: {res['worlds']} worlds, bubbles log-uniform in [16, 4096] ({res['bubbles']} total), "
                         f"radii U[6,20], 0-16 kinematic bodies per world, walls, dt 1/60",
-            "worlds": res["worlds"], "bubbles": res["bubbles"], "parallelism": "one workgroup per world",
+            "worlds": res["worlds"], "bubbles": res["bubbles"], "parallelism": "one workgroup per world, pooled per class",
         },
         "p50_step_ms": percentile(res["latencies"], 50),     # one step of every world, host clock
         "p99_step_ms": percentile(res["latencies"], 99),
         "failed_worlds": res["failed_worlds"],
         "candidate_overflow_max": res["overflow"],
-        "plain": {"workload": "plain: 1 world, 1024 bubbles + 64 kinematic bodies", "value": plain["value"],
-                  "p50_step_ms": plain["p50_ms"], "p99_step_ms": plain["p99_ms"]},
-        "batched_over_plain": value / plain["value"] if plain["value"] > 0 else None,   # NaN compares false
+        "class_ms": {kernel_name(j, res["class_threads"], res["class_capacity"]): float(res["class_ms"][j])
+                     for j in range(len(res["class_ms"])) if res["class_ms"][j] > 0},
+        "ab": ab,
+        "plain": None if plain is None else {"workload": "plain: config 1, 1 world of 256 bubbles, walls, gravity, "
+                                             "one launch per frame", **plain},
+        "plain_config2": None if plain2 is None else {"workload": "plain: config 2, 1 world of 1,024 bubbles + 64 kinematic "
+                                                      "bodies, one launch per frame", **plain2},
+        "batched_over_plain": value / plain_value if plain_value > 0 else None,
+        "batched_over_plain_config2": value / plain2["value"] if plain2 and plain2["value"] > 0 else None,
         "large": None if large is None else {
             "workload": f"config 3: 1 world, {large['bubbles']} bubbles + {large['bodies']} bodies (kinematic and walls), "
                         "cooperative grid", "value": large["value"], "ms_per_step": large["ms_per_step"],
             "gpu_ms_per_step": large["gpu_ms_per_step"], "statuses": large["statuses"]},
         "served_batch_worlds": None if served_c4 is None else {
-            "workload": f"serving the {res['worlds']} batched worlds above: Poisson {args.serve_c4_rate:.0f} step "
-                        f"requests/s for {args.serve_c4_seconds:.0f}s, 10% with transactions",
-            "over_plain": served_c4["bubble_substeps_per_s"] / plain["value"] if plain["value"] > 0 else None,
+            "workload": f"serving the {res['worlds']} batched worlds above: Poisson step requests at rising offered loads, "
+                        f"{args.serve_c4_seconds:.0f}s each, 10% with transactions; over_plain against config 1's plain step",
             **served_c4},
         "served": None if served is None else {
             "workload": f"serving: {served['worlds']} resident worlds, bubbles log-uniform in [32, 16384] "
                         f"({served['bubbles']} total), Poisson {args.serve_rate:.0f} step requests/s for "
                         f"{args.serve_seconds:.0f}s, 10% with transactions, budget 200 GB",
-            **{k: v for k, v in served.items() if k not in ("worlds", "bubbles")}},
+            **{kk: v for kk, v in served.items() if kk not in ("worlds", "bubbles")},
+            "under_eviction": evict},
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
-            "kernel": kernel_name(k, res),
+            "kernel": kname,
             "kernel_ms_per_launch": float(res["class_ms"][k]),
         },
         "cpu_baseline": None if cpu is None else {
             "value": cpu["value"], "unit": "bubble-substeps/s", "cores": cpu["threads"], "kind": "port",
-            "sample": f"oracle FP32, {cpu['steps']} steps of each of the first {cpu['worlds']} worlds of the same "
-                      f"batch from their initial state, worlds spread over {cpu['threads']} host threads "
-                      f"({cpu['seconds']:.1f}s wall, about {cpu['seconds'] * cpu['threads']:.0f} thread-seconds)",
+            "sample": f"oracle FP32 on {cpu['worlds']} randomly chosen worlds ({cpu['bubbles']} bubbles) of the same "
+                      f"batch, each {cpu['warmup']} untimed + {cpu['steps']} timed steps (the GPU line's schedule), "
+                      f"worlds spread over {cpu['threads']} host threads ({cpu['seconds']:.1f}s wall for the timed part)",
+            "host": {"nproc": os.cpu_count(), "model": cpu_model(), "threads_used": cpu["threads"]},
+            "single_world": cpu["single"],
             "fp64_drift": cpu["drift"],
         },
     }

@@ -629,23 +629,8 @@ bp_status bp_batch::launch(const uint32_t *subset, uint32_t subsetCount, uint32_
         // with hundreds of substeps per step starts first instead of last, and its class is
         // launched first.
         if (orderDirty) {
-            for (int k = 0; k < kClassCount; ++k) {
-                // Every non-empty list is uploaded: a class's membership may have changed (the
-                // autotuner moves worlds between classes and marks the order dirty).
-                auto &list = classWorlds[k];
-                if (list.empty()) continue;
-                std::stable_sort(list.begin(), list.end(), [&](uint32_t a, uint32_t b) { return workEstimate[a] > workEstimate[b]; });
-                bp_status s = upload(orders[k], list);
-                if (s != BP_OK) return s;
-            }
-            uint64_t longest[kClassCount];
-            for (int k = 0; k < kClassCount; ++k) {
-                classOrder[k] = k;
-                longest[k] = classWorlds[k].empty() ? 0 : workEstimate[classWorlds[k].front()];
-            }
-            std::stable_sort(classOrder, classOrder + kClassCount, [&](int a, int b) { return longest[a] > longest[b]; });
-            orderDirty = false;
-            plan_pools();
+            bp_status s = refresh_order();
+            if (s != BP_OK) return s;
         }
     }
     if (!settling)
@@ -667,10 +652,64 @@ bp_status bp_batch::launch(const uint32_t *subset, uint32_t subsetCount, uint32_
     }
     BP_HIP(hipEventRecord(startEvent, streams[0]));
     for (int k = 1; k < kStreams; ++k) BP_HIP(hipStreamWaitEvent(streams[k], startEvent, 0));
-    // With pools, the pooled classes go first and the others wait behind a gate until every
-    // pooled workgroup has started (or 2 ms have passed): the pools' workgroups need whole CUs,
-    // and the small workgroups of the other classes would otherwise be scattered over the CUs
-    // before the pools are placed.
+    {
+        bp_status s = launch_classes(pooled, steps, subset != nullptr, true);
+        if (s != BP_OK) return s;
+    }
+    // Cooperative worlds last, alone on the device: a cooperative grid needs all its workgroups
+    // resident at once, so it never shares the device with the other classes' launches.
+    if (launchCount[kCoopClass]) {
+        hipStream_t stream = streams[0];
+        for (int k = 1; k < kStreams; ++k) {
+            BP_HIP(hipEventRecord(endEvents[k], streams[k]));
+            BP_HIP(hipStreamWaitEvent(stream, endEvents[k], 0));
+        }
+        BP_HIP(hipEventRecord(classStart[kCoopClass], stream));
+        for (uint32_t st = 0; st < steps; ++st)
+            for (uint32_t j = 0; j < launchCount[kCoopClass]; ++j) {
+                const uint32_t wi = worldOf(kCoopClass, j);
+                hipError_t e = bp_launch_world_coop(arenaDev, descs, stats + (size_t)st * W, orderOf(kCoopClass) + j, wi,
+                                                    coopGroups[wi], stream);
+                if (e != hipSuccess) { set_error(std::string("cooperative launch: ") + hipGetErrorString(e)); return BP_ERR_COMMAND_FAILED; }
+            }
+        BP_HIP(hipEventRecord(classEnd[kCoopClass], stream));
+    }
+    for (int k = 0; k < kStreams; ++k) BP_HIP(hipEventRecord(endEvents[k], streams[k]));
+    pending = true;
+    return BP_OK;
+}
+
+// The longest-first order inside every class (uploaded) and across classes, from the measured
+// device times, and the pools planned from them.
+bp_status bp_batch::refresh_order() {
+    for (int k = 0; k < kClassCount; ++k) {
+        // Every non-empty list is uploaded: a class's membership may have changed (the autotuner
+        // moves worlds between classes and marks the order dirty).
+        auto &list = classWorlds[k];
+        if (list.empty()) continue;
+        std::stable_sort(list.begin(), list.end(), [&](uint32_t a, uint32_t b) { return workEstimate[a] > workEstimate[b]; });
+        bp_status s = upload(orders[k], list);
+        if (s != BP_OK) return s;
+    }
+    uint64_t longest[kClassCount];
+    for (int k = 0; k < kClassCount; ++k) {
+        classOrder[k] = k;
+        longest[k] = classWorlds[k].empty() ? 0 : workEstimate[classWorlds[k].front()];
+    }
+    std::stable_sort(classOrder, classOrder + kClassCount, [&](int a, int b) { return longest[a] > longest[b]; });
+    orderDirty = false;
+    plan_pools();
+    return BP_OK;
+}
+
+// The class launches of `steps` steps (every class but the cooperative one) on the classes'
+// streams, which already wait for the start. With pools, the pooled classes go first and the
+// others wait behind a gate until every pooled workgroup has started (or 2 ms have passed): the
+// pools' workgroups need whole CUs, and the small workgroups of the other classes would otherwise
+// be scattered over the CUs before the pools are placed. `timed` records the class events.
+bp_status bp_batch::launch_classes(bool pooled, uint32_t steps, bool subset, bool timed) {
+    const size_t W = worlds.size();
+    auto orderOf = [&](int k) -> const uint32_t * { return subset ? subsetOrders + launchOffset[k] : orders[k]; };
     int sequence[kClassCount], pooledClasses = 0, listed = 0;
     uint32_t poolTarget = 0;
     for (int idx = 0; idx < kClassCount; ++idx)
@@ -697,7 +736,7 @@ bp_status bp_batch::launch(const uint32_t *subset, uint32_t subsetCount, uint32_
                 BP_HIP(hipStreamWaitEvent(stream, gateEvent, 0));
             }
         }
-        BP_HIP(hipEventRecord(classStart[k], stream));
+        if (timed) BP_HIP(hipEventRecord(classStart[k], stream));
         // An LDS configuration runs all the steps in one launch: each workgroup carries its world
         // through them without waiting for the slowest world of the class at every step.
         if (k == kGlobalClass) {
@@ -710,28 +749,8 @@ bp_status bp_batch::launch(const uint32_t *subset, uint32_t subsetCount, uint32_
                                                pooled ? poolGroups[k] : 0u, cursors + k, cursors + kClassCount, stream);
             if (e != hipSuccess) { set_error(std::string("kernel launch: ") + hipGetErrorString(e)); return BP_ERR_COMMAND_FAILED; }
         }
-        BP_HIP(hipEventRecord(classEnd[k], stream));
+        if (timed) BP_HIP(hipEventRecord(classEnd[k], stream));
     }
-    // Cooperative worlds last, alone on the device: a cooperative grid needs all its workgroups
-    // resident at once, so it never shares the device with the other classes' launches.
-    if (launchCount[kCoopClass]) {
-        hipStream_t stream = streams[0];
-        for (int k = 1; k < kStreams; ++k) {
-            BP_HIP(hipEventRecord(endEvents[k], streams[k]));
-            BP_HIP(hipStreamWaitEvent(stream, endEvents[k], 0));
-        }
-        BP_HIP(hipEventRecord(classStart[kCoopClass], stream));
-        for (uint32_t st = 0; st < steps; ++st)
-            for (uint32_t j = 0; j < launchCount[kCoopClass]; ++j) {
-                const uint32_t wi = worldOf(kCoopClass, j);
-                hipError_t e = bp_launch_world_coop(arenaDev, descs, stats + (size_t)st * W, orderOf(kCoopClass) + j, wi,
-                                                    coopGroups[wi], stream);
-                if (e != hipSuccess) { set_error(std::string("cooperative launch: ") + hipGetErrorString(e)); return BP_ERR_COMMAND_FAILED; }
-            }
-        BP_HIP(hipEventRecord(classEnd[kCoopClass], stream));
-    }
-    for (int k = 0; k < kStreams; ++k) BP_HIP(hipEventRecord(endEvents[k], streams[k]));
-    pending = true;
     return BP_OK;
 }
 
@@ -1089,9 +1108,11 @@ bp_status bp_batch::step(uint32_t steps, bp_frame_info *infos) {
     return finish(infos, true);
 }
 
-// Captures `steps` steps of every resident world into a hipGraph: a fill kernel clears the
-// result records (no memset node), the capture forks from the first stream to the others, each
-// kernel class is launched on its stream as launch() does, and the streams join back.
+// Captures `steps` steps of every resident world into hipGraphs, one per kernel class, each
+// captured on the class's stream: the class's launch (behind the gate when pools are planned and
+// the class is not pooled). replay() clears the result records and the pool words with fill
+// kernels (no memset node), then launches every class's graph on its stream and the cooperative
+// class's graph last, as launch() orders direct launches.
 bp_status bp_batch::capture(uint32_t steps) {
     if (steps == 0) return BP_ERR_INVALID_ARGUMENT;
     if (!packed) {
@@ -1099,6 +1120,10 @@ bp_status bp_batch::capture(uint32_t steps) {
         if (s != BP_OK) return s;
     }
     release_graph();
+    if (orderDirty) {
+        bp_status s = refresh_order();
+        if (s != BP_OK) return s;
+    }
     const size_t W = worlds.size();
     const size_t need = (size_t)steps * W;
     if (statsCapacity < need) {                       // allocations happen before the capture
@@ -1106,69 +1131,96 @@ bp_status bp_batch::capture(uint32_t steps) {
         if (s != BP_OK) return s;
         statsCapacity = need;
     }
-    if (!forkEvent) {
-        BP_HIP(hipEventCreateWithFlags(&forkEvent, hipEventDisableTiming));
-        for (auto &e : joinEvents) BP_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    }
-    hipStream_t origin = streams[0];
-    BP_HIP(hipStreamBeginCapture(origin, hipStreamCaptureModeThreadLocal));
-    hipError_t e = bp_launch_fill_u32(reinterpret_cast<uint32_t *>(stats), need * sizeof(bpd::WorldStatus) / 4, 0u, origin);
-    if (e == hipSuccess) e = bp_launch_fill_u32(cursors, kClassCount + 1, 0u, origin);
-    if (e == hipSuccess) e = hipEventRecord(forkEvent, origin);
-    for (int k = 1; k < kStreams && e == hipSuccess; ++k) e = hipStreamWaitEvent(streams[k], forkEvent, 0);
-    for (int idx = 0; idx < kClassCount && e == hipSuccess; ++idx) {
-        const int k = classOrder[idx];
-        const uint32_t count = (uint32_t)classWorlds[k].size();
-        if (!count || k == kCoopClass) continue;
-        hipStream_t stream = streams[k % kStreams];
-        if (k == kGlobalClass) {
+    if (!forkEvent) BP_HIP(hipEventCreateWithFlags(&forkEvent, hipEventDisableTiming));
+    for (int k = 0; k < kClassCount; ++k) { launchOffset[k] = 0; launchCount[k] = (uint32_t)classWorlds[k].size(); }
+    const bool pooled = std::any_of(poolGroups, poolGroups + kClassCount, [](uint32_t g) { return g != 0; });
+    uint32_t poolTarget = 0;
+    for (int k = 0; k < kClassCount; ++k) if (pooled && launchCount[k]) poolTarget += poolGroups[k];
+    for (int k = 0; k < kClassCount; ++k) {
+        const uint32_t count = launchCount[k];
+        if (!count) continue;
+        hipStream_t stream = k == kCoopClass ? streams[0] : streams[k % kStreams];
+        BP_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        hipError_t e = hipSuccess;
+        if (k == kCoopClass) {
             for (uint32_t st = 0; st < steps && e == hipSuccess; ++st)
-                e = bp_launch_world_step(1024, arenaDev, descs, stats + (size_t)st * W, orders[k], count, 0u, stream);
+                for (uint32_t j = 0; j < count && e == hipSuccess; ++j) {
+                    const uint32_t wi = classWorlds[kCoopClass][j];
+                    e = bp_launch_world_coop(arenaDev, descs, stats + (size_t)st * W, orders[kCoopClass] + j, wi, coopGroups[wi], stream);
+                }
         } else {
-            e = bp_launch_world_lds(k, arenaDev, descs, stats, orders[k], count, steps, (uint32_t)W, 0u, poolGroups[k],
-                                    cursors + k, cursors + kClassCount, stream);
+            if (pooled && !poolGroups[k]) e = bp_launch_gate(cursors + kClassCount, poolTarget, 200000u, stream);
+            if (k == kGlobalClass) {
+                for (uint32_t st = 0; st < steps && e == hipSuccess; ++st)
+                    e = bp_launch_world_step(1024, arenaDev, descs, stats + (size_t)st * W, orders[k], count, 0u, stream);
+            } else if (e == hipSuccess) {
+                e = bp_launch_world_lds(k, arenaDev, descs, stats, orders[k], count, steps, (uint32_t)W, 0u,
+                                        pooled ? poolGroups[k] : 0u, cursors + k, cursors + kClassCount, stream);
+            }
+        }
+        hipGraph_t g = nullptr;
+        hipError_t end = hipStreamEndCapture(stream, &g);
+        if (e != hipSuccess || end != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            release_graph();
+            set_error(std::string("graph capture: ") + hipGetErrorString(e != hipSuccess ? e : end));
+            return BP_ERR_COMMAND_SETUP_FAILED;
+        }
+        graphs[k] = g;
+        hipError_t ie = hipGraphInstantiate(&graphExecs[k], g, nullptr, nullptr, 0);
+        if (ie != hipSuccess) {
+            release_graph();
+            set_error(std::string("graph instantiate: ") + hipGetErrorString(ie));
+            return BP_ERR_COMMAND_SETUP_FAILED;
         }
     }
-    for (int k = 1; k < kStreams && e == hipSuccess; ++k) {
-        e = hipEventRecord(joinEvents[k], streams[k]);
-        if (e == hipSuccess) e = hipStreamWaitEvent(origin, joinEvents[k], 0);
-    }
-    if (!classWorlds[kCoopClass].empty())
-        for (uint32_t st = 0; st < steps && e == hipSuccess; ++st)
-            for (uint32_t j = 0; j < classWorlds[kCoopClass].size() && e == hipSuccess; ++j) {
-                const uint32_t wi = classWorlds[kCoopClass][j];
-                e = bp_launch_world_coop(arenaDev, descs, stats + (size_t)st * W, orders[kCoopClass] + j, wi, coopGroups[wi], origin);
-            }
-    hipGraph_t g = nullptr;
-    hipError_t end = hipStreamEndCapture(origin, &g);
-    if (e != hipSuccess || end != hipSuccess) {
-        if (g) (void)hipGraphDestroy(g);
-        set_error(std::string("graph capture: ") + hipGetErrorString(e != hipSuccess ? e : end));
-        return BP_ERR_COMMAND_SETUP_FAILED;
-    }
-    graph = g;
-    hipError_t ie = hipGraphInstantiate(&graphExec, graph, nullptr, nullptr, 0);
-    if (ie != hipSuccess) {
-        release_graph();
-        set_error(std::string("graph instantiate: ") + hipGetErrorString(ie));
-        return BP_ERR_COMMAND_SETUP_FAILED;
-    }
     graphSteps = steps;
+    graphPooled = pooled;
     return BP_OK;
 }
 
-// One launch of the captured graph, then the result records of its steps for every world.
+// One launch of the captured graphs, then the result records of their steps for every world.
 bp_status bp_batch::replay(bp_frame_info *infos) {
-    if (!graphExec || !packed) { set_error("no captured graph (capture again after the batch changed)"); return BP_ERR_INVALID_ARGUMENT; }
+    bool any = false;
+    for (auto g : graphExecs) any = any || g;
+    if (!any || !packed || !graphSteps) { set_error("no captured graph (capture again after the batch changed)"); return BP_ERR_INVALID_ARGUMENT; }
+    const size_t W = worlds.size();
+    const size_t need = (size_t)graphSteps * W;
     hipStream_t origin = streams[0];
     BP_HIP(hipEventRecord(startEvent, origin));
-    BP_HIP(hipGraphLaunch(graphExec, origin));
+    hipError_t e = bp_launch_fill_u32(reinterpret_cast<uint32_t *>(stats), need * sizeof(bpd::WorldStatus) / 4, 0u, origin);
+    if (e == hipSuccess) e = bp_launch_fill_u32(cursors, kClassCount + 1, 0u, origin);
+    if (e != hipSuccess) { set_error(std::string("kernel launch: ") + hipGetErrorString(e)); return BP_ERR_COMMAND_FAILED; }
+    BP_HIP(hipEventRecord(forkEvent, origin));
+    for (int k = 1; k < kStreams; ++k) BP_HIP(hipStreamWaitEvent(streams[k], forkEvent, 0));
+    // Pooled classes first (their workgroups need whole CUs), then the others (gated inside their graphs).
+    for (int pass = 0; pass < 2; ++pass)
+        for (int idx = 0; idx < kClassCount; ++idx) {
+            const int k = classOrder[idx];
+            if (!graphExecs[k] || k == kCoopClass) continue;
+            if ((pass == 0) != (graphPooled && poolGroups[k] != 0)) continue;
+            BP_HIP(hipEventRecord(classStart[k], streams[k % kStreams]));
+            BP_HIP(hipGraphLaunch(graphExecs[k], streams[k % kStreams]));
+            BP_HIP(hipEventRecord(classEnd[k], streams[k % kStreams]));
+        }
+    for (int k = 1; k < kStreams; ++k) {
+        BP_HIP(hipEventRecord(endEvents[k], streams[k]));
+        BP_HIP(hipStreamWaitEvent(origin, endEvents[k], 0));
+    }
+    if (graphExecs[kCoopClass]) {
+        BP_HIP(hipEventRecord(classStart[kCoopClass], origin));
+        BP_HIP(hipGraphLaunch(graphExecs[kCoopClass], origin));
+        BP_HIP(hipEventRecord(classEnd[kCoopClass], origin));
+    }
     BP_HIP(hipEventRecord(endEvents[0], origin));
     BP_HIP(hipEventSynchronize(endEvents[0]));
     float ms = 0.0f;
     BP_HIP(hipEventElapsedTime(&ms, startEvent, endEvents[0]));
-    for (int k = 0; k < kClassCount; ++k) lastClassMs[k] = 0.0f;
-    std::vector<uint32_t> all(worlds.size());
+    for (int k = 0; k < kClassCount; ++k) {
+        lastClassMs[k] = 0.0f;
+        if (graphExecs[k]) BP_HIP(hipEventElapsedTime(&lastClassMs[k], classStart[k], classEnd[k]));
+    }
+    std::vector<uint32_t> all(W);
     for (uint32_t wi = 0; wi < all.size(); ++wi) all[wi] = wi;
     for (auto &w : worlds) w.removedIds.clear();
     bp_status s = collect(graphSteps, infos, all, ms);

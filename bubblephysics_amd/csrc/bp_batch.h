@@ -199,10 +199,13 @@ struct bp_batch {
     uint32_t totalBubbles = 0;
     std::mutex mutex;
     // Captured step chain (bp_batch_capture): the launches of `graphSteps` steps of every world.
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t graphExec = nullptr;
+    // One graph per kernel class, each replayed on the class's own stream (so the classes run
+    // concurrently, as direct launches do); the cooperative class's graph runs last on stream 0.
+    hipGraph_t graphs[bph::kClassCount] = {};
+    hipGraphExec_t graphExecs[bph::kClassCount] = {};
+    bool graphPooled = false;
     uint32_t graphSteps = 0;
-    hipEvent_t forkEvent = nullptr, joinEvents[bph::kStreams] = {};
+    hipEvent_t forkEvent = nullptr;     // replay(): the classes' streams wait for the cleared records
     // Region removals: a world whose step removed bubbles is held on the device until they are
     // deleted. A batch deletes them and runs the rest of the call's steps (the public world's
     // bookkeeping, NativePublicWorld.swift:264-267); a plain world (bp_world) keeps them in its
@@ -219,9 +222,13 @@ struct bp_batch {
     }
 
     void release_graph() {
-        if (graphExec) (void)hipGraphExecDestroy(graphExec);
-        if (graph) (void)hipGraphDestroy(graph);
-        graphExec = nullptr; graph = nullptr; graphSteps = 0;
+        for (int k = 0; k < bph::kClassCount; ++k) {
+            if (graphExecs[k]) (void)hipGraphExecDestroy(graphExecs[k]);
+            if (graphs[k]) (void)hipGraphDestroy(graphs[k]);
+            graphExecs[k] = nullptr; graphs[k] = nullptr;
+        }
+        graphSteps = 0;
+        graphPooled = false;
     }
 
     // Streams and events live as long as the batch (a repack keeps them, so a server that launches
@@ -260,7 +267,6 @@ struct bp_batch {
         release_graph();
         if (forkEvent) (void)hipEventDestroy(forkEvent);
         forkEvent = nullptr;
-        for (auto &e : joinEvents) { if (e) (void)hipEventDestroy(e); e = nullptr; }
         for (void *p : allocations) (void)hipFree(p);
         allocations.clear();
         stats = nullptr; statsCapacity = 0; descs = nullptr; arenaDev = nullptr; subsetOrders = nullptr; cursors = nullptr;
@@ -298,6 +304,8 @@ struct bp_batch {
     uint64_t footprint(uint32_t wi) const;
     bp_status launch(const uint32_t *subset, uint32_t subsetCount, uint32_t steps);
     void plan_pools();
+    bp_status refresh_order();
+    bp_status launch_classes(bool pooled, uint32_t steps, bool subset, bool timed);
     bp_status finish(bp_frame_info *infos, bool wait);
     bp_status step(uint32_t steps, bp_frame_info *infos);
     bp_status audit(bp_audit *out, uint32_t count);
