@@ -140,7 +140,7 @@ bp_status bp_batch::pack() {
     BP_ALLOC(A.rowStart, nb); BP_ALLOC(A.rowLen, nb); BP_ALLOC(A.grid, nb);
     BP_HIP(hipMemset(A.rowStart, 0, std::max<size_t>(nb, 1) * sizeof(uint32_t)));
     BP_HIP(hipMemset(A.rowLen, 0, std::max<size_t>(nb, 1) * sizeof(uint32_t)));
-    BP_ALLOC(A.omega2, 1); BP_ALLOC(A.bound, 1); BP_ALLOC(A.bodyHits, nbod);
+    BP_ALLOC(A.omega2, 1); BP_ALLOC(A.bound, 1); BP_ALLOC(A.bodyHits, nbod); BP_ALLOC(A.bodyCut, nbod);
     BP_ALLOC(A.nbrA, nb); BP_ALLOC(A.nbrB, nb); BP_ALLOC(A.invmEff, nb);
     BP_ALLOC(A.cellA, nb); BP_ALLOC(A.cellB, nb);
     BP_ALLOC(A.adj, adj);

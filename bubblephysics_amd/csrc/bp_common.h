@@ -254,14 +254,25 @@ BPD uint32_t forward_cut(const T *row, uint32_t pairEnd, uint32_t i, uint32_t li
     return forwardPairs > limit ? ((uint32_t)row[lo + limit - 1] & RowCodec<T>::kMask) : kNoCut;
 }
 
-// Marks the entries of the body run row[segEnd, end) whose body keeps k bubbles of lower index
-// than i. `hits[k]` is the length of body k's full row; `below(k)` counts, stopping at `limit`,
-// the bubbles of index < i in it.
-template <typename T, class Below>
-BPD void mark_body_cuts(T *row, uint32_t segEnd, uint32_t end, uint32_t limit, const uint32_t *hits, Below below) {
+// Index of the limit-th bubble (in index order) of body k's full row, or kNoCut when the row
+// holds at most `limit` bubbles: a body's k-entry row keeps exactly the bubbles of lower index.
+// `hits` is the row's length, `candidate(a)` the candidate predicate of bubble a.
+template <class Candidate>
+BPD uint32_t body_cut_index(uint32_t n, uint32_t limit, uint32_t hits, Candidate candidate) {
+    if (hits <= limit) return kNoCut;
+    uint32_t count = 0;
+    for (uint32_t a = 0; a < n; ++a)
+        if (candidate(a) && ++count == limit) return a;
+    return kNoCut;
+}
+
+// Marks the entries of the body run row[segEnd, end) of bubble i whose body's row does not keep
+// it: bodyCut[k] (body_cut_index) is below i, so k bubbles of lower index fill the body's row.
+template <typename T>
+BPD void mark_body_cuts(T *row, uint32_t segEnd, uint32_t end, uint32_t i, const uint32_t *bodyCut) {
     for (uint32_t e = segEnd; e < end; ++e) {
         const uint32_t k = (uint32_t)row[e] & RowCodec<T>::kMask;
-        if (hits[k] > limit && below(k) >= limit) row[e] = RowCodec<T>::kDrop;
+        if (bodyCut[k] != kNoCut && bodyCut[k] < i) row[e] = RowCodec<T>::kDrop;
     }
 }
 

@@ -63,6 +63,7 @@ struct Arena {
     float4 *cellA;
     uint4 *cellB;
     uint32_t *bodyHits;       // per body: bubbles in its candidate row at the last grid build
+    uint32_t *bodyCut;        // per body: index of the k-th bubble of its row (a row cut keeps those below)
     // Adjacency entries: (kind << 30) | neighbour, CSR per world.
     uint32_t *adj;
     // Per adjacency slot, the contact geometry the last force pass computed for that row entry

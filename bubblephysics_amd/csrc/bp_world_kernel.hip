@@ -37,17 +37,17 @@ template <int BLOCK>
 BPD void cut_rows(World &W, Shared<BLOCK> &sh, uint32_t limit, RowStats &stats, uint32_t &entries) {
     const uint32_t n = W.n;
     const uint32_t *hits = W.A->bodyHits + W.d->bodyOffset;
+    uint32_t *bodyCut = W.A->bodyCut + W.d->bodyOffset;
+    for (uint32_t k = threadIdx.x; k < W.nb; k += BLOCK)        // one lane per body
+        bodyCut[k] = body_cut_index(n, limit, hits[k], [&](uint32_t a) { return W.body_candidate(a, k); });
+    __syncthreads();
     for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
         uint32_t *row = W.adj + W.rowStart[i];
         const uint32_t len = W.rowLen[i];
         uint32_t pairEnd, segEnd, forwardPairs;
         row_blocks(row, len, pairEnd, segEnd);
         W.grid[i] = forward_cut(row, pairEnd, i, limit, forwardPairs);
-        mark_body_cuts(row, segEnd, len, limit, hits, [&](uint32_t k) {
-            uint32_t below = 0;
-            for (uint32_t a = 0; a < i && below < limit; ++a) below += W.body_candidate(a, k) ? 1u : 0u;
-            return below;
-        });
+        mark_body_cuts(row, segEnd, len, i, bodyCut);
     }
     __syncthreads();
     uint32_t dropped = 0, widest = 0, kept = 0;

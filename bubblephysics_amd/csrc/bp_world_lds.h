@@ -384,25 +384,27 @@ __device__ __attribute__((always_inline)) void lds_world_steps(LdsWorld<BLOCK, P
         // (the contact assertion hashes the grid again); the margins are still in S.aux.
         auto cut_rows = [&](uint32_t *globalRows) __attribute__((always_inline)) {
             const uint32_t *hits = g.bodyHits();
+            uint32_t *bodyCut = A->bodyCut + d->bodyOffset;
+            // One lane per body: the index of the k-th bubble of its row (a scan of the bubbles in
+            // index order that stops there), so each body entry is then decided by one compare.
+            for (uint32_t k = lane; k < g.nb; k += BLOCK)
+                bodyCut[k] = body_cut_index(n, limit, hits[k], [&](uint32_t a) {
+                    return body_candidate(S.pos[a], S.r[a], S.aux[a], catG[a], maskG[a], k);
+                });
+            __syncthreads();
     #pragma unroll
             for (int s = 0; s < PER; ++s) {
                 if (!valid[s]) continue;
                 const uint32_t i = lane + s * BLOCK, pc = rpq[s] & 0xffffu, segEnd = rpq[s] >> 16;
-                auto below = [&](uint32_t k) {
-                    uint32_t count = 0;
-                    for (uint32_t a = 0; a < i && count < limit; ++a)
-                        count += body_candidate(S.pos[a], S.r[a], S.aux[a], catG[a], maskG[a], k) ? 1u : 0u;
-                    return count;
-                };
                 uint32_t fp, cut;
                 if (rows.inLds) {
                     uint16_t *row = S.rows + rs[s];
                     cut = forward_cut(row, pc, i, limit, fp);
-                    mark_body_cuts(row, segEnd, rl[s], limit, hits, below);
+                    mark_body_cuts(row, segEnd, rl[s], i, bodyCut);
                 } else {
                     uint32_t *row = globalRows + rs[s];
                     cut = forward_cut(row, pc, i, limit, fp);
-                    mark_body_cuts(row, segEnd, rl[s], limit, hits, below);
+                    mark_body_cuts(row, segEnd, rl[s], i, bodyCut);
                 }
                 S.grid[i] = cut == kNoCut ? (uint16_t)0xffffu : (uint16_t)cut;
             }
