@@ -33,6 +33,20 @@ bp_status validate_world(const HostWorld &w) {
             return BP_ERR_INVALID_SNAPSHOT;
         }
         if (b.vertex_count == 0) { set_error("body without vertices"); return BP_ERR_INVALID_SNAPSHOT; }
+        // The bounding radius holds the body (its vertices, with the core radius around a round
+        // body's core): the kernels skip a body's exact tests for centres outside that circle.
+        double extent = 0.0;
+        const uint32_t corners = b.shape == BP_SHAPE_POLYGON ? b.vertex_count : std::min<uint32_t>(b.vertex_count, 2);
+        for (uint32_t v = 0; v < corners; ++v) {
+            const double x = w.vertices[2 * ((size_t)b.vertex_offset + v)], y = w.vertices[2 * ((size_t)b.vertex_offset + v) + 1];
+            extent = std::max(extent, std::sqrt(x * x + y * y));
+        }
+        if (b.shape != BP_SHAPE_POLYGON) extent += (double)b.core_radius;
+        if (!((double)b.bounding_radius >= extent * (1.0 - 1e-6) - 1e-6)) {
+            set_error("body " + std::to_string(b.id) + ": bounding_radius " + std::to_string(b.bounding_radius) +
+                      " is smaller than the body's extent " + std::to_string(extent));
+            return BP_ERR_INVALID_SNAPSHOT;
+        }
     }
     for (const bp_body &b : w.bodies)
         for (uint32_t t = 0; t < b.triangle_count; ++t)

@@ -302,3 +302,19 @@ def test_config3_large_world_matches_the_oracle_fixture():
             h.update(np.ascontiguousarray(f.bubbles[field]).tobytes())
         assert h.hexdigest() == want["state_sha256"], k
         assert f.execution_path == abi.BP_PATH_COOPERATIVE
+
+
+def test_undersized_bounding_radius_is_refused():
+    # The kernels skip a body's exact inside test for centres outside its bounding circle, so a
+    # body whose bounding radius does not hold its vertices (and a round body's core radius) is
+    # refused as an invalid snapshot instead of missing violations.
+    import reference_scenes as rs
+    for shrink in ("square", "capsule"):
+        w = rs.World().bubble(1, (0.0, 200.0))
+        w = w.square(2, (0.0, 0.0), 100.0, role="kinematic") if shrink == "square" else w.capsule(2, (0.0, 0.0), 40.0, 10.0)
+        scene = w.build()
+        scene.bodies[0]["bounding_radius"] = np.float32(0.9 * float(scene.bodies[0]["bounding_radius"]))
+        frame = solver.NativeWorld(scene).step(0)
+        assert frame.status == abi.BP_ERR_INVALID_SNAPSHOT, shrink
+        scene.bodies[0]["bounding_radius"] = np.float32(float(scene.bodies[0]["bounding_radius"]) / 0.9 * 1.0001)
+        assert solver.NativeWorld(scene).step(0).status == abi.BP_OK, shrink
