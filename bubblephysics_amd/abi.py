@@ -217,6 +217,62 @@ class StageScene(ctypes.Structure):
                 ("region", ctypes.c_float * 4)]
 
 
+
+# Rigid solver for worlds without bubbles (bp_rigid_*).
+class RigidBody(ctypes.Structure):      # bp_rigid_body, NativeRigidBody field for field
+    _fields_ = [("id", ctypes.c_uint64), ("mass", ctypes.c_float), ("reserved", ctypes.c_uint32),
+                ("pose", ctypes.c_float * 4), ("previous", ctypes.c_float * 4), ("motion", ctypes.c_float * 4),
+                ("physical", ctypes.c_float * 4), ("ranges", ctypes.c_uint32 * 4), ("flags", ctypes.c_uint32 * 4),
+                ("filter", ctypes.c_uint32 * 4), ("shape", ctypes.c_float * 4), ("limits", ctypes.c_float * 4)]
+
+
+class Joint(ctypes.Structure):          # bp_joint, NativeJointRecord
+    _fields_ = [("id", ctypes.c_uint64), ("reserved", ctypes.c_uint64), ("ends", ctypes.c_uint32 * 4),
+                ("anchors", ctypes.c_float * 4), ("parameters", ctypes.c_float * 4), ("limits", ctypes.c_float * 4),
+                ("state", ctypes.c_float * 4)]
+
+
+class JointDesc(ctypes.Structure):      # bp_joint_desc
+    _fields_ = [("id", ctypes.c_uint64), ("body_a", ctypes.c_uint64), ("body_b", ctypes.c_uint64),
+                ("has_body_b", ctypes.c_uint32), ("kind", ctypes.c_uint32), ("anchor_a", ctypes.c_float * 2),
+                ("anchor_b", ctypes.c_float * 2), ("length", ctypes.c_float), ("softness", ctypes.c_float),
+                ("damping", ctypes.c_float), ("reference_angle", ctypes.c_float), ("flags", ctypes.c_uint32),
+                ("collide_connected", ctypes.c_uint32), ("lower", ctypes.c_float), ("upper", ctypes.c_float),
+                ("motor_speed", ctypes.c_float), ("max_torque", ctypes.c_float)]
+
+
+class RigidConfig(ctypes.Structure):    # bp_rigid_config
+    _fields_ = [("time_step", ctypes.c_float), ("region", ctypes.c_float * 4), ("candidate_limit", ctypes.c_uint32),
+                ("iterations", ctypes.c_uint32), ("motion_prelude", ctypes.c_uint32), ("global_force", ctypes.c_float * 2)]
+
+
+class RigidFrameInfo(ctypes.Structure):  # bp_rigid_frame_info
+    _fields_ = [("status", ctypes.c_int32), ("guard_cause", ctypes.c_uint32), ("candidate_pairs", ctypes.c_uint32),
+                ("candidate_overflow", ctypes.c_uint32), ("max_candidates_per_object", ctypes.c_uint32),
+                ("removed", ctypes.c_uint32), ("wrapped", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+
+class RigidWorldDesc(ctypes.Structure):  # bp_rigid_world_desc
+    _fields_ = [("config", RigidConfig), ("bodies", ctypes.c_void_p), ("body_count", ctypes.c_uint32),
+                ("vertices", ctypes.c_void_p), ("vertex_count", ctypes.c_uint32),
+                ("triangles", ctypes.c_void_p), ("triangle_count", ctypes.c_uint32),
+                ("segments", ctypes.c_void_p), ("segment_count", ctypes.c_uint32),
+                ("joints", ctypes.c_void_p), ("joint_count", ctypes.c_uint32),
+                ("joint_references", ctypes.c_void_p), ("joint_reference_count", ctypes.c_uint32),
+                ("joint_ranges", ctypes.c_void_p),
+                ("joint_partners", ctypes.c_void_p), ("joint_partner_count", ctypes.c_uint32)]
+
+
+BP_JOINT_WORLD_END = 0xFFFFFFFF
+BP_JOINT_DISTANCE, BP_JOINT_HINGE, BP_JOINT_WELD = 0, 1, 2
+BP_JOINT_HAS_LIMITS, BP_JOINT_HAS_MOTOR = 1, 2
+BP_GUARD_NONE, BP_GUARD_RIGID_ROTATION_BUDGET = 0, 1
+BP_CONTACT_RIGID = 3
+RIGID_BODY_DTYPE = np.dtype(RigidBody)
+JOINT_DTYPE = np.dtype(Joint)
+JOINT_DESC_DTYPE = np.dtype(JointDesc)
+RIGID_FRAME_DTYPE = np.dtype(RigidFrameInfo)
+
 BP_PLACEMENT_OK, BP_PLACEMENT_OUTSIDE_REGION, BP_PLACEMENT_INSIDE_RIGID_BODY, BP_PLACEMENT_COINCIDENT_CENTER = 0, 1, 2, 3
 (BP_REJECT_NONE, BP_REJECT_INVALID_STRUCTURAL_OPERATION, BP_REJECT_INVALID_PLACEMENT, BP_REJECT_DUPLICATE_ID,
  BP_REJECT_UNKNOWN_OBJECT, BP_REJECT_INCOMPATIBLE_MOTION_ROLE, BP_REJECT_INVALID_VALUE) = 0, 1, 2, 3, 4, 5, 6
@@ -306,6 +362,17 @@ SIGNATURES = {
     "bp_batch_replay": ([_P, _P], _I32),
     "bp_batch_autotune": ([_P, _U32, _P, _U32, _P], _I32),
     "bp_batch_apply_tuning": ([_P, _P, _U32], _I32),
+    "bp_rigid_joint_tables": ([_P, _U32, _P, _U32, _P, _P, _P, _P, _P, _P, _P], _I32),
+    "bp_rigid_batch_create": ([_P], _I32),
+    "bp_rigid_batch_destroy": ([_P], None),
+    "bp_rigid_batch_add_world": ([_P, _P, _P], _I32),
+    "bp_rigid_batch_world_count": ([_P], _U32),
+    "bp_rigid_batch_set_world": ([_P, _U32, _P], _I32),
+    "bp_rigid_batch_step": ([_P, _U32, _P], _I32),
+    "bp_rigid_batch_read_bodies": ([_P, _U32, _P, _U32], _I32),
+    "bp_rigid_batch_read_joints": ([_P, _U32, _P, _U32], _I32),
+    "bp_rigid_batch_contacts": ([_P, _U32, _P, _U32, _P], _I32),
+    "bp_rigid_batch_last_gpu_milliseconds": ([_P], _F),
     "bp_server_record_sizes": ([_P], None),
     "bp_server_config_default": ([_P], None),
     "bp_server_create": ([_P, _P, _P], _I32),

@@ -13,8 +13,9 @@
 // normalised (zero when the sum is zero) and the impulse sum; ended events follow, in key order,
 // for keys of the previous set that are not current. The current keys become the new set. With
 // no object of interest and an empty previous set there are no events and the set stays empty.
-// Bubble-body records (BP_CONTACT_BODY) are the reference's rigid records (:377-380): key (bubble,
-// body), weight 1, their normal from the bubble to the body, impulse / dt, and no penetration test.
+// Bubble-body records (BP_CONTACT_BODY) and the rigid solver's body-pair and wall records
+// (BP_CONTACT_RIGID) are the reference's rigid records (:377-380): key (first, second), weight 1,
+// their normal from first to second, impulse / dt, and no penetration test.
 #include <algorithm>
 #include <cmath>
 #include <map>
@@ -59,7 +60,7 @@ extern "C" bp_status bp_contact_transitions(const bp_contact *contacts, uint32_t
     for (uint32_t k = 0; k < count; ++k) {
         const bp_contact &c = contacts[k];
         const uint64_t first = (uint64_t)c.first;
-        const bool rigid = c.kind == BP_CONTACT_BODY;
+        const bool rigid = c.kind == BP_CONTACT_BODY || c.kind == BP_CONTACT_RIGID;
         uint64_t second;
         if (c.kind == BP_CONTACT_PAIR || rigid) second = (uint64_t)c.second;
         else if (!owner_of(c.second, second)) continue;

@@ -1,0 +1,71 @@
+// bp_rigid.h -- internal: device layout of the rigid solver's batch (bp_rigid.hip kernels,
+// bp_rigid_api.cpp host side).
+#ifndef BP_RIGID_H
+#define BP_RIGID_H
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../../include/bubblephysics.h"
+
+namespace bpr {
+
+// A body-pair candidate of a body's row: NativeCandidate (NativeGeometryKernels.metal-source:27)
+// with kind 3, the pair's lower and higher body index and, in place of metadata.w, the sum of the
+// iterations' normal corrections. 32 bytes.
+struct RigidCand {
+    int64_t first, second;
+    uint32_t kind, low, high;
+    float impulse;
+};
+
+// NativeRigidCorrection (NativeGeometryKernels.metal-source:51): position and velocity
+// corrections of both ends; the w lanes carry the normal and the point. 64 bytes.
+struct RigidCorr {
+    float first[4], second[4], vfirst[4], vsecond[4];
+};
+
+// NativeRigidWall (NativeGeometryKernels.metal-source:54): the last sweep a body stopped at
+// against a static body's edge. 48 bytes.
+struct RigidWall {
+    int64_t body, owner;
+    float geometry[4];
+    float extra[4];
+};
+
+// One world of the batch: ranges of the packed arrays and the world's controls.
+struct RigidDesc {
+    uint32_t bodyOffset, bodyCount, vertexOffset, triangleOffset;
+    uint32_t segmentOffset, segmentCount, jointOffset, jointCount;
+    uint32_t refOffset, partnerOffset, hasJointTables, rowOffset;
+    uint32_t stride, k, iterations, prelude;
+    float dt, force[2], pad0;
+    float region[4];
+    uint32_t side, failed, lastOk, pad1;
+};
+
+struct RigidArena {
+    bp_rigid_body *bodies[2];     // published / working sides, indexed by bodyOffset
+    const float2 *verts;
+    const uint4 *tris;
+    const bp_segment *segs;
+    const bp_joint *jointInput;   // the world's joint records as given (state ignored)
+    bp_joint *joints;             // the step's records (state = accumulated lambdas)
+    RigidCorr *jcorr;
+    const uint2 *refs;
+    const uint4 *jranges;         // indexed by bodyOffset
+    const uint64_t *partners;
+    RigidCand *rows;              // indexed by rowOffset + body * stride + slot
+    RigidCorr *corr;
+    uint32_t *slots;              // used row slots of the step, compacted
+    uint32_t *rowCount;           // indexed by bodyOffset
+    uint32_t *rowStart;
+    RigidWall *walls;
+};
+
+}  // namespace bpr
+
+extern "C" hipError_t bp_launch_rigid_step(const bpr::RigidArena *arena, bpr::RigidDesc *descs, bp_rigid_frame_info *infos,
+                                           uint32_t worlds, uint32_t steps, hipStream_t stream);
+
+#endif  // BP_RIGID_H

@@ -340,7 +340,7 @@ bp_status bp_world_audit(bp_world *world, bp_audit *out);
  * segments; `impulse` is the contact law's force at the record's geometry times dt^2
  * (nativeContactImpulses, NativeForceKernels.metal-source:261-280). The world's rows must be kept
  * (export_adjacency, always on for bp_world) and its last step must have succeeded. 64 bytes. */
-enum { BP_CONTACT_PAIR = 0, BP_CONTACT_SEGMENT = 1, BP_CONTACT_BODY = 2 };
+enum { BP_CONTACT_PAIR = 0, BP_CONTACT_SEGMENT = 1, BP_CONTACT_BODY = 2, BP_CONTACT_RIGID = 3 };
 typedef struct bp_contact {
     uint64_t id;                /* FNV-1a of (kind, first, second), NativeMetalSolver.swift:1047-1052 */
     int64_t first;              /* bubble A id                                                     */
@@ -391,7 +391,8 @@ bp_status bp_world_contacts_noalloc(bp_world *world, bp_contact *out, uint32_t c
  * than history_capacity returns BP_ERR_CAPACITY_OVERFLOW). Host-only: no device work. Bubble-body
  * records (BP_CONTACT_BODY, bp_world_body_contacts) enter the sums as the reference's rigid records
  * (NativePublicWorld.swift:377-380): key (bubble, body), weight 1, their normal, impulse / dt, and no
- * penetration test; pass them after the pair and segment records, the reference's order. */
+ * penetration test; pass them after the pair and segment records, the reference's order. Rigid
+ * solver records (BP_CONTACT_RIGID, bp_rigid_batch_contacts) enter the same way. */
 enum { BP_EVENT_CONTACT_BEGAN = 1, BP_EVENT_CONTACT_ENDED = 2 };
 typedef struct bp_contact_event {
     uint32_t kind;              /* BP_EVENT_CONTACT_*                                             */
@@ -712,6 +713,157 @@ bp_status bp_server_read_bubbles(bp_server *server, uint32_t world, bp_bubble *o
 bp_status bp_server_get_stats(bp_server *server, bp_server_stats *out);
 /* Seconds since the server started (the clock of bp_server_result). */
 double bp_server_clock(const bp_server *server);
+
+/* ------------------------------------------------------------------------ */
+/* Rigid solver for worlds without bubbles.                                   */
+/* The rigid-body step the reference runs once per frame for every world with */
+/* rigid bodies (NativeMetalSolver.encodeStepPrefix, NativeMetalSolver.swift: */
+/* 766-799, then nativeRigidFinish at the end gate, :917-921): prediction with */
+/* the swept test against static edges, body-pair candidate rows, body-pair   */
+/* CCD, 8 iterations of contacts, joints and the wall guard, and the region   */
+/* rule. In a world with bubbles its bodies are static or kinematic and the   */
+/* bubble step above covers them; these entry points are for the worlds whose */
+/* bodies move under contacts and joints. One workgroup per world.            */
+/* ------------------------------------------------------------------------ */
+
+/* One rigid body: NativeRigidBody (NativeRigidPolygonState.swift:24-41;
+ * NativeGeometryKernels.metal-source:35-41), field for field. 160 bytes. A world's bodies are in
+ * ascending id order. Vertices are centroid-relative (bp_polygon_body_geometry); a circle has one
+ * vertex (0, 0), a capsule two (-half_length, 0) and (half_length, 0). */
+typedef struct bp_rigid_body {
+    uint64_t id;                /* identity.x: ObjectID                                                 */
+    float mass;                 /* identity.y (unused by the reference's kernels): the body's mass, read
+                                   only by the motion prelude of bp_rigid_config                        */
+    uint32_t reserved;
+    float pose[4];              /* centre of mass x, y, angle; w: angle at the start of the last step    */
+    float previous[4];          /* centre of mass at the start of the last step x, y; local centroid x, y */
+    float motion[4];            /* vx, vy, angular velocity, angular damping (1/s; 0 for kinematic)      */
+    float physical[4];          /* inverse mass, inverse moment (0 = static or kinematic), bounding radius, friction */
+    uint32_t ranges[4];         /* vertex offset, vertex count, triangle offset, triangle count          */
+    uint32_t flags[4];          /* x static, y wrap at the region, z region result (BP_REGION_*), w restitution (float bits) */
+    uint32_t filter[4];         /* category, mask, kinematic (1), layer                                  */
+    float shape[4];             /* kind (BP_SHAPE_*), core radius, half length, 0                          */
+    float limits[4];            /* maximum linear and angular speed (0 = none), CCD scratch, 0           */
+} bp_rigid_body;
+
+/* One joint: NativeJointRecord (NativeJointState.swift:6-14). Ends are body indices of the world;
+ * BP_JOINT_WORLD_END is a world point. 96 bytes. */
+#define BP_JOINT_WORLD_END 0xffffffffu
+enum { BP_JOINT_DISTANCE = 0, BP_JOINT_HINGE = 1, BP_JOINT_WELD = 2 };
+enum { BP_JOINT_HAS_LIMITS = 1u, BP_JOINT_HAS_MOTOR = 2u };
+typedef struct bp_joint {
+    uint64_t id;                /* JointID                                                               */
+    uint64_t reserved;
+    uint32_t ends[4];           /* end 0, end 1, kind (BP_JOINT_*), flags (BP_JOINT_HAS_*)               */
+    float anchors[4];           /* anchor of end 0 and end 1: body-local (geometry origin) or world point */
+    float parameters[4];        /* length, softness (compliance), damping, reference angle               */
+    float limits[4];            /* lower and upper angle, motor target angular velocity, maximum torque   */
+    float state[4];             /* accumulated lambda of the step: distance, motor (zeroed every step)    */
+} bp_joint;
+
+/* A joint as the public model states it (Joint; SceneBuilders.distanceJoint / hingeJoint /
+ * weldJoint), the input of bp_rigid_joint_tables. */
+typedef struct bp_joint_desc {
+    uint64_t id;
+    uint64_t body_a;
+    uint64_t body_b;            /* ignored unless has_body_b                                             */
+    uint32_t has_body_b;        /* 0: end b is the world point anchor_b                                  */
+    uint32_t kind;              /* BP_JOINT_*                                                            */
+    float anchor_a[2];
+    float anchor_b[2];
+    float length, softness, damping;   /* distance                                                      */
+    float reference_angle;             /* hinge and weld                                                */
+    uint32_t flags;                    /* hinge: BP_JOINT_HAS_LIMITS, BP_JOINT_HAS_MOTOR                */
+    uint32_t collide_connected;
+    float lower, upper;                /* hinge limits                                                  */
+    float motor_speed, max_torque;     /* hinge motor                                                   */
+} bp_joint_desc;
+
+/* Joint tables of one world (NativeJointState.prepare, NativeJointState.swift:73-141, for a world
+ * without bubbles): records sorted by JointID with a world end moved to end 0; per body its
+ * joint references (joint index, side) and its sorted, distinct non-colliding partners (ids).
+ * Joints naming an unknown body are skipped, as the reference skips them.
+ * ranges[4 * body_count] = (reference start, reference count, partner start, partner count);
+ * references holds 2 * joint_count pairs at most, partners 2 * joint_count ids at most. */
+bp_status bp_rigid_joint_tables(const bp_joint_desc *joints, uint32_t joint_count, const uint64_t *body_ids,
+                                uint32_t body_count, bp_joint *records, uint32_t *out_record_count,
+                                uint32_t *references, uint32_t *out_reference_count, uint32_t *ranges,
+                                uint64_t *partners, uint32_t *out_partner_count);
+
+typedef struct bp_rigid_config {
+    float time_step;            /* dt                                                                     */
+    float region[4];            /* simulation region: min.x, min.y, max.x, max.y                          */
+    uint32_t candidate_limit;   /* k per body, 1..128 (default 128)                                       */
+    uint32_t iterations;        /* contact iterations per step (NativeMetalSolver.rigidSolverIterationCount = 8) */
+    uint32_t motion_prelude;    /* 1: before each step, every dynamic body (not static, not kinematic) takes
+                                   the runtime's velocity update of MotionControllers.prepare
+                                   (MotionControllers.swift:57-77, role .dynamic, no fields):
+                                   v = v * exp(-damping dt), then v += global_force / mass * dt; damping is
+                                   motion[3]. It lets one launch carry a world with a global force through
+                                   several steps, as the public world's frames would.                    */
+    float global_force[2];      /* WorldConfiguration.globalForce (a force, not an acceleration)          */
+} bp_rigid_config;
+
+/* Cause of BP_ERR_GUARD_FAILURE (NativeMetalGuardCause, NativeSolverInput.swift:292-311). */
+enum { BP_GUARD_NONE = 0, BP_GUARD_RIGID_ROTATION_BUDGET = 1 };
+
+typedef struct bp_rigid_frame_info {
+    bp_status status;           /* BP_OK, BP_ERR_NON_FINITE or BP_ERR_GUARD_FAILURE (reference precedence) */
+    uint32_t guard_cause;       /* BP_GUARD_*                                                             */
+    uint32_t candidate_pairs;   /* body-pair entries over all rows                                        */
+    uint32_t candidate_overflow;/* candidates dropped by full rows                                        */
+    uint32_t max_candidates_per_object;
+    uint32_t removed;           /* bodies the region rule removed (flags[2] == BP_REGION_REMOVED)         */
+    uint32_t wrapped;
+    uint32_t reserved;
+} bp_rigid_frame_info;
+
+typedef struct bp_rigid_batch bp_rigid_batch;
+
+/* One world of a rigid batch. Triangles are uint4 records (a, b, c, 0) with vertex numbers
+ * local to their body; segments are the static polygon edges (bp_static_polygon_edges with the
+ * static body as owner), which the prediction's swept test reads. Joint arrays are the output of
+ * bp_rigid_joint_tables (all null / zero for a world without joints). */
+typedef struct bp_rigid_world_desc {
+    bp_rigid_config config;
+    const bp_rigid_body *bodies;  uint32_t body_count;
+    const float *vertices;        uint32_t vertex_count;    /* float2 pairs */
+    const uint32_t *triangles;    uint32_t triangle_count;  /* uint4 records */
+    const bp_segment *segments;   uint32_t segment_count;
+    const bp_joint *joints;       uint32_t joint_count;
+    const uint32_t *joint_references; uint32_t joint_reference_count;   /* (joint, side) pairs */
+    const uint32_t *joint_ranges;                                       /* uint4 per body      */
+    const uint64_t *joint_partners; uint32_t joint_partner_count;
+} bp_rigid_world_desc;
+
+bp_status bp_rigid_batch_create(bp_rigid_batch **out_batch);
+void bp_rigid_batch_destroy(bp_rigid_batch *batch);
+/* Adds a world (validated: sorted ids, ranges inside the arrays, a triangle's corners inside its
+ * body, known shapes, finite records); returns its index. Worlds are packed on the next step. */
+bp_status bp_rigid_batch_add_world(bp_rigid_batch *batch, const bp_rigid_world_desc *desc, uint32_t *out_index);
+uint32_t bp_rigid_batch_world_count(const bp_rigid_batch *batch);
+/* Replaces a world (bodies, geometry, segments, joints, configuration): the next frame's input, as
+ * the public world re-stages its bodies every frame (NativeRigidPolygonState.prepare,
+ * NativeRigidPolygonState.swift:193-301); for instance after dropping the bodies the region rule
+ * removed. A failed world steps again after new input. */
+bp_status bp_rigid_batch_set_world(bp_rigid_batch *batch, uint32_t world, const bp_rigid_world_desc *desc);
+/* Advances every world by `steps` steps in one launch (one workgroup per world, all steps inside
+ * the launch); infos[world_count * steps] (step-major) may be null. A world whose step fails keeps
+ * its last published bodies and is not stepped again until new input. */
+bp_status bp_rigid_batch_step(bp_rigid_batch *batch, uint32_t steps, bp_rigid_frame_info *infos);
+bp_status bp_rigid_batch_read_bodies(bp_rigid_batch *batch, uint32_t world, bp_rigid_body *out, uint32_t count);
+/* Joint records of a world after its last step (state = the step's accumulated lambdas). */
+bp_status bp_rigid_batch_read_joints(bp_rigid_batch *batch, uint32_t world, bp_joint *out, uint32_t count);
+/* Rigid contact records of a world's last published step (NativeRigidPolygonState.contactRecords,
+ * NativeRigidPolygonState.swift:367-428): each body pair once, from the row of its lower body
+ * index, with the last iteration's normal (from first to second) and point and the sum of the
+ * iterations' normal corrections; then the prediction's stopped sweeps against static bodies that
+ * the rows did not report. kind BP_CONTACT_RIGID, first / second = body ids, index_b = the second
+ * body's index, penetration 0. Writes up to `capacity`; *out_count receives the number of records. */
+bp_status bp_rigid_batch_contacts(bp_rigid_batch *batch, uint32_t world, bp_contact *out, uint32_t capacity,
+                                  uint32_t *out_count);
+/* Device time of the last bp_rigid_batch_step (HIP events around the launch). */
+float bp_rigid_batch_last_gpu_milliseconds(const bp_rigid_batch *batch);
 
 #ifdef __cplusplus
 }

@@ -254,3 +254,38 @@ def candidate_rows(bubbles, margins, segments, cell_width, separation_tolerance,
     lib.bpo_candidate_rows(_p(b), _p(m), n, _p(s), len(s), cell_width, separation_tolerance, limit, 1 if hashed else 0,
                            buckets, _p(rows), _p(counts), _p(overflow))
     return rows[:n], counts[:n], overflow
+
+
+def run_rigid(scene, steps: int, *, plant: int = 0, contact_capacity: int = 4096):
+    """Runs `steps` steps of a rigid world (bubblephysics_amd.rigid.RigidScene) in the oracle's
+    restatement of the reference's rigid-body frame (oracle/bp_oracle_rigid.cpp).
+
+    Returns (bodies, joints, infos[ran], contacts of the last published step). A failed step
+    publishes nothing and ends the run."""
+    from bubblephysics_amd import abi
+    lib = load()
+    if not hasattr(lib, "_rigid_ready"):
+        P, U32 = ctypes.c_void_p, ctypes.c_uint32
+        lib.bpo_rigid_run.argtypes = [P, P, U32, P, P, P, U32, P, U32, P, P, P, U32, P, P, U32, P]
+        lib.bpo_rigid_run.restype = U32
+        lib.bpo_rigid_set_plant.argtypes = [ctypes.c_int]
+        lib._rigid_ready = True
+    bodies = np.ascontiguousarray(scene.bodies.copy())
+    joints = np.ascontiguousarray(scene.joints.copy())
+    infos = np.zeros(steps, abi.RIGID_FRAME_DTYPE)
+    contacts = np.zeros(contact_capacity, abi.CONTACT_DTYPE)
+    count = ctypes.c_uint32()
+    verts = np.ascontiguousarray(scene.vertices, dtype=np.float32)
+    tris = np.ascontiguousarray(scene.triangles, dtype=np.uint32)
+    segs = np.ascontiguousarray(scene.segments)
+    refs = np.ascontiguousarray(scene.references, dtype=np.uint32)
+    ranges = None if scene.ranges is None or len(joints) == 0 else np.ascontiguousarray(scene.ranges, dtype=np.uint32)
+    partners = np.ascontiguousarray(scene.partners, dtype=np.uint64)
+    lib.bpo_rigid_set_plant(int(plant))
+    try:
+        ran = lib.bpo_rigid_run(ctypes.byref(scene.config), _p(bodies), len(bodies), _p(verts), _p(tris), _p(segs), len(segs),
+                                _p(joints), len(joints), _p(refs), _p(ranges), _p(partners), steps, _p(infos),
+                                _p(contacts), contact_capacity, ctypes.byref(count))
+    finally:
+        lib.bpo_rigid_set_plant(0)
+    return bodies, joints, infos[:ran], contacts[:min(count.value, contact_capacity)].copy()

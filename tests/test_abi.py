@@ -72,3 +72,15 @@ def test_status_strings_cover_every_code(hip_library):
     for code, name in abi.STATUS_NAMES.items():
         assert hip_library.bp_status_string(code)
     assert hip_library.bp_status_string(999) == b"unknown status"
+
+
+def test_rigid_solver_records_match_the_reference_layouts():
+    # NativeRigidBody (160 B), NativeJointRecord (96 B), NativeRigidCorrection-sized frame records.
+    assert ctypes.sizeof(abi.RigidBody) == 160 and abi.RIGID_BODY_DTYPE.itemsize == 160
+    assert ctypes.sizeof(abi.Joint) == 96 and abi.JOINT_DTYPE.itemsize == 96
+    assert ctypes.sizeof(abi.RigidFrameInfo) == 32 and ctypes.sizeof(abi.RigidConfig) == 40
+    text = (ROOT / "include" / "bubblephysics.h").read_text()
+    for name in ("BP_JOINT_DISTANCE", "BP_JOINT_HINGE", "BP_JOINT_WELD", "BP_JOINT_HAS_LIMITS", "BP_JOINT_HAS_MOTOR",
+                 "BP_GUARD_NONE", "BP_GUARD_RIGID_ROTATION_BUDGET", "BP_CONTACT_RIGID"):
+        (value,) = re.findall(rf"\b{name} = (\d+)", text)
+        assert int(value) == getattr(abi, name), name
