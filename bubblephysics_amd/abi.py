@@ -263,6 +263,23 @@ class RigidWorldDesc(ctypes.Structure):  # bp_rigid_world_desc
                 ("joint_partners", ctypes.c_void_p), ("joint_partner_count", ctypes.c_uint32)]
 
 
+
+class ContourConfig(ctypes.Structure):    # bp_contour_config
+    _fields_ = [("max_segment_length", ctypes.c_float), ("surface_tension", ctypes.c_float),
+                ("position_tolerance", ctypes.c_float), ("reserved", ctypes.c_uint32)]
+
+
+class BubbleContour(ctypes.Structure):    # bp_bubble_contour
+    _fields_ = [("count", ctypes.c_uint32), ("maximum", ctypes.c_float), ("effective_radius", ctypes.c_float),
+                ("reserved", ctypes.c_float), ("bounds", ctypes.c_float * 4)]
+
+
+def contour_config(max_segment_length=8.0, surface_tension=20.0, position_tolerance=1e-4) -> ContourConfig:
+    """NativeSolverConfiguration's contour defaults (NativeSolverInput.swift:128-132)."""
+    c = ContourConfig()
+    c.max_segment_length, c.surface_tension, c.position_tolerance = max_segment_length, surface_tension, position_tolerance
+    return c
+
 BP_JOINT_WORLD_END = 0xFFFFFFFF
 BP_JOINT_DISTANCE, BP_JOINT_HINGE, BP_JOINT_WELD = 0, 1, 2
 BP_JOINT_HAS_LIMITS, BP_JOINT_HAS_MOTOR = 1, 2
@@ -272,6 +289,7 @@ RIGID_BODY_DTYPE = np.dtype(RigidBody)
 JOINT_DTYPE = np.dtype(Joint)
 JOINT_DESC_DTYPE = np.dtype(JointDesc)
 RIGID_FRAME_DTYPE = np.dtype(RigidFrameInfo)
+CONTOUR_DTYPE = np.dtype(BubbleContour)
 
 BP_PLACEMENT_OK, BP_PLACEMENT_OUTSIDE_REGION, BP_PLACEMENT_INSIDE_RIGID_BODY, BP_PLACEMENT_COINCIDENT_CENTER = 0, 1, 2, 3
 (BP_REJECT_NONE, BP_REJECT_INVALID_STRUCTURAL_OPERATION, BP_REJECT_INVALID_PLACEMENT, BP_REJECT_DUPLICATE_ID,
@@ -362,6 +380,9 @@ SIGNATURES = {
     "bp_batch_replay": ([_P, _P], _I32),
     "bp_batch_autotune": ([_P, _U32, _P, _U32, _P], _I32),
     "bp_batch_apply_tuning": ([_P, _P, _U32], _I32),
+    "bp_world_contour_stride": ([_P, _P, _P], _I32),
+    "bp_world_contours": ([_P, _P, _U32, _P, _P, _U32], _I32),
+    "bp_batch_contours": ([_P, _U32, _P, _U32, _P, _P, _U32], _I32),
     "bp_rigid_joint_tables": ([_P, _U32, _P, _U32, _P, _P, _P, _P, _P, _P, _P], _I32),
     "bp_rigid_batch_create": ([_P], _I32),
     "bp_rigid_batch_destroy": ([_P], None),

@@ -1116,6 +1116,63 @@ bp_status bp_batch::contacts(uint32_t wi, bp_contact *out, uint32_t capacity, ui
     return BP_OK;
 }
 
+// Contour stride of a world's bubbles (NativeMetalSolver.swift:407-411): max over bubbles of
+// max(64, ceil(4 pi r / maxContourSegmentLength)), in double as the reference computes it.
+bp_status bp_batch::contour_stride(uint32_t wi, const bp_contour_config *cfg, uint32_t *stride) {
+    if (wi >= worlds.size() || !cfg || !stride) return BP_ERR_INVALID_ARGUMENT;
+    if (!(cfg->max_segment_length > 0.0f) || !std::isfinite(cfg->max_segment_length)) {
+        set_error("max_segment_length must be finite and positive");
+        return BP_ERR_INVALID_ARGUMENT;
+    }
+    double best = 64.0;
+    for (const bp_bubble &b : worlds[wi].bubbles)
+        best = std::max(best, std::ceil(4.0 * M_PI * (double)b.radius / (double)cfg->max_segment_length));
+    if (!(best <= 4294967295.0)) { set_error("contour stride overflows"); return BP_ERR_ALLOCATION_FAILED; }
+    *stride = (uint32_t)best;
+    return BP_OK;
+}
+
+bp_status bp_batch::contours(uint32_t wi, const bp_contour_config *cfg, uint32_t stride, float *points,
+                             bp_bubble_contour *shapes, uint32_t count) {
+    if (wi >= worlds.size() || !cfg || !points || !shapes) return BP_ERR_INVALID_ARGUMENT;
+    if (!(cfg->max_segment_length > 0.0f) || !std::isfinite(cfg->max_segment_length) || !(cfg->surface_tension >= 0.0f) ||
+        !std::isfinite(cfg->surface_tension) || !(cfg->position_tolerance >= 0.0f) || !std::isfinite(cfg->position_tolerance)) {
+        set_error("contour configuration: positive segment length, non-negative tension and tolerance");
+        return BP_ERR_INVALID_ARGUMENT;
+    }
+    if (!packed) {
+        bp_status s = pack();
+        if (s != BP_OK) return s;
+    }
+    if (!worlds[wi].resident) { set_error("world is not resident"); return BP_ERR_UNAVAILABLE; }
+    bpd::WorldDesc d;
+    BP_HIP(hipMemcpy(&d, descs + wi, sizeof(d), hipMemcpyDeviceToHost));
+    if (count != d.bubbleCount) { set_error("count must equal the world's bubble count"); return BP_ERR_INVALID_ARGUMENT; }
+    if (!d.exportAdjacency) {
+        set_error("contours read the rows of the last grid build: set export_adjacency");
+        return BP_ERR_UNAVAILABLE;
+    }
+    uint32_t need = 0;
+    bp_status s = contour_stride(wi, cfg, &need);
+    if (s != BP_OK) return s;
+    if (stride < need) { set_error("stride is below the world's contour stride"); return BP_ERR_INVALID_ARGUMENT; }
+    if (stride > bp_contour_capacity()) { set_error("contour stride above the kernel's 2,048 samples"); return BP_ERR_CAPACITY_OVERFLOW; }
+    if (count == 0) return BP_OK;
+    void *dev = nullptr;
+    const size_t pointBytes = (size_t)count * stride * sizeof(float2);
+    const size_t shapeOffset = (pointBytes + 255) / 256 * 256;
+    BP_HIP(hipMalloc(&dev, shapeOffset + (size_t)count * sizeof(bp_bubble_contour)));
+    float2 *devPoints = static_cast<float2 *>(dev);
+    auto *devShapes = reinterpret_cast<bp_bubble_contour *>(static_cast<char *>(dev) + shapeOffset);
+    hipError_t e = bp_launch_contours(arenaDev, descs, wi, count, *cfg, stride, devPoints, devShapes, streams[0]);
+    if (e == hipSuccess) e = hipMemcpyAsync(points, devPoints, pointBytes, hipMemcpyDeviceToHost, streams[0]);
+    if (e == hipSuccess) e = hipMemcpyAsync(shapes, devShapes, count * sizeof(bp_bubble_contour), hipMemcpyDeviceToHost, streams[0]);
+    if (e == hipSuccess) e = hipStreamSynchronize(streams[0]);
+    (void)hipFree(dev);
+    if (e != hipSuccess) { set_error(std::string("contours: ") + hipGetErrorString(e)); return BP_ERR_COMMAND_FAILED; }
+    return BP_OK;
+}
+
 bp_status bp_batch::step(uint32_t steps, bp_frame_info *infos) {
     bp_status s = launch(nullptr, 0, steps);
     if (s != BP_OK) return s;
@@ -1524,6 +1581,14 @@ bp_status bp_batch_body_contacts(bp_batch *b, uint32_t world, bp_contact *out, u
     return b->contacts(world, out, capacity, out_count, true);
 }
 
+bp_status bp_batch_contours(bp_batch *b, uint32_t world, const bp_contour_config *config, uint32_t stride, float *points,
+                            bp_bubble_contour *shapes, uint32_t bubble_count) {
+    if (!b) return BP_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(b->mutex);
+    if (b->inFlight || b->pending || b->serving) return BP_ERR_FRAME_IN_FLIGHT;
+    return b->contours(world, config, stride, points, shapes, bubble_count);
+}
+
 bp_status bp_batch_set_world_class(bp_batch *b, uint32_t world, int32_t cls) {
     if (!b || world >= b->worlds.size() || cls < -1 || cls >= kClassCount) return BP_ERR_INVALID_ARGUMENT;
     std::lock_guard<std::mutex> lock(b->mutex);
@@ -1798,6 +1863,18 @@ bp_status bp_world_contacts(bp_world *w, bp_contact *out, uint32_t capacity, uin
     if (!w || !out_count) return BP_ERR_INVALID_ARGUMENT;
     if (w->inFlight) return BP_ERR_FRAME_IN_FLIGHT;
     return w->batch.contacts(0, out, capacity, out_count);
+}
+
+bp_status bp_world_contour_stride(bp_world *w, const bp_contour_config *config, uint32_t *out_stride) {
+    if (!w) return BP_ERR_INVALID_ARGUMENT;
+    return w->batch.contour_stride(0, config, out_stride);
+}
+
+bp_status bp_world_contours(bp_world *w, const bp_contour_config *config, uint32_t stride, float *points,
+                            bp_bubble_contour *shapes, uint32_t bubble_count) {
+    if (!w) return BP_ERR_INVALID_ARGUMENT;
+    if (w->inFlight) return BP_ERR_FRAME_IN_FLIGHT;
+    return w->batch.contours(0, config, stride, points, shapes, bubble_count);
 }
 
 bp_status bp_world_body_contacts(bp_world *w, bp_contact *out, uint32_t capacity, uint32_t *out_count) {

@@ -42,6 +42,10 @@ extern "C" hipError_t bp_launch_contacts(const bpd::Arena *arena, const bpd::Wor
 extern "C" hipError_t bp_launch_body_contacts(const bpd::Arena *arena, const bpd::WorldDesc *descs, uint32_t world,
                                               uint32_t *offsets, void *out, uint32_t capacity, uint32_t *total,
                                               hipStream_t stream);
+extern "C" hipError_t bp_launch_contours(const bpd::Arena *arena, const bpd::WorldDesc *descs, uint32_t world,
+                                         uint32_t bubbles, bp_contour_config cfg, uint32_t stride, float2 *points,
+                                         bp_bubble_contour *shapes, hipStream_t stream);
+extern "C" uint32_t bp_contour_capacity();
 extern "C" hipError_t bp_launch_world_coop(const bpd::Arena *arena, bpd::WorldDesc *descs, bpd::WorldStatus *stats,
                                            const uint32_t *orderEntry, uint32_t world, uint32_t groups, hipStream_t stream);
 
@@ -310,6 +314,9 @@ struct bp_batch {
     bp_status step(uint32_t steps, bp_frame_info *infos);
     bp_status audit(bp_audit *out, uint32_t count);
     bp_status contacts(uint32_t world, bp_contact *out, uint32_t capacity, uint32_t *count, bool bodies = false);
+    bp_status contour_stride(uint32_t world, const bp_contour_config *cfg, uint32_t *stride);
+    bp_status contours(uint32_t world, const bp_contour_config *cfg, uint32_t stride, float *points, bp_bubble_contour *shapes,
+                       uint32_t count);
     bp_status capture(uint32_t steps);
     bp_status replay(bp_frame_info *infos);
     bp_status collect(uint32_t steps, bp_frame_info *infos, const std::vector<uint32_t> &listed, float ms);

@@ -207,6 +207,24 @@ class NativeWorld:
         return _read_contacts(lambda out, cap, count: self._lib.bp_world_contacts(self._handle, out, cap, count),
                               "bp_world_contacts", capacity)
 
+    def contour_stride(self, config: abi.ContourConfig | None = None) -> int:
+        cfg = config or abi.contour_config()
+        out = ctypes.c_uint32()
+        abi.check(self._lib.bp_world_contour_stride(self._handle, ctypes.byref(cfg), ctypes.byref(out)), "bp_world_contour_stride")
+        return out.value
+
+    def contours(self, config: abi.ContourConfig | None = None, stride: int | None = None):
+        """Draw-only contours of the published state (bp_world_contours): (points [n, stride, 2],
+        shapes [n])."""
+        cfg = config or abi.contour_config()
+        stride = stride or self.contour_stride(cfg)
+        n = self._count
+        points = np.zeros((n, stride, 2), np.float32)
+        shapes = np.zeros(n, abi.CONTOUR_DTYPE)
+        abi.check(self._lib.bp_world_contours(self._handle, ctypes.byref(cfg), stride, abi.ptr(points), abi.ptr(shapes), n),
+                  "bp_world_contours")
+        return points, shapes
+
     def body_contacts(self, capacity: int | None = None) -> np.ndarray:
         """End-of-step bubble-body contact records (bp_contact, kind BP_CONTACT_BODY)."""
         return _read_contacts(lambda out, cap, count: self._lib.bp_world_body_contacts(self._handle, out, cap, count),
@@ -383,6 +401,16 @@ class Batch:
         """End-of-step bubble-body contact records (bp_contact, kind BP_CONTACT_BODY) of one world."""
         return _read_contacts(lambda out, cap, count: self._lib.bp_batch_body_contacts(self._handle, world, out, cap, count),
                               "bp_batch_body_contacts", capacity)
+
+    def contours(self, world: int, stride: int, config: abi.ContourConfig | None = None):
+        """Draw-only contours of one world (bp_batch_contours): (points [n, stride, 2], shapes [n])."""
+        cfg = config or abi.contour_config()
+        n = self._counts[world]
+        points = np.zeros((n, stride, 2), np.float32)
+        shapes = np.zeros(n, abi.CONTOUR_DTYPE)
+        abi.check(self._lib.bp_batch_contours(self._handle, world, ctypes.byref(cfg), stride, abi.ptr(points), abi.ptr(shapes),
+                                              n), "bp_batch_contours")
+        return points, shapes
 
     def capture(self, steps: int = 1):
         """Captures `steps` steps of every world into a hipGraph (bp_batch_capture)."""

@@ -380,6 +380,39 @@ bp_status bp_world_body_contacts(bp_world *world, bp_contact *out, uint32_t capa
  * step since its last upload), else BP_ERR_UNAVAILABLE. Results equal bp_world_contacts. */
 bp_status bp_world_contacts_noalloc(bp_world *world, bp_contact *out, uint32_t capacity, uint32_t *out_count);
 
+/* Contour and render preparation of a world's published state: the draw-only contour the
+ * reference builds at the end of every frame (NativeMetalSolver.swift:884-903). Per bubble, rays at
+ * `count` angles are cut by every segment the bubble's filter accepts and by the flat chord each
+ * overlapping pair partner leaves at r (1 - eps) from the centre, eps the contact law's deflection
+ * of this bubble (nativeAccumulateContactPressure and nativeContourRayLimit,
+ * NativePostSolveKernels.metal-source:268-337); the deficits are smoothed by 13 Jacobi passes that
+ * never go below the cut (nativeGenerateContours, :341-386), closed by the smallest slope-bounded
+ * majorant from the deepest sample, and turned into points, bounds, the largest deficit and the
+ * effective radius (nativePrepareRenderData, :388-441). count = min(stride, max(64, ceil(4 pi r /
+ * max_segment_length))). Pair partners are the overlapping pair entries of the bubble's row of the
+ * last grid build (the end-of-step contact records). The contour never feeds back into the step. */
+typedef struct bp_contour_config {
+    float max_segment_length;   /* maxContourSegmentLength (default 8)                        */
+    float surface_tension;      /* contourSurfaceTension (default 20)                          */
+    float position_tolerance;   /* positionTolerance (default 1e-4): the rays' clearance       */
+    uint32_t reserved;
+} bp_contour_config;
+typedef struct bp_bubble_contour {
+    uint32_t count;             /* points of the bubble's contour                              */
+    float maximum;              /* deformation.x: the largest deficit (fraction of the radius) */
+    float effective_radius;     /* deformation.y: the nearest contour point's distance          */
+    float reserved;
+    float bounds[4];            /* min.x, min.y, max.x, max.y of the contour and the centre    */
+} bp_bubble_contour;
+/* The contour stride of a world's published bubbles: max over bubbles of
+ * max(64, ceil(4 pi r / max_segment_length)) (NativeMetalSolver.swift:407-411). */
+bp_status bp_world_contour_stride(bp_world *world, const bp_contour_config *config, uint32_t *out_stride);
+/* points holds 2 * stride floats per bubble (bubble i's samples from 2 * i * stride); stride must be
+ * at least the world's contour stride and at most 2048; shapes holds one record per bubble. The
+ * world's rows must be kept (export_adjacency) and its last step must have succeeded. */
+bp_status bp_world_contours(bp_world *world, const bp_contour_config *config, uint32_t stride, float *points,
+                            bp_bubble_contour *shapes, uint32_t bubble_count);
+
 /* Contact transitions of one step: the contactBegan / contactEnded events of
  * NativePublicWorld.swift:340-391 derived from the step's contact records (bp_world_contacts) and
  * the previous step's set of contact pairs. Keys are (bubble A, bubble B) or (bubble A, segment
@@ -559,6 +592,9 @@ bp_status bp_batch_audit(bp_batch *batch, bp_audit *out, uint32_t count);
 bp_status bp_batch_contacts(bp_batch *batch, uint32_t world, bp_contact *out, uint32_t capacity, uint32_t *out_count);
 /* End-of-step bubble-body contact records (bp_world_body_contacts) of one world of a batch. */
 bp_status bp_batch_body_contacts(bp_batch *batch, uint32_t world, bp_contact *out, uint32_t capacity, uint32_t *out_count);
+/* Contours (bp_world_contours) of one world of a batch. */
+bp_status bp_batch_contours(bp_batch *batch, uint32_t world, const bp_contour_config *config, uint32_t stride, float *points,
+                            bp_bubble_contour *shapes, uint32_t bubble_count);
 /* Autotuner: for each bin of world sizes (bin b: 2^b to 2^(b+1) - 1 bubbles), times one step of
  * the bin's worlds in every kernel configuration that holds them all (the LDS-resident
  * configurations and the one-workgroup global-state kernel: workgroup shape and the LDS-to-HBM

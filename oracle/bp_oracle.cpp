@@ -1641,6 +1641,124 @@ void bpo_candidate_rows(const bp_bubble *bubbles, const float *margins, uint32_t
 }
 
 // Planted mistake for the tests that show a pin catches it (0 switches every plant off).
+// Contour and render preparation of a world's published state (draw-only):
+// nativeAccumulateContactPressure (PostSolve:280-337) with nativeContourRayLimit (:268-277),
+// 13 passes of nativeGenerateContours (:341-355; NativeMetalContourSmoothing) and
+// nativePrepareRenderData (:382-441). A bubble's pair chords come from its row's pair entries
+// (the end-of-step contact records of the last grid build) that overlap it. `stride` is the
+// contour stride of the world (at least 64); points[2 * (i * stride + s)] receives the samples and
+// shapes[i] the bubble's count, largest deficit, effective radius and bounds.
+// Planted mistake 6: the pair chord at the radius instead of r (1 - eps).
+void bpo_contours(const bp_contour_config *config, const bp_bubble *bubbles, uint32_t n, const bp_segment *segments, uint32_t ns,
+                  const uint32_t *row_offsets, const uint32_t *entries, uint32_t stride, float *points, bp_bubble_contour *shapes) {
+    using P = bpo::V2<float>;
+    const float maxEdge = config->max_segment_length, tensionParam = config->surface_tension,
+                clearance = config->position_tolerance;
+    const float pi = 3.14159265358979323846f;
+    auto ray = [](P center, P direction, P a, P b, float radius, float clear) {   // PostSolve:268-277
+        const P edge = b - a, offset = a - center;
+        const float denominator = bpo::cross(direction, edge);
+        if (std::fabs(denominator) < 1e-6f) return radius;
+        const float distance = bpo::cross(offset, edge) / denominator;
+        const float fraction = bpo::cross(offset, direction) / denominator;
+        return distance >= 0 && fraction >= 0 && fraction <= 1 ? bpo::vmin(radius, bpo::vmax(0.0f, distance - clear)) : radius;
+    };
+    std::vector<float> x, y, next;
+    for (uint32_t i = 0; i < n; ++i) {
+        const bp_bubble &bubble = bubbles[i];
+        const P pos = bpo::v2(bubble.center[0], bubble.center[1]);
+        const float r = bubble.radius;
+        const uint32_t count = std::min(stride, std::max(64u, (uint32_t)std::ceil((r / maxEdge) * (4 * pi))));
+        x.assign(count, 0.0f);
+        y.assign(count, 0.0f);
+        for (uint32_t sample = 0; sample < count; ++sample) {
+            const float angle = 2 * pi * float(sample) / float(count);
+            const P direction = bpo::v2(bpm::cosf_det(angle), bpm::sinf_det(angle));
+            float limit = r;
+            for (uint32_t j = 0; j < ns; ++j) {
+                const bp_segment &sg = segments[j];
+                if (!((bubble.category & sg.mask) != 0 && (sg.category & bubble.mask) != 0)) continue;
+                limit = ray(pos, direction, bpo::v2(sg.a[0], sg.a[1]), bpo::v2(sg.b[0], sg.b[1]), limit, clearance);
+            }
+            for (uint32_t e = row_offsets[i]; e < row_offsets[i + 1]; ++e) {
+                if ((entries[e] >> 30) != 0) continue;
+                const uint32_t j = entries[e] & ((1u << 30) - 1u);
+                const bp_bubble &other = bubbles[j];
+                const P delta = bpo::v2(other.center[0], other.center[1]) - pos;
+                const float distance = bpo::length(delta);
+                const float otherRadius = other.radius;
+                if (distance <= FLT_EPSILON || distance >= r + otherRadius) continue;
+                const P normal = delta / distance;
+                const bool lowIsSelf = i < j;
+                const bp_bubble &first = lowIsSelf ? bubble : other, &second = lowIsSelf ? other : bubble;
+                const bpo::Law<float> law = bpo::pair_contact_law<float>(first.contact_stiffness, first.radius,
+                                                                         second.contact_stiffness, second.radius, distance);
+                const float epsilon = lowIsSelf ? law.epsA : law.epsB;
+                float planeDistance = bpo::vclamp(r * (1 - epsilon), 0.0f, r);
+                if (bpo::g_plant == 6) planeDistance = r;
+                const float halfLength = std::sqrt(bpo::vmax(0.0f, r * r - planeDistance * planeDistance));
+                const P tangent = bpo::v2(-normal.y, normal.x);
+                const P point = pos + normal * planeDistance;
+                limit = ray(pos, direction, point - tangent * halfLength, point + tangent * halfLength, limit, clearance);
+            }
+            const float deficit = bpo::vclamp(1 - limit / r, 0.0f, 1.0f);
+            x[sample] = deficit;
+            y[sample] = deficit;
+        }
+        const float tension = bpo::vmin(0.8f, tensionParam / (tensionParam + 10));
+        next.assign(count, 0.0f);
+        for (int pass = 0; pass < 13; ++pass) {
+            for (uint32_t sample = 0; sample < count; ++sample) {
+                const float neighbor = (x[(sample + count - 1) % count] + x[(sample + 1) % count]) * 0.5f;
+                const float mixed = x[sample] + (neighbor - x[sample]) * tension;
+                next[sample] = bpo::vclamp(bpo::vmax(y[sample], mixed), 0.0f, 1.0f);
+            }
+            x.swap(next);
+        }
+        float maximum = 0;
+        uint32_t anchor = 0;
+        for (uint32_t sample = 0; sample < count; ++sample)
+            if (x[sample] > maximum) { maximum = x[sample]; anchor = sample; }
+        const float slope = maxEdge / (2 * r);
+        float previous = maximum;
+        for (uint32_t step = 1; step < count; ++step) {
+            const uint32_t sample = (anchor + step) % count;
+            const float value = bpo::vmax(x[sample], previous - slope);
+            x[sample] = value;
+            previous = value;
+        }
+        previous = maximum;
+        for (uint32_t step = 1; step < count; ++step) {
+            const uint32_t sample = (anchor + count - step) % count;
+            const float value = bpo::vmax(x[sample], previous - slope);
+            x[sample] = value;
+            previous = value;
+        }
+        float effectiveRadius = r;
+        P lo = pos, hi = pos;
+        for (uint32_t sample = 0; sample < count; ++sample) {
+            const float angle = 2 * pi * float(sample) / float(count);
+            const P direction = bpo::v2(bpm::cosf_det(angle), bpm::sinf_det(angle));
+            const P point = pos + direction * r * (1 - x[sample]);
+            points[2 * ((size_t)i * stride + sample)] = point.x;
+            points[2 * ((size_t)i * stride + sample) + 1] = point.y;
+            lo = bpo::vmin2(lo, point);
+            hi = bpo::vmax2(hi, point);
+            effectiveRadius = bpo::vmin(effectiveRadius, bpo::length(point - pos));
+            maximum = bpo::vmax(maximum, x[sample]);
+        }
+        bp_bubble_contour &out = shapes[i];
+        out.count = count;
+        out.maximum = maximum;
+        out.effective_radius = effectiveRadius;
+        out.reserved = 0;
+        out.bounds[0] = lo.x;
+        out.bounds[1] = lo.y;
+        out.bounds[2] = hi.x;
+        out.bounds[3] = hi.y;
+    }
+}
+
 void bpo_set_plant(int code) { bpo::g_plant = code; }
 void bpo_set_threads(int threads) { bpo::g_threads = threads < 1 ? 1 : threads; }
 

@@ -289,3 +289,33 @@ def run_rigid(scene, steps: int, *, plant: int = 0, contact_capacity: int = 4096
     finally:
         lib.bpo_rigid_set_plant(0)
     return bodies, joints, infos[:ran], contacts[:min(count.value, contact_capacity)].copy()
+
+
+def contour_stride(bubbles, config) -> int:
+    """max over bubbles of max(64, ceil(4 pi r / max_segment_length)) in double (NativeMetalSolver.swift:407-411)."""
+    import math
+    best = 64.0
+    for r in bubbles["radius"]:
+        best = max(best, math.ceil(4.0 * math.pi * float(r) / float(np.float32(config.max_segment_length))))
+    return int(best)
+
+
+def contours(bubbles, segments, row_offsets, entries, config, stride=None):
+    """Draw-only contours of a world state from the rows of its last grid build (oracle restatement
+    of nativeAccumulateContactPressure, nativeGenerateContours and nativePrepareRenderData):
+    (points [n, stride, 2], shapes [n])."""
+    from bubblephysics_amd import abi
+    lib = load()
+    if not hasattr(lib, "_contours_ready"):
+        P, U32 = ctypes.c_void_p, ctypes.c_uint32
+        lib.bpo_contours.argtypes = [P, P, U32, P, U32, P, P, U32, P, P]
+        lib._contours_ready = True
+    stride = stride or contour_stride(bubbles, config)
+    b = np.ascontiguousarray(bubbles)
+    n = len(b)
+    points = np.zeros((n, stride, 2), np.float32)
+    shapes = np.zeros(n, abi.CONTOUR_DTYPE)
+    segs = np.ascontiguousarray(segments)
+    lib.bpo_contours(ctypes.byref(config), _p(b), n, _p(segs), len(segs), _p(np.ascontiguousarray(row_offsets, np.uint32)),
+                     _p(np.ascontiguousarray(entries, np.uint32)), stride, _p(points), _p(shapes))
+    return points, shapes
