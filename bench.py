@@ -13,7 +13,9 @@ frames. Beside it, on the same box:
   * serving the batched worlds at rising offered loads, and the 8,192-world serving workload
     (60 s), then again with a device-memory budget that forces evictions;
   * the CPU oracle on the host cores: the batch, and the single worlds on one thread and with its
-    per-phase threads.
+    per-phase threads;
+  * the rigid solver for worlds without bubbles: 4,096 seeded rigid crowds in one launch
+    (body-steps/s), with the oracle's rigid solver on the host cores beside it.
 
 One JSON line on stdout. value = bubble-substeps/s = sum over worlds and frames of
 (bubbles x substeps) / wall time of the timed call, with every world resident on the GPU first.
@@ -262,6 +264,55 @@ def run_served(args):
 # ---------------------------------------------------------------------------------------------
 # CPU baselines (the oracle; test infrastructure used here only as the timed CPU restatement)
 # ---------------------------------------------------------------------------------------------
+def rigid_scenes(worlds: int, bodies: int):
+    """Seeded rigid crowds (bubblephysics_amd.rigid.crowd): `bodies` dynamic bodies of four shapes,
+    six statics and kinematics, five joints, gravity, in a walled 400 x 500 box."""
+    from bubblephysics_amd import rigid
+    out = []
+    for w in range(worlds):
+        objs, cfg, joints = rigid.crowd(50_000 + w, bodies=bodies)
+        out.append(rigid.records_for(objs, cfg, joints=joints))
+    return out
+
+
+def run_rigid(args, scenes_rigid):
+    """Rigid solver for worlds without bubbles: every world in one launch, all timed steps inside it."""
+    from bubblephysics_amd import rigid
+    batch = rigid.RigidBatch(scenes_rigid)
+    batch.step(args.warmup)
+    t0 = time.perf_counter()
+    infos = batch.step(args.rigid_steps)
+    elapsed = time.perf_counter() - t0
+    bodies = sum(len(s.bodies) for s in scenes_rigid)
+    ok = int((infos["status"] == 0).all(axis=0).sum())
+    return {"value": bodies * args.rigid_steps / elapsed, "elapsed": elapsed, "gpu_ms": batch.last_gpu_ms(),
+            "bodies": bodies, "worlds": len(scenes_rigid), "ok_worlds": ok,
+            "pairs_per_step": float(infos["candidate_pairs"].mean(axis=1).sum())}
+
+
+def cpu_rigid(scenes_rigid, threads, warmup, steps, budget_s):
+    """The oracle's rigid solver on the host cores: worlds spread over `threads` threads, each run
+    for the GPU line's warm-up and timed steps; as many worlds as fit about `budget_s` of work."""
+    from oracle import bp_oracle
+    import concurrent.futures as cf
+    probe = time.perf_counter()
+    bp_oracle.run_rigid(scenes_rigid[0], warmup + steps)
+    per_world = max(time.perf_counter() - probe, 1e-4)
+    count = int(min(len(scenes_rigid), max(threads, budget_s * threads / per_world)))
+    sample = scenes_rigid[:count]
+    warmed = []
+    for sc in sample:
+        b, j, _, _ = bp_oracle.run_rigid(sc, warmup)
+        c = type(sc)(**{**sc.__dict__, "bodies": b})
+        warmed.append(c)
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(max_workers=threads) as pool:
+        list(pool.map(lambda sc: bp_oracle.run_rigid(sc, steps), warmed))
+    elapsed = time.perf_counter() - t0
+    bodies = sum(len(sc.bodies) for sc in sample)
+    return {"value": bodies * steps / elapsed, "worlds": count, "bodies": bodies, "threads": threads, "seconds": elapsed}
+
+
 def cpu_batch(scenes_list, budget_s, threads, warmup, steps):
     """The oracle (FP32) on a random sample of the batch's worlds with the GPU line's schedule:
     first every sampled world's `warmup` untimed steps (worlds are added to the sample until the
@@ -395,6 +446,9 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=min(os.cpu_count() or 1, HOST_THREADS))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-bytes", type=int, default=0, help="only run the copy calibration kernel of this size")
+    ap.add_argument("--rigid-worlds", type=int, default=4096, help="rigid worlds without bubbles (0 skips the line)")
+    ap.add_argument("--rigid-bodies", type=int, default=48, help="dynamic bodies per rigid world")
+    ap.add_argument("--rigid-steps", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: the worlds are independent, so a multi-GPU run would shard them by "
@@ -424,6 +478,16 @@ def main():
     large = run_large(args.large_steps) if args.large_steps > 0 else None
     if large:
         log(f"large done: {large['value']:.3g} bubble-substeps/s")
+    rigid_line = None
+    if args.rigid_worlds > 0:
+        t_build = time.perf_counter()
+        rigid_list = rigid_scenes(args.rigid_worlds, args.rigid_bodies)
+        log(f"built {len(rigid_list)} rigid worlds in {time.perf_counter() - t_build:.1f}s")
+        rigid_line = run_rigid(args, rigid_list)
+        log(f"rigid: {rigid_line['value']:.3g} body-steps/s, {rigid_line['gpu_ms']:.2f} ms per {args.rigid_steps}-step launch")
+        if not args.no_cpu:
+            rigid_line["cpu"] = cpu_rigid(rigid_list, args.cpu_threads, args.warmup, args.rigid_steps, args.cpu_seconds / 3)
+            log(f"rigid cpu: {rigid_line['cpu']['value']:.3g} body-steps/s")
     served_c4 = run_served_sweep(args, scenes_list, plain_value) if args.serve_c4_seconds > 0 and args.serve_c4_rates else None
     served, evict = run_served(args) if args.serve_seconds > 0 else (None, None)
     cpu = None
@@ -481,6 +545,18 @@ def main():
             "workload": f"config 3: 1 world, {large['bubbles']} bubbles + {large['bodies']} bodies (kinematic and walls), "
                         "cooperative grid", "value": large["value"], "ms_per_step": large["ms_per_step"],
             "gpu_ms_per_step": large["gpu_ms_per_step"], "statuses": large["statuses"]},
+        "rigid": None if rigid_line is None else {
+            "workload": f"rigid worlds without bubbles: {rigid_line['worlds']} seeded crowds of {args.rigid_bodies} dynamic "
+                        f"bodies (boxes, circles, capsules, convex polygons) + 6 statics and kinematics, 5 joints, gravity; "
+                        f"{args.rigid_steps} steps in one launch, one workgroup per world",
+            "metric": "rigid body-steps/s", "value": rigid_line["value"], "unit": "body-steps/s",
+            "ms_per_step": rigid_line["elapsed"] / args.rigid_steps * 1e3,
+            "kernel_ms_per_launch": rigid_line["gpu_ms"], "bodies": rigid_line["bodies"], "ok_worlds": rigid_line["ok_worlds"],
+            "candidate_pairs_per_step": rigid_line["pairs_per_step"],
+            "cpu_baseline": None if "cpu" not in rigid_line else {
+                "value": rigid_line["cpu"]["value"], "unit": "body-steps/s", "cores": rigid_line["cpu"]["threads"],
+                "kind": "port", "sample": f"oracle FP32 rigid solver on the first {rigid_line['cpu']['worlds']} worlds, "
+                f"{args.warmup} untimed + {args.rigid_steps} timed steps each, over {rigid_line['cpu']['threads']} threads"}},
         "served_batch_worlds": None if served_c4 is None else {
             "workload": f"serving the {res['worlds']} batched worlds above: Poisson step requests at rising offered loads, "
                         f"{args.serve_c4_seconds:.0f}s each, 10% with transactions; over_plain against config 1's plain step",

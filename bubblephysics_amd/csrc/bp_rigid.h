@@ -58,6 +58,8 @@ struct RigidArena {
     RigidCand *rows;              // indexed by rowOffset + body * stride + slot
     RigidCorr *corr;
     uint32_t *slots;              // used row slots of the step, compacted
+    uint32_t *twin;               // per used slot: the slot of the same pair in the other body's row
+    float *toi;                   // per used slot: the pair's time of impact
     uint32_t *rowCount;           // indexed by bodyOffset
     uint32_t *rowStart;
     RigidWall *walls;
@@ -65,7 +67,15 @@ struct RigidArena {
 
 }  // namespace bpr
 
-extern "C" hipError_t bp_launch_rigid_step(const bpr::RigidArena *arena, bpr::RigidDesc *descs, bp_rigid_frame_info *infos,
-                                           uint32_t worlds, uint32_t steps, hipStream_t stream);
+namespace bpr {
+// Worlds of at most this many bodies run one wavefront per world with their bodies in LDS;
+// larger worlds run 256 lanes per world with their bodies in global memory.
+constexpr uint32_t kRigidLdsBodies = 96;
+}
+
+// Launches the worlds listed in order[count]: small = 1 for worlds of at most kRigidLdsBodies bodies.
+extern "C" hipError_t bp_launch_rigid_step(const bpr::RigidArena *arena, bpr::RigidDesc *descs, const uint32_t *order,
+                                           uint32_t count, int small, bp_rigid_frame_info *infos, uint32_t worlds,
+                                           uint32_t steps, hipStream_t stream);
 
 #endif  // BP_RIGID_H

@@ -48,6 +48,17 @@ BPD float2 clamp_speed(float2 v, float limit) {
 }
 BPD float clamp_spin(float w, float limit) { return limit > 0.0f ? clampf(w, -limit, limit) : w; }
 
+// A body's pose with the cosine and sine of its angle evaluated once: every vertex transform
+// of one evaluation reuses them (the same values rot() would compute each time).
+struct Xf {
+    float2 p;
+    float c, s;
+    float a;      // the angle itself (inverse rotations take cos and sin of -a)
+};
+BPD Xf xf_at(float x, float y, float angle) { return Xf{f2(x, y), bpm::cosf_det(angle), bpm::sinf_det(angle), angle}; }
+BPD Xf xf_of(const bp_rigid_body &b) { return xf_at(b.pose[0], b.pose[1], b.pose[2]); }
+BPD float2 apply(const Xf &x, float2 v) { return add(x.p, f2(v.x * x.c - v.y * x.s, v.x * x.s + v.y * x.c)); }
+
 // The world's geometry and joint tables, offsets already applied.
 struct View {
     const float2 *verts;
@@ -56,9 +67,7 @@ struct View {
     uint32_t segCount;
     const uint4 *jranges;   // null: no joint tables
     const uint64_t *partners;
-    BPD float2 vertex(const bp_rigid_body &b, uint32_t i) const {
-        return add(xy(b.pose), rot(verts[b.ranges[0] + i], b.pose[2]));
-    }
+    BPD float2 vertex(const bp_rigid_body &b, const Xf &x, uint32_t i) const { return apply(x, verts[b.ranges[0] + i]); }
     BPD bool excluded(uint32_t object, uint64_t partner) const {
         if (!jranges) return false;
         uint4 r = jranges[object];
@@ -100,20 +109,23 @@ struct Core {
     float2 a, b;
     float radius;
 };
-BPD Core core(const View &w, const bp_rigid_body &b) {
+BPD Core core(const View &w, const bp_rigid_body &b, const Xf &x) {
     Core c;
-    c.a = w.vertex(b, 0);
-    c.b = b.ranges[1] > 1u ? w.vertex(b, 1) : c.a;
+    c.a = w.vertex(b, x, 0);
+    c.b = b.ranges[1] > 1u ? w.vertex(b, x, 1) : c.a;
     c.radius = b.shape[1];
     return c;
 }
 
-BPD bool inside(const View &w, float2 p, const bp_rigid_body &b) {
+// Inside test with the body's inverse rotation given (cos and sin of -angle), for callers that
+// test many points against one body.
+BPD bool inside_with(const View &w, float2 p, const bp_rigid_body &b, const Xf &x, float ic, float is) {
     if (is_round(b)) {
-        Core c = core(w, b);
+        Core c = core(w, b, x);
         return dist2(p, closest_point(p, c.a, c.b)) <= c.radius;
     }
-    float2 local = rot(sub(p, xy(b.pose)), -b.pose[2]);
+    const float2 d = sub(p, x.p);
+    const float2 local = f2(d.x * ic - d.y * is, d.x * is + d.y * ic);
     for (uint32_t i = 0; i < b.ranges[3]; ++i) {
         uint4 t = w.tris[b.ranges[2] + i];
         float2 a = w.verts[b.ranges[0] + t.x], bb = w.verts[b.ranges[0] + t.y], c = w.verts[b.ranges[0] + t.z];
@@ -122,6 +134,10 @@ BPD bool inside(const View &w, float2 p, const bp_rigid_body &b) {
             return true;
     }
     return false;
+}
+BPD bool inside(const View &w, float2 p, const bp_rigid_body &b, const Xf &x) {
+    if (is_round(b)) return inside_with(w, p, b, x, 0.0f, 0.0f);
+    return inside_with(w, p, b, x, bpm::cosf_det(-x.a), bpm::sinf_det(-x.a));
 }
 
 BPD void round_round(const Core &A, const Core &B, float2 fallback, float2 &normal, float &depth, float2 &point) {
@@ -244,19 +260,20 @@ BPD bool sat_triangles(const float2 *a, const float2 *b, float2 &normal, float &
 }
 
 // Normal from the round body R to the polygon P.
-BPD void round_polygon(const View &w, const Core &R, const bp_rigid_body &P, float2 &normal, float &depth, float2 &point) {
+BPD void round_polygon(const View &w, const Core &R, const bp_rigid_body &P, const Xf &xp, float2 &normal, float &depth,
+                       float2 &point) {
     const uint32_t n = P.ranges[1];
     float insideDepth = -1.0f;
     float2 insideOut = f2(0.0f, 0.0f), insidePoint = f2(0.0f, 0.0f);
     uint32_t ends = (R.b.x != R.a.x || R.b.y != R.a.y) ? 2u : 1u, endsInside = 0;
     for (uint32_t k = 0; k < ends; ++k) {
         float2 end = k == 0 ? R.a : R.b;
-        if (!inside(w, end, P)) continue;
+        if (!inside(w, end, P, xp)) continue;
         ++endsInside;
         float nearest = inff();
         float2 nearestPoint = f2(0.0f, 0.0f), nearestOut = f2(0.0f, 0.0f);
         for (uint32_t e = 0; e < n; ++e) {
-            float2 p = w.vertex(P, e), q = w.vertex(P, (e + 1) % n);
+            float2 p = w.vertex(P, xp, e), q = w.vertex(P, xp, (e + 1) % n);
             float2 c = closest_point(end, p, q);
             float d = dist2(end, c);
             if (d < nearest) {
@@ -282,7 +299,7 @@ BPD void round_polygon(const View &w, const Core &R, const bp_rigid_body &P, flo
     float best = inff();
     float2 pc = f2(0.0f, 0.0f), pe = f2(0.0f, 0.0f);
     for (uint32_t e = 0; e < n; ++e) {
-        float2 p = w.vertex(P, e), q = w.vertex(P, (e + 1) % n);
+        float2 p = w.vertex(P, xp, e), q = w.vertex(P, xp, (e + 1) % n);
         float2 ca, ce;
         float d = seg_distance(R.a, R.b, p, q, ca, ce);
         if (d < best) {
@@ -300,7 +317,7 @@ BPD void round_polygon(const View &w, const Core &R, const bp_rigid_body &P, flo
     float minimum = inff();
     float2 minOut = f2(0.0f, 1.0f), minPoint = pe;
     for (uint32_t e = 0; e < n; ++e) {
-        float2 p = w.vertex(P, e), q = w.vertex(P, (e + 1) % n);
+        float2 p = w.vertex(P, xp, e), q = w.vertex(P, xp, (e + 1) % n);
         float2 edge = sub(q, p);
         float size = len2(edge);
         if (size <= kFloatMin) continue;
@@ -318,10 +335,10 @@ BPD void round_polygon(const View &w, const Core &R, const bp_rigid_body &P, flo
     point = minPoint;
 }
 
-BPD bool core_crosses(const View &w, const Core &R, const bp_rigid_body &P) {
-    if (inside(w, R.a, P) || inside(w, R.b, P)) return true;
+BPD bool core_crosses(const View &w, const Core &R, const bp_rigid_body &P, const Xf &xp) {
+    if (inside(w, R.a, P, xp) || inside(w, R.b, P, xp)) return true;
     for (uint32_t e = 0; e < P.ranges[1]; ++e) {
-        float2 p = w.vertex(P, e), q = w.vertex(P, (e + 1) % P.ranges[1]);
+        float2 p = w.vertex(P, xp, e), q = w.vertex(P, xp, (e + 1) % P.ranges[1]);
         float2 ca, ce;
         if (seg_distance(R.a, R.b, p, q, ca, ce) <= 1e-4f) return true;
     }
@@ -329,25 +346,38 @@ BPD bool core_crosses(const View &w, const Core &R, const bp_rigid_body &P) {
 }
 
 // Normal from a to b.
-BPD bool pair_contact(const View &w, const bp_rigid_body &a, const bp_rigid_body &b, float2 &n, float &depth, float2 &point) {
+constexpr uint32_t kLocalVerts = 8;    // polygon vertices transformed once per pair evaluation
+
+// Normal from a to b, with the bodies at the poses xa and xb (their own or interpolated ones).
+BPD bool pair_contact_at(const View &w, const bp_rigid_body &a, const Xf &xa, const bp_rigid_body &b, const Xf &xb,
+                         float2 &n, float &depth, float2 &point) {
     const bool ra = is_round(a), rb = is_round(b);
     if (ra && rb) {
-        round_round(core(w, a), core(w, b), f2(1.0f, 0.0f), n, depth, point);
+        round_round(core(w, a, xa), core(w, b, xb), f2(1.0f, 0.0f), n, depth, point);
         return true;
     }
     if (ra != rb) {
         float2 normal;
-        round_polygon(w, core(w, ra ? a : b), ra ? b : a, normal, depth, point);
+        round_polygon(w, ra ? core(w, a, xa) : core(w, b, xb), ra ? b : a, ra ? xb : xa, normal, depth, point);
         n = ra ? normal : neg(normal);
         return true;
     }
+    // Both polygons: their vertices in the world frame once, then the triangle pairs.
+    float2 wa[kLocalVerts], wb[kLocalVerts];
+    const bool localA = a.ranges[1] <= kLocalVerts, localB = b.ranges[1] <= kLocalVerts;
+    if (localA)
+        for (uint32_t k = 0; k < a.ranges[1]; ++k) wa[k] = w.vertex(a, xa, k);
+    if (localB)
+        for (uint32_t k = 0; k < b.ranges[1]; ++k) wb[k] = w.vertex(b, xb, k);
     bool found = false;
     for (uint32_t i = 0; i < a.ranges[3]; ++i) {
         uint4 ta = w.tris[a.ranges[2] + i];
-        float2 av[3] = {w.vertex(a, ta.x), w.vertex(a, ta.y), w.vertex(a, ta.z)};
+        float2 av[3] = {localA ? wa[ta.x] : w.vertex(a, xa, ta.x), localA ? wa[ta.y] : w.vertex(a, xa, ta.y),
+                        localA ? wa[ta.z] : w.vertex(a, xa, ta.z)};
         for (uint32_t j = 0; j < b.ranges[3]; ++j) {
             uint4 tb = w.tris[b.ranges[2] + j];
-            float2 bv[3] = {w.vertex(b, tb.x), w.vertex(b, tb.y), w.vertex(b, tb.z)};
+            float2 bv[3] = {localB ? wb[tb.x] : w.vertex(b, xb, tb.x), localB ? wb[tb.y] : w.vertex(b, xb, tb.y),
+                            localB ? wb[tb.z] : w.vertex(b, xb, tb.z)};
             float2 normal, p;
             float d;
             if (sat_triangles(av, bv, normal, d, p) && (!found || d > depth)) {
@@ -361,6 +391,10 @@ BPD bool pair_contact(const View &w, const bp_rigid_body &a, const bp_rigid_body
     return found;
 }
 
+BPD bool pair_contact(const View &w, const bp_rigid_body &a, const bp_rigid_body &b, float2 &n, float &depth, float2 &point) {
+    return pair_contact_at(w, a, xf_of(a), b, xf_of(b), n, depth, point);
+}
+
 struct P3 {
     float x, y, a;
 };
@@ -369,6 +403,7 @@ BPD P3 lerp3(P3 s, P3 e, float t) { return P3{mixs(s.x, e.x, t), mixs(s.y, e.y, 
 // Swept vertices of body b from pose p to q against the world's static edges (PostSolve:29-62).
 BPD bool swept_hit(const View &w, const bp_rigid_body &b, P3 p, P3 q, uint32_t entity, float2 &point, float2 &normal,
                    int64_t &owner) {
+    const Xf xp = xf_at(p.x, p.y, p.a), xq = xf_at(q.x, q.y, q.a);
     for (uint32_t j = 0; j < w.segCount; ++j) {
         const bp_segment &s = w.segs[j];
         const bool owned = (s.flags & BP_SEGMENT_HAS_OWNER) != 0u;
@@ -378,14 +413,13 @@ BPD bool swept_hit(const View &w, const bp_rigid_body &b, P3 p, P3 q, uint32_t e
         if (!accepts(b.filter, sf)) continue;
         const float2 sa = f2(s.a[0], s.a[1]), sb = f2(s.b[0], s.b[1]);
         const Sweep still{sa, sb, sa, sb};
-        const float2 from = f2(p.x, p.y), to = f2(q.x, q.y);
         for (uint32_t k = 0; k < b.ranges[1]; ++k) {
             float2 v = w.verts[b.ranges[0] + k];
-            float2 v0 = add(from, rot(v, p.a)), v1 = add(to, rot(v, q.a));
+            float2 v0 = apply(xp, v), v1 = apply(xq, v);
             bool hit = crosses(v0, v1, still);
             if (!hit) {
                 float2 next = w.verts[b.ranges[0] + (k + 1) % b.ranges[1]];
-                Sweep edge{add(from, rot(v, p.a)), add(from, rot(next, p.a)), add(to, rot(v, q.a)), add(to, rot(next, q.a))};
+                Sweep edge{v0, apply(xp, next), v1, apply(xq, next)};
                 hit = crosses(sa, sa, edge) || crosses(sb, sb, edge);
             }
             if (hit) {
@@ -401,16 +435,12 @@ BPD bool swept_hit(const View &w, const bp_rigid_body &b, P3 p, P3 q, uint32_t e
     return false;
 }
 
-BPD bool overlap_at(const View &w, bp_rigid_body a, bp_rigid_body b, float t) {
-    a.pose[0] = mixs(a.previous[0], a.pose[0], t);
-    a.pose[1] = mixs(a.previous[1], a.pose[1], t);
-    a.pose[2] = mixs(a.pose[3], a.pose[2], t);
-    b.pose[0] = mixs(b.previous[0], b.pose[0], t);
-    b.pose[1] = mixs(b.previous[1], b.pose[1], t);
-    b.pose[2] = mixs(b.pose[3], b.pose[2], t);
+BPD bool overlap_at(const View &w, const bp_rigid_body &a, const bp_rigid_body &b, float t) {
+    const Xf xa = xf_at(mixs(a.previous[0], a.pose[0], t), mixs(a.previous[1], a.pose[1], t), mixs(a.pose[3], a.pose[2], t));
+    const Xf xb = xf_at(mixs(b.previous[0], b.pose[0], t), mixs(b.previous[1], b.pose[1], t), mixs(b.pose[3], b.pose[2], t));
     float2 n, point;
     float depth;
-    return pair_contact(w, a, b, n, depth, point) && depth > 1e-3f;
+    return pair_contact_at(w, a, xa, b, xb, n, depth, point) && depth > 1e-3f;
 }
 
 BPD float impact_time(const View &w, const bp_rigid_body &a, const bp_rigid_body &b) {
@@ -438,10 +468,74 @@ BPD float impact_time(const View &w, const bp_rigid_body &a, const bp_rigid_body
     return 2.0f;
 }
 
+constexpr uint32_t kNoTwin = 0xffffffffu;
+
 BPD bool key_less(const RigidCand &a, const RigidCand &b) {
     if (a.kind != b.kind) return a.kind < b.kind;
     if (a.first != b.first) return a.first < b.first;
     return a.second < b.second;
+}
+
+// One body-pair row entry of a contact iteration (Contact:382-433): the correction `r` (its w lanes
+// carried over from `prev` when nothing is found) and the normal correction lambda it adds to the
+// entry's impulse sum (0 = none).
+BPD void evaluate_entry(const View &w, const bp_rigid_body *B, const RigidCand &cand, const RigidCorr &prev, RigidCorr &r,
+                        bool &added, float &lambdaOut) {
+    r = RigidCorr{};
+    r.first[3] = prev.first[3];
+    r.second[3] = prev.second[3];
+    r.vfirst[3] = prev.vfirst[3];
+    r.vsecond[3] = prev.vsecond[3];
+    added = false;
+    lambdaOut = 0.0f;
+    const uint32_t ai = cand.low, bi = cand.high;
+    if (cand.kind != 3u || ai >= bi) return;
+    const bp_rigid_body &a = B[ai], &b = B[bi];
+    if (immovable(a) && immovable(b)) return;
+    float2 gap = sub(xy(a.pose), xy(b.pose));
+    float radius = a.physical[2] + b.physical[2];
+    if (dot2(gap, gap) > radius * radius) return;
+    float2 nrm, point;
+    float depth;
+    if (!pair_contact(w, a, b, nrm, depth, point)) return;
+    const float wa = a.physical[0], wb = b.physical[0], ia = a.physical[1], ib = b.physical[1];
+    const float2 ra = sub(point, xy(a.pose)), rb = sub(point, xy(b.pose));
+    const float2 va = f2(a.motion[0], a.motion[1]), vb = f2(b.motion[0], b.motion[1]);
+    const float spinA = a.motion[2], spinB = b.motion[2];
+    const float ca = cross2(ra, nrm), cb = cross2(rb, nrm);
+    const float denom = wa + wb + ia * ca * ca + ib * cb * cb;
+    if (depth < -0.001f || denom <= kFloatMin) return;
+    const float lambda = fmaxf(0.0f, depth) / denom * 0.5f;
+    added = true;
+    lambdaOut = lambda;
+    float2 m = mul(mul(neg(nrm), wa), lambda);
+    r.first[0] = m.x;
+    r.first[1] = m.y;
+    r.first[2] = -ca * ia * lambda;
+    r.first[3] = nrm.x;
+    m = mul(mul(nrm, wb), lambda);
+    r.second[0] = m.x;
+    r.second[1] = m.y;
+    r.second[2] = cb * ib * lambda;
+    r.second[3] = nrm.y;
+    const float2 dv = sub(vb, va);
+    const float closing = dot2(dv, nrm) + spinB * cb - spinA * ca;
+    const float friction = __builtin_sqrtf(fmaxf(0.0f, a.physical[3] * b.physical[3]));
+    const float restitution = fmaxf(__uint_as_float(a.flags[3]), __uint_as_float(b.flags[3]));
+    const float impulse = fmaxf(0.0f, -closing) * (1.0f + restitution) / denom * 0.5f;
+    const float2 tangent = f2(-nrm.y, nrm.x);
+    const float tangentSpeed = dot2(dv, tangent) + spinB * cross2(rb, tangent) - spinA * cross2(ra, tangent);
+    const float tangentImpulse = -tangentSpeed * friction / denom * 0.5f;
+    m = sub(mul(mul(neg(nrm), wa), impulse), mul(mul(tangent, wa), tangentImpulse));
+    r.vfirst[0] = m.x;
+    r.vfirst[1] = m.y;
+    r.vfirst[2] = -ca * ia * impulse - cross2(ra, tangent) * ia * tangentImpulse;
+    r.vfirst[3] = point.x;
+    m = add(mul(mul(nrm, wb), impulse), mul(mul(tangent, wb), tangentImpulse));
+    r.vsecond[0] = m.x;
+    r.vsecond[1] = m.y;
+    r.vsecond[2] = cb * ib * impulse + cross2(rb, tangent) * ib * tangentImpulse;
+    r.vsecond[3] = point.y;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -586,12 +680,16 @@ __device__ uint32_t block_prefix(uint32_t *scan, const uint32_t *counts, uint32_
 // ---------------------------------------------------------------------------------------
 // The step kernel.
 // ---------------------------------------------------------------------------------------
-template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void rigid_step_kernel(RigidArena A, RigidDesc *descs, bp_rigid_frame_info *infos,
-                                                           uint32_t worldCount, uint32_t steps) {
+// BLOCK lanes per world; CAP > 0 keeps the world's bodies (at most CAP) in LDS for the step and
+// writes them to the unpublished side when the step succeeds, CAP = 0 steps them in place on the
+// unpublished side in global memory. `order` lists the worlds of the launch.
+template <int BLOCK, int CAP>
+__global__ __launch_bounds__(BLOCK) void rigid_step_kernel(RigidArena A, RigidDesc *descs, const uint32_t *order,
+                                                           bp_rigid_frame_info *infos, uint32_t worldCount, uint32_t steps) {
     __shared__ uint32_t sScan[BLOCK];
+    __shared__ bp_rigid_body sBodies[CAP > 0 ? CAP : 1];
     __shared__ uint32_t sSafety, sOverflow, sMaxPer, sPairs, sRemoved, sWrapped, sTotal;
-    const uint32_t wi = blockIdx.x;
+    const uint32_t wi = order[blockIdx.x];
     const uint32_t tid = threadIdx.x;
     RigidDesc &D = descs[wi];
     const uint32_t n = D.bodyCount;
@@ -606,6 +704,8 @@ __global__ __launch_bounds__(BLOCK) void rigid_step_kernel(RigidArena A, RigidDe
     RigidCand *rows = A.rows + D.rowOffset;
     RigidCorr *corr = A.corr + D.rowOffset;
     uint32_t *slots = A.slots + D.rowOffset;
+    uint32_t *twin = A.twin + D.rowOffset;
+    float *toi = A.toi + D.rowOffset;
     uint32_t *rowCount = A.rowCount + D.bodyOffset;
     uint32_t *rowStart = A.rowStart + D.bodyOffset;
     RigidWall *walls = A.walls + D.bodyOffset;
@@ -625,7 +725,10 @@ __global__ __launch_bounds__(BLOCK) void rigid_step_kernel(RigidArena A, RigidDe
             continue;
         }
         bp_rigid_body *pub = A.bodies[D.side] + D.bodyOffset;
-        bp_rigid_body *B = A.bodies[D.side ^ 1u] + D.bodyOffset;
+        bp_rigid_body *out = A.bodies[D.side ^ 1u] + D.bodyOffset;
+        bp_rigid_body *B;
+        if constexpr (CAP > 0) B = sBodies;
+        else B = out;
         if (tid == 0) {
             sSafety = 0;
             sOverflow = 0;
@@ -792,18 +895,43 @@ __global__ __launch_bounds__(BLOCK) void rigid_step_kernel(RigidArena A, RigidDe
             const uint32_t s0 = rowStart[i], c = rowCount[i];
             for (uint32_t s = 0; s < c; ++s) slots[s0 + s] = i * stride + s;
         }
+        __syncthreads();
 
-        // Time of impact of each movable body against its row (PostSolve:243-257), then the rollback
+        // Each used entry's twin: the same pair in the other body's row (rows are sorted by key).
+        const uint32_t total = sTotal;
+        for (uint32_t e = tid; e < total; e += BLOCK) {
+            const uint32_t index = slots[e];
+            const uint32_t owner = index / stride;
+            const RigidCand c = rows[index];
+            const uint32_t other = owner == c.low ? c.high : c.low;
+            const RigidCand *orow = rows + (size_t)other * stride;
+            uint32_t lo = 0, hi = rowCount[other];
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (key_less(orow[mid], c)) lo = mid + 1;
+                else hi = mid;
+            }
+            const bool found = lo < rowCount[other] && orow[lo].kind == c.kind && orow[lo].first == c.first &&
+                               orow[lo].second == c.second;
+            twin[e] = found ? rowStart[other] + lo : kNoTwin;
+        }
+        __syncthreads();
+        // Time of impact of each pair (PostSolve:220-238), once per pair, one lane per entry; then each
+        // movable body takes the earliest over its row (PostSolve:243-257) and rolls back to it
         // (PostSolve:260-271).
+        for (uint32_t e = tid; e < total; e += BLOCK) {
+            const uint32_t index = slots[e], pair = twin[e];
+            const RigidCand c = rows[index];
+            if (pair != kNoTwin && index / stride != c.low) continue;
+            const float t = impact_time(w, B[c.low], B[c.high]);
+            toi[e] = t;
+            if (pair != kNoTwin) toi[pair] = t;
+        }
+        __syncthreads();
         for (uint32_t i = tid; i < n; i += BLOCK) {
-            const bp_rigid_body &body = B[i];
             float earliest = 2.0f;
-            if (!immovable(body))
-                for (uint32_t s = 0; s < rowCount[i]; ++s) {
-                    const RigidCand &c = rows[(size_t)i * stride + s];
-                    if (c.kind != 3u) continue;
-                    earliest = fminf(earliest, impact_time(w, B[c.low], B[c.high]));
-                }
+            if (!immovable(B[i]))
+                for (uint32_t s = 0; s < rowCount[i]; ++s) earliest = fminf(earliest, toi[rowStart[i] + s]);
             B[i].limits[2] = earliest <= 1.0f ? 1.0f + earliest : 0.0f;
         }
         __syncthreads();
@@ -819,66 +947,26 @@ __global__ __launch_bounds__(BLOCK) void rigid_step_kernel(RigidArena A, RigidDe
         }
         __syncthreads();
 
-        const uint32_t total = sTotal;
         for (uint32_t it = 0; it < D.iterations; ++it) {
-            // Pair contacts, one lane per used row slot (Contact:382-433).
+            // Pair contacts (Contact:382-433). A pair sits in the rows of both its bodies, where the
+            // reference evaluates it twice on the same inputs; here the lower body's entry is evaluated
+            // once and written to both entries (an entry without a twin, when a row cut or one-sided
+            // joint tables leave the pair in one row only, is evaluated on its own).
             for (uint32_t e = tid; e < total; e += BLOCK) {
-                const uint32_t index = slots[e];
-                RigidCand &cand = rows[index];
-                const RigidCorr prev = corr[index];
-                RigidCorr r = {};
-                r.first[3] = prev.first[3];
-                r.second[3] = prev.second[3];
-                r.vfirst[3] = prev.vfirst[3];
-                r.vsecond[3] = prev.vsecond[3];
-                const uint32_t ai = cand.low, bi = cand.high;
-                if (cand.kind != 3u || ai >= bi) { corr[index] = r; continue; }
-                const bp_rigid_body a = B[ai], b = B[bi];
-                if (immovable(a) && immovable(b)) { corr[index] = r; continue; }
-                float2 gap = sub(xy(a.pose), xy(b.pose));
-                float radius = a.physical[2] + b.physical[2];
-                if (dot2(gap, gap) > radius * radius) { corr[index] = r; continue; }
-                float2 nrm, point;
-                float depth;
-                if (!pair_contact(w, a, b, nrm, depth, point)) { corr[index] = r; continue; }
-                const float wa = a.physical[0], wb = b.physical[0], ia = a.physical[1], ib = b.physical[1];
-                const float2 ra = sub(point, xy(a.pose)), rb = sub(point, xy(b.pose));
-                const float2 va = f2(a.motion[0], a.motion[1]), vb = f2(b.motion[0], b.motion[1]);
-                const float spinA = a.motion[2], spinB = b.motion[2];
-                const float ca = cross2(ra, nrm), cb = cross2(rb, nrm);
-                const float denom = wa + wb + ia * ca * ca + ib * cb * cb;
-                if (depth < -0.001f || denom <= kFloatMin) { corr[index] = r; continue; }
-                const float lambda = fmaxf(0.0f, depth) / denom * 0.5f;
-                cand.impulse = cand.impulse + lambda;
-                float2 m = mul(mul(neg(nrm), wa), lambda);
-                r.first[0] = m.x;
-                r.first[1] = m.y;
-                r.first[2] = -ca * ia * lambda;
-                r.first[3] = nrm.x;
-                m = mul(mul(nrm, wb), lambda);
-                r.second[0] = m.x;
-                r.second[1] = m.y;
-                r.second[2] = cb * ib * lambda;
-                r.second[3] = nrm.y;
-                const float2 dv = sub(vb, va);
-                const float closing = dot2(dv, nrm) + spinB * cb - spinA * ca;
-                const float friction = __builtin_sqrtf(fmaxf(0.0f, a.physical[3] * b.physical[3]));
-                const float restitution = fmaxf(__uint_as_float(a.flags[3]), __uint_as_float(b.flags[3]));
-                const float impulse = fmaxf(0.0f, -closing) * (1.0f + restitution) / denom * 0.5f;
-                const float2 tangent = f2(-nrm.y, nrm.x);
-                const float tangentSpeed = dot2(dv, tangent) + spinB * cross2(rb, tangent) - spinA * cross2(ra, tangent);
-                const float tangentImpulse = -tangentSpeed * friction / denom * 0.5f;
-                m = sub(mul(mul(neg(nrm), wa), impulse), mul(mul(tangent, wa), tangentImpulse));
-                r.vfirst[0] = m.x;
-                r.vfirst[1] = m.y;
-                r.vfirst[2] = -ca * ia * impulse - cross2(ra, tangent) * ia * tangentImpulse;
-                r.vfirst[3] = point.x;
-                m = add(mul(mul(nrm, wb), impulse), mul(mul(tangent, wb), tangentImpulse));
-                r.vsecond[0] = m.x;
-                r.vsecond[1] = m.y;
-                r.vsecond[2] = cb * ib * impulse + cross2(rb, tangent) * ib * tangentImpulse;
-                r.vsecond[3] = point.y;
+                const uint32_t index = slots[e], pair = twin[e];
+                const RigidCand cand = rows[index];
+                if (pair != kNoTwin && index / stride != cand.low) continue;
+                RigidCorr r;
+                bool added;
+                float lambda;
+                evaluate_entry(w, B, cand, corr[index], r, added, lambda);
                 corr[index] = r;
+                if (added) rows[index].impulse = cand.impulse + lambda;
+                if (pair != kNoTwin) {
+                    const uint32_t other = slots[pair];
+                    corr[other] = r;
+                    if (added) rows[other].impulse = rows[other].impulse + lambda;
+                }
             }
             __syncthreads();
             // Each body sums its row in slot order (Contact:435-449).
@@ -936,19 +1024,24 @@ __global__ __launch_bounds__(BLOCK) void rigid_step_kernel(RigidArena A, RigidDe
             for (uint32_t i = tid; i < n; i += BLOCK) {
                 if (B[i].flags[0] || B[i].filter[2]) continue;
                 bp_rigid_body b = B[i];
+                // The guard moves a body, never turns it: the angle's rotation terms hold for the pass.
+                const Xf now = xf_of(b);
+                const Xf then = xf_at(b.previous[0], b.previous[1], b.pose[3]);
                 for (uint32_t j = 0; j < n; ++j) {
                     const bp_rigid_body &wall = B[j];
                     if (!wall.flags[0] || is_round(wall) || !accepts(b.filter, wall.filter)) continue;
                     if (w.excluded(i, wall.id)) continue;
+                    const Xf xw = xf_of(wall);
+                    const float wic = bpm::cosf_det(-wall.pose[2]), wis = bpm::sinf_det(-wall.pose[2]);
                     const uint32_t edges = wall.ranges[1];
                     for (uint32_t k = 0; k < b.ranges[1]; ++k) {
-                        const float2 p = w.vertex(b, k);
-                        const float2 start = add(xy(b.previous), rot(w.verts[b.ranges[0] + k], b.pose[3]));
-                        const bool in = inside(w, p, wall);
-                        bool startedIn = inside(w, start, wall);
+                        const float2 p = apply(Xf{xy(b.pose), now.c, now.s, now.a}, w.verts[b.ranges[0] + k]);
+                        const float2 start = apply(then, w.verts[b.ranges[0] + k]);
+                        const bool in = inside_with(w, p, wall, xw, wic, wis);
+                        bool startedIn = inside_with(w, start, wall, xw, wic, wis);
                         if (startedIn)
                             for (uint32_t e = 0; e < edges; ++e) {
-                                float2 ea = w.vertex(wall, e), eb = w.vertex(wall, (e + 1) % edges);
+                                float2 ea = w.vertex(wall, xw, e), eb = w.vertex(wall, xw, (e + 1) % edges);
                                 if (dist2(start, closest_point(start, ea, eb)) < 1e-5f) {
                                     startedIn = false;
                                     break;
@@ -957,7 +1050,7 @@ __global__ __launch_bounds__(BLOCK) void rigid_step_kernel(RigidArena A, RigidDe
                         float nearest = inff(), depth = 0.0f;
                         float2 normal = f2(0.0f, 0.0f);
                         for (uint32_t e = 0; e < edges; ++e) {
-                            const float2 ea = w.vertex(wall, e), eb = w.vertex(wall, (e + 1) % edges);
+                            const float2 ea = w.vertex(wall, xw, e), eb = w.vertex(wall, xw, (e + 1) % edges);
                             const Sweep edge{ea, eb, ea, eb};
                             const float2 dir = sub(eb, ea), out = unit(f2(dir.y, -dir.x));
                             const float signedStart = dot2(sub(start, ea), out);
@@ -975,13 +1068,7 @@ __global__ __launch_bounds__(BLOCK) void rigid_step_kernel(RigidArena A, RigidDe
                             }
                         }
                         if (isfinite(nearest) && depth > 0.0f) {
-                            if (is_round(b)) {
-                                bp_rigid_body old = b;
-                                old.pose[0] = b.previous[0];
-                                old.pose[1] = b.previous[1];
-                                old.pose[2] = b.pose[3];
-                                if (core_crosses(w, core(w, old), wall)) break;
-                            }
+                            if (is_round(b) && core_crosses(w, core(w, b, then), wall, xw)) break;
                             const float2 shift = mul(normal, depth + 1e-5f);
                             b.pose[0] += shift.x;
                             b.pose[1] += shift.y;
@@ -1015,8 +1102,9 @@ __global__ __launch_bounds__(BLOCK) void rigid_step_kernel(RigidArena A, RigidDe
             b.motion[1] = v.y;
             b.motion[2] = clamp_spin(b.motion[2], b.limits[1]);
             float2 lo = f2(inff(), inff()), hi = f2(-inff(), -inff());
+            const Xf xb = xf_of(b);
             for (uint32_t k = 0; k < b.ranges[1]; ++k) {
-                float2 p = w.vertex(b, k);
+                float2 p = w.vertex(b, xb, k);
                 lo = min2(lo, p);
                 hi = max2(hi, p);
             }
@@ -1046,6 +1134,11 @@ __global__ __launch_bounds__(BLOCK) void rigid_step_kernel(RigidArena A, RigidDe
         }
         __syncthreads();
         const uint32_t safety = sSafety;
+        if constexpr (CAP > 0) {
+            if (!(safety & 3u))
+                for (uint32_t i = tid; i < n; i += BLOCK) out[i] = B[i];
+            __syncthreads();
+        }
         if (tid == 0) {
             bp_rigid_frame_info r = {};
             r.status = (safety & 1u) ? BP_ERR_NON_FINITE : ((safety & 2u) ? BP_ERR_GUARD_FAILURE : BP_OK);
@@ -1070,9 +1163,17 @@ __global__ __launch_bounds__(BLOCK) void rigid_step_kernel(RigidArena A, RigidDe
 
 }  // namespace bpr
 
-extern "C" hipError_t bp_launch_rigid_step(const bpr::RigidArena *arena, bpr::RigidDesc *descs, bp_rigid_frame_info *infos,
-                                           uint32_t worlds, uint32_t steps, hipStream_t stream) {
-    if (worlds == 0) return hipSuccess;
-    hipLaunchKernelGGL((bpr::rigid_step_kernel<256>), dim3(worlds), dim3(256), 0, stream, *arena, descs, infos, worlds, steps);
+extern "C" uint32_t bp_rigid_lds_capacity() { return bpr::kRigidLdsBodies; }
+
+extern "C" hipError_t bp_launch_rigid_step(const bpr::RigidArena *arena, bpr::RigidDesc *descs, const uint32_t *order,
+                                           uint32_t count, int small, bp_rigid_frame_info *infos, uint32_t worlds,
+                                           uint32_t steps, hipStream_t stream) {
+    if (count == 0) return hipSuccess;
+    if (small)
+        hipLaunchKernelGGL((bpr::rigid_step_kernel<64, bpr::kRigidLdsBodies>), dim3(count), dim3(64), 0, stream, *arena, descs,
+                           order, infos, worlds, steps);
+    else
+        hipLaunchKernelGGL((bpr::rigid_step_kernel<256, 0>), dim3(count), dim3(256), 0, stream, *arena, descs, order, infos,
+                           worlds, steps);
     return hipGetLastError();
 }

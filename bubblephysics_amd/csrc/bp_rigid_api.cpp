@@ -154,7 +154,7 @@ struct bp_rigid_batch {
     bp_rigid_frame_info *infos = nullptr;
     size_t capBodies[2] = {0, 0}, capVerts = 0, capTris = 0, capSegs = 0, capJoints = 0, capJointWork = 0, capJcorr = 0,
            capRefs = 0, capRanges = 0, capPartners = 0, capRows = 0, capCorr = 0, capSlots = 0, capRowCount = 0,
-           capRowStart = 0, capWalls = 0, capDescs = 0, capInfos = 0;
+           capRowStart = 0, capWalls = 0, capDescs = 0, capInfos = 0, capTwin = 0, capToi = 0;
     bp_rigid_body *bodies[2] = {nullptr, nullptr};
     float2 *verts = nullptr;
     uint4 *tris = nullptr;
@@ -165,13 +165,17 @@ struct bp_rigid_batch {
     uint4 *jranges = nullptr;
     uint64_t *partners = nullptr;
     bpr::RigidCand *rows = nullptr;
-    uint32_t *slots = nullptr, *rowCount = nullptr, *rowStart = nullptr;
+    uint32_t *slots = nullptr, *rowCount = nullptr, *rowStart = nullptr, *twin = nullptr;
+    float *toi = nullptr;
+    uint32_t *orders = nullptr;
+    size_t capOrders = 0;
+    uint32_t smallCount = 0, largeCount = 0;
     bpr::RigidWall *walls = nullptr;
 
     ~bp_rigid_batch() {
         if (stream) (void)hipStreamSynchronize(stream);
         void *ptrs[] = {bodies[0], bodies[1], verts, tris, segs, jointInput, joints, jcorr, corr, refs, jranges,
-                        partners, rows, slots, rowCount, rowStart, walls, descs, infos};
+                        partners, rows, slots, rowCount, rowStart, walls, descs, infos, twin, toi, orders};
         for (void *p : ptrs)
             if (p) (void)hipFree(p);
         if (start) (void)hipEventDestroy(start);
@@ -187,21 +191,26 @@ struct bp_rigid_batch {
         return BP_OK;
     }
 
-    // Reads every world's published bodies and flags back into the host copies.
+    // Reads every world's published bodies and flags back into the host copies (both body sides
+    // in two copies; each world takes its published side).
     bp_status sync_from_device() {
         if (!packed) return BP_OK;
         BP_HIP(hipMemcpyAsync(hostDescs.data(), descs, hostDescs.size() * sizeof(bpr::RigidDesc), hipMemcpyDeviceToHost, stream));
+        size_t total = 0;
+        for (const HostRigidWorld &w : worlds) total += w.bodies.size();
+        std::vector<bp_rigid_body> side[2];
+        for (int k = 0; k < 2; ++k) {
+            side[k].resize(total);
+            if (total) BP_HIP(hipMemcpyAsync(side[k].data(), bodies[k], total * sizeof(bp_rigid_body), hipMemcpyDeviceToHost, stream));
+        }
         BP_HIP(hipStreamSynchronize(stream));
         for (size_t wi = 0; wi < worlds.size(); ++wi) {
             HostRigidWorld &w = worlds[wi];
             const bpr::RigidDesc &d = hostDescs[wi];
-            if (!w.bodies.empty())
-                BP_HIP(hipMemcpyAsync(w.bodies.data(), bodies[d.side] + d.bodyOffset, w.bodies.size() * sizeof(bp_rigid_body),
-                                      hipMemcpyDeviceToHost, stream));
+            std::copy(side[d.side].begin() + d.bodyOffset, side[d.side].begin() + d.bodyOffset + w.bodies.size(), w.bodies.begin());
             w.failed = d.failed != 0;
             w.lastOk = d.lastOk != 0;
         }
-        BP_HIP(hipStreamSynchronize(stream));
         return BP_OK;
     }
 
@@ -262,32 +271,69 @@ struct bp_rigid_batch {
         G(grow(rows, capRows, nrows));
         G(grow(corr, capCorr, nrows));
         G(grow(slots, capSlots, nrows));
+        G(grow(twin, capTwin, nrows));
+        G(grow(toi, capToi, nrows));
         G(grow(rowCount, capRowCount, nb));
         G(grow(rowStart, capRowStart, nb));
         G(grow(walls, capWalls, nb));
         G(grow(descs, capDescs, worlds.size()));
+        G(grow(orders, capOrders, worlds.size()));
         if (g != BP_OK) return g;
+        // Every array staged contiguously on the host and uploaded with one copy (a copy per world and
+        // array would be tens of thousands of small transfers for a large batch).
+        std::vector<bp_rigid_body> hb;
+        std::vector<float> hv;
+        std::vector<uint32_t> ht, hr, hj;
+        std::vector<bp_segment> hs;
+        std::vector<bp_joint> hjo;
+        std::vector<uint64_t> hp;
+        hb.reserve(nb);
+        hv.reserve(2 * nv);
+        ht.reserve(4 * nt);
+        hs.reserve(ns);
+        hjo.reserve(nj);
+        hr.reserve(2 * nr);
+        hj.assign(4 * nb, 0u);
+        hp.reserve(np);
         for (size_t wi = 0; wi < worlds.size(); ++wi) {
             const HostRigidWorld &w = worlds[wi];
-            const bpr::RigidDesc &d = hostDescs[wi];
-            auto up = [&](void *dst, const void *src, size_t bytes) -> hipError_t {
-                return bytes ? hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream) : hipSuccess;
-            };
-            BP_HIP(up(bodies[0] + d.bodyOffset, w.bodies.data(), w.bodies.size() * sizeof(bp_rigid_body)));
-            BP_HIP(up(verts + d.vertexOffset, w.vertices.data(), w.vertices.size() * sizeof(float)));
-            BP_HIP(up(tris + d.triangleOffset, w.triangles.data(), w.triangles.size() * sizeof(uint32_t)));
-            BP_HIP(up(segs + d.segmentOffset, w.segments.data(), w.segments.size() * sizeof(bp_segment)));
-            BP_HIP(up(jointInput + d.jointOffset, w.joints.data(), w.joints.size() * sizeof(bp_joint)));
-            BP_HIP(up(joints + d.jointOffset, w.joints.data(), w.joints.size() * sizeof(bp_joint)));
-            BP_HIP(up(refs + d.refOffset, w.references.data(), w.references.size() * sizeof(uint32_t)));
-            BP_HIP(up(jranges + d.bodyOffset, w.ranges.data(), w.ranges.size() * sizeof(uint32_t)));
-            BP_HIP(up(partners + d.partnerOffset, w.partners.data(), w.partners.size() * sizeof(uint64_t)));
+            hb.insert(hb.end(), w.bodies.begin(), w.bodies.end());
+            hv.insert(hv.end(), w.vertices.begin(), w.vertices.end());
+            ht.insert(ht.end(), w.triangles.begin(), w.triangles.end());
+            hs.insert(hs.end(), w.segments.begin(), w.segments.end());
+            hjo.insert(hjo.end(), w.joints.begin(), w.joints.end());
+            hr.insert(hr.end(), w.references.begin(), w.references.end());
+            if (!w.ranges.empty()) std::copy(w.ranges.begin(), w.ranges.end(), hj.begin() + 4 * (size_t)hostDescs[wi].bodyOffset);
+            hp.insert(hp.end(), w.partners.begin(), w.partners.end());
         }
+        auto up = [&](void *dst, const void *src, size_t bytes) -> hipError_t {
+            return bytes ? hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream) : hipSuccess;
+        };
+        BP_HIP(up(bodies[0], hb.data(), hb.size() * sizeof(bp_rigid_body)));
+        BP_HIP(up(verts, hv.data(), hv.size() * sizeof(float)));
+        BP_HIP(up(tris, ht.data(), ht.size() * sizeof(uint32_t)));
+        BP_HIP(up(segs, hs.data(), hs.size() * sizeof(bp_segment)));
+        BP_HIP(up(jointInput, hjo.data(), hjo.size() * sizeof(bp_joint)));
+        BP_HIP(up(joints, hjo.data(), hjo.size() * sizeof(bp_joint)));
+        BP_HIP(up(refs, hr.data(), hr.size() * sizeof(uint32_t)));
+        BP_HIP(up(jranges, hj.data(), hj.size() * sizeof(uint32_t)));
+        BP_HIP(up(partners, hp.data(), hp.size() * sizeof(uint64_t)));
         if (nb) BP_HIP(hipMemsetAsync(rowCount, 0, nb * sizeof(uint32_t), stream));
         if (nb) BP_HIP(hipMemsetAsync(walls, 0, nb * sizeof(bpr::RigidWall), stream));
         if (nrows) BP_HIP(hipMemsetAsync(corr, 0, nrows * sizeof(bpr::RigidCorr), stream));
         if (!hostDescs.empty())
             BP_HIP(hipMemcpyAsync(descs, hostDescs.data(), hostDescs.size() * sizeof(bpr::RigidDesc), hipMemcpyHostToDevice, stream));
+        // Launch lists: worlds that fit the LDS kernel first, then the others.
+        std::vector<uint32_t> order;
+        order.reserve(worlds.size());
+        for (size_t wi = 0; wi < worlds.size(); ++wi)
+            if (worlds[wi].bodies.size() <= bpr::kRigidLdsBodies) order.push_back((uint32_t)wi);
+        smallCount = (uint32_t)order.size();
+        for (size_t wi = 0; wi < worlds.size(); ++wi)
+            if (worlds[wi].bodies.size() > bpr::kRigidLdsBodies) order.push_back((uint32_t)wi);
+        largeCount = (uint32_t)(order.size() - smallCount);
+        if (!order.empty())
+            BP_HIP(hipMemcpyAsync(orders, order.data(), order.size() * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
         BP_HIP(hipStreamSynchronize(stream));
         arena.bodies[0] = bodies[0];
         arena.bodies[1] = bodies[1];
@@ -303,6 +349,8 @@ struct bp_rigid_batch {
         arena.rows = rows;
         arena.corr = corr;
         arena.slots = slots;
+        arena.twin = twin;
+        arena.toi = toi;
         arena.rowCount = rowCount;
         arena.rowStart = rowStart;
         arena.walls = walls;
@@ -321,7 +369,9 @@ struct bp_rigid_batch {
         bp_status s = grow(infos, capInfos, W * steps);
         if (s != BP_OK) return s;
         BP_HIP(hipEventRecord(start, stream));
-        hipError_t e = bp_launch_rigid_step(&arena, descs, infos, (uint32_t)W, steps, stream);
+        hipError_t e = bp_launch_rigid_step(&arena, descs, orders, smallCount, 1, infos, (uint32_t)W, steps, stream);
+        if (e == hipSuccess)
+            e = bp_launch_rigid_step(&arena, descs, orders + smallCount, largeCount, 0, infos, (uint32_t)W, steps, stream);
         if (e != hipSuccess) { set_error(std::string("rigid kernel launch: ") + hipGetErrorString(e)); return BP_ERR_COMMAND_FAILED; }
         BP_HIP(hipEventRecord(stop, stream));
         BP_HIP(hipMemcpyAsync(hostDescs.data(), descs, W * sizeof(bpr::RigidDesc), hipMemcpyDeviceToHost, stream));

@@ -232,44 +232,9 @@ def region_exit(wrap):
 
 
 def mixed_world(seed, bodies=48, statics=6, joints=True):
-    """A seeded crowd of boxes, circles, capsules and convex polygons in a walled box, with a few
-    joints: the parity workload of the rigid kernel."""
-    rng = np.random.default_rng(seed)
-    objs = [rigid.box(1, (200, 5), (200, 5), role="static"), rigid.box(2, (5, 200), (5, 200), role="static"),
-            rigid.box(3, (395, 200), (5, 200), role="static"),
-            rigid.RigidObject(id=4, shape="circle", radius=12.0, position=(200.0, 120.0), role="static"),
-            rigid.RigidObject(id=5, shape="capsule", radius=6.0, half_length=20.0, position=(120.0, 200.0), rotation=0.4,
-                              role="static"),
-            rigid.box(6, (300, 90), (30, 4), role="kinematic", velocity=(-20, 0))][:statics]
-    kinds = ("box", "circle", "capsule", "poly")
-    for k in range(bodies):
-        x, y = 30 + (k % 8) * 45 + rng.uniform(-4, 4), 160 + (k // 8) * 40 + rng.uniform(-4, 4)
-        v = (float(rng.normal(0, 40)), float(rng.normal(0, 40)))
-        kind = kinds[k % 4]
-        common = dict(velocity=v, angular_velocity=float(rng.normal(0, 2)), friction=float(rng.uniform(0, 0.6)),
-                      restitution=float(rng.uniform(0, 0.4)), damping=float(rng.uniform(0, 0.5)), mass=float(rng.uniform(0.5, 2)))
-        if kind == "box":
-            objs.append(rigid.box(10 + k, (x, y), (float(rng.uniform(5, 12)), float(rng.uniform(4, 10))), role="dynamic",
-                                  rotation=float(rng.uniform(0, 3)), **common))
-        elif kind == "circle":
-            objs.append(rigid.circle(10 + k, (x, y), float(rng.uniform(5, 12)), **common))
-        elif kind == "capsule":
-            objs.append(rigid.capsule(10 + k, (x, y), float(rng.uniform(4, 10)), float(rng.uniform(3, 7)),
-                                      rotation=float(rng.uniform(0, 3)), **common))
-        else:
-            m = int(rng.integers(3, 8))
-            r = float(rng.uniform(6, 13))
-            pts = [(r * math.cos(2 * math.pi * i / m), r * math.sin(2 * math.pi * i / m)) for i in range(m)]
-            objs.append(rigid.RigidObject(id=10 + k, shape="polygon", vertices=pts, position=(x, y), role="dynamic",
-                                          rotation=float(rng.uniform(0, 3)), **common))
-    js = []
-    if joints:
-        js = [J(1, 10, 11, kind=abi.BP_JOINT_HINGE, anchor_a=(8, 0), anchor_b=(-8, 0), limits=(-0.3, 0.3)),
-              J(2, 12, None, anchor_b=(130.0, 330.0), length=60.0, softness=0.001, damping=1.0),
-              J(3, 14, 15, kind=abi.BP_JOINT_WELD, anchor_a=(10, 0), anchor_b=(-10, 0)),
-              J(4, 16, 17, length=45.0, collide_connected=True),
-              J(5, 18, None, kind=abi.BP_JOINT_HINGE, anchor_b=(240.0, 300.0), motor=(1.5, 5_000))]
-    return Case(f"mixed-{seed}", objs, cfg((0, 0, 400, 500), gravity=(0, -300)), 40, joints=js, interested=(10, 12))
+    """The seeded rigid crowd of bubblephysics_amd.rigid.crowd (the rigid bench workload) as a case."""
+    objs, config, js = rigid.crowd(seed, bodies=bodies, statics=statics, joints=joints)
+    return Case(f"mixed-{seed}", objs, config, 40, joints=js, interested=(10, 12))
 
 
 def reference_cases():
