@@ -23,6 +23,7 @@ One JSON line on stdout. value = bubble-substeps/s = sum over worlds and frames 
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import math
 import os
@@ -276,10 +277,13 @@ def rigid_scenes(worlds: int, bodies: int):
 
 
 def run_rigid(args, scenes_rigid):
-    """Rigid solver for worlds without bubbles: every world in one launch, all timed steps inside it."""
+    """Rigid solver for worlds without bubbles: every world in one launch, all timed steps inside it.
+    The untimed warm-up steps let the crowds fall and pile up, so the timed steps are contact-heavy.
+    Returns the line and the warmed-up states of the first worlds (the CPU leg starts from them)."""
     from bubblephysics_amd import rigid
     batch = rigid.RigidBatch(scenes_rigid)
-    batch.step(args.warmup)
+    batch.step(args.rigid_warmup)
+    warmed = [dataclasses.replace(sc, bodies=batch.read_bodies(w)) for w, sc in enumerate(scenes_rigid[:256])]
     t0 = time.perf_counter()
     infos = batch.step(args.rigid_steps)
     elapsed = time.perf_counter() - t0
@@ -287,27 +291,24 @@ def run_rigid(args, scenes_rigid):
     ok = int((infos["status"] == 0).all(axis=0).sum())
     return {"value": bodies * args.rigid_steps / elapsed, "elapsed": elapsed, "gpu_ms": batch.last_gpu_ms(),
             "bodies": bodies, "worlds": len(scenes_rigid), "ok_worlds": ok,
-            "pairs_per_step": float(infos["candidate_pairs"].mean(axis=1).sum())}
+            "pairs_per_step": float(infos["candidate_pairs"].mean(axis=1).sum()),
+            "contacts": sum(len(batch.contacts(w)) for w in range(min(64, len(scenes_rigid)))) / min(64, len(scenes_rigid))}, warmed
 
 
-def cpu_rigid(scenes_rigid, threads, warmup, steps, budget_s):
-    """The oracle's rigid solver on the host cores: worlds spread over `threads` threads, each run
-    for the GPU line's warm-up and timed steps; as many worlds as fit about `budget_s` of work."""
+def cpu_rigid(warmed, threads, steps, budget_s):
+    """The oracle's rigid solver on the host cores, from the GPU line's warmed-up states (the same
+    bits as the oracle's own warm-up): worlds spread over `threads` threads, each run for the GPU
+    line's timed steps; as many worlds as fit about `budget_s` of work."""
     from oracle import bp_oracle
     import concurrent.futures as cf
     probe = time.perf_counter()
-    bp_oracle.run_rigid(scenes_rigid[0], warmup + steps)
+    bp_oracle.run_rigid(warmed[0], steps)
     per_world = max(time.perf_counter() - probe, 1e-4)
-    count = int(min(len(scenes_rigid), max(threads, budget_s * threads / per_world)))
-    sample = scenes_rigid[:count]
-    warmed = []
-    for sc in sample:
-        b, j, _, _ = bp_oracle.run_rigid(sc, warmup)
-        c = type(sc)(**{**sc.__dict__, "bodies": b})
-        warmed.append(c)
+    count = int(min(len(warmed), max(threads, budget_s * threads / per_world)))
+    sample = warmed[:count]
     t0 = time.perf_counter()
     with cf.ThreadPoolExecutor(max_workers=threads) as pool:
-        list(pool.map(lambda sc: bp_oracle.run_rigid(sc, steps), warmed))
+        list(pool.map(lambda sc: bp_oracle.run_rigid(sc, steps), sample))
     elapsed = time.perf_counter() - t0
     bodies = sum(len(sc.bodies) for sc in sample)
     return {"value": bodies * steps / elapsed, "worlds": count, "bodies": bodies, "threads": threads, "seconds": elapsed}
@@ -449,6 +450,7 @@ def main():
     ap.add_argument("--rigid-worlds", type=int, default=4096, help="rigid worlds without bubbles (0 skips the line)")
     ap.add_argument("--rigid-bodies", type=int, default=48, help="dynamic bodies per rigid world")
     ap.add_argument("--rigid-steps", type=int, default=20)
+    ap.add_argument("--rigid-warmup", type=int, default=120, help="untimed rigid steps before the timed ones")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: the worlds are independent, so a multi-GPU run would shard them by "
@@ -483,10 +485,10 @@ def main():
         t_build = time.perf_counter()
         rigid_list = rigid_scenes(args.rigid_worlds, args.rigid_bodies)
         log(f"built {len(rigid_list)} rigid worlds in {time.perf_counter() - t_build:.1f}s")
-        rigid_line = run_rigid(args, rigid_list)
+        rigid_line, rigid_warmed = run_rigid(args, rigid_list)
         log(f"rigid: {rigid_line['value']:.3g} body-steps/s, {rigid_line['gpu_ms']:.2f} ms per {args.rigid_steps}-step launch")
         if not args.no_cpu:
-            rigid_line["cpu"] = cpu_rigid(rigid_list, args.cpu_threads, args.warmup, args.rigid_steps, args.cpu_seconds / 3)
+            rigid_line["cpu"] = cpu_rigid(rigid_warmed, args.cpu_threads, args.rigid_steps, args.cpu_seconds / 3)
             log(f"rigid cpu: {rigid_line['cpu']['value']:.3g} body-steps/s")
     served_c4 = run_served_sweep(args, scenes_list, plain_value) if args.serve_c4_seconds > 0 and args.serve_c4_rates else None
     served, evict = run_served(args) if args.serve_seconds > 0 else (None, None)
@@ -548,15 +550,16 @@ def main():
         "rigid": None if rigid_line is None else {
             "workload": f"rigid worlds without bubbles: {rigid_line['worlds']} seeded crowds of {args.rigid_bodies} dynamic "
                         f"bodies (boxes, circles, capsules, convex polygons) + 6 statics and kinematics, 5 joints, gravity; "
-                        f"{args.rigid_steps} steps in one launch, one workgroup per world",
+                        f"{args.rigid_warmup} untimed steps (the crowds fall and pile up), then {args.rigid_steps} "
+                        f"timed steps in one launch, one workgroup per world",
             "metric": "rigid body-steps/s", "value": rigid_line["value"], "unit": "body-steps/s",
             "ms_per_step": rigid_line["elapsed"] / args.rigid_steps * 1e3,
             "kernel_ms_per_launch": rigid_line["gpu_ms"], "bodies": rigid_line["bodies"], "ok_worlds": rigid_line["ok_worlds"],
-            "candidate_pairs_per_step": rigid_line["pairs_per_step"],
+            "candidate_pairs_per_step": rigid_line["pairs_per_step"], "contacts_per_world": rigid_line["contacts"],
             "cpu_baseline": None if "cpu" not in rigid_line else {
                 "value": rigid_line["cpu"]["value"], "unit": "body-steps/s", "cores": rigid_line["cpu"]["threads"],
-                "kind": "port", "sample": f"oracle FP32 rigid solver on the first {rigid_line['cpu']['worlds']} worlds, "
-                f"{args.warmup} untimed + {args.rigid_steps} timed steps each, over {rigid_line['cpu']['threads']} threads"}},
+                "kind": "port", "sample": f"oracle FP32 rigid solver on the first {rigid_line['cpu']['worlds']} worlds from "
+                f"their warmed-up states, {args.rigid_steps} timed steps each, over {rigid_line['cpu']['threads']} threads"}},
         "served_batch_worlds": None if served_c4 is None else {
             "workload": f"serving the {res['worlds']} batched worlds above: Poisson step requests at rising offered loads, "
                         f"{args.serve_c4_seconds:.0f}s each, 10% with transactions; over_plain against config 1's plain step",

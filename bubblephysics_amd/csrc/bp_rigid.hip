@@ -11,13 +11,14 @@
 //   speed clamps and the region rule (PostSolve:180-206).
 // Each of those is a dispatch in the reference whose threads write only their own body, row slot
 // or joint; here each is a loop over the workgroup's lanes followed by a workgroup barrier, so the
-// passes see the same inputs and give the same bits. Bodies live in global memory at the world's
-// working side (L1/L2-resident for the world's workgroup); rows, corrections and wall records stay
+// passes see the same inputs and give the same bits. Worlds of at most kRigidLdsBodies bodies run
+// one wavefront with their bodies in LDS; larger worlds run 256 lanes with their bodies in global
+// memory at the world's working side. Rows, corrections and wall records stay in global memory,
 // where the host reads the step's contact records.
 //
-// Publication: the kernel copies the published side into the working side, steps there, and on
-// success makes the working side the published one (NativeMetalSolver.swift:1179-1201: a failed
-// step publishes nothing). A failed world is not stepped again until new input.
+// Publication: the kernel copies the published side into the working bodies, steps there, and on
+// success makes them the published side (NativeMetalSolver.swift:1179-1201: a failed step
+// publishes nothing). A failed world is not stepped again until new input.
 #include <hip/hip_runtime.h>
 
 #include "bp_device.h"
