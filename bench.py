@@ -291,7 +291,7 @@ def run_rigid(args, scenes_rigid):
     ok = int((infos["status"] == 0).all(axis=0).sum())
     return {"value": bodies * args.rigid_steps / elapsed, "elapsed": elapsed, "gpu_ms": batch.last_gpu_ms(),
             "bodies": bodies, "worlds": len(scenes_rigid), "ok_worlds": ok,
-            "pairs_per_step": float(infos["candidate_pairs"].mean(axis=1).sum()),
+            "pairs_per_step": float(infos["candidate_pairs"].sum(axis=1).mean()),
             "contacts": sum(len(batch.contacts(w)) for w in range(min(64, len(scenes_rigid)))) / min(64, len(scenes_rigid))}, warmed
 
 

@@ -60,6 +60,9 @@ struct RigidArena {
     uint32_t *slots;              // used row slots of the step, compacted
     uint32_t *twin;               // per used slot: the slot of the same pair in the other body's row
     float *toi;                   // per used slot: the pair's time of impact
+    uint32_t *toiSteps;           // per used slot: slices of the pair's impact-time search (0: none)
+    uint32_t *toiStart;           // per used slot: first slice's index in the world's slice list
+    uint32_t *toiFirst;           // per used slot: first slice whose pose overlaps
     uint32_t *rowCount;           // indexed by bodyOffset
     uint32_t *rowStart;
     RigidWall *walls;

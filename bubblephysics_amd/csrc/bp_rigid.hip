@@ -78,6 +78,21 @@ struct View {
     }
 };
 
+// Pruning. A body's vertices lie within its reach of its pose (the farthest vertex plus the shape
+// radius), so geometry farther than the reach from every pose a test visits can give no hit, no
+// overlap and no crossing there. The tests below skip such geometry; the slack is far above the
+// rounding of the exact tests at the coordinates involved, so a skipped test is one whose exact
+// evaluation returns "nothing": results are unchanged.
+BPD float body_reach(const View &w, const bp_rigid_body &b) {
+    float r = 0.0f;
+    for (uint32_t k = 0; k < b.ranges[1]; ++k) r = fmaxf(r, len2(w.verts[b.ranges[0] + k]));
+    return r + b.shape[1];
+}
+BPD float prune_slack(float reach, float magnitude) { return 1e-2f + 1e-4f * reach + 1e-5f * magnitude; }
+BPD float magnitude2(float2 p) { return fmaxf(fabsf(p.x), fabsf(p.y)); }
+// Axis-aligned boxes (lo.x, lo.y, hi.x, hi.y) that do not meet.
+BPD bool boxes_apart(float4 a, float4 b) { return a.z < b.x || b.z < a.x || a.w < b.y || b.w < a.y; }
+
 BPD float seg_distance(float2 a, float2 b, float2 c, float2 d, float2 &pa, float2 &pb) {
     float2 r = sub(b, a), s = sub(d, c);
     float den = cross2(r, s);
@@ -150,16 +165,18 @@ BPD void round_round(const Core &A, const Core &B, float2 fallback, float2 &norm
         point = mul(0.5f, add(add(pa, mul(normal, A.radius)), sub(pb, mul(normal, B.radius))));
         return;
     }
-    float2 axes[2];
-    uint32_t count = 0;
+    // The candidate axes are the normals of the non-degenerate cores, A's first.
     float2 ea = sub(A.b, A.a), eb = sub(B.b, B.a);
     float la = len2(ea), lb = len2(eb);
-    if (la > 1e-6f) axes[count++] = dvs(f2(-ea.y, ea.x), la);
-    if (lb > 1e-6f) axes[count++] = dvs(f2(-eb.y, eb.x), lb);
+    const bool useA = la > 1e-6f, useB = lb > 1e-6f;
+    const float2 axisA = useA ? dvs(f2(-ea.y, ea.x), la) : f2(0.0f, 0.0f);
+    const float2 axisB = useB ? dvs(f2(-eb.y, eb.x), lb) : f2(0.0f, 0.0f);
     float2 axis = fallback;
     float overlap = 0.0f, orientation = 1.0f, best = inff();
-    for (uint32_t i = 0; i < count; ++i) {
-        float2 c = axes[i];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        if (!(i == 0 ? useA : useB)) continue;
+        float2 c = i == 0 ? axisA : axisB;
         float amin = fminf(dot2(A.a, c), dot2(A.b, c)), amax = fmaxf(dot2(A.a, c), dot2(A.b, c));
         float bmin = fminf(dot2(B.a, c), dot2(B.b, c)), bmax = fmaxf(dot2(B.a, c), dot2(B.b, c));
         float plus = amax - bmin, minus = bmax - amin;
@@ -221,14 +238,29 @@ BPD bool crosses(float2 start, float2 end, const Sweep &s) {
     return false;
 }
 
-BPD bool sat_triangles(const float2 *a, const float2 *b, float2 &normal, float &depth, float2 &point) {
+// The unit normals of a triangle's edges (the axes of Contact's triangle test); false when an edge
+// is degenerate, which refuses every pair with that triangle.
+BPD bool tri_axes(const float2 *t, float2 *axis) {
+    bool ok = true;
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+        const float2 d = sub(t[(e + 1) % 3], t[e]);
+        const float m = len2(d);
+        ok = ok && m > kFloatMin;
+        axis[e] = m > kFloatMin ? dvs(f2(-d.y, d.x), m) : f2(0.0f, 0.0f);
+    }
+    return ok;
+}
+
+// Separating-axis test of two triangles over a's edge normals, then b's, with the axes given.
+BPD bool sat_triangles(const float2 *a, const float2 *axA, const float2 *b, const float2 *axB, float2 &normal, float &depth,
+                       float2 &point) {
     depth = inff();
+#pragma unroll
     for (int shape = 0; shape < 2; ++shape)
+#pragma unroll
         for (int e = 0; e < 3; ++e) {
-            float2 d = shape == 0 ? sub(a[(e + 1) % 3], a[e]) : sub(b[(e + 1) % 3], b[e]);
-            float m = len2(d);
-            if (m <= kFloatMin) return false;
-            float2 axis = dvs(f2(-d.y, d.x), m);
+            const float2 axis = shape == 0 ? axA[e] : axB[e];
             float amin = inff(), amax = -inff(), bmin = inff(), bmax = -inff();
             for (int k = 0; k < 3; ++k) {
                 float pa = dot2(a[k], axis), pb = dot2(b[k], axis);
@@ -346,9 +378,6 @@ BPD bool core_crosses(const View &w, const Core &R, const bp_rigid_body &P, cons
     return false;
 }
 
-// Normal from a to b.
-constexpr uint32_t kLocalVerts = 8;    // polygon vertices transformed once per pair evaluation
-
 // Normal from a to b, with the bodies at the poses xa and xb (their own or interpolated ones).
 BPD bool pair_contact_at(const View &w, const bp_rigid_body &a, const Xf &xa, const bp_rigid_body &b, const Xf &xb,
                          float2 &n, float &depth, float2 &point) {
@@ -358,30 +387,30 @@ BPD bool pair_contact_at(const View &w, const bp_rigid_body &a, const Xf &xa, co
         return true;
     }
     if (ra != rb) {
+        // The poses are selected by value: a select between references would put both in memory.
+        const Xf xr = ra ? xa : xb, xp = ra ? xb : xa;
         float2 normal;
-        round_polygon(w, ra ? core(w, a, xa) : core(w, b, xb), ra ? b : a, ra ? xb : xa, normal, depth, point);
+        round_polygon(w, core(w, ra ? a : b, xr), ra ? b : a, xp, normal, depth, point);
         n = ra ? normal : neg(normal);
         return true;
     }
-    // Both polygons: their vertices in the world frame once, then the triangle pairs.
-    float2 wa[kLocalVerts], wb[kLocalVerts];
-    const bool localA = a.ranges[1] <= kLocalVerts, localB = b.ranges[1] <= kLocalVerts;
-    if (localA)
-        for (uint32_t k = 0; k < a.ranges[1]; ++k) wa[k] = w.vertex(a, xa, k);
-    if (localB)
-        for (uint32_t k = 0; k < b.ranges[1]; ++k) wb[k] = w.vertex(b, xb, k);
+    // Both polygons: the triangle pairs, with each corner transformed where it is used (the same
+    // operations as a transform of every vertex first, without a per-lane array in memory).
     bool found = false;
+    // A triangle's axes are the same for every triangle it meets: a's are taken once per triangle.
     for (uint32_t i = 0; i < a.ranges[3]; ++i) {
         uint4 ta = w.tris[a.ranges[2] + i];
-        float2 av[3] = {localA ? wa[ta.x] : w.vertex(a, xa, ta.x), localA ? wa[ta.y] : w.vertex(a, xa, ta.y),
-                        localA ? wa[ta.z] : w.vertex(a, xa, ta.z)};
+        float2 av[3] = {w.vertex(a, xa, ta.x), w.vertex(a, xa, ta.y), w.vertex(a, xa, ta.z)};
+        float2 axA[3];
+        if (!tri_axes(av, axA)) continue;
         for (uint32_t j = 0; j < b.ranges[3]; ++j) {
             uint4 tb = w.tris[b.ranges[2] + j];
-            float2 bv[3] = {localB ? wb[tb.x] : w.vertex(b, xb, tb.x), localB ? wb[tb.y] : w.vertex(b, xb, tb.y),
-                            localB ? wb[tb.z] : w.vertex(b, xb, tb.z)};
+            float2 bv[3] = {w.vertex(b, xb, tb.x), w.vertex(b, xb, tb.y), w.vertex(b, xb, tb.z)};
+            float2 axB[3];
+            if (!tri_axes(bv, axB)) continue;
             float2 normal, p;
             float d;
-            if (sat_triangles(av, bv, normal, d, p) && (!found || d > depth)) {
+            if (sat_triangles(av, axA, bv, axB, normal, d, p) && (!found || d > depth)) {
                 found = true;
                 depth = d;
                 n = normal;
@@ -402,11 +431,20 @@ struct P3 {
 BPD P3 lerp3(P3 s, P3 e, float t) { return P3{mixs(s.x, e.x, t), mixs(s.y, e.y, t), mixs(s.a, e.a, t)}; }
 
 // Swept vertices of body b from pose p to q against the world's static edges (PostSolve:29-62).
-BPD bool swept_hit(const View &w, const bp_rigid_body &b, P3 p, P3 q, uint32_t entity, float2 &point, float2 &normal,
-                   int64_t &owner) {
+BPD bool swept_hit(const View &w, const bp_rigid_body &b, float reach, P3 p, P3 q, uint32_t entity, float2 &point,
+                   float2 &normal, int64_t &owner) {
     const Xf xp = xf_at(p.x, p.y, p.a), xq = xf_at(q.x, q.y, q.a);
+    // Every swept vertex and edge stays within `reach` of the centre's path from p to q.
+    const float2 c0 = f2(p.x, p.y), c1 = f2(q.x, q.y);
+    const float pathMagnitude = fmaxf(magnitude2(c0), magnitude2(c1));
     for (uint32_t j = 0; j < w.segCount; ++j) {
         const bp_segment &s = w.segs[j];
+        {
+            const float2 a = f2(s.a[0], s.a[1]), e = f2(s.b[0], s.b[1]);
+            const float r = reach + prune_slack(reach, fmaxf(pathMagnitude, fmaxf(magnitude2(a), magnitude2(e))));
+            const float4 path = make_float4(fminf(c0.x, c1.x) - r, fminf(c0.y, c1.y) - r, fmaxf(c0.x, c1.x) + r, fmaxf(c0.y, c1.y) + r);
+            if (boxes_apart(path, make_float4(fminf(a.x, e.x), fminf(a.y, e.y), fmaxf(a.x, e.x), fmaxf(a.y, e.y)))) continue;
+        }
         const bool owned = (s.flags & BP_SEGMENT_HAS_OWNER) != 0u;
         if (owned && (uint64_t)s.owner_id == b.id) continue;
         if (owned && w.excluded(entity, (uint64_t)s.owner_id)) continue;
@@ -436,40 +474,29 @@ BPD bool swept_hit(const View &w, const bp_rigid_body &b, P3 p, P3 q, uint32_t e
     return false;
 }
 
-BPD bool overlap_at(const View &w, const bp_rigid_body &a, const bp_rigid_body &b, float t) {
-    const Xf xa = xf_at(mixs(a.previous[0], a.pose[0], t), mixs(a.previous[1], a.pose[1], t), mixs(a.pose[3], a.pose[2], t));
-    const Xf xb = xf_at(mixs(b.previous[0], b.pose[0], t), mixs(b.previous[1], b.pose[1], t), mixs(b.pose[3], b.pose[2], t));
+// `reach` is the sum of both bodies' reaches: farther apart, the pair has no contact of positive
+// depth at t.
+BPD bool overlap_at(const View &w, const bp_rigid_body &a, const bp_rigid_body &b, float reach, float t) {
+    const float2 pa = f2(mixs(a.previous[0], a.pose[0], t), mixs(a.previous[1], a.pose[1], t));
+    const float2 pb = f2(mixs(b.previous[0], b.pose[0], t), mixs(b.previous[1], b.pose[1], t));
+    const float limit = reach + prune_slack(reach, fmaxf(magnitude2(pa), magnitude2(pb)));
+    const float2 gap = sub(pa, pb);
+    if (dot2(gap, gap) > limit * limit) return false;
+    const Xf xa = xf_at(pa.x, pa.y, mixs(a.pose[3], a.pose[2], t));
+    const Xf xb = xf_at(pb.x, pb.y, mixs(b.pose[3], b.pose[2], t));
     float2 n, point;
     float depth;
     return pair_contact_at(w, a, xa, b, xb, n, depth, point) && depth > 1e-3f;
 }
 
-BPD float impact_time(const View &w, const bp_rigid_body &a, const bp_rigid_body &b) {
-    float travel = len2(sub(sub(xy(a.pose), xy(a.previous)), sub(xy(b.pose), xy(b.previous)))) +
-                   fabsf(a.pose[2] - a.pose[3]) * a.physical[2] + fabsf(b.pose[2] - b.pose[3]) * b.physical[2];
-    if (!(travel > 0.5f)) return 2.0f;
-    if (overlap_at(w, a, b, 0.0f)) return 2.0f;
-    float slices = ceilf(travel / 0.5f);
-    uint32_t steps = slices >= 64.0f ? 64u : max(2u, (uint32_t)slices);
-    float last = 0.0f;
-    for (uint32_t step = 1; step <= steps; ++step) {
-        float t = float(step) / float(steps);
-        if (!overlap_at(w, a, b, t)) {
-            last = t;
-            continue;
-        }
-        float low = last, high = t;
-        for (int i = 0; i < 12; ++i) {
-            float mid = 0.5f * (low + high);
-            if (overlap_at(w, a, b, mid)) high = mid;
-            else low = mid;
-        }
-        return low;
-    }
-    return 2.0f;
+// The travel that decides whether a pair's impact time is searched (PostSolve:220-226).
+BPD float pair_travel(const bp_rigid_body &a, const bp_rigid_body &b) {
+    return len2(sub(sub(xy(a.pose), xy(a.previous)), sub(xy(b.pose), xy(b.previous)))) +
+           fabsf(a.pose[2] - a.pose[3]) * a.physical[2] + fabsf(b.pose[2] - b.pose[3]) * b.physical[2];
 }
 
 constexpr uint32_t kNoTwin = 0xffffffffu;
+constexpr uint32_t kGuardWalls = 32;
 
 BPD bool key_less(const RigidCand &a, const RigidCand &b) {
     if (a.kind != b.kind) return a.kind < b.kind;
@@ -690,6 +717,11 @@ __global__ __launch_bounds__(BLOCK) void rigid_step_kernel(RigidArena A, RigidDe
     __shared__ uint32_t sScan[BLOCK];
     __shared__ bp_rigid_body sBodies[CAP > 0 ? CAP : 1];
     __shared__ uint32_t sSafety, sOverflow, sMaxPer, sPairs, sRemoved, sWrapped, sTotal;
+    __shared__ uint32_t sKinds[4];
+    // The static polygons the wall guard tests, in body order, with their rotation terms and boxes
+    // at the current iteration (worlds with more walls test every body as walls in place).
+    __shared__ uint32_t sWalls, sWallIndex[kGuardWalls];
+    __shared__ float4 sWallRot[kGuardWalls], sWallBox[kGuardWalls];
     const uint32_t wi = order[blockIdx.x];
     const uint32_t tid = threadIdx.x;
     RigidDesc &D = descs[wi];
@@ -707,6 +739,7 @@ __global__ __launch_bounds__(BLOCK) void rigid_step_kernel(RigidArena A, RigidDe
     uint32_t *slots = A.slots + D.rowOffset;
     uint32_t *twin = A.twin + D.rowOffset;
     float *toi = A.toi + D.rowOffset;
+    uint32_t *toiSteps = A.toiSteps + D.rowOffset, *toiStart = A.toiStart + D.rowOffset, *toiFirst = A.toiFirst + D.rowOffset;
     uint32_t *rowCount = A.rowCount + D.bodyOffset;
     uint32_t *rowStart = A.rowStart + D.bodyOffset;
     RigidWall *walls = A.walls + D.bodyOffset;
@@ -750,7 +783,6 @@ __global__ __launch_bounds__(BLOCK) void rigid_step_kernel(RigidArena A, RigidDe
             B[i] = b;
             walls[i] = RigidWall{};
         }
-        for (uint32_t e = tid; e < n * stride; e += BLOCK) corr[e] = RigidCorr{};
         for (uint32_t j = tid; j < D.jointCount; j += BLOCK) {
             bp_joint jt = A.jointInput[D.jointOffset + j];
             jt.state[0] = jt.state[1] = jt.state[2] = jt.state[3] = 0.0f;
@@ -758,6 +790,15 @@ __global__ __launch_bounds__(BLOCK) void rigid_step_kernel(RigidArena A, RigidDe
         }
         __syncthreads();
 
+        if (tid == 0) {
+            uint32_t count = 0;
+            for (uint32_t j = 0; j < n; ++j)
+                if (B[j].flags[0] && !is_round(B[j])) {
+                    if (count < kGuardWalls) sWallIndex[count] = j;
+                    ++count;
+                }
+            sWalls = count;
+        }
         // Prediction (PostSolve:64-112).
         for (uint32_t i = tid; i < n; i += BLOCK) {
             bp_rigid_body b = B[i];
@@ -796,6 +837,7 @@ __global__ __launch_bounds__(BLOCK) void rigid_step_kernel(RigidArena A, RigidDe
             }
             const uint32_t samples = max(1u, (uint32_t)turns);
             const P3 end{b.pose[0], b.pose[1], b.pose[2]};
+            const float reach = body_reach(w, b);
             bool crossed = false;
             float2 hitPoint = f2(0.0f, 0.0f), hitNormal = f2(0.0f, 0.0f);
             int64_t hitOwner = 0;
@@ -803,14 +845,14 @@ __global__ __launch_bounds__(BLOCK) void rigid_step_kernel(RigidArena A, RigidDe
             for (uint32_t s = 0; s < samples && !crossed; ++s) {
                 P3 p = lerp3(start, end, float(s) / float(samples));
                 P3 q = lerp3(start, end, float(s + 1) / float(samples));
-                crossed = swept_hit(w, b, p, q, i, hitPoint, hitNormal, hitOwner);
+                crossed = swept_hit(w, b, reach, p, q, i, hitPoint, hitNormal, hitOwner);
                 if (!crossed) continue;
                 P3 last = p, hit = q;
                 for (int k = 0; k < 12; ++k) {
                     P3 mid{0.5f * (last.x + hit.x), 0.5f * (last.y + hit.y), 0.5f * (last.a + hit.a)};
                     float2 pt, nm;
                     int64_t ow;
-                    if (swept_hit(w, b, p, mid, i, pt, nm, ow)) hit = mid;
+                    if (swept_hit(w, b, reach, p, mid, i, pt, nm, ow)) hit = mid;
                     else last = mid;
                 }
                 stop = last;
@@ -898,10 +940,12 @@ __global__ __launch_bounds__(BLOCK) void rigid_step_kernel(RigidArena A, RigidDe
         }
         __syncthreads();
 
-        // Each used entry's twin: the same pair in the other body's row (rows are sorted by key).
+        // Each used entry's twin: the same pair in the other body's row (rows are sorted by key). The
+        // entry's correction starts the step at zero (only used entries are ever read).
         const uint32_t total = sTotal;
         for (uint32_t e = tid; e < total; e += BLOCK) {
             const uint32_t index = slots[e];
+            corr[index] = RigidCorr{};
             const uint32_t owner = index / stride;
             const RigidCand c = rows[index];
             const uint32_t other = owner == c.low ? c.high : c.low;
@@ -917,16 +961,106 @@ __global__ __launch_bounds__(BLOCK) void rigid_step_kernel(RigidArena A, RigidDe
             twin[e] = found ? rowStart[other] + lo : kNoTwin;
         }
         __syncthreads();
-        // Time of impact of each pair (PostSolve:220-238), once per pair, one lane per entry; then each
-        // movable body takes the earliest over its row (PostSolve:243-257) and rolls back to it
-        // (PostSolve:260-271).
+        // Time of impact of each pair (PostSolve:220-238), once per pair; then each movable body takes
+        // the earliest over its row (PostSolve:243-257) and rolls back to it (PostSolve:260-271). The
+        // reference searches a pair's slices in order and bisects after the first that overlaps; here
+        // (a) one lane per pair tests the start and counts the slices, (b) one lane per slice tests
+        // that slice's poses and each pair keeps its earliest overlapping slice, (c) one lane per pair
+        // bisects between that slice and the one before. The tests are the same pure function of the
+        // same inputs, and the earliest overlapping slice is the one where the search in order stops.
         for (uint32_t e = tid; e < total; e += BLOCK) {
             const uint32_t index = slots[e], pair = twin[e];
             const RigidCand c = rows[index];
-            if (pair != kNoTwin && index / stride != c.low) continue;
-            const float t = impact_time(w, B[c.low], B[c.high]);
-            toi[e] = t;
-            if (pair != kNoTwin) toi[pair] = t;
+            uint32_t slices = 0;
+            if (!(pair != kNoTwin && index / stride != c.low)) {
+                const bp_rigid_body &a = B[c.low], &b = B[c.high];
+                if (pair_travel(a, b) > 0.5f) {
+                    const float reach = body_reach(w, a) + body_reach(w, b);
+                    if (!overlap_at(w, a, b, reach, 0.0f)) {
+                        const float count = ceilf(pair_travel(a, b) / 0.5f);
+                        slices = count >= 64.0f ? 64u : max(2u, (uint32_t)count);
+                        toi[e] = reach;                  // kept for (b) and (c), then replaced
+                        toiFirst[e] = kNoTwin;
+                    }
+                }
+                if (slices == 0) {
+                    toi[e] = 2.0f;
+                    if (pair != kNoTwin) toi[pair] = 2.0f;
+                }
+            }
+            toiSteps[e] = slices;
+        }
+        __syncthreads();
+        const uint32_t sliceTotal = block_prefix<BLOCK>(sScan, toiSteps, toiStart, total);
+        __syncthreads();
+        for (uint32_t item = tid; item < sliceTotal; item += BLOCK) {
+            uint32_t lo = 0, hi = total;             // the last entry whose slices start at or before item
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (toiStart[mid] <= item) lo = mid + 1;
+                else hi = mid;
+            }
+            const uint32_t e = lo - 1, step = item - toiStart[e] + 1, count = toiSteps[e];
+            const RigidCand c = rows[slots[e]];
+            if (overlap_at(w, B[c.low], B[c.high], toi[e], float(step) / float(count))) atomicMin(&toiFirst[e], step);
+        }
+        __syncthreads();
+        // (c) The pairs with an overlapping slice, listed (in toiStart, free after (b)); the others
+        // have no impact in the step.
+        if (tid == 0) sKinds[0] = 0u;
+        __syncthreads();
+        for (uint32_t e = tid; e < total; e += BLOCK) {
+            if (toiSteps[e] == 0u) continue;
+            if (toiFirst[e] != kNoTwin) {
+                toiStart[atomicAdd(&sKinds[0], 1u)] = e;
+            } else {
+                toi[e] = 2.0f;
+                if (twin[e] != kNoTwin) toi[twin[e]] = 2.0f;
+            }
+        }
+        __syncthreads();
+        // The bisection's 12 halvings, two at a time by a group of four lanes: the midpoint and the
+        // two midpoints that can follow it are tested together, and the group takes the path the
+        // sequential bisection takes (the same midpoints, so the same bits).
+        {
+            const uint32_t hits = sKinds[0], member = tid & 3u;
+            for (uint32_t base = 0; base < hits; base += BLOCK / 4) {
+                const uint32_t h = base + tid / 4u;
+                const bool active = h < hits;
+                const uint32_t e = active ? toiStart[h] : 0u;
+                float low = 0.0f, high = 0.0f, reach = 0.0f;
+                uint32_t lowBody = 0, highBody = 0;
+                if (active) {
+                    const uint32_t count = toiSteps[e], first = toiFirst[e];
+                    const RigidCand c = rows[slots[e]];
+                    lowBody = c.low;
+                    highBody = c.high;
+                    reach = toi[e];
+                    low = first == 1u ? 0.0f : float(first - 1u) / float(count);
+                    high = float(first) / float(count);
+                }
+                for (int round = 0; round < 6; ++round) {
+                    const float mid = 0.5f * (low + high), left = 0.5f * (low + mid), right = 0.5f * (mid + high);
+                    const float t = member == 0u ? mid : (member == 1u ? left : right);
+                    const bool hit = active && member < 3u && overlap_at(w, B[lowBody], B[highBody], reach, t);
+                    const int lane = (int)(threadIdx.x & 63u) & ~3;
+                    const bool atMid = __shfl((int)hit, lane, 64) != 0, atLeft = __shfl((int)hit, lane + 1, 64) != 0,
+                               atRight = __shfl((int)hit, lane + 2, 64) != 0;
+                    if (atMid) {
+                        high = mid;
+                        if (atLeft) high = left;
+                        else low = left;
+                    } else {
+                        low = mid;
+                        if (atRight) high = right;
+                        else low = right;
+                    }
+                }
+                if (active && member == 0u) {
+                    toi[e] = low;
+                    if (twin[e] != kNoTwin) toi[twin[e]] = low;
+                }
+            }
         }
         __syncthreads();
         for (uint32_t i = tid; i < n; i += BLOCK) {
@@ -948,15 +1082,42 @@ __global__ __launch_bounds__(BLOCK) void rigid_step_kernel(RigidArena A, RigidDe
         }
         __syncthreads();
 
+        // The contact rounds' work list (in toiStart, free after the impact times): the entries that
+        // are evaluated, grouped by the pair's shapes (two round, mixed, two polygons) so that the
+        // lanes of a wavefront run the same contact code. Entries are independent of each other, so
+        // their order changes no result.
+        if (tid < 4) sKinds[tid] = 0u;
+        __syncthreads();
+        for (uint32_t e = tid; e < total; e += BLOCK) {
+            const uint32_t index = slots[e], pair = twin[e];
+            const RigidCand c = rows[index];
+            uint32_t tag = 0xffffffffu;
+            if (!(pair != kNoTwin && index / stride != c.low)) {
+                const uint32_t kind = (uint32_t)is_round(B[c.low]) + (uint32_t)is_round(B[c.high]);
+                tag = (kind << 24) | atomicAdd(&sKinds[kind], 1u);
+            }
+            toiSteps[e] = tag;
+        }
+        __syncthreads();
+        const uint32_t work = sKinds[0] + sKinds[1] + sKinds[2];
+        for (uint32_t e = tid; e < total; e += BLOCK) {
+            const uint32_t tag = toiSteps[e];
+            if (tag == 0xffffffffu) continue;
+            const uint32_t kind = tag >> 24;
+            const uint32_t base = kind == 0u ? 0u : (kind == 1u ? sKinds[0] : sKinds[0] + sKinds[1]);
+            toiStart[base + (tag & 0xffffffu)] = e;
+        }
+        __syncthreads();
+
         for (uint32_t it = 0; it < D.iterations; ++it) {
             // Pair contacts (Contact:382-433). A pair sits in the rows of both its bodies, where the
             // reference evaluates it twice on the same inputs; here the lower body's entry is evaluated
             // once and written to both entries (an entry without a twin, when a row cut or one-sided
             // joint tables leave the pair in one row only, is evaluated on its own).
-            for (uint32_t e = tid; e < total; e += BLOCK) {
+            for (uint32_t k = tid; k < work; k += BLOCK) {
+                const uint32_t e = toiStart[k];
                 const uint32_t index = slots[e], pair = twin[e];
                 const RigidCand cand = rows[index];
-                if (pair != kNoTwin && index / stride != cand.low) continue;
                 RigidCorr r;
                 bool added;
                 float lambda;
@@ -1022,22 +1183,79 @@ __global__ __launch_bounds__(BLOCK) void rigid_step_kernel(RigidArena A, RigidDe
                 __syncthreads();
             }
             // Wall guard of the dynamic bodies against static polygons (PostSolve:116-178).
+            const uint32_t wallCount = sWalls;
+            const bool listed = wallCount <= kGuardWalls;
+            if (listed) {
+                for (uint32_t q = tid; q < wallCount; q += BLOCK) {
+                    const bp_rigid_body &wall = B[sWallIndex[q]];
+                    const Xf xw = xf_of(wall);
+                    float2 lo = f2(inff(), inff()), hi = f2(-inff(), -inff());
+                    for (uint32_t e = 0; e < wall.ranges[1]; ++e) {
+                        const float2 v = w.vertex(wall, xw, e);
+                        lo = min2(lo, v);
+                        hi = max2(hi, v);
+                    }
+                    sWallRot[q] = make_float4(xw.c, xw.s, bpm::cosf_det(-wall.pose[2]), bpm::sinf_det(-wall.pose[2]));
+                    sWallBox[q] = make_float4(lo.x, lo.y, hi.x, hi.y);
+                }
+                __syncthreads();
+            }
             for (uint32_t i = tid; i < n; i += BLOCK) {
                 if (B[i].flags[0] || B[i].filter[2]) continue;
                 bp_rigid_body b = B[i];
                 // The guard moves a body, never turns it: the angle's rotation terms hold for the pass.
                 const Xf now = xf_of(b);
                 const Xf then = xf_at(b.previous[0], b.previous[1], b.pose[3]);
-                for (uint32_t j = 0; j < n; ++j) {
+                // The body's vertices at both poses (the current one as earlier walls left it) lie in
+                // the box below; the guard's tests reach at most 1e-5 beyond them.
+                const float reach = body_reach(w, b);
+                const float2 c0 = xy(b.previous);
+                for (uint32_t q = 0; q < (listed ? wallCount : n); ++q) {
+                    const uint32_t j = listed ? sWallIndex[q] : q;
                     const bp_rigid_body &wall = B[j];
-                    if (!wall.flags[0] || is_round(wall) || !accepts(b.filter, wall.filter)) continue;
+                    if (!listed && (!wall.flags[0] || is_round(wall))) continue;
+                    if (!accepts(b.filter, wall.filter)) continue;
                     if (w.excluded(i, wall.id)) continue;
-                    const Xf xw = xf_of(wall);
-                    const float wic = bpm::cosf_det(-wall.pose[2]), wis = bpm::sinf_det(-wall.pose[2]);
+                    Xf xw;
+                    float2 lo, hi;
+                    if (listed) {
+                        const float4 rt = sWallRot[q], bx = sWallBox[q];
+                        xw = Xf{xy(wall.pose), rt.x, rt.y, wall.pose[2]};
+                        lo = f2(bx.x, bx.y);
+                        hi = f2(bx.z, bx.w);
+                    } else {
+                        xw = xf_of(wall);
+                        lo = f2(inff(), inff());
+                        hi = f2(-inff(), -inff());
+                        for (uint32_t e = 0; e < wall.ranges[1]; ++e) {
+                            const float2 v = w.vertex(wall, xw, e);
+                            lo = min2(lo, v);
+                            hi = max2(hi, v);
+                        }
+                    }
+                    {
+                        const float2 c1 = xy(b.pose);
+                        const float bodyMagnitude = fmaxf(magnitude2(c0), magnitude2(c1));
+                        const float r = reach + prune_slack(reach, fmaxf(bodyMagnitude, fmaxf(magnitude2(lo), magnitude2(hi))));
+                        const float4 box = make_float4(fminf(c0.x, c1.x) - r, fminf(c0.y, c1.y) - r, fmaxf(c0.x, c1.x) + r,
+                                                       fmaxf(c0.y, c1.y) + r);
+                        if (boxes_apart(box, make_float4(lo.x, lo.y, hi.x, hi.y))) continue;
+                    }
+                    const float wic = listed ? sWallRot[q].z : bpm::cosf_det(-wall.pose[2]);
+                    const float wis = listed ? sWallRot[q].w : bpm::sinf_det(-wall.pose[2]);
                     const uint32_t edges = wall.ranges[1];
                     for (uint32_t k = 0; k < b.ranges[1]; ++k) {
                         const float2 p = apply(Xf{xy(b.pose), now.c, now.s, now.a}, w.verts[b.ranges[0] + k]);
                         const float2 start = apply(then, w.verts[b.ranges[0] + k]);
+                        {
+                            // A vertex whose path stays away from the wall's box is inside no part of the
+                            // wall and meets none of its edges.
+                            const float r = prune_slack(0.0f, fmaxf(fmaxf(magnitude2(start), magnitude2(p)),
+                                                                    fmaxf(magnitude2(lo), magnitude2(hi))));
+                            const float4 path = make_float4(fminf(start.x, p.x) - r, fminf(start.y, p.y) - r,
+                                                            fmaxf(start.x, p.x) + r, fmaxf(start.y, p.y) + r);
+                            if (boxes_apart(path, make_float4(lo.x, lo.y, hi.x, hi.y))) continue;
+                        }
                         const bool in = inside_with(w, p, wall, xw, wic, wis);
                         bool startedIn = inside_with(w, start, wall, xw, wic, wis);
                         if (startedIn)
@@ -1050,8 +1268,20 @@ __global__ __launch_bounds__(BLOCK) void rigid_step_kernel(RigidArena A, RigidDe
                             }
                         float nearest = inff(), depth = 0.0f;
                         float2 normal = f2(0.0f, 0.0f);
+                        // A vertex outside the wall acts only through an edge that its path from start
+                        // to p comes near (a crossing, or a start on the edge).
+                        const float4 path = make_float4(fminf(start.x, p.x), fminf(start.y, p.y), fmaxf(start.x, p.x),
+                                                        fmaxf(start.y, p.y));
+                        const float pathMagnitude = fmaxf(magnitude2(start), magnitude2(p));
                         for (uint32_t e = 0; e < edges; ++e) {
                             const float2 ea = w.vertex(wall, xw, e), eb = w.vertex(wall, xw, (e + 1) % edges);
+                            if (!in) {
+                                const float r = prune_slack(0.0f, fmaxf(pathMagnitude, fmaxf(magnitude2(ea), magnitude2(eb))));
+                                const float4 grown = make_float4(path.x - r, path.y - r, path.z + r, path.w + r);
+                                if (boxes_apart(grown, make_float4(fminf(ea.x, eb.x), fminf(ea.y, eb.y), fmaxf(ea.x, eb.x),
+                                                                   fmaxf(ea.y, eb.y))))
+                                    continue;
+                            }
                             const Sweep edge{ea, eb, ea, eb};
                             const float2 dir = sub(eb, ea), out = unit(f2(dir.y, -dir.x));
                             const float signedStart = dot2(sub(start, ea), out);

@@ -154,7 +154,7 @@ struct bp_rigid_batch {
     bp_rigid_frame_info *infos = nullptr;
     size_t capBodies[2] = {0, 0}, capVerts = 0, capTris = 0, capSegs = 0, capJoints = 0, capJointWork = 0, capJcorr = 0,
            capRefs = 0, capRanges = 0, capPartners = 0, capRows = 0, capCorr = 0, capSlots = 0, capRowCount = 0,
-           capRowStart = 0, capWalls = 0, capDescs = 0, capInfos = 0, capTwin = 0, capToi = 0;
+           capRowStart = 0, capWalls = 0, capDescs = 0, capInfos = 0, capTwin = 0, capToi = 0, capToiWork = 0;
     bp_rigid_body *bodies[2] = {nullptr, nullptr};
     float2 *verts = nullptr;
     uint4 *tris = nullptr;
@@ -167,6 +167,7 @@ struct bp_rigid_batch {
     bpr::RigidCand *rows = nullptr;
     uint32_t *slots = nullptr, *rowCount = nullptr, *rowStart = nullptr, *twin = nullptr;
     float *toi = nullptr;
+    uint32_t *toiWork = nullptr;   // per used slot: impact-time slices, their prefix, the first hit
     uint32_t *orders = nullptr;
     size_t capOrders = 0;
     uint32_t smallCount = 0, largeCount = 0;
@@ -175,7 +176,7 @@ struct bp_rigid_batch {
     ~bp_rigid_batch() {
         if (stream) (void)hipStreamSynchronize(stream);
         void *ptrs[] = {bodies[0], bodies[1], verts, tris, segs, jointInput, joints, jcorr, corr, refs, jranges,
-                        partners, rows, slots, rowCount, rowStart, walls, descs, infos, twin, toi, orders};
+                        partners, rows, slots, rowCount, rowStart, walls, descs, infos, twin, toi, toiWork, orders};
         for (void *p : ptrs)
             if (p) (void)hipFree(p);
         if (start) (void)hipEventDestroy(start);
@@ -273,6 +274,7 @@ struct bp_rigid_batch {
         G(grow(slots, capSlots, nrows));
         G(grow(twin, capTwin, nrows));
         G(grow(toi, capToi, nrows));
+        G(grow(toiWork, capToiWork, 3 * nrows));
         G(grow(rowCount, capRowCount, nb));
         G(grow(rowStart, capRowStart, nb));
         G(grow(walls, capWalls, nb));
@@ -351,6 +353,9 @@ struct bp_rigid_batch {
         arena.slots = slots;
         arena.twin = twin;
         arena.toi = toi;
+        arena.toiSteps = toiWork;
+        arena.toiStart = toiWork + nrows;
+        arena.toiFirst = toiWork + 2 * nrows;
         arena.rowCount = rowCount;
         arena.rowStart = rowStart;
         arena.walls = walls;
