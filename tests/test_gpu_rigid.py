@@ -118,3 +118,28 @@ def test_a_failing_world_stops_alone_and_steps_again_after_new_input(oracle):
     # The other world went on from where it was.
     expect = oracle_run(oracle, good, 5)
     assert batch.read_bodies(0).tobytes() == expect[0].tobytes()
+
+
+def test_piled_crowds_match_the_oracle_over_many_launches(oracle):
+    # The bench workload after the crowds have fallen and piled up: resting contacts, walls pushing
+    # back, pairs colliding inside a step (impact-time bisection), joints under load.
+    cases = [mixed_world(200 + s) for s in range(3)]
+    batch = rigid.RigidBatch([c.scene() for c in cases])
+    for _ in range(4):
+        infos = batch.step(40)
+        assert (infos["status"] == abi.BP_OK).all()
+    for w, case in enumerate(cases):
+        expect = oracle_run(oracle, case, 160)
+        print(f"piled crowd {w}: {len(expect[3])} contact records", flush=True)
+        assert len(expect[3]) > 20, "the crowd must have piled up"
+        assert_world_equal(batch, w, case, expect, infos[-1, w])
+
+
+def test_a_world_beyond_the_lds_class_matches_the_oracle(oracle):
+    # More bodies than the one-wavefront class holds: the 256-lane kernel with bodies in global memory.
+    case = mixed_world(300, bodies=120)
+    assert len(case.objects) > 96
+    expect = oracle_run(oracle, case, 60)
+    batch = rigid.RigidBatch([case.scene()])
+    infos = batch.step(60)
+    assert_world_equal(batch, 0, case, expect, infos[-1, 0])
