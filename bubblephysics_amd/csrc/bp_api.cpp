@@ -77,6 +77,100 @@ bp_status validate_world(const HostWorld &w) {
 
 }  // namespace
 
+namespace {
+
+// The controls of a world's descriptor from its configuration (offsets and counts are set by the
+// caller).
+void desc_controls(const HostWorld &w, uint32_t n, bpd::WorldDesc &d) {
+    const bp_world_config &c = w.config;
+    d.dt = c.time_step; d.contactTol = c.contact_tolerance; d.sepTol = c.separation_tolerance;
+    for (int k = 0; k < 4; ++k) d.region[k] = c.region[k];
+    d.globalForce[0] = c.global_force[0]; d.globalForce[1] = c.global_force[1];
+    d.applyRegion = c.apply_region;
+    d.limit = std::min(std::max(c.candidate_limit, 1u), 128u);
+    d.split = std::max(c.substep_split, 1u);
+    d.minimumMarginOnly = c.minimum_margin_only;
+    d.buckets = c.grid_bucket_count == 0 ? auto_buckets(n) : std::min(c.grid_bucket_count, bpd::kMaxBuckets);
+    d.assertEverySubstep = c.assert_every_substep;
+    d.injectionEnabled = c.injection_enabled; d.injectionAfter = c.injection_after_substep;
+    d.injectionObject = c.injection_object;
+    d.injectionPosition[0] = c.injection_position[0]; d.injectionPosition[1] = c.injection_position[1];
+    d.failed = w.failed ? 1u : 0u;
+    d.exportAdjacency = (w.exportAdjacency || w.config.export_adjacency) ? 1u : 0u;
+}
+
+// A world's geometry and force records in their device form (segments, bodies, vertices,
+// triangles, motors and bands sorted by bubble id, fields sorted by id).
+struct GeometryStage {
+    std::vector<float4> segAB, segRegion, bodyPose, bodyMotion, bodyShape, motors, bands;
+    std::vector<uint32_t> segFlags, segCat, segMask;
+    std::vector<int64_t> segId, segOwner;
+    std::vector<float2> bodyPrev, bodyLimits, verts, fieldVerts;
+    std::vector<uint4> bodyRanges, bodyKind, tris;
+    std::vector<uint64_t> bodyId, fieldIds;
+    std::vector<bpd::DevField> fields;
+};
+
+void stage_geometry(const HostWorld &w, GeometryStage &g) {
+    const size_t ns = w.segments.size(), nb = w.bodies.size();
+    g.segAB.resize(ns); g.segRegion.resize(ns); g.segFlags.resize(ns); g.segCat.resize(ns); g.segMask.resize(ns);
+    g.segId.resize(ns); g.segOwner.resize(ns);
+    for (size_t j = 0; j < ns; ++j) {
+        const bp_segment &sg = w.segments[j];
+        g.segAB[j] = make_float4(sg.a[0], sg.a[1], sg.b[0], sg.b[1]);
+        g.segRegion[j] = make_float4(sg.previous_start[0], sg.previous_start[1], sg.outward_sign, sg.allowed_side);
+        g.segFlags[j] = sg.flags; g.segCat[j] = sg.category; g.segMask[j] = sg.mask; g.segId[j] = sg.id; g.segOwner[j] = sg.owner_id;
+    }
+    g.bodyPose.resize(nb); g.bodyPrev.resize(nb); g.bodyMotion.resize(nb); g.bodyShape.resize(nb); g.bodyLimits.resize(nb);
+    g.bodyRanges.resize(nb); g.bodyKind.resize(nb); g.bodyId.resize(nb);
+    for (size_t k = 0; k < nb; ++k) {
+        const bp_body &b = w.bodies[k];
+        g.bodyPose[k] = make_float4(b.pose[0], b.pose[1], b.pose[2], b.previous[2]);
+        g.bodyPrev[k] = make_float2(b.previous[0], b.previous[1]);
+        g.bodyMotion[k] = make_float4(b.velocity[0], b.velocity[1], b.velocity[2], 0.0f);
+        g.bodyShape[k] = make_float4(b.bounding_radius, b.core_radius, b.half_length, 0.0f);
+        g.bodyLimits[k] = make_float2(b.max_linear_speed, b.max_angular_speed);
+        g.bodyRanges[k] = make_uint4(b.vertex_offset, b.vertex_count, b.triangle_offset, b.triangle_count);
+        g.bodyKind[k] = make_uint4(b.shape, b.flags, b.category, b.mask);
+        g.bodyId[k] = b.id;
+    }
+    g.verts.resize(w.vertices.size() / 2);
+    for (size_t v = 0; v < g.verts.size(); ++v) g.verts[v] = make_float2(w.vertices[2 * v], w.vertices[2 * v + 1]);
+    g.tris.resize(w.triangles.size() / 4);
+    for (size_t t = 0; t < g.tris.size(); ++t)
+        g.tris[t] = make_uint4(w.triangles[4 * t], w.triangles[4 * t + 1], w.triangles[4 * t + 2], w.triangles[4 * t + 3]);
+    std::vector<bp_motor> ms = w.motors;
+    std::sort(ms.begin(), ms.end(), [](const bp_motor &a, const bp_motor &b) { return a.bubble_id < b.bubble_id; });
+    g.motors.resize(ms.size());
+    for (size_t k = 0; k < ms.size(); ++k) g.motors[k] = make_float4(ms[k].force[0], ms[k].force[1], ms[k].torque, ms[k].inertia);
+    std::vector<bp_band> bs = w.bands;
+    std::sort(bs.begin(), bs.end(), [](const bp_band &a, const bp_band &b) { return a.bubble_id < b.bubble_id; });
+    g.bands.resize(bs.size());
+    for (size_t k = 0; k < bs.size(); ++k)
+        g.bands[k] = make_float4(bs[k].target[0], bs[k].target[1], bs[k].stiffness, bs[k].maximum_force);
+    // Fields act in ascending id order (NativeForceInput sorts its objects, NativeForceRecords.swift:187).
+    std::vector<bp_field> fs = w.fields;
+    std::stable_sort(fs.begin(), fs.end(), [](const bp_field &a, const bp_field &b) { return a.id < b.id; });
+    g.fields.resize(fs.size());
+    for (size_t k = 0; k < fs.size(); ++k) {
+        const bp_field &f = fs[k];
+        g.fields[k].pose = make_float4(f.position[0], f.position[1], f.rotation, f.falloff);
+        g.fields[k].strength = make_float4(f.direction[0], f.direction[1], f.strength, f.radius);
+        g.fields[k].shape = make_uint4(f.kind, f.vertex_offset, f.vertex_count, 0u);
+        g.fields[k].filter = make_uint4(f.filter_offset, f.filter_count, f.has_filter ? 1u : 0u, 0u);
+    }
+    g.fieldVerts.resize(w.fieldVertices.size() / 2);
+    for (size_t v = 0; v < g.fieldVerts.size(); ++v) g.fieldVerts[v] = make_float2(w.fieldVertices[2 * v], w.fieldVertices[2 * v + 1]);
+    g.fieldIds.resize(w.fieldIds.size());
+    for (size_t k = 0; k < w.fieldIds.size(); ++k) g.fieldIds[k] = (uint64_t)w.fieldIds[k];
+}
+
+template <class T> void place(std::vector<T> &dst, size_t offset, const std::vector<T> &src) {
+    std::copy(src.begin(), src.end(), dst.begin() + offset);
+}
+
+}  // namespace
+
 bp_status bp_batch::pack() {
     // A failed world stays failed across a repack (it is not advanced again until it receives
     // new input): keep the device's sticky flags of the worlds packed last time.
@@ -110,6 +204,19 @@ bp_status bp_batch::pack() {
     uint32_t coopWorlds = 0;
     const uint32_t coopMax = bp_coop_max_groups();
     std::vector<int> worldClass(worlds.size());
+    // Geometry ranges: each world's ranges hold the batch's largest counts up to a small bound, so
+    // that a served world can take over the ranges of an evicted one without a repack.
+    uint32_t gSeg = 0, gBody = 0, gVert = 0, gTri = 0;
+    for (const HostWorld &w : worlds) {
+        gSeg = std::max(gSeg, (uint32_t)std::min<size_t>(w.segments.size(), kReserveSegments));
+        gBody = std::max(gBody, (uint32_t)std::min<size_t>(w.bodies.size(), kReserveBodies));
+        gVert = std::max(gVert, (uint32_t)std::min<size_t>(w.vertices.size() / 2, kReserveVertices));
+        gTri = std::max(gTri, (uint32_t)std::min<size_t>(w.triangles.size() / 4, kReserveTriangles));
+    }
+    auto segRange = [&](const HostWorld &w) { return std::max((uint32_t)w.segments.size(), gSeg); };
+    auto bodyRange = [&](const HostWorld &w) { return std::max((uint32_t)w.bodies.size(), gBody); };
+    auto vertRange = [&](const HostWorld &w) { return std::max((uint32_t)(w.vertices.size() / 2), gVert); };
+    auto triRange = [&](const HostWorld &w) { return std::max((uint32_t)(w.triangles.size() / 4), gTri); };
     coopGroups.assign(worlds.size(), 0);
     for (size_t wi = 0; wi < worlds.size(); ++wi) {
         const HostWorld &w = worlds[wi];
@@ -131,14 +238,15 @@ bp_status bp_batch::pack() {
             const uint32_t block = bp_coop_block();
             coopGroups[wi] = std::max<uint32_t>(1, std::min<uint32_t>(coopMax, ((uint32_t)w.bubbles.size() + block - 1) / block));
         }
-        nb += slots; ns += (uint32_t)w.segments.size(); nbod += (uint32_t)w.bodies.size();
-        nv += (uint32_t)(w.vertices.size() / 2); nt += (uint32_t)(w.triangles.size() / 4);
+        nb += slots; ns += segRange(w); nbod += bodyRange(w);
+        nv += vertRange(w); nt += triRange(w);
         nm += (uint32_t)w.motors.size(); nband += (uint32_t)w.bands.size();
         nf += (uint32_t)w.fields.size(); nfv += (uint32_t)(w.fieldVertices.size() / 2); nfi += (uint32_t)w.fieldIds.size();
         adj += w.resident ? (uint64_t)std::max<uint32_t>(slots, 16) * w.adjPerBubble : 0;
     }
     if (adj > 0xffffffffull) { set_error("adjacency capacity exceeds 2^32 entries"); return BP_ERR_CAPACITY_OVERFLOW; }
     totalBubbles = nb;
+    totalBodies = nbod;
     bpd::Arena &A = arena;
     bp_status s = BP_OK;
 #define BP_ALLOC(field, count) do { if ((s = alloc(field, count)) != BP_OK) return s; } while (0)
@@ -195,6 +303,7 @@ bp_status bp_batch::pack() {
     std::vector<uint64_t> fieldIds(nfi);
     std::vector<uint64_t> bodyId(nbod);
     hostDescs.assign(worlds.size(), bpd::WorldDesc{});
+    regions.assign(worlds.size(), Region{});
     uint32_t ob = 0, os = 0, obd = 0, ov = 0, ot = 0, om_ = 0, oband = 0, of = 0, ofv = 0, ofi = 0, occ = 0, ocs = 0;
     uint64_t oadj = 0;
     for (auto &k : classWorlds) k.clear();
@@ -213,21 +322,7 @@ bp_status bp_batch::pack() {
         d.adjOffset = (uint32_t)oadj; d.adjCapacity = w.resident ? std::max<uint32_t>(w.capacity, 16) * w.adjPerBubble : 0;
         d.historyOffset = (uint32_t)(wi * bpd::kFrameLimit);
         d.pub = 0;
-        const bp_world_config &c = w.config;
-        d.dt = c.time_step; d.contactTol = c.contact_tolerance; d.sepTol = c.separation_tolerance;
-        for (int k = 0; k < 4; ++k) d.region[k] = c.region[k];
-        d.globalForce[0] = c.global_force[0]; d.globalForce[1] = c.global_force[1];
-        d.applyRegion = c.apply_region;
-        d.limit = std::min(std::max(c.candidate_limit, 1u), 128u);
-        d.split = std::max(c.substep_split, 1u);
-        d.minimumMarginOnly = c.minimum_margin_only;
-        d.buckets = c.grid_bucket_count == 0 ? auto_buckets(n) : std::min(c.grid_bucket_count, bpd::kMaxBuckets);
-        d.assertEverySubstep = c.assert_every_substep;
-        d.injectionEnabled = c.injection_enabled; d.injectionAfter = c.injection_after_substep;
-        d.injectionObject = c.injection_object;
-        d.injectionPosition[0] = c.injection_position[0]; d.injectionPosition[1] = c.injection_position[1];
-        d.failed = w.failed ? 1u : 0u;
-        d.exportAdjacency = (w.exportAdjacency || w.config.export_adjacency) ? 1u : 0u;
+        desc_controls(w, n, d);
         d.bubbleCapacity = w.capacity;
         const uint32_t resident = w.resident ? n : 0u;       // bubbles staged in the device range
         for (uint32_t i = 0; i < resident; ++i) {
@@ -241,57 +336,36 @@ bp_status bp_batch::pack() {
             gamma[g] = b.damping; flags[g] = b.flags; cat[g] = b.category; mask[g] = b.mask; ids[g] = b.id;
         }
         // Motor and band records are resolved to bubble indices once (ids are unique and sorted).
-        std::vector<bp_motor> ms = w.motors;
-        std::sort(ms.begin(), ms.end(), [](const bp_motor &a, const bp_motor &b) { return a.bubble_id < b.bubble_id; });
-        for (uint32_t k = 0; k < ms.size(); ++k) {
-            motors[om_ + k] = make_float4(ms[k].force[0], ms[k].force[1], ms[k].torque, ms[k].inertia);
-            auto it = std::lower_bound(w.bubbles.begin(), w.bubbles.end(), ms[k].bubble_id,
+        GeometryStage gs;
+        stage_geometry(w, gs);
+        const std::vector<bp_motor> &msRaw = w.motors;
+        std::vector<int64_t> motorIds(msRaw.size()), bandIds(w.bands.size());
+        for (size_t k = 0; k < msRaw.size(); ++k) motorIds[k] = msRaw[k].bubble_id;
+        for (size_t k = 0; k < w.bands.size(); ++k) bandIds[k] = w.bands[k].bubble_id;
+        std::sort(motorIds.begin(), motorIds.end());
+        std::sort(bandIds.begin(), bandIds.end());
+        for (uint32_t k = 0; k < motorIds.size(); ++k) {
+            auto it = std::lower_bound(w.bubbles.begin(), w.bubbles.end(), motorIds[k],
                                        [](const bp_bubble &b, int64_t id) { return b.id < id; });
-            if (w.resident && it != w.bubbles.end() && it->id == ms[k].bubble_id) motorIdx[ob + (it - w.bubbles.begin())] = (int32_t)k;
+            if (w.resident && it != w.bubbles.end() && it->id == motorIds[k]) motorIdx[ob + (it - w.bubbles.begin())] = (int32_t)k;
         }
-        std::vector<bp_band> bs = w.bands;
-        std::sort(bs.begin(), bs.end(), [](const bp_band &a, const bp_band &b) { return a.bubble_id < b.bubble_id; });
-        for (uint32_t k = 0; k < bs.size(); ++k) {
-            bands[oband + k] = make_float4(bs[k].target[0], bs[k].target[1], bs[k].stiffness, bs[k].maximum_force);
-            auto it = std::lower_bound(w.bubbles.begin(), w.bubbles.end(), bs[k].bubble_id,
+        for (uint32_t k = 0; k < bandIds.size(); ++k) {
+            auto it = std::lower_bound(w.bubbles.begin(), w.bubbles.end(), bandIds[k],
                                        [](const bp_bubble &b, int64_t id) { return b.id < id; });
-            if (w.resident && it != w.bubbles.end() && it->id == bs[k].bubble_id) bandIdx[ob + (it - w.bubbles.begin())] = (int32_t)k;
+            if (w.resident && it != w.bubbles.end() && it->id == bandIds[k]) bandIdx[ob + (it - w.bubbles.begin())] = (int32_t)k;
         }
-        // Fields act in ascending id order (NativeForceInput sorts its objects, NativeForceRecords.swift:187).
-        std::vector<bp_field> fs = w.fields;
-        std::stable_sort(fs.begin(), fs.end(), [](const bp_field &a, const bp_field &b) { return a.id < b.id; });
-        for (uint32_t k = 0; k < fs.size(); ++k) {
-            const bp_field &f = fs[k];
-            fields[of + k].pose = make_float4(f.position[0], f.position[1], f.rotation, f.falloff);
-            fields[of + k].strength = make_float4(f.direction[0], f.direction[1], f.strength, f.radius);
-            fields[of + k].shape = make_uint4(f.kind, f.vertex_offset, f.vertex_count, 0u);
-            fields[of + k].filter = make_uint4(f.filter_offset, f.filter_count, f.has_filter ? 1u : 0u, 0u);
-        }
-        for (size_t v = 0; v < w.fieldVertices.size() / 2; ++v)
-            fieldVerts[ofv + v] = make_float2(w.fieldVertices[2 * v], w.fieldVertices[2 * v + 1]);
-        for (size_t k = 0; k < w.fieldIds.size(); ++k) fieldIds[ofi + k] = (uint64_t)w.fieldIds[k];
-        for (uint32_t j = 0; j < w.segments.size(); ++j) {
-            const bp_segment &sg = w.segments[j];
-            uint32_t g = os + j;
-            segAB[g] = make_float4(sg.a[0], sg.a[1], sg.b[0], sg.b[1]);
-            segRegion[g] = make_float4(sg.previous_start[0], sg.previous_start[1], sg.outward_sign, sg.allowed_side);
-            segFlags[g] = sg.flags; segCat[g] = sg.category; segMask[g] = sg.mask; segId[g] = sg.id; segOwner[g] = sg.owner_id;
-        }
-        for (uint32_t k = 0; k < w.bodies.size(); ++k) {
-            const bp_body &b = w.bodies[k];
-            uint32_t g = obd + k;
-            bodyPose[g] = make_float4(b.pose[0], b.pose[1], b.pose[2], b.previous[2]);
-            bpose2[g] = make_float2(b.previous[0], b.previous[1]);
-            bodyMotion[g] = make_float4(b.velocity[0], b.velocity[1], b.velocity[2], 0.0f);
-            bodyShape[g] = make_float4(b.bounding_radius, b.core_radius, b.half_length, 0.0f);
-            bodyLimits[g] = make_float2(b.max_linear_speed, b.max_angular_speed);
-            bodyRanges[g] = make_uint4(b.vertex_offset, b.vertex_count, b.triangle_offset, b.triangle_count);
-            bodyKind[g] = make_uint4(b.shape, b.flags, b.category, b.mask);
-            bodyId[g] = b.id;
-        }
-        for (size_t v = 0; v < w.vertices.size() / 2; ++v) verts[ov + v] = make_float2(w.vertices[2 * v], w.vertices[2 * v + 1]);
-        for (size_t t = 0; t < w.triangles.size() / 4; ++t)
-            tris[ot + t] = make_uint4(w.triangles[4 * t], w.triangles[4 * t + 1], w.triangles[4 * t + 2], w.triangles[4 * t + 3]);
+        place(motors, om_, gs.motors); place(bands, oband, gs.bands);
+        place(fields, of, gs.fields); place(fieldVerts, ofv, gs.fieldVerts); place(fieldIds, ofi, gs.fieldIds);
+        place(segAB, os, gs.segAB); place(segRegion, os, gs.segRegion); place(segFlags, os, gs.segFlags);
+        place(segCat, os, gs.segCat); place(segMask, os, gs.segMask); place(segId, os, gs.segId); place(segOwner, os, gs.segOwner);
+        place(bodyPose, obd, gs.bodyPose); place(bpose2, obd, gs.bodyPrev); place(bodyMotion, obd, gs.bodyMotion);
+        place(bodyShape, obd, gs.bodyShape); place(bodyLimits, obd, gs.bodyLimits); place(bodyRanges, obd, gs.bodyRanges);
+        place(bodyKind, obd, gs.bodyKind); place(bodyId, obd, gs.bodyId);
+        place(verts, ov, gs.verts); place(tris, ot, gs.tris);
+        regions[wi] = Region{w.capacity, segRange(w), bodyRange(w), vertRange(w), triRange(w),
+                             (uint32_t)gs.motors.size(), (uint32_t)gs.bands.size(),
+                             (uint32_t)gs.fields.size(), (uint32_t)gs.fieldVerts.size(), (uint32_t)gs.fieldIds.size(),
+                             d.adjCapacity};
         const int cls = worldClass[wi];
         if (cls >= 0) classWorlds[cls].push_back((uint32_t)wi);
         if (cls == kCoopClass) {
@@ -299,8 +373,8 @@ bp_status bp_batch::pack() {
             d.coopCellOffset = occ; occ += d.coopBuckets;
             d.coopScratchOffset = ocs; ocs += bpd::kCoopScratchWords;
         }
-        ob += w.capacity; os += (uint32_t)w.segments.size(); obd += (uint32_t)w.bodies.size();
-        ov += (uint32_t)(w.vertices.size() / 2); ot += (uint32_t)(w.triangles.size() / 4);
+        ob += w.capacity; os += segRange(w); obd += bodyRange(w);
+        ov += vertRange(w); ot += triRange(w);
         om_ += (uint32_t)w.motors.size(); oband += (uint32_t)w.bands.size();
         of += (uint32_t)w.fields.size(); ofv += (uint32_t)(w.fieldVertices.size() / 2); ofi += (uint32_t)w.fieldIds.size();
         oadj += (uint64_t)d.adjCapacity;
@@ -394,8 +468,7 @@ bp_status bp_batch::download_state(bool full) {
     }
     std::vector<float2> pos[2], start[2], vel[2];
     std::vector<float> rot[2], om[2];
-    size_t nbod = 0;
-    for (auto &w : worlds) nbod += w.bodies.size();
+    const size_t nbod = totalBodies;
     std::vector<float4> bodyPose[2];
     std::vector<float2> bodyPrev[2];
     std::vector<float4> bodyMotion(nbod);
@@ -605,6 +678,145 @@ uint64_t bp_batch::footprint(uint32_t wi) const {
     const uint64_t geometry = w.segments.size() * 64 + w.bodies.size() * 160 + w.vertices.size() * sizeof(float) +
                               w.triangles.size() * sizeof(uint32_t) + (w.motors.size() + w.bands.size()) * sizeof(float4);
     return slots * perSlot + adjacency + geometry + bpd::kFrameLimit * sizeof(float) + sizeof(bpd::WorldDesc);
+}
+
+bool bp_batch::fits_ranges_of(uint32_t out, uint32_t in) const {
+    if (!packed || out >= worlds.size() || in >= worlds.size() || out == in || regions.size() != worlds.size()) return false;
+    const HostWorld &o = worlds[out], &w = worlds[in];
+    // A world moved in gets no cooperative storage (it runs the one-workgroup kernels, as served
+    // worlds do); one that asks for the cooperative kernel waits for a repack.
+    if (!o.resident || w.resident || o.packedClass < 0 || w.config.force_cooperative) return false;
+    const Region &r = regions[out];
+    const uint32_t n = (uint32_t)w.bubbles.size();
+    return n + std::max<uint32_t>(8, n / 8) <= r.bubbles &&                     // the room to grow pack() gives
+           (uint64_t)std::max<uint32_t>(r.bubbles, 16) * w.adjPerBubble <= r.adjacency &&
+           w.segments.size() <= r.segments && w.bodies.size() <= r.bodies && w.vertices.size() / 2 <= r.vertices &&
+           w.triangles.size() / 4 <= r.triangles && w.motors.size() <= r.motors && w.bands.size() <= r.bands &&
+           w.fields.size() <= r.fields && w.fieldVertices.size() / 2 <= r.fieldVertices && w.fieldIds.size() <= r.fieldIds;
+}
+
+bp_status bp_batch::swap_residency(uint32_t out, uint32_t in, hipStream_t stream) {
+    if (!fits_ranges_of(out, in)) { set_error("world does not fit the ranges of the world it would replace"); return BP_ERR_CAPACITY_OVERFLOW; }
+    HostWorld &o = worlds[out], &w = worlds[in];
+    const Region r = regions[out];
+    // 1. The published state of `out` into its host copy (transactions may have changed its list).
+    bpd::WorldDesc d;
+    BP_HIP(hipMemcpyAsync(&d, descs + out, sizeof d, hipMemcpyDeviceToHost, stream));
+    BP_HIP(hipStreamSynchronize(stream));
+    if (d.failed == bpd::kWorldHeld) { set_error("world holds region-removed bubbles"); return BP_ERR_UNAVAILABLE; }
+    const uint32_t n = std::min(d.bubbleCount, r.bubbles), p = d.pub;
+    const uint32_t m = (uint32_t)w.bubbles.size();
+    if (std::max(r.bubbles, m) > xferCapacity) { set_error("world larger than the transfer buffer"); return BP_ERR_CAPACITY_OVERFLOW; }
+    if (pinnedCapacity < xferCapacity) {
+        if (pinnedRecs) (void)hipHostFree(pinnedRecs);
+        if (pinnedIdx) (void)hipHostFree(pinnedIdx);
+        pinnedRecs = nullptr; pinnedIdx = nullptr; pinnedCapacity = 0;
+        BP_HIP(hipHostMalloc((void **)&pinnedRecs, (size_t)xferCapacity * sizeof(bp_bubble), hipHostMallocDefault));
+        BP_HIP(hipHostMalloc((void **)&pinnedIdx, 2 * (size_t)xferCapacity * sizeof(int32_t), hipHostMallocDefault));
+        pinnedCapacity = xferCapacity;
+    }
+    {
+        const size_t nb = o.bodies.size();
+        std::vector<float4> bodyPose(nb), bodyMotion(nb);
+        std::vector<float2> bodyPrev(nb);
+        BP_HIP(bp_launch_gather_world_full(arenaDev, descs + out, xferBubbles, r.bubbles, stream));
+        if (n) BP_HIP(hipMemcpyAsync(pinnedRecs, xferBubbles, n * sizeof(bp_bubble), hipMemcpyDeviceToHost, stream));
+        if (nb) {
+            BP_HIP(hipMemcpyAsync(bodyPose.data(), arena.bodyPose[p] + d.bodyOffset, nb * sizeof(float4), hipMemcpyDeviceToHost, stream));
+            BP_HIP(hipMemcpyAsync(bodyPrev.data(), arena.bodyPrev[p] + d.bodyOffset, nb * sizeof(float2), hipMemcpyDeviceToHost, stream));
+            BP_HIP(hipMemcpyAsync(bodyMotion.data(), arena.bodyMotion + d.bodyOffset, nb * sizeof(float4), hipMemcpyDeviceToHost, stream));
+        }
+        BP_HIP(hipStreamSynchronize(stream));
+        o.bubbles.assign(pinnedRecs, pinnedRecs + n);
+        for (size_t k = 0; k < nb; ++k) {
+            bp_body &b = o.bodies[k];
+            b.pose[0] = bodyPose[k].x; b.pose[1] = bodyPose[k].y; b.pose[2] = bodyPose[k].z;
+            b.previous[0] = bodyPrev[k].x; b.previous[1] = bodyPrev[k].y; b.previous[2] = bodyPose[k].w;
+            b.velocity[0] = bodyMotion[k].x; b.velocity[1] = bodyMotion[k].y; b.velocity[2] = bodyMotion[k].z;
+        }
+        o.failed = d.failed == bpd::kWorldFailed;
+    }
+    // 2. `in` into the ranges, as pack() would write a world there (published side 0).
+    bpd::WorldDesc dn = d;
+    dn.bubbleCount = m;
+    dn.segmentCount = (uint32_t)w.segments.size(); dn.bodyCount = (uint32_t)w.bodies.size();
+    dn.vertexCount = (uint32_t)(w.vertices.size() / 2); dn.triangleCount = (uint32_t)(w.triangles.size() / 4);
+    dn.motorCount = (uint32_t)w.motors.size(); dn.bandCount = (uint32_t)w.bands.size(); dn.fieldCount = (uint32_t)w.fields.size();
+    dn.coopCellOffset = dn.coopBuckets = dn.coopScratchOffset = 0;
+    dn.historyOffset = in * bpd::kFrameLimit;
+    dn.pub = 0;
+    dn.bubbleCapacity = r.bubbles;
+    dn.adjCapacity = r.adjacency;
+    dn.mirror = nullptr; dn.bodyMirror = nullptr; dn.mirrorIdx = nullptr; dn.mirrorCapacity = 0;
+    desc_controls(w, m, dn);
+    GeometryStage gs;
+    stage_geometry(w, gs);
+    // The geometry goes up from one pinned blob (asynchronous copies, one wait at the end).
+    size_t blobBytes = 0;
+    auto bytes_of = [&](const auto &src) { blobBytes += (src.size() * sizeof(src[0]) + 15) & ~(size_t)15; };
+    bytes_of(gs.segAB); bytes_of(gs.segRegion); bytes_of(gs.segFlags); bytes_of(gs.segCat); bytes_of(gs.segMask);
+    bytes_of(gs.segId); bytes_of(gs.segOwner); bytes_of(gs.bodyPose); bytes_of(gs.bodyPrev); bytes_of(gs.bodyMotion);
+    bytes_of(gs.bodyShape); bytes_of(gs.bodyLimits); bytes_of(gs.bodyRanges); bytes_of(gs.bodyKind); bytes_of(gs.bodyId);
+    bytes_of(gs.verts); bytes_of(gs.tris); bytes_of(gs.motors); bytes_of(gs.bands); bytes_of(gs.fields);
+    bytes_of(gs.fieldVerts); bytes_of(gs.fieldIds);
+    if (pinnedBlobBytes < blobBytes) {
+        if (pinnedBlob) (void)hipHostFree(pinnedBlob);
+        pinnedBlob = nullptr; pinnedBlobBytes = 0;
+        BP_HIP(hipHostMalloc(&pinnedBlob, blobBytes, hipHostMallocDefault));
+        pinnedBlobBytes = blobBytes;
+    }
+    size_t blobAt = 0;
+    auto put = [&](auto *dst, const auto &src) -> hipError_t {
+        if (src.empty()) return hipSuccess;
+        const size_t size = src.size() * sizeof(src[0]);
+        char *at = static_cast<char *>(pinnedBlob) + blobAt;
+        std::memcpy(at, src.data(), size);
+        blobAt += (size + 15) & ~(size_t)15;
+        return hipMemcpyAsync(dst, at, size, hipMemcpyHostToDevice, stream);
+    };
+    if (m) {
+        std::memcpy(pinnedRecs, w.bubbles.data(), m * sizeof(bp_bubble));
+        motor_band_indices(in, w.bubbles.data(), m, pinnedIdx);
+        BP_HIP(hipMemcpyAsync(xferBubbles, pinnedRecs, m * sizeof(bp_bubble), hipMemcpyHostToDevice, stream));
+        BP_HIP(hipMemcpyAsync(xferIndices, pinnedIdx, 2 * (size_t)m * sizeof(int32_t), hipMemcpyHostToDevice, stream));
+        BP_HIP(bp_launch_scatter_world(arenaDev, dn.bubbleOffset, m, xferBubbles, xferIndices, xferIndices + m, stream));
+        BP_HIP(hipMemsetAsync(arena.acc + dn.bubbleOffset, 0, m * sizeof(float2), stream));
+    }
+    BP_HIP(put(arena.segAB + dn.segmentOffset, gs.segAB)); BP_HIP(put(arena.segRegion + dn.segmentOffset, gs.segRegion));
+    BP_HIP(put(arena.segFlags + dn.segmentOffset, gs.segFlags)); BP_HIP(put(arena.segCat + dn.segmentOffset, gs.segCat));
+    BP_HIP(put(arena.segMask + dn.segmentOffset, gs.segMask)); BP_HIP(put(arena.segId + dn.segmentOffset, gs.segId));
+    BP_HIP(put(arena.segOwner + dn.segmentOffset, gs.segOwner));
+    BP_HIP(put(arena.bodyPose[0] + dn.bodyOffset, gs.bodyPose)); BP_HIP(put(arena.bodyPrev[0] + dn.bodyOffset, gs.bodyPrev));
+    BP_HIP(put(arena.bodyMotion + dn.bodyOffset, gs.bodyMotion)); BP_HIP(put(arena.bodyShape + dn.bodyOffset, gs.bodyShape));
+    BP_HIP(put(arena.bodyLimits + dn.bodyOffset, gs.bodyLimits)); BP_HIP(put(arena.bodyRanges + dn.bodyOffset, gs.bodyRanges));
+    BP_HIP(put(arena.bodyKind + dn.bodyOffset, gs.bodyKind)); BP_HIP(put(arena.bodyId + dn.bodyOffset, gs.bodyId));
+    BP_HIP(put(arena.verts + dn.vertexOffset, gs.verts)); BP_HIP(put(arena.tris + dn.triangleOffset, gs.tris));
+    BP_HIP(put(arena.motors + dn.motorOffset, gs.motors)); BP_HIP(put(arena.bands + dn.bandOffset, gs.bands));
+    BP_HIP(put(arena.fields + dn.fieldOffset, gs.fields)); BP_HIP(put(arena.fieldVerts + dn.fieldVertexOffset, gs.fieldVerts));
+    BP_HIP(put(arena.fieldIds + dn.fieldIdOffset, gs.fieldIds));
+    bpd::WorldDesc dOut = d;                         // the device's record: its failure flag is current
+    dOut.bubbleCount = 0;
+    dOut.mirror = nullptr; dOut.bodyMirror = nullptr; dOut.mirrorIdx = nullptr; dOut.mirrorCapacity = 0;
+    BP_HIP(hipMemcpyAsync(descs + in, &dn, sizeof dn, hipMemcpyHostToDevice, stream));
+    BP_HIP(hipMemcpyAsync(descs + out, &dOut, sizeof dOut, hipMemcpyHostToDevice, stream));
+    BP_HIP(hipStreamSynchronize(stream));
+    // 3. Bookkeeping: residency, ranges, kernel classes (the batch's own class lists), graphs.
+    hostDescs[in] = dn;
+    hostDescs[out] = dOut;
+    regions[in] = r;
+    regions[out] = Region{};
+    const int oldClass = o.packedClass, newClass = default_class_for(w, false);
+    w.resident = true; w.capacity = r.bubbles; w.packedClass = newClass;
+    o.resident = false; o.capacity = 0; o.packedClass = -1;
+    auto &from = classWorlds[oldClass];
+    from.erase(std::remove(from.begin(), from.end(), out), from.end());
+    classWorlds[newClass].push_back(in);
+    if (workEstimate.size() == worlds.size()) workEstimate[in] = m;
+    if (coopGroups.size() == worlds.size()) { coopGroups[in] = 0; coopGroups[out] = 0; }
+    orderDirty = true;                               // the class lists go up before the next batch launch
+    for (auto &g : poolGroups) g = 0;
+    release_graph();
+    return BP_OK;
 }
 
 // Queues `steps` steps of the listed worlds (every resident world when `subset` is null) on the

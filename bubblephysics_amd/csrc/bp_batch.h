@@ -27,6 +27,8 @@ extern "C" void bp_lds_config(int id, uint32_t *threads, uint32_t *per, uint32_t
 extern "C" uint32_t bp_coop_max_groups();
 extern "C" hipError_t bp_launch_gather_world(const bpd::Arena *arena, const bpd::WorldDesc *desc, bp_bubble *out, uint32_t count,
                                              hipStream_t stream);
+extern "C" hipError_t bp_launch_gather_world_full(const bpd::Arena *arena, const bpd::WorldDesc *desc, bp_bubble *out,
+                                                  uint32_t count, hipStream_t stream);
 extern "C" hipError_t bp_launch_scatter_world(const bpd::Arena *arena, uint32_t offset, uint32_t count, const bp_bubble *in,
                                               const int32_t *motorIdx, const int32_t *bandIdx, hipStream_t stream);
 extern "C" uint32_t bp_coop_block();
@@ -69,6 +71,9 @@ constexpr int kClassCount = 12;              // ten LDS configurations, the glob
 constexpr int kGlobalClass = 10;
 constexpr int kCoopClass = 11;
 constexpr uint32_t kLdsLargest = 4096;       // bubbles of the widest LDS configuration
+// Geometry every world's device ranges hold at least (when the batch has worlds that large), so
+// that a served world can move into an evicted world's ranges.
+constexpr uint32_t kReserveSegments = 64, kReserveBodies = 32, kReserveVertices = 256, kReserveTriangles = 256;
 constexpr int kStreams = kClassCount;          // one stream per kernel class: classes never queue behind each other
 
 inline uint32_t class_threads(int k) {
@@ -161,6 +166,12 @@ struct bp_batch {
     bpd::Arena *arenaDev = nullptr;
     bpd::WorldDesc *descs = nullptr;
     std::vector<bpd::WorldDesc> hostDescs;
+    // The lengths of each world's device ranges (set by pack; a world moved into another world's
+    // ranges by swap_residency takes them over), which bound what a world moved in may hold.
+    struct Region {
+        uint32_t bubbles, segments, bodies, vertices, triangles, motors, bands, fields, fieldVertices, fieldIds, adjacency;
+    };
+    std::vector<Region> regions;
     uint32_t *orders[bph::kClassCount] = {};
     std::vector<uint32_t> classWorlds[bph::kClassCount];
     std::vector<uint32_t> coopGroups;   // per world: workgroups of its cooperative launch (0 = not cooperative)
@@ -199,8 +210,14 @@ struct bp_batch {
     bp_bubble *xferBubbles = nullptr;  // one world's records in transit (transactions)
     int32_t *xferIndices = nullptr;    // their motor and band indices
     uint32_t xferCapacity = 0;
+    bp_bubble *pinnedRecs = nullptr;   // pinned host staging of one world's records (swap_residency)
+    int32_t *pinnedIdx = nullptr;
+    uint32_t pinnedCapacity = 0;
+    void *pinnedBlob = nullptr;        // pinned host staging of one world's geometry
+    size_t pinnedBlobBytes = 0;
     uint32_t launchOffset[bph::kClassCount] = {}, launchCount[bph::kClassCount] = {};
     uint32_t totalBubbles = 0;
+    uint32_t totalBodies = 0;          // body records of the arena (every world's body range)
     std::mutex mutex;
     // Captured step chain (bp_batch_capture): the launches of `graphSteps` steps of every world.
     // One graph per kernel class, each replayed on the class's own stream (so the classes run
@@ -223,6 +240,9 @@ struct bp_batch {
     ~bp_batch() {
         release();
         destroy_streams();
+        if (pinnedRecs) (void)hipHostFree(pinnedRecs);
+        if (pinnedIdx) (void)hipHostFree(pinnedIdx);
+        if (pinnedBlob) (void)hipHostFree(pinnedBlob);
     }
 
     void release_graph() {
@@ -306,6 +326,11 @@ struct bp_batch {
     void written_desc(uint32_t wi, uint32_t count, bpd::WorldDesc &d) const;
     bp_status adopt_written(uint32_t wi, const bp_bubble *bubbles, uint32_t count, const bpd::WorldDesc &d);
     uint64_t footprint(uint32_t wi) const;
+    // Serving residency without a repack: whether the held world `in` fits the device ranges of
+    // the resident world `out`, and the exchange (out's published state is read back into its
+    // host copy, in's host copy is written into the ranges). Both worlds must be idle.
+    bool fits_ranges_of(uint32_t out, uint32_t in) const;
+    bp_status swap_residency(uint32_t out, uint32_t in, hipStream_t stream);
     bp_status launch(const uint32_t *subset, uint32_t subsetCount, uint32_t steps);
     void plan_pools();
     bp_status refresh_order();

@@ -14,9 +14,12 @@
 // A world takes part in a new launch only once its previous launch has completed (its event),
 // so a step always reads the state that the previous step published.
 //
-// Residency: worlds are admitted against a device-memory budget in index order; a request for
-// a world held in host memory waits until the launches in flight drain, then the least recently
-// used idle worlds are evicted to make room and the batch is repacked.
+// Residency: worlds are admitted against a device-memory budget in index order. A request for a
+// world held in host memory takes the device ranges of an idle world whose ranges hold it (the
+// one with the fewest bubble slots, least recently used among equals; its state is read back to
+// host memory first), with no other world moved and nothing drained; when no such world exists,
+// the request waits until the launches in flight drain, then the least recently used idle worlds
+// are evicted and the batch is repacked.
 //
 // The reference has no counterpart: its game loop steps one world per call
 // (BubbleWorld.submit/advance, Sources/BubblePhysicsPublic/BubbleWorld.swift:252-283).
@@ -177,6 +180,7 @@ struct bp_server {
     bool form_launches();
     bp_status start_launch(int cls);
     bp_status admit_waiting();
+    bp_status admit_in_place();
     int class_for(uint32_t w, uint32_t n) const;
     uint32_t buckets_for(uint32_t w, uint32_t n) const;
     void fail(bp_status s);
@@ -885,9 +889,20 @@ bp_status bp_server::admit_waiting() {
     bp_status s = b.download_state(true);
     if (s != BP_OK) return s;
     std::vector<uint8_t> admitted(W, 0);
+    std::vector<uint32_t> deferred;                  // waiting worlds left for the next round
+    uint32_t admittedCount = 0;
     for (uint32_t w : admissions) {
         if (resident[w]) continue;
+        if (!deferred.empty()) { deferred.push_back(w); continue; }
         const uint64_t need = b.footprint(w);
+        // Whether evicting every resident world not admitted in this round makes room.
+        uint64_t freeable = 0;
+        for (uint32_t v = 0; v < W; ++v) if (resident[v] && !admitted[v]) freeable += footprintOf[v];
+        if (cfg.memory_budget && used - freeable + need > cfg.memory_budget) {
+            if (admittedCount == 0) { set_error("memory budget too small for the waiting worlds"); return BP_ERR_ALLOCATION_FAILED; }
+            deferred.push_back(w);                    // the worlds admitted now go first
+            continue;
+        }
         while (cfg.memory_budget && used + need > cfg.memory_budget) {
             // Least recently used resident world with no waiting request and not admitted now.
             int64_t victim = -1;
@@ -910,6 +925,7 @@ bp_status bp_server::admit_waiting() {
         }
         resident[w] = 1;
         admitted[w] = 1;
+        ++admittedCount;
         b.worlds[w].resident = true;
         footprintOf[w] = need;
         used += need;
@@ -930,6 +946,7 @@ bp_status bp_server::admit_waiting() {
     std::vector<uint32_t> waiting;
     waiting.swap(admissions);
     for (uint32_t w : waiting) wantAdmission[w] = 0;
+    for (uint32_t w : deferred) { wantAdmission[w] = 1; admissions.push_back(w); }
     // Ready lists hold class choices made with the old layout; rebuild them.
     for (auto &rq : ready) rq.clear();
     for (uint32_t w = 0; w < W; ++w) {
@@ -939,11 +956,84 @@ bp_status bp_server::admit_waiting() {
     return BP_OK;
 }
 
+// Admits waiting worlds one at a time into the device ranges of evicted worlds: for each, the
+// least recently used resident world that is idle (no launch, request, staging or pending
+// removal) and whose ranges hold it, within the budget. Worlds for which none qualifies stay
+// in the admission list for the repacking round.
+bp_status bp_server::admit_in_place() {
+    bp_batch &b = *batch;
+    std::vector<uint32_t> left;
+    for (uint32_t w : admissions) {
+        if (resident[w]) { wantAdmission[w] = 0; continue; }
+        // The world takes over the victim's device memory, so the device footprint does not grow.
+        // Among the idle worlds whose ranges hold it, the one with the fewest bubble slots (so that
+        // large ranges stay for large worlds), the least recently used of those.
+        int64_t victim = -1;
+        for (uint32_t v = 0; v < W; ++v) {
+            if (!resident[v] || busy[v] || queued[v] || staging[v] || dropPending[v] || !queue[v].empty()) continue;
+            if (victim >= 0) {
+                const uint32_t sv = b.regions[v].bubbles, sb = b.regions[victim].bubbles;
+                if (sv > sb || (sv == sb && lastUse[v] >= lastUse[victim])) continue;
+            }
+            if (b.fits_ranges_of(v, w)) victim = v;
+        }
+        if (victim < 0) {
+            left.push_back(w);
+            continue;
+        }
+        const uint32_t v = (uint32_t)victim;
+        bp_status s = b.swap_residency(v, w, readStream);
+        if (s == BP_ERR_UNAVAILABLE || s == BP_ERR_CAPACITY_OVERFLOW) { left.push_back(w); continue; }
+        if (s != BP_OK) return s;
+        resident[v] = 0;
+        used -= footprintOf[v];
+        footprintOf[v] = b.footprint(v);
+        mirrorAttached[v] = 0;
+        mirrorValid[v] = 0;
+        count[v] = (uint32_t)b.worlds[v].bubbles.size();
+        ++local.evictions;
+        resident[w] = 1;
+        footprintOf[w] = b.footprint(w);
+        used += footprintOf[w];                          // at most the victim's: its ranges hold the world
+        count[w] = (uint32_t)b.worlds[w].bubbles.size();
+        dropPending[w] = 0;
+        wantAdmission[w] = 0;
+        ++local.admissions;
+        // The world's mirror, filled from its host copy (the state just written to the device).
+        bpd::WorldDesc &d = b.hostDescs[w];
+        const HostWorld &hw = b.worlds[w];
+        const uint32_t n = (uint32_t)hw.bubbles.size();
+        if (n <= mirrorCap[w] && n == d.bubbleCount) {
+            d.mirror = mirrorDev + mirrorOffset[w];
+            d.bodyMirror = bodyMirrorDev + bodyMirrorOffset[w];
+            d.mirrorIdx = mirrorIdxDev + 2 * mirrorOffset[w];
+            d.mirrorCapacity = mirrorCap[w];
+            if (n) std::memcpy(mirrorHost + mirrorOffset[w], hw.bubbles.data(), n * sizeof(bp_bubble));
+            for (size_t k = 0; k < hw.bodies.size(); ++k) {
+                const bp_body &bd = hw.bodies[k];
+                bodyMirrorHost[bodyMirrorOffset[w] + k] = make_float4(bd.pose[0], bd.pose[1], bd.pose[2], bd.pose[2]);
+            }
+            BP_HIP(hipMemcpyAsync(b.descs + w, &d, sizeof d, hipMemcpyHostToDevice, readStream));
+            BP_HIP(hipStreamSynchronize(readStream));
+            mirrorAttached[w] = 1;
+            mirrorValid[w] = 1;
+        }
+        prepare_head(w);
+    }
+    admissions.swap(left);
+    local.resident_bytes = used;
+    return BP_OK;
+}
+
 // Launches the ready worlds. While worlds wait for admission, no new launches start (the repack
 // needs an idle device) except right after an admission round: then every world that is ready
 // is launched once before the next round, so each round makes progress even when the budget
 // forces the next admission to evict a world that still has requests.
 bool bp_server::form_launches() {
+    if (!admissions.empty()) {
+        bp_status s = admit_in_place();
+        if (s != BP_OK) { fail(s); return false; }
+    }
     if (!admissions.empty() && !launchAfterAdmission) {
         if (!inflight.empty() || !activeStage.empty()) return false;   // drain first: the repack moves every world
         bp_status s = admit_waiting();

@@ -5,8 +5,10 @@
 
 namespace bpd {
 
-// State of the published side of one world into bp_bubble records (dynamic fields only; the
-// host keeps the properties it wrote).
+// State of the published side of one world into bp_bubble records: the dynamic fields only (the
+// host keeps the properties it wrote), or, with FULL, every field (a world leaving the device,
+// whose properties transactions may have changed).
+template <bool FULL>
 __global__ void gather_world_kernel(const Arena *__restrict__ A, const WorldDesc *d, bp_bubble *out) {
     const uint32_t n = d->bubbleCount, o = d->bubbleOffset, p = d->pub;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -18,6 +20,11 @@ __global__ void gather_world_kernel(const Arena *__restrict__ A, const WorldDesc
         b.velocity[0] = v.x; b.velocity[1] = v.y;
         b.rotation = A->rot[p][g];
         b.angular_velocity = A->omega[p][g];
+        if (FULL) {
+            b.id = A->id[g];
+            b.mass = A->mass[g]; b.inverse_mass = A->invm[g]; b.radius = A->radius[g]; b.contact_stiffness = A->Kc[g];
+            b.damping = A->gamma[g]; b.flags = A->flags[g]; b.category = A->cat[g]; b.mask = A->mask[g];
+        }
     }
 }
 
@@ -49,7 +56,18 @@ extern "C" __attribute__((visibility("hidden"))) hipError_t bp_launch_gather_wor
                                                                                bp_bubble *out, uint32_t count, hipStream_t stream) {
     if (count == 0) return hipSuccess;
     const uint32_t blocks = (count + 255) / 256;
-    hipLaunchKernelGGL(bpd::gather_world_kernel, dim3(blocks), dim3(256), 0, stream, arena, desc, out);
+    hipLaunchKernelGGL(bpd::gather_world_kernel<false>, dim3(blocks), dim3(256), 0, stream, arena, desc, out);
+    return hipGetLastError();
+}
+
+// Every field of the world's records (count: at least the world's bubble count on the device, the
+// grid's size; `out` holds that many).
+extern "C" __attribute__((visibility("hidden"))) hipError_t bp_launch_gather_world_full(const bpd::Arena *arena,
+                                                                                    const bpd::WorldDesc *desc, bp_bubble *out,
+                                                                                    uint32_t count, hipStream_t stream) {
+    if (count == 0) return hipSuccess;
+    const uint32_t blocks = (count + 255) / 256;
+    hipLaunchKernelGGL(bpd::gather_world_kernel<true>, dim3(blocks), dim3(256), 0, stream, arena, desc, out);
     return hipGetLastError();
 }
 

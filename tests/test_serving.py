@@ -260,11 +260,16 @@ def test_transaction_beyond_the_packed_slots_is_rejected():
 
 
 @pytest.mark.gpu
-def test_each_served_world_gets_the_result_it_gets_alone():
+@pytest.mark.parametrize("sizes,in_place", [([30, 200, 60, 700, 45, 120, 2500], False),
+                                            ([150, 140, 160, 150, 145, 155, 150, 140], True)],
+                         ids=["uneven-repacks", "even-in-place"])
+def test_each_served_world_gets_the_result_it_gets_alone(sizes, in_place):
     # Continuous batching with a budget that forces evictions: every world ends in the same
     # state as when its own requests (and transactions) are replayed on it alone, with the
-    # transactions applied on the host by their statement in numpy.
-    ws = worlds(25, [30, 200, 60, 700, 45, 120, 2500])
+    # transactions applied on the host by their statement in numpy. Uneven worlds need repacking
+    # rounds (the large world fits no evicted world's ranges); worlds of about one size move into
+    # the ranges of the worlds they evict, with no repack.
+    ws = worlds(25, sizes)
     requests = serving.poisson_requests(len(ws), 600.0, 0.4, seed=9, transaction_fraction=0.3)
     probe = solver.Batch(ws)
     budget = sum(sorted(probe.footprint(w) for w in range(len(ws)))[-3:])   # about three worlds at a time
@@ -272,7 +277,11 @@ def test_each_served_world_gets_the_result_it_gets_alone():
     server = serving.Server(ws, memory_budget=budget)
     stats = server.run(requests)
     counters = server.stats()
-    assert counters["completed"] == len(requests) and counters["evictions"] > 0 and counters["repacks"] > 0
+    assert counters["completed"] == len(requests) and counters["evictions"] > 0
+    if in_place:
+        assert counters["evictions"] > counters["repacks"], counters
+    else:
+        assert counters["repacks"] > 0
     final = []
     for w in range(len(ws)):
         final.append(server.read_bubbles(w))
