@@ -69,6 +69,8 @@ BPD int prefix_length(const uint64_t *keys, uint32_t count, int64_t i, int64_t j
 // from the longest common prefix of its key range).
 __global__ __launch_bounds__(kBvhThreads) void body_bvh_kernel(const Arena *__restrict__ A, WorldDesc *descs, const uint32_t *order) {
     __shared__ uint64_t keys[kBvhMaxBodies];
+    __shared__ uint16_t parentNode[kBvhMaxBodies], parentLeaf[kBvhMaxBodies];
+    __shared__ uint32_t arrivals[kBvhMaxBodies];
     __shared__ Red<kBvhThreads> red;
     __shared__ uint32_t flag;
     WorldDesc *d = descs + order[blockIdx.x];
@@ -149,15 +151,39 @@ __global__ __launch_bounds__(kBvhThreads) void body_bvh_kernel(const Arena *__re
         const uint32_t left = first == split ? (kBvhLeaf | (uint32_t)(keys[split] & 0xffffffffu)) : (uint32_t)split;
         const uint32_t right = last == split + 1 ? (kBvhLeaf | (uint32_t)(keys[split + 1] & 0xffffffffu)) : (uint32_t)(split + 1);
         A->bvhChild[base + i] = make_uint2(left, right);
-        // The node's box is the union of its leaves' boxes (exact min and max).
-        // A leaf with a non-finite box overlaps every query, so its ancestors must as well.
-        float4 box = make_float4(inff(), inff(), -inff(), -inff());
-        for (int64_t q = first; q <= last; ++q) {
-            const float4 b = A->bvhBox[base + nb + (uint32_t)(keys[q] & 0xffffffffu)];
-            if (isnan(b.x) || isnan(b.y) || isnan(b.z) || isnan(b.w)) { box = make_float4(-inff(), -inff(), inff(), inff()); break; }
-            box = make_float4(fminf(box.x, b.x), fminf(box.y, b.y), fmaxf(box.z, b.z), fmaxf(box.w, b.w));
+        if (left & kBvhLeaf) parentLeaf[left & ~kBvhLeaf] = (uint16_t)i;
+        else parentNode[left] = (uint16_t)i;
+        if (right & kBvhLeaf) parentLeaf[right & ~kBvhLeaf] = (uint16_t)i;
+        else parentNode[right] = (uint16_t)i;
+        arrivals[i] = 0u;
+    }
+    __syncthreads();
+    // Node boxes bottom-up: each leaf climbs towards the root; at every node the first child to
+    // arrive stops and the second writes the union of both children's boxes (exact min and max;
+    // a leaf with a non-finite box makes every box above it the whole plane, so it overlaps every
+    // query), so each box is written once, after both of its children.
+    for (uint32_t k = threadIdx.x; k < nb; k += kBvhThreads) {
+        uint32_t node = parentLeaf[k];
+        for (;;) {
+            __threadfence();                                  // this lane's box before its arrival
+            if (atomicAdd(&arrivals[node], 1u) == 0u) break;
+            __threadfence();                                  // the sibling's box after its arrival
+            const uint2 ch = A->bvhChild[base + node];
+            float4 box = make_float4(inff(), inff(), -inff(), -inff());
+            const uint32_t kids[2] = {ch.x, ch.y};
+            for (int c = 0; c < 2; ++c) {
+                const bool leaf = (kids[c] & kBvhLeaf) != 0u;
+                const float4 b = leaf ? A->bvhBox[base + nb + (kids[c] & ~kBvhLeaf)] : A->bvhBox[base + kids[c]];
+                if (leaf && (isnan(b.x) || isnan(b.y) || isnan(b.z) || isnan(b.w))) {
+                    box = make_float4(-inff(), -inff(), inff(), inff());
+                    break;
+                }
+                box = make_float4(fminf(box.x, b.x), fminf(box.y, b.y), fmaxf(box.z, b.z), fmaxf(box.w, b.w));
+            }
+            A->bvhBox[base + node] = box;
+            if (node == 0u) break;
+            node = parentNode[node];
         }
-        A->bvhBox[base + i] = box;
     }
     for (uint32_t q = threadIdx.x; q < nb; q += kBvhThreads) A->bvhLeaf[d->bodyOffset + q] = (uint32_t)(keys[q] & 0xffffffffu);
     __syncthreads();
