@@ -699,12 +699,9 @@ bp_status bp_batch::swap_residency(uint32_t out, uint32_t in, hipStream_t stream
     if (!fits_ranges_of(out, in)) { set_error("world does not fit the ranges of the world it would replace"); return BP_ERR_CAPACITY_OVERFLOW; }
     HostWorld &o = worlds[out], &w = worlds[in];
     const Region r = regions[out];
-    // 1. The published state of `out` into its host copy (transactions may have changed its list).
-    bpd::WorldDesc d;
-    BP_HIP(hipMemcpyAsync(&d, descs + out, sizeof d, hipMemcpyDeviceToHost, stream));
-    BP_HIP(hipStreamSynchronize(stream));
-    if (d.failed == bpd::kWorldHeld) { set_error("world holds region-removed bubbles"); return BP_ERR_UNAVAILABLE; }
-    const uint32_t n = std::min(d.bubbleCount, r.bubbles), p = d.pub;
+    // 1. The published state of `out` into its host copy (transactions may have changed its list):
+    //    its descriptor, its records (every field, gathered on the device) and its bodies' state on
+    //    both sides, in one round trip.
     const uint32_t m = (uint32_t)w.bubbles.size();
     if (std::max(r.bubbles, m) > xferCapacity) { set_error("world larger than the transfer buffer"); return BP_ERR_CAPACITY_OVERFLOW; }
     if (pinnedCapacity < xferCapacity) {
@@ -715,27 +712,44 @@ bp_status bp_batch::swap_residency(uint32_t out, uint32_t in, hipStream_t stream
         BP_HIP(hipHostMalloc((void **)&pinnedIdx, 2 * (size_t)xferCapacity * sizeof(int32_t), hipHostMallocDefault));
         pinnedCapacity = xferCapacity;
     }
-    {
-        const size_t nb = o.bodies.size();
-        std::vector<float4> bodyPose(nb), bodyMotion(nb);
-        std::vector<float2> bodyPrev(nb);
-        BP_HIP(bp_launch_gather_world_full(arenaDev, descs + out, xferBubbles, r.bubbles, stream));
-        if (n) BP_HIP(hipMemcpyAsync(pinnedRecs, xferBubbles, n * sizeof(bp_bubble), hipMemcpyDeviceToHost, stream));
-        if (nb) {
-            BP_HIP(hipMemcpyAsync(bodyPose.data(), arena.bodyPose[p] + d.bodyOffset, nb * sizeof(float4), hipMemcpyDeviceToHost, stream));
-            BP_HIP(hipMemcpyAsync(bodyPrev.data(), arena.bodyPrev[p] + d.bodyOffset, nb * sizeof(float2), hipMemcpyDeviceToHost, stream));
-            BP_HIP(hipMemcpyAsync(bodyMotion.data(), arena.bodyMotion + d.bodyOffset, nb * sizeof(float4), hipMemcpyDeviceToHost, stream));
-        }
-        BP_HIP(hipStreamSynchronize(stream));
-        o.bubbles.assign(pinnedRecs, pinnedRecs + n);
-        for (size_t k = 0; k < nb; ++k) {
-            bp_body &b = o.bodies[k];
-            b.pose[0] = bodyPose[k].x; b.pose[1] = bodyPose[k].y; b.pose[2] = bodyPose[k].z;
-            b.previous[0] = bodyPrev[k].x; b.previous[1] = bodyPrev[k].y; b.previous[2] = bodyPose[k].w;
-            b.velocity[0] = bodyMotion[k].x; b.velocity[1] = bodyMotion[k].y; b.velocity[2] = bodyMotion[k].z;
-        }
-        o.failed = d.failed == bpd::kWorldFailed;
+    const size_t nb = o.bodies.size();
+    const uint32_t bo = hostDescs[out].bodyOffset;          // a world's ranges do not move while it is resident
+    const size_t readBytes = sizeof(bpd::WorldDesc) + nb * (2 * sizeof(float4) + 2 * sizeof(float2) + sizeof(float4)) + 64;
+    if (pinnedBlobBytes < readBytes) {
+        if (pinnedBlob) (void)hipHostFree(pinnedBlob);
+        pinnedBlob = nullptr; pinnedBlobBytes = 0;
+        BP_HIP(hipHostMalloc(&pinnedBlob, readBytes, hipHostMallocDefault));
+        pinnedBlobBytes = readBytes;
     }
+    char *rb = static_cast<char *>(pinnedBlob);
+    bpd::WorldDesc *dPinned = reinterpret_cast<bpd::WorldDesc *>(rb);
+    float4 *pose0 = reinterpret_cast<float4 *>(rb + ((sizeof(bpd::WorldDesc) + 15) & ~(size_t)15));
+    float4 *pose1 = pose0 + nb, *motion = pose1 + nb;
+    float2 *prev0 = reinterpret_cast<float2 *>(motion + nb), *prev1 = prev0 + nb;
+    BP_HIP(hipMemcpyAsync(dPinned, descs + out, sizeof(bpd::WorldDesc), hipMemcpyDeviceToHost, stream));
+    BP_HIP(bp_launch_gather_world_full(arenaDev, descs + out, xferBubbles, r.bubbles, stream));
+    BP_HIP(hipMemcpyAsync(pinnedRecs, xferBubbles, (size_t)r.bubbles * sizeof(bp_bubble), hipMemcpyDeviceToHost, stream));
+    if (nb) {
+        BP_HIP(hipMemcpyAsync(pose0, arena.bodyPose[0] + bo, nb * sizeof(float4), hipMemcpyDeviceToHost, stream));
+        BP_HIP(hipMemcpyAsync(pose1, arena.bodyPose[1] + bo, nb * sizeof(float4), hipMemcpyDeviceToHost, stream));
+        BP_HIP(hipMemcpyAsync(prev0, arena.bodyPrev[0] + bo, nb * sizeof(float2), hipMemcpyDeviceToHost, stream));
+        BP_HIP(hipMemcpyAsync(prev1, arena.bodyPrev[1] + bo, nb * sizeof(float2), hipMemcpyDeviceToHost, stream));
+        BP_HIP(hipMemcpyAsync(motion, arena.bodyMotion + bo, nb * sizeof(float4), hipMemcpyDeviceToHost, stream));
+    }
+    BP_HIP(hipStreamSynchronize(stream));
+    const bpd::WorldDesc d = *dPinned;
+    if (d.failed == bpd::kWorldHeld) { set_error("world holds region-removed bubbles"); return BP_ERR_UNAVAILABLE; }
+    const uint32_t n = std::min(d.bubbleCount, r.bubbles), p = d.pub;
+    o.bubbles.assign(pinnedRecs, pinnedRecs + n);
+    for (size_t k = 0; k < nb; ++k) {
+        bp_body &b = o.bodies[k];
+        const float4 pose = p ? pose1[k] : pose0[k];
+        const float2 prev = p ? prev1[k] : prev0[k];
+        b.pose[0] = pose.x; b.pose[1] = pose.y; b.pose[2] = pose.z;
+        b.previous[0] = prev.x; b.previous[1] = prev.y; b.previous[2] = pose.w;
+        b.velocity[0] = motion[k].x; b.velocity[1] = motion[k].y; b.velocity[2] = motion[k].z;
+    }
+    o.failed = d.failed == bpd::kWorldFailed;
     // 2. `in` into the ranges, as pack() would write a world there (published side 0).
     bpd::WorldDesc dn = d;
     dn.bubbleCount = m;

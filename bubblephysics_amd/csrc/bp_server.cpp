@@ -88,6 +88,8 @@ struct StageJob {
 
 constexpr int kMaxLaunches = 64;
 constexpr int kStageJobs = 16;
+constexpr double kUrgentWait = 0.02;             // seconds after which a waiting world may displace a world whose requests only wait
+constexpr double kRepackDelay = 0.25;            // seconds a world waits for in-place admission before a repack
 // Classes a served world may run in: the LDS configurations and the one-workgroup global kernel
 // (a cooperative grid needs the whole device and would stall every other launch).
 const int kServeClasses[] = {0, 1, 2, 3, 4, 7, 9};
@@ -125,6 +127,8 @@ struct bp_server {
     uint64_t useClock = 0, used = 0;
     std::deque<uint32_t> ready[kClassCount];
     std::vector<uint32_t> admissions;
+    std::vector<double> waitingSince;           // per world: when it began to wait for admission
+    bool admissionRetry = true;                 // worlds became idle or began to wait since the last attempt
     bool launchAfterAdmission = false;          // the worlds ready after an admission round launch first
     int inflightPerClass[kClassCount] = {};
     Launch launches[kMaxLaunches];
@@ -283,7 +287,7 @@ bp_status bp_server::init() {
             return s;
     }
     queue.assign(W, {});
-    busy.assign(W, 0); queued.assign(W, 0); wantAdmission.assign(W, 0);
+    busy.assign(W, 0); queued.assign(W, 0); wantAdmission.assign(W, 0); waitingSince.assign(W, 0.0);
     worldFailed.assign(W, 0); dropPending.assign(W, 0);
     for (uint32_t w = 0; w < W; ++w) worldFailed[w] = b.worlds[w].failed ? 1 : 0;
     count.resize(W); headCount.assign(W, 0); headClass.assign(W, -1); lastUse.assign(W, 0);
@@ -380,7 +384,7 @@ void bp_server::enqueue(const Pending &p) {
 void bp_server::prepare_head(uint32_t w) {
     while (!queue[w].empty()) {
         if (!resident[w]) {
-            if (!wantAdmission[w]) { wantAdmission[w] = 1; admissions.push_back(w); }
+            if (!wantAdmission[w]) { wantAdmission[w] = 1; admissions.push_back(w); waitingSince[w] = now(); admissionRetry = true; }
             return;
         }
         Pending &p = queue[w].front();
@@ -946,7 +950,7 @@ bp_status bp_server::admit_waiting() {
     std::vector<uint32_t> waiting;
     waiting.swap(admissions);
     for (uint32_t w : waiting) wantAdmission[w] = 0;
-    for (uint32_t w : deferred) { wantAdmission[w] = 1; admissions.push_back(w); }
+    for (uint32_t w : deferred) { wantAdmission[w] = 1; admissions.push_back(w); waitingSince[w] = now(); }
     // Ready lists hold class choices made with the old layout; rebuild them.
     for (auto &rq : ready) rq.clear();
     for (uint32_t w = 0; w < W; ++w) {
@@ -967,15 +971,24 @@ bp_status bp_server::admit_in_place() {
         if (resident[w]) { wantAdmission[w] = 0; continue; }
         // The world takes over the victim's device memory, so the device footprint does not grow.
         // Among the idle worlds whose ranges hold it, the one with the fewest bubble slots (so that
-        // large ranges stay for large worlds), the least recently used of those.
+        // large ranges stay for large worlds), the least recently used of those. A world that has
+        // waited kUrgentWait may also displace a world whose requests only wait (none on the device
+        // or being staged): those go back to waiting for admission.
+        const bool urgent = now() - waitingSince[w] >= kUrgentWait;
         int64_t victim = -1;
+        bool victimIdle = false;
         for (uint32_t v = 0; v < W; ++v) {
-            if (!resident[v] || busy[v] || queued[v] || staging[v] || dropPending[v] || !queue[v].empty()) continue;
+            if (!resident[v] || busy[v] || staging[v] || dropPending[v]) continue;
+            const bool idle = !queued[v] && queue[v].empty();
+            if (!idle && !urgent) continue;
             if (victim >= 0) {
-                const uint32_t sv = b.regions[v].bubbles, sb = b.regions[victim].bubbles;
-                if (sv > sb || (sv == sb && lastUse[v] >= lastUse[victim])) continue;
+                if (victimIdle && !idle) continue;
+                if (idle == victimIdle) {
+                    const uint32_t sv = b.regions[v].bubbles, sb = b.regions[victim].bubbles;
+                    if (sv > sb || (sv == sb && lastUse[v] >= lastUse[victim])) continue;
+                }
             }
-            if (b.fits_ranges_of(v, w)) victim = v;
+            if (b.fits_ranges_of(v, w)) { victim = v; victimIdle = idle; }
         }
         if (victim < 0) {
             left.push_back(w);
@@ -992,6 +1005,22 @@ bp_status bp_server::admit_in_place() {
         mirrorValid[v] = 0;
         count[v] = (uint32_t)b.worlds[v].bubbles.size();
         ++local.evictions;
+        if (!victimIdle) {
+            // The displaced world's waiting requests wait for its admission; a transaction staged
+            // on its mirror is staged again from its state after admission.
+            if (queued[v]) {
+                auto &rq = ready[headClass[v]];
+                rq.erase(std::remove(rq.begin(), rq.end(), v), rq.end());
+                queued[v] = 0;
+            }
+            if (!queue[v].empty()) {
+                Pending &p = queue[v].front();
+                if (p.replace) { p.staged = false; p.replace = false; p.replaceCount = 0; --local.transactions; }
+                wantAdmission[v] = 1;
+                waitingSince[v] = now();
+                left.push_back(v);
+            }
+        }
         resident[w] = 1;
         footprintOf[w] = b.footprint(w);
         used += footprintOf[w];                          // at most the victim's: its ranges hold the world
@@ -1030,11 +1059,20 @@ bp_status bp_server::admit_in_place() {
 // is launched once before the next round, so each round makes progress even when the budget
 // forces the next admission to evict a world that still has requests.
 bool bp_server::form_launches() {
-    if (!admissions.empty()) {
+    if (!admissions.empty() && admissionRetry) {
+        admissionRetry = false;
         bp_status s = admit_in_place();
         if (s != BP_OK) { fail(s); return false; }
     }
+    // A world that no idle world's ranges hold usually finds one as launches complete; the device
+    // drains for a repacking round only once the oldest of them has waited kRepackDelay.
+    bool repack = false;
     if (!admissions.empty() && !launchAfterAdmission) {
+        double oldest = now();
+        for (uint32_t w : admissions) oldest = std::min(oldest, waitingSince[w]);
+        repack = now() - oldest >= kRepackDelay;
+    }
+    if (repack) {
         if (!inflight.empty() || !activeStage.empty()) return false;   // drain first: the repack moves every world
         bp_status s = admit_waiting();
         if (s != BP_OK) { fail(s); return false; }
@@ -1084,6 +1122,7 @@ void bp_server::run() {
         for (const Pending &p : taken) enqueue(p);
         bool progressed = poll_launches();
         progressed |= poll_stages();
+        if (progressed) admissionRetry = true;           // worlds may have become idle
         {
             std::lock_guard<std::mutex> lock(mu);
             if (fatal != BP_OK) break;
@@ -1093,7 +1132,7 @@ void bp_server::run() {
             }
         }
         progressed |= form_launches();
-        if (!progressed && (!inflight.empty() || !activeStage.empty())) std::this_thread::yield();
+        if (!progressed && (!inflight.empty() || !activeStage.empty() || !admissions.empty())) std::this_thread::yield();
     }
     // Drain whatever is still on the device before the dispatcher ends.
     for (int id : inflight) (void)hipEventSynchronize(launches[id].done);
