@@ -765,54 +765,67 @@ bp_status bp_batch::swap_residency(uint32_t out, uint32_t in, hipStream_t stream
     desc_controls(w, m, dn);
     GeometryStage gs;
     stage_geometry(w, gs);
-    // The geometry goes up from one pinned blob (asynchronous copies, one wait at the end).
+    bpd::WorldDesc dOut = d;                         // the device's record: its failure flag is current
+    dOut.bubbleCount = 0;
+    dOut.mirror = nullptr; dOut.bodyMirror = nullptr; dOut.mirrorIdx = nullptr; dOut.mirrorCapacity = 0;
+    // Everything goes up in one blob (records, indices, geometry, both descriptors and the list of
+    // copies), then one kernel copies the pieces to their places and the scatter kernel writes the
+    // records into the structure-of-arrays ranges.
+    std::vector<bpd::CopyOp> ops;
+    ops.reserve(32);
     size_t blobBytes = 0;
-    auto bytes_of = [&](const auto &src) { blobBytes += (src.size() * sizeof(src[0]) + 15) & ~(size_t)15; };
-    bytes_of(gs.segAB); bytes_of(gs.segRegion); bytes_of(gs.segFlags); bytes_of(gs.segCat); bytes_of(gs.segMask);
-    bytes_of(gs.segId); bytes_of(gs.segOwner); bytes_of(gs.bodyPose); bytes_of(gs.bodyPrev); bytes_of(gs.bodyMotion);
-    bytes_of(gs.bodyShape); bytes_of(gs.bodyLimits); bytes_of(gs.bodyRanges); bytes_of(gs.bodyKind); bytes_of(gs.bodyId);
-    bytes_of(gs.verts); bytes_of(gs.tris); bytes_of(gs.motors); bytes_of(gs.bands); bytes_of(gs.fields);
-    bytes_of(gs.fieldVerts); bytes_of(gs.fieldIds);
+    auto room = [&](size_t size) { const size_t at = blobBytes; blobBytes += (size + 15) & ~(size_t)15; return at; };
+    const size_t recsAt = room((size_t)m * sizeof(bp_bubble)), idxAt = room(2 * (size_t)m * sizeof(int32_t));
+    struct Piece { void *dst; const void *src; size_t size; size_t at; };
+    std::vector<Piece> pieces;
+    auto piece = [&](void *dst, const void *src, size_t size) {
+        if (size) pieces.push_back(Piece{dst, src, size, room(size)});
+    };
+    auto vec = [&](auto *dst, const auto &src) { piece(dst, src.data(), src.size() * sizeof(src[0])); };
+    vec(arena.segAB + dn.segmentOffset, gs.segAB); vec(arena.segRegion + dn.segmentOffset, gs.segRegion);
+    vec(arena.segFlags + dn.segmentOffset, gs.segFlags); vec(arena.segCat + dn.segmentOffset, gs.segCat);
+    vec(arena.segMask + dn.segmentOffset, gs.segMask); vec(arena.segId + dn.segmentOffset, gs.segId);
+    vec(arena.segOwner + dn.segmentOffset, gs.segOwner);
+    vec(arena.bodyPose[0] + dn.bodyOffset, gs.bodyPose); vec(arena.bodyPrev[0] + dn.bodyOffset, gs.bodyPrev);
+    vec(arena.bodyMotion + dn.bodyOffset, gs.bodyMotion); vec(arena.bodyShape + dn.bodyOffset, gs.bodyShape);
+    vec(arena.bodyLimits + dn.bodyOffset, gs.bodyLimits); vec(arena.bodyRanges + dn.bodyOffset, gs.bodyRanges);
+    vec(arena.bodyKind + dn.bodyOffset, gs.bodyKind); vec(arena.bodyId + dn.bodyOffset, gs.bodyId);
+    vec(arena.verts + dn.vertexOffset, gs.verts); vec(arena.tris + dn.triangleOffset, gs.tris);
+    vec(arena.motors + dn.motorOffset, gs.motors); vec(arena.bands + dn.bandOffset, gs.bands);
+    vec(arena.fields + dn.fieldOffset, gs.fields); vec(arena.fieldVerts + dn.fieldVertexOffset, gs.fieldVerts);
+    vec(arena.fieldIds + dn.fieldIdOffset, gs.fieldIds);
+    piece(descs + in, &dn, sizeof dn);
+    piece(descs + out, &dOut, sizeof dOut);
+    const size_t opsAt = room(pieces.size() * sizeof(bpd::CopyOp));
     if (pinnedBlobBytes < blobBytes) {
         if (pinnedBlob) (void)hipHostFree(pinnedBlob);
         pinnedBlob = nullptr; pinnedBlobBytes = 0;
         BP_HIP(hipHostMalloc(&pinnedBlob, blobBytes, hipHostMallocDefault));
         pinnedBlobBytes = blobBytes;
     }
-    size_t blobAt = 0;
-    auto put = [&](auto *dst, const auto &src) -> hipError_t {
-        if (src.empty()) return hipSuccess;
-        const size_t size = src.size() * sizeof(src[0]);
-        char *at = static_cast<char *>(pinnedBlob) + blobAt;
-        std::memcpy(at, src.data(), size);
-        blobAt += (size + 15) & ~(size_t)15;
-        return hipMemcpyAsync(dst, at, size, hipMemcpyHostToDevice, stream);
-    };
+    if (deviceBlobBytes < blobBytes) {
+        bp_status st = alloc(deviceBlob, blobBytes);
+        if (st != BP_OK) return st;
+        deviceBlobBytes = blobBytes;
+    }
+    char *hb = static_cast<char *>(pinnedBlob);
     if (m) {
-        std::memcpy(pinnedRecs, w.bubbles.data(), m * sizeof(bp_bubble));
-        motor_band_indices(in, w.bubbles.data(), m, pinnedIdx);
-        BP_HIP(hipMemcpyAsync(xferBubbles, pinnedRecs, m * sizeof(bp_bubble), hipMemcpyHostToDevice, stream));
-        BP_HIP(hipMemcpyAsync(xferIndices, pinnedIdx, 2 * (size_t)m * sizeof(int32_t), hipMemcpyHostToDevice, stream));
-        BP_HIP(bp_launch_scatter_world(arenaDev, dn.bubbleOffset, m, xferBubbles, xferIndices, xferIndices + m, stream));
+        std::memcpy(hb + recsAt, w.bubbles.data(), m * sizeof(bp_bubble));
+        motor_band_indices(in, w.bubbles.data(), m, reinterpret_cast<int32_t *>(hb + idxAt));
+    }
+    bpd::CopyOp *op = reinterpret_cast<bpd::CopyOp *>(hb + opsAt);
+    for (const Piece &pc : pieces) {
+        std::memcpy(hb + pc.at, pc.src, pc.size);
+        *op++ = bpd::CopyOp{pc.at, pc.dst, (uint32_t)pc.size, 0u};
+    }
+    BP_HIP(hipMemcpyAsync(deviceBlob, hb, blobBytes, hipMemcpyHostToDevice, stream));
+    BP_HIP(bp_launch_multi_copy(deviceBlob, reinterpret_cast<const bpd::CopyOp *>(deviceBlob + opsAt), (uint32_t)pieces.size(), stream));
+    if (m) {
+        const int32_t *idx = reinterpret_cast<const int32_t *>(deviceBlob + idxAt);
+        BP_HIP(bp_launch_scatter_world(arenaDev, dn.bubbleOffset, m, reinterpret_cast<const bp_bubble *>(deviceBlob + recsAt),
+                                       idx, idx + m, stream));
         BP_HIP(hipMemsetAsync(arena.acc + dn.bubbleOffset, 0, m * sizeof(float2), stream));
     }
-    BP_HIP(put(arena.segAB + dn.segmentOffset, gs.segAB)); BP_HIP(put(arena.segRegion + dn.segmentOffset, gs.segRegion));
-    BP_HIP(put(arena.segFlags + dn.segmentOffset, gs.segFlags)); BP_HIP(put(arena.segCat + dn.segmentOffset, gs.segCat));
-    BP_HIP(put(arena.segMask + dn.segmentOffset, gs.segMask)); BP_HIP(put(arena.segId + dn.segmentOffset, gs.segId));
-    BP_HIP(put(arena.segOwner + dn.segmentOffset, gs.segOwner));
-    BP_HIP(put(arena.bodyPose[0] + dn.bodyOffset, gs.bodyPose)); BP_HIP(put(arena.bodyPrev[0] + dn.bodyOffset, gs.bodyPrev));
-    BP_HIP(put(arena.bodyMotion + dn.bodyOffset, gs.bodyMotion)); BP_HIP(put(arena.bodyShape + dn.bodyOffset, gs.bodyShape));
-    BP_HIP(put(arena.bodyLimits + dn.bodyOffset, gs.bodyLimits)); BP_HIP(put(arena.bodyRanges + dn.bodyOffset, gs.bodyRanges));
-    BP_HIP(put(arena.bodyKind + dn.bodyOffset, gs.bodyKind)); BP_HIP(put(arena.bodyId + dn.bodyOffset, gs.bodyId));
-    BP_HIP(put(arena.verts + dn.vertexOffset, gs.verts)); BP_HIP(put(arena.tris + dn.triangleOffset, gs.tris));
-    BP_HIP(put(arena.motors + dn.motorOffset, gs.motors)); BP_HIP(put(arena.bands + dn.bandOffset, gs.bands));
-    BP_HIP(put(arena.fields + dn.fieldOffset, gs.fields)); BP_HIP(put(arena.fieldVerts + dn.fieldVertexOffset, gs.fieldVerts));
-    BP_HIP(put(arena.fieldIds + dn.fieldIdOffset, gs.fieldIds));
-    bpd::WorldDesc dOut = d;                         // the device's record: its failure flag is current
-    dOut.bubbleCount = 0;
-    dOut.mirror = nullptr; dOut.bodyMirror = nullptr; dOut.mirrorIdx = nullptr; dOut.mirrorCapacity = 0;
-    BP_HIP(hipMemcpyAsync(descs + in, &dn, sizeof dn, hipMemcpyHostToDevice, stream));
-    BP_HIP(hipMemcpyAsync(descs + out, &dOut, sizeof dOut, hipMemcpyHostToDevice, stream));
     BP_HIP(hipStreamSynchronize(stream));
     // 3. Bookkeeping: residency, ranges, kernel classes (the batch's own class lists), graphs.
     hostDescs[in] = dn;

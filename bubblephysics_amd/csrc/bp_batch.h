@@ -29,6 +29,7 @@ extern "C" hipError_t bp_launch_gather_world(const bpd::Arena *arena, const bpd:
                                              hipStream_t stream);
 extern "C" hipError_t bp_launch_gather_world_full(const bpd::Arena *arena, const bpd::WorldDesc *desc, bp_bubble *out,
                                                   uint32_t count, hipStream_t stream);
+extern "C" hipError_t bp_launch_multi_copy(const char *blob, const bpd::CopyOp *ops, uint32_t count, hipStream_t stream);
 extern "C" hipError_t bp_launch_scatter_world(const bpd::Arena *arena, uint32_t offset, uint32_t count, const bp_bubble *in,
                                               const int32_t *motorIdx, const int32_t *bandIdx, hipStream_t stream);
 extern "C" uint32_t bp_coop_block();
@@ -215,6 +216,8 @@ struct bp_batch {
     uint32_t pinnedCapacity = 0;
     void *pinnedBlob = nullptr;        // pinned host staging of one world's geometry
     size_t pinnedBlobBytes = 0;
+    char *deviceBlob = nullptr;        // its device copy (released with the arena)
+    size_t deviceBlobBytes = 0;
     uint32_t launchOffset[bph::kClassCount] = {}, launchCount[bph::kClassCount] = {};
     uint32_t totalBubbles = 0;
     uint32_t totalBodies = 0;          // body records of the arena (every world's body range)
@@ -295,6 +298,7 @@ struct bp_batch {
         allocations.clear();
         stats = nullptr; statsCapacity = 0; descs = nullptr; arenaDev = nullptr; subsetOrders = nullptr; cursors = nullptr;
         xferBubbles = nullptr; xferIndices = nullptr; xferCapacity = 0;
+        deviceBlob = nullptr; deviceBlobBytes = 0;
         dropRecs = nullptr; dropIds = nullptr; dropCapacity = dropIdCapacity = 0;
         pending = false;
         for (auto &o : orders) o = nullptr;

@@ -150,6 +150,13 @@ struct WorldDesc {
 
 enum : uint32_t { kWorldRunnable = 0u, kWorldFailed = 1u, kWorldHeld = 2u };
 
+// One copy of a staged blob: `bytes` (a multiple of 4) from blob offset `src` to device address `dst`.
+struct CopyOp {
+    uint64_t src;
+    void *dst;
+    uint32_t bytes, pad;
+};
+
 // A serving transaction on one world, applied on the device before the world's step
 // (bp_request of include/bubblephysics.h; `buckets` is the grid bucket count for the bubble
 // count after the transaction, computed by the host, which tracks the count).

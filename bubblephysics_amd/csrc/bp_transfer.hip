@@ -50,6 +50,15 @@ __global__ void scatter_world_kernel(const Arena *__restrict__ A, uint32_t offse
     }
 }
 
+// Copies of one staged blob to many destinations (a world moved into another world's ranges):
+// one workgroup per copy, 4-byte words (every record array has 4-byte elements or wider).
+__global__ void multi_copy_kernel(const char *blob, const CopyOp *ops) {
+    const CopyOp op = ops[blockIdx.x];
+    const uint32_t *src = reinterpret_cast<const uint32_t *>(blob + op.src);
+    uint32_t *dst = reinterpret_cast<uint32_t *>(op.dst);
+    for (uint32_t i = threadIdx.x; i < op.bytes / 4u; i += blockDim.x) dst[i] = src[i];
+}
+
 }  // namespace bpd
 
 extern "C" __attribute__((visibility("hidden"))) hipError_t bp_launch_gather_world(const bpd::Arena *arena, const bpd::WorldDesc *desc,
@@ -314,5 +323,12 @@ extern "C" __attribute__((visibility("hidden"))) hipError_t bp_launch_fill_u32(u
     if (words == 0) return hipSuccess;
     const size_t blocks = std::min<size_t>((words + 255) / 256, 4096);
     hipLaunchKernelGGL(bpd::fill_u32_kernel, dim3((uint32_t)blocks), dim3(256), 0, stream, p, words, value);
+    return hipGetLastError();
+}
+
+extern "C" __attribute__((visibility("hidden"))) hipError_t bp_launch_multi_copy(const char *blob, const bpd::CopyOp *ops,
+                                                                             uint32_t count, hipStream_t stream) {
+    if (count == 0) return hipSuccess;
+    hipLaunchKernelGGL(bpd::multi_copy_kernel, dim3(count), dim3(256), 0, stream, blob, ops);
     return hipGetLastError();
 }
