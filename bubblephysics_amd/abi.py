@@ -204,7 +204,7 @@ class ContactEvent(ctypes.Structure):     # bp_contact_event
 class ServerStats(ctypes.Structure):
     _fields_ = [(name, ctypes.c_uint64) for name in (
         "submitted", "completed", "launches", "launched_worlds", "transactions", "rejected", "evictions", "admissions",
-        "repacks", "resident_bytes")]
+        "repacks", "resident_bytes", "admission_microseconds")]
 
 
 class StageRejection(ctypes.Structure):

@@ -30,6 +30,10 @@ extern "C" hipError_t bp_launch_gather_world(const bpd::Arena *arena, const bpd:
 extern "C" hipError_t bp_launch_gather_world_full(const bpd::Arena *arena, const bpd::WorldDesc *desc, bp_bubble *out,
                                                   uint32_t count, hipStream_t stream);
 extern "C" hipError_t bp_launch_multi_copy(const char *blob, const bpd::CopyOp *ops, uint32_t count, hipStream_t stream);
+extern "C" hipError_t bp_launch_gather_worlds_full(const bpd::Arena *arena, const bpd::WorldMove *moves, uint32_t count,
+                                                   uint32_t maxRecords, hipStream_t stream);
+extern "C" hipError_t bp_launch_scatter_worlds(const bpd::Arena *arena, const bpd::WorldMove *moves, uint32_t count,
+                                               uint32_t maxRecords, hipStream_t stream);
 extern "C" hipError_t bp_launch_scatter_world(const bpd::Arena *arena, uint32_t offset, uint32_t count, const bp_bubble *in,
                                               const int32_t *motorIdx, const int32_t *bandIdx, hipStream_t stream);
 extern "C" uint32_t bp_coop_block();
@@ -212,6 +216,7 @@ struct bp_batch {
     int32_t *xferIndices = nullptr;    // their motor and band indices
     uint32_t xferCapacity = 0;
     bp_bubble *pinnedRecs = nullptr;   // pinned host staging of one world's records (swap_residency)
+    void *pinnedBlobDev = nullptr;       // device address of pinnedBlob
     int32_t *pinnedIdx = nullptr;
     uint32_t pinnedCapacity = 0;
     void *pinnedBlob = nullptr;        // pinned host staging of one world's geometry
@@ -333,8 +338,17 @@ struct bp_batch {
     // Serving residency without a repack: whether the held world `in` fits the device ranges of
     // the resident world `out`, and the exchange (out's published state is read back into its
     // host copy, in's host copy is written into the ranges). Both worlds must be idle.
+    // swap_residency exchanges the pairs (outs[i], ins[i]) (all of them distinct worlds), with
+    // one read-back round trip and one upload for as many pairs as the transfer buffer holds;
+    // results[i] is BP_OK, BP_ERR_CAPACITY_OVERFLOW (the world does not fit) or
+    // BP_ERR_UNAVAILABLE (the resident world holds region-removed bubbles), and the return value
+    // a device or allocation error.
     bool fits_ranges_of(uint32_t out, uint32_t in) const;
-    bp_status swap_residency(uint32_t out, uint32_t in, hipStream_t stream);
+    // mirrors (null, or one per pair; capacity 0 for none) are the serving mirror ranges the
+    // descriptor of a world moved in points to.
+    struct MirrorRange { void *mirror; float4 *bodyMirror; int32_t *mirrorIdx; uint32_t capacity; };
+    bp_status swap_residency(const uint32_t *outs, const uint32_t *ins, uint32_t count, const MirrorRange *mirrors,
+                             bp_status *results, hipStream_t stream);
     bp_status launch(const uint32_t *subset, uint32_t subsetCount, uint32_t steps);
     void plan_pools();
     bp_status refresh_order();

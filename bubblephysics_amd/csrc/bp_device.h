@@ -150,11 +150,20 @@ struct WorldDesc {
 
 enum : uint32_t { kWorldRunnable = 0u, kWorldFailed = 1u, kWorldHeld = 2u };
 
-// One copy of a staged blob: `bytes` (a multiple of 4) from blob offset `src` to device address `dst`.
+// One copy of a staged blob: `bytes` (a multiple of 4) from blob offset `src` to device address `dst`
+// (with no blob, `src` is itself an address).
 struct CopyOp {
     uint64_t src;
     void *dst;
     uint32_t bytes, pad;
+};
+
+// One world of a many-world gather (its records out of the arena) or scatter (into it).
+struct WorldMove {
+    const WorldDesc *desc;      // gather: the world's descriptor on the device
+    void *records;              // bp_bubble records; gather: destination; scatter: source
+    const int32_t *indices;     // scatter: each record's motor index, then each record's band index
+    uint32_t offset, count;     // scatter: first slot of the world's range and records in it
 };
 
 // A serving transaction on one world, applied on the device before the world's step

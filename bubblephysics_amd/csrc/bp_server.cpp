@@ -90,6 +90,7 @@ constexpr int kMaxLaunches = 64;
 constexpr int kStageJobs = 16;
 constexpr double kUrgentWait = 0.02;             // seconds after which a waiting world may displace a world whose requests only wait
 constexpr double kRepackDelay = 0.25;            // seconds a world waits for in-place admission before a repack
+constexpr double kAdmissionInterval = 5e-4;      // seconds between in-place admission rounds (a round's exchanges share one round trip)
 // Classes a served world may run in: the LDS configurations and the one-workgroup global kernel
 // (a cooperative grid needs the whole device and would stall every other launch).
 const int kServeClasses[] = {0, 1, 2, 3, 4, 7, 9};
@@ -118,6 +119,7 @@ struct bp_server {
     uint32_t W = 0;
     std::vector<std::deque<Pending>> queue;
     std::vector<uint8_t> busy, queued, resident, wantAdmission;
+    std::vector<uint8_t> taken;                 // victims already chosen in the current admission round
     std::vector<uint8_t> worldFailed;           // a step failed: the world takes no more steps or transactions
     std::vector<uint32_t> dropPending;          // bubbles the world's last step removed from the region
     bpd::DropRecord *dropDev = nullptr;
@@ -128,6 +130,7 @@ struct bp_server {
     std::deque<uint32_t> ready[kClassCount];
     std::vector<uint32_t> admissions;
     std::vector<double> waitingSince;           // per world: when it began to wait for admission
+    double lastAdmissionRound = -1.0;           // server clock of the last in-place admission round
     bool admissionRetry = true;                 // worlds became idle or began to wait since the last attempt
     bool launchAfterAdmission = false;          // the worlds ready after an admission round launch first
     int inflightPerClass[kClassCount] = {};
@@ -192,6 +195,7 @@ struct bp_server {
         st.launches = local.launches; st.launched_worlds = local.launched_worlds; st.transactions = local.transactions;
         st.rejected = local.rejected; st.evictions = local.evictions; st.admissions = local.admissions;
         st.repacks = local.repacks; st.resident_bytes = local.resident_bytes;
+        st.admission_microseconds = local.admission_microseconds;
     }
 };
 
@@ -287,7 +291,7 @@ bp_status bp_server::init() {
             return s;
     }
     queue.assign(W, {});
-    busy.assign(W, 0); queued.assign(W, 0); wantAdmission.assign(W, 0); waitingSince.assign(W, 0.0);
+    busy.assign(W, 0); queued.assign(W, 0); wantAdmission.assign(W, 0); taken.assign(W, 0); waitingSince.assign(W, 0.0);
     worldFailed.assign(W, 0); dropPending.assign(W, 0);
     for (uint32_t w = 0; w < W; ++w) worldFailed[w] = b.worlds[w].failed ? 1 : 0;
     count.resize(W); headCount.assign(W, 0); headClass.assign(W, -1); lastUse.assign(W, 0);
@@ -960,13 +964,40 @@ bp_status bp_server::admit_waiting() {
     return BP_OK;
 }
 
-// Admits waiting worlds one at a time into the device ranges of evicted worlds: for each, the
-// least recently used resident world that is idle (no launch, request, staging or pending
-// removal) and whose ranges hold it, within the budget. Worlds for which none qualifies stay
-// in the admission list for the repacking round.
+// Admits waiting worlds into the device ranges of evicted worlds: for each, the least recently
+// used resident world that is idle (no launch, request, staging or pending removal) and whose
+// ranges hold it, within the budget. The exchanges of one round share one read-back round trip
+// and one upload. Worlds for which none qualifies stay in the admission list for the repacking
+// round.
 bp_status bp_server::admit_in_place() {
     bp_batch &b = *batch;
-    std::vector<uint32_t> left;
+    std::vector<uint32_t> left, outs, ins;
+    std::vector<uint8_t> outIdle;
+    // The worlds that may be displaced, gathered once per round.
+    struct Candidate { uint32_t v, slots; uint64_t use; uint8_t idle; };
+    std::vector<Candidate> cands;
+    cands.reserve(W);
+    for (uint32_t v = 0; v < W; ++v) {
+        if (!resident[v] || busy[v] || staging[v] || dropPending[v]) continue;
+        cands.push_back(Candidate{v, b.regions[v].bubbles, lastUse[v], (uint8_t)(!queued[v] && queue[v].empty())});
+    }
+    // The best candidate whose ranges hold w: idle before waiting (those only when `urgent`), then
+    // the fewest bubble slots, then the least recently used; ranges are checked only for a
+    // candidate that beats the best so far.
+    auto pick = [&](uint32_t need, uint32_t w, bool urgent) -> int64_t {
+        int64_t best = -1;
+        for (size_t i = 0; i < cands.size(); ++i) {
+            const Candidate &c = cands[i];
+            if (c.slots < need || (!c.idle && !urgent) || taken[c.v]) continue;
+            if (best >= 0) {
+                const Candidate &o = cands[(size_t)best];
+                if (o.idle && !c.idle) continue;
+                if (o.idle == c.idle && (c.slots > o.slots || (c.slots == o.slots && c.use >= o.use))) continue;
+            }
+            if (b.fits_ranges_of(c.v, w)) best = (int64_t)i;
+        }
+        return best;
+    };
     for (uint32_t w : admissions) {
         if (resident[w]) { wantAdmission[w] = 0; continue; }
         // The world takes over the victim's device memory, so the device footprint does not grow.
@@ -974,30 +1005,36 @@ bp_status bp_server::admit_in_place() {
         // large ranges stay for large worlds), the least recently used of those. A world that has
         // waited kUrgentWait may also displace a world whose requests only wait (none on the device
         // or being staged): those go back to waiting for admission.
-        const bool urgent = now() - waitingSince[w] >= kUrgentWait;
-        int64_t victim = -1;
-        bool victimIdle = false;
-        for (uint32_t v = 0; v < W; ++v) {
-            if (!resident[v] || busy[v] || staging[v] || dropPending[v]) continue;
-            const bool idle = !queued[v] && queue[v].empty();
-            if (!idle && !urgent) continue;
-            if (victim >= 0) {
-                if (victimIdle && !idle) continue;
-                if (idle == victimIdle) {
-                    const uint32_t sv = b.regions[v].bubbles, sb = b.regions[victim].bubbles;
-                    if (sv > sb || (sv == sb && lastUse[v] >= lastUse[victim])) continue;
-                }
-            }
-            if (b.fits_ranges_of(v, w)) { victim = v; victimIdle = idle; }
-        }
-        if (victim < 0) {
+        const uint32_t n = (uint32_t)b.worlds[w].bubbles.size(), need = n + std::max<uint32_t>(8, n / 8);
+        const int64_t at = pick(need, w, now() - waitingSince[w] >= kUrgentWait);
+        if (at < 0) {
             left.push_back(w);
             continue;
         }
-        const uint32_t v = (uint32_t)victim;
-        bp_status s = b.swap_residency(v, w, readStream);
-        if (s == BP_ERR_UNAVAILABLE || s == BP_ERR_CAPACITY_OVERFLOW) { left.push_back(w); continue; }
+        const uint32_t victim = cands[(size_t)at].v;
+        taken[victim] = 1;
+        outs.push_back(victim);
+        ins.push_back(w);
+        outIdle.push_back(cands[(size_t)at].idle);
+    }
+    for (uint32_t v : outs) taken[v] = 0;
+    // The mirror of a world moved in, when it has room: its descriptor goes up pointing there.
+    std::vector<bp_batch::MirrorRange> mirrors(outs.size(), bp_batch::MirrorRange{nullptr, nullptr, nullptr, 0u});
+    for (size_t i = 0; i < outs.size(); ++i) {
+        const uint32_t w = ins[i];
+        if (b.worlds[w].bubbles.size() <= mirrorCap[w])
+            mirrors[i] = bp_batch::MirrorRange{mirrorDev + mirrorOffset[w], bodyMirrorDev + bodyMirrorOffset[w],
+                                               mirrorIdxDev + 2 * mirrorOffset[w], mirrorCap[w]};
+    }
+    std::vector<bp_status> results(outs.size(), BP_OK);
+    if (!outs.empty()) {
+        bp_status s = b.swap_residency(outs.data(), ins.data(), (uint32_t)outs.size(), mirrors.data(), results.data(), readStream);
         if (s != BP_OK) return s;
+    }
+    for (size_t i = 0; i < outs.size(); ++i) {
+        const uint32_t v = outs[i], w = ins[i];
+        if (results[i] == BP_ERR_UNAVAILABLE || results[i] == BP_ERR_CAPACITY_OVERFLOW) { left.push_back(w); continue; }
+        if (results[i] != BP_OK) return results[i];
         resident[v] = 0;
         used -= footprintOf[v];
         footprintOf[v] = b.footprint(v);
@@ -1005,7 +1042,7 @@ bp_status bp_server::admit_in_place() {
         mirrorValid[v] = 0;
         count[v] = (uint32_t)b.worlds[v].bubbles.size();
         ++local.evictions;
-        if (!victimIdle) {
+        if (!outIdle[i]) {
             // The displaced world's waiting requests wait for its admission; a transaction staged
             // on its mirror is staged again from its state after admission.
             if (queued[v]) {
@@ -1028,22 +1065,16 @@ bp_status bp_server::admit_in_place() {
         dropPending[w] = 0;
         wantAdmission[w] = 0;
         ++local.admissions;
-        // The world's mirror, filled from its host copy (the state just written to the device).
-        bpd::WorldDesc &d = b.hostDescs[w];
-        const HostWorld &hw = b.worlds[w];
-        const uint32_t n = (uint32_t)hw.bubbles.size();
-        if (n <= mirrorCap[w] && n == d.bubbleCount) {
-            d.mirror = mirrorDev + mirrorOffset[w];
-            d.bodyMirror = bodyMirrorDev + bodyMirrorOffset[w];
-            d.mirrorIdx = mirrorIdxDev + 2 * mirrorOffset[w];
-            d.mirrorCapacity = mirrorCap[w];
+        // The world's mirror (its descriptor on the device points there), filled from its host
+        // copy: the state just written to the device.
+        if (mirrors[i].capacity) {
+            const HostWorld &hw = b.worlds[w];
+            const uint32_t n = (uint32_t)hw.bubbles.size();
             if (n) std::memcpy(mirrorHost + mirrorOffset[w], hw.bubbles.data(), n * sizeof(bp_bubble));
             for (size_t k = 0; k < hw.bodies.size(); ++k) {
                 const bp_body &bd = hw.bodies[k];
                 bodyMirrorHost[bodyMirrorOffset[w] + k] = make_float4(bd.pose[0], bd.pose[1], bd.pose[2], bd.pose[2]);
             }
-            BP_HIP(hipMemcpyAsync(b.descs + w, &d, sizeof d, hipMemcpyHostToDevice, readStream));
-            BP_HIP(hipStreamSynchronize(readStream));
             mirrorAttached[w] = 1;
             mirrorValid[w] = 1;
         }
@@ -1059,9 +1090,12 @@ bp_status bp_server::admit_in_place() {
 // is launched once before the next round, so each round makes progress even when the budget
 // forces the next admission to evict a world that still has requests.
 bool bp_server::form_launches() {
-    if (!admissions.empty() && admissionRetry) {
+    if (!admissions.empty() && admissionRetry && now() - lastAdmissionRound >= kAdmissionInterval) {
         admissionRetry = false;
+        const double t0 = now();
+        lastAdmissionRound = t0;
         bp_status s = admit_in_place();
+        local.admission_microseconds += (uint64_t)((now() - t0) * 1e6);
         if (s != BP_OK) { fail(s); return false; }
     }
     // A world that no idle world's ranges hold usually finds one as launches complete; the device
@@ -1074,7 +1108,9 @@ bool bp_server::form_launches() {
     }
     if (repack) {
         if (!inflight.empty() || !activeStage.empty()) return false;   // drain first: the repack moves every world
+        const double t0 = now();
         bp_status s = admit_waiting();
+        local.admission_microseconds += (uint64_t)((now() - t0) * 1e6);
         if (s != BP_OK) { fail(s); return false; }
         launchAfterAdmission = true;
         return true;

@@ -9,9 +9,9 @@ namespace bpd {
 // host keeps the properties it wrote), or, with FULL, every field (a world leaving the device,
 // whose properties transactions may have changed).
 template <bool FULL>
-__global__ void gather_world_kernel(const Arena *__restrict__ A, const WorldDesc *d, bp_bubble *out) {
+__device__ void gather_range(const Arena *__restrict__ A, const WorldDesc *d, bp_bubble *out, uint32_t first, uint32_t stride) {
     const uint32_t n = d->bubbleCount, o = d->bubbleOffset, p = d->pub;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    for (uint32_t i = first; i < n; i += stride) {
         const uint32_t g = o + i;
         const float2 c = A->pos[p][g], s = A->start[p][g], v = A->vel[p][g];
         bp_bubble &b = out[i];
@@ -28,10 +28,21 @@ __global__ void gather_world_kernel(const Arena *__restrict__ A, const WorldDesc
     }
 }
 
+template <bool FULL>
+__global__ void gather_world_kernel(const Arena *__restrict__ A, const WorldDesc *d, bp_bubble *out) {
+    gather_range<FULL>(A, d, out, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+}
+
+// Every field of several worlds' records, one row of workgroups per world.
+__global__ void gather_worlds_full_kernel(const Arena *__restrict__ A, const WorldMove *moves) {
+    const WorldMove m = moves[blockIdx.y];
+    gather_range<true>(A, m.desc, static_cast<bp_bubble *>(m.records), blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+}
+
 // Records (and their motor and band indices) into both sides of the world's range.
-__global__ void scatter_world_kernel(const Arena *__restrict__ A, uint32_t offset, uint32_t count, const bp_bubble *in,
-                                     const int32_t *motorIdx, const int32_t *bandIdx) {
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
+__device__ void scatter_range(const Arena *__restrict__ A, uint32_t offset, uint32_t count, const bp_bubble *in,
+                              const int32_t *motorIdx, const int32_t *bandIdx, bool clearAcc, uint32_t first, uint32_t stride) {
+    for (uint32_t i = first; i < count; i += stride) {
         const bp_bubble b = in[i];
         const uint32_t g = offset + i;
         for (int side = 0; side < 2; ++side) {
@@ -47,14 +58,28 @@ __global__ void scatter_world_kernel(const Arena *__restrict__ A, uint32_t offse
         A->gamma[g] = b.damping; A->flags[g] = b.flags; A->cat[g] = b.category; A->mask[g] = b.mask;
         A->motor[g] = motorIdx[i];
         A->band[g] = bandIdx[i];
+        if (clearAcc) A->acc[g] = make_float2(0.f, 0.f);
     }
+}
+
+__global__ void scatter_world_kernel(const Arena *__restrict__ A, uint32_t offset, uint32_t count, const bp_bubble *in,
+                                     const int32_t *motorIdx, const int32_t *bandIdx) {
+    scatter_range(A, offset, count, in, motorIdx, bandIdx, false, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+}
+
+// Several worlds' records into their ranges, one row of workgroups per world; the accelerations
+// of the range start at zero (a world moved in has no previous force evaluation there).
+__global__ void scatter_worlds_kernel(const Arena *__restrict__ A, const WorldMove *moves) {
+    const WorldMove m = moves[blockIdx.y];
+    scatter_range(A, m.offset, m.count, static_cast<const bp_bubble *>(m.records), m.indices, m.indices + m.count, true,
+                  blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
 }
 
 // Copies of one staged blob to many destinations (a world moved into another world's ranges):
 // one workgroup per copy, 4-byte words (every record array has 4-byte elements or wider).
 __global__ void multi_copy_kernel(const char *blob, const CopyOp *ops) {
     const CopyOp op = ops[blockIdx.x];
-    const uint32_t *src = reinterpret_cast<const uint32_t *>(blob + op.src);
+    const uint32_t *src = blob ? reinterpret_cast<const uint32_t *>(blob + op.src) : reinterpret_cast<const uint32_t *>(op.src);
     uint32_t *dst = reinterpret_cast<uint32_t *>(op.dst);
     for (uint32_t i = threadIdx.x; i < op.bytes / 4u; i += blockDim.x) dst[i] = src[i];
 }
@@ -330,5 +355,25 @@ extern "C" __attribute__((visibility("hidden"))) hipError_t bp_launch_multi_copy
                                                                              uint32_t count, hipStream_t stream) {
     if (count == 0) return hipSuccess;
     hipLaunchKernelGGL(bpd::multi_copy_kernel, dim3(count), dim3(256), 0, stream, blob, ops);
+    return hipGetLastError();
+}
+
+// Many-world gather and scatter: `moves` (device-readable) lists `count` worlds of at most
+// `maxRecords` records each.
+extern "C" __attribute__((visibility("hidden"))) hipError_t bp_launch_gather_worlds_full(const bpd::Arena *arena,
+                                                                                     const bpd::WorldMove *moves, uint32_t count,
+                                                                                     uint32_t maxRecords, hipStream_t stream) {
+    if (count == 0 || maxRecords == 0) return hipSuccess;
+    const uint32_t blocks = std::min<uint32_t>((maxRecords + 255) / 256, 64);
+    hipLaunchKernelGGL(bpd::gather_worlds_full_kernel, dim3(blocks, count), dim3(256), 0, stream, arena, moves);
+    return hipGetLastError();
+}
+
+extern "C" __attribute__((visibility("hidden"))) hipError_t bp_launch_scatter_worlds(const bpd::Arena *arena,
+                                                                                 const bpd::WorldMove *moves, uint32_t count,
+                                                                                 uint32_t maxRecords, hipStream_t stream) {
+    if (count == 0 || maxRecords == 0) return hipSuccess;
+    const uint32_t blocks = std::min<uint32_t>((maxRecords + 255) / 256, 64);
+    hipLaunchKernelGGL(bpd::scatter_worlds_kernel, dim3(blocks, count), dim3(256), 0, stream, arena, moves);
     return hipGetLastError();
 }

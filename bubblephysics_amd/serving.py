@@ -173,6 +173,7 @@ class ServeStats:
             "mean_worlds_per_launch": self.counters["launched_worlds"] / max(self.counters["launches"], 1),
             **{k: self.counters[k] for k in ("launches", "transactions", "rejected", "evictions", "admissions", "repacks",
                                               "resident_bytes")},
+            "admission_s": self.counters.get("admission_microseconds", 0) * 1e-6,
         }
 
 

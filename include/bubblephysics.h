@@ -724,6 +724,7 @@ typedef struct bp_server_stats {
     uint64_t submitted, completed, launches, launched_worlds, transactions, rejected;
     uint64_t evictions, admissions, repacks;
     uint64_t resident_bytes;
+    uint64_t admission_microseconds;  /* dispatcher time spent moving worlds in and out of residency */
 } bp_server_stats;
 
 /* sizeof of bp_request, bp_server_result, bp_server_config, bp_server_stats, in that order. */
