@@ -695,8 +695,49 @@ bool bp_batch::fits_ranges_of(uint32_t out, uint32_t in) const {
            w.fields.size() <= r.fields && w.fieldVertices.size() / 2 <= r.fieldVertices && w.fieldIds.size() <= r.fieldIds;
 }
 
-bp_status bp_batch::swap_residency(const uint32_t *outs, const uint32_t *ins, uint32_t count, const MirrorRange *mirrors,
-                                   bp_status *results, hipStream_t stream) {
+// One exchange of a swap round: the resident world `out` leaves its ranges to the held world `in`.
+struct bp_batch::SwapPair {
+    uint32_t out, in, result;
+    size_t readAt;                     // in the pinned read-back area
+    uint32_t recAt;                    // in the transfer buffer and the pinned records
+    MirrorRange mirror;
+    bpd::WorldDesc dn, dOut;
+    GeometryStage gs;
+    size_t recsAt, idxAt;              // in the upload blob
+};
+
+// The pairs of the swap round in flight, and those of them whose upload is queued.
+struct bp_batch::SwapRound {
+    std::vector<SwapPair> pairs, upload;
+};
+
+bp_batch::SwapRound &bp_batch::swap_round() {
+    if (!swapRound) swapRound = new SwapRound();
+    return *swapRound;
+}
+
+namespace {
+inline size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
+// Read-back layout of one resident world: its descriptor, then its bodies' poses on both sides,
+// their motion, and their previous poses on both sides.
+inline size_t read_size(size_t nb) { return align16(sizeof(bpd::WorldDesc)) + nb * (3 * sizeof(float4) + 2 * sizeof(float2)); }
+}  // namespace
+
+bp_status bp_batch::ensure_pinned_blob(size_t bytes) {
+    if (pinnedBlobBytes >= bytes) return BP_OK;
+    if (pinnedBlob) (void)hipHostFree(pinnedBlob);
+    pinnedBlob = nullptr; pinnedBlobDev = nullptr; pinnedBlobBytes = 0;
+    BP_HIP(hipHostMalloc(&pinnedBlob, bytes, hipHostMallocDefault));
+    BP_HIP(hipHostGetDevicePointer(&pinnedBlobDev, pinnedBlob, 0));
+    pinnedBlobBytes = bytes;
+    return BP_OK;
+}
+
+bp_status bp_batch::swap_begin(const uint32_t *outs, const uint32_t *ins, uint32_t count, const MirrorRange *mirrors,
+                               bp_status *results, hipStream_t stream) {
+    SwapRound &sr = swap_round();
+    sr.pairs.clear();
+    sr.upload.clear();
     if (pinnedCapacity < xferCapacity) {
         if (pinnedRecs) (void)hipHostFree(pinnedRecs);
         if (pinnedIdx) (void)hipHostFree(pinnedIdx);
@@ -705,225 +746,217 @@ bp_status bp_batch::swap_residency(const uint32_t *outs, const uint32_t *ins, ui
         BP_HIP(hipHostMalloc((void **)&pinnedIdx, 2 * (size_t)xferCapacity * sizeof(int32_t), hipHostMallocDefault));
         pinnedCapacity = xferCapacity;
     }
-    auto align16 = [](size_t x) { return (x + 15) & ~(size_t)15; };
-    // Read-back layout of one resident world: its descriptor, then its bodies' poses and previous
-    // poses on both sides and their motion.
-    auto readSize = [&](size_t nb) { return align16(sizeof(bpd::WorldDesc)) + nb * (3 * sizeof(float4) + 2 * sizeof(float2)); };
-    auto ensurePinned = [&](size_t bytes) -> bp_status {
-        if (pinnedBlobBytes >= bytes) return BP_OK;
-        if (pinnedBlob) (void)hipHostFree(pinnedBlob);
-        pinnedBlob = nullptr; pinnedBlobBytes = 0;
-        BP_HIP(hipHostMalloc(&pinnedBlob, bytes, hipHostMallocDefault));
-        BP_HIP(hipHostGetDevicePointer(&pinnedBlobDev, pinnedBlob, 0));
-        pinnedBlobBytes = bytes;
-        return BP_OK;
+    // The pairs whose resident worlds' records fit the transfer buffer together; the others wait
+    // for a later round.
+    size_t readBytes = 0;
+    uint32_t recs = 0;
+    for (uint32_t i = 0; i < count; ++i) {
+        const uint32_t out = outs[i], in = ins[i];
+        results[i] = BP_ERR_CAPACITY_OVERFLOW;
+        if (!fits_ranges_of(out, in) || regions[out].bubbles > xferCapacity) continue;
+        const uint32_t rb = regions[out].bubbles;
+        results[i] = BP_ERR_UNAVAILABLE;
+        if (recs + rb > xferCapacity) continue;
+        results[i] = BP_OK;
+        SwapPair pr{};
+        pr.out = out; pr.in = in; pr.result = i;
+        pr.readAt = readBytes; pr.recAt = recs;
+        pr.mirror = mirrors ? mirrors[i] : MirrorRange{nullptr, nullptr, nullptr, 0u};
+        readBytes += read_size(worlds[out].bodies.size());
+        recs += rb;
+        sr.pairs.push_back(std::move(pr));
+    }
+    swapRecords = recs;
+    if (sr.pairs.empty()) return BP_OK;
+    // The published state of every resident world of the round, for its host copy (transactions
+    // may have changed its list): its descriptor, its records (every field) and its bodies' state
+    // on both sides. One kernel copies the descriptors and body arrays straight into pinned host
+    // memory, one gathers the records into the transfer buffer, which goes down in one copy.
+    const size_t copies = 6 * sr.pairs.size(), movesAt = align16(copies * sizeof(bpd::CopyOp));
+    swapReadAt = movesAt + align16(sr.pairs.size() * sizeof(bpd::WorldMove));
+    bp_status s = ensure_pinned_blob(swapReadAt + readBytes);
+    if (s != BP_OK) return s;
+    char *rbDev = static_cast<char *>(pinnedBlobDev) + swapReadAt;
+    bpd::CopyOp *ops = static_cast<bpd::CopyOp *>(pinnedBlob);
+    bpd::WorldMove *moves = reinterpret_cast<bpd::WorldMove *>(static_cast<char *>(pinnedBlob) + movesAt);
+    uint32_t nops = 0, maxRecords = 0;
+    auto copyOut = [&](const void *src, size_t dstAt, size_t bytes) {
+        if (bytes) ops[nops++] = bpd::CopyOp{(uint64_t)(uintptr_t)src, rbDev + dstAt, (uint32_t)bytes, 0u};
     };
-    struct Pair {
-        uint32_t out, in, result;
-        size_t readAt;                     // in the pinned read-back blob
-        uint32_t recAt;                    // in the transfer buffer and the pinned records
-        bpd::WorldDesc dn, dOut;
-        GeometryStage gs;
-        size_t recsAt, idxAt;              // in the upload blob
+    for (size_t j = 0; j < sr.pairs.size(); ++j) {
+        const SwapPair &pr = sr.pairs[j];
+        const uint32_t out = pr.out, nb = (uint32_t)worlds[out].bodies.size();
+        const uint32_t bo = hostDescs[out].bodyOffset;      // a world's ranges do not move while it is resident
+        const size_t at = pr.readAt, poseAt = at + align16(sizeof(bpd::WorldDesc));
+        copyOut(descs + out, at, sizeof(bpd::WorldDesc));
+        copyOut(arena.bodyPose[0] + bo, poseAt, nb * sizeof(float4));
+        copyOut(arena.bodyPose[1] + bo, poseAt + nb * sizeof(float4), nb * sizeof(float4));
+        copyOut(arena.bodyMotion + bo, poseAt + 2 * nb * sizeof(float4), nb * sizeof(float4));
+        copyOut(arena.bodyPrev[0] + bo, poseAt + 3 * nb * sizeof(float4), nb * sizeof(float2));
+        copyOut(arena.bodyPrev[1] + bo, poseAt + 3 * nb * sizeof(float4) + nb * sizeof(float2), nb * sizeof(float2));
+        moves[j] = bpd::WorldMove{descs + out, xferBubbles + pr.recAt, nullptr, 0u, regions[out].bubbles};
+        maxRecords = std::max(maxRecords, regions[out].bubbles);
+    }
+    BP_HIP(bp_launch_multi_copy(nullptr, static_cast<const bpd::CopyOp *>(pinnedBlobDev), nops, stream));
+    BP_HIP(bp_launch_gather_worlds_full(arenaDev, reinterpret_cast<const bpd::WorldMove *>(static_cast<char *>(pinnedBlobDev) + movesAt),
+                                        (uint32_t)sr.pairs.size(), maxRecords, stream));
+    BP_HIP(hipMemcpyAsync(pinnedRecs, xferBubbles, (size_t)recs * sizeof(bp_bubble), hipMemcpyDeviceToHost, stream));
+    return BP_OK;
+}
+
+bp_status bp_batch::swap_upload(bp_status *results, hipStream_t stream) {
+    SwapRound &sr = swap_round();
+    sr.upload.clear();
+    swapUploads = 0;
+    const char *rb = static_cast<const char *>(pinnedBlob) + swapReadAt;
+    for (SwapPair &pr : sr.pairs) {
+        HostWorld &o = worlds[pr.out];
+        const HostWorld &w = worlds[pr.in];
+        const Region &r = regions[pr.out];
+        const size_t nb = o.bodies.size();
+        const char *at = rb + pr.readAt;
+        const bpd::WorldDesc d = *reinterpret_cast<const bpd::WorldDesc *>(at);
+        if (d.failed == bpd::kWorldHeld) { results[pr.result] = BP_ERR_UNAVAILABLE; continue; }
+        const float4 *pose0 = reinterpret_cast<const float4 *>(at + align16(sizeof(bpd::WorldDesc)));
+        const float4 *pose1 = pose0 + nb, *motion = pose1 + nb;
+        const float2 *prev0 = reinterpret_cast<const float2 *>(motion + nb), *prev1 = prev0 + nb;
+        const uint32_t n = std::min(d.bubbleCount, r.bubbles), p = d.pub;
+        o.bubbles.assign(pinnedRecs + pr.recAt, pinnedRecs + pr.recAt + n);
+        for (size_t k = 0; k < nb; ++k) {
+            bp_body &b = o.bodies[k];
+            const float4 pose = p ? pose1[k] : pose0[k];
+            const float2 prev = p ? prev1[k] : prev0[k];
+            b.pose[0] = pose.x; b.pose[1] = pose.y; b.pose[2] = pose.z;
+            b.previous[0] = prev.x; b.previous[1] = prev.y; b.previous[2] = pose.w;
+            b.velocity[0] = motion[k].x; b.velocity[1] = motion[k].y; b.velocity[2] = motion[k].z;
+        }
+        o.failed = d.failed == bpd::kWorldFailed;
+        // `in` as pack() would write a world into these ranges (published side 0).
+        const uint32_t m = (uint32_t)w.bubbles.size();
+        bpd::WorldDesc &dn = pr.dn;
+        dn = d;
+        dn.bubbleCount = m;
+        dn.segmentCount = (uint32_t)w.segments.size(); dn.bodyCount = (uint32_t)w.bodies.size();
+        dn.vertexCount = (uint32_t)(w.vertices.size() / 2); dn.triangleCount = (uint32_t)(w.triangles.size() / 4);
+        dn.motorCount = (uint32_t)w.motors.size(); dn.bandCount = (uint32_t)w.bands.size(); dn.fieldCount = (uint32_t)w.fields.size();
+        dn.coopCellOffset = dn.coopBuckets = dn.coopScratchOffset = 0;
+        dn.historyOffset = pr.in * bpd::kFrameLimit;
+        dn.pub = 0;
+        dn.bubbleCapacity = r.bubbles;
+        dn.adjCapacity = r.adjacency;
+        dn.mirror = pr.mirror.mirror; dn.bodyMirror = pr.mirror.bodyMirror; dn.mirrorIdx = pr.mirror.mirrorIdx;
+        dn.mirrorCapacity = pr.mirror.capacity;
+        desc_controls(w, m, dn);
+        stage_geometry(w, pr.gs);
+        pr.dOut = d;                                         // the device's record: its failure flag is current
+        pr.dOut.bubbleCount = 0;
+        pr.dOut.mirror = nullptr; pr.dOut.bodyMirror = nullptr; pr.dOut.mirrorIdx = nullptr; pr.dOut.mirrorCapacity = 0;
+        sr.upload.push_back(std::move(pr));
+    }
+    swapUploads = (uint32_t)sr.upload.size();
+    if (sr.upload.empty()) return BP_OK;
+    // Every world moved in goes up in one blob (records, indices, geometry, both descriptors of
+    // each pair and the list of copies); one kernel copies the pieces to their places and one
+    // writes each world's records into its structure-of-arrays ranges.
+    size_t blobBytes = 0;
+    auto room = [&](size_t size) { const size_t at = blobBytes; blobBytes += align16(size); return at; };
+    struct Piece { void *dst; const void *src; size_t size; size_t at; };
+    std::vector<Piece> pieces;
+    pieces.reserve(sr.upload.size() * 24);
+    auto piece = [&](void *dst, const void *src, size_t size) {
+        if (size) pieces.push_back(Piece{dst, src, size, room(size)});
     };
-    uint32_t i = 0;
-    while (i < count) {
-        // The next pairs whose resident worlds' records fit the transfer buffer together.
-        std::vector<Pair> chunk;
-        size_t readBytes = 0;
-        uint32_t recs = 0;
-        for (; i < count; ++i) {
-            const uint32_t out = outs[i], in = ins[i];
-            results[i] = BP_ERR_CAPACITY_OVERFLOW;
-            if (!fits_ranges_of(out, in) || regions[out].bubbles > xferCapacity) continue;
-            const uint32_t rb = regions[out].bubbles;
-            if (!chunk.empty() && recs + rb > xferCapacity) break;
-            Pair pr{};
-            pr.out = out; pr.in = in; pr.result = i;
-            pr.readAt = readBytes; pr.recAt = recs;
-            readBytes += readSize(worlds[out].bodies.size());
-            recs += rb;
-            chunk.push_back(std::move(pr));
-        }
-        if (chunk.empty()) continue;
-        // 1. The published state of every resident world of the chunk into its host copy
-        //    (transactions may have changed its list): its descriptor, its records (every field)
-        //    and its bodies' state on both sides. One kernel copies the descriptors and body arrays
-        //    straight into pinned host memory, one gathers the records into the transfer buffer,
-        //    which goes down in one copy; one round trip.
-        const size_t copies = 6 * chunk.size();
-        const size_t header = align16(copies * sizeof(bpd::CopyOp)) + align16(chunk.size() * sizeof(bpd::WorldMove));
-        bp_status s = ensurePinned(header + readBytes);
-        if (s != BP_OK) return s;
-        char *rb = static_cast<char *>(pinnedBlob) + header;
-        char *rbDev = static_cast<char *>(pinnedBlobDev) + header;
-        bpd::CopyOp *ops = static_cast<bpd::CopyOp *>(pinnedBlob);
-        bpd::WorldMove *moves = reinterpret_cast<bpd::WorldMove *>(static_cast<char *>(pinnedBlob) + align16(copies * sizeof(bpd::CopyOp)));
-        uint32_t nops = 0, maxRecords = 0;
-        auto copyOut = [&](const void *src, size_t dstAt, size_t bytes) {
-            if (bytes) ops[nops++] = bpd::CopyOp{(uint64_t)(uintptr_t)src, rbDev + dstAt, (uint32_t)bytes, 0u};
-        };
-        for (size_t j = 0; j < chunk.size(); ++j) {
-            const Pair &pr = chunk[j];
-            const uint32_t out = pr.out, nb = (uint32_t)worlds[out].bodies.size();
-            const uint32_t bo = hostDescs[out].bodyOffset;      // a world's ranges do not move while it is resident
-            const size_t at = pr.readAt, poseAt = at + align16(sizeof(bpd::WorldDesc));
-            copyOut(descs + out, at, sizeof(bpd::WorldDesc));
-            copyOut(arena.bodyPose[0] + bo, poseAt, nb * sizeof(float4));
-            copyOut(arena.bodyPose[1] + bo, poseAt + nb * sizeof(float4), nb * sizeof(float4));
-            copyOut(arena.bodyMotion + bo, poseAt + 2 * nb * sizeof(float4), nb * sizeof(float4));
-            copyOut(arena.bodyPrev[0] + bo, poseAt + 3 * nb * sizeof(float4), nb * sizeof(float2));
-            copyOut(arena.bodyPrev[1] + bo, poseAt + 3 * nb * sizeof(float4) + nb * sizeof(float2), nb * sizeof(float2));
-            moves[j] = bpd::WorldMove{descs + out, xferBubbles + pr.recAt, nullptr, 0u, regions[out].bubbles};
-            maxRecords = std::max(maxRecords, regions[out].bubbles);
-        }
-        BP_HIP(bp_launch_multi_copy(nullptr, static_cast<const bpd::CopyOp *>(pinnedBlobDev), nops, stream));
-        BP_HIP(bp_launch_gather_worlds_full(arenaDev, reinterpret_cast<const bpd::WorldMove *>(
-                                                          static_cast<char *>(pinnedBlobDev) + align16(copies * sizeof(bpd::CopyOp))),
-                                            (uint32_t)chunk.size(), maxRecords, stream));
-        BP_HIP(hipMemcpyAsync(pinnedRecs, xferBubbles, (size_t)recs * sizeof(bp_bubble), hipMemcpyDeviceToHost, stream));
-        BP_HIP(hipStreamSynchronize(stream));
-        std::vector<Pair> go;                                    // the pairs whose resident world is not held
-        go.reserve(chunk.size());
-        for (Pair &pr : chunk) {
-            HostWorld &o = worlds[pr.out];
-            const HostWorld &w = worlds[pr.in];
-            const Region &r = regions[pr.out];
-            const size_t nb = o.bodies.size();
-            const char *at = rb + pr.readAt;
-            const bpd::WorldDesc d = *reinterpret_cast<const bpd::WorldDesc *>(at);
-            if (d.failed == bpd::kWorldHeld) { results[pr.result] = BP_ERR_UNAVAILABLE; continue; }
-            const float4 *pose0 = reinterpret_cast<const float4 *>(at + align16(sizeof(bpd::WorldDesc)));
-            const float4 *pose1 = pose0 + nb, *motion = pose1 + nb;
-            const float2 *prev0 = reinterpret_cast<const float2 *>(motion + nb), *prev1 = prev0 + nb;
-            const uint32_t n = std::min(d.bubbleCount, r.bubbles), p = d.pub;
-            o.bubbles.assign(pinnedRecs + pr.recAt, pinnedRecs + pr.recAt + n);
-            for (size_t k = 0; k < nb; ++k) {
-                bp_body &b = o.bodies[k];
-                const float4 pose = p ? pose1[k] : pose0[k];
-                const float2 prev = p ? prev1[k] : prev0[k];
-                b.pose[0] = pose.x; b.pose[1] = pose.y; b.pose[2] = pose.z;
-                b.previous[0] = prev.x; b.previous[1] = prev.y; b.previous[2] = pose.w;
-                b.velocity[0] = motion[k].x; b.velocity[1] = motion[k].y; b.velocity[2] = motion[k].z;
-            }
-            o.failed = d.failed == bpd::kWorldFailed;
-            // `in` as pack() would write a world into these ranges (published side 0).
-            const uint32_t m = (uint32_t)w.bubbles.size();
-            bpd::WorldDesc &dn = pr.dn;
-            dn = d;
-            dn.bubbleCount = m;
-            dn.segmentCount = (uint32_t)w.segments.size(); dn.bodyCount = (uint32_t)w.bodies.size();
-            dn.vertexCount = (uint32_t)(w.vertices.size() / 2); dn.triangleCount = (uint32_t)(w.triangles.size() / 4);
-            dn.motorCount = (uint32_t)w.motors.size(); dn.bandCount = (uint32_t)w.bands.size(); dn.fieldCount = (uint32_t)w.fields.size();
-            dn.coopCellOffset = dn.coopBuckets = dn.coopScratchOffset = 0;
-            dn.historyOffset = pr.in * bpd::kFrameLimit;
-            dn.pub = 0;
-            dn.bubbleCapacity = r.bubbles;
-            dn.adjCapacity = r.adjacency;
-            dn.mirror = nullptr; dn.bodyMirror = nullptr; dn.mirrorIdx = nullptr; dn.mirrorCapacity = 0;
-            if (mirrors && mirrors[pr.result].capacity) {
-                const MirrorRange &mr = mirrors[pr.result];
-                dn.mirror = mr.mirror; dn.bodyMirror = mr.bodyMirror; dn.mirrorIdx = mr.mirrorIdx; dn.mirrorCapacity = mr.capacity;
-            }
-            desc_controls(w, m, dn);
-            stage_geometry(w, pr.gs);
-            pr.dOut = d;                                         // the device's record: its failure flag is current
-            pr.dOut.bubbleCount = 0;
-            pr.dOut.mirror = nullptr; pr.dOut.bodyMirror = nullptr; pr.dOut.mirrorIdx = nullptr; pr.dOut.mirrorCapacity = 0;
-            go.push_back(std::move(pr));
-        }
-        if (go.empty()) continue;
-        // 2. Every world moved in goes up in one blob (records, indices, geometry, both
-        //    descriptors of each pair and the list of copies); one kernel copies the pieces to
-        //    their places and the scatter kernel writes each world's records into its
-        //    structure-of-arrays ranges.
-        size_t blobBytes = 0;
-        auto room = [&](size_t size) { const size_t at = blobBytes; blobBytes += align16(size); return at; };
-        struct Piece { void *dst; const void *src; size_t size; size_t at; };
-        std::vector<Piece> pieces;
-        pieces.reserve(go.size() * 24);
-        auto piece = [&](void *dst, const void *src, size_t size) {
-            if (size) pieces.push_back(Piece{dst, src, size, room(size)});
-        };
-        auto vec = [&](auto *dst, const auto &src) { piece(dst, src.data(), src.size() * sizeof(src[0])); };
-        for (Pair &pr : go) {
-            const uint32_t m = (uint32_t)worlds[pr.in].bubbles.size();
-            const bpd::WorldDesc &dn = pr.dn;
-            const GeometryStage &gs = pr.gs;
-            pr.recsAt = room((size_t)m * sizeof(bp_bubble));
-            pr.idxAt = room(2 * (size_t)m * sizeof(int32_t));
-            vec(arena.segAB + dn.segmentOffset, gs.segAB); vec(arena.segRegion + dn.segmentOffset, gs.segRegion);
-            vec(arena.segFlags + dn.segmentOffset, gs.segFlags); vec(arena.segCat + dn.segmentOffset, gs.segCat);
-            vec(arena.segMask + dn.segmentOffset, gs.segMask); vec(arena.segId + dn.segmentOffset, gs.segId);
-            vec(arena.segOwner + dn.segmentOffset, gs.segOwner);
-            vec(arena.bodyPose[0] + dn.bodyOffset, gs.bodyPose); vec(arena.bodyPrev[0] + dn.bodyOffset, gs.bodyPrev);
-            vec(arena.bodyMotion + dn.bodyOffset, gs.bodyMotion); vec(arena.bodyShape + dn.bodyOffset, gs.bodyShape);
-            vec(arena.bodyLimits + dn.bodyOffset, gs.bodyLimits); vec(arena.bodyRanges + dn.bodyOffset, gs.bodyRanges);
-            vec(arena.bodyKind + dn.bodyOffset, gs.bodyKind); vec(arena.bodyId + dn.bodyOffset, gs.bodyId);
-            vec(arena.verts + dn.vertexOffset, gs.verts); vec(arena.tris + dn.triangleOffset, gs.tris);
-            vec(arena.motors + dn.motorOffset, gs.motors); vec(arena.bands + dn.bandOffset, gs.bands);
-            vec(arena.fields + dn.fieldOffset, gs.fields); vec(arena.fieldVerts + dn.fieldVertexOffset, gs.fieldVerts);
-            vec(arena.fieldIds + dn.fieldIdOffset, gs.fieldIds);
-            piece(descs + pr.in, &pr.dn, sizeof pr.dn);
-            piece(descs + pr.out, &pr.dOut, sizeof pr.dOut);
-        }
-        const size_t opsAt = room(pieces.size() * sizeof(bpd::CopyOp));
-        const size_t movesAt = room(go.size() * sizeof(bpd::WorldMove));
-        if ((s = ensurePinned(blobBytes)) != BP_OK) return s;
-        if (deviceBlobBytes < blobBytes) {
-            if ((s = alloc(deviceBlob, blobBytes)) != BP_OK) return s;
-            deviceBlobBytes = blobBytes;
-        }
-        char *hb = static_cast<char *>(pinnedBlob);
-        for (const Pair &pr : go) {
-            const HostWorld &w = worlds[pr.in];
-            const uint32_t m = (uint32_t)w.bubbles.size();
-            if (!m) continue;
-            std::memcpy(hb + pr.recsAt, w.bubbles.data(), m * sizeof(bp_bubble));
-            motor_band_indices(pr.in, w.bubbles.data(), m, reinterpret_cast<int32_t *>(hb + pr.idxAt));
-        }
-        bpd::CopyOp *op = reinterpret_cast<bpd::CopyOp *>(hb + opsAt);
-        for (const Piece &pc : pieces) {
-            std::memcpy(hb + pc.at, pc.src, pc.size);
-            *op++ = bpd::CopyOp{pc.at, pc.dst, (uint32_t)pc.size, 0u};
-        }
-        bpd::WorldMove *mv = reinterpret_cast<bpd::WorldMove *>(hb + movesAt);
-        uint32_t nmoves = 0, maxMoved = 0;
-        for (const Pair &pr : go) {
-            const uint32_t m = (uint32_t)worlds[pr.in].bubbles.size();
-            if (!m) continue;
-            mv[nmoves++] = bpd::WorldMove{nullptr, deviceBlob + pr.recsAt,
-                                          reinterpret_cast<const int32_t *>(deviceBlob + pr.idxAt), pr.dn.bubbleOffset, m};
-            maxMoved = std::max(maxMoved, m);
-        }
-        BP_HIP(hipMemcpyAsync(deviceBlob, hb, blobBytes, hipMemcpyHostToDevice, stream));
-        BP_HIP(bp_launch_multi_copy(deviceBlob, reinterpret_cast<const bpd::CopyOp *>(deviceBlob + opsAt), (uint32_t)pieces.size(), stream));
-        BP_HIP(bp_launch_scatter_worlds(arenaDev, reinterpret_cast<const bpd::WorldMove *>(deviceBlob + movesAt), nmoves, maxMoved, stream));
-        BP_HIP(hipStreamSynchronize(stream));
-        // 3. Bookkeeping: residency, ranges, kernel classes (the batch's own class lists), graphs.
-        for (const Pair &pr : go) {
-            const uint32_t out = pr.out, in = pr.in;
-            HostWorld &o = worlds[out], &w = worlds[in];
-            const Region r = regions[out];
-            hostDescs[in] = pr.dn;
-            hostDescs[out] = pr.dOut;
-            regions[in] = r;
-            regions[out] = Region{};
-            const int oldClass = o.packedClass, newClass = default_class_for(w, false);
-            w.resident = true; w.capacity = r.bubbles; w.packedClass = newClass;
-            o.resident = false; o.capacity = 0; o.packedClass = -1;
-            auto &from = classWorlds[oldClass];
-            from.erase(std::remove(from.begin(), from.end(), out), from.end());
-            classWorlds[newClass].push_back(in);
-            if (workEstimate.size() == worlds.size()) workEstimate[in] = (uint32_t)w.bubbles.size();
-            if (coopGroups.size() == worlds.size()) { coopGroups[in] = 0; coopGroups[out] = 0; }
-            results[pr.result] = BP_OK;
-        }
-        orderDirty = true;                               // the class lists go up before the next batch launch
+    auto vec = [&](auto *dst, const auto &src) { piece(dst, src.data(), src.size() * sizeof(src[0])); };
+    for (SwapPair &pr : sr.upload) {
+        const uint32_t m = (uint32_t)worlds[pr.in].bubbles.size();
+        const bpd::WorldDesc &dn = pr.dn;
+        const GeometryStage &gs = pr.gs;
+        pr.recsAt = room((size_t)m * sizeof(bp_bubble));
+        pr.idxAt = room(2 * (size_t)m * sizeof(int32_t));
+        vec(arena.segAB + dn.segmentOffset, gs.segAB); vec(arena.segRegion + dn.segmentOffset, gs.segRegion);
+        vec(arena.segFlags + dn.segmentOffset, gs.segFlags); vec(arena.segCat + dn.segmentOffset, gs.segCat);
+        vec(arena.segMask + dn.segmentOffset, gs.segMask); vec(arena.segId + dn.segmentOffset, gs.segId);
+        vec(arena.segOwner + dn.segmentOffset, gs.segOwner);
+        vec(arena.bodyPose[0] + dn.bodyOffset, gs.bodyPose); vec(arena.bodyPrev[0] + dn.bodyOffset, gs.bodyPrev);
+        vec(arena.bodyMotion + dn.bodyOffset, gs.bodyMotion); vec(arena.bodyShape + dn.bodyOffset, gs.bodyShape);
+        vec(arena.bodyLimits + dn.bodyOffset, gs.bodyLimits); vec(arena.bodyRanges + dn.bodyOffset, gs.bodyRanges);
+        vec(arena.bodyKind + dn.bodyOffset, gs.bodyKind); vec(arena.bodyId + dn.bodyOffset, gs.bodyId);
+        vec(arena.verts + dn.vertexOffset, gs.verts); vec(arena.tris + dn.triangleOffset, gs.tris);
+        vec(arena.motors + dn.motorOffset, gs.motors); vec(arena.bands + dn.bandOffset, gs.bands);
+        vec(arena.fields + dn.fieldOffset, gs.fields); vec(arena.fieldVerts + dn.fieldVertexOffset, gs.fieldVerts);
+        vec(arena.fieldIds + dn.fieldIdOffset, gs.fieldIds);
+        piece(descs + pr.in, &pr.dn, sizeof pr.dn);
+        piece(descs + pr.out, &pr.dOut, sizeof pr.dOut);
+    }
+    const size_t opsAt = room(pieces.size() * sizeof(bpd::CopyOp));
+    const size_t movesAt = room(sr.upload.size() * sizeof(bpd::WorldMove));
+    bp_status s = ensure_pinned_blob(blobBytes);
+    if (s != BP_OK) return s;
+    if (deviceBlobBytes < blobBytes) {
+        if ((s = alloc(deviceBlob, blobBytes)) != BP_OK) return s;
+        deviceBlobBytes = blobBytes;
+    }
+    char *hb = static_cast<char *>(pinnedBlob);
+    for (const SwapPair &pr : sr.upload) {
+        const HostWorld &w = worlds[pr.in];
+        const uint32_t m = (uint32_t)w.bubbles.size();
+        if (!m) continue;
+        std::memcpy(hb + pr.recsAt, w.bubbles.data(), m * sizeof(bp_bubble));
+        motor_band_indices(pr.in, w.bubbles.data(), m, reinterpret_cast<int32_t *>(hb + pr.idxAt));
+    }
+    bpd::CopyOp *op = reinterpret_cast<bpd::CopyOp *>(hb + opsAt);
+    for (const Piece &pc : pieces) {
+        std::memcpy(hb + pc.at, pc.src, pc.size);
+        *op++ = bpd::CopyOp{pc.at, pc.dst, (uint32_t)pc.size, 0u};
+    }
+    bpd::WorldMove *mv = reinterpret_cast<bpd::WorldMove *>(hb + movesAt);
+    uint32_t nmoves = 0, maxMoved = 0;
+    for (const SwapPair &pr : sr.upload) {
+        const uint32_t m = (uint32_t)worlds[pr.in].bubbles.size();
+        if (!m) continue;
+        mv[nmoves++] = bpd::WorldMove{nullptr, deviceBlob + pr.recsAt, reinterpret_cast<const int32_t *>(deviceBlob + pr.idxAt),
+                                      pr.dn.bubbleOffset, m};
+        maxMoved = std::max(maxMoved, m);
+    }
+    BP_HIP(hipMemcpyAsync(deviceBlob, hb, blobBytes, hipMemcpyHostToDevice, stream));
+    BP_HIP(bp_launch_multi_copy(deviceBlob, reinterpret_cast<const bpd::CopyOp *>(deviceBlob + opsAt), (uint32_t)pieces.size(), stream));
+    BP_HIP(bp_launch_scatter_worlds(arenaDev, reinterpret_cast<const bpd::WorldMove *>(deviceBlob + movesAt), nmoves, maxMoved, stream));
+    return BP_OK;
+}
+
+void bp_batch::swap_finish() {
+    SwapRound &sr = swap_round();
+    // Residency, ranges, kernel classes (the batch's own class lists), graphs.
+    for (const SwapPair &pr : sr.upload) {
+        const uint32_t out = pr.out, in = pr.in;
+        HostWorld &o = worlds[out], &w = worlds[in];
+        const Region r = regions[out];
+        hostDescs[in] = pr.dn;
+        hostDescs[out] = pr.dOut;
+        regions[in] = r;
+        regions[out] = Region{};
+        const int oldClass = o.packedClass, newClass = default_class_for(w, false);
+        w.resident = true; w.capacity = r.bubbles; w.packedClass = newClass;
+        o.resident = false; o.capacity = 0; o.packedClass = -1;
+        auto &from = classWorlds[oldClass];
+        from.erase(std::remove(from.begin(), from.end(), out), from.end());
+        classWorlds[newClass].push_back(in);
+        if (workEstimate.size() == worlds.size()) workEstimate[in] = (uint32_t)w.bubbles.size();
+        if (coopGroups.size() == worlds.size()) { coopGroups[in] = 0; coopGroups[out] = 0; }
+    }
+    if (!sr.upload.empty()) {
+        orderDirty = true;                           // the class lists go up before the next batch launch
         for (auto &g : poolGroups) g = 0;
         release_graph();
     }
-    return BP_OK;
+    sr.pairs.clear();
+    sr.upload.clear();
+}
+
+
+void bp_batch::free_swap_round() {
+    delete swapRound;
+    swapRound = nullptr;
 }
 
 // Queues `steps` steps of the listed worlds (every resident world when `subset` is null) on the

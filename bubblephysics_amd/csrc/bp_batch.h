@@ -172,7 +172,7 @@ struct bp_batch {
     bpd::WorldDesc *descs = nullptr;
     std::vector<bpd::WorldDesc> hostDescs;
     // The lengths of each world's device ranges (set by pack; a world moved into another world's
-    // ranges by swap_residency takes them over), which bound what a world moved in may hold.
+    // ranges in an in-place swap takes them over), which bound what a world moved in may hold.
     struct Region {
         uint32_t bubbles, segments, bodies, vertices, triangles, motors, bands, fields, fieldVertices, fieldIds, adjacency;
     };
@@ -215,7 +215,7 @@ struct bp_batch {
     bp_bubble *xferBubbles = nullptr;  // one world's records in transit (transactions)
     int32_t *xferIndices = nullptr;    // their motor and band indices
     uint32_t xferCapacity = 0;
-    bp_bubble *pinnedRecs = nullptr;   // pinned host staging of one world's records (swap_residency)
+    bp_bubble *pinnedRecs = nullptr;   // pinned host staging of the records of a swap round
     void *pinnedBlobDev = nullptr;       // device address of pinnedBlob
     int32_t *pinnedIdx = nullptr;
     uint32_t pinnedCapacity = 0;
@@ -251,6 +251,7 @@ struct bp_batch {
         if (pinnedRecs) (void)hipHostFree(pinnedRecs);
         if (pinnedIdx) (void)hipHostFree(pinnedIdx);
         if (pinnedBlob) (void)hipHostFree(pinnedBlob);
+        free_swap_round();
     }
 
     void release_graph() {
@@ -338,17 +339,32 @@ struct bp_batch {
     // Serving residency without a repack: whether the held world `in` fits the device ranges of
     // the resident world `out`, and the exchange (out's published state is read back into its
     // host copy, in's host copy is written into the ranges). Both worlds must be idle.
-    // swap_residency exchanges the pairs (outs[i], ins[i]) (all of them distinct worlds), with
-    // one read-back round trip and one upload for as many pairs as the transfer buffer holds;
-    // results[i] is BP_OK, BP_ERR_CAPACITY_OVERFLOW (the world does not fit) or
-    // BP_ERR_UNAVAILABLE (the resident world holds region-removed bubbles), and the return value
-    // a device or allocation error.
+    // A swap round exchanges the pairs (outs[i], ins[i]) (all of them distinct worlds), with one
+    // read-back and one upload for as many pairs as the transfer buffer holds; results[i] ends
+    // BP_OK, BP_ERR_CAPACITY_OVERFLOW (the world does not fit) or BP_ERR_UNAVAILABLE (the
+    // resident world holds region-removed bubbles, or the pair waits for a later round), and the
+    // return values are device or allocation errors.
     bool fits_ranges_of(uint32_t out, uint32_t in) const;
     // mirrors (null, or one per pair; capacity 0 for none) are the serving mirror ranges the
     // descriptor of a world moved in points to.
+    // A round runs in three calls, each after the stream's work of the previous one has finished:
+    // swap_begin queues the read-back (results[i]: BP_ERR_UNAVAILABLE also for a pair left for a
+    // later round because the transfer buffer is full), swap_upload fills the host copies of the
+    // worlds leaving and queues the upload of the worlds moving in (swapUploads of them; none
+    // when every resident world of the round is held), swap_finish records the new residency.
     struct MirrorRange { void *mirror; float4 *bodyMirror; int32_t *mirrorIdx; uint32_t capacity; };
-    bp_status swap_residency(const uint32_t *outs, const uint32_t *ins, uint32_t count, const MirrorRange *mirrors,
-                             bp_status *results, hipStream_t stream);
+    struct SwapPair;
+    struct SwapRound;
+    SwapRound *swapRound = nullptr;
+    size_t swapReadAt = 0;
+    uint32_t swapRecords = 0, swapUploads = 0;
+    SwapRound &swap_round();
+    void free_swap_round();
+    bp_status ensure_pinned_blob(size_t bytes);
+    bp_status swap_begin(const uint32_t *outs, const uint32_t *ins, uint32_t count, const MirrorRange *mirrors,
+                         bp_status *results, hipStream_t stream);
+    bp_status swap_upload(bp_status *results, hipStream_t stream);
+    void swap_finish();
     bp_status launch(const uint32_t *subset, uint32_t subsetCount, uint32_t steps);
     void plan_pools();
     bp_status refresh_order();

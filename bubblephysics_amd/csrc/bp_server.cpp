@@ -119,7 +119,15 @@ struct bp_server {
     uint32_t W = 0;
     std::vector<std::deque<Pending>> queue;
     std::vector<uint8_t> busy, queued, resident, wantAdmission;
-    std::vector<uint8_t> taken;                 // victims already chosen in the current admission round
+    std::vector<uint8_t> moving;                // worlds of the in-place exchanges in flight
+    struct SwapRoundState {
+        int phase = 0;                          // 0: none in flight, 1: read-back queued, 2: upload queued
+        std::vector<uint32_t> outs, ins;
+        std::vector<uint8_t> outIdle;
+        std::vector<bp_batch::MirrorRange> mirrors;
+        std::vector<bp_status> results;
+    } swapState;
+    hipEvent_t swapDone = nullptr;
     std::vector<uint8_t> worldFailed;           // a step failed: the world takes no more steps or transactions
     std::vector<uint32_t> dropPending;          // bubbles the world's last step removed from the region
     bpd::DropRecord *dropDev = nullptr;
@@ -187,7 +195,9 @@ struct bp_server {
     bool form_launches();
     bp_status start_launch(int cls);
     bp_status admit_waiting();
-    bp_status admit_in_place();
+    bp_status start_swap_round();
+    bp_status advance_swap_round(bool wait);
+    bp_status finish_swap_round();
     int class_for(uint32_t w, uint32_t n) const;
     uint32_t buckets_for(uint32_t w, uint32_t n) const;
     void fail(bp_status s);
@@ -291,7 +301,7 @@ bp_status bp_server::init() {
             return s;
     }
     queue.assign(W, {});
-    busy.assign(W, 0); queued.assign(W, 0); wantAdmission.assign(W, 0); taken.assign(W, 0); waitingSince.assign(W, 0.0);
+    busy.assign(W, 0); queued.assign(W, 0); wantAdmission.assign(W, 0); moving.assign(W, 0); waitingSince.assign(W, 0.0);
     worldFailed.assign(W, 0); dropPending.assign(W, 0);
     for (uint32_t w = 0; w < W; ++w) worldFailed[w] = b.worlds[w].failed ? 1 : 0;
     count.resize(W); headCount.assign(W, 0); headClass.assign(W, -1); lastUse.assign(W, 0);
@@ -335,6 +345,7 @@ bp_status bp_server::init() {
     idleStreams = (int)streams.size();
     BP_HIP(hipStreamCreateWithFlags(&readStream, hipStreamNonBlocking));
     BP_HIP(hipStreamCreateWithFlags(&stageStream, hipStreamNonBlocking));
+    BP_HIP(hipEventCreateWithFlags(&swapDone, hipEventDisableTiming));
     staging.assign(W, 0);
     for (int j = 0; j < kStageJobs; ++j) {
         stageJobs[j].stream = stageStream;
@@ -355,6 +366,8 @@ void bp_server::release() {
     if (readStream) (void)hipStreamDestroy(readStream);
     readStream = nullptr;
     if (stageStream) (void)hipStreamDestroy(stageStream);
+    if (swapDone) (void)hipEventDestroy(swapDone);
+    swapDone = nullptr;
     stageStream = nullptr;
     for (StageJob &j : stageJobs) {
         if (j.done) (void)hipEventDestroy(j.done);
@@ -386,6 +399,7 @@ void bp_server::enqueue(const Pending &p) {
 // Makes the world's oldest request ready: decides the transaction's effect on the bubble count
 // (rejecting one that needs more slots than the world holds) and the class of the step.
 void bp_server::prepare_head(uint32_t w) {
+    if (moving[w]) return;                      // the round that moves it calls back here
     while (!queue[w].empty()) {
         if (!resident[w]) {
             if (!wantAdmission[w]) { wantAdmission[w] = 1; admissions.push_back(w); waitingSince[w] = now(); admissionRetry = true; }
@@ -966,19 +980,21 @@ bp_status bp_server::admit_waiting() {
 
 // Admits waiting worlds into the device ranges of evicted worlds: for each, the least recently
 // used resident world that is idle (no launch, request, staging or pending removal) and whose
-// ranges hold it, within the budget. The exchanges of one round share one read-back round trip
-// and one upload. Worlds for which none qualifies stay in the admission list for the repacking
-// round.
-bp_status bp_server::admit_in_place() {
+// ranges hold it, within the budget. A round's exchanges share one read-back and one upload, and
+// the dispatcher goes on launching while they are on the device: both worlds of an exchange are
+// `moving` (neither launches nor takes part in another exchange) until the round ends. Worlds for
+// which no victim qualifies stay in the admission list for the repacking round.
+bp_status bp_server::start_swap_round() {
     bp_batch &b = *batch;
-    std::vector<uint32_t> left, outs, ins;
-    std::vector<uint8_t> outIdle;
+    std::vector<uint32_t> left;
+    SwapRoundState &r = swapState;
+    r.outs.clear(); r.ins.clear(); r.outIdle.clear(); r.mirrors.clear();
     // The worlds that may be displaced, gathered once per round.
     struct Candidate { uint32_t v, slots; uint64_t use; uint8_t idle; };
     std::vector<Candidate> cands;
     cands.reserve(W);
     for (uint32_t v = 0; v < W; ++v) {
-        if (!resident[v] || busy[v] || staging[v] || dropPending[v]) continue;
+        if (!resident[v] || busy[v] || staging[v] || dropPending[v] || moving[v]) continue;
         cands.push_back(Candidate{v, b.regions[v].bubbles, lastUse[v], (uint8_t)(!queued[v] && queue[v].empty())});
     }
     // The best candidate whose ranges hold w: idle before waiting (those only when `urgent`), then
@@ -988,7 +1004,7 @@ bp_status bp_server::admit_in_place() {
         int64_t best = -1;
         for (size_t i = 0; i < cands.size(); ++i) {
             const Candidate &c = cands[i];
-            if (c.slots < need || (!c.idle && !urgent) || taken[c.v]) continue;
+            if (c.slots < need || (!c.idle && !urgent) || moving[c.v]) continue;
             if (best >= 0) {
                 const Candidate &o = cands[(size_t)best];
                 if (o.idle && !c.idle) continue;
@@ -1000,6 +1016,7 @@ bp_status bp_server::admit_in_place() {
     };
     for (uint32_t w : admissions) {
         if (resident[w]) { wantAdmission[w] = 0; continue; }
+        if (moving[w]) { left.push_back(w); continue; }
         // The world takes over the victim's device memory, so the device footprint does not grow.
         // Among the idle worlds whose ranges hold it, the one with the fewest bubble slots (so that
         // large ranges stay for large worlds), the least recently used of those. A world that has
@@ -1011,30 +1028,74 @@ bp_status bp_server::admit_in_place() {
             left.push_back(w);
             continue;
         }
-        const uint32_t victim = cands[(size_t)at].v;
-        taken[victim] = 1;
-        outs.push_back(victim);
-        ins.push_back(w);
-        outIdle.push_back(cands[(size_t)at].idle);
+        const uint32_t v = cands[(size_t)at].v;
+        moving[v] = 1;
+        moving[w] = 1;
+        if (queued[v]) {                                 // a displaced world launches no more until the round ends
+            auto &rq = ready[headClass[v]];
+            rq.erase(std::remove(rq.begin(), rq.end(), v), rq.end());
+            queued[v] = 0;
+        }
+        r.outs.push_back(v);
+        r.ins.push_back(w);
+        r.outIdle.push_back(cands[(size_t)at].idle);
+        // The mirror of the world moved in, when it has room: its descriptor goes up pointing there.
+        r.mirrors.push_back(n <= mirrorCap[w] ? bp_batch::MirrorRange{mirrorDev + mirrorOffset[w], bodyMirrorDev + bodyMirrorOffset[w],
+                                                                      mirrorIdxDev + 2 * mirrorOffset[w], mirrorCap[w]}
+                                              : bp_batch::MirrorRange{nullptr, nullptr, nullptr, 0u});
     }
-    for (uint32_t v : outs) taken[v] = 0;
-    // The mirror of a world moved in, when it has room: its descriptor goes up pointing there.
-    std::vector<bp_batch::MirrorRange> mirrors(outs.size(), bp_batch::MirrorRange{nullptr, nullptr, nullptr, 0u});
-    for (size_t i = 0; i < outs.size(); ++i) {
-        const uint32_t w = ins[i];
-        if (b.worlds[w].bubbles.size() <= mirrorCap[w])
-            mirrors[i] = bp_batch::MirrorRange{mirrorDev + mirrorOffset[w], bodyMirrorDev + bodyMirrorOffset[w],
-                                               mirrorIdxDev + 2 * mirrorOffset[w], mirrorCap[w]};
+    admissions.swap(left);
+    if (r.outs.empty()) return BP_OK;
+    r.results.assign(r.outs.size(), BP_OK);
+    bp_status s = b.swap_begin(r.outs.data(), r.ins.data(), (uint32_t)r.outs.size(), r.mirrors.data(), r.results.data(), readStream);
+    if (s != BP_OK) return s;
+    if (b.swapRecords == 0) return finish_swap_round();
+    BP_HIP(hipEventRecord(swapDone, readStream));
+    r.phase = 1;
+    return BP_OK;
+}
+
+// Advances the round in flight when its device work has finished.
+bp_status bp_server::advance_swap_round(bool wait) {
+    SwapRoundState &r = swapState;
+    if (r.phase == 0) return BP_OK;
+    if (wait) {
+        BP_HIP(hipEventSynchronize(swapDone));
+    } else {
+        const hipError_t e = hipEventQuery(swapDone);
+        if (e == hipErrorNotReady) return BP_OK;
+        BP_HIP(e);
     }
-    std::vector<bp_status> results(outs.size(), BP_OK);
-    if (!outs.empty()) {
-        bp_status s = b.swap_residency(outs.data(), ins.data(), (uint32_t)outs.size(), mirrors.data(), results.data(), readStream);
+    if (r.phase == 1) {
+        bp_batch &b = *batch;
+        bp_status s = b.swap_upload(r.results.data(), readStream);
         if (s != BP_OK) return s;
+        if (b.swapUploads) {
+            BP_HIP(hipEventRecord(swapDone, readStream));
+            r.phase = 2;
+            return wait ? advance_swap_round(true) : BP_OK;
+        }
     }
-    for (size_t i = 0; i < outs.size(); ++i) {
-        const uint32_t v = outs[i], w = ins[i];
-        if (results[i] == BP_ERR_UNAVAILABLE || results[i] == BP_ERR_CAPACITY_OVERFLOW) { left.push_back(w); continue; }
-        if (results[i] != BP_OK) return results[i];
+    return finish_swap_round();
+}
+
+bp_status bp_server::finish_swap_round() {
+    bp_batch &b = *batch;
+    SwapRoundState &r = swapState;
+    b.swap_finish();
+    r.phase = 0;
+    admissionRetry = true;
+    for (size_t i = 0; i < r.outs.size(); ++i) {
+        const uint32_t v = r.outs[i], w = r.ins[i];
+        moving[v] = 0;
+        moving[w] = 0;
+        if (r.results[i] != BP_OK) {
+            // Not exchanged (held, or left for a later round): both go on as they were.
+            if (r.results[i] != BP_ERR_UNAVAILABLE && r.results[i] != BP_ERR_CAPACITY_OVERFLOW) return r.results[i];
+            admissions.push_back(w);
+            if (!busy[v] && !queued[v] && !staging[v]) prepare_head(v);
+            continue;
+        }
         resident[v] = 0;
         used -= footprintOf[v];
         footprintOf[v] = b.footprint(v);
@@ -1042,21 +1103,12 @@ bp_status bp_server::admit_in_place() {
         mirrorValid[v] = 0;
         count[v] = (uint32_t)b.worlds[v].bubbles.size();
         ++local.evictions;
-        if (!outIdle[i]) {
+        if (!queue[v].empty()) {
             // The displaced world's waiting requests wait for its admission; a transaction staged
             // on its mirror is staged again from its state after admission.
-            if (queued[v]) {
-                auto &rq = ready[headClass[v]];
-                rq.erase(std::remove(rq.begin(), rq.end(), v), rq.end());
-                queued[v] = 0;
-            }
-            if (!queue[v].empty()) {
-                Pending &p = queue[v].front();
-                if (p.replace) { p.staged = false; p.replace = false; p.replaceCount = 0; --local.transactions; }
-                wantAdmission[v] = 1;
-                waitingSince[v] = now();
-                left.push_back(v);
-            }
+            Pending &p = queue[v].front();
+            if (p.replace) { p.staged = false; p.replace = false; p.replaceCount = 0; --local.transactions; }
+            prepare_head(v);
         }
         resident[w] = 1;
         footprintOf[w] = b.footprint(w);
@@ -1067,7 +1119,7 @@ bp_status bp_server::admit_in_place() {
         ++local.admissions;
         // The world's mirror (its descriptor on the device points there), filled from its host
         // copy: the state just written to the device.
-        if (mirrors[i].capacity) {
+        if (r.mirrors[i].capacity) {
             const HostWorld &hw = b.worlds[w];
             const uint32_t n = (uint32_t)hw.bubbles.size();
             if (n) std::memcpy(mirrorHost + mirrorOffset[w], hw.bubbles.data(), n * sizeof(bp_bubble));
@@ -1080,7 +1132,6 @@ bp_status bp_server::admit_in_place() {
         }
         prepare_head(w);
     }
-    admissions.swap(left);
     local.resident_bytes = used;
     return BP_OK;
 }
@@ -1090,18 +1141,24 @@ bp_status bp_server::admit_in_place() {
 // is launched once before the next round, so each round makes progress even when the budget
 // forces the next admission to evict a world that still has requests.
 bool bp_server::form_launches() {
-    if (!admissions.empty() && admissionRetry && now() - lastAdmissionRound >= kAdmissionInterval) {
+    if (swapState.phase != 0) {
+        const double t0 = now();
+        bp_status s = advance_swap_round(false);
+        local.admission_microseconds += (uint64_t)((now() - t0) * 1e6);
+        if (s != BP_OK) { fail(s); return false; }
+    }
+    if (swapState.phase == 0 && !admissions.empty() && admissionRetry && now() - lastAdmissionRound >= kAdmissionInterval) {
         admissionRetry = false;
         const double t0 = now();
         lastAdmissionRound = t0;
-        bp_status s = admit_in_place();
+        bp_status s = start_swap_round();
         local.admission_microseconds += (uint64_t)((now() - t0) * 1e6);
         if (s != BP_OK) { fail(s); return false; }
     }
     // A world that no idle world's ranges hold usually finds one as launches complete; the device
     // drains for a repacking round only once the oldest of them has waited kRepackDelay.
     bool repack = false;
-    if (!admissions.empty() && !launchAfterAdmission) {
+    if (!admissions.empty() && !launchAfterAdmission && swapState.phase == 0) {
         double oldest = now();
         for (uint32_t w : admissions) oldest = std::min(oldest, waitingSince[w]);
         repack = now() - oldest >= kRepackDelay;
@@ -1148,7 +1205,7 @@ void bp_server::run() {
             bool idle = inflight.empty() && activeStage.empty();
             if (idle)
                 for (int k = 0; k < kClassCount && idle; ++k) idle = ready[k].empty();
-            idle = idle && admissions.empty();
+            idle = idle && admissions.empty() && swapState.phase == 0;
             if (stop && inflight.empty()) break;
             if (incoming.empty() && idle && !stop) cvWork.wait_for(lock, std::chrono::milliseconds(2));
             taken.assign(incoming.begin(), incoming.end());
@@ -1173,6 +1230,7 @@ void bp_server::run() {
     // Drain whatever is still on the device before the dispatcher ends.
     for (int id : inflight) (void)hipEventSynchronize(launches[id].done);
     for (int j : activeStage) (void)hipEventSynchronize(stageJobs[j].done);
+    if (swapState.phase != 0 && advance_swap_round(true) != BP_OK) swapState.phase = 0;
     poll_launches();
     std::lock_guard<std::mutex> lock(mu);
     publish_counters();
